@@ -1,0 +1,116 @@
+/*
+ * CPU oracle for the FLUXO split-form DGSEM right-hand side.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (project-fluxo/fluxo, Fortran) used as the parity checker for the
+ * HIP engine.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker or the reported CPU baseline.  The
+ * product path never links or calls anything in this directory.
+ *
+ * Every routine cites the reference file:line it restates.  Array layouts follow
+ * the reference (Fortran column-major, first index fastest):
+ *   U(nVar,0:N,0:N,0:N,nElems)        -> U[(((e*n+k)*n+j)*n+i)*nVar+v]
+ *   Metrics_fTilde(3,0:N,0:N,0:N,E)   -> M[(((e*n+k)*n+j)*n+i)*3+d]
+ *   sJ(0:N,0:N,0:N,E)                 -> sJ[((e*n+k)*n+j)*n+i]
+ *   NormVec(3,0:N,0:N,nSides)         -> nv[((s*n+q)*n+p)*3+d]
+ *   SurfElem(0:N,0:N,nSides)          -> se[(s*n+q)*n+p]
+ *   ElemToSide(2,6,nElems)            -> e2s[(e*6+locSide-1)*2+{0:SideID,1:flip}]
+ *   SideToElem(5,nSides)              -> s2e[s*5+{0:Elem,1:nbElem,2:locSide,3:nbLocSide,4:flip}]
+ * Element and side IDs stored in the connectivity arrays are 1-based as in the
+ * reference; -1 marks "none".
+ *
+ * Parity status of each part is listed in oracle/README.md.
+ */
+#ifndef FLUXO_ORACLE_H
+#define FLUXO_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* node types of GetNodesAndWeights (src/interpolation/interpolation.f90:190-265) */
+enum { ORA_NODE_EQUI = 0, ORA_NODE_GL = 1, ORA_NODE_GAUSS = 2 };
+
+/* local side ids (src/defines.h:68-73) */
+enum { ORA_ZETA_MINUS = 1, ORA_ETA_MINUS = 2, ORA_XI_PLUS = 3, ORA_ETA_PLUS = 4,
+       ORA_XI_MINUS = 5, ORA_ZETA_PLUS = 6 };
+
+/* ---- basis (src/interpolation/basis.f90, interpolation.f90, changeBasis.f90) ---- */
+void ora_legendre(int N, double x, double *L, double *Lder);
+void ora_gauss_nodes(int N, double *x, double *w);
+void ora_lgl_nodes(int N, double *x, double *w);
+void ora_nodes(int N, int type, double *x, double *w);
+void ora_barycentric_weights(int N, const double *x, double *wb);
+void ora_derivative_matrix(int N, const double *x, double *D);
+void ora_lagrange_polys(double x, int N, const double *xGP, const double *wb, double *L);
+void ora_legendre_vdm(int N, const double *xi, double *Vdm_Leg, double *sVdm_Leg);
+void ora_vandermonde(int Nin, int typeIn, int Nout, int typeOut, int modal, double *Vdm);
+void ora_change_basis_3d(int dim1, int Nin, int Nout, const double *Vdm, const double *Xin, double *Xout);
+/* D, D_Hat, Dvolsurf, L_HatMinus, L_HatPlus for LGL nodes (src/dg/dg.f90:181-252) */
+void ora_dg_operators(int N, double *xGP, double *wGP, double *D, double *D_hat,
+                      double *Dvolsurf, double *L_hatminus, double *L_hatplus);
+
+/* ---- mesh (periodic box, my generator) and mappings (src/mesh/mappings.f90) ---- */
+void ora_cgns_side_to_vol(int N, int l, int p, int q, int locSide, int ijk[3]);
+void ora_flip_s2m(int N, int p, int q, int flip, int pq[2]);
+void ora_side_to_vol(int N, int l, int p, int q, int flip, int locSide, int ijk[3]);
+int  ora_box_connectivity(int Kx, int Ky, int Kz, int *ElemToSide, int *SideToElem);
+void ora_box_nodecoords(int Kx, int Ky, int Kz, const double *x0, const double *len, int Ngeo,
+                        int warp, double amp, double *NodeCoords);
+int  ora_calc_metrics(int N, int Ngeo, int nElems, int nSides, const double *NodeCoords,
+                      const int *ElemToSide, double *Mf, double *Mg, double *Mh, double *sJ,
+                      double *NormVec, double *TangVec1, double *TangVec2, double *SurfElem,
+                      double *Elem_xGP);
+
+/* ---- equation options (ints and reals), mirrors parameter.ini keys ---- */
+typedef struct {
+  int N, nVar, nElems, nSides;
+  int volflux;       /* VolumeFlux: 0 standard DG, 5 Chandrashekar EC-KEP, 32 Ranocha EC-KEP */
+  int riemann;       /* Riemann: 1 LLF (Rusanov), 5 EC Chandrashekar surface, 32 EC Ranocha surface */
+  int parabolic;     /* 0 Euler, 1 Navier-Stokes */
+  int lifting;       /* 1 BR1, 2 BR2 */
+  double kappa, mu0, Pr, etaBR2;
+  /* mesh */
+  const int *ElemToSide, *SideToElem;
+  const double *Mf, *Mg, *Mh, *sJ, *NormVec, *TangVec1, *TangVec2, *SurfElem;
+} ora_problem;
+
+/* two-point and surface fluxes; U/aux layout per point: U(5), aux(6)=(1/rho,v1,v2,v3,p,|v|^2) */
+double ora_ln_mean(double aL, double aR);
+void ora_eval_uaux(double kappa, const double *U, double *aux);
+void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
+                 const double *auxR, const double *mL, const double *mR, double *F);
+void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
+                 const double *t1, const double *t2, double *F);
+void ora_cons_to_entropy(double kappa, const double *U, double *w);
+
+/* ---- DG operator ---- */
+/* full DGTimeDerivative_weakForm (src/dg/dg.f90:282-452) */
+void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, double *Ut);
+/* individual contributions (no sJ scaling): volume split form, surface integral */
+void ora_volint_splitform(const ora_problem *P, const double *U, double *Ut);
+void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
+                          const double *gz, double *Ut);
+/* BR1/BR2 lifting: primitive gradients gradPx/y/z(5,n,n,n,E) */
+void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz);
+/* Carpenter-Kennedy RK4(5) 2N low-storage step (src/timedisc/timedisc.f90:424-480) */
+void ora_lserk_step(const ora_problem *P, double *U, double *Ut_tmp, double t, double dt);
+/* CALCTIMESTEP (src/equation/navierstokes/calctimestep.f90:39-170) with FillCFL_DFL scaling */
+double ora_calc_timestep(const ora_problem *P, const double *U, double CFLscale, double DFLscale,
+                         int *errType);
+/* CalcEntropy (src/equation/navierstokes/analyze_equation.f90:549-599) */
+void ora_calc_entropy(const ora_problem *P, const double *U, const double *Ut, double *S, double *dSdU_Ut);
+/* exact functions 2 (density wave) and 7 (Taylor-Green) on the volume nodes
+   (src/equation/navierstokes/equation.f90:632-651, 802-811) */
+void ora_exactfunc(int which, double kappa, const double *advvel, double amp, double t,
+                   int npts, const double *x, double *U);
+
+/* Carpenter RK4-5 coefficients and the LGL CFL/DFL alpha tables
+   (src/timedisc/timedisc_vars.f90:79-90, 204-233) */
+void ora_rk_coefs(double *A, double *b, double *c);
+double ora_cfl_alpha(int N);
+double ora_dfl_alpha(int N);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

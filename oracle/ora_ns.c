@@ -1,0 +1,730 @@
+/*
+ * Oracle (test infrastructure only): compressible Euler / Navier-Stokes physics and
+ * the split-form DGSEM operator on LGL nodes, restating
+ *   src/equation/navierstokes/{flux_average,riemann,flux,equation_vars,equation,
+ *                              calctimestep,analyze_equation}.f90,
+ *   src/dg/{dg,volint,fillflux,surfint}.f90, src/dg/br1/br1{,_volint,_fillflux,_surfint}.f90,
+ *   src/timedisc/{timedisc,timedisc_vars}.f90.
+ * Each function names the routine and lines it follows.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "ora.h"
+
+#define NV 5
+
+/* LN_MEAN (flux_average.f90:1945-1965) */
+double ora_ln_mean(double aL, double aR)
+{
+  const double eps = 1.0e-4;
+  double xi = aR / aL;
+  double u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1.0);
+  if (u < eps) return (aL + aR) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + u * 15.0)));
+  return (aR - aL) / log(xi);
+}
+
+/* EvalUaux (flux_average.f90:305-332): aux = (1/rho, v1, v2, v3, p, |v|^2) */
+void ora_eval_uaux(double kappa, const double *U, double *aux)
+{
+  double srho = 1.0 / U[0];
+  double v1 = U[1] * srho, v2 = U[2] * srho, v3 = U[3] * srho;
+  double vv = v1 * v1 + v2 * v2 + v3 * v3;
+  aux[0] = srho; aux[1] = v1; aux[2] = v2; aux[3] = v3; aux[5] = vv;
+  aux[4] = (kappa - 1.0) * (U[4] - 0.5 * U[0] * vv);
+}
+
+/* StandardDGFluxVec (flux_average.f90:394-439) */
+static void flux_standard(const double *UL, const double *UR, const double *aL, const double *aR,
+                          const double *mL, const double *mR, double *F)
+{
+  double qL = aL[1] * mL[0] + aL[2] * mL[1] + aL[3] * mL[2];
+  double qR = aR[1] * mR[0] + aR[2] * mR[1] + aR[3] * mR[2];
+  F[0] = 0.5 * (UL[0] * qL + UR[0] * qR);
+  F[1] = 0.5 * (UL[1] * qL + UR[1] * qR + mL[0] * aL[4] + mR[0] * aR[4]);
+  F[2] = 0.5 * (UL[2] * qL + UR[2] * qR + mL[1] * aL[4] + mR[1] * aR[4]);
+  F[3] = 0.5 * (UL[3] * qL + UR[3] * qR + mL[2] * aL[4] + mR[2] * aR[4]);
+  F[4] = 0.5 * ((UL[4] + aL[4]) * qL + (UR[4] + aR[4]) * qR);
+}
+
+/* EntropyAndEnergyConservingFluxVec, Chandrashekar EC-KEP (flux_average.f90:1277-1356) */
+static void flux_chandrashekar(double kappa, const double *UL, const double *UR, const double *aL,
+                               const double *aR, const double *mL, const double *mR, double *F)
+{
+  double skm1 = 1.0 / (kappa - 1.0);
+  double m[3] = {0.5 * (mL[0] + mR[0]), 0.5 * (mL[1] + mR[1]), 0.5 * (mL[2] + mR[2])};
+  double bL = 0.5 * UL[0] / aL[4], bR = 0.5 * UR[0] / aR[4];
+  double rho_mean = 0.5 * (UL[0] + UR[0]);
+  double rhoHat = ora_ln_mean(UL[0], UR[0]);
+  double beta_mean = 0.5 * (bL + bR);
+  double beta_hat = ora_ln_mean(bL, bR);
+  double uH = 0.5 * (aL[1] + aR[1]), vH = 0.5 * (aL[2] + aR[2]), wH = 0.5 * (aL[3] + aR[3]);
+  double p1 = 0.5 * rho_mean / beta_mean;
+  double qH = uH * m[0] + vH * m[1] + wH * m[2];
+  F[0] = rhoHat * qH;
+  F[1] = F[0] * uH + m[0] * p1;
+  F[2] = F[0] * vH + m[1] * p1;
+  F[3] = F[0] * wH + m[2] * p1;
+  F[4] = F[0] * 0.5 * (skm1 / beta_hat - 0.5 * (aL[5] + aR[5])) + uH * F[1] + vH * F[2] + wH * F[3];
+}
+
+/* RanochaFluxVec (flux_average.f90:1876-1929) */
+static void flux_ranocha(double kappa, const double *UL, const double *UR, const double *aL,
+                         const double *aR, const double *mL, const double *mR, double *F)
+{
+  double skm1 = 1.0 / (kappa - 1.0);
+  double m[3] = {0.5 * (mL[0] + mR[0]), 0.5 * (mL[1] + mR[1]), 0.5 * (mL[2] + mR[2])};
+  double rhoLN = ora_ln_mean(UL[0], UR[0]);
+  double va[3] = {0.5 * (aL[1] + aR[1]), 0.5 * (aL[2] + aR[2]), 0.5 * (aL[3] + aR[3])};
+  double v2zip = 0.5 * (aL[1] * aR[1] + aL[2] * aR[2] + aL[3] * aR[3]);
+  double pavg = 0.5 * (aL[4] + aR[4]);
+  double vmL = aL[1] * m[0] + aL[2] * m[1] + aL[3] * m[2];
+  double vmR = aR[1] * m[0] + aR[2] * m[1] + aR[3] * m[2];
+  double rspL = UL[0] / aL[4], rspR = UR[0] / aR[4];
+  double rspM = ora_ln_mean(rspL, rspR);
+  F[0] = rhoLN * 0.5 * (vmL + vmR);
+  F[1] = F[0] * va[0] + m[0] * pavg;
+  F[2] = F[0] * va[1] + m[1] * pavg;
+  F[3] = F[0] * va[2] + m[2] * pavg;
+  F[4] = F[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (aL[4] * vmR + aR[4] * vmL);
+}
+
+void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
+                 const double *auxR, const double *mL, const double *mR, double *F)
+{
+  switch (which) {
+  case 5: flux_chandrashekar(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 32: flux_ranocha(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  default: flux_standard(UL, UR, auxL, auxR, mL, mR, F); break;
+  }
+}
+
+/* EvalEulerFlux1D (flux.f90:298-329) */
+static void euler_flux_1d(double kappa, const double *U, double *F)
+{
+  double srho = 1.0 / U[0];
+  double v1 = U[1] * srho, v2 = U[2] * srho, v3 = U[3] * srho;
+  double p = (kappa - 1.0) * (U[4] - 0.5 * (U[1] * v1 + U[2] * v2 + U[3] * v3));
+  F[0] = U[1];
+  F[1] = U[1] * v1 + p;
+  F[2] = U[1] * v2;
+  F[3] = U[1] * v3;
+  F[4] = (U[4] + p) * v1;
+}
+
+/* SoundSpeed2 (equation_vars.f90:342-356) */
+static double sound_speed2(double kappa, const double *U)
+{
+  return kappa * (kappa - 1.0) * (U[4] * U[0] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3])) / (U[0] * U[0]);
+}
+
+/* 1D (already rotated) two-point fluxes used as EC surface fluxes:
+   EntropyAndEnergyConservingFlux (flux_average.f90:1214-1270) and RanochaFlux (:1806-1866) */
+static void flux1d_chandrashekar(double kappa, const double *UL, const double *UR, double *F)
+{
+  double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  double uL = sL * UL[1], vL = sL * UL[2], wL = sL * UL[3];
+  double uR = sR * UR[1], vR = sR * UR[2], wR = sR * UR[3];
+  double v2s2L = 0.5 * (uL * uL + vL * vL + wL * wL), v2s2R = 0.5 * (uR * uR + vR * vR + wR * wR);
+  double pL = km1 * (UL[4] - UL[0] * v2s2L), pR = km1 * (UR[4] - UR[0] * v2s2R);
+  double bL = 0.5 * UL[0] / pL, bR = 0.5 * UR[0] / pR;
+  double rho_mean = 0.5 * (UL[0] + UR[0]);
+  double rhoHat = ora_ln_mean(UL[0], UR[0]);
+  double beta_mean = 0.5 * (bL + bR);
+  double beta_hat = ora_ln_mean(bL, bR);
+  double uH = 0.5 * (uL + uR), vH = 0.5 * (vL + vR), wH = 0.5 * (wL + wR);
+  double p1 = 0.5 * rho_mean / beta_mean;
+  double v2m = v2s2L + v2s2R;
+  F[0] = rhoHat * uH;
+  F[1] = F[0] * uH + p1;
+  F[2] = F[0] * vH;
+  F[3] = F[0] * wH;
+  F[4] = F[0] * 0.5 * (skm1 / beta_hat - v2m) + uH * F[1] + vH * F[2] + wH * F[3];
+}
+
+static void flux1d_ranocha(double kappa, const double *UL, const double *UR, double *F)
+{
+  double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  double vL[3] = {UL[1] * sL, UL[2] * sL, UL[3] * sL}, vR[3] = {UR[1] * sR, UR[2] * sR, UR[3] * sR};
+  double v2L = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
+  double v2R = vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2];
+  double pL = km1 * (UL[4] - 0.5 * UL[0] * v2L), pR = km1 * (UR[4] - 0.5 * UR[0] * v2R);
+  double rhoHat = ora_ln_mean(UL[0], UR[0]);
+  double uH = 0.5 * (vL[0] + vR[0]), vH = 0.5 * (vL[1] + vR[1]), wH = 0.5 * (vL[2] + vR[2]);
+  double v2zip = 0.5 * (vL[0] * vR[0] + vL[1] * vR[1] + vL[2] * vR[2]);
+  double p1 = 0.5 * (pL + pR);
+  double rspM = ora_ln_mean(UL[0] / pL, UR[0] / pR);
+  F[0] = rhoHat * uH;
+  F[1] = F[0] * uH + p1;
+  F[2] = F[0] * vH;
+  F[3] = F[0] * wH;
+  F[4] = F[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (pL * vR[0] + pR * vL[0]);
+}
+
+/* Riemann -> AdvRiemann: rotate, solve, rotate back (riemann.f90:174-246); solvers:
+   0 central (:251-272), 1 Rusanov/LLF (:279-313), 9 EC-KEP and 32 Ranocha through
+   RiemannSolver_VolumeFluxAverage (:980-1005). Point-wise (one face node). */
+void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
+                 const double *t1, const double *t2, double *F)
+{
+  double uLL[5], uRR[5];
+  uLL[0] = UL[0]; uRR[0] = UR[0];
+  uLL[1] = UL[1] * nv[0] + UL[2] * nv[1] + UL[3] * nv[2];
+  uLL[2] = UL[1] * t1[0] + UL[2] * t1[1] + UL[3] * t1[2];
+  uLL[3] = UL[1] * t2[0] + UL[2] * t2[1] + UL[3] * t2[2];
+  uRR[1] = UR[1] * nv[0] + UR[2] * nv[1] + UR[3] * nv[2];
+  uRR[2] = UR[1] * t1[0] + UR[2] * t1[1] + UR[3] * t1[2];
+  uRR[3] = UR[1] * t2[0] + UR[2] * t2[1] + UR[3] * t2[2];
+  uLL[4] = UL[4]; uRR[4] = UR[4];
+  double f[5];
+  if (which == 0 || which == 1) {
+    double FL[5], FR[5];
+    euler_flux_1d(kappa, uLL, FL);
+    euler_flux_1d(kappa, uRR, FR);
+    if (which == 0) {
+      for (int v = 0; v < 5; v++) f[v] = 0.5 * (FL[v] + FR[v]);
+    } else {
+      double a = fabs(uLL[1] / uLL[0]), b = fabs(uRR[1] / uRR[0]);
+      double c2L = sound_speed2(kappa, uLL), c2R = sound_speed2(kappa, uRR);
+      double lam = (a > b ? a : b) + sqrt(c2L > c2R ? c2L : c2R);
+      for (int v = 0; v < 5; v++) f[v] = 0.5 * ((FL[v] + FR[v]) - lam * (uRR[v] - uLL[v]));
+    }
+  } else if (which == 9) {
+    flux1d_chandrashekar(kappa, uLL, uRR, f);
+  } else {
+    flux1d_ranocha(kappa, uLL, uRR, f);
+  }
+  F[0] = f[0];
+  F[1] = nv[0] * f[1] + t1[0] * f[2] + t2[0] * f[3];
+  F[2] = nv[1] * f[1] + t1[1] * f[2] + t2[1] * f[3];
+  F[3] = nv[2] * f[1] + t1[2] * f[2] + t2[2] * f[3];
+  F[4] = f[4];
+}
+
+/* ConsToEntropy (equation_vars.f90:362-391) */
+void ora_cons_to_entropy(double kappa, const double *U, double *w)
+{
+  double km1 = kappa - 1.0;
+  double srho = 1.0 / U[0];
+  double u = U[1] * srho, v = U[2] * srho, ww = U[3] * srho;
+  double v2s2 = 0.5 * (u * u + v * v + ww * ww);
+  double rho_sp = U[0] / (km1 * (U[4] - U[0] * v2s2));
+  double s = -log(rho_sp * pow(U[0], km1));
+  w[0] = (kappa - s) / km1 - rho_sp * v2s2;
+  w[1] = rho_sp * u;
+  w[2] = rho_sp * v;
+  w[3] = rho_sp * ww;
+  w[4] = -rho_sp;
+}
+
+/* ConvertToGradPrimVec for entropy-variable lifting (equation_vars.f90:571-647) */
+static void convert_to_grad_prim(double kappa, const double *U, double *g)
+{
+  double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  double srho = 1.0 / U[0];
+  double u = U[1] * srho, v = U[2] * srho, w = U[3] * srho;
+  double ekin = 0.5 * (U[1] * u + U[2] * v + U[3] * w);
+  double p = km1 * (U[4] - ekin);
+  double rho_sp = U[0] / p;
+  double p_srho = p * srho;
+  double g1 = g[0];
+  double gu = p_srho * (g[1] + u * g[4]);
+  double gv = p_srho * (g[2] + v * g[4]);
+  double gw = p_srho * (g[3] + w * g[4]);
+  double g5 = g[4];
+  g[0] = U[0] * g1 - g5 * (ekin - p * skm1) + rho_sp * (U[1] * gu + U[2] * gv + U[3] * gw);
+  g[1] = gu; g[2] = gv; g[3] = gw;
+  g[4] = p_srho * (g[0] + p * g5);
+}
+
+/* EvalDiffFlux3D for constant viscosity (flux.f90:407-517), one point */
+static void diff_flux_3d(const ora_problem *P, const double *U, const double *gx, const double *gy,
+                         const double *gz, double *f, double *g, double *h)
+{
+  double kappa = P->kappa, km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  double KappasPr = kappa / P->Pr, s23 = 2.0 / 3.0;
+  double srho = 1.0 / U[0];
+  double v1 = U[1] * srho, v2 = U[2] * srho, v3 = U[3] * srho;
+  double p = km1 * (U[4] - 0.5 * (U[1] * v1 + U[2] * v2 + U[3] * v3));
+  double muS = P->mu0, lambda = muS * KappasPr;
+  double divv = gx[1] + gy[2] + gz[3];
+  double cTx = skm1 * srho * (gx[4] - srho * p * gx[0]);
+  double cTy = skm1 * srho * (gy[4] - srho * p * gy[0]);
+  double cTz = skm1 * srho * (gz[4] - srho * p * gz[0]);
+  f[0] = 0.0;
+  f[1] = -muS * (2 * gx[1] - s23 * divv);
+  f[2] = -muS * (gx[2] + gy[1]);
+  f[3] = -muS * (gx[3] + gz[1]);
+  f[4] = v1 * f[1] + v2 * f[2] + v3 * f[3] - lambda * cTx;
+  g[0] = 0.0;
+  g[1] = f[2];
+  g[2] = -muS * (2 * gy[2] - s23 * divv);
+  g[3] = -muS * (gy[3] + gz[2]);
+  g[4] = v1 * g[1] + v2 * g[2] + v3 * g[3] - lambda * cTy;
+  h[0] = 0.0;
+  h[1] = f[3];
+  h[2] = g[3];
+  h[3] = -muS * (2 * gz[3] - s23 * divv);
+  h[4] = v1 * h[1] + v2 * h[2] + v3 * h[3] - lambda * cTz;
+}
+
+/* ---------------------------------------------------------------------------- */
+typedef struct {
+  int N, n, n3, E, S;
+  double xGP[16], wGP[16], D[256], Dhat[256], Ds[256], Lhm[16], Lhp[16];
+} ops_t;
+
+static void make_ops(const ora_problem *P, ops_t *o)
+{
+  o->N = P->N; o->n = P->N + 1; o->n3 = o->n * o->n * o->n; o->E = P->nElems; o->S = P->nSides;
+  ora_dg_operators(P->N, o->xGP, o->wGP, o->D, o->Dhat, o->Ds, o->Lhm, o->Lhp);
+}
+
+#define VOL(e, i, j, k) ((((size_t)(e) * n + (k)) * n + (j)) * n + (i))
+#define FACE(s, p, q) (((size_t)(s) * n + (q)) * n + (p))
+
+/* VolInt_SplitForm + EvalAdvFluxAverage3D (volint.f90:173-218, flux_average.f90:198-269).
+   Dvolsurf_T(l,i) = Dvolsurf(i,l). */
+void ora_volint_splitform(const ora_problem *P, const double *U, double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n;
+  double *aux = malloc(sizeof(double) * 6 * o.n3);
+  double *ft = malloc(sizeof(double) * NV * n * o.n3), *gt = malloc(sizeof(double) * NV * n * o.n3),
+         *ht = malloc(sizeof(double) * NV * n * o.n3);
+  const double *Mm[3] = {P->Mf, P->Mg, P->Mh};
+#define T4(l, i, j, k) ((((size_t)(k) * n + (j)) * n + (i)) * n + (l))
+  for (int e = 0; e < o.E; e++) {
+    const double *Ue = U + (size_t)NV * o.n3 * e;
+    for (int p = 0; p < o.n3; p++) ora_eval_uaux(P->kappa, Ue + NV * p, aux + 6 * p);
+    memset(ft, 0, sizeof(double) * NV * n * o.n3);
+    memset(gt, 0, sizeof(double) * NV * n * o.n3);
+    memset(ht, 0, sizeof(double) * NV * n * o.n3);
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          int a = i + n * (j + n * k);
+          for (int l = i + 1; l < n; l++) {
+            int b = l + n * (j + n * k);
+            double F[5];
+            ora_volflux(P->volflux, P->kappa, Ue + NV * a, Ue + NV * b, aux + 6 * a, aux + 6 * b,
+                        Mm[0] + 3 * VOL(e, i, j, k), Mm[0] + 3 * VOL(e, l, j, k), F);
+            for (int v = 0; v < NV; v++) { ft[NV * T4(l, i, j, k) + v] = F[v]; ft[NV * T4(i, l, j, k) + v] = F[v]; }
+          }
+          for (int l = j + 1; l < n; l++) {
+            int b = i + n * (l + n * k);
+            double F[5];
+            ora_volflux(P->volflux, P->kappa, Ue + NV * a, Ue + NV * b, aux + 6 * a, aux + 6 * b,
+                        Mm[1] + 3 * VOL(e, i, j, k), Mm[1] + 3 * VOL(e, i, l, k), F);
+            for (int v = 0; v < NV; v++) { gt[NV * T4(l, i, j, k) + v] = F[v]; gt[NV * T4(j, i, l, k) + v] = F[v]; }
+          }
+          for (int l = k + 1; l < n; l++) {
+            int b = i + n * (j + n * l);
+            double F[5];
+            ora_volflux(P->volflux, P->kappa, Ue + NV * a, Ue + NV * b, aux + 6 * a, aux + 6 * b,
+                        Mm[2] + 3 * VOL(e, i, j, k), Mm[2] + 3 * VOL(e, i, j, l), F);
+            for (int v = 0; v < NV; v++) { ht[NV * T4(l, i, j, k) + v] = F[v]; ht[NV * T4(k, i, j, l) + v] = F[v]; }
+          }
+        }
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double *ut = Ut + NV * VOL(e, i, j, k);
+          for (int l = 0; l < n; l++)
+            for (int v = 0; v < NV; v++)
+              ut[v] = ut[v] + o.Ds[i + n * l] * ft[NV * T4(l, i, j, k) + v]
+                            + o.Ds[j + n * l] * gt[NV * T4(l, i, j, k) + v]
+                            + o.Ds[k + n * l] * ht[NV * T4(l, i, j, k) + v];
+        }
+  }
+#undef T4
+  free(aux); free(ft); free(gt); free(ht);
+}
+
+/* ProlongToFace for LGL: face node = volume node S2V(:,0,p,q,flip,locSide) (prolongtoface.f90:97-102) */
+static void prolong_to_face(const ora_problem *P, const ops_t *o, int nvar, const double *Uv,
+                            double *Um, double *Us)
+{
+  int n = o->n;
+  for (int s = 0; s < o->S; s++) {
+    const int *se = P->SideToElem + 5 * s;
+    int em = se[0] - 1, es = se[1] - 1, lm = se[2], ls = se[3], fl = se[4];
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        int ijk[3];
+        ora_side_to_vol(o->N, 0, p, q, 0, lm, ijk);
+        memcpy(Um + nvar * FACE(s, p, q), Uv + nvar * VOL(em, ijk[0], ijk[1], ijk[2]), sizeof(double) * nvar);
+        ora_side_to_vol(o->N, 0, p, q, fl, ls, ijk);
+        memcpy(Us + nvar * FACE(s, p, q), Uv + nvar * VOL(es, ijk[0], ijk[1], ijk[2]), sizeof(double) * nvar);
+      }
+  }
+}
+
+/* SurfInt for LGL (surfint.f90:177-184, 253-259): master +F*L_hatMinus0, slave -F*L_hatMinus0 */
+static void surf_int(const ora_problem *P, const ops_t *o, int nvar, const double *Fm, const double *Fs,
+                     double sign_slave, double *Ut)
+{
+  int n = o->n;
+  double l0 = o->Lhm[0];
+  for (int s = 0; s < o->S; s++) {
+    const int *se = P->SideToElem + 5 * s;
+    int em = se[0] - 1, es = se[1] - 1, lm = se[2], ls = se[3], fl = se[4];
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        int ijk[3];
+        ora_side_to_vol(o->N, 0, p, q, 0, lm, ijk);
+        double *u = Ut + nvar * VOL(em, ijk[0], ijk[1], ijk[2]);
+        for (int v = 0; v < nvar; v++) u[v] = u[v] + Fm[nvar * FACE(s, p, q) + v] * l0;
+        ora_side_to_vol(o->N, 0, p, q, fl, ls, ijk);
+        u = Ut + nvar * VOL(es, ijk[0], ijk[1], ijk[2]);
+        for (int v = 0; v < nvar; v++) u[v] = u[v] + sign_slave * Fs[nvar * FACE(s, p, q) + v] * l0;
+      }
+  }
+}
+
+/* BR1 lifting (br1.f90:154-364, br1_volint.f90:45-98, br1_fillflux.f90:49-91,
+   br1_surfint.f90:52-131).
+   Output: primitive gradients at volume nodes. */
+void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz)
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n, N = o.N;
+  size_t nvol = (size_t)NV * o.n3 * o.E, nface = (size_t)NV * n * n * o.S;
+  double *w = malloc(sizeof(double) * nvol);
+  double *wm = malloc(sizeof(double) * nface), *ws = malloc(sizeof(double) * nface);
+  double *fx = malloc(sizeof(double) * nface), *fy = malloc(sizeof(double) * nface),
+         *fz = malloc(sizeof(double) * nface);
+  for (size_t p = 0; p < (size_t)o.n3 * o.E; p++) ora_cons_to_entropy(P->kappa, U + NV * p, w + NV * p);
+  memset(gx, 0, sizeof(double) * nvol);
+  memset(gy, 0, sizeof(double) * nvol);
+  memset(gz, 0, sizeof(double) * nvol);
+  /* Lifting_VolInt: strong-form D contraction with the node's metrics */
+  for (int e = 0; e < o.E; e++)
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double gxi[5], geta[5], gzeta[5];
+          for (int v = 0; v < NV; v++) {
+            gxi[v] = o.D[i + n * 0] * w[NV * VOL(e, 0, j, k) + v];
+            geta[v] = o.D[j + n * 0] * w[NV * VOL(e, i, 0, k) + v];
+            gzeta[v] = o.D[k + n * 0] * w[NV * VOL(e, i, j, 0) + v];
+          }
+          for (int l = 1; l <= N; l++)
+            for (int v = 0; v < NV; v++) {
+              gxi[v] = gxi[v] + o.D[i + n * l] * w[NV * VOL(e, l, j, k) + v];
+              geta[v] = geta[v] + o.D[j + n * l] * w[NV * VOL(e, i, l, k) + v];
+              gzeta[v] = gzeta[v] + o.D[k + n * l] * w[NV * VOL(e, i, j, l) + v];
+            }
+          size_t c = VOL(e, i, j, k);
+          const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
+          for (int v = 0; v < NV; v++) {
+            gx[NV * c + v] = gx[NV * c + v] + (mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v]);
+            gy[NV * c + v] = gy[NV * c + v] + (mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v]);
+            gz[NV * c + v] = gz[NV * c + v] + (mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v]);
+          }
+        }
+  /* Lifting_FillFlux: strong flux 1/2(w_s-w_m)*SurfElem*nv (flux.f90:677-716) */
+  prolong_to_face(P, &o, NV, w, wm, ws);
+  for (int s = 0; s < o.S; s++)
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        size_t f = FACE(s, p, q);
+        const double *nv = P->NormVec + 3 * f;
+        double se = P->SurfElem[f];
+        for (int v = 0; v < NV; v++) {
+          double fl = 0.5 * (ws[NV * f + v] - wm[NV * f + v]) * se;
+          fx[NV * f + v] = fl * nv[0];
+          fy[NV * f + v] = fl * nv[1];
+          fz[NV * f + v] = fl * nv[2];
+        }
+      }
+  /* Lifting_SurfInt: both sides add +Flux*L_hatMinus0 */
+  surf_int(P, &o, NV, fx, fx, 1.0, gx);
+  surf_int(P, &o, NV, fy, fy, 1.0, gy);
+  surf_int(P, &o, NV, fz, fz, 1.0, gz);
+  for (size_t p = 0; p < (size_t)o.n3 * o.E; p++) {
+    double sj = P->sJ[p];
+    for (int v = 0; v < NV; v++) {
+      gx[NV * p + v] = gx[NV * p + v] * sj;
+      gy[NV * p + v] = gy[NV * p + v] * sj;
+      gz[NV * p + v] = gz[NV * p + v] * sj;
+    }
+    convert_to_grad_prim(P->kappa, U + NV * p, gx + NV * p);
+    convert_to_grad_prim(P->kappa, U + NV * p, gy + NV * p);
+    convert_to_grad_prim(P->kappa, U + NV * p, gz + NV * p);
+  }
+  free(w); free(wm); free(ws); free(fx); free(fy); free(fz);
+}
+
+/* FillFlux + SurfInt (fillflux.f90:41-110, riemann.f90:104-169, surfint.f90) */
+void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
+                          const double *gz, double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n;
+  size_t nface = (size_t)NV * n * n * o.S;
+  double *Um = malloc(sizeof(double) * nface), *Us = malloc(sizeof(double) * nface);
+  double *Fm = malloc(sizeof(double) * nface);
+  double *gxm = NULL, *gxs = NULL, *gym = NULL, *gys = NULL, *gzm = NULL, *gzs = NULL;
+  prolong_to_face(P, &o, NV, U, Um, Us);
+  if (P->parabolic) {
+    gxm = malloc(sizeof(double) * nface); gxs = malloc(sizeof(double) * nface);
+    gym = malloc(sizeof(double) * nface); gys = malloc(sizeof(double) * nface);
+    gzm = malloc(sizeof(double) * nface); gzs = malloc(sizeof(double) * nface);
+    prolong_to_face(P, &o, NV, gx, gxm, gxs);
+    prolong_to_face(P, &o, NV, gy, gym, gys);
+    prolong_to_face(P, &o, NV, gz, gzm, gzs);
+  }
+  for (int s = 0; s < o.S; s++)
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        size_t f = FACE(s, p, q);
+        double *F = Fm + NV * f;
+        ora_riemann(P->riemann, P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f,
+                    P->TangVec1 + 3 * f, P->TangVec2 + 3 * f, F);
+        if (P->parabolic) {
+          double kL[5], gL[5], jL[5], kR[5], gR[5], jR[5];
+          const double *nv = P->NormVec + 3 * f;
+          diff_flux_3d(P, Um + NV * f, gxm + NV * f, gym + NV * f, gzm + NV * f, kL, gL, jL);
+          diff_flux_3d(P, Us + NV * f, gxs + NV * f, gys + NV * f, gzs + NV * f, kR, gR, jR);
+          for (int v = 1; v < NV; v++)
+            F[v] = F[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
+        }
+        double se = P->SurfElem[f];
+        for (int v = 0; v < NV; v++) F[v] = F[v] * se;
+      }
+  surf_int(P, &o, NV, Fm, Fm, -1.0, Ut);
+  free(Um); free(Us); free(Fm);
+  if (P->parabolic) { free(gxm); free(gxs); free(gym); free(gys); free(gzm); free(gzs); }
+}
+
+/* VolInt_visc + EvalDiffFluxTilde3D (volint.f90:228-271, flux.f90:523-635) */
+static void volint_visc(const ora_problem *P, const double *U, const double *gx, const double *gy,
+                        const double *gz, double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n;
+  double *ftd = malloc(sizeof(double) * NV * o.n3), *gtd = malloc(sizeof(double) * NV * o.n3),
+         *htd = malloc(sizeof(double) * NV * o.n3);
+  for (int e = 0; e < o.E; e++) {
+    for (int p = 0; p < o.n3; p++) {
+      size_t c = (size_t)o.n3 * e + p;
+      double f[5], g[5], h[5];
+      diff_flux_3d(P, U + NV * c, gx + NV * c, gy + NV * c, gz + NV * c, f, g, h);
+      const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
+      for (int v = 0; v < NV; v++) {
+        ftd[NV * p + v] = f[v] * mf[0] + g[v] * mf[1] + h[v] * mf[2];
+        gtd[NV * p + v] = f[v] * mg[0] + g[v] * mg[1] + h[v] * mg[2];
+        htd[NV * p + v] = f[v] * mh[0] + g[v] * mh[1] + h[v] * mh[2];
+      }
+    }
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double *ut = Ut + NV * VOL(e, i, j, k);
+          for (int l = 0; l < n; l++)
+            for (int v = 0; v < NV; v++)
+              ut[v] = ut[v] + o.Dhat[i + n * l] * ftd[NV * (l + n * (j + n * k)) + v]
+                            + o.Dhat[j + n * l] * gtd[NV * (i + n * (l + n * k)) + v]
+                            + o.Dhat[k + n * l] * htd[NV * (i + n * (j + n * l)) + v];
+        }
+  }
+  free(ftd); free(gtd); free(htd);
+}
+
+/* DGTimeDerivative_weakForm (dg.f90:282-452) for periodic conforming LGL meshes */
+void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, double *Ut)
+{
+  (void)t;
+  size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  memset(Ut, 0, sizeof(double) * NV * n3E);
+  ora_volint_splitform(P, U, Ut);
+  double *gx = NULL, *gy = NULL, *gz = NULL;
+  if (P->parabolic) {
+    gx = malloc(sizeof(double) * NV * n3E); gy = malloc(sizeof(double) * NV * n3E);
+    gz = malloc(sizeof(double) * NV * n3E);
+    ora_lifting(P, U, gx, gy, gz);
+    volint_visc(P, U, gx, gy, gz, Ut);
+  }
+  ora_surface_integral(P, U, gx, gy, gz, Ut);
+  for (size_t p = 0; p < n3E; p++) {
+    double s = -P->sJ[p];
+    for (int v = 0; v < NV; v++) Ut[NV * p + v] = Ut[NV * p + v] * s;
+  }
+  if (P->parabolic) { free(gx); free(gy); free(gz); }
+}
+
+/* Carpenter RK4-5 (timedisc_vars.f90:204-233) */
+void ora_rk_coefs(double *A, double *b, double *c)
+{
+  A[0] = 0.0;
+  A[1] = 567301805773.0 / 1357537059087.0;
+  A[2] = 2404267990393.0 / 2016746695238.0;
+  A[3] = 3550918686646.0 / 2091501179385.0;
+  A[4] = 1275806237668.0 / 842570457699.0;
+  b[0] = 1432997174477.0 / 9575080441755.0;
+  b[1] = 5161836677717.0 / 13612068292357.0;
+  b[2] = 1720146321549.0 / 2090206949498.0;
+  b[3] = 3134564353537.0 / 4481467310338.0;
+  b[4] = 2277821191437.0 / 14882151754819.0;
+  c[0] = 0.0;
+  c[1] = 1432997174477.0 / 9575080441755.0;
+  c[2] = 2526269341429.0 / 6820363962896.0;
+  c[3] = 2006345519317.0 / 3224310063776.0;
+  c[4] = 2802321613138.0 / 2924317926251.0;
+}
+
+/* CFLScaleAlpha for LGL + Carpenter RK4-5 (timedisc_vars.f90:212-213) */
+double ora_cfl_alpha(int N)
+{
+  static const double a[15] = {4.7497, 3.4144, 2.8451, 2.4739, 2.2027, 1.9912, 1.8225, 1.6830,
+                               1.5682, 1.4692, 1.3849, 1.3106, 1.2454, 1.1880, 1.1362};
+  return a[(N < 15 ? N : 15) - 1];
+}
+
+/* DFLScaleAlpha(LGL) * RelativeDFL(RK4-5 = 1.853) (timedisc_vars.f90:79-84, 214-216) */
+double ora_dfl_alpha(int N)
+{
+  static const double a[10] = {4.50, 1.95, 1.18, 0.79, 0.56, 0.42, 0.32, 0.25, 0.20, 0.17};
+  return a[(N < 10 ? N : 10) - 1] * 1.853;
+}
+
+/* TimeStepByLSERKW2 (timedisc.f90:424-480) with VCopy/VAXPBY (vector.f90:68-117) */
+void ora_lserk_step(const ora_problem *P, double *U, double *Ut_tmp, double t, double dt)
+{
+  double A[5], b[5], c[5];
+  ora_rk_coefs(A, b, c);
+  size_t nt = (size_t)NV * (P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  double *Ut = malloc(sizeof(double) * nt);
+  for (int s = 0; s < 5; s++) {
+    double bdt = b[s] * dt;
+    ora_dg_time_derivative(P, U, t + (s ? dt * c[s] : 0.0), Ut);
+    if (s == 0) {
+      for (size_t i = 0; i < nt; i++) { Ut_tmp[i] = Ut[i]; U[i] = U[i] + Ut[i] * bdt; }
+    } else {
+      for (size_t i = 0; i < nt; i++) {
+        Ut_tmp[i] = Ut_tmp[i] * (-A[s]) + Ut[i];
+        U[i] = U[i] + Ut_tmp[i] * bdt;
+      }
+    }
+  }
+  free(Ut);
+}
+
+/* CALCTIMESTEP (calctimestep.f90:39-170) with FillCFL_DFL (timedisc.f90:697-741) */
+double ora_calc_timestep(const ora_problem *P, const double *U, double CFLscale, double DFLscale,
+                         int *errType)
+{
+  int N = P->N, n = N + 1, n3 = n * n * n;
+  double kappa = P->kappa, km1 = kappa - 1.0;
+  double cfl = CFLscale * ora_cfl_alpha(N) / (2.0 * N + 1.0);
+  double dfl = DFLscale * ora_dfl_alpha(N) / ((2.0 * N + 1.0) * (2.0 * N + 1.0));
+  double KsPr = kappa / P->Pr, KsPr_max = (4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr;
+  double dtc = 1e308, dtv = 1e308;
+  *errType = 0;
+  for (int e = 0; e < P->nElems; e++) {
+    double l1 = 1e-12, l2 = 1e-12, l3 = 1e-12, lv1 = 1e-12, lv2 = 1e-12, lv3 = 1e-12;
+    for (int p = 0; p < n3; p++) {
+      size_t c = (size_t)n3 * e + p;
+      const double *u = U + NV * c;
+      if (isnan(u[0])) *errType = 1;
+      double srho = 1.0 / u[0];
+      double v[3] = {u[1] * srho, u[2] * srho, u[3] * srho};
+      double pr = km1 * (u[4] - 0.5 * u[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      if (pr <= 0.0) *errType = 2;
+      double cs = sqrt(kappa * pr * srho), sj = P->sJ[c];
+      const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
+      double a1 = sj * (fabs(mf[0] * v[0] + mf[1] * v[1] + mf[2] * v[2]) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
+      double a2 = sj * (fabs(mg[0] * v[0] + mg[1] * v[1] + mg[2] * v[2]) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
+      double a3 = sj * (fabs(mh[0] * v[0] + mh[1] * v[1] + mh[2] * v[2]) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
+      if (a1 > l1) l1 = a1;
+      if (a2 > l2) l2 = a2;
+      if (a3 > l3) l3 = a3;
+      if (P->parabolic) {
+        double muX = srho * KsPr_max * P->mu0;
+        double b1 = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
+        double b2 = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
+        double b3 = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
+        if (b1 > lv1) lv1 = b1;
+        if (b2 > lv2) lv2 = b2;
+        if (b3 > lv3) lv3 = b3;
+      }
+    }
+    double dte = cfl * 2.0 / (l1 + l2 + l3);
+    if (dte < dtc) dtc = dte;
+    if (isnan(dtc)) *errType = 3;
+    if (P->parabolic) {
+      double dve = dfl * 4.0 / (lv1 + lv2 + lv3);
+      if (dve < dtv) dtv = dve;
+      if (isnan(dtv)) *errType = 4;
+    }
+    if (*errType) break;
+  }
+  return dtc < dtv ? dtc : dtv;
+}
+
+/* CalcEntropy (analyze_equation.f90:549-599) */
+void ora_calc_entropy(const ora_problem *P, const double *U, const double *Ut, double *S, double *dSdU_Ut)
+{
+  int N = P->N, n = N + 1, n3 = n * n * n;
+  double xg[16], wg[16];
+  ora_lgl_nodes(N, xg, wg);
+  double kappa = P->kappa, km1 = kappa - 1.0;
+  double ent = 0.0, dsu = 0.0;
+  for (int e = 0; e < P->nElems; e++)
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          size_t c = (size_t)n3 * e + i + n * (j + n * k);
+          const double *u = U + NV * c;
+          double srho = 1.0 / u[0];
+          double v1 = u[1] * srho, v2 = u[2] * srho, v3 = u[3] * srho;
+          double pr = km1 * (u[4] - 0.5 * (u[1] * v1 + u[2] * v2 + u[3] * v3));
+          double wv = wg[i] * wg[j] * wg[k];
+          double el = -u[0] * (log(pr) - kappa * log(u[0]));
+          ent = ent + el * wv / P->sJ[c];
+          double w[5];
+          ora_cons_to_entropy(kappa, u, w);
+          double s = 0.0;
+          for (int v = 0; v < NV; v++) s += w[v] * Ut[NV * c + v];
+          dsu = dsu + s * wv / P->sJ[c];
+        }
+  *S = ent / km1;
+  *dSdU_Ut = dsu;
+}
+
+/* ExactFunc 2 (density wave) and 7 (Taylor-Green vortex) (equation.f90:632-651, 802-811) */
+void ora_exactfunc(int which, double kappa, const double *advvel, double amp, double t,
+                   int npts, const double *x, double *U)
+{
+  const double pi = acos(-1.0);
+  double skm1 = 1.0 / (kappa - 1.0);
+  for (int p = 0; p < npts; p++) {
+    const double *xp = x + 3 * p;
+    double *u = U + NV * p;
+    if (which == 2) {
+      double prim[5] = {1.0, advvel[0], advvel[1], advvel[2], 1.0};
+      double cent[3] = {xp[0] - advvel[0] * t, xp[1] - advvel[1] * t, xp[2] - advvel[2] * t};
+      prim[0] = prim[0] * (1.0 + amp * sin(pi * (1.0 * cent[0] + 1.0 * cent[1] + 1.0 * cent[2])));
+      u[0] = prim[0];
+      u[1] = prim[0] * prim[1]; u[2] = prim[0] * prim[2]; u[3] = prim[0] * prim[3];
+      u[4] = prim[4] * skm1 + 0.5 * prim[0] * (prim[1] * prim[1] + prim[2] * prim[2] + prim[3] * prim[3]);
+    } else {
+      double A = 1.0, Ms = 0.1, prim[5];
+      prim[0] = 1.0;
+      prim[1] = A * sin(xp[0]) * cos(xp[1]) * cos(xp[2]);
+      prim[2] = -A * cos(xp[0]) * sin(xp[1]) * cos(xp[2]);
+      prim[3] = 0.0;
+      prim[4] = (A / Ms * A / Ms / kappa * prim[0]);
+      prim[4] = prim[4] + 1.0 / 16.0 * A * A * prim[0] *
+                (cos(2 * xp[0]) * cos(2.0 * xp[2]) + 2.0 * cos(2.0 * xp[1]) + 2.0 * cos(2.0 * xp[0]) +
+                 cos(2 * xp[1]) * cos(2.0 * xp[2]));
+      /* PrimToCons (equation_vars.f90) */
+      u[0] = prim[0];
+      u[1] = prim[1] * prim[0]; u[2] = prim[2] * prim[0]; u[3] = prim[3] * prim[0];
+      u[4] = skm1 * prim[4] + 0.5 * (u[1] * prim[1] + u[2] * prim[2] + u[3] * prim[3]);
+    }
+  }
+}

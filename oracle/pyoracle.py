@@ -1,0 +1,213 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle is a plain-C restatement of the reference
+(project-fluxo/fluxo) split-form DGSEM path.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may import this module, and only as the checker or
+the reported CPU baseline.
+
+Array shapes use C order with the reference's Fortran index order reversed, so the
+memory layout equals the reference's column-major arrays:
+  U(nVar,0:N,0:N,0:N,nElems)  <->  numpy (nElems, n, n, n, nVar)
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.ora_ln_mean.restype = C.c_double
+        L.ora_ln_mean.argtypes = [C.c_double, C.c_double]
+        L.ora_box_connectivity.restype = C.c_int
+        L.ora_box_connectivity.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip]
+        L.ora_box_nodecoords.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                         C.c_double, _dp]
+        L.ora_calc_metrics.restype = C.c_int
+        L.ora_calc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip] + [_dp] * 9
+        L.ora_calc_timestep.restype = C.c_double
+        L.ora_calc_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double,
+                                        C.POINTER(C.c_int)]
+        L.ora_dg_time_derivative.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ora_volint_splitform.argtypes = [C.c_void_p, _dp, _dp]
+        L.ora_lifting.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ora_lserk_step.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double]
+        L.ora_calc_entropy.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.ora_exactfunc.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_double, C.c_int,
+                                    _dp, _dp]
+        L.ora_volflux.argtypes = [C.c_int, C.c_double] + [_dp] * 7
+        L.ora_riemann.argtypes = [C.c_int, C.c_double] + [_dp] * 6
+        L.ora_cons_to_entropy.argtypes = [C.c_double, _dp, _dp]
+        L.ora_rk_coefs.argtypes = [_dp, _dp, _dp]
+        L.ora_lgl_nodes.argtypes = [C.c_int, _dp, _dp]
+        L.ora_dg_operators.argtypes = [C.c_int] + [_dp] * 7
+        _LIB = L
+    return _LIB
+
+
+class _Problem(C.Structure):
+    _fields_ = [("N", C.c_int), ("nVar", C.c_int), ("nElems", C.c_int), ("nSides", C.c_int),
+                ("volflux", C.c_int), ("riemann", C.c_int), ("parabolic", C.c_int),
+                ("lifting", C.c_int),
+                ("kappa", C.c_double), ("mu0", C.c_double), ("Pr", C.c_double),
+                ("etaBR2", C.c_double),
+                ("ElemToSide", C.c_void_p), ("SideToElem", C.c_void_p),
+                ("Mf", C.c_void_p), ("Mg", C.c_void_p), ("Mh", C.c_void_p), ("sJ", C.c_void_p),
+                ("NormVec", C.c_void_p), ("TangVec1", C.c_void_p), ("TangVec2", C.c_void_p),
+                ("SurfElem", C.c_void_p)]
+
+
+def box_mesh(K, N, x0=(0.0, 0.0, 0.0), length=(2.0, 2.0, 2.0), Ngeo=1, warp=False, amp=0.1):
+    """Periodic box mesh: connectivity, node coordinates and FLUXO metrics."""
+    L = lib()
+    Kx, Ky, Kz = (K, K, K) if np.isscalar(K) else K
+    E = Kx * Ky * Kz
+    S = 3 * E
+    n, ng = N + 1, Ngeo + 1
+    e2s = np.zeros((E, 6, 2), np.int32)
+    s2e = np.zeros((S, 5), np.int32)
+    assert L.ora_box_connectivity(Kx, Ky, Kz, e2s, s2e) == E
+    X = np.zeros((E, ng, ng, ng, 3))
+    L.ora_box_nodecoords(Kx, Ky, Kz, np.array(x0, float), np.array(length, float), Ngeo,
+                         int(bool(warp)), amp, X)
+    m = dict(N=N, Ngeo=Ngeo, nElems=E, nSides=S, K=(Kx, Ky, Kz), ElemToSide=e2s, SideToElem=s2e,
+             NodeCoords=X)
+    m["Mf"] = np.zeros((E, n, n, n, 3)); m["Mg"] = np.zeros((E, n, n, n, 3))
+    m["Mh"] = np.zeros((E, n, n, n, 3)); m["sJ"] = np.zeros((E, n, n, n))
+    for k in ("NormVec", "TangVec1", "TangVec2"):
+        m[k] = np.zeros((S, n, n, 3))
+    m["SurfElem"] = np.zeros((S, n, n))
+    m["Elem_xGP"] = np.zeros((E, n, n, n, 3))
+    rc = L.ora_calc_metrics(N, Ngeo, E, S, X, e2s, m["Mf"], m["Mg"], m["Mh"], m["sJ"], m["NormVec"],
+                            m["TangVec1"], m["TangVec2"], m["SurfElem"], m["Elem_xGP"])
+    if rc != 0:
+        raise ValueError("mesh has non-positive or badly scaled Jacobians")
+    return m
+
+
+class Problem:
+    """An equation/mesh setup; keeps references to every array the C struct points to."""
+
+    def __init__(self, mesh, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
+                 lifting=1):
+        self.mesh = mesh
+        self.N = mesh["N"]
+        self.nElems = mesh["nElems"]
+        self.kappa = kappa
+        self.parabolic = bool(parabolic)
+        self.mu0 = mu0
+        self.Pr = Pr
+        self.volflux, self.riemann = volflux, riemann
+        p = _Problem()
+        p.N, p.nVar, p.nElems, p.nSides = self.N, 5, mesh["nElems"], mesh["nSides"]
+        p.volflux, p.riemann, p.parabolic, p.lifting = volflux, riemann, int(bool(parabolic)), lifting
+        p.kappa, p.mu0, p.Pr, p.etaBR2 = kappa, mu0, Pr, 2.0
+        for k in ("ElemToSide", "SideToElem", "Mf", "Mg", "Mh", "sJ", "NormVec", "TangVec1",
+                  "TangVec2", "SurfElem"):
+            setattr(p, k, mesh[k].ctypes.data)
+        self._p = p
+
+    @property
+    def ptr(self):
+        return C.byref(self._p)
+
+    def shape(self, nvar=5):
+        n = self.N + 1
+        return (self.nElems, n, n, n, nvar)
+
+    def rhs(self, U, t=0.0):
+        Ut = np.zeros(self.shape())
+        lib().ora_dg_time_derivative(self.ptr, np.ascontiguousarray(U), t, Ut)
+        return Ut
+
+    def volint(self, U):
+        Ut = np.zeros(self.shape())
+        lib().ora_volint_splitform(self.ptr, np.ascontiguousarray(U), Ut)
+        return Ut
+
+    def lifting(self, U):
+        g = [np.zeros(self.shape()) for _ in range(3)]
+        lib().ora_lifting(self.ptr, np.ascontiguousarray(U), *g)
+        return g
+
+    def lserk_step(self, U, Ut_tmp, t, dt):
+        lib().ora_lserk_step(self.ptr, U, Ut_tmp, t, dt)
+
+    def calc_dt(self, U, cfl, dfl=0.0):
+        err = C.c_int(0)
+        dt = lib().ora_calc_timestep(self.ptr, np.ascontiguousarray(U), cfl, dfl, C.byref(err))
+        return dt, err.value
+
+    def entropy(self, U, Ut):
+        s, d = C.c_double(), C.c_double()
+        lib().ora_calc_entropy(self.ptr, np.ascontiguousarray(U), np.ascontiguousarray(Ut),
+                               C.byref(s), C.byref(d))
+        return s.value, d.value
+
+
+def exactfunc(which, x, kappa=1.4, advvel=(1.0, 1.0, 1.0), amp=0.1, t=0.0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    npts = x.size // 3
+    U = np.zeros(x.shape[:-1] + (5,))
+    lib().ora_exactfunc(which, kappa, np.array(advvel, float), amp, t, npts, x, U)
+    return U
+
+
+def ln_mean(a, b):
+    return lib().ora_ln_mean(a, b)
+
+
+def volflux(which, UL, UR, mL, mR, kappa=1.4):
+    UL, UR = np.asarray(UL, float), np.asarray(UR, float)
+    aL, aR = np.zeros(6), np.zeros(6)
+    lib().ora_eval_uaux.argtypes = [C.c_double, _dp, _dp]
+    lib().ora_eval_uaux(kappa, UL, aL)
+    lib().ora_eval_uaux(kappa, UR, aR)
+    F = np.zeros(5)
+    lib().ora_volflux(which, kappa, UL, UR, aL, aR, np.asarray(mL, float), np.asarray(mR, float), F)
+    return F
+
+
+def riemann(which, UL, UR, nv, t1, t2, kappa=1.4):
+    F = np.zeros(5)
+    lib().ora_riemann(which, kappa, *(np.asarray(a, float) for a in (UL, UR, nv, t1, t2)), F)
+    return F
+
+
+def cons_to_entropy(U, kappa=1.4):
+    w = np.zeros(5)
+    lib().ora_cons_to_entropy(kappa, np.asarray(U, float), w)
+    return w
+
+
+def rk_coefs():
+    A, b, c = np.zeros(5), np.zeros(5), np.zeros(5)
+    lib().ora_rk_coefs(A, b, c)
+    return A, b, c
+
+
+def dg_operators(N):
+    n = N + 1
+    x, w = np.zeros(n), np.zeros(n)
+    D, Dh, Ds = np.zeros(n * n), np.zeros(n * n), np.zeros(n * n)
+    Lm, Lp = np.zeros(n), np.zeros(n)
+    lib().ora_dg_operators(N, x, w, D, Dh, Ds, Lm, Lp)
+    f = lambda a: a.reshape(n, n).T.copy()  # column-major -> A[r, c]
+    return dict(x=x, w=w, D=f(D), D_hat=f(Dh), Dvolsurf=f(Ds), L_hatminus=Lm, L_hatplus=Lp)
