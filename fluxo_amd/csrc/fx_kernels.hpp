@@ -1,0 +1,578 @@
+// HIP kernels of the split-form DGSEM right-hand side for gfx950 (CDNA4).
+//
+// Data in HBM keeps the reference's layout (variables fastest, then i, j, k, element):
+//   U, Ut, Ut_tmp : nVar x (N+1)^3 x nElems doubles
+//   Mf, Mg, Mh    : 3 x (N+1)^3 x nElems       sJ : (N+1)^3 x nElems
+//   nvse          : per side face point (nx, ny, nz, SurfElem)
+//   Flux          : nVar x (N+1)^2 x nSides (Flux_master * SurfElem; slave uses -Flux)
+//   Fv            : 12 x (N+1)^3 x nElems, Cartesian viscous fluxes (f,g,h; momentum+energy)
+//
+// Kernel roles per RHS (one LSERK stage):
+//   k_lifting  (Navier-Stokes only) element-centric BR1 gradients -> viscous fluxes Fv
+//   k_surface  side-centric numerical flux at every face point  -> Flux
+//   k_volume   element-centric flux differencing + viscous volume term + surface
+//              gather + Jacobian + (optionally) the low-storage RK stage update.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fx_physics.hpp"
+
+namespace fx {
+
+constexpr int MAXN = 7;  // highest supported polynomial degree
+
+// Basis operators for one N, in constant memory (uniform loads go through the scalar cache).
+struct Ops {
+  double Ds[MAXN + 1][MAXN + 1];    // Dvolsurf(i,l)  (dg.f90:241-249)
+  double Dhat[MAXN + 1][MAXN + 1];  // D_Hat(i,l)     (dg.f90:214-216)
+  double D[MAXN + 1][MAXN + 1];     // D(i,l)         (dg.f90:210)
+  double w[MAXN + 1];               // LGL weights
+  double L0;                        // L_HatMinus0 = L_HatMinus(0) (dg.f90:228-229)
+};
+extern __constant__ Ops c_ops[MAXN + 1];
+
+// local side ids (defines.h:68-73)
+enum { ZETA_MINUS = 1, ETA_MINUS = 2, XI_PLUS = 3, ETA_PLUS = 4, XI_MINUS = 5, ZETA_PLUS = 6 };
+
+// Flip_S2M (mappings.f90:262-284); an involution, so it also maps master->slave.
+__device__ __forceinline__ void flip_pq(int N, int flip, int &p, int &q)
+{
+  int a = p, b = q;
+  switch (flip) {
+  case 1: p = b; q = a; break;
+  case 2: p = N - a; q = b; break;
+  case 3: p = N - b; q = N - a; break;
+  case 4: p = a; q = N - b; break;
+  default: break;
+  }
+}
+
+// Volume node (i + n*(j + n*k)) of side point (p,q) given in the master frame:
+// SideToVol(N,0,p,q,flip,locSide) (mappings.f90:459-472).
+template <int N>
+__device__ __forceinline__ int side_to_vol(int loc, int flip, int p, int q)
+{
+  constexpr int n = N + 1;
+  flip_pq(N, flip, p, q);
+  int i, j, k;
+  switch (loc) {
+  case XI_MINUS: i = 0; j = q; k = p; break;
+  case XI_PLUS: i = N; j = p; k = q; break;
+  case ETA_MINUS: i = p; j = 0; k = q; break;
+  case ETA_PLUS: i = N - p; j = N; k = q; break;
+  case ZETA_MINUS: i = q; j = p; k = 0; break;
+  default: i = p; j = q; k = N; break;
+  }
+  return i + n * (j + n * k);
+}
+
+// Master-frame side point of volume node (i,j,k): VolToSide (mappings.f90:412-426).
+template <int N>
+__device__ __forceinline__ void vol_to_side(int loc, int flip, int i, int j, int k, int &p, int &q)
+{
+  switch (loc) {
+  case XI_MINUS: p = k; q = j; break;
+  case XI_PLUS: p = j; q = k; break;
+  case ETA_MINUS: p = i; q = k; break;
+  case ETA_PLUS: p = N - i; q = k; break;
+  case ZETA_MINUS: p = j; q = i; break;
+  default: p = i; q = j; break;
+  }
+  flip_pq(N, flip, p, q);
+}
+
+// -----------------------------------------------------------------------------------
+// Launch geometry of the volume kernel: EPB elements per block, W threads per
+// direction (a multiple of the 64-wide wavefront), 3*W threads per block.
+template <int N> struct VolGeom;
+template <> struct VolGeom<1> { static constexpr int EPB = 16, W = 64; };
+template <> struct VolGeom<2> { static constexpr int EPB = 7, W = 64; };
+template <> struct VolGeom<3> { static constexpr int EPB = 4, W = 64; };
+template <> struct VolGeom<4> { static constexpr int EPB = 5, W = 128; };
+template <> struct VolGeom<5> { static constexpr int EPB = 3, W = 128; };
+template <> struct VolGeom<6> { static constexpr int EPB = 1, W = 64; };
+template <> struct VolGeom<7> { static constexpr int EPB = 1, W = 64; };
+
+// LDS node index with the xi-rows padded to n+1 so xi-line owners (stride n) do not
+// hit one bank.
+template <int N> struct Lds {
+  static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  static constexpr int NP = (n + 1) * n2;  // padded nodes per element
+  __device__ static __forceinline__ int li(int i, int j, int k) { return i + (n + 1) * (j + n * k); }
+  __device__ static __forceinline__ int li(int node)
+  {
+    int i = node % n, jk = node / n;
+    return i + (n + 1) * jk;
+  }
+};
+
+enum { VMODE_RHS = 0, VMODE_RK = 1, VMODE_VOLINT = 2 };
+
+struct RKArgs {
+  const double *dt;  // device scalar (time step of the current step)
+  double A, b;       // RKA(stage) (0 for stage 1) and RKb(stage)
+  int first;         // 1 on stage 1 (Ut_tmp = Ut)
+};
+
+// Element-centric volume kernel (VolInt_SplitForm volint.f90:173-218 + VolInt_visc
+// volint.f90:228-271 + SurfInt gather surfint.f90 + V2D_M_V1D(-sJ) dg.f90:446 +
+// the LSERK stage update timedisc.f90:454-470).
+template <int N, int VF, bool PARA, int MODE>
+__global__ void __launch_bounds__(3 * VolGeom<N>::W)
+k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
+         const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
+         const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
+         double *__restrict__ Uttmp, RKArgs rk, int nElems, double kappa)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int EPB = VolGeom<N>::EPB, W = VolGeom<N>::W, NP = Lds<N>::NP;
+  static_assert(EPB * n2 <= W, "volume launch geometry");
+  __shared__ double s_nd[EPB][NF_COUNT][NP];
+  __shared__ double s_acc[2][EPB][5][NP];
+
+  const Ops &op = c_ops[N];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * EPB;
+  const double kappaM1 = kappa - 1.0, skappaM1 = 1.0 / kappaM1;
+
+  // phase 1: per-node fields into LDS
+  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+    int el = idx / n3, node = idx - el * n3;
+    int e = e0 + el;
+    if (e < nElems) {
+      const double *u = U + ((size_t)e * n3 + node) * 5;
+      double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
+      double f[NF_COUNT];
+      node_fields<VF>(uu, kappaM1, f);
+      int l = Lds<N>::li(node);
+#pragma unroll
+      for (int c = 0; c < NF_COUNT; c++) s_nd[el][c][l] = f[c];
+    }
+  }
+  __syncthreads();
+
+  // phase 2: one thread per (direction, element, line)
+  const int dir = tid / W, lt = tid - dir * W;
+  const int el = lt / n2, line = lt - el * n2;
+  const int e = e0 + el;
+  const bool active = (el < EPB) && (e < nElems);
+  const int a = line % n, b = line / n;
+  double acc[n][5];
+#pragma unroll
+  for (int i = 0; i < n; i++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) acc[i][v] = 0.0;
+
+  if (active) {
+    int nodes[n], lds[n];
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+      int i = (dir == 0) ? l : a, j = (dir == 1) ? l : (dir == 0 ? a : b), k = (dir == 2) ? l : b;
+      nodes[l] = i + n * (j + n * k);
+      lds[l] = Lds<N>::li(i, j, k);
+    }
+    const double *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
+    double m[n][3];
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+      const double *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
+      m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
+    }
+    if (MODE == VMODE_VOLINT || true) {
+      double nd[n][NF_COUNT];
+#pragma unroll
+      for (int l = 0; l < n; l++)
+#pragma unroll
+        for (int c = 0; c < NF_COUNT; c++) nd[l][c] = s_nd[el][c][lds[l]];
+#pragma unroll
+      for (int i = 0; i < n; i++)
+#pragma unroll
+        for (int l = i + 1; l < n; l++) {
+          double F[5];
+          volume_flux<VF, double>(nd[i], nd[l], m[i], m[l], skappaM1, F);
+          const double dil = op.Ds[i][l], dli = op.Ds[l][i];
+#pragma unroll
+          for (int v = 0; v < 5; v++) {
+            acc[i][v] += dil * F[v];
+            acc[l][v] += dli * F[v];
+          }
+        }
+    }
+    if (PARA && MODE != VMODE_VOLINT) {
+      // transformed viscous flux along the line: ft(l) = f*M1 + g*M2 + h*M3
+      double ft[n][4];
+#pragma unroll
+      for (int l = 0; l < n; l++) {
+        const double *fv = Fv + ((size_t)e * n3 + nodes[l]) * 12;
+#pragma unroll
+        for (int c = 0; c < 4; c++) ft[l][c] = fv[c] * m[l][0] + fv[4 + c] * m[l][1] + fv[8 + c] * m[l][2];
+      }
+#pragma unroll
+      for (int i = 0; i < n; i++)
+#pragma unroll
+        for (int l = 0; l < n; l++) {
+          const double dh = op.Dhat[i][l];
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[i][c + 1] += dh * ft[l][c];
+        }
+    }
+    if (dir > 0) {
+#pragma unroll
+      for (int l = 0; l < n; l++)
+#pragma unroll
+        for (int v = 0; v < 5; v++) s_acc[dir - 1][el][v][lds[l]] = acc[l][v];
+    }
+  }
+  __syncthreads();
+
+  // phase 3: xi-line owners finish their n nodes
+  if (active && dir == 0) {
+    const int j = a, k = b;
+    int sideE[6], flipE[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      sideE[s] = e2s[(e * 6 + s) * 2] - 1;
+      flipE[s] = e2s[(e * 6 + s) * 2 + 1];
+    }
+    const double L0 = op.L0;
+#pragma unroll
+    for (int i = 0; i < n; i++) {
+      const int node = i + n * (j + n * k);
+      const int l = Lds<N>::li(i, j, k);
+      double tot[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) tot[v] = acc[i][v] + s_acc[0][el][v][l] + s_acc[1][el][v][l];
+      if (MODE != VMODE_VOLINT) {
+        // SurfInt gather: faces touched by node (i,j,k), in local side order
+        int locs[3], nl = 0;
+        if (k == 0) locs[nl++] = ZETA_MINUS;
+        if (j == 0) locs[nl++] = ETA_MINUS;
+        if (i == N) locs[nl++] = XI_PLUS;
+        if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
+        if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
+        if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+        for (int t = 0; t < nl; t++) {
+          const int loc = locs[t];
+          const int s = sideE[loc - 1], fl = flipE[loc - 1];
+          int p, q;
+          vol_to_side<N>(loc, fl, i, j, k, p, q);
+          const double *F = Flux + ((size_t)(s * n + q) * n + p) * 5;
+          if (fl == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) tot[v] = tot[v] + F[v] * L0;
+          } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) tot[v] = tot[v] - F[v] * L0;
+          }
+        }
+        const double msj = -sJ[(size_t)e * n3 + node];
+#pragma unroll
+        for (int v = 0; v < 5; v++) tot[v] = tot[v] * msj;
+      }
+      const size_t o = ((size_t)e * n3 + node) * 5;
+      if (MODE == VMODE_RK) {
+        const double bdt = rk.b * (*rk.dt);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
+          Uttmp[o + v] = ut;
+          U[o + v] = U[o + v] + ut * bdt;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) Ut[o + v] = tot[v];
+      }
+    }
+  }
+}
+
+// Side-centric numerical flux: Flux(:,p,q,s) = F*(U_master, U_slave; n) * SurfElem, with
+// the BR1 viscous average for Navier-Stokes (FillFlux fillflux.f90:41-110, Riemann
+// riemann.f90:104-169).
+template <int N, int RS, bool PARA>
+__global__ void __launch_bounds__(256)
+k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int *__restrict__ s2e,
+          const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nSides * n2) return;
+  const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
+  const int em = s2e[5 * s] - 1, es = s2e[5 * s + 1] - 1;
+  const int lm = s2e[5 * s + 2], ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
+  const int vm = side_to_vol<N>(lm, 0, p, q), vs = side_to_vol<N>(ls, fl, p, q);
+  const double *um = U + ((size_t)em * n3 + vm) * 5, *us = U + ((size_t)es * n3 + vs) * 5;
+  double UL[5] = {um[0], um[1], um[2], um[3], um[4]};
+  double UR[5] = {us[0], us[1], us[2], us[3], us[4]};
+  const double4 g = nvse[gid];
+  const double nv[3] = {g.x, g.y, g.z};
+  double F[5];
+  riemann<RS>(UL, UR, nv, kappa, F);
+  if (PARA) {
+    const double *fL = Fv + ((size_t)em * n3 + vm) * 12, *fR = Fv + ((size_t)es * n3 + vs) * 12;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      F[c + 1] = F[c + 1] + 0.5 * (nv[0] * (fL[c] + fR[c]) + nv[1] * (fL[4 + c] + fR[4 + c]) +
+                                   nv[2] * (fL[8 + c] + fR[8 + c]));
+  }
+  double *o = Flux + (size_t)gid * 5;
+#pragma unroll
+  for (int v = 0; v < 5; v++) o[v] = F[v] * g.w;
+}
+
+// BR1 lifting with entropy variables (Lifting br1.f90:154-364: Lifting_VolInt
+// br1_volint.f90:45-98, Lifting_FillFlux br1_fillflux.f90:49-91 with EvalLiftingSurfFlux
+// flux.f90:677-716, Lifting_SurfInt br1_surfint.f90:52-131, V2D_M_V1D(sJ),
+// ConvertToGradPrimVec).  Element-centric: the surface term of every face is formed from
+// the neighbour's face node directly.  OUT=0 writes the primitive gradients (gradPx/y/z),
+// OUT=1 writes the Cartesian viscous fluxes (EvalDiffFlux3D) consumed by the surface and
+// volume kernels.
+template <int N> struct LiftGeom { static constexpr int EPB = (256 / ((N + 1) * (N + 1) * (N + 1))) > 0 ? (256 / ((N + 1) * (N + 1) * (N + 1))) : 1; };
+
+template <int N, int OUT>
+__global__ void __launch_bounds__(256)
+k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
+          const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
+          const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
+          double *__restrict__ out1, double *__restrict__ out2, int nElems, double kappa,
+          double mu0, double KappasPr)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int EPB = LiftGeom<N>::EPB;
+  __shared__ double s_w[EPB][5][n3];
+  const Ops &op = c_ops[N];
+  const int tid = threadIdx.x;
+  const int el = tid / n3, node = tid - el * n3;
+  const int e = blockIdx.x * EPB + el;
+  const bool active = (el < EPB) && (e < nElems);
+  double u[5], w[5];
+  if (active) {
+    const double *up = U + ((size_t)e * n3 + node) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = up[v];
+    cons_to_entropy(u, kappa, w);
+#pragma unroll
+    for (int v = 0; v < 5; v++) s_w[el][v][node] = w[v];
+  }
+  __syncthreads();
+  if (!active) return;
+  const int i = node % n, j = (node / n) % n, k = node / n2;
+  double gxi[5], geta[5], gzeta[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    gxi[v] = op.D[i][0] * s_w[el][v][0 + n * (j + n * k)];
+    geta[v] = op.D[j][0] * s_w[el][v][i + n * (0 + n * k)];
+    gzeta[v] = op.D[k][0] * s_w[el][v][i + n * (j + n * 0)];
+  }
+#pragma unroll
+  for (int l = 1; l < n; l++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      gxi[v] = gxi[v] + op.D[i][l] * s_w[el][v][l + n * (j + n * k)];
+      geta[v] = geta[v] + op.D[j][l] * s_w[el][v][i + n * (l + n * k)];
+      gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
+    }
+  const size_t c = (size_t)e * n3 + node;
+  const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+  double gx[5], gy[5], gz[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    gx[v] = mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v];
+    gy[v] = mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v];
+    gz[v] = mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v];
+  }
+  // surface terms: +1/2 (w_slave - w_master) SurfElem n_master L_HatMinus0 on both sides
+  int locs[3], nl = 0;
+  if (k == 0) locs[nl++] = ZETA_MINUS;
+  if (j == 0) locs[nl++] = ETA_MINUS;
+  if (i == N) locs[nl++] = XI_PLUS;
+  if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
+  if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
+  if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+  const double L0 = op.L0;
+  for (int t = 0; t < nl; t++) {
+    const int loc = locs[t];
+    const int s = e2s[(e * 6 + loc - 1) * 2] - 1, fl = e2s[(e * 6 + loc - 1) * 2 + 1];
+    int p, q;
+    vol_to_side<N>(loc, fl, i, j, k, p, q);
+    int enb, vnb;
+    if (fl == 0) {
+      enb = s2e[5 * s + 1] - 1;
+      vnb = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
+    } else {
+      enb = s2e[5 * s] - 1;
+      vnb = side_to_vol<N>(s2e[5 * s + 2], 0, p, q);
+    }
+    const double *unbp = U + ((size_t)enb * n3 + vnb) * 5;
+    double unb[5] = {unbp[0], unbp[1], unbp[2], unbp[3], unbp[4]}, wnb[5];
+    cons_to_entropy(unb, kappa, wnb);
+    const double4 g = nvse[(size_t)(s * n + q) * n + p];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const double ws = (fl == 0) ? wnb[v] : w[v];
+      const double wm = (fl == 0) ? w[v] : wnb[v];
+      const double f = 0.5 * (ws - wm) * g.w;
+      gx[v] = gx[v] + (f * g.x) * L0;
+      gy[v] = gy[v] + (f * g.y) * L0;
+      gz[v] = gz[v] + (f * g.z) * L0;
+    }
+  }
+  const double sj = sJ[c];
+#pragma unroll
+  for (int v = 0; v < 5; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+  grad_entropy_to_prim(u, kappa, gx);
+  grad_entropy_to_prim(u, kappa, gy);
+  grad_entropy_to_prim(u, kappa, gz);
+  if (OUT == 0) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) { out0[5 * c + v] = gx[v]; out1[5 * c + v] = gy[v]; out2[5 * c + v] = gz[v]; }
+  } else {
+    double fv[12];
+    diff_flux(u, gx, gy, gz, kappa, mu0, KappasPr, fv);
+#pragma unroll
+    for (int m = 0; m < 12; m++) out0[12 * c + m] = fv[m];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// CALCTIMESTEP (navierstokes/calctimestep.f90:39-170): per-element convective and
+// viscous eigenvalue bounds, global minimum via an order-independent atomic min on the
+// bit pattern of the positive dt (deterministic).
+struct DtState {
+  unsigned long long dt_bits;  // running min of positive dt
+  int errType;                 // max error code seen
+  int errStep;                 // first step with an error (-1 none)
+  double dt;                   // dt of the current step
+  double t;                    // simulation time
+  int step;                    // steps done
+  int pad;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256)
+k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
+          const double *__restrict__ Mh, const double *__restrict__ sJ, DtState *st, int nElems,
+          double kappa, double cfl, double dfl, double mu0, double KsPr_max, int parabolic)
+{
+  constexpr int n = N + 1, n3 = n * n * n;
+  constexpr int EPB = LiftGeom<N>::EPB;
+  __shared__ double s_l[EPB][6][n3];
+  __shared__ int s_err[EPB][n3];
+  const int tid = threadIdx.x;
+  const int el = tid / n3, node = tid - el * n3;
+  const int e = blockIdx.x * EPB + el;
+  const bool active = (el < EPB) && (e < nElems);
+  if (active) {
+    const size_t c = (size_t)e * n3 + node;
+    const double *u = U + 5 * c;
+    int err = 0;
+    if (isnan(u[0])) err = 1;
+    const double srho = 1.0 / u[0];
+    const double v0 = u[1] * srho, v1 = u[2] * srho, v2 = u[3] * srho;
+    const double p = (kappa - 1.0) * (u[4] - 0.5 * u[0] * (v0 * v0 + v1 * v1 + v2 * v2));
+    if (p <= 0.0) err = 2;
+    const double cs = sqrt(kappa * p * srho), sj = sJ[c];
+    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+    s_l[el][0][node] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
+    s_l[el][1][node] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
+    s_l[el][2][node] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
+    if (parabolic) {
+      const double muX = srho * KsPr_max * mu0;
+      s_l[el][3][node] = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
+      s_l[el][4][node] = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
+      s_l[el][5][node] = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
+    }
+    s_err[el][node] = err;
+  }
+  __syncthreads();
+  if (active && node == 0) {
+    double l[6] = {1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12};
+    int err = 0;
+    for (int m = 0; m < n3; m++) {
+      // MAX as in the reference: a NaN lambda leaves the running max unchanged
+#pragma unroll
+      for (int d = 0; d < 3; d++) l[d] = (s_l[el][d][m] > l[d]) ? s_l[el][d][m] : l[d];
+      if (parabolic) {
+#pragma unroll
+        for (int d = 3; d < 6; d++) l[d] = (s_l[el][d][m] > l[d]) ? s_l[el][d][m] : l[d];
+      }
+      err = max(err, s_err[el][m]);
+    }
+    double dte = cfl * 2.0 / (l[0] + l[1] + l[2]);
+    if (isnan(dte)) err = max(err, 3);
+    if (parabolic) {
+      double dve = dfl * 4.0 / (l[3] + l[4] + l[5]);
+      if (isnan(dve)) err = max(err, 4);
+      dte = fmin(dte, dve);
+    }
+    if (!isnan(dte) && dte > 0.0) atomicMin(&st->dt_bits, (unsigned long long)__double_as_longlong(dte));
+    if (err) atomicMax(&st->errType, err);
+  }
+}
+
+__global__ void k_dt_begin(DtState *st)
+{
+  st->dt_bits = 0x7FEFFFFFFFFFFFFFull;  // largest finite double
+}
+
+__global__ void k_dt_end(DtState *st, int advance)
+{
+  st->dt = __longlong_as_double((long long)st->dt_bits);
+  if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
+  if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+}
+
+// CalcEntropy (analyze_equation.f90:549-599): per-block partial sums (fixed order),
+// reduced by a second pass in block order.
+template <int N>
+__global__ void __launch_bounds__(256)
+k_entropy(const double *__restrict__ U, const double *__restrict__ Ut, const double *__restrict__ sJ,
+          double *__restrict__ part, long long nNodes, double kappa)
+{
+  constexpr int n = N + 1, n3 = n * n * n;
+  __shared__ double s0[256], s1[256];
+  const Ops &op = c_ops[N];
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (gid < nNodes) {
+    const int node = (int)(gid % n3);
+    const int i = node % n, j = (node / n) % n, k = node / (n * n);
+    const double *u = U + 5 * gid;
+    double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
+    const double km1 = kappa - 1.0;
+    double srho = 1.0 / uu[0];
+    double v1 = uu[1] * srho, v2 = uu[2] * srho, v3 = uu[3] * srho;
+    double p = km1 * (uu[4] - 0.5 * (uu[1] * v1 + uu[2] * v2 + uu[3] * v3));
+    double wv = op.w[i] * op.w[j] * op.w[k];
+    double el = -uu[0] * (log(p) - kappa * log(uu[0]));
+    a = el * wv / sJ[gid];
+    double w[5];
+    cons_to_entropy(uu, kappa, w);
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < 5; v++) s += w[v] * Ut[5 * gid + v];
+    b = s * wv / sJ[gid];
+  }
+  s0[threadIdx.x] = a; s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) { s0[threadIdx.x] += s0[threadIdx.x + st]; s1[threadIdx.x] += s1[threadIdx.x + st]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+}
+
+__global__ void k_sum_pairs(const double *__restrict__ part, int nPart, double *__restrict__ out)
+{
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nPart; i += 256) { a += part[2 * i]; b += part[2 * i + 1]; }
+  s0[threadIdx.x] = a; s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) { s0[threadIdx.x] += s0[threadIdx.x + st]; s1[threadIdx.x] += s1[threadIdx.x + st]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+}
+
+}  // namespace fx

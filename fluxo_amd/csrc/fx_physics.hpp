@@ -1,0 +1,246 @@
+// Device-side physics of the compressible Euler / Navier-Stokes split-form path.
+//
+// Each function names the reference routine (project-fluxo/fluxo) it computes; the
+// arithmetic follows the reference's formulas so that FP64 results agree with the
+// reference to rounding.  Two deliberate reformulations, both exact in real
+// arithmetic:
+//  * the logarithmic mean takes ln(a), ln(b) precomputed once per node, so
+//    (b-a)/ln(b/a) becomes (b-a)/(ln b - ln a); the near-equal branch keeps the
+//    reference's series and switch (LN_MEAN, navierstokes/flux_average.f90:1945-1965);
+//  * surface fluxes are evaluated in the Cartesian frame instead of rotating the
+//    momentum into (n,t1,t2) and back (riemann.f90:174-246), which is an
+//    orthogonal change of basis.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fx {
+
+constexpr double LNMEAN_EPS = 1.0e-4;
+
+// Log mean of a, b given la = ln a, lb = ln b.
+template <typename R>
+__device__ __forceinline__ R ln_mean(R a, R b, R la, R lb)
+{
+  R d = b - a, s = a + b;
+  if (d * d < R(LNMEAN_EPS) * (s * s)) {
+    R u = (d * d) / (s * s);
+    return s * R(52.5) / (R(105.0) + u * (R(35.0) + u * (R(21.0) + u * R(15.0))));
+  }
+  return d / (lb - la);
+}
+
+// 1 / ln_mean(a, b)
+template <typename R>
+__device__ __forceinline__ R inv_ln_mean(R a, R b, R la, R lb)
+{
+  R d = b - a, s = a + b;
+  if (d * d < R(LNMEAN_EPS) * (s * s)) {
+    R u = (d * d) / (s * s);
+    return (R(105.0) + u * (R(35.0) + u * (R(21.0) + u * R(15.0)))) / (s * R(52.5));
+  }
+  return (lb - la) / d;
+}
+
+// Per-node quantities staged for the two-point fluxes.
+//   rho, v1, v2, v3, p, ln(rho), X, ln(X), |v|^2, rhoE
+// X = rho/p for Ranocha (RanochaFluxVec), X = beta = rho/(2p) for Chandrashekar
+// (EntropyAndEnergyConservingFluxVec); EvalUaux (flux_average.f90:305-332).
+enum { NF_RHO = 0, NF_V1, NF_V2, NF_V3, NF_P, NF_LNRHO, NF_X, NF_LNX, NF_VV, NF_E, NF_COUNT };
+
+template <int VF>
+__device__ __forceinline__ void node_fields(const double *u, double kappaM1, double *f)
+{
+  double srho = 1.0 / u[0];
+  double v1 = u[1] * srho, v2 = u[2] * srho, v3 = u[3] * srho;
+  double vv = v1 * v1 + v2 * v2 + v3 * v3;
+  double p = kappaM1 * (u[4] - 0.5 * u[0] * vv);
+  f[NF_RHO] = u[0]; f[NF_V1] = v1; f[NF_V2] = v2; f[NF_V3] = v3; f[NF_P] = p; f[NF_VV] = vv;
+  f[NF_E] = u[4];
+  if (VF == 32) {
+    double X = u[0] / p;
+    f[NF_LNRHO] = log(u[0]); f[NF_X] = X; f[NF_LNX] = log(X);
+  } else if (VF == 5) {
+    double X = 0.5 * u[0] / p;
+    f[NF_LNRHO] = log(u[0]); f[NF_X] = X; f[NF_LNX] = log(X);
+  } else {
+    f[NF_LNRHO] = 0.0; f[NF_X] = 0.0; f[NF_LNX] = 0.0;
+  }
+}
+
+// Metric-dealiased two-point volume flux Fstar(U_L, U_R, {{m}}).
+//  VF=32: RanochaFluxVec (flux_average.f90:1876-1929)
+//  VF=5 : EntropyAndEnergyConservingFluxVec (flux_average.f90:1277-1356)
+//  VF=0 : StandardDGFluxVec (flux_average.f90:394-439)
+template <int VF, typename R>
+__device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL, const R *mR,
+                                            R skappaM1, R *F)
+{
+  if (VF == 32) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R rhoLN = ln_mean(L[NF_RHO], Rt[NF_RHO], L[NF_LNRHO], Rt[NF_LNRHO]);
+    R va0 = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
+    R va1 = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
+    R va2 = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
+    R v2zip = R(0.5) * (L[NF_V1] * Rt[NF_V1] + L[NF_V2] * Rt[NF_V2] + L[NF_V3] * Rt[NF_V3]);
+    R pavg = R(0.5) * (L[NF_P] + Rt[NF_P]);
+    R vmL = L[NF_V1] * m0 + L[NF_V2] * m1 + L[NF_V3] * m2;
+    R vmR = Rt[NF_V1] * m0 + Rt[NF_V2] * m1 + Rt[NF_V3] * m2;
+    R srspM = inv_ln_mean(L[NF_X], Rt[NF_X], L[NF_LNX], Rt[NF_LNX]);
+    F[0] = rhoLN * R(0.5) * (vmL + vmR);
+    F[1] = F[0] * va0 + m0 * pavg;
+    F[2] = F[0] * va1 + m1 * pavg;
+    F[3] = F[0] * va2 + m2 * pavg;
+    F[4] = F[0] * (v2zip + srspM * skappaM1) + R(0.5) * (L[NF_P] * vmR + Rt[NF_P] * vmL);
+  } else if (VF == 5) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R rho_mean = R(0.5) * (L[NF_RHO] + Rt[NF_RHO]);
+    R rhoHat = ln_mean(L[NF_RHO], Rt[NF_RHO], L[NF_LNRHO], Rt[NF_LNRHO]);
+    R beta_mean = R(0.5) * (L[NF_X] + Rt[NF_X]);
+    R sbeta_hat = inv_ln_mean(L[NF_X], Rt[NF_X], L[NF_LNX], Rt[NF_LNX]);
+    R uH = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
+    R vH = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
+    R wH = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
+    R p1 = R(0.5) * rho_mean / beta_mean;
+    R qH = uH * m0 + vH * m1 + wH * m2;
+    F[0] = rhoHat * qH;
+    F[1] = F[0] * uH + m0 * p1;
+    F[2] = F[0] * vH + m1 * p1;
+    F[3] = F[0] * wH + m2 * p1;
+    F[4] = F[0] * R(0.5) * (skappaM1 * sbeta_hat - R(0.5) * (L[NF_VV] + Rt[NF_VV])) + uH * F[1] +
+           vH * F[2] + wH * F[3];
+  } else {
+    R qL = L[NF_V1] * mL[0] + L[NF_V2] * mL[1] + L[NF_V3] * mL[2];
+    R qR = Rt[NF_V1] * mR[0] + Rt[NF_V2] * mR[1] + Rt[NF_V3] * mR[2];
+    F[0] = R(0.5) * (L[NF_RHO] * qL + Rt[NF_RHO] * qR);
+    F[1] = R(0.5) * (L[NF_RHO] * L[NF_V1] * qL + Rt[NF_RHO] * Rt[NF_V1] * qR + mL[0] * L[NF_P] + mR[0] * Rt[NF_P]);
+    F[2] = R(0.5) * (L[NF_RHO] * L[NF_V2] * qL + Rt[NF_RHO] * Rt[NF_V2] * qR + mL[1] * L[NF_P] + mR[1] * Rt[NF_P]);
+    F[3] = R(0.5) * (L[NF_RHO] * L[NF_V3] * qL + Rt[NF_RHO] * Rt[NF_V3] * qR + mL[2] * L[NF_P] + mR[2] * Rt[NF_P]);
+    F[4] = R(0.5) * ((L[NF_E] + L[NF_P]) * qL + (Rt[NF_E] + Rt[NF_P]) * qR);
+  }
+}
+
+// Numerical surface flux in the normal direction n (unit), unscaled by SurfElem.
+//  RS=1 : RiemannSolverByRusanov (riemann.f90:279-313) with EvalEulerFlux1D (flux.f90:298-329)
+//  RS=0 : RiemannSolverCentral (riemann.f90:251-272)
+//  RS=32: RiemannSolver_VolumeFluxAverage + RanochaFlux (riemann.f90:980-1005,
+//         flux_average.f90:1806-1866)
+//  RS=9 : RiemannSolver_VolumeFluxAverage + EntropyAndEnergyConservingFlux
+//         (flux_average.f90:1214-1270)
+template <int RS>
+__device__ __forceinline__ void riemann(const double *UL, const double *UR, const double *nv,
+                                        double kappa, double *F)
+{
+  const double km1 = kappa - 1.0;
+  double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  double vL[3] = {UL[1] * sL, UL[2] * sL, UL[3] * sL};
+  double vR[3] = {UR[1] * sR, UR[2] * sR, UR[3] * sR};
+  if (RS == 0 || RS == 1) {
+    double mnL = UL[1] * nv[0] + UL[2] * nv[1] + UL[3] * nv[2];   // rotated rho*u_n
+    double mnR = UR[1] * nv[0] + UR[2] * nv[1] + UR[3] * nv[2];
+    double unL = mnL * sL, unR = mnR * sR;
+    double pL = km1 * (UL[4] - 0.5 * (UL[1] * vL[0] + UL[2] * vL[1] + UL[3] * vL[2]));
+    double pR = km1 * (UR[4] - 0.5 * (UR[1] * vR[0] + UR[2] * vR[1] + UR[3] * vR[2]));
+    double FL[5], FR[5];
+    FL[0] = mnL; FR[0] = mnR;
+    FL[1] = UL[1] * unL + pL * nv[0]; FR[1] = UR[1] * unR + pR * nv[0];
+    FL[2] = UL[2] * unL + pL * nv[1]; FR[2] = UR[2] * unR + pR * nv[1];
+    FL[3] = UL[3] * unL + pL * nv[2]; FR[3] = UR[3] * unR + pR * nv[2];
+    FL[4] = (UL[4] + pL) * unL;       FR[4] = (UR[4] + pR) * unR;
+    if (RS == 0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = 0.5 * (FL[v] + FR[v]);
+    } else {
+      double mm2L = mnL * mnL, mm2R = mnR * mnR;
+      (void)mm2L; (void)mm2R;
+      double c2L = kappa * km1 * (UL[4] * UL[0] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3])) / (UL[0] * UL[0]);
+      double c2R = kappa * km1 * (UR[4] * UR[0] - 0.5 * (UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3])) / (UR[0] * UR[0]);
+      double lam = fmax(fabs(mnL / UL[0]), fabs(mnR / UR[0])) + sqrt(fmax(c2L, c2R));
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
+    }
+  } else {
+    double vv_L = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
+    double vv_R = vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2];
+    double pL = km1 * (UL[4] - 0.5 * UL[0] * vv_L);
+    double pR = km1 * (UR[4] - 0.5 * UR[0] * vv_R);
+    double fL[NF_COUNT], fR[NF_COUNT];
+    fL[NF_RHO] = UL[0]; fL[NF_V1] = vL[0]; fL[NF_V2] = vL[1]; fL[NF_V3] = vL[2]; fL[NF_P] = pL;
+    fR[NF_RHO] = UR[0]; fR[NF_V1] = vR[0]; fR[NF_V2] = vR[1]; fR[NF_V3] = vR[2]; fR[NF_P] = pR;
+    fL[NF_VV] = vv_L; fR[NF_VV] = vv_R; fL[NF_E] = UL[4]; fR[NF_E] = UR[4];
+    fL[NF_LNRHO] = log(UL[0]); fR[NF_LNRHO] = log(UR[0]);
+    if (RS == 32) { fL[NF_X] = UL[0] / pL; fR[NF_X] = UR[0] / pR; }
+    else { fL[NF_X] = 0.5 * UL[0] / pL; fR[NF_X] = 0.5 * UR[0] / pR; }
+    fL[NF_LNX] = log(fL[NF_X]); fR[NF_LNX] = log(fR[NF_X]);
+    volume_flux<(RS == 32 ? 32 : 5), double>(fL, fR, nv, nv, 1.0 / km1, F);
+  }
+}
+
+// ConsToEntropy (navierstokes/equation_vars.f90:362-391); s = -ln(rho_sp*rho^(kappa-1))
+// is evaluated as -(ln rho_sp + (kappa-1) ln rho).
+__device__ __forceinline__ void cons_to_entropy(const double *U, double kappa, double *w)
+{
+  double km1 = kappa - 1.0;
+  double srho = 1.0 / U[0];
+  double u = U[1] * srho, v = U[2] * srho, ww = U[3] * srho;
+  double v2s2 = 0.5 * (u * u + v * v + ww * ww);
+  double rho_sp = U[0] / (km1 * (U[4] - U[0] * v2s2));
+  double s = -(log(rho_sp) + km1 * log(U[0]));
+  w[0] = (kappa - s) / km1 - rho_sp * v2s2;
+  w[1] = rho_sp * u;
+  w[2] = rho_sp * v;
+  w[3] = rho_sp * ww;
+  w[4] = -rho_sp;
+}
+
+// ConvertToGradPrimVec for entropy-variable lifting (equation_vars.f90:571-647).
+__device__ __forceinline__ void grad_entropy_to_prim(const double *U, double kappa, double *g)
+{
+  double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  double srho = 1.0 / U[0];
+  double u = U[1] * srho, v = U[2] * srho, w = U[3] * srho;
+  double ekin = 0.5 * (U[1] * u + U[2] * v + U[3] * w);
+  double p = km1 * (U[4] - ekin);
+  double rho_sp = U[0] / p;
+  double p_srho = p * srho;
+  double g1 = g[0];
+  double gu = p_srho * (g[1] + u * g[4]);
+  double gv = p_srho * (g[2] + v * g[4]);
+  double gw = p_srho * (g[3] + w * g[4]);
+  double g5 = g[4];
+  g[0] = U[0] * g1 - g5 * (ekin - p * skm1) + rho_sp * (U[1] * gu + U[2] * gv + U[3] * gw);
+  g[1] = gu; g[2] = gv; g[3] = gw;
+  g[4] = p_srho * (g[0] + p * g5);
+}
+
+// EvalDiffFlux3D with constant viscosity (flux.f90:407-517): Cartesian viscous fluxes
+// f, g, h; component 0 (mass) is identically zero and omitted: out[d*4 + (v-1)].
+__device__ __forceinline__ void diff_flux(const double *U, const double *gx, const double *gy,
+                                          const double *gz, double kappa, double mu0,
+                                          double KappasPr, double *out)
+{
+  const double km1 = kappa - 1.0, skm1 = 1.0 / km1, s23 = 2.0 / 3.0;
+  double srho = 1.0 / U[0];
+  double v1 = U[1] * srho, v2 = U[2] * srho, v3 = U[3] * srho;
+  double p = km1 * (U[4] - 0.5 * (U[1] * v1 + U[2] * v2 + U[3] * v3));
+  double muS = mu0, lambda = muS * KappasPr;
+  double divv = gx[1] + gy[2] + gz[3];
+  double cTx = skm1 * srho * (gx[4] - srho * p * gx[0]);
+  double cTy = skm1 * srho * (gy[4] - srho * p * gy[0]);
+  double cTz = skm1 * srho * (gz[4] - srho * p * gz[0]);
+  double f1 = -muS * (2 * gx[1] - s23 * divv);
+  double f2 = -muS * (gx[2] + gy[1]);
+  double f3 = -muS * (gx[3] + gz[1]);
+  double f4 = v1 * f1 + v2 * f2 + v3 * f3 - lambda * cTx;
+  double g1 = f2;
+  double g2 = -muS * (2 * gy[2] - s23 * divv);
+  double g3 = -muS * (gy[3] + gz[2]);
+  double g4 = v1 * g1 + v2 * g2 + v3 * g3 - lambda * cTy;
+  double h1 = f3, h2 = g3;
+  double h3 = -muS * (2 * gz[3] - s23 * divv);
+  double h4 = v1 * h1 + v2 * h2 + v3 * h3 - lambda * cTz;
+  out[0] = f1; out[1] = f2; out[2] = f3; out[3] = f4;
+  out[4] = g1; out[5] = g2; out[6] = g3; out[7] = g4;
+  out[8] = h1; out[9] = h2; out[10] = h3; out[11] = h4;
+}
+
+}  // namespace fx

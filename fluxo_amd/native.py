@@ -1,0 +1,186 @@
+"""ctypes binding of the C ABI in include/fluxo_gpu.h (libfluxo_gpu.so).
+
+This is the Python-side host of the engine: it mirrors the reference's
+DGTimeDerivative / TimeStepByLSERKW2 / CALCTIMESTEP / CalcEntropy entry points with the
+same argument meaning.  Arrays use numpy C order with the Fortran index order reversed,
+so their memory equals the reference's column-major arrays:
+  U(nVar,0:N,0:N,0:N,nElems)  <->  numpy (nElems, N+1, N+1, N+1, nVar)
+
+There is no CPU fallback: every call runs the HIP kernels, and loading fails loudly if
+the native library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfluxo_gpu.so")
+_LIB = None
+
+FX_PREC_FP64 = 0
+FX_PREC_FP32_SPLIT = 1
+
+
+class FxError(RuntimeError):
+    pass
+
+
+class fx_config(C.Structure):
+    _fields_ = [("N", C.c_int), ("nVar", C.c_int), ("nElems", C.c_int), ("nSides", C.c_int),
+                ("volumeFlux", C.c_int), ("riemann", C.c_int), ("parabolic", C.c_int),
+                ("lifting", C.c_int), ("precision", C.c_int), ("reserved", C.c_int),
+                ("kappa", C.c_double), ("mu0", C.c_double), ("Pr", C.c_double),
+                ("reserved2", C.c_double)]
+
+
+class fx_mesh(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ElemToSide", "SideToElem", "Metrics_fTilde",
+                                          "Metrics_gTilde", "Metrics_hTilde", "sJ", "NormVec",
+                                          "TangVec1", "TangVec2", "SurfElem")]
+
+
+EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
+           "fx_state_upload", "fx_state_download", "fx_ut_download", "fx_rhs", "fx_volint",
+           "fx_lifting", "fx_calc_dt", "fx_rk_step", "fx_run", "fx_entropy", "fx_synchronize",
+           "fx_ctx_bytes"]
+
+
+def lib():
+    """Load libfluxo_gpu.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise FxError(f"native engine not built: {LIB_PATH} missing (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.fx_ctx_create.argtypes = [C.POINTER(fx_config), C.POINTER(vp)]
+    L.fx_ctx_destroy.argtypes = [vp]
+    L.fx_ctx_destroy.restype = None
+    L.fx_last_error.restype = C.c_char_p
+    L.fx_mesh_upload.argtypes = [vp, C.POINTER(fx_mesh)]
+    for f in ("fx_state_upload", "fx_state_download", "fx_ut_download", "fx_volint"):
+        getattr(L, f).argtypes = [vp, vp]
+    L.fx_rhs.argtypes = [vp, C.c_double, vp]
+    L.fx_lifting.argtypes = [vp, vp, vp, vp]
+    L.fx_calc_dt.argtypes = [vp, C.c_double, C.c_double, dp, ip]
+    L.fx_rk_step.argtypes = [vp, C.c_double, C.c_double]
+    L.fx_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip]
+    L.fx_entropy.argtypes = [vp, dp, dp]
+    L.fx_synchronize.argtypes = [vp]
+    L.fx_ctx_bytes.argtypes = [C.POINTER(fx_config)]
+    L.fx_ctx_bytes.restype = C.c_longlong
+    _LIB = L
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise FxError(f"fluxo_gpu error {rc}: {lib().fx_last_error().decode()}")
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, kappa=1.4,
+                mu0=0.0, Pr=0.72, precision=FX_PREC_FP64):
+    c = fx_config()
+    c.N, c.nVar, c.nElems, c.nSides = N, 5, nElems, nSides
+    c.volumeFlux, c.riemann, c.parabolic, c.lifting = volume_flux, riemann, int(bool(parabolic)), 1
+    c.precision = precision
+    c.kappa, c.mu0, c.Pr = kappa, mu0, Pr
+    return c
+
+
+def ctx_bytes(cfg):
+    return lib().fx_ctx_bytes(C.byref(cfg))
+
+
+class Engine:
+    """One device context: mesh, state and the RHS/RK entry points."""
+
+    def __init__(self, mesh, volume_flux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0,
+                 Pr=0.72, precision=FX_PREC_FP64):
+        L = lib()
+        self.N = int(mesh["N"])
+        self.nElems = int(mesh["nElems"])
+        self.nSides = int(mesh["nSides"])
+        self.cfg = make_config(self.N, self.nElems, self.nSides, volume_flux, riemann, parabolic,
+                               kappa, mu0, Pr, precision)
+        self._ctx = C.c_void_p()
+        _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
+        m = fx_mesh()
+        keep = {}
+        for field, key in (("ElemToSide", "ElemToSide"), ("SideToElem", "SideToElem"),
+                           ("Metrics_fTilde", "Mf"), ("Metrics_gTilde", "Mg"),
+                           ("Metrics_hTilde", "Mh"), ("sJ", "sJ"), ("NormVec", "NormVec"),
+                           ("TangVec1", "TangVec1"), ("TangVec2", "TangVec2"),
+                           ("SurfElem", "SurfElem")):
+            dt = np.int32 if key in ("ElemToSide", "SideToElem") else np.float64
+            a = np.ascontiguousarray(mesh[key], dtype=dt)
+            keep[key] = a
+            setattr(m, field, _ptr(a))
+        _chk(L.fx_mesh_upload(self._ctx, C.byref(m)))
+
+    def close(self):
+        if self._ctx:
+            lib().fx_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self):
+        n = self.N + 1
+        return (self.nElems, n, n, n, 5)
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == self.shape()
+        _chk(lib().fx_state_upload(self._ctx, _ptr(U)))
+
+    def download(self):
+        U = np.empty(self.shape())
+        _chk(lib().fx_state_download(self._ctx, _ptr(U)))
+        return U
+
+    def rhs(self, t=0.0, copy=True):
+        Ut = np.empty(self.shape()) if copy else None
+        _chk(lib().fx_rhs(self._ctx, t, _ptr(Ut) if copy else None))
+        return Ut
+
+    def volint(self):
+        Ut = np.empty(self.shape())
+        _chk(lib().fx_volint(self._ctx, _ptr(Ut)))
+        return Ut
+
+    def lifting(self):
+        g = [np.empty(self.shape()) for _ in range(3)]
+        _chk(lib().fx_lifting(self._ctx, *(_ptr(a) for a in g)))
+        return g
+
+    def calc_dt(self, cfl, dfl=0.0):
+        dt, err = C.c_double(), C.c_int()
+        _chk(lib().fx_calc_dt(self._ctx, cfl, dfl, C.byref(dt), C.byref(err)))
+        return dt.value, err.value
+
+    def rk_step(self, t, dt):
+        _chk(lib().fx_rk_step(self._ctx, t, dt))
+
+    def run(self, nsteps, cfl, dfl=0.0, t0=0.0):
+        t, dt, err = C.c_double(t0), C.c_double(), C.c_int()
+        _chk(lib().fx_run(self._ctx, nsteps, cfl, dfl, C.byref(t), C.byref(dt), C.byref(err)))
+        return t.value, dt.value, err.value
+
+    def entropy(self):
+        s, d = C.c_double(), C.c_double()
+        _chk(lib().fx_entropy(self._ctx, C.byref(s), C.byref(d)))
+        return s.value, d.value
+
+    def sync(self):
+        _chk(lib().fx_synchronize(self._ctx))

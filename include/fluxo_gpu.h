@@ -1,0 +1,125 @@
+/*
+ * fluxo_gpu.h -- C ABI of the MI355X split-form DGSEM right-hand-side engine.
+ *
+ * The engine replaces the FP64 CPU path of the reference's spatial operator and
+ * low-storage Runge-Kutta stage.  Each entry point names the reference routine
+ * (project-fluxo/fluxo, file:line) whose work it takes over.  The Fortran host
+ * binds these symbols with ISO_C_BINDING (fluxo_amd/fortran/fluxo_gpu_bind.f90).
+ *
+ * Conventions
+ *  - Return 0 on success or a negative FX_E* code; fx_last_error() gives text.
+ *    The library never aborts the process; the Fortran wrapper maps a non-zero
+ *    code to Abort() like the reference's CollectiveStop/Abort (globals.f90:161-201).
+ *  - Host arrays use the reference's layout (Fortran column-major, first index
+ *    fastest) and are borrowed only for the duration of the call.
+ *  - Device memory belongs to the context.  A context is used by one host thread
+ *    at a time; work is ordered on the context's HIP stream.
+ *  - Element and side ids inside ElemToSide/SideToElem are 1-based, as in the
+ *    reference (mesh_vars.f90; S2E_ and E2S_ positions in defines.h:75-84).
+ */
+#ifndef FLUXO_GPU_H
+#define FLUXO_GPU_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FX_OK = 0,
+  FX_EINVAL = -1,    /* invalid argument or unsupported option */
+  FX_EHIP = -2,      /* HIP runtime error (message in fx_last_error) */
+  FX_ENOMEM = -3,    /* device allocation failed */
+  FX_ESTATE = -4,    /* call out of order (e.g. rhs before mesh upload) */
+  FX_ENODEV = -5     /* no usable gfx950 device */
+};
+
+enum { FX_PREC_FP64 = 0, FX_PREC_FP32_SPLIT = 1 };
+
+/* Options that the reference reads from parameter.ini / CMake defines. */
+typedef struct {
+  int N;            /* polynomial degree, 'N' (interpolation.f90:125-126); 1..7 */
+  int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) */
+  int nElems;       /* local elements */
+  int nSides;       /* sides (all inner sides on periodic conforming meshes) */
+  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32 */
+  int riemann;      /* 'Riemann' (navierstokes/equation.f90:103-133): 0, 1, 9, 32 */
+  int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler, 1 Navier-Stokes */
+  int lifting;      /* PP_Lifting: 1 = BR1 with entropy-variable lifting */
+  int precision;    /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
+  int reserved;
+  double kappa;     /* 'kappa' (equation.f90:218) */
+  double mu0;       /* 'mu0' constant viscosity (equation.f90:229-230) */
+  double Pr;        /* 'Pr' (equation.f90:226) */
+  double reserved2;
+} fx_config;
+
+/* Mesh data produced by the host's InitMesh/CalcMetrics (mesh.f90:100-300,
+   metrics.f90:78-483), all FP64 except the connectivity. */
+typedef struct {
+  const int *ElemToSide;          /* (2,6,nElems)  [SideID, flip] per local side */
+  const int *SideToElem;          /* (5,nSides)    [Elem, nbElem, locSide, nbLocSide, flip] */
+  const double *Metrics_fTilde;   /* (3,0:N,0:N,0:N,nElems) */
+  const double *Metrics_gTilde;   /* (3,0:N,0:N,0:N,nElems) */
+  const double *Metrics_hTilde;   /* (3,0:N,0:N,0:N,nElems) */
+  const double *sJ;               /* (0:N,0:N,0:N,nElems) */
+  const double *NormVec;          /* (3,0:N,0:N,nSides) */
+  const double *TangVec1;         /* (3,0:N,0:N,nSides) (accepted for interface parity) */
+  const double *TangVec2;         /* (3,0:N,0:N,nSides) (accepted for interface parity) */
+  const double *SurfElem;         /* (0:N,0:N,nSides) */
+} fx_mesh;
+
+typedef struct fx_ctx fx_ctx;
+
+/* InitDG/InitEquation/InitLifting equivalent (dg.f90:61-175, equation.f90:162-470). */
+int fx_ctx_create(const fx_config *cfg, fx_ctx **out);
+void fx_ctx_destroy(fx_ctx *ctx);
+const char *fx_last_error(void);
+
+/* Mesh arrays the reference keeps in MOD_Mesh_Vars (mesh_vars.f90). */
+int fx_mesh_upload(fx_ctx *ctx, const fx_mesh *mesh);
+
+/* U(nVar,0:N,0:N,0:N,nElems) of MOD_DG_Vars (dg_vars.f90:75-80). */
+int fx_state_upload(fx_ctx *ctx, const double *U);
+int fx_state_download(fx_ctx *ctx, double *U);
+int fx_ut_download(fx_ctx *ctx, double *Ut);
+
+/* DGTimeDerivative_weakForm(tIn) (dg.f90:282-452): Ut = RHS(U).  Ut is kept on the
+   device; if Ut_host is not NULL it is also copied back. */
+int fx_rhs(fx_ctx *ctx, double t, double *Ut_host);
+
+/* VolInt_SplitForm (volint.f90:173-218) contribution alone, before Jacobian
+   scaling: Ut_host = sum_l Dvolsurf_T * {f~,g~,h~}. */
+int fx_volint(fx_ctx *ctx, double *Ut_host);
+
+/* Lifting (br1.f90:154-364): primitive-variable gradients gradPx/y/z
+   (nVar,0:N,0:N,0:N,nElems). */
+int fx_lifting(fx_ctx *ctx, double *gradPx, double *gradPy, double *gradPz);
+
+/* CALCTIMESTEP (navierstokes/calctimestep.f90:39-170) with the CFL/DFL scaling of
+   FillCFL_DFL (timedisc.f90:697-741): CFLscale/DFLscale are the parameter.ini
+   values.  errType as in the reference: 1 NaN density, 2 p<=0, 3/4 NaN dt. */
+int fx_calc_dt(fx_ctx *ctx, double CFLscale, double DFLscale, double *dt, int *errType);
+
+/* One TimeStepByLSERKW2 step (timedisc.f90:424-480), Carpenter RK4-5, fixed dt. */
+int fx_rk_step(fx_ctx *ctx, double t, double dt);
+
+/* The TimeDisc loop body (timedisc.f90:274-413) for nSteps steps with dt from
+   CALCTIMESTEP before every step, entirely on the device (one HIP graph per step).
+   On return *t and *dt hold the final time and last time step, *errType the first
+   CALCTIMESTEP error (0 if none). */
+int fx_run(fx_ctx *ctx, int nSteps, double CFLscale, double DFLscale, double *t, double *dt,
+           int *errType);
+
+/* CalcEntropy (navierstokes/analyze_equation.f90:549-599) on the device state and
+   the last Ut: S = int -rho s/(kappa-1), dSdU_Ut = int dS/dU . Ut. */
+int fx_entropy(fx_ctx *ctx, double *S, double *dSdU_Ut);
+
+/* Wait for all work queued on the context's stream. */
+int fx_synchronize(fx_ctx *ctx);
+
+/* Device bytes held by a context with this configuration (for HBM admission). */
+long long fx_ctx_bytes(const fx_config *cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

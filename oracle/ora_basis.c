@@ -275,7 +275,7 @@ void ora_dg_operators(int N, double *xGP, double *wGP, double *D, double *D_hat,
   ora_derivative_matrix(N, xGP, D);
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++)
-      D_hat[i + n * j] = -(1.0 / wGP[i]) * D[j + n * i] * wGP[j];
+      D_hat[i + n * j] = -((1.0 / wGP[i]) * (D[j + n * i] * wGP[j]));
   ora_lagrange_polys(1.0, N, xGP, wb, Lp);
   ora_lagrange_polys(-1.0, N, xGP, wb, Lm);
   for (int i = 0; i < n; i++) { L_hatplus[i] = Lp[i] / wGP[i]; L_hatminus[i] = Lm[i] / wGP[i]; }

@@ -1,0 +1,31 @@
+"""The C-ABI library loads and exports every symbol include/fluxo_gpu.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from fluxo_amd import native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "fluxo_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fx_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_lists_the_python_binding_exports():
+    assert set(declared_symbols()) == set(native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(native.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_ctx_bytes_is_host_only_and_scales():
+    small = native.make_config(3, 8, 24)
+    big = native.make_config(3, 64, 192)
+    a, b = native.ctx_bytes(small), native.ctx_bytes(big)
+    assert 0 < a < b and abs(b / a - 8.0) < 0.2
