@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X split-form DGSEM engine on one GPU.
+
+Workload (default): compressible Navier-Stokes Taylor-Green vortex, FP64, Re=1600, Ma=0.1,
+Pr=0.72, on [-pi,pi]^3 periodic with 32^3 hexahedra whose Ngeo=3 geometry is warped by
+0.1*prod sin(x_k+pi); N=3 Legendre-Gauss-Lobatto (2,097,152 nodes x 5 variables); BR1
+lifting with entropy variables; Ranocha EC volume flux (VolumeFlux=32) + LLF surface flux
+(Riemann=1); Carpenter RK4-5 with CALCTIMESTEP (CFL 0.9, DFL 0.9) before every step.
+
+A step is one full RK time step (5 right-hand-side evaluations and stage updates); the
+metric counts DOF-updates: one node (all 5 variables) advanced through one RK stage.
+The mesh and initial state are synthetic (generated here); U is resident in HBM before
+the timed region and stays there.
+
+Prints one JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-updates/s on 1 GPU (FP64 vs FP32-split) + p50/p99 serving latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def tgv_state(xgp, kappa=1.4):
+    """Taylor-Green vortex, Ma=0.1 (the reference's ExactFunc 7, equation.f90:802-811)."""
+    x, y, z = xgp[..., 0], xgp[..., 1], xgp[..., 2]
+    A, Ms = 1.0, 0.1
+    rho = np.ones_like(x)
+    u = A * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -A * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    p = (A / Ms * A / Ms / kappa * rho)
+    p = p + 1.0 / 16.0 * A * A * rho * (np.cos(2 * x) * np.cos(2.0 * z) + 2.0 * np.cos(2.0 * y)
+                                        + 2.0 * np.cos(2.0 * x) + np.cos(2 * y) * np.cos(2.0 * z))
+    U = np.empty(x.shape + (5,))
+    U[..., 0] = rho
+    U[..., 1], U[..., 2], U[..., 3] = rho * u, rho * v, rho * w
+    U[..., 4] = p / (kappa - 1.0) + 0.5 * (U[..., 1] * u + U[..., 2] * v + U[..., 3] * w)
+    return U
+
+
+def volume_kernel_bytes(N, nElems, parabolic):
+    """Algorithmic HBM bytes of one launch of the volume kernel in its RK-stage form:
+    U read + write, Ut_tmp read + write, metrics (9), sJ (1), the face fluxes of the
+    element's 3 owned sides (each side's flux is read once per side), and for
+    Navier-Stokes the 12 Cartesian viscous flux components per node."""
+    n = N + 1
+    n3, n2 = n ** 3, n ** 2
+    per_elem = 8 * (5 * n3 * 4 + 9 * n3 + n3 + 3 * n2 * 5 + (12 * n3 if parabolic else 0))
+    return per_elem * nElems
+
+
+def cpu_baseline(steps):
+    """The CPU oracle (plain-C restatement of the reference algorithm, one core) on a
+    bounded sample of the same workload: 8^3 warped TGV elements, N=3, NS + BR1."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    K = 8
+    m = O.box_mesh(K, 3, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3, Ngeo=3, warp=True)
+    U = tgv_state(m["Elem_xGP"])
+    P = O.Problem(m, volflux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72)
+    Ut = np.zeros_like(U)
+    t0 = time.perf_counter()
+    t = 0.0
+    for _ in range(steps):
+        dt, _ = P.calc_dt(U, 0.9, 0.9)
+        P.lserk_step(U, Ut, t, dt)
+        t += dt
+    el = time.perf_counter() - t0
+    dofs = K ** 3 * 64 * 5 * steps
+    return {"value": dofs / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle, NS TGV N=3 BR1 on 8^3 warped elements, {steps} RK steps "
+                      f"({el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--elems", type=int, default=32, help="elements per direction")
+    ap.add_argument("--cpu-steps", type=int, default=80)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU element partitioning is a design "
+              "only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import Engine, FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT
+
+    K, N = args.elems, 3
+    mesh = box_mesh(K, N, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3, Ngeo=3, warp=True)
+    U0 = tgv_state(mesh["Elem_xGP"])
+    eng = Engine(mesh, volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72)
+    eng.upload(U0)
+    nodes = mesh["nElems"] * (N + 1) ** 3
+
+    if args.warmup > 0:
+        eng.run(args.warmup, 0.9, 0.9)
+    eng.sync()
+    t0 = time.perf_counter()
+    t_end, dt_last, err = eng.run(args.steps, 0.9, 0.9)
+    eng.sync()
+    el = time.perf_counter() - t0
+    if err != 0:
+        print(f"bench.py: CALCTIMESTEP reported errType={err}", file=sys.stderr)
+        return 3
+    value = nodes * 5 * args.steps / el
+
+    # dominant kernel: the volume kernel in its RK-stage form, timed with HIP events
+    reps = 20
+    ms = {"volume": eng.profile_kernel(FX_KERNEL_VOLUME_RK, reps),
+          "surface": eng.profile_kernel(FX_KERNEL_SURFACE, reps),
+          "lifting": eng.profile_kernel(FX_KERNEL_LIFTING, reps),
+          "calc_dt": eng.profile_kernel(FX_KERNEL_DT, reps)}
+    vbytes = volume_kernel_bytes(N, mesh["nElems"], True)
+    achieved = vbytes / (ms["volume"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_volume<N=3,Ranocha,BR1,RK>", "launch_ms": round(ms["volume"], 4),
+            "bytes_per_launch": vbytes}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps)
+    out = {
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Navier-Stokes TGV Re=1600 Ma=0.1, [-pi,pi]^3 periodic, {K}^3 warped "
+                               f"hexes (Ngeo=3), N=3 LGL, BR1 entropy-variable lifting, Ranocha EC "
+                               f"volume flux + LLF, Carpenter RK4-5 CFL/DFL 0.9",
+                   "nElems": mesh["nElems"], "nodes": nodes, "N": N, "precision": "FP64",
+                   "t_end": t_end, "dt_last": dt_last},
+        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "pid_s_per_dof_timestep": 5.0 / value,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

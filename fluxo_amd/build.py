@@ -27,6 +27,34 @@ def build_oracle(verbose=False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
 
 
+
+
+FORTRAN_SRC = ["fxh_basis.f90", "fxh_mesh.f90", "fluxo_gpu_bind.f90", "fxh_readini.f90", "fxh_dg.f90"]
+FLANG = os.environ.get("FLANG", "flang")
+F_FLAGS = ["-O2", "-fPIC"]
+
+
+def build_host(verbose=False):
+    """Fortran host: libfluxo_host.so (mesh/metrics + FLUXO-shaped wrappers) and the
+    parameter.ini driver fluxo_gpu, both linked against libfluxo_gpu.so."""
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for f in FORTRAN_SRC + ["fluxo_gpu_main.f90"]:
+        o = os.path.join(bdir, f.replace(".f90", ".o"))
+        _run([FLANG, "-c", *F_FLAGS, "-J", bdir, "-I", bdir, os.path.join(fdir, f), "-o", o], verbose)
+        objs.append(o)
+    rpath = "-Wl,-rpath,$ORIGIN"
+    so = os.path.join(HERE, "libfluxo_host.so")
+    _run([FLANG, "-shared", "-o", so + ".tmp", *objs[:-1], "-L" + HERE, "-lfluxo_gpu", rpath], verbose)
+    os.replace(so + ".tmp", so)
+    exe = os.path.join(HERE, "fluxo_gpu")
+    _run([FLANG, "-o", exe + ".tmp", *objs, "-L" + HERE, "-lfluxo_gpu", rpath], verbose)
+    os.replace(exe + ".tmp", exe)
+
+
 def build_all(verbose=False):
     build_engine(verbose)
+    build_host(verbose)
     build_oracle(verbose)

@@ -131,6 +131,8 @@ struct fx_ctx {
   double *U = nullptr, *Ut = nullptr, *Uttmp = nullptr;
   double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr;
   double *Flux = nullptr, *Fv = nullptr, *grad = nullptr, *part = nullptr, *red = nullptr;
+  double *dtElem = nullptr;
+  int *errElem = nullptr;
   double4 *nvse = nullptr;
   int *e2s = nullptr, *s2e = nullptr;
   DtState *dts = nullptr;
@@ -255,12 +257,12 @@ void launch_lifting_n(fx_ctx *c, int out)
 template <int N>
 void launch_dt_n(fx_ctx *c, double cfl, double dfl)
 {
-  const int epb = LiftGeom<N>::EPB;
+  const int epb = DtGeom<N>::EPB;
   const int nb = (c->cfg.nElems + epb - 1) / epb;
   const double KsPr = c->cfg.kappa / c->cfg.Pr;
   const double KsPr_max = (4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr;
-  hipLaunchKernelGGL((k_calc_dt<N>), dim3(nb), dim3(256), 0, c->stream, c->U, c->Mf, c->Mg, c->Mh, c->sJ,
-                     c->dts, c->cfg.nElems, c->cfg.kappa, cfl, dfl, c->cfg.mu0, KsPr_max,
+  hipLaunchKernelGGL((k_calc_dt<N>), dim3(nb), dim3(DtGeom<N>::T), 0, c->stream, c->U, c->Mf, c->Mg, c->Mh, c->sJ,
+                     c->dtElem, c->errElem, c->cfg.nElems, c->cfg.kappa, cfl, dfl, c->cfg.mu0, KsPr_max,
                      c->cfg.parabolic);
 }
 
@@ -286,7 +288,13 @@ void launch_entropy_n(fx_ctx *c)
 void q_volume(fx_ctx *c, int mode, RKArgs rk) { FX_DISPATCH_N(c->cfg.N, launch_volume_n, c, mode, rk); }
 void q_surface(fx_ctx *c) { FX_DISPATCH_N(c->cfg.N, launch_surface_n, c); }
 void q_lifting(fx_ctx *c, int out) { FX_DISPATCH_N(c->cfg.N, launch_lifting_n, c, out); }
-void q_dt(fx_ctx *c, double cfl, double dfl) { FX_DISPATCH_N(c->cfg.N, launch_dt_n, c, cfl, dfl); }
+// CALCTIMESTEP into c->dts (advance=1 also moves t and the step counter)
+void q_dt(fx_ctx *c, double cfl, double dfl, int advance)
+{
+  FX_DISPATCH_N(c->cfg.N, launch_dt_n, c, cfl, dfl);
+  hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, c->stream, c->dtElem, c->errElem, c->cfg.nElems,
+                     c->dts, advance);
+}
 void q_entropy(fx_ctx *c) { FX_DISPATCH_N(c->cfg.N, launch_entropy_n, c); }
 
 // one RHS evaluation, followed by the RK stage update when rk != nullptr
@@ -397,11 +405,13 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   }
   const size_t npart = (n3E + 255) / 256 + 1;
   ALLOC(c->part, 2 * npart * 8);
+  ALLOC(c->dtElem, (size_t)cfg->nElems * 8);
+  ALLOC(c->errElem, (size_t)cfg->nElems * 4);
   ALLOC(c->red, 2 * 8);
   ALLOC(c->dts, sizeof(DtState));
 #undef ALLOC
   DtState z{};
-  z.dt_bits = 0; z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
+  z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
   if ((e = hipMemcpy(c->dts, &z, sizeof(z), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "init");
   *out = c;
   return FX_OK;
@@ -413,7 +423,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {c->U, c->Ut, c->Uttmp, c->Mf, c->Mg, c->Mh, c->sJ, c->Flux, c->Fv, c->grad,
-                  c->part, c->red, c->nvse, c->e2s, c->s2e, c->dts};
+                  c->part, c->red, c->nvse, c->e2s, c->s2e, c->dts, c->dtElem, c->errElem};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -541,9 +551,7 @@ int fx_calc_dt(fx_ctx *c, double CFLscale, double DFLscale, double *dt, int *err
   const double cfl = CFLscale * cfl_alpha(N) / (2.0 * N + 1.0);
   const double dfl = DFLscale * dfl_alpha(N) / ((2.0 * N + 1.0) * (2.0 * N + 1.0));
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->stream, c->dts, 0.0, 0.0, 1);
-  hipLaunchKernelGGL(k_dt_begin, dim3(1), dim3(1), 0, c->stream, c->dts);
-  q_dt(c, cfl, dfl);
-  hipLaunchKernelGGL(k_dt_end, dim3(1), dim3(1), 0, c->stream, c->dts, 0);
+  q_dt(c, cfl, dfl, 0);
   HIPCHK(hipGetLastError());
   DtState h;
   HIPCHK(hipMemcpyAsync(&h, c->dts, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -577,9 +585,7 @@ int fx_run(fx_ctx *c, int nSteps, double CFLscale, double DFLscale, double *t, d
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_dt_begin, dim3(1), dim3(1), 0, c->stream, c->dts);
-    q_dt(c, cfl, dfl);
-    hipLaunchKernelGGL(k_dt_end, dim3(1), dim3(1), 0, c->stream, c->dts, 1);
+    q_dt(c, cfl, dfl, 1);
     q_rk_step(c);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -608,6 +614,48 @@ int fx_entropy(fx_ctx *c, double *S, double *dSdU_Ut)
   HIPCHK(hipStreamSynchronize(c->stream));
   if (S) *S = h[0] / (c->cfg.kappa - 1.0);
   if (dSdU_Ut) *dSdU_Ut = h[1];
+  return FX_OK;
+}
+
+int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (reps < 1 || !avg_ms) return set_err(FX_EINVAL, "reps < 1 or null avg_ms");
+  if (which == FX_KERNEL_LIFTING && !c->cfg.parabolic) return set_err(FX_EINVAL, "lifting needs parabolic=1");
+  if (which < 0 || which > 4) return set_err(FX_EINVAL, "unknown kernel %d", which);
+  // make the inputs of the timed kernel valid: viscous fluxes, face fluxes and a time step
+  if (c->cfg.parabolic) q_lifting(c, 1);
+  q_surface(c);
+  if (which == FX_KERNEL_DT || which == FX_KERNEL_VOLUME_RK)
+    q_dt(c, 0.9 * cfl_alpha(c->cfg.N) / (2.0 * c->cfg.N + 1.0), 0.0, 0);
+  RKArgs rk;
+  rk.dt = &c->dts->dt; rk.A = c->rkA[1]; rk.b = c->rkb[1]; rk.first = 0;
+  if (which == FX_KERNEL_VOLUME_RK) {  // Ut_tmp must hold a previous stage: run stage 1 once
+    RKArgs r1 = rk; r1.A = 0.0; r1.b = c->rkb[0]; r1.first = 1;
+    q_volume(c, VMODE_RK, r1);
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; r++) {
+    switch (which) {
+    case FX_KERNEL_VOLUME_RK: q_volume(c, VMODE_RK, rk); break;
+    case FX_KERNEL_SURFACE: q_surface(c); break;
+    case FX_KERNEL_LIFTING: q_lifting(c, 1); break;
+    case FX_KERNEL_DT: q_dt(c, 0.9, 0.9, 0); break;
+    default: q_volume(c, VMODE_RHS, rk); break;
+    }
+  }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *avg_ms = (double)ms / reps;
   return FX_OK;
 }
 

@@ -116,8 +116,21 @@ struct RKArgs {
 // Element-centric volume kernel (VolInt_SplitForm volint.f90:173-218 + VolInt_visc
 // volint.f90:228-271 + SurfInt gather surfint.f90 + V2D_M_V1D(-sJ) dg.f90:446 +
 // the LSERK stage update timedisc.f90:454-470).
+//   phase 1  one thread per node stages the flux fields of EPB elements in LDS
+//   phase 2  one thread per (direction, element, line) evaluates the n(n-1)/2 symmetric
+//            two-point fluxes of its line (each pair once) and the weak-form viscous
+//            term, accumulating both Dvolsurf contributions in registers
+//   phase 3  the three direction partial sums are added in LDS (reusing the field
+//            storage), then one thread per node adds the surface fluxes, applies -sJ and
+//            writes Ut or performs the RK stage update (coalesced by node).
+#ifndef FX_VOL_WAVES
+#define FX_VOL_WAVES 1
+#endif
+#ifndef FX_VOL_ND_LDS
+#define FX_VOL_ND_LDS 1
+#endif
 template <int N, int VF, bool PARA, int MODE>
-__global__ void __launch_bounds__(3 * VolGeom<N>::W)
+__global__ void __launch_bounds__(3 * VolGeom<N>::W, FX_VOL_WAVES)
 k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
          const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
          const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
@@ -125,9 +138,23 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   constexpr int EPB = VolGeom<N>::EPB, W = VolGeom<N>::W, NP = Lds<N>::NP;
+  constexpr int NFV = n_fields<VF>();
+  constexpr int SLOTS = NFV > 5 ? NFV : 5;
   static_assert(EPB * n2 <= W, "volume launch geometry");
-  __shared__ double s_nd[EPB][NF_COUNT][NP];
-  __shared__ double s_acc[2][EPB][5][NP];
+#ifndef FX_VOL_EPAD
+#define FX_VOL_EPAD 0
+#endif
+  // per-element LDS blocks padded by FX_VOL_EPAD doubles to spread elements over banks
+  __shared__ double s_lds_raw[EPB][SLOTS * NP + FX_VOL_EPAD];  // node fields
+  auto s_lds = reinterpret_cast<double (*)[SLOTS * NP + FX_VOL_EPAD]>(s_lds_raw);
+#define SL(el_, slot_, l_) s_lds[el_][(slot_) * NP + (l_)]
+#if FX_VOL_ND_LDS
+  __shared__ double s_acc_raw[EPB][5 * NP + FX_VOL_EPAD];
+  auto s_acc = reinterpret_cast<double (*)[5 * NP + FX_VOL_EPAD]>(s_acc_raw);
+#define SA(el_, v_, l_) s_acc[el_][(v_) * NP + (l_)]
+#else
+#define SA(el_, v_, l_) SL(el_, v_, l_)
+#endif
 
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
@@ -136,16 +163,17 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
 
   // phase 1: per-node fields into LDS
   for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
-    int el = idx / n3, node = idx - el * n3;
-    int e = e0 + el;
+    const int el = idx / n3, node = idx - el * n3;
+    const int e = e0 + el;
     if (e < nElems) {
       const double *u = U + ((size_t)e * n3 + node) * 5;
       double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
       double f[NF_COUNT];
       node_fields<VF>(uu, kappaM1, f);
-      int l = Lds<N>::li(node);
+      const int l = Lds<N>::li(node);
 #pragma unroll
-      for (int c = 0; c < NF_COUNT; c++) s_nd[el][c][l] = f[c];
+      for (int c = 0; c < NF_COUNT; c++)
+        if (uses_field<VF>(c)) SL(el, field_slot<VF>(c), l) = f[c];
     }
   }
   __syncthreads();
@@ -156,20 +184,29 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
   const int e = e0 + el;
   const bool active = (el < EPB) && (e < nElems);
   const int a = line % n, b = line / n;
+  int nodes[n], lds[n];
+#pragma unroll
+  for (int l = 0; l < n; l++) {
+    const int i = (dir == 0) ? l : a, j = (dir == 1) ? l : (dir == 0 ? a : b), k = (dir == 2) ? l : b;
+    nodes[l] = i + n * (j + n * k);
+    lds[l] = Lds<N>::li(i, j, k);
+  }
+#if !FX_VOL_ND_LDS
+  double nd[n][NF_COUNT];
+#pragma unroll
+  for (int l = 0; l < n; l++)
+#pragma unroll
+    for (int c = 0; c < NF_COUNT; c++)
+      nd[l][c] = (active && uses_field<VF>(c)) ? SL(active ? el : 0, field_slot<VF>(c), lds[l]) : 0.0;
+  __syncthreads();  // field storage is free from here on
+#endif
+
   double acc[n][5];
 #pragma unroll
   for (int i = 0; i < n; i++)
 #pragma unroll
     for (int v = 0; v < 5; v++) acc[i][v] = 0.0;
-
   if (active) {
-    int nodes[n], lds[n];
-#pragma unroll
-    for (int l = 0; l < n; l++) {
-      int i = (dir == 0) ? l : a, j = (dir == 1) ? l : (dir == 0 ? a : b), k = (dir == 2) ? l : b;
-      nodes[l] = i + n * (j + n * k);
-      lds[l] = Lds<N>::li(i, j, k);
-    }
     const double *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
     double m[n][3];
 #pragma unroll
@@ -177,26 +214,29 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
       const double *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
       m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
     }
-    if (MODE == VMODE_VOLINT || true) {
-      double nd[n][NF_COUNT];
 #pragma unroll
-      for (int l = 0; l < n; l++)
+    for (int i = 0; i < n; i++)
 #pragma unroll
-        for (int c = 0; c < NF_COUNT; c++) nd[l][c] = s_nd[el][c][lds[l]];
+      for (int l = i + 1; l < n; l++) {
+        double F[5];
+#if FX_VOL_ND_LDS
+        double ndi[NF_COUNT], ndl[NF_COUNT];
 #pragma unroll
-      for (int i = 0; i < n; i++)
-#pragma unroll
-        for (int l = i + 1; l < n; l++) {
-          double F[5];
-          volume_flux<VF, double>(nd[i], nd[l], m[i], m[l], skappaM1, F);
-          const double dil = op.Ds[i][l], dli = op.Ds[l][i];
-#pragma unroll
-          for (int v = 0; v < 5; v++) {
-            acc[i][v] += dil * F[v];
-            acc[l][v] += dli * F[v];
-          }
+        for (int c = 0; c < NF_COUNT; c++) {
+          ndi[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[i]) : 0.0;
+          ndl[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[l]) : 0.0;
         }
-    }
+        volume_flux<VF, double>(ndi, ndl, m[i], m[l], skappaM1, F);
+#else
+        volume_flux<VF, double>(nd[i], nd[l], m[i], m[l], skappaM1, F);
+#endif
+        const double dil = op.Ds[i][l], dli = op.Ds[l][i];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          acc[i][v] += dil * F[v];
+          acc[l][v] += dli * F[v];
+        }
+      }
     if (PARA && MODE != VMODE_VOLINT) {
       // transformed viscous flux along the line: ft(l) = f*M1 + g*M2 + h*M3
       double ft[n][4];
@@ -215,75 +255,76 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
           for (int c = 0; c < 4; c++) acc[i][c + 1] += dh * ft[l][c];
         }
     }
-    if (dir > 0) {
+  }
+  // phase 3a: xi + eta + zeta partial sums, in this order, into LDS
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (active && dir == d) {
 #pragma unroll
       for (int l = 0; l < n; l++)
 #pragma unroll
-        for (int v = 0; v < 5; v++) s_acc[dir - 1][el][v][lds[l]] = acc[l][v];
+        for (int v = 0; v < 5; v++)
+          SA(el, v, lds[l]) = (d == 0) ? acc[l][v] : SA(el, v, lds[l]) + acc[l][v];
     }
+    __syncthreads();
   }
-  __syncthreads();
 
-  // phase 3: xi-line owners finish their n nodes
-  if (active && dir == 0) {
-    const int j = a, k = b;
-    int sideE[6], flipE[6];
+  // phase 3b: one thread per node
+  const double L0 = op.L0;
+  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+    const int el2 = idx / n3, node = idx - el2 * n3;
+    const int e2 = e0 + el2;
+    if (e2 >= nElems) continue;
+    const int i = node % n, j = (node / n) % n, k = node / n2;
+    const int l = Lds<N>::li(node);
+    double tot[5];
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      sideE[s] = e2s[(e * 6 + s) * 2] - 1;
-      flipE[s] = e2s[(e * 6 + s) * 2 + 1];
+    for (int v = 0; v < 5; v++) tot[v] = SA(el2, v, l);
+    if (MODE != VMODE_VOLINT) {
+      // SurfInt gather: faces touched by node (i,j,k), in local side order
+      int locs[3], nl = 0;
+      if (k == 0) locs[nl++] = ZETA_MINUS;
+      if (j == 0) locs[nl++] = ETA_MINUS;
+      if (i == N) locs[nl++] = XI_PLUS;
+      if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
+      if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
+      if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+      for (int t = 0; t < nl; t++) {
+        const int loc = locs[t];
+        const int s = e2s[(e2 * 6 + loc - 1) * 2] - 1, fl = e2s[(e2 * 6 + loc - 1) * 2 + 1];
+        int p, q;
+        vol_to_side<N>(loc, fl, i, j, k, p, q);
+        const double *F = Flux + ((size_t)(s * n + q) * n + p) * 5;
+        if (fl == 0) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) tot[v] = tot[v] + F[v] * L0;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; v++) tot[v] = tot[v] - F[v] * L0;
+        }
+      }
+      const double msj = -sJ[(size_t)e2 * n3 + node];
+#pragma unroll
+      for (int v = 0; v < 5; v++) tot[v] = tot[v] * msj;
     }
-    const double L0 = op.L0;
+    const size_t o = ((size_t)e2 * n3 + node) * 5;
+    if (MODE == VMODE_RK) {
+      const double bdt = rk.b * (*rk.dt);
 #pragma unroll
-    for (int i = 0; i < n; i++) {
-      const int node = i + n * (j + n * k);
-      const int l = Lds<N>::li(i, j, k);
-      double tot[5];
-#pragma unroll
-      for (int v = 0; v < 5; v++) tot[v] = acc[i][v] + s_acc[0][el][v][l] + s_acc[1][el][v][l];
-      if (MODE != VMODE_VOLINT) {
-        // SurfInt gather: faces touched by node (i,j,k), in local side order
-        int locs[3], nl = 0;
-        if (k == 0) locs[nl++] = ZETA_MINUS;
-        if (j == 0) locs[nl++] = ETA_MINUS;
-        if (i == N) locs[nl++] = XI_PLUS;
-        if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
-        if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
-        if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
-        for (int t = 0; t < nl; t++) {
-          const int loc = locs[t];
-          const int s = sideE[loc - 1], fl = flipE[loc - 1];
-          int p, q;
-          vol_to_side<N>(loc, fl, i, j, k, p, q);
-          const double *F = Flux + ((size_t)(s * n + q) * n + p) * 5;
-          if (fl == 0) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) tot[v] = tot[v] + F[v] * L0;
-          } else {
-#pragma unroll
-            for (int v = 0; v < 5; v++) tot[v] = tot[v] - F[v] * L0;
-          }
-        }
-        const double msj = -sJ[(size_t)e * n3 + node];
-#pragma unroll
-        for (int v = 0; v < 5; v++) tot[v] = tot[v] * msj;
+      for (int v = 0; v < 5; v++) {
+        const double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
+        Uttmp[o + v] = ut;
+        U[o + v] = U[o + v] + ut * bdt;
       }
-      const size_t o = ((size_t)e * n3 + node) * 5;
-      if (MODE == VMODE_RK) {
-        const double bdt = rk.b * (*rk.dt);
+    } else {
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-          double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
-          Uttmp[o + v] = ut;
-          U[o + v] = U[o + v] + ut * bdt;
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < 5; v++) Ut[o + v] = tot[v];
-      }
+      for (int v = 0; v < 5; v++) Ut[o + v] = tot[v];
     }
   }
 }
+
+#undef SL
+#undef SA
 
 // Side-centric numerical flux: Flux(:,p,q,s) = F*(U_master, U_slave; n) * SurfElem, with
 // the BR1 viscous average for Navier-Stokes (FillFlux fillflux.f90:41-110, Riemann
@@ -435,10 +476,9 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 
 // ---------------------------------------------------------------------------------
 // CALCTIMESTEP (navierstokes/calctimestep.f90:39-170): per-element convective and
-// viscous eigenvalue bounds, global minimum via an order-independent atomic min on the
-// bit pattern of the positive dt (deterministic).
+// viscous eigenvalue bounds into dtElem (as the reference's dtElem), then a one-block
+// minimum in fixed order (deterministic, no atomics).
 struct DtState {
-  unsigned long long dt_bits;  // running min of positive dt
   int errType;                 // max error code seen
   int errStep;                 // first step with an error (-1 none)
   double dt;                   // dt of the current step
@@ -447,24 +487,32 @@ struct DtState {
   int pad;
 };
 
+// one group of G = 2^ceil(log2 n^3) threads per element, tree max-reduction in LDS
+template <int N> struct DtGeom {
+  static constexpr int n3 = (N + 1) * (N + 1) * (N + 1);
+  static constexpr int G = n3 <= 32 ? 32 : (n3 <= 64 ? 64 : (n3 <= 128 ? 128 : (n3 <= 256 ? 256 : 512)));
+  static constexpr int EPB = 256 / G > 0 ? 256 / G : 1;
+  static constexpr int T = EPB * G;
+};
+
 template <int N>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(DtGeom<N>::T)
 k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
-          const double *__restrict__ Mh, const double *__restrict__ sJ, DtState *st, int nElems,
-          double kappa, double cfl, double dfl, double mu0, double KsPr_max, int parabolic)
+          const double *__restrict__ Mh, const double *__restrict__ sJ, double *__restrict__ dtElem,
+          int *__restrict__ errElem, int nElems, double kappa, double cfl, double dfl, double mu0,
+          double KsPr_max, int parabolic)
 {
-  constexpr int n = N + 1, n3 = n * n * n;
-  constexpr int EPB = LiftGeom<N>::EPB;
-  __shared__ double s_l[EPB][6][n3];
-  __shared__ int s_err[EPB][n3];
+  constexpr int n3 = DtGeom<N>::n3, G = DtGeom<N>::G, EPB = DtGeom<N>::EPB, T = DtGeom<N>::T;
+  __shared__ double s_l[6][T];
+  __shared__ int s_err[T];
   const int tid = threadIdx.x;
-  const int el = tid / n3, node = tid - el * n3;
+  const int el = tid / G, node = tid - el * G;
   const int e = blockIdx.x * EPB + el;
-  const bool active = (el < EPB) && (e < nElems);
-  if (active) {
+  double l[6] = {1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12};
+  int err = 0;
+  if (node < n3 && e < nElems) {
     const size_t c = (size_t)e * n3 + node;
     const double *u = U + 5 * c;
-    int err = 0;
     if (isnan(u[0])) err = 1;
     const double srho = 1.0 / u[0];
     const double v0 = u[1] * srho, v1 = u[2] * srho, v2 = u[3] * srho;
@@ -472,53 +520,76 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     if (p <= 0.0) err = 2;
     const double cs = sqrt(kappa * p * srho), sj = sJ[c];
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-    s_l[el][0][node] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
-    s_l[el][1][node] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
-    s_l[el][2][node] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
+    l[0] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
+    l[1] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
+    l[2] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
     if (parabolic) {
       const double muX = srho * KsPr_max * mu0;
-      s_l[el][3][node] = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
-      s_l[el][4][node] = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
-      s_l[el][5][node] = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
+      l[3] = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
+      l[4] = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
+      l[5] = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
     }
-    s_err[el][node] = err;
+    // a NaN eigenvalue leaves the running maximum unchanged (fmax)
+#pragma unroll
+    for (int d = 0; d < 6; d++) l[d] = fmax(l[d], 1.0e-12);
   }
+#pragma unroll
+  for (int d = 0; d < 6; d++) s_l[d][tid] = l[d];
+  s_err[tid] = err;
   __syncthreads();
-  if (active && node == 0) {
-    double l[6] = {1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12};
-    int err = 0;
-    for (int m = 0; m < n3; m++) {
-      // MAX as in the reference: a NaN lambda leaves the running max unchanged
+  for (int stp = G / 2; stp > 0; stp >>= 1) {
+    if (node < stp) {
 #pragma unroll
-      for (int d = 0; d < 3; d++) l[d] = (s_l[el][d][m] > l[d]) ? s_l[el][d][m] : l[d];
-      if (parabolic) {
-#pragma unroll
-        for (int d = 3; d < 6; d++) l[d] = (s_l[el][d][m] > l[d]) ? s_l[el][d][m] : l[d];
-      }
-      err = max(err, s_err[el][m]);
+      for (int d = 0; d < 6; d++) s_l[d][tid] = fmax(s_l[d][tid], s_l[d][tid + stp]);
+      s_err[tid] = max(s_err[tid], s_err[tid + stp]);
     }
-    double dte = cfl * 2.0 / (l[0] + l[1] + l[2]);
-    if (isnan(dte)) err = max(err, 3);
+    __syncthreads();
+  }
+  if (node == 0 && e < nElems) {
+    int er = s_err[tid];
+    double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
+    if (isnan(dte)) er = max(er, 3);
     if (parabolic) {
-      double dve = dfl * 4.0 / (l[3] + l[4] + l[5]);
-      if (isnan(dve)) err = max(err, 4);
+      const double dve = dfl * 4.0 / (s_l[3][tid] + s_l[4][tid] + s_l[5][tid]);
+      if (isnan(dve)) er = max(er, 4);
       dte = fmin(dte, dve);
     }
-    if (!isnan(dte) && dte > 0.0) atomicMin(&st->dt_bits, (unsigned long long)__double_as_longlong(dte));
-    if (err) atomicMax(&st->errType, err);
+    dtElem[e] = dte;
+    errElem[e] = er;
   }
 }
 
-__global__ void k_dt_begin(DtState *st)
+// Minimum of dtElem (NaN entries skipped) and maximum error code, one block, fixed order.
+__global__ void __launch_bounds__(1024)
+k_dt_reduce(const double *__restrict__ dtElem, const int *__restrict__ errElem, int nElems, DtState *st,
+            int advance)
 {
-  st->dt_bits = 0x7FEFFFFFFFFFFFFFull;  // largest finite double
-}
-
-__global__ void k_dt_end(DtState *st, int advance)
-{
-  st->dt = __longlong_as_double((long long)st->dt_bits);
-  if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
-  if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+  __shared__ double s_dt[1024];
+  __shared__ int s_er[1024];
+  const int tid = threadIdx.x;
+  double m = 1.79e308;
+  int er = 0;
+  for (int e = tid; e < nElems; e += 1024) {
+    const double d = dtElem[e];
+    if (d < m) m = d;
+    er = max(er, errElem[e]);
+  }
+  s_dt[tid] = m;
+  s_er[tid] = er;
+  __syncthreads();
+  for (int stp = 512; stp > 0; stp >>= 1) {
+    if (tid < stp) {
+      s_dt[tid] = fmin(s_dt[tid], s_dt[tid + stp]);
+      s_er[tid] = max(s_er[tid], s_er[tid + stp]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st->dt = s_dt[0];
+    st->errType = max(st->errType, s_er[0]);
+    if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
+    if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+  }
 }
 
 // CalcEntropy (analyze_equation.f90:549-599): per-block partial sums (fixed order),

@@ -2,14 +2,16 @@
 //
 // Each function names the reference routine (project-fluxo/fluxo) it computes; the
 // arithmetic follows the reference's formulas so that FP64 results agree with the
-// reference to rounding.  Two deliberate reformulations, both exact in real
+// reference to rounding.  Three deliberate reformulations, all exact in real
 // arithmetic:
 //  * the logarithmic mean takes ln(a), ln(b) precomputed once per node, so
 //    (b-a)/ln(b/a) becomes (b-a)/(ln b - ln a); the near-equal branch keeps the
 //    reference's series and switch (LN_MEAN, navierstokes/flux_average.f90:1945-1965);
 //  * surface fluxes are evaluated in the Cartesian frame instead of rotating the
 //    momentum into (n,t1,t2) and back (riemann.f90:174-246), which is an
-//    orthogonal change of basis.
+//    orthogonal change of basis;
+//  * inside the two-point fluxes, quotients use the hardware reciprocal refined by a
+//    Newton step (about 1 ulp) instead of a correctly rounded division.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,28 +19,46 @@ namespace fx {
 
 constexpr double LNMEAN_EPS = 1.0e-4;
 
-// Log mean of a, b given la = ln a, lb = ln b.
+// Reciprocal from the hardware estimate refined by one Newton step (error ~1 ulp); used
+// where a correctly rounded quotient is not needed for FP64 parity at 1e-12.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Log mean of a, b given la = ln a, lb = ln b, evaluated without branches: both the
+// near-equal series and the log quotient are formed and one is selected, so divergent
+// lanes cost one quotient instead of both paths.  u = ((b-a)/(b+a))^2 only feeds the
+// series (u < 1e-4 there), where its ~1-ulp reciprocal error is below 1e-20.
+template <typename R>
+__device__ __forceinline__ void ln_mean_parts(R a, R b, R la, R lb, R &num, R &den)
+{
+  const R d = b - a, s = a + b, d2 = d * d, s2 = s * s;
+  const R u = d2 * rcp_nr(s2);
+  const R P = R(105.0) + u * (R(35.0) + u * (R(21.0) + u * R(15.0)));
+  const bool series = d2 < R(LNMEAN_EPS) * s2;
+  num = series ? s * R(52.5) : d;
+  den = series ? P : (lb - la);
+}
+
 template <typename R>
 __device__ __forceinline__ R ln_mean(R a, R b, R la, R lb)
 {
-  R d = b - a, s = a + b;
-  if (d * d < R(LNMEAN_EPS) * (s * s)) {
-    R u = (d * d) / (s * s);
-    return s * R(52.5) / (R(105.0) + u * (R(35.0) + u * (R(21.0) + u * R(15.0))));
-  }
-  return d / (lb - la);
+  R num, den;
+  ln_mean_parts(a, b, la, lb, num, den);
+  return num * rcp_nr(den);
 }
 
 // 1 / ln_mean(a, b)
 template <typename R>
 __device__ __forceinline__ R inv_ln_mean(R a, R b, R la, R lb)
 {
-  R d = b - a, s = a + b;
-  if (d * d < R(LNMEAN_EPS) * (s * s)) {
-    R u = (d * d) / (s * s);
-    return (R(105.0) + u * (R(35.0) + u * (R(21.0) + u * R(15.0)))) / (s * R(52.5));
-  }
-  return (lb - la) / d;
+  R num, den;
+  ln_mean_parts(a, b, la, lb, num, den);
+  return den * rcp_nr(num);
 }
 
 // Per-node quantities staged for the two-point fluxes.
@@ -46,6 +66,28 @@ __device__ __forceinline__ R inv_ln_mean(R a, R b, R la, R lb)
 // X = rho/p for Ranocha (RanochaFluxVec), X = beta = rho/(2p) for Chandrashekar
 // (EntropyAndEnergyConservingFluxVec); EvalUaux (flux_average.f90:305-332).
 enum { NF_RHO = 0, NF_V1, NF_V2, NF_V3, NF_P, NF_LNRHO, NF_X, NF_LNX, NF_VV, NF_E, NF_COUNT };
+
+// Fields a volume flux reads (the others are never staged).
+template <int VF>
+__host__ __device__ constexpr bool uses_field(int f)
+{
+  return VF == 32 ? (f <= NF_LNX) : (VF == 5 ? (f <= NF_VV) : (f <= NF_P || f == NF_E));
+}
+template <int VF>
+__host__ __device__ constexpr int n_fields()
+{
+  int c = 0;
+  for (int f = 0; f < NF_COUNT; f++) c += uses_field<VF>(f) ? 1 : 0;
+  return c;
+}
+// LDS slot of field f (fields are packed in enum order)
+template <int VF>
+__host__ __device__ constexpr int field_slot(int f)
+{
+  int c = 0;
+  for (int g = 0; g < f; g++) c += uses_field<VF>(g) ? 1 : 0;
+  return c;
+}
 
 template <int VF>
 __device__ __forceinline__ void node_fields(const double *u, double kappaM1, double *f)

@@ -1,0 +1,126 @@
+!==================================================================================================
+!> ISO_C_BINDING interface to the engine's C ABI (include/fluxo_gpu.h).  The derived types are
+!> interoperable with fx_config / fx_mesh; every function returns 0 or a negative FX_E* code.
+!==================================================================================================
+module fluxo_gpu_bind
+  use, intrinsic :: iso_c_binding
+  implicit none
+  public
+
+  integer(c_int), parameter :: FX_OK = 0, FX_PREC_FP64 = 0, FX_PREC_FP32_SPLIT = 1
+
+  type, bind(C) :: fx_config
+    integer(c_int) :: N = 3, nVar = 5, nElems = 0, nSides = 0
+    integer(c_int) :: volumeFlux = 0, riemann = 1, parabolic = 0, lifting = 1
+    integer(c_int) :: precision = 0, reserved = 0
+    real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, reserved2 = 0.0d0
+  end type fx_config
+
+  type, bind(C) :: fx_mesh
+    type(c_ptr) :: ElemToSide = c_null_ptr, SideToElem = c_null_ptr
+    type(c_ptr) :: Metrics_fTilde = c_null_ptr, Metrics_gTilde = c_null_ptr, Metrics_hTilde = c_null_ptr
+    type(c_ptr) :: sJ = c_null_ptr, NormVec = c_null_ptr, TangVec1 = c_null_ptr, TangVec2 = c_null_ptr
+    type(c_ptr) :: SurfElem = c_null_ptr
+  end type fx_mesh
+
+  interface
+    integer(c_int) function fx_ctx_create(cfg, ctx) bind(C, name="fx_ctx_create")
+      import :: c_int, c_ptr, fx_config
+      type(fx_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+    end function fx_ctx_create
+    subroutine fx_ctx_destroy(ctx) bind(C, name="fx_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine fx_ctx_destroy
+    type(c_ptr) function fx_last_error() bind(C, name="fx_last_error")
+      import :: c_ptr
+    end function fx_last_error
+    integer(c_int) function fx_mesh_upload(ctx, mesh) bind(C, name="fx_mesh_upload")
+      import :: c_int, c_ptr, fx_mesh
+      type(c_ptr), value :: ctx
+      type(fx_mesh), intent(in) :: mesh
+    end function fx_mesh_upload
+    integer(c_int) function fx_state_upload(ctx, U) bind(C, name="fx_state_upload")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: U(*)
+    end function fx_state_upload
+    integer(c_int) function fx_state_download(ctx, U) bind(C, name="fx_state_download")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: U(*)
+    end function fx_state_download
+    integer(c_int) function fx_ut_download(ctx, Ut) bind(C, name="fx_ut_download")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: Ut(*)
+    end function fx_ut_download
+    integer(c_int) function fx_rhs(ctx, t, Ut) bind(C, name="fx_rhs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t
+      real(c_double), intent(out) :: Ut(*)
+    end function fx_rhs
+    integer(c_int) function fx_volint(ctx, Ut) bind(C, name="fx_volint")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: Ut(*)
+    end function fx_volint
+    integer(c_int) function fx_lifting(ctx, gx, gy, gz) bind(C, name="fx_lifting")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: gx(*), gy(*), gz(*)
+    end function fx_lifting
+    integer(c_int) function fx_calc_dt(ctx, cfl, dfl, dt, errType) bind(C, name="fx_calc_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cfl, dfl
+      real(c_double), intent(out) :: dt
+      integer(c_int), intent(out) :: errType
+    end function fx_calc_dt
+    integer(c_int) function fx_rk_step(ctx, t, dt) bind(C, name="fx_rk_step")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t, dt
+    end function fx_rk_step
+    integer(c_int) function fx_run(ctx, nSteps, cfl, dfl, t, dt, errType) bind(C, name="fx_run")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nSteps
+      real(c_double), value :: cfl, dfl
+      real(c_double), intent(inout) :: t
+      real(c_double), intent(out) :: dt
+      integer(c_int), intent(out) :: errType
+    end function fx_run
+    integer(c_int) function fx_entropy(ctx, S, dSdU_Ut) bind(C, name="fx_entropy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: S, dSdU_Ut
+    end function fx_entropy
+    integer(c_int) function fx_synchronize(ctx) bind(C, name="fx_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function fx_synchronize
+    integer(c_long_long) function fx_ctx_bytes(cfg) bind(C, name="fx_ctx_bytes")
+      import :: c_long_long, fx_config
+      type(fx_config), intent(in) :: cfg
+    end function fx_ctx_bytes
+  end interface
+
+contains
+
+  !> Text of the library's last error as a Fortran string.
+  function fx_error_text() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(fx_last_error(), p, [512])
+    do i = 1, 512
+      if (p(i) == c_null_char) exit
+      msg(i:i) = p(i)
+    end do
+  end function fx_error_text
+
+end module fluxo_gpu_bind

@@ -1,0 +1,213 @@
+!==================================================================================================
+!> FLUXO-shaped host interface over the MI355X engine.
+!>
+!> The routine names and argument meanings are those of the reference so that a FLUXO-style
+!> driver can call them unchanged:
+!>   DGTimeDerivative_weakForm(t)    src/dg/dg.f90:282-452      (drop-in: U up, Ut down)
+!>   VolInt_SplitForm(Ut)            src/dg/volint.f90:173-218
+!>   Lifting(t)                      src/dg/br1/br1.f90:154-364
+!>   CALCTIMESTEP(errType)           src/equation/navierstokes/calctimestep.f90:39-170
+!>   TimeStepByLSERKW2(t)            src/timedisc/timedisc.f90:424-480 (device-resident U)
+!>   CalcEntropy(S,dSdU_Ut)          src/equation/navierstokes/analyze_equation.f90:549-599
+!>   ExactFunc(2|7)                  src/equation/navierstokes/equation.f90:632-651, 802-811
+!> Module variables play the role of MOD_DG_Vars / MOD_Mesh_Vars / MOD_TimeDisc_Vars.
+!==================================================================================================
+module fxh_dg
+  use, intrinsic :: iso_c_binding
+  use fxh_basis, only: rk
+  use fluxo_gpu_bind
+  implicit none
+  private
+  public :: InitDG_GPU, FinalizeDG_GPU, DGTimeDerivative_weakForm, VolInt_SplitForm, Lifting
+  public :: CALCTIMESTEP, TimeStepByLSERKW2, TimeDiscRun, CalcEntropy, ExactFunc, FillIni
+  public :: StateToHost, StateToDevice, CalcErrorNorms
+  public :: U, Ut, gradPx, gradPy, gradPz, Elem_xGP, sJ, PP_N, nElems, nSides, dt, CFLScale, DFLScale
+  public :: Kappa, AdvVel, IniAmplitude, IniExactFunc, cfg
+
+  integer :: PP_N = 3, nElems = 0, nSides = 0, IniExactFunc = 2
+  real(rk), allocatable, target :: U(:, :, :, :, :), Ut(:, :, :, :, :)
+  real(rk), allocatable, target :: gradPx(:, :, :, :, :), gradPy(:, :, :, :, :), gradPz(:, :, :, :, :)
+  real(rk), allocatable, target :: Elem_xGP(:, :, :, :, :), sJ(:, :, :, :)
+  real(rk) :: dt = 0.0_rk, CFLScale = 0.9_rk, DFLScale = 0.9_rk
+  real(rk) :: Kappa = 1.4_rk, AdvVel(3) = 1.0_rk, IniAmplitude = 0.1_rk
+  type(fx_config) :: cfg
+  type(c_ptr) :: ctx = c_null_ptr
+
+contains
+
+  subroutine chk(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= FX_OK) then
+      write(*, '(A,I0,A)') ' ERROR in '//where//' (code ', rc, '): '//trim(fx_error_text())
+      stop 2
+    end if
+  end subroutine chk
+
+  !> InitDG + mesh upload: cfgIn carries the parameter.ini options, the mesh arrays are in
+  !> the reference layout.
+  subroutine InitDG_GPU(cfgIn, e2s, s2e, Mf, Mg, Mh, sJin, NormVec, TangVec1, TangVec2, SurfElem, xGP)
+    type(fx_config), intent(in) :: cfgIn
+    integer(c_int), intent(in), target :: e2s(:, :, :), s2e(:, :)
+    real(rk), intent(in), target :: Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
+    real(rk), intent(in), target :: sJin(:, :, :, :), NormVec(:, :, :, :), TangVec1(:, :, :, :)
+    real(rk), intent(in), target :: TangVec2(:, :, :, :), SurfElem(:, :, :), xGP(:, :, :, :, :)
+    type(fx_mesh) :: m
+    cfg = cfgIn
+    PP_N = cfg%N; nElems = cfg%nElems; nSides = cfg%nSides; Kappa = cfg%kappa
+    call chk(fx_ctx_create(cfg, ctx), 'fx_ctx_create')
+    m%ElemToSide = c_loc(e2s); m%SideToElem = c_loc(s2e)
+    m%Metrics_fTilde = c_loc(Mf); m%Metrics_gTilde = c_loc(Mg); m%Metrics_hTilde = c_loc(Mh)
+    m%sJ = c_loc(sJin); m%NormVec = c_loc(NormVec); m%TangVec1 = c_loc(TangVec1)
+    m%TangVec2 = c_loc(TangVec2); m%SurfElem = c_loc(SurfElem)
+    call chk(fx_mesh_upload(ctx, m), 'fx_mesh_upload')
+    allocate(U(5, 0:PP_N, 0:PP_N, 0:PP_N, nElems), Ut(5, 0:PP_N, 0:PP_N, 0:PP_N, nElems))
+    allocate(Elem_xGP(3, 0:PP_N, 0:PP_N, 0:PP_N, nElems), sJ(0:PP_N, 0:PP_N, 0:PP_N, nElems))
+    Elem_xGP = xGP; sJ = sJin
+    U = 0.0_rk; Ut = 0.0_rk
+  end subroutine InitDG_GPU
+
+  subroutine FinalizeDG_GPU()
+    if (c_associated(ctx)) call fx_ctx_destroy(ctx)
+    ctx = c_null_ptr
+    if (allocated(U)) deallocate(U, Ut, Elem_xGP, sJ)
+    if (allocated(gradPx)) deallocate(gradPx, gradPy, gradPz)
+  end subroutine FinalizeDG_GPU
+
+  subroutine StateToDevice()
+    call chk(fx_state_upload(ctx, U), 'fx_state_upload')
+  end subroutine StateToDevice
+
+  subroutine StateToHost()
+    call chk(fx_state_download(ctx, U), 'fx_state_download')
+  end subroutine StateToHost
+
+  !> Drop-in form: U is taken from the host array, Ut is returned in the host array.
+  subroutine DGTimeDerivative_weakForm(tIn)
+    real(rk), intent(in) :: tIn
+    call chk(fx_state_upload(ctx, U), 'fx_state_upload')
+    call chk(fx_rhs(ctx, tIn, Ut), 'fx_rhs')
+  end subroutine DGTimeDerivative_weakForm
+
+  !> Volume split-form contribution alone (before -sJ), on the current host U.
+  subroutine VolInt_SplitForm(UtOut)
+    real(rk), intent(out) :: UtOut(:, :, :, :, :)
+    call chk(fx_state_upload(ctx, U), 'fx_state_upload')
+    call chk(fx_volint(ctx, UtOut), 'fx_volint')
+  end subroutine VolInt_SplitForm
+
+  !> BR1 lifting of the current host U into gradPx/y/z (primitive-variable gradients).
+  subroutine Lifting(tIn)
+    real(rk), intent(in) :: tIn
+    if (.not. allocated(gradPx)) then
+      allocate(gradPx, mold=U); allocate(gradPy, mold=U); allocate(gradPz, mold=U)
+    end if
+    call chk(fx_state_upload(ctx, U), 'fx_state_upload')
+    call chk(fx_lifting(ctx, gradPx, gradPy, gradPz), 'fx_lifting')
+  end subroutine Lifting
+
+  !> CFL/DFL time step of the device state, CFLScale/DFLScale as read from parameter.ini.
+  real(rk) function CALCTIMESTEP(errType)
+    integer, intent(out) :: errType
+    integer(c_int) :: e
+    real(c_double) :: d
+    call chk(fx_calc_dt(ctx, CFLScale, DFLScale, d, e), 'fx_calc_dt')
+    errType = e
+    CALCTIMESTEP = d
+  end function CALCTIMESTEP
+
+  !> One Carpenter RK4-5 step of the device-resident state with the module's dt.
+  subroutine TimeStepByLSERKW2(t)
+    real(rk), intent(in) :: t
+    call chk(fx_rk_step(ctx, t, dt), 'fx_rk_step')
+  end subroutine TimeStepByLSERKW2
+
+  !> nSteps of the TimeDisc loop (CALCTIMESTEP + TimeStepByLSERKW2) on the device.
+  subroutine TimeDiscRun(nSteps, t, errType)
+    integer, intent(in) :: nSteps
+    real(rk), intent(inout) :: t
+    integer, intent(out) :: errType
+    integer(c_int) :: e
+    real(c_double) :: tt, d
+    tt = t
+    call chk(fx_run(ctx, nSteps, CFLScale, DFLScale, tt, d, e), 'fx_run')
+    t = tt; dt = d; errType = e
+  end subroutine TimeDiscRun
+
+  !> Entropy integral and dS/dU.Ut of the device state (Ut of the last RHS evaluation).
+  subroutine CalcEntropy(S, dSdU_Ut)
+    real(rk), intent(out) :: S, dSdU_Ut
+    real(c_double) :: a, b
+    call chk(fx_rhs(ctx, 0.0_rk, Ut), 'fx_rhs')
+    call chk(fx_entropy(ctx, a, b), 'fx_entropy')
+    S = a; dSdU_Ut = b
+  end subroutine CalcEntropy
+
+  subroutine ExactFunc(ExactFunction, tIn, x, Resu)
+    integer, intent(in) :: ExactFunction
+    real(rk), intent(in) :: tIn, x(3)
+    real(rk), intent(out) :: Resu(5)
+    real(rk) :: prim(5), cent(3), pi, A, Ms, skm1
+    pi = acos(-1.0_rk)
+    skm1 = 1.0_rk/(Kappa - 1.0_rk)
+    select case (ExactFunction)
+    case (7)  ! Taylor-Green vortex
+      A = 1.0_rk; Ms = 0.1_rk
+      prim(1) = 1.0_rk
+      prim(2) = A*sin(x(1))*cos(x(2))*cos(x(3))
+      prim(3) = -A*cos(x(1))*sin(x(2))*cos(x(3))
+      prim(4) = 0.0_rk
+      prim(5) = (A/Ms*A/Ms/Kappa*prim(1))
+      prim(5) = prim(5) + 1.0_rk/16.0_rk*A*A*prim(1)*(cos(2*x(1))*cos(2.0_rk*x(3)) + 2.0_rk*cos(2.0_rk*x(2)) &
+                + 2.0_rk*cos(2.0_rk*x(1)) + cos(2*x(2))*cos(2.0_rk*x(3)))
+      Resu(1) = prim(1)
+      Resu(2:4) = prim(2:4)*prim(1)
+      Resu(5) = skm1*prim(5) + 0.5_rk*sum(Resu(2:4)*prim(2:4))
+    case default  ! 2: density wave
+      prim(1) = 1.0_rk; prim(2:4) = AdvVel; prim(5) = 1.0_rk
+      cent = x - AdvVel*tIn
+      prim(1) = prim(1)*(1.0_rk + IniAmplitude*sin(pi*(1.0_rk*cent(1) + 1.0_rk*cent(2) + 1.0_rk*cent(3))))
+      Resu(1) = prim(1)
+      Resu(2:4) = prim(1)*prim(2:4)
+      Resu(5) = prim(5)*skm1 + 0.5_rk*prim(1)*sum(prim(2:4)*prim(2:4))
+    end select
+  end subroutine ExactFunc
+
+  subroutine FillIni()
+    integer :: i, j, k, e
+    do e = 1, nElems
+      do k = 0, PP_N
+        do j = 0, PP_N
+          do i = 0, PP_N
+            call ExactFunc(IniExactFunc, 0.0_rk, Elem_xGP(:, i, j, k, e), U(:, i, j, k, e))
+          end do
+        end do
+      end do
+    end do
+  end subroutine FillIni
+
+  !> L2 / Linf errors of the host U against ExactFunc at time t, on the volume nodes with the
+  !> LGL quadrature (the reference's CalcErrorNorms interpolates to NAnalyze first).
+  subroutine CalcErrorNorms(t, w, L2, Linf)
+    real(rk), intent(in) :: t, w(0:PP_N)
+    real(rk), intent(out) :: L2(5), Linf(5)
+    real(rk) :: ex(5), vol, wv
+    integer :: i, j, k, e
+    L2 = 0.0_rk; Linf = 0.0_rk; vol = 0.0_rk
+    do e = 1, nElems
+      do k = 0, PP_N
+        do j = 0, PP_N
+          do i = 0, PP_N
+            call ExactFunc(IniExactFunc, t, Elem_xGP(:, i, j, k, e), ex)
+            wv = w(i)*w(j)*w(k)/sJ(i, j, k, e)
+            L2 = L2 + (U(:, i, j, k, e) - ex)**2*wv
+            Linf = max(Linf, abs(U(:, i, j, k, e) - ex))
+            vol = vol + wv
+          end do
+        end do
+      end do
+    end do
+    L2 = sqrt(L2/vol)
+  end subroutine CalcErrorNorms
+
+end module fxh_dg

@@ -43,7 +43,9 @@ class fx_mesh(C.Structure):
 EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_state_upload", "fx_state_download", "fx_ut_download", "fx_rhs", "fx_volint",
            "fx_lifting", "fx_calc_dt", "fx_rk_step", "fx_run", "fx_entropy", "fx_synchronize",
-           "fx_ctx_bytes"]
+           "fx_ctx_bytes", "fx_profile_kernel"]
+
+FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS = range(5)
 
 
 def lib():
@@ -71,6 +73,7 @@ def lib():
     L.fx_synchronize.argtypes = [vp]
     L.fx_ctx_bytes.argtypes = [C.POINTER(fx_config)]
     L.fx_ctx_bytes.restype = C.c_longlong
+    L.fx_profile_kernel.argtypes = [vp, C.c_int, C.c_int, dp]
     _LIB = L
     return L
 
@@ -181,6 +184,12 @@ class Engine:
         s, d = C.c_double(), C.c_double()
         _chk(lib().fx_entropy(self._ctx, C.byref(s), C.byref(d)))
         return s.value, d.value
+
+    def profile_kernel(self, which, reps=20):
+        """Average ms per launch of one engine kernel (HIP events on the engine stream)."""
+        ms = C.c_double()
+        _chk(lib().fx_profile_kernel(self._ctx, which, reps, C.byref(ms)))
+        return ms.value
 
     def sync(self):
         _chk(lib().fx_synchronize(self._ctx))

@@ -116,6 +116,15 @@ int fx_entropy(fx_ctx *ctx, double *S, double *dSdU_Ut);
 /* Wait for all work queued on the context's stream. */
 int fx_synchronize(fx_ctx *ctx);
 
+/* Average duration (ms) of one launch of a single engine kernel, timed with HIP events on
+   the context's stream over `reps` back-to-back launches on the current device state.
+   which: FX_KERNEL_*.  The volume kernel is timed in its LSERK form (stage >= 2 traffic)
+   and leaves U advanced by `reps` partial stages.  No counterpart in the reference; it
+   gives the benchmark per-kernel durations for the roofline figures. */
+enum { FX_KERNEL_VOLUME_RK = 0, FX_KERNEL_SURFACE = 1, FX_KERNEL_LIFTING = 2, FX_KERNEL_DT = 3,
+       FX_KERNEL_VOLUME_RHS = 4 };
+int fx_profile_kernel(fx_ctx *ctx, int which, int reps, double *avg_ms);
+
 /* Device bytes held by a context with this configuration (for HBM admission). */
 long long fx_ctx_bytes(const fx_config *cfg);
 

@@ -203,7 +203,8 @@ def test_run_graph_matches_oracle_navier_stokes():
 
 
 def test_c1_density_wave_100_steps():
-    """8^3 Cartesian, N=3, Ranocha+LLF, LSERK4-5, CFL 0.9, 100 steps."""
+    """Density wave on [0,2]^3, 8^3 Cartesian elements, N=3, Ranocha+LLF, LSERK4-5, CFL 0.9,
+    100 steps."""
     m = O.box_mesh(8, 3, Ngeo=1)
     U0 = density_wave(m)
     P = O.Problem(m, volflux=32, riemann=1)
