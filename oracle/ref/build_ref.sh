@@ -1,0 +1,54 @@
+#!/bin/bash
+# Build the reference's own compute modules (project-fluxo/fluxo Fortran, compiled where they
+# lie under /root/reference/src) together with oracle/ref/fluxo_ref_harness.f90 into
+# oracle/_ref/.  Test infrastructure only: the binaries produce golden values and the
+# reported CPU baseline; the product never links or calls them.
+#
+# The reference's CMake build is not used.  The defines below are the ones its CMakeLists.txt
+# sets for the named configurations (LGL nodes, split form, no MPI/AMR/shock capturing).  The
+# two-point volume flux is fixed at compile time (FLUXO_EQN_VOLFLUX, CMakeLists.txt:242-245):
+# flang's preprocessor rejects the run-time default's "#if PP_VolFlux==-1".  Translation units
+# that need HDF5, the HOPR mesh reader, restart or the time-disc driver do not compile here and are not needed by the harness; the
+# loop compiles in dependency order until no further file compiles; object files already
+# built are kept, so a rebuild only recompiles the harness and relinks.
+set -e
+REF=${REF:-/root/reference/src}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$(cd "$HERE/.." && pwd)/_ref
+FC=${FC:-flang}
+[ -d "$REF" ] || { echo "reference sources not found at $REF; nothing built"; exit 0; }
+mkdir -p "$OUT"
+COMMON="-cpp -O2 -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0 -I$REF"
+
+build_variant() {
+  local name=$1; shift
+  local defs="$*"
+  local obj="$OUT/obj_$name"
+  mkdir -p "$obj"
+  # the lifting directory not selected by PP_Lifting is left out, as the reference's CMake does
+  local exclude='/amr/|/equation/(mhd|maxwell|linearscalaradvection)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/br2/'
+  find "$REF" -name "*.f90" | grep -v -E "$exclude" | sort > "$obj/todo.txt"
+  local pass n f b
+  for pass in $(seq 1 15); do
+    : > "$obj/next.txt"; n=0
+    while read -r f; do
+      b=$(basename "$f" .f90)
+      if [ -f "$obj/$b.o" ] || $FC $COMMON $defs -module-dir "$obj" -c "$f" -o "$obj/$b.o" 2> "$obj/$b.err"; then
+        n=$((n+1))
+      else
+        echo "$f" >> "$obj/next.txt"
+      fi
+    done < "$obj/todo.txt"
+    cp "$obj/next.txt" "$obj/todo.txt"
+    [ $n -eq 0 ] && break
+  done
+  $FC $COMMON $defs -module-dir "$obj" -I "$obj" -c "$HERE/fluxo_ref_harness.f90" -o "$obj/harness.o"
+  $FC -o "$OUT/fluxo_ref_$name" "$obj/harness.o" $(ls "$obj"/*.o | grep -v harness.o)
+  echo "built $OUT/fluxo_ref_$name; reference files not compiled:"
+  sed 's|^|    |' "$obj/todo.txt"
+}
+
+NS="-DPP_nVar=5 -Dnavierstokes -DPP_entropy_vars_exist -DPP_u_aux_exist"
+build_variant euler_vf32 $NS -DPP_VolFlux=32 -DPARABOLIC=0
+build_variant euler_vf5 $NS -DPP_VolFlux=5 -DPARABOLIC=0
+build_variant ns_br1_vf32 $NS -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0

@@ -1,0 +1,82 @@
+"""Driver for oracle/_ref/fluxo_ref_* (the reference's own code behind a harness).
+
+TEST INFRASTRUCTURE ONLY.  Runs in the container that holds /root/reference (the binaries
+are built there by oracle/ref/build_ref.sh); used to generate golden vectors under
+tests/golden/ and to pin the C oracle.  Array shapes follow oracle/pyoracle.py.
+"""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
+
+
+def binary(parabolic, volflux):
+    name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}_vf{volflux}"
+    path = os.path.join(REF_DIR, name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (oracle/ref/build_ref.sh)")
+    return path
+
+
+def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
+        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600):
+    """Metrics, VolInt, RHS (+ gradients), entropy, dt and an nsteps trajectory computed by
+    the reference's code on the given mesh (connectivity + NodeCoords) and state."""
+    N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
+    n = N + 1
+    exe = binary(parabolic, volflux)
+    with tempfile.TemporaryDirectory() as d:
+        par = os.path.join(d, "parameter.ini")
+        with open(par, "w") as f:
+            f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = 1\n"
+                    f"IniRefState = 1\nRefState = (/1.,0.,0.,0.,1./)\nkappa = {kappa!r}\n"
+                    f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\n")
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([N, Ngeo, E, S, nsteps, int(parabolic), 0, 0], np.int32).tofile(f)
+            np.array([cfl, dfl, t0, 0.0], np.float64).tofile(f)
+            np.ascontiguousarray(mesh["NodeCoords"], np.float64).tofile(f)
+            np.ascontiguousarray(mesh["ElemToSide"], np.int32).tofile(f)
+            np.ascontiguousarray(mesh["SideToElem"], np.int32).tofile(f)
+            np.ascontiguousarray(U, np.float64).tofile(f)
+        p = subprocess.run([exe, par, fin, fout], cwd=d, capture_output=True, text=True, errors="replace",
+                           timeout=timeout)
+        if p.returncode != 0 or not os.path.exists(fout):
+            raise RuntimeError(f"reference harness failed ({p.returncode}):\n{p.stdout[-2000:]}\n{p.stderr[-2000:]}")
+        raw = np.fromfile(fout, np.float64)
+    out, o = {}, 0
+
+    def take(name, shape):
+        nonlocal o
+        cnt = int(np.prod(shape))
+        out[name] = raw[o:o + cnt].reshape(shape)
+        o += cnt
+
+    for k in ("Mf", "Mg", "Mh"):
+        take(k, (E, n, n, n, 3))
+    take("sJ", (E, n, n, n))
+    for k in ("NormVec", "TangVec1", "TangVec2"):
+        take(k, (S, n, n, 3))
+    take("SurfElem", (S, n, n))
+    take("volint", (E, n, n, n, 5))
+    take("Ut", (E, n, n, n, 5))
+    if parabolic:
+        for k in ("gradPx", "gradPy", "gradPz"):
+            take(k, (E, n, n, n, 5))
+    take("dt0", (2,))
+    out["errType0"] = int(out["dt0"][1])
+    out["dt0"] = float(out["dt0"][0])
+    take("U_end", (E, n, n, n, 5))
+    take("tail", (2,))
+    out["t_end"], out["dt_last"] = float(out["tail"][0]), float(out["tail"][1])
+    del out["tail"]
+    m = re.search(r"Entropy\s*:\s*([-+0-9.Ee]+)", p.stdout)
+    out["Entropy"] = float(m.group(1)) if m else None
+    m = re.search(r"dSdU\*Ut\s*:\s*([-+0-9.Ee]+)", p.stdout)
+    out["dSdU_Ut"] = float(m.group(1)) if m else None
+    return out

@@ -1,0 +1,37 @@
+"""Loader for the golden vectors in tests/golden (made by the reference's own code with
+tests/golden/make_golden.py)."""
+import glob
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    for k in ("K", "N", "Ngeo", "warp", "volflux", "riemann", "parabolic", "nsteps", "errType0"):
+        d[k] = int(d[k])
+    for k in ("mu0", "Pr", "kappa", "cfl", "dfl", "dt0", "t_end", "Entropy", "dSdU_Ut"):
+        d[k] = float(d[k])
+    return d
+
+
+def mesh_args(g):
+    return dict(x0=tuple(g["x0"]), length=tuple(g["L"]), Ngeo=g["Ngeo"], warp=bool(g["warp"]))
+
+
+def initial_state(g, exactfunc, mesh):
+    """The stored U, or for the large noise-free case the ExactFunc state it was made from."""
+    if "U" in g:
+        return g["U"]
+    return exactfunc(2, mesh["Elem_xGP"])
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))

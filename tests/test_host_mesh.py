@@ -22,3 +22,14 @@ def test_host_mesh_matches_oracle(K, N, Ngeo, warp, x0, L):
     for k in ("Mf", "Mg", "Mh", "sJ", "NormVec", "TangVec1", "TangVec2", "SurfElem", "Elem_xGP"):
         scale = np.abs(o[k]).max()
         assert np.abs(h[k] - o[k]).max() <= 1e-14 * scale, k
+
+
+@pytest.mark.parametrize("name", [n for n in __import__("golden_cases").names()])
+def test_host_metrics_match_reference(name):
+    from golden_cases import load, mesh_args
+    g = load(name)
+    if "Mf" not in g:
+        pytest.skip("case stores no metrics")
+    h = box_mesh(g["K"], g["N"], **mesh_args(g))
+    for k in ("Mf", "sJ", "SurfElem"):
+        assert np.abs(h[k] - g[k]).max() <= 1e-14 * np.abs(g[k]).max(), k
