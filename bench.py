@@ -12,6 +12,10 @@ metric counts DOF-updates: one node (all 5 variables) advanced through one RK st
 The mesh and initial state are synthetic (generated here); U is resident in HBM before
 the timed region and stays there.
 
+The same workload is then run by the split-precision engine (FP32 flux differencing and
+accumulation, FP64 RK/CFL/surface/lifting): its throughput and the relative L2/Linf
+difference of its final state from the FP64 run are reported under "fp32_split".
+
 Prints one JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
 """
 import argparse
@@ -49,14 +53,16 @@ def tgv_state(xgp, kappa=1.4):
     return U
 
 
-def volume_kernel_bytes(N, nElems, parabolic):
+def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
     """Algorithmic HBM bytes of one launch of the volume kernel in its RK-stage form:
-    U read + write, Ut_tmp read + write, metrics (9), sJ (1), the face fluxes of the
-    element's 3 owned sides (each side's flux is read once per side), and for
-    Navier-Stokes the 12 Cartesian viscous flux components per node."""
+    U read + write, Ut_tmp read + write, metrics (9 per node, FP64 or the FP32 copies of
+    the split-precision path), sJ (1), the face fluxes of the element's 3 owned sides
+    (each side's flux is read once per side), and for Navier-Stokes the 12 Cartesian
+    viscous flux components per node."""
     n = N + 1
     n3, n2 = n ** 3, n ** 2
-    per_elem = 8 * (5 * n3 * 4 + 9 * n3 + n3 + 3 * n2 * 5 + (12 * n3 if parabolic else 0))
+    per_elem = (8 * (5 * n3 * 4 + n3 + 3 * n2 * 5 + (12 * n3 if parabolic else 0))
+                + metric_bytes * 9 * n3)
     return per_elem * nElems
 
 
@@ -125,7 +131,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--elems", type=int, default=32, help="elements per direction")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -135,39 +141,61 @@ def main():
         return 2
 
     from fluxo_amd.mesh import box_mesh
-    from fluxo_amd.native import Engine, FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT
+    from fluxo_amd.native import (Engine, FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING,
+                                  FX_KERNEL_DT, FX_PREC_FP64, FX_PREC_FP32_SPLIT)
 
     K, N = args.elems, 3
     mesh = box_mesh(K, N, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3, Ngeo=3, warp=True)
     U0 = tgv_state(mesh["Elem_xGP"])
-    eng = Engine(mesh, volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72)
-    eng.upload(U0)
     nodes = mesh["nElems"] * (N + 1) ** 3
 
-    if args.warmup > 0:
-        eng.run(args.warmup, 0.9, 0.9)
-    eng.sync()
-    t0 = time.perf_counter()
-    t_end, dt_last, err = eng.run(args.steps, 0.9, 0.9)
-    eng.sync()
-    el = time.perf_counter() - t0
-    if err != 0:
-        print(f"bench.py: CALCTIMESTEP reported errType={err}", file=sys.stderr)
-        return 3
-    value = nodes * 5 * args.steps / el
+    def leg(precision):
+        """Warmup + exactly `steps` timed RK steps (CALCTIMESTEP before each) of one engine;
+        returns the timing, the final state and per-kernel durations."""
+        eng = Engine(mesh, volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72,
+                     precision=precision)
+        eng.upload(U0)
+        if args.warmup > 0:
+            eng.run(args.warmup, 0.9, 0.9)
+        eng.sync()
+        t0 = time.perf_counter()
+        t_end, dt_last, err = eng.run(args.steps, 0.9, 0.9)
+        eng.sync()
+        el = time.perf_counter() - t0
+        if err != 0:
+            raise RuntimeError(f"CALCTIMESTEP reported errType={err}")
+        U_end = eng.download()
+        reps = 20
+        ms = {"volume": eng.profile_kernel(FX_KERNEL_VOLUME_RK, reps),
+              "surface": eng.profile_kernel(FX_KERNEL_SURFACE, reps),
+              "lifting": eng.profile_kernel(FX_KERNEL_LIFTING, reps),
+              "calc_dt": eng.profile_kernel(FX_KERNEL_DT, reps)}
+        eng.close()
+        return dict(el=el, t_end=t_end, dt_last=dt_last, U=U_end, ms=ms,
+                    value=nodes * 5 * args.steps / el)
+
+    r64 = leg(FX_PREC_FP64)
+    r32 = leg(FX_PREC_FP32_SPLIT)
+    value, el, ms = r64["value"], r64["el"], r64["ms"]
+    d = r32["U"] - r64["U"]
+    fp32 = {"value": r32["value"], "unit": "DOF-updates/s", "ms_per_step": r32["el"] / args.steps * 1e3,
+            "speedup_vs_fp64": r32["value"] / value,
+            "relL2_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(r64["U"])),
+            "relLinf_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(r64["U"]))),
+            "t_end": r32["t_end"],
+            "kernel_ms": {k: round(v, 4) for k, v in r32["ms"].items()},
+            "volume_kernel_GBps": round(volume_kernel_bytes(N, mesh["nElems"], True, 4)
+                                        / (r32["ms"]["volume"] * 1e-3) / 1e9, 1),
+            "note": "state after warmup+steps RK steps vs the FP64 run from the same initial state"}
 
     # dominant kernel: the volume kernel in its RK-stage form, timed with HIP events
-    reps = 20
-    ms = {"volume": eng.profile_kernel(FX_KERNEL_VOLUME_RK, reps),
-          "surface": eng.profile_kernel(FX_KERNEL_SURFACE, reps),
-          "lifting": eng.profile_kernel(FX_KERNEL_LIFTING, reps),
-          "calc_dt": eng.profile_kernel(FX_KERNEL_DT, reps)}
     vbytes = volume_kernel_bytes(N, mesh["nElems"], True)
     achieved = vbytes / (ms["volume"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "k_volume<N=3,Ranocha,BR1,RK>", "launch_ms": round(ms["volume"], 4),
             "bytes_per_launch": vbytes}
+    t_end, dt_last = r64["t_end"], r64["dt_last"]
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
@@ -181,6 +209,7 @@ def main():
                    "t_end": t_end, "dt_last": dt_last},
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         "pid_s_per_dof_timestep": 5.0 / value,
+        "fp32_split": fp32,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

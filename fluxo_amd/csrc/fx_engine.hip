@@ -107,6 +107,11 @@ void make_ops(int N, Ops &o)
     }
   }
   o.L0 = Lm[0] / w[0];
+  for (int i = 0; i <= MAXN; i++)
+    for (int j = 0; j <= MAXN; j++) {
+      o.Ds32[i][j] = (float)o.Ds[i][j];
+      o.Dhat32[i][j] = (float)o.Dhat[i][j];
+    }
 }
 
 bool g_ops_ready[MAXN + 1] = {false};
@@ -130,6 +135,7 @@ struct fx_ctx {
   hipStream_t stream = nullptr;
   double *U = nullptr, *Ut = nullptr, *Uttmp = nullptr;
   double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr;
+  float *Mf32 = nullptr, *Mg32 = nullptr, *Mh32 = nullptr;  // split precision only
   double *Flux = nullptr, *Fv = nullptr, *grad = nullptr, *part = nullptr, *red = nullptr;
   double *dtElem = nullptr;
   int *errElem = nullptr;
@@ -184,9 +190,14 @@ template <int N, int VF, bool PARA, int MODE>
 void launch_volume_t(fx_ctx *c, RKArgs rk)
 {
   const int nb = (c->cfg.nElems + VolGeom<N>::EPB - 1) / VolGeom<N>::EPB;
-  hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE>), dim3(nb), dim3(3 * VolGeom<N>::W), 0, c->stream,
-                     c->U, c->Mf, c->Mg, c->Mh, c->sJ, c->e2s, c->Flux, c->Fv, c->Ut, c->Uttmp, rk,
-                     c->cfg.nElems, c->cfg.kappa);
+  if (c->cfg.precision == FX_PREC_FP32_SPLIT)
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
+                       c->stream, c->U, c->Mf32, c->Mg32, c->Mh32, c->sJ, c->e2s, c->Flux, c->Fv, c->Ut,
+                       c->Uttmp, rk, c->cfg.nElems, c->cfg.kappa);
+  else
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
+                       c->stream, c->U, c->Mf, c->Mg, c->Mh, c->sJ, c->e2s, c->Flux, c->Fv, c->Ut,
+                       c->Uttmp, rk, c->cfg.nElems, c->cfg.kappa);
 }
 
 template <int N, int VF>
@@ -335,7 +346,8 @@ int check_config(const fx_config *c)
   if (c->riemann != 0 && c->riemann != 1 && c->riemann != 9 && c->riemann != 32)
     return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,9,32)", c->riemann);
   if (c->parabolic && c->lifting != 1) return set_err(FX_EINVAL, "only BR1 lifting (lifting=1)");
-  if (c->precision != FX_PREC_FP64) return set_err(FX_EINVAL, "precision %d not supported", c->precision);
+  if (c->precision != FX_PREC_FP64 && c->precision != FX_PREC_FP32_SPLIT)
+    return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", c->precision);
   if (!(c->kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");
   if (c->parabolic && !(c->Pr > 0.0)) return set_err(FX_EINVAL, "Pr must be > 0");
   return FX_OK;
@@ -355,6 +367,7 @@ long long fx_ctx_bytes(const fx_config *cfg)
   long long b = 0;
   b += 3LL * 5 * n3 * E * 8;         // U, Ut, Ut_tmp
   b += 9LL * n3 * E * 8 + n3 * E * 8; // metrics, sJ
+  if (cfg->precision == FX_PREC_FP32_SPLIT) b += 9LL * n3 * E * 4;  // FP32 metrics
   b += 5LL * n * n * S * 8;          // Flux
   b += 4LL * n * n * S * 8;          // nvse
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
@@ -395,6 +408,11 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   ALLOC(c->Mg, 3 * n3E * 8);
   ALLOC(c->Mh, 3 * n3E * 8);
   ALLOC(c->sJ, n3E * 8);
+  if (cfg->precision == FX_PREC_FP32_SPLIT) {
+    ALLOC(c->Mf32, 3 * n3E * 4);
+    ALLOC(c->Mg32, 3 * n3E * 4);
+    ALLOC(c->Mh32, 3 * n3E * 4);
+  }
   ALLOC(c->Flux, 5 * nf * 8);
   ALLOC(c->nvse, nf * sizeof(double4));
   ALLOC(c->e2s, (size_t)12 * cfg->nElems * 4);
@@ -422,7 +440,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  void *ptrs[] = {c->U, c->Ut, c->Uttmp, c->Mf, c->Mg, c->Mh, c->sJ, c->Flux, c->Fv, c->grad,
+  void *ptrs[] = {c->U, c->Ut, c->Uttmp, c->Mf, c->Mg, c->Mh, c->Mf32, c->Mg32, c->Mh32, c->sJ, c->Flux, c->Fv, c->grad,
                   c->part, c->red, c->nvse, c->e2s, c->s2e, c->dts, c->dtElem, c->errElem};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -464,6 +482,16 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   HIPCHK(hipMemcpyAsync(c->Mg, m->Metrics_gTilde, 3 * n3E * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->Mh, m->Metrics_hTilde, 3 * n3E * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->sJ, m->sJ, n3E * 8, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> m32;
+  if (c->cfg.precision == FX_PREC_FP32_SPLIT) {
+    m32.resize(9 * n3E);
+    const double *src[3] = {m->Metrics_fTilde, m->Metrics_gTilde, m->Metrics_hTilde};
+    float *dst[3] = {c->Mf32, c->Mg32, c->Mh32};
+    for (int d = 0; d < 3; d++) {
+      for (size_t i = 0; i < 3 * n3E; i++) m32[d * 3 * n3E + i] = (float)src[d][i];
+      HIPCHK(hipMemcpyAsync(dst[d], m32.data() + d * 3 * n3E, 3 * n3E * 4, hipMemcpyHostToDevice, c->stream));
+    }
+  }
   HIPCHK(hipMemcpyAsync(c->nvse, nvse.data(), nf * sizeof(double4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->e2s, m->ElemToSide, (size_t)12 * E * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->s2e, m->SideToElem, (size_t)5 * S * 4, hipMemcpyHostToDevice, c->stream));

@@ -27,6 +27,18 @@ struct Ops {
   double D[MAXN + 1][MAXN + 1];     // D(i,l)         (dg.f90:210)
   double w[MAXN + 1];               // LGL weights
   double L0;                        // L_HatMinus0 = L_HatMinus(0) (dg.f90:228-229)
+  float Ds32[MAXN + 1][MAXN + 1];   // FP32 copies for the split-precision volume integral
+  float Dhat32[MAXN + 1][MAXN + 1];
+};
+
+template <typename VR> struct OpsOf;
+template <> struct OpsOf<double> {
+  __device__ static __forceinline__ double ds(const Ops &o, int i, int l) { return o.Ds[i][l]; }
+  __device__ static __forceinline__ double dhat(const Ops &o, int i, int l) { return o.Dhat[i][l]; }
+};
+template <> struct OpsOf<float> {
+  __device__ static __forceinline__ float ds(const Ops &o, int i, int l) { return o.Ds32[i][l]; }
+  __device__ static __forceinline__ float dhat(const Ops &o, int i, int l) { return o.Dhat32[i][l]; }
 };
 extern __constant__ Ops c_ops[MAXN + 1];
 
@@ -123,16 +135,20 @@ struct RKArgs {
 //   phase 3  the three direction partial sums are added in LDS (reusing the field
 //            storage), then one thread per node adds the surface fluxes, applies -sJ and
 //            writes Ut or performs the RK stage update (coalesced by node).
+// VR is the arithmetic type of phases 1-3a (the flux differencing and its accumulation):
+// double for the FP64 path, float for the split-precision path, whose metrics are FP32
+// copies.  The per-node fields are formed in FP64 and rounded once; the surface gather,
+// the Jacobian and the RK update are FP64 in both.
 #ifndef FX_VOL_WAVES
 #define FX_VOL_WAVES 1
 #endif
 #ifndef FX_VOL_ND_LDS
 #define FX_VOL_ND_LDS 1
 #endif
-template <int N, int VF, bool PARA, int MODE>
+template <int N, int VF, bool PARA, int MODE, typename VR>
 __global__ void __launch_bounds__(3 * VolGeom<N>::W, FX_VOL_WAVES)
-k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
-         const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
+k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg,
+         const VR *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
          const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
          double *__restrict__ Uttmp, RKArgs rk, int nElems, double kappa)
 {
@@ -145,12 +161,12 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
 #define FX_VOL_EPAD 0
 #endif
   // per-element LDS blocks padded by FX_VOL_EPAD doubles to spread elements over banks
-  __shared__ double s_lds_raw[EPB][SLOTS * NP + FX_VOL_EPAD];  // node fields
-  auto s_lds = reinterpret_cast<double (*)[SLOTS * NP + FX_VOL_EPAD]>(s_lds_raw);
+  __shared__ VR s_lds_raw[EPB][SLOTS * NP + FX_VOL_EPAD];  // node fields
+  auto s_lds = reinterpret_cast<VR (*)[SLOTS * NP + FX_VOL_EPAD]>(s_lds_raw);
 #define SL(el_, slot_, l_) s_lds[el_][(slot_) * NP + (l_)]
 #if FX_VOL_ND_LDS
-  __shared__ double s_acc_raw[EPB][5 * NP + FX_VOL_EPAD];
-  auto s_acc = reinterpret_cast<double (*)[5 * NP + FX_VOL_EPAD]>(s_acc_raw);
+  __shared__ VR s_acc_raw[EPB][5 * NP + FX_VOL_EPAD];
+  auto s_acc = reinterpret_cast<VR (*)[5 * NP + FX_VOL_EPAD]>(s_acc_raw);
 #define SA(el_, v_, l_) s_acc[el_][(v_) * NP + (l_)]
 #else
 #define SA(el_, v_, l_) SL(el_, v_, l_)
@@ -159,7 +175,8 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
   const int e0 = blockIdx.x * EPB;
-  const double kappaM1 = kappa - 1.0, skappaM1 = 1.0 / kappaM1;
+  const double kappaM1 = kappa - 1.0;
+  const VR skappaM1 = VR(1.0 / kappaM1);
 
   // phase 1: per-node fields into LDS
   for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
@@ -173,7 +190,7 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
       const int l = Lds<N>::li(node);
 #pragma unroll
       for (int c = 0; c < NF_COUNT; c++)
-        if (uses_field<VF>(c)) SL(el, field_slot<VF>(c), l) = f[c];
+        if (uses_field<VF>(c)) SL(el, field_slot<VF>(c), l) = VR(f[c]);
     }
   }
   __syncthreads();
@@ -192,45 +209,45 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
     lds[l] = Lds<N>::li(i, j, k);
   }
 #if !FX_VOL_ND_LDS
-  double nd[n][NF_COUNT];
+  VR nd[n][NF_COUNT];
 #pragma unroll
   for (int l = 0; l < n; l++)
 #pragma unroll
     for (int c = 0; c < NF_COUNT; c++)
-      nd[l][c] = (active && uses_field<VF>(c)) ? SL(active ? el : 0, field_slot<VF>(c), lds[l]) : 0.0;
+      nd[l][c] = (active && uses_field<VF>(c)) ? SL(active ? el : 0, field_slot<VF>(c), lds[l]) : VR(0);
   __syncthreads();  // field storage is free from here on
 #endif
 
-  double acc[n][5];
+  VR acc[n][5];
 #pragma unroll
   for (int i = 0; i < n; i++)
 #pragma unroll
-    for (int v = 0; v < 5; v++) acc[i][v] = 0.0;
+    for (int v = 0; v < 5; v++) acc[i][v] = VR(0);
   if (active) {
-    const double *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
-    double m[n][3];
+    const VR *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
+    VR m[n][3];
 #pragma unroll
     for (int l = 0; l < n; l++) {
-      const double *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
+      const VR *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
       m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
     }
 #pragma unroll
     for (int i = 0; i < n; i++)
 #pragma unroll
       for (int l = i + 1; l < n; l++) {
-        double F[5];
+        VR F[5];
 #if FX_VOL_ND_LDS
-        double ndi[NF_COUNT], ndl[NF_COUNT];
+        VR ndi[NF_COUNT], ndl[NF_COUNT];
 #pragma unroll
         for (int c = 0; c < NF_COUNT; c++) {
-          ndi[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[i]) : 0.0;
-          ndl[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[l]) : 0.0;
+          ndi[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[i]) : VR(0);
+          ndl[c] = uses_field<VF>(c) ? SL(el, field_slot<VF>(c), lds[l]) : VR(0);
         }
-        volume_flux<VF, double>(ndi, ndl, m[i], m[l], skappaM1, F);
+        volume_flux<VF, VR>(ndi, ndl, m[i], m[l], skappaM1, F);
 #else
-        volume_flux<VF, double>(nd[i], nd[l], m[i], m[l], skappaM1, F);
+        volume_flux<VF, VR>(nd[i], nd[l], m[i], m[l], skappaM1, F);
 #endif
-        const double dil = op.Ds[i][l], dli = op.Ds[l][i];
+        const VR dil = OpsOf<VR>::ds(op, i, l), dli = OpsOf<VR>::ds(op, l, i);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
           acc[i][v] += dil * F[v];
@@ -239,18 +256,19 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
       }
     if (PARA && MODE != VMODE_VOLINT) {
       // transformed viscous flux along the line: ft(l) = f*M1 + g*M2 + h*M3
-      double ft[n][4];
+      VR ft[n][4];
 #pragma unroll
       for (int l = 0; l < n; l++) {
         const double *fv = Fv + ((size_t)e * n3 + nodes[l]) * 12;
 #pragma unroll
-        for (int c = 0; c < 4; c++) ft[l][c] = fv[c] * m[l][0] + fv[4 + c] * m[l][1] + fv[8 + c] * m[l][2];
+        for (int c = 0; c < 4; c++)
+          ft[l][c] = VR(fv[c]) * m[l][0] + VR(fv[4 + c]) * m[l][1] + VR(fv[8 + c]) * m[l][2];
       }
 #pragma unroll
       for (int i = 0; i < n; i++)
 #pragma unroll
         for (int l = 0; l < n; l++) {
-          const double dh = op.Dhat[i][l];
+          const VR dh = OpsOf<VR>::dhat(op, i, l);
 #pragma unroll
           for (int c = 0; c < 4; c++) acc[i][c + 1] += dh * ft[l][c];
         }
@@ -279,7 +297,7 @@ k_volume(double *__restrict__ U, const double *__restrict__ Mf, const double *__
     const int l = Lds<N>::li(node);
     double tot[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) tot[v] = SA(el2, v, l);
+    for (int v = 0; v < 5; v++) tot[v] = double(SA(el2, v, l));
     if (MODE != VMODE_VOLINT) {
       // SurfInt gather: faces touched by node (i,j,k), in local side order
       int locs[3], nl = 0;

@@ -61,6 +61,34 @@ __device__ __forceinline__ R inv_ln_mean(R a, R b, R la, R lb)
   return den * rcp_nr(num);
 }
 
+// FP32 log mean of the split-precision volume integral.  The near-equal branch is kept in
+// FP64: with per-node logs rounded to FP32, (b-a)/(ln b - ln a) loses ~|ln a|/|ln b - ln a|
+// ulps, so the series of LN_MEAN is used up to u < 4e-2 (|b-a|/(b+a) < 0.2) instead of
+// 1e-4; its truncation error u^4/9 stays below 3e-7 there.  u itself is formed in FP32
+// (its rounding moves the series value by < 1e-9); the series and its quotient are FP64.
+constexpr float LNMEAN_EPS32 = 4.0e-2f;
+
+template <bool INV>
+__device__ __forceinline__ float ln_mean32(float a, float b, float la, float lb)
+{
+  const float d = b - a, s = a + b, d2 = d * d, s2 = s * s;
+  const bool series = d2 < LNMEAN_EPS32 * s2;
+  const double u = (double)(d2 * rcp_nr(s2));
+  const double P = 105.0 + u * (35.0 + u * (21.0 + u * 15.0));
+  const double s52 = (double)a * 52.5 + (double)b * 52.5;
+  const double ser = INV ? P * rcp_nr(s52) : s52 * rcp_nr(P);
+  const float q = INV ? (lb - la) * rcp_nr(d) : d * rcp_nr(lb - la);
+  return series ? (float)ser : q;
+}
+__device__ __forceinline__ float ln_mean(float a, float b, float la, float lb)
+{
+  return ln_mean32<false>(a, b, la, lb);
+}
+__device__ __forceinline__ float inv_ln_mean(float a, float b, float la, float lb)
+{
+  return ln_mean32<true>(a, b, la, lb);
+}
+
 // Per-node quantities staged for the two-point fluxes.
 //   rho, v1, v2, v3, p, ln(rho), X, ln(X), |v|^2, rhoE
 // X = rho/p for Ranocha (RanochaFluxVec), X = beta = rho/(2p) for Chandrashekar
