@@ -16,6 +16,12 @@ The same workload is then run by the split-precision engine (FP32 flux differenc
 accumulation, FP64 RK/CFL/surface/lifting): its throughput and the relative L2/Linf
 difference of its final state from the FP64 run are reported under "fp32_split".
 
+Serving: a stream of independent requests (fluxo_amd.workloads, 512 by default, 50 steps
+each) goes through the batched server, first all at once (offline throughput), then as a
+Poisson stream at 0.7x that request rate (p50/p99 latency from submission to result on the
+host), in FP64 and FP32-split; every 8th request also runs alone through its own context
+(the plain per-request path) for the batched/plain throughput ratio.
+
 Prints one JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
 """
 import argparse
@@ -64,6 +70,102 @@ def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
     per_elem = (8 * (5 * n3 * 4 + n3 + 3 * n2 * 5 + (12 * n3 if parabolic else 0))
                 + metric_bytes * 9 * n3)
     return per_elem * nElems
+
+
+# CFLscale/DFLscale of the serving stream: at 0.9 the N=5 density wave (Ranocha + LLF,
+# Carpenter RK4-5) loses positivity after ~33 steps, in the CPU oracle as on the GPU.
+SERVE_CFL = 0.5
+
+
+def serving_stream(n_req, nsteps, seed):
+    """The mixed request stream (fluxo_amd.workloads): N in {2..5}, K^3 periodic meshes with K
+    log-uniform in [4, 24], density wave / inviscid TGV / Navier-Stokes TGV."""
+    from fluxo_amd import workloads as W
+    cache = W.MeshCache()
+    return [W.make_request(seed, s, cache, nsteps) for s in range(n_req)]
+
+
+def run_server(reqs, precision, arrivals=None):
+    """Submit every request (at the given arrival offsets in seconds, else all at once) to one
+    server; returns makespan, per-request latencies (s) and the DOF-updates done."""
+    from fluxo_amd.native import Server, FX_REQ_DONE
+    srv = Server(precision=precision, mu0=1.0 / 1600, Pr=0.72, cfl=SERVE_CFL, dfl=SERVE_CFL,
+                 hbm_budget=96 << 30, max_requests=len(reqs) + 1)
+    ids = []
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        if arrivals is not None:
+            wait = arrivals[i] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        ids.append(srv.submit(r["mesh"], r["U0"], r["nsteps"], parabolic=r["parabolic"])[0])
+    srv.wait(-1, timeout=1200)
+    st = [srv.status(i) for i in ids]
+    stats = srv.stats()
+    err = srv.last_error()
+    srv.close()
+    bad = [(i, s) for i, s in enumerate(st) if s["state"] != FX_REQ_DONE]
+    if bad:
+        for i, s in bad[:10]:
+            r = reqs[i]
+            print(f"serving: request {i} (N={r['N']} K={r['K']} {r['ic']} parabolic={r['parabolic']}) "
+                  f"failed: {s}", file=sys.stderr)
+        raise RuntimeError(f"serving: {len(bad)} requests failed; worker error: {err!r}; stats {stats}")
+    start = min(s["t_submit"] for s in st)
+    makespan = max(s["t_done"] for s in st) - start
+    lat = np.array([s["t_done"] - s["t_submit"] for s in st])
+    dofs = sum(r["nodes"] * 5 * r["nsteps"] for r in reqs)
+    return dict(makespan=makespan, lat=lat, dofs=dofs, stats=stats)
+
+
+def run_plain(reqs, precision):
+    """The same requests one at a time, each through its own context (create, upload, run,
+    download, destroy): the plain per-request GPU path."""
+    from fluxo_amd.native import Engine
+    t0 = time.perf_counter()
+    for r in reqs:
+        e = Engine(r["mesh"], volume_flux=32, riemann=1, parabolic=r["parabolic"], mu0=1.0 / 1600,
+                   Pr=0.72, precision=precision)
+        e.upload(r["U0"])
+        e.run(r["nsteps"], SERVE_CFL, SERVE_CFL)
+        e.download()
+        e.close()
+    el = time.perf_counter() - t0
+    return sum(r["nodes"] * 5 * r["nsteps"] for r in reqs) / el
+
+
+def serving_leg(n_req, nsteps, seed, load, plain_every):
+    """Offline batch throughput, then a Poisson stream at `load` x that request rate with
+    p50/p99 latency, for FP64 and FP32-split; the plain per-request path on every
+    `plain_every`-th request."""
+    from fluxo_amd.native import FX_PREC_FP64, FX_PREC_FP32_SPLIT
+    reqs = serving_stream(n_req, nsteps, seed)
+    out = {"requests": n_req, "steps_per_request": nsteps, "seed": seed,
+           "mix": "N uniform {2,3,4,5}; K^3 periodic Cartesian, K log-uniform [4,24]; density wave "
+                  "(Euler) / TGV inviscid / TGV Navier-Stokes BR1; Ranocha + LLF; CFL/DFL 0.5"}
+    rng = np.random.default_rng(seed + 1)
+    for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
+        off = run_server(reqs, prec)
+        rate = load * n_req / off["makespan"]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        arrivals -= arrivals[0]
+        live = run_server(reqs, prec, arrivals)
+        out[name] = {
+            "offline_DOFps": off["dofs"] / off["makespan"],
+            "offline_makespan_s": off["makespan"],
+            "poisson_rate_per_s": rate,
+            "p50_ms": float(np.percentile(live["lat"], 50) * 1e3),
+            "p99_ms": float(np.percentile(live["lat"], 99) * 1e3),
+            "stream_DOFps": live["dofs"] / live["makespan"],
+            "graph_captures": live["stats"]["graph_captures"],
+            "peak_request_bytes": live["stats"]["bytes_peak"],
+        }
+    sub = reqs[::plain_every]
+    plain = run_plain(sub, FX_PREC_FP64)
+    out["plain_per_request_DOFps"] = plain
+    out["plain_sample"] = f"every {plain_every}th request ({len(sub)} requests), FP64"
+    out["batched_vs_plain"] = out["fp64"]["offline_DOFps"] / plain
+    return out
 
 
 def _sample_mesh_and_state(K):
@@ -134,6 +236,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-steps", type=int, default=50)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the offline request throughput")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU element partitioning is a design "
@@ -196,6 +303,8 @@ def main():
             "kernel": "k_volume<N=3,Ranocha,BR1,RK>", "launch_ms": round(ms["volume"], 4),
             "bytes_per_launch": vbytes}
     t_end, dt_last = r64["t_end"], r64["dt_last"]
+    serving = None if args.no_serving else serving_leg(args.serve_requests, args.serve_steps, 2026,
+                                                          args.serve_load, 8)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
@@ -210,6 +319,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         "pid_s_per_dof_timestep": 5.0 / value,
         "fp32_split": fp32,
+        "serving": serving,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

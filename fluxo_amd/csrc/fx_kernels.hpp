@@ -119,10 +119,26 @@ template <int N> struct Lds {
 
 enum { VMODE_RHS = 0, VMODE_RK = 1, VMODE_VOLINT = 2 };
 
+// Time-step state of one problem (device resident).
+struct DtState {
+  int errType;                 // max error code seen
+  int errStep;                 // first step with an error (-1 none)
+  double dt;                   // dt of the current step
+  double t;                    // simulation time
+  int step;                    // steps done
+  int pad;
+};
+
+// Element -> problem map of a batch of independent meshes sharing one launch: eprob[e] is
+// the problem (index into the per-problem time-step states) of element e, or -1 for an
+// unused slot, which every kernel skips.  nullptr means one problem owning all elements.
+__device__ __forceinline__ int elem_problem(const int *eprob, int e) { return eprob ? eprob[e] : 0; }
+
 struct RKArgs {
-  const double *dt;  // device scalar (time step of the current step)
-  double A, b;       // RKA(stage) (0 for stage 1) and RKb(stage)
-  int first;         // 1 on stage 1 (Ut_tmp = Ut)
+  const DtState *st;  // per-problem time-step states (dt of the current step)
+  const int *eprob;   // element -> problem (nullptr: one problem)
+  double A, b;        // RKA(stage) (0 for stage 1) and RKb(stage)
+  int first;          // 1 on stage 1 (Ut_tmp = Ut)
 };
 
 // Element-centric volume kernel (VolInt_SplitForm volint.f90:173-218 + VolInt_visc
@@ -145,8 +161,11 @@ struct RKArgs {
 #ifndef FX_VOL_ND_LDS
 #define FX_VOL_ND_LDS 1
 #endif
+#ifndef FX_VOL_WAVES32
+#define FX_VOL_WAVES32 1
+#endif
 template <int N, int VF, bool PARA, int MODE, typename VR>
-__global__ void __launch_bounds__(3 * VolGeom<N>::W, FX_VOL_WAVES)
+__global__ void __launch_bounds__(3 * VolGeom<N>::W, sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES)
 k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg,
          const VR *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
          const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
@@ -182,7 +201,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
     const int el = idx / n3, node = idx - el * n3;
     const int e = e0 + el;
-    if (e < nElems) {
+    if (e < nElems && elem_problem(rk.eprob, e) >= 0) {
       const double *u = U + ((size_t)e * n3 + node) * 5;
       double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
       double f[NF_COUNT];
@@ -199,7 +218,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   const int dir = tid / W, lt = tid - dir * W;
   const int el = lt / n2, line = lt - el * n2;
   const int e = e0 + el;
-  const bool active = (el < EPB) && (e < nElems);
+  const bool active = (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
   const int a = line % n, b = line / n;
   int nodes[n], lds[n];
 #pragma unroll
@@ -293,6 +312,8 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     const int el2 = idx / n3, node = idx - el2 * n3;
     const int e2 = e0 + el2;
     if (e2 >= nElems) continue;
+    const int prob = elem_problem(rk.eprob, e2);
+    if (prob < 0) continue;
     const int i = node % n, j = (node / n) % n, k = node / n2;
     const int l = Lds<N>::li(node);
     double tot[5];
@@ -327,7 +348,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     }
     const size_t o = ((size_t)e2 * n3 + node) * 5;
     if (MODE == VMODE_RK) {
-      const double bdt = rk.b * (*rk.dt);
+      const double bdt = rk.b * rk.st[prob].dt;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         const double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
@@ -357,6 +378,7 @@ k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int
   if (gid >= nSides * n2) return;
   const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
   const int em = s2e[5 * s] - 1, es = s2e[5 * s + 1] - 1;
+  if (em < 0) return;  // unused side slot of a batch (SideToElem zeroed)
   const int lm = s2e[5 * s + 2], ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
   const int vm = side_to_vol<N>(lm, 0, p, q), vs = side_to_vol<N>(ls, fl, p, q);
   const double *um = U + ((size_t)em * n3 + vm) * 5, *us = U + ((size_t)es * n3 + vs) * 5;
@@ -392,8 +414,8 @@ __global__ void __launch_bounds__(256)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
-          double *__restrict__ out1, double *__restrict__ out2, int nElems, double kappa,
-          double mu0, double KappasPr)
+          double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
+          int nElems, double kappa, double mu0, double KappasPr)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   constexpr int EPB = LiftGeom<N>::EPB;
@@ -402,7 +424,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   const int tid = threadIdx.x;
   const int el = tid / n3, node = tid - el * n3;
   const int e = blockIdx.x * EPB + el;
-  const bool active = (el < EPB) && (e < nElems);
+  const bool active = (el < EPB) && (e < nElems) && elem_problem(eprob, e) >= 0;
   double u[5], w[5];
   if (active) {
     const double *up = U + ((size_t)e * n3 + node) * 5;
@@ -496,14 +518,6 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 // CALCTIMESTEP (navierstokes/calctimestep.f90:39-170): per-element convective and
 // viscous eigenvalue bounds into dtElem (as the reference's dtElem), then a one-block
 // minimum in fixed order (deterministic, no atomics).
-struct DtState {
-  int errType;                 // max error code seen
-  int errStep;                 // first step with an error (-1 none)
-  double dt;                   // dt of the current step
-  double t;                    // simulation time
-  int step;                    // steps done
-  int pad;
-};
 
 // one group of G = 2^ceil(log2 n^3) threads per element, tree max-reduction in LDS
 template <int N> struct DtGeom {
@@ -517,8 +531,8 @@ template <int N>
 __global__ void __launch_bounds__(DtGeom<N>::T)
 k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, double *__restrict__ dtElem,
-          int *__restrict__ errElem, int nElems, double kappa, double cfl, double dfl, double mu0,
-          double KsPr_max, int parabolic)
+          int *__restrict__ errElem, const int *__restrict__ eprob, int nElems, double kappa,
+          double cfl, double dfl, double mu0, double KsPr_max, int parabolic)
 {
   constexpr int n3 = DtGeom<N>::n3, G = DtGeom<N>::G, EPB = DtGeom<N>::EPB, T = DtGeom<N>::T;
   __shared__ double s_l[6][T];
@@ -526,9 +540,10 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   const int tid = threadIdx.x;
   const int el = tid / G, node = tid - el * G;
   const int e = blockIdx.x * EPB + el;
+  const bool used = e < nElems && elem_problem(eprob, e) >= 0;
   double l[6] = {1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12, 1.0e-12};
   int err = 0;
-  if (node < n3 && e < nElems) {
+  if (node < n3 && used) {
     const size_t c = (size_t)e * n3 + node;
     const double *u = U + 5 * c;
     if (isnan(u[0])) err = 1;
@@ -563,7 +578,7 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     }
     __syncthreads();
   }
-  if (node == 0 && e < nElems) {
+  if (node == 0 && used) {
     int er = s_err[tid];
     double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
     if (isnan(dte)) er = max(er, 3);
@@ -607,6 +622,45 @@ k_dt_reduce(const double *__restrict__ dtElem, const int *__restrict__ errElem, 
     st->errType = max(st->errType, s_er[0]);
     if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
     if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+  }
+}
+
+// Per-problem CALCTIMESTEP minimum for a batch: block r reduces the contiguous element
+// range [first[r], first[r] + count[r]) of problem r into st[r] (count 0: unused slot).
+// The minimum is exact in any summation order, so a problem's dt in a batch equals its dt
+// computed alone (k_dt_reduce).
+__global__ void __launch_bounds__(256)
+k_dt_reduce_seg(const double *__restrict__ dtElem, const int *__restrict__ errElem,
+                const int *__restrict__ first, const int *__restrict__ count, DtState *st, int advance)
+{
+  __shared__ double s_dt[256];
+  __shared__ int s_er[256];
+  const int r = blockIdx.x, tid = threadIdx.x;
+  const int e0 = first[r], ne = count[r];
+  if (ne <= 0) return;
+  double m = 1.79e308;
+  int er = 0;
+  for (int e = e0 + tid; e < e0 + ne; e += 256) {
+    const double d = dtElem[e];
+    if (d < m) m = d;
+    er = max(er, errElem[e]);
+  }
+  s_dt[tid] = m;
+  s_er[tid] = er;
+  __syncthreads();
+  for (int stp = 128; stp > 0; stp >>= 1) {
+    if (tid < stp) {
+      s_dt[tid] = fmin(s_dt[tid], s_dt[tid + stp]);
+      s_er[tid] = max(s_er[tid], s_er[tid + stp]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    DtState &S = st[r];
+    S.dt = s_dt[0];
+    S.errType = max(S.errType, s_er[0]);
+    if (S.errType != 0 && S.errStep < 0) S.errStep = S.step;
+    if (advance) { S.t = S.t + S.dt; S.step = S.step + 1; }
   }
 }
 

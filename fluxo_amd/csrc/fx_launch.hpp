@@ -1,0 +1,385 @@
+// Host-side launch layer shared by the single-problem context and the batched server:
+// basis operators in constant memory, error reporting, and the per-stage kernel sequence
+// over a "Batch" (device arrays of one or more independent meshes sharing launches).
+//
+// Compiled as part of fx_engine.hip (one device-code unit, so every kernel sees the one
+// c_ops table).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/fluxo_gpu.h"
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+// ---- errors ---------------------------------------------------------------------------
+inline char *err_buf()
+{
+  static thread_local char buf[512] = "";
+  return buf;
+}
+
+inline int set_err(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(err_buf(), 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t _e = (x);                                                                       \
+    if (_e != hipSuccess)                                                                      \
+      return ::fx::set_err(FX_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, \
+                           __LINE__);                                                          \
+  } while (0)
+
+// ---- basis ----------------------------------------------------------------------------
+// Host-side LGL basis (the engine's InitDGbasis, dg.f90:181-252, with the node/weight
+// Newton iteration of LegGaussLobNodesAndWeights basis.f90:544-608 and the barycentric
+// derivative matrix of PolynomialDerivativeMatrix basis.f90:641-663).
+inline void lgl_nodes(int N, double *x, double *w)
+{
+  const double pi = std::acos(-1.0);
+  x[0] = -1.0; x[N] = 1.0;
+  w[0] = 2.0 / (double)(N * (N + 1)); w[N] = w[0];
+  auto qL = [N](double xx, double &q, double &qd, double &L) {
+    double Lm2 = 1.0, Lm1 = xx, Ld, Ldm2 = 0.0, Ldm1 = 1.0;
+    L = 0.0;
+    for (int i = 2; i <= N; i++) {
+      L = ((double)(2 * i - 1) * xx * Lm1 - (double)(i - 1) * Lm2) / (double)i;
+      Ld = Ldm2 + (double)(2 * i - 1) * Lm1;
+      Lm2 = Lm1; Lm1 = L; Ldm2 = Ldm1; Ldm1 = Ld;
+    }
+    q = (double)(2 * N + 1) / (double)(N + 1) * (xx * L - Lm2);
+    qd = (double)(2 * N + 1) * L;
+  };
+  if (N > 1) {
+    double c1 = pi / (double)N, c2 = 3.0 / ((double)(8 * N) * pi);
+    for (int i = 1; i <= (N + 1) / 2 - 1; i++) {
+      double xi = -std::cos(c1 * ((double)i + 0.25) - c2 / ((double)i + 0.25)), q, qd, L;
+      for (int it = 0; it <= 10; it++) {
+        qL(xi, q, qd, L);
+        double dx = -q / qd;
+        xi += dx;
+        if (std::fabs(dx) < 1e-15 * std::fabs(xi)) break;
+      }
+      qL(xi, q, qd, L);
+      x[i] = xi; x[N - i] = -xi;
+      w[i] = w[0] / (L * L); w[N - i] = w[i];
+    }
+  }
+  if (N % 2 == 0) {
+    double q, qd, L;
+    x[N / 2] = 0.0;
+    qL(0.0, q, qd, L);
+    w[N / 2] = w[0] / (L * L);
+  }
+}
+
+inline void make_ops(int N, Ops &o)
+{
+  std::memset(&o, 0, sizeof(o));
+  const int n = N + 1;
+  double x[MAXN + 1], w[MAXN + 1], wb[MAXN + 1];
+  lgl_nodes(N, x, w);
+  for (int i = 0; i < n; i++) wb[i] = 1.0;
+  for (int i = 1; i <= N; i++)
+    for (int j = 0; j < i; j++) { wb[j] *= (x[j] - x[i]); wb[i] *= (x[i] - x[j]); }
+  for (int i = 0; i < n; i++) wb[i] = 1.0 / wb[i];
+  for (int l = 0; l < n; l++)
+    for (int i = 0; i < n; i++)
+      if (l != i) {
+        o.D[i][l] = wb[l] / (wb[i] * (x[i] - x[l]));
+        o.D[i][i] -= o.D[i][l];
+      }
+  // Lagrange polynomials at -1, +1 are unit vectors on LGL nodes
+  double Lm[MAXN + 1] = {0}, Lp[MAXN + 1] = {0};
+  Lm[0] = 1.0; Lp[N] = 1.0;
+  for (int i = 0; i < n; i++) {
+    o.w[i] = w[i];
+    for (int j = 0; j < n; j++) {
+      o.Dhat[i][j] = -((1.0 / w[i]) * (o.D[j][i] * w[j]));
+      double vbv = -Lm[i] * Lm[j] + Lp[i] * Lp[j];
+      o.Ds[i][j] = 2.0 * o.D[i][j] - (1.0 / w[i]) * vbv;
+    }
+  }
+  o.L0 = Lm[0] / w[0];
+  for (int i = 0; i <= MAXN; i++)
+    for (int j = 0; j <= MAXN; j++) {
+      o.Ds32[i][j] = (float)o.Ds[i][j];
+      o.Dhat32[i][j] = (float)o.Dhat[i][j];
+    }
+}
+
+// Operators of degree N into constant memory of the current device (once per process).
+inline int ensure_ops(int N)
+{
+  static bool ready[MAXN + 1] = {false};
+  if (ready[N]) return FX_OK;
+  Ops o;
+  make_ops(N, o);
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_ops), &o, sizeof(Ops), sizeof(Ops) * N, hipMemcpyHostToDevice));
+  ready[N] = true;
+  return FX_OK;
+}
+
+// ---- time integration constants ---------------------------------------------------------
+inline void rk_coefs(double *A, double *b, double *c)
+{
+  // Carpenter & Kennedy RK4(5) 2N-storage (timedisc_vars.f90:219-233)
+  A[0] = 0.0;
+  A[1] = 567301805773.0 / 1357537059087.0;
+  A[2] = 2404267990393.0 / 2016746695238.0;
+  A[3] = 3550918686646.0 / 2091501179385.0;
+  A[4] = 1275806237668.0 / 842570457699.0;
+  b[0] = 1432997174477.0 / 9575080441755.0;
+  b[1] = 5161836677717.0 / 13612068292357.0;
+  b[2] = 1720146321549.0 / 2090206949498.0;
+  b[3] = 3134564353537.0 / 4481467310338.0;
+  b[4] = 2277821191437.0 / 14882151754819.0;
+  c[0] = 0.0;
+  c[1] = 1432997174477.0 / 9575080441755.0;
+  c[2] = 2526269341429.0 / 6820363962896.0;
+  c[3] = 2006345519317.0 / 3224310063776.0;
+  c[4] = 2802321613138.0 / 2924317926251.0;
+}
+
+// CFLScaleAlpha / DFLScaleAlpha for LGL and Carpenter RK4-5 (timedisc_vars.f90:79-84, 212-216)
+inline double cfl_alpha(int N)
+{
+  static const double a[15] = {4.7497, 3.4144, 2.8451, 2.4739, 2.2027, 1.9912, 1.8225, 1.6830,
+                               1.5682, 1.4692, 1.3849, 1.3106, 1.2454, 1.1880, 1.1362};
+  return a[(N < 15 ? N : 15) - 1];
+}
+inline double dfl_alpha(int N)
+{
+  static const double a[10] = {4.50, 1.95, 1.18, 0.79, 0.56, 0.42, 0.32, 0.25, 0.20, 0.17};
+  return a[(N < 10 ? N : 10) - 1] * 1.853;
+}
+// FillCFL_DFL (timedisc.f90:697-741): the scaled CFL and DFL numbers of CALCTIMESTEP
+inline double cfl_number(int N, double CFLscale) { return CFLscale * cfl_alpha(N) / (2.0 * N + 1.0); }
+inline double dfl_number(int N, double DFLscale)
+{
+  return DFLscale * dfl_alpha(N) / ((2.0 * N + 1.0) * (2.0 * N + 1.0));
+}
+
+// ---- batch ----------------------------------------------------------------------------
+// Device arrays of one launch domain.  cfg.nElems / cfg.nSides are the launch extents;
+// with eprob == nullptr all elements belong to problem 0 (st[0]), otherwise eprob maps
+// elements to problems (-1: unused slot) and rfirst/rcount give each problem's
+// contiguous element range for the per-problem time-step minimum.
+struct Batch {
+  fx_config cfg{};
+  int n = 0, n2 = 0, n3 = 0;
+  hipStream_t stream = nullptr;
+  double *U = nullptr, *Ut = nullptr, *Uttmp = nullptr;
+  double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr;
+  float *Mf32 = nullptr, *Mg32 = nullptr, *Mh32 = nullptr;  // split precision only
+  double *Flux = nullptr, *Fv = nullptr, *grad = nullptr;
+  double4 *nvse = nullptr;
+  int *e2s = nullptr, *s2e = nullptr;
+  int *eprob = nullptr;
+  double *dtElem = nullptr;
+  int *errElem = nullptr;
+  DtState *st = nullptr;
+  int *rfirst = nullptr, *rcount = nullptr;
+  int nProb = 1;
+  double rkA[5], rkb[5], rkc[5];
+
+  void set_degree(int N)
+  {
+    cfg.N = N;
+    n = N + 1; n2 = n * n; n3 = n2 * n;
+    rk_coefs(rkA, rkb, rkc);
+  }
+};
+
+template <int N, int VF, bool PARA, int MODE>
+void launch_volume_t(const Batch &b, RKArgs rk)
+{
+  const int nb = (b.cfg.nElems + VolGeom<N>::EPB - 1) / VolGeom<N>::EPB;
+  if (nb <= 0) return;
+  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
+                       b.stream, b.U, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp,
+                       rk, b.cfg.nElems, b.cfg.kappa);
+  else
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
+                       b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk,
+                       b.cfg.nElems, b.cfg.kappa);
+}
+
+template <int N, int VF>
+void launch_volume_v(const Batch &b, int mode, RKArgs rk)
+{
+  const bool para = b.cfg.parabolic != 0;
+  if (mode == VMODE_VOLINT) launch_volume_t<N, VF, false, VMODE_VOLINT>(b, rk);
+  else if (mode == VMODE_RK) {
+    if (para) launch_volume_t<N, VF, true, VMODE_RK>(b, rk);
+    else launch_volume_t<N, VF, false, VMODE_RK>(b, rk);
+  } else {
+    if (para) launch_volume_t<N, VF, true, VMODE_RHS>(b, rk);
+    else launch_volume_t<N, VF, false, VMODE_RHS>(b, rk);
+  }
+}
+
+template <int N>
+void launch_volume_n(const Batch &b, int mode, RKArgs rk)
+{
+  switch (b.cfg.volumeFlux) {
+  case 5: launch_volume_v<N, 5>(b, mode, rk); break;
+  case 32: launch_volume_v<N, 32>(b, mode, rk); break;
+  default: launch_volume_v<N, 0>(b, mode, rk); break;
+  }
+}
+
+template <int N, int RS>
+void launch_surface_t(const Batch &b)
+{
+  const long long tot = (long long)b.cfg.nSides * b.n2;
+  const int nb = (int)((tot + 255) / 256);
+  if (nb <= 0) return;
+  if (b.cfg.parabolic)
+    hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e,
+                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
+  else
+    hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e,
+                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
+}
+
+template <int N>
+void launch_surface_n(const Batch &b)
+{
+  switch (b.cfg.riemann) {
+  case 0: launch_surface_t<N, 0>(b); break;
+  case 9: launch_surface_t<N, 9>(b); break;
+  case 32: launch_surface_t<N, 32>(b); break;
+  default: launch_surface_t<N, 1>(b); break;
+  }
+}
+
+template <int N>
+void launch_lifting_n(const Batch &b, int out)
+{
+  const int epb = LiftGeom<N>::EPB;
+  const int nb = (b.cfg.nElems + epb - 1) / epb;
+  if (nb <= 0) return;
+  const double KsPr = b.cfg.kappa / b.cfg.Pr;
+  const size_t nv = (size_t)5 * b.n3 * b.cfg.nElems;
+  if (out == 0)
+    hipLaunchKernelGGL((k_lifting<N, 0>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems,
+                       b.cfg.kappa, b.cfg.mu0, KsPr);
+  else
+    hipLaunchKernelGGL((k_lifting<N, 1>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.Fv, nullptr, nullptr, b.eprob, b.cfg.nElems, b.cfg.kappa,
+                       b.cfg.mu0, KsPr);
+}
+
+template <int N>
+void launch_dt_n(const Batch &b, double cfl, double dfl)
+{
+  const int epb = DtGeom<N>::EPB;
+  const int nb = (b.cfg.nElems + epb - 1) / epb;
+  if (nb <= 0) return;
+  const double KsPr = b.cfg.kappa / b.cfg.Pr;
+  const double KsPr_max = (4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr;
+  hipLaunchKernelGGL((k_calc_dt<N>), dim3(nb), dim3(DtGeom<N>::T), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                     b.dtElem, b.errElem, b.eprob, b.cfg.nElems, b.cfg.kappa, cfl, dfl, b.cfg.mu0, KsPr_max,
+                     b.cfg.parabolic);
+}
+
+#define FX_DISPATCH_N(N_, fn, ...)             \
+  switch (N_) {                                \
+  case 2: fn<2>(__VA_ARGS__); break;           \
+  case 3: fn<3>(__VA_ARGS__); break;           \
+  case 4: fn<4>(__VA_ARGS__); break;           \
+  case 5: fn<5>(__VA_ARGS__); break;           \
+  default: break;                              \
+  }
+
+inline void q_volume(const Batch &b, int mode, RKArgs rk) { FX_DISPATCH_N(b.cfg.N, launch_volume_n, b, mode, rk); }
+inline void q_surface(const Batch &b) { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
+inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out); }
+
+// CALCTIMESTEP into the problem states (advance=1 also moves t and the step counter)
+inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
+{
+  FX_DISPATCH_N(b.cfg.N, launch_dt_n, b, cfl, dfl);
+  if (b.eprob) {
+    if (b.nProb > 0)
+      hipLaunchKernelGGL(k_dt_reduce_seg, dim3(b.nProb), dim3(256), 0, b.stream, b.dtElem, b.errElem,
+                         b.rfirst, b.rcount, b.st, advance);
+  } else {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.errElem, b.cfg.nElems,
+                       b.st, advance);
+  }
+}
+
+// one RHS evaluation, followed by the RK stage update in VMODE_RK
+inline void q_stage(const Batch &b, int mode, RKArgs rk)
+{
+  if (b.cfg.parabolic) q_lifting(b, 1);
+  q_surface(b);
+  q_volume(b, mode, rk);
+}
+
+// the five stages of one TimeStepByLSERKW2 step with dt from the problem states
+inline void q_rk_step(const Batch &b)
+{
+  for (int s = 0; s < 5; s++) {
+    RKArgs rk;
+    rk.st = b.st;
+    rk.eprob = b.eprob;
+    rk.A = b.rkA[s];
+    rk.b = b.rkb[s];
+    rk.first = (s == 0);
+    q_stage(b, VMODE_RK, rk);
+  }
+}
+
+// Connectivity check of one conforming periodic mesh (every side an inner side): every
+// index the kernels derive from ElemToSide/SideToElem must stay inside the mesh.
+inline int check_connectivity(const int *ElemToSide, const int *SideToElem, int E, int S)
+{
+  for (int s = 0; s < S; s++) {
+    const int *t = SideToElem + 5 * s;
+    if (t[0] < 1 || t[0] > E || t[1] < 1 || t[1] > E || t[2] < 1 || t[2] > 6 || t[3] < 1 || t[3] > 6 ||
+        t[4] < 1 || t[4] > 4)
+      return set_err(FX_EINVAL, "SideToElem(:,%d) invalid (only inner conforming sides are supported)", s + 1);
+    const int *am = ElemToSide + 2 * (6 * (t[0] - 1) + t[2] - 1);
+    const int *as = ElemToSide + 2 * (6 * (t[1] - 1) + t[3] - 1);
+    if (am[0] != s + 1 || am[1] != 0 || as[0] != s + 1 || as[1] != t[4])
+      return set_err(FX_EINVAL, "ElemToSide inconsistent with SideToElem at side %d", s + 1);
+  }
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 6; l++) {
+      const int *a = ElemToSide + 2 * (6 * e + l);
+      if (a[0] < 1 || a[0] > S || a[1] < 0 || a[1] > 4)
+        return set_err(FX_EINVAL, "ElemToSide(:,%d,%d) invalid", l + 1, e + 1);
+    }
+  return FX_OK;
+}
+
+// Physics options shared by contexts and servers.
+inline int check_physics(int volumeFlux, int riemann, int precision, double kappa, double Pr, int parabolic)
+{
+  if (volumeFlux != 0 && volumeFlux != 5 && volumeFlux != 32)
+    return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0,5,32)", volumeFlux);
+  if (riemann != 0 && riemann != 1 && riemann != 9 && riemann != 32)
+    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,9,32)", riemann);
+  if (precision != FX_PREC_FP64 && precision != FX_PREC_FP32_SPLIT)
+    return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", precision);
+  if (!(kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");
+  if (parabolic && !(Pr > 0.0)) return set_err(FX_EINVAL, "Pr must be > 0");
+  return FX_OK;
+}
+
+}  // namespace fx

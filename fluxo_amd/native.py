@@ -40,10 +40,40 @@ class fx_mesh(C.Structure):
                                           "TangVec1", "TangVec2", "SurfElem")]
 
 
+class fx_server_config(C.Structure):
+    _fields_ = [("volumeFlux", C.c_int), ("riemann", C.c_int), ("precision", C.c_int),
+                ("lifting", C.c_int), ("kappa", C.c_double), ("mu0", C.c_double),
+                ("Pr", C.c_double), ("CFLscale", C.c_double), ("DFLscale", C.c_double),
+                ("hbmBudget", C.c_longlong), ("maxRequests", C.c_int), ("reserved", C.c_int)]
+
+
+class fx_request(C.Structure):
+    _fields_ = [("N", C.c_int), ("parabolic", C.c_int), ("nElems", C.c_int), ("nSides", C.c_int),
+                ("nSteps", C.c_int), ("reserved", C.c_int), ("mesh", fx_mesh),
+                ("U0", C.c_void_p), ("U_out", C.c_void_p)]
+
+
+class fx_request_status(C.Structure):
+    _fields_ = [("state", C.c_int), ("errType", C.c_int), ("steps", C.c_int),
+                ("reserved", C.c_int), ("t_end", C.c_double), ("dt_last", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class fx_server_stats(C.Structure):
+    _fields_ = [("steps_launched", C.c_longlong), ("graph_captures", C.c_longlong),
+                ("bytes_in_use", C.c_longlong), ("bytes_peak", C.c_longlong),
+                ("bytes_allocated", C.c_longlong), ("active", C.c_int), ("queued", C.c_int),
+                ("done", C.c_int), ("failed", C.c_int)]
+
+
+FX_REQ_QUEUED, FX_REQ_RUNNING, FX_REQ_DONE, FX_REQ_FAILED = range(4)
+
 EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_state_upload", "fx_state_download", "fx_ut_download", "fx_rhs", "fx_volint",
            "fx_lifting", "fx_calc_dt", "fx_rk_step", "fx_run", "fx_entropy", "fx_synchronize",
-           "fx_ctx_bytes", "fx_profile_kernel"]
+           "fx_ctx_bytes", "fx_profile_kernel", "fx_server_create", "fx_server_submit",
+           "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
+           "fx_request_bytes", "fx_server_last_error"]
 
 FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS = range(5)
 
@@ -74,6 +104,17 @@ def lib():
     L.fx_ctx_bytes.argtypes = [C.POINTER(fx_config)]
     L.fx_ctx_bytes.restype = C.c_longlong
     L.fx_profile_kernel.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.fx_server_create.argtypes = [C.POINTER(fx_server_config), C.POINTER(vp)]
+    L.fx_server_submit.argtypes = [vp, C.POINTER(fx_request), ip]
+    L.fx_server_status.argtypes = [vp, C.c_int, C.POINTER(fx_request_status)]
+    L.fx_server_wait.argtypes = [vp, C.c_int, C.c_double]
+    L.fx_server_get_stats.argtypes = [vp, C.POINTER(fx_server_stats)]
+    L.fx_server_destroy.argtypes = [vp]
+    L.fx_server_destroy.restype = None
+    L.fx_request_bytes.argtypes = [C.POINTER(fx_server_config), C.c_int, C.c_int, C.c_int, C.c_int]
+    L.fx_request_bytes.restype = C.c_longlong
+    L.fx_server_last_error.argtypes = [vp]
+    L.fx_server_last_error.restype = C.c_char_p
     _LIB = L
     return L
 
@@ -101,6 +142,24 @@ def ctx_bytes(cfg):
     return lib().fx_ctx_bytes(C.byref(cfg))
 
 
+_MESH_KEYS = (("ElemToSide", "ElemToSide"), ("SideToElem", "SideToElem"),
+              ("Metrics_fTilde", "Mf"), ("Metrics_gTilde", "Mg"), ("Metrics_hTilde", "Mh"),
+              ("sJ", "sJ"), ("NormVec", "NormVec"), ("TangVec1", "TangVec1"),
+              ("TangVec2", "TangVec2"), ("SurfElem", "SurfElem"))
+
+
+def _mesh_struct(mesh):
+    """fx_mesh pointing at contiguous copies of the mesh arrays (returned to keep alive)."""
+    m = fx_mesh()
+    keep = {}
+    for field, key in _MESH_KEYS:
+        dt = np.int32 if key in ("ElemToSide", "SideToElem") else np.float64
+        a = np.ascontiguousarray(mesh[key], dtype=dt)
+        keep[key] = a
+        setattr(m, field, _ptr(a))
+    return m, keep
+
+
 class Engine:
     """One device context: mesh, state and the RHS/RK entry points."""
 
@@ -114,17 +173,7 @@ class Engine:
                                kappa, mu0, Pr, precision)
         self._ctx = C.c_void_p()
         _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
-        m = fx_mesh()
-        keep = {}
-        for field, key in (("ElemToSide", "ElemToSide"), ("SideToElem", "SideToElem"),
-                           ("Metrics_fTilde", "Mf"), ("Metrics_gTilde", "Mg"),
-                           ("Metrics_hTilde", "Mh"), ("sJ", "sJ"), ("NormVec", "NormVec"),
-                           ("TangVec1", "TangVec1"), ("TangVec2", "TangVec2"),
-                           ("SurfElem", "SurfElem")):
-            dt = np.int32 if key in ("ElemToSide", "SideToElem") else np.float64
-            a = np.ascontiguousarray(mesh[key], dtype=dt)
-            keep[key] = a
-            setattr(m, field, _ptr(a))
+        m, _keep = _mesh_struct(mesh)
         _chk(L.fx_mesh_upload(self._ctx, C.byref(m)))
 
     def close(self):
@@ -193,3 +242,73 @@ class Engine:
 
     def sync(self):
         _chk(lib().fx_synchronize(self._ctx))
+
+
+def server_config(volume_flux=32, riemann=1, precision=FX_PREC_FP64, kappa=1.4, mu0=0.0, Pr=0.72,
+                  cfl=0.9, dfl=0.9, hbm_budget=64 << 30, max_requests=4096):
+    c = fx_server_config()
+    c.volumeFlux, c.riemann, c.precision, c.lifting = volume_flux, riemann, precision, 1
+    c.kappa, c.mu0, c.Pr, c.CFLscale, c.DFLscale = kappa, mu0, Pr, cfl, dfl
+    c.hbmBudget, c.maxRequests = hbm_budget, max_requests
+    return c
+
+
+def request_bytes(cfg, N, parabolic, nElems, nSides):
+    return lib().fx_request_bytes(C.byref(cfg), N, int(bool(parabolic)), nElems, nSides)
+
+
+class Server:
+    """Batched server: submit() queues a whole run (nSteps LSERK steps with CALCTIMESTEP) on
+    one mesh; a worker thread in the library batches the running requests per
+    (N, parabolic) pool and admits new ones between steps.  Arrays handed to submit() are
+    kept referenced here until the request completes."""
+
+    def __init__(self, **kw):
+        L = lib()
+        self.cfg = server_config(**kw)
+        self._srv = C.c_void_p()
+        _chk(L.fx_server_create(C.byref(self.cfg), C.byref(self._srv)))
+        self._keep = {}
+
+    def submit(self, mesh, U0, nsteps, parabolic=False):
+        """Queue one request; returns (id, U_out) where U_out receives the final state."""
+        m, keep = _mesh_struct(mesh)
+        U0 = np.ascontiguousarray(U0, dtype=np.float64)
+        out = np.empty_like(U0)
+        r = fx_request()
+        r.N, r.parabolic = int(mesh["N"]), int(bool(parabolic))
+        r.nElems, r.nSides, r.nSteps = int(mesh["nElems"]), int(mesh["nSides"]), int(nsteps)
+        r.mesh = m
+        r.U0, r.U_out = _ptr(U0), _ptr(out)
+        i = C.c_int(-1)
+        _chk(lib().fx_server_submit(self._srv, C.byref(r), C.byref(i)))
+        self._keep[i.value] = (keep, U0, out)
+        return i.value, out
+
+    def status(self, rid):
+        st = fx_request_status()
+        _chk(lib().fx_server_status(self._srv, rid, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in fx_request_status._fields_ if k != "reserved"}
+
+    def wait(self, rid=-1, timeout=600.0):
+        _chk(lib().fx_server_wait(self._srv, rid, timeout))
+
+    def last_error(self):
+        return lib().fx_server_last_error(self._srv).decode()
+
+    def stats(self):
+        st = fx_server_stats()
+        _chk(lib().fx_server_get_stats(self._srv, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in fx_server_stats._fields_}
+
+    def close(self):
+        if self._srv:
+            lib().fx_server_destroy(self._srv)
+            self._srv = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

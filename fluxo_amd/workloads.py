@@ -1,0 +1,86 @@
+"""Synthetic workloads: initial states and the mixed serving request stream.
+
+Initial states follow the reference's ExactFunc (navierstokes/equation.f90):
+  2 density wave      (:632-651)  rho = 1 + A sin(pi (x + y + z - |AdvVel|_1 t)), v = AdvVel, p = 1
+  7 Taylor-Green      (:802-811)  Ma = 0.1, rho = 1
+Arrays are numpy (nElems, n, n, n, 5), the memory layout of the reference's
+U(5,0:N,0:N,0:N,nElems).
+
+The serving stream: request s draws, from a generator seeded with (seed, s),
+  N uniform in {2,3,4,5}; K = round(exp(U(ln 4, ln 24))) elements per direction;
+  s even: density wave on [0,2]^3 (Euler);
+  s odd : Taylor-Green vortex on [-pi,pi]^3, inviscid for s = 1 mod 4, Navier-Stokes with
+          BR1 (Re 1600, Pr 0.72) for s = 3 mod 4;
+  periodic Cartesian K^3 mesh (Ngeo 1); nSteps time steps each.
+"""
+import math
+
+import numpy as np
+
+from .mesh import box_mesh
+
+
+def density_wave(xgp, kappa=1.4, advvel=(1.0, 1.0, 1.0), amp=0.1, t=0.0):
+    x = xgp[..., 0] - advvel[0] * t
+    y = xgp[..., 1] - advvel[1] * t
+    z = xgp[..., 2] - advvel[2] * t
+    rho = 1.0 * (1.0 + amp * np.sin(np.pi * (1.0 * x + 1.0 * y + 1.0 * z)))
+    U = np.empty(xgp.shape[:-1] + (5,))
+    U[..., 0] = rho
+    U[..., 1], U[..., 2], U[..., 3] = rho * advvel[0], rho * advvel[1], rho * advvel[2]
+    U[..., 4] = 1.0 / (kappa - 1.0) + 0.5 * rho * (advvel[0] ** 2 + advvel[1] ** 2 + advvel[2] ** 2)
+    return U
+
+
+def taylor_green(xgp, kappa=1.4):
+    x, y, z = xgp[..., 0], xgp[..., 1], xgp[..., 2]
+    A, Ms = 1.0, 0.1
+    rho = np.ones_like(x)
+    u = A * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -A * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    p = (A / Ms * A / Ms / kappa * rho)
+    p = p + 1.0 / 16.0 * A * A * rho * (np.cos(2 * x) * np.cos(2.0 * z) + 2.0 * np.cos(2.0 * y)
+                                        + 2.0 * np.cos(2.0 * x) + np.cos(2 * y) * np.cos(2.0 * z))
+    U = np.empty(x.shape + (5,))
+    U[..., 0] = rho
+    U[..., 1], U[..., 2], U[..., 3] = rho * u, rho * v, rho * w
+    U[..., 4] = p / (kappa - 1.0) + 0.5 * (U[..., 1] * u + U[..., 2] * v + U[..., 3] * w)
+    return U
+
+
+DOMAINS = {"wave": ((0.0,) * 3, (2.0,) * 3), "tgv": ((-math.pi,) * 3, (2 * math.pi,) * 3)}
+
+
+def request_params(seed, s):
+    """(N, K, ic, parabolic) of request s of the stream seeded with `seed`."""
+    rng = np.random.default_rng([seed, s])
+    N = int(rng.integers(2, 6))
+    K = int(round(math.exp(rng.uniform(math.log(4.0), math.log(24.0)))))
+    if s % 2 == 0:
+        return N, K, "wave", False
+    return N, K, "tgv", s % 4 == 3
+
+
+class MeshCache:
+    """Periodic Cartesian box meshes (metrics from the Fortran host), shared by every
+    request with the same (K, N, domain)."""
+
+    def __init__(self):
+        self._m = {}
+
+    def get(self, K, N, ic):
+        key = (K, N, ic)
+        if key not in self._m:
+            x0, L = DOMAINS[ic]
+            self._m[key] = box_mesh(K, N, x0=x0, length=L, Ngeo=1, warp=False)
+        return self._m[key]
+
+
+def make_request(seed, s, cache, nsteps=50):
+    """Inputs of request s: dict(N, K, ic, parabolic, mesh, U0, nsteps, nodes)."""
+    N, K, ic, par = request_params(seed, s)
+    mesh = cache.get(K, N, ic)
+    U0 = density_wave(mesh["Elem_xGP"]) if ic == "wave" else taylor_green(mesh["Elem_xGP"])
+    return dict(N=N, K=K, ic=ic, parabolic=par, mesh=mesh, U0=U0, nsteps=nsteps,
+                nodes=mesh["nElems"] * (N + 1) ** 3)

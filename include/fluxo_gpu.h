@@ -128,6 +128,78 @@ int fx_profile_kernel(fx_ctx *ctx, int which, int reps, double *avg_ms);
 /* Device bytes held by a context with this configuration (for HBM admission). */
 long long fx_ctx_bytes(const fx_config *cfg);
 
+/* ---- Batched serving ---------------------------------------------------------------
+   No counterpart in the reference, which runs one solve per process (fluxo.f90:180 ->
+   TimeDisc, timedisc.f90:144-415); a request here is that whole run: nSteps steps of
+   TimeStepByLSERKW2 with CALCTIMESTEP before each, on one periodic conforming mesh.
+   Requests of mixed N, mesh size and physics share per-(N, parabolic) element pools in
+   HBM; each pool advances all its requests with one HIP graph per time step (every kernel
+   of the step covers the pool's elements/sides of all requests; each request keeps its own
+   dt from its own CALCTIMESTEP minimum).  A worker thread admits queued requests between
+   steps as the HBM budget and pool slots allow, so requests arriving while others run join
+   the running batch.  A request's result is bitwise the same as the same run through
+   fx_ctx_create/fx_run. */
+typedef struct fx_server fx_server;
+
+typedef struct {
+  int volumeFlux;      /* as fx_config */
+  int riemann;
+  int precision;       /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
+  int lifting;         /* 1 = BR1 for the parabolic requests */
+  double kappa, mu0, Pr;
+  double CFLscale, DFLscale;
+  long long hbmBudget; /* device bytes the admitted requests may hold (fx_request_bytes) */
+  int maxRequests;     /* request ids available over the server's life */
+  int reserved;
+} fx_server_config;
+
+typedef struct {
+  int N;               /* 2..5 */
+  int parabolic;       /* 0 Euler, 1 Navier-Stokes */
+  int nElems, nSides;
+  int nSteps;          /* time steps to run */
+  int reserved;
+  fx_mesh mesh;        /* host arrays as for fx_mesh_upload; borrowed until the request completes */
+  const double *U0;    /* (5,0:N,0:N,0:N,nElems), borrowed until the request completes */
+  double *U_out;       /* (5,0:N,0:N,0:N,nElems), written when the request completes */
+} fx_request;
+
+enum { FX_REQ_QUEUED = 0, FX_REQ_RUNNING = 1, FX_REQ_DONE = 2, FX_REQ_FAILED = 3 };
+
+typedef struct {
+  int state;           /* FX_REQ_* */
+  int errType;         /* first CALCTIMESTEP error (0 none); a request with an error fails */
+  int steps;           /* steps done */
+  int reserved;
+  double t_end, dt_last;                /* simulation time and last dt */
+  double t_submit, t_admit, t_done;     /* seconds since fx_server_create (steady clock) */
+} fx_request_status;
+
+typedef struct {
+  long long steps_launched;   /* pool step graphs launched */
+  long long graph_captures;   /* step graphs (re)captured after pool extent changes */
+  long long bytes_in_use;     /* fx_request_bytes of the admitted requests */
+  long long bytes_peak;
+  long long bytes_allocated;  /* device memory held by the pools */
+  int active, queued, done, failed;
+} fx_server_stats;
+
+int fx_server_create(const fx_server_config *cfg, fx_server **out);
+/* Queue a request (never waits for GPU work); *id identifies it. */
+int fx_server_submit(fx_server *srv, const fx_request *req, int *id);
+int fx_server_status(fx_server *srv, int id, fx_request_status *st);
+/* Wait until request id (or every submitted request if id < 0) has completed or failed,
+   at most timeout_s seconds; FX_ESTATE on time-out. */
+int fx_server_wait(fx_server *srv, int id, double timeout_s);
+int fx_server_get_stats(fx_server *srv, fx_server_stats *out);
+/* Text of the last error met by the server's worker thread ("" if none): why a request
+   failed other than through CALCTIMESTEP's errType. */
+const char *fx_server_last_error(fx_server *srv);
+/* Stops the worker after its current step; requests not completed are marked failed. */
+void fx_server_destroy(fx_server *srv);
+/* Device bytes one request holds in its pool (the admission charge). */
+long long fx_request_bytes(const fx_server_config *cfg, int N, int parabolic, int nElems, int nSides);
+
 #ifdef __cplusplus
 }
 #endif
