@@ -19,8 +19,9 @@ difference of its final state from the FP64 run are reported under "fp32_split".
 Serving: a stream of independent requests (fluxo_amd.workloads, 512 by default, 50 steps
 each) goes through the batched server, first all at once (offline throughput), then as a
 Poisson stream at 0.7x that request rate (p50/p99 latency from submission to result on the
-host), in FP64 and FP32-split; every 8th request also runs alone through its own context
-(the plain per-request path) for the batched/plain throughput ratio.
+host), in FP64 and FP32-split; every request also runs alone through its own context, back
+to back (the plain per-request path), for the batched/plain throughput ratio and the FIFO
+latency a one-at-a-time server would give at the same arrivals.
 
 Prints one JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
 """
@@ -120,35 +121,75 @@ def run_server(reqs, precision, arrivals=None):
 
 def run_plain(reqs, precision):
     """The same requests one at a time, each through its own context (create, upload, run,
-    download, destroy): the plain per-request GPU path."""
+    download, destroy): the plain per-request GPU path.  Returns each request's service
+    time (s)."""
     from fluxo_amd.native import Engine
-    t0 = time.perf_counter()
+    svc = []
     for r in reqs:
+        t0 = time.perf_counter()
         e = Engine(r["mesh"], volume_flux=32, riemann=1, parabolic=r["parabolic"], mu0=1.0 / 1600,
                    Pr=0.72, precision=precision)
         e.upload(r["U0"])
         e.run(r["nsteps"], SERVE_CFL, SERVE_CFL)
         e.download()
         e.close()
-    el = time.perf_counter() - t0
-    return sum(r["nodes"] * 5 * r["nsteps"] for r in reqs) / el
+        svc.append(time.perf_counter() - t0)
+    return np.array(svc)
 
 
-def serving_leg(n_req, nsteps, seed, load, plain_every):
+def tune_serving_tiles():
+    """Run the volume-kernel tile tuner for every pool kind of the stream (N 2..5, Euler and
+    Navier-Stokes, FP64 and FP32-split) on a 12^3 Taylor-Green mesh; the choices are
+    process-wide, so the server pools created afterwards use them."""
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import Engine, FX_PREC_FP64, FX_PREC_FP32_SPLIT
+    from fluxo_amd.workloads import taylor_green
+    chosen = {}
+    for N in (2, 3, 4, 5):
+        m = box_mesh(12, N, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3)
+        U0 = taylor_green(m["Elem_xGP"])
+        for par in (False, True):
+            for prec in (FX_PREC_FP64, FX_PREC_FP32_SPLIT):
+                e = Engine(m, volume_flux=32, riemann=1, parabolic=par, mu0=1.0 / 1600, Pr=0.72,
+                           precision=prec)
+                e.upload(U0)
+                best, _ = e.tune_volume(reps=10)
+                e.close()
+                chosen[f"N{N}{'_ns' if par else '_euler'}{'_fp32' if prec else '_fp64'}"] = best
+    return chosen
+
+
+def fifo_latency(arrivals, service):
+    """Latencies of a single FIFO server (one request on the GPU at a time) with the given
+    arrival times and measured service times."""
+    lat, free = np.empty(len(service)), 0.0
+    for i, (a, sv) in enumerate(zip(arrivals, service)):
+        start = max(a, free)
+        free = start + sv
+        lat[i] = free - a
+    return lat
+
+
+def serving_leg(n_req, nsteps, seed, load):
     """Offline batch throughput, then a Poisson stream at `load` x that request rate with
-    p50/p99 latency, for FP64 and FP32-split; the plain per-request path on every
-    `plain_every`-th request."""
+    p50/p99 latency, for FP64 and FP32-split; the plain per-request path (every request run
+    alone, back to back) for its throughput and, at the same arrivals, FIFO latency."""
     from fluxo_amd.native import FX_PREC_FP64, FX_PREC_FP32_SPLIT
     reqs = serving_stream(n_req, nsteps, seed)
-    out = {"requests": n_req, "steps_per_request": nsteps, "seed": seed,
+    tiles = tune_serving_tiles()
+    dofs = sum(r["nodes"] * 5 * r["nsteps"] for r in reqs)
+    out = {"requests": n_req, "steps_per_request": nsteps, "seed": seed, "DOF_updates": dofs,
            "mix": "N uniform {2,3,4,5}; K^3 periodic Cartesian, K log-uniform [4,24]; density wave "
-                  "(Euler) / TGV inviscid / TGV Navier-Stokes BR1; Ranocha + LLF; CFL/DFL 0.5"}
+                  "(Euler) / TGV inviscid / TGV Navier-Stokes BR1; Ranocha + LLF; CFL/DFL 0.5",
+           "latency": "submission to result in host memory", "volume_tiles": tiles}
     rng = np.random.default_rng(seed + 1)
+    arrivals = None
     for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
         off = run_server(reqs, prec)
-        rate = load * n_req / off["makespan"]
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-        arrivals -= arrivals[0]
+        if arrivals is None:
+            rate = load * n_req / off["makespan"]
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+            arrivals -= arrivals[0]
         live = run_server(reqs, prec, arrivals)
         out[name] = {
             "offline_DOFps": off["dofs"] / off["makespan"],
@@ -160,11 +201,21 @@ def serving_leg(n_req, nsteps, seed, load, plain_every):
             "graph_captures": live["stats"]["graph_captures"],
             "peak_request_bytes": live["stats"]["bytes_peak"],
         }
-    sub = reqs[::plain_every]
-    plain = run_plain(sub, FX_PREC_FP64)
-    out["plain_per_request_DOFps"] = plain
-    out["plain_sample"] = f"every {plain_every}th request ({len(sub)} requests), FP64"
-    out["batched_vs_plain"] = out["fp64"]["offline_DOFps"] / plain
+    svc = run_plain(reqs, FX_PREC_FP64)
+    lat = fifo_latency(arrivals, svc)
+    small = np.array([r["nodes"] < 100000 for r in reqs])
+    sdofs = sum(r["nodes"] * 5 * r["nsteps"] for r, m in zip(reqs, small) if m)
+    out["plain_fp64"] = {
+        "DOFps": dofs / svc.sum(),
+        "fifo_p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "fifo_p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "small_requests": int(small.sum()),
+        "small_DOFps": sdofs / svc[small].sum(),
+        "note": "every request alone through its own context, back to back; FIFO latency at the "
+                "batched run's Poisson arrivals from the measured service times; small = fewer "
+                "than 1e5 nodes"}
+    out["batched_vs_plain_throughput"] = out["fp64"]["offline_DOFps"] / out["plain_fp64"]["DOFps"]
+    out["batched_vs_plain_small_throughput"] = out["fp64"]["offline_DOFps"] / out["plain_fp64"]["small_DOFps"]
     return out
 
 
@@ -262,6 +313,7 @@ def main():
         eng = Engine(mesh, volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72,
                      precision=precision)
         eng.upload(U0)
+        tile, tile_ms = eng.tune_volume()
         if args.warmup > 0:
             eng.run(args.warmup, 0.9, 0.9)
         eng.sync()
@@ -279,7 +331,9 @@ def main():
               "calc_dt": eng.profile_kernel(FX_KERNEL_DT, reps)}
         eng.close()
         return dict(el=el, t_end=t_end, dt_last=dt_last, U=U_end, ms=ms,
-                    value=nodes * 5 * args.steps / el)
+                    value=nodes * 5 * args.steps / el,
+                    tiles={"chosen_elems_per_workgroup": tile,
+                           "ms_by_tile": {str(k): round(v, 4) for k, v in tile_ms.items()}})
 
     r64 = leg(FX_PREC_FP64)
     r32 = leg(FX_PREC_FP32_SPLIT)
@@ -291,6 +345,7 @@ def main():
             "relLinf_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(r64["U"]))),
             "t_end": r32["t_end"],
             "kernel_ms": {k: round(v, 4) for k, v in r32["ms"].items()},
+            "volume_tile": r32["tiles"],
             "volume_kernel_GBps": round(volume_kernel_bytes(N, mesh["nElems"], True, 4)
                                         / (r32["ms"]["volume"] * 1e-3) / 1e9, 1),
             "note": "state after warmup+steps RK steps vs the FP64 run from the same initial state"}
@@ -304,7 +359,7 @@ def main():
             "bytes_per_launch": vbytes}
     t_end, dt_last = r64["t_end"], r64["dt_last"]
     serving = None if args.no_serving else serving_leg(args.serve_requests, args.serve_steps, 2026,
-                                                          args.serve_load, 8)
+                                                          args.serve_load)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
@@ -317,6 +372,7 @@ def main():
                    "nElems": mesh["nElems"], "nodes": nodes, "N": N, "precision": "FP64",
                    "t_end": t_end, "dt_last": dt_last},
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "volume_tile": r64["tiles"],
         "pid_s_per_dof_timestep": 5.0 / value,
         "fp32_split": fp32,
         "serving": serving,

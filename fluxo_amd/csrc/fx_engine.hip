@@ -52,7 +52,8 @@ int check_config(const fx_config *c)
   if (c->nVar != 5) return set_err(FX_EINVAL, "nVar=%d not supported (5)", c->nVar);
   if (c->nElems < 1 || c->nSides < 1) return set_err(FX_EINVAL, "empty mesh");
   if (c->parabolic && c->lifting != 1) return set_err(FX_EINVAL, "only BR1 lifting (lifting=1)");
-  return check_physics(c->volumeFlux, c->riemann, c->precision, c->kappa, c->Pr, c->parabolic);
+  const int rc = check_physics(c->volumeFlux, c->riemann, c->precision, c->kappa, c->Pr, c->parabolic);
+  return rc ? rc : check_nfvse(c);
 }
 
 }  // namespace
@@ -74,6 +75,7 @@ long long fx_ctx_bytes(const fx_config *cfg)
   b += 4LL * n * n * S * 8;          // nvse
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
   if (cfg->parabolic) b += 12LL * n3 * E * 8 + 15LL * n3 * E * 8;
+  if (cfg->nfvse) b += 5LL * n3 * E * 8 + 2LL * E * 8;  // Ffv, alpha buffers
   b += 2LL * ((n3 * E + 255) / 256 + 1) * 8 + 64;
   return b;
 }
@@ -92,6 +94,7 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   Batch &b = c->b;
   b.cfg = *cfg;
   b.set_degree(cfg->N);
+  b.vol_tile = tuned_tile(cfg->N, cfg->parabolic, cfg->precision);
   const size_t nv = vol_doubles(*cfg), n3E = (size_t)b.n3 * cfg->nElems;
   const size_t nf = (size_t)b.n2 * cfg->nSides;
   auto fail = [&](hipError_t e, const char *what) {
@@ -123,6 +126,11 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
     ALLOC(b.Fv, 12 * n3E * 8);
     ALLOC(b.grad, 3 * nv * 8);
   }
+  if (cfg->nfvse) {
+    ALLOC(b.alpha, (size_t)cfg->nElems * 8);
+    ALLOC(b.alpha2, (size_t)cfg->nElems * 8);
+    ALLOC(b.Ffv, nv * 8);
+  }
   const size_t npart = (n3E + 255) / 256 + 1;
   ALLOC(c->part, 2 * npart * 8);
   ALLOC(b.dtElem, (size_t)cfg->nElems * 8);
@@ -144,7 +152,8 @@ void fx_ctx_destroy(fx_ctx *c)
   if (b.stream) (void)hipStreamSynchronize(b.stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Fv, b.grad,
-                  c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem};
+                  c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
+                  b.Ffv};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -258,6 +267,19 @@ int fx_lifting(fx_ctx *c, double *gx, double *gy, double *gz)
   return FX_OK;
 }
 
+int fx_blending(fx_ctx *c, double *alpha)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (!c->b.cfg.nfvse) return set_err(FX_ESTATE, "blending needs nfvse=1");
+  if (!alpha) return set_err(FX_EINVAL, "null alpha");
+  const double *a = q_nfvse_alpha(c->b);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(alpha, a, (size_t)c->b.cfg.nElems * 8, hipMemcpyDeviceToHost, c->b.stream));
+  HIPCHK(hipStreamSynchronize(c->b.stream));
+  return FX_OK;
+}
+
 int fx_calc_dt(fx_ctx *c, double CFLscale, double DFLscale, double *dt, int *errType)
 {
   int rc = ready(c);
@@ -344,6 +366,7 @@ int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
     q_dt(c->b, cfl_number(c->b.cfg.N, 0.9), 0.0, 0);
   RKArgs rk;
   rk.st = c->b.st; rk.eprob = nullptr; rk.A = c->b.rkA[1]; rk.b = c->b.rkb[1]; rk.first = 0;
+  rk.alpha = nullptr; rk.Ffv = nullptr;
   if (which == FX_KERNEL_VOLUME_RK) {  // Ut_tmp must hold a previous stage: run stage 1 once
     RKArgs r1 = rk; r1.A = 0.0; r1.b = c->b.rkb[0]; r1.first = 1;
     q_volume(c->b, VMODE_RK, r1);
@@ -370,6 +393,91 @@ int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
   HIPCHK(hipGetLastError());
   *avg_ms = (double)ms / reps;
   return FX_OK;
+}
+
+int fx_volume_tile(int N, int parabolic, int precision, int *epb)
+{
+  if (N < 2 || N > 5 || !epb) return set_err(FX_EINVAL, "N=%d not supported or null epb", N);
+  *epb = tile_epb(N, tuned_tile(N, parabolic, precision));
+  return FX_OK;
+}
+
+int fx_use_volume_tile(fx_ctx *c, int epb)
+{
+  if (!c) return set_err(FX_EINVAL, "null context");
+  const int N = c->b.cfg.N, nt = (c->b.cfg.volumeFlux == 32) ? 3 : 1;
+  for (int t = 0; t < nt; t++)
+    if (tile_epb(N, t) == epb) {
+      c->b.vol_tile = t;
+      if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+      return FX_OK;
+    }
+  return set_err(FX_EINVAL, "no volume tile with %d elements per workgroup for N=%d", epb, N);
+}
+
+int fx_tune_volume(fx_ctx *c, int reps, int *best_epb, int *tiles, double *ms, int *ntiles)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (reps < 1) return set_err(FX_EINVAL, "reps < 1");
+  Batch &b = c->b;
+  const int N = b.cfg.N;
+  const int nt = (b.cfg.volumeFlux == 32) ? 3 : 1;  // only the Ranocha LSERK form has alternatives
+  const size_t nv = vol_doubles(b.cfg);
+  double *save = nullptr;
+  HIPCHK(hipMalloc((void **)&save, 2 * nv * 8));
+  auto cleanup = [&]() { (void)hipFree(save); };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int best = 0;
+  double bestms = 1e300;
+  hipError_t err = hipSuccess;
+#define TCHK(x) if ((err = (x)) != hipSuccess) goto fail;
+  TCHK(hipMemcpyAsync(save, b.U, nv * 8, hipMemcpyDeviceToDevice, b.stream));
+  TCHK(hipMemcpyAsync(save + nv, b.Uttmp, nv * 8, hipMemcpyDeviceToDevice, b.stream));
+  TCHK(hipEventCreate(&e0));
+  TCHK(hipEventCreate(&e1));
+  // valid inputs of the timed kernel: viscous and face fluxes, a time step, a previous stage
+  if (b.cfg.parabolic) q_lifting(b, 1);
+  q_surface(b);
+  q_dt(b, cfl_number(N, 0.9), 0.0, 0);
+  for (int t = 0; t < nt; t++) {
+    RKArgs rk;
+    rk.st = b.st; rk.eprob = nullptr; rk.A = b.rkA[0]; rk.b = b.rkb[0]; rk.first = 1;
+    rk.alpha = nullptr; rk.Ffv = nullptr;
+    b.vol_tile = t;
+    q_volume(b, VMODE_RK, rk);  // stage 1 form once (warm-up; fills Ut_tmp)
+    rk.A = b.rkA[1]; rk.b = b.rkb[1]; rk.first = 0;
+    TCHK(hipEventRecord(e0, b.stream));
+    for (int r = 0; r < reps; r++) q_volume(b, VMODE_RK, rk);
+    TCHK(hipEventRecord(e1, b.stream));
+    TCHK(hipEventSynchronize(e1));
+    float f = 0.0f;
+    TCHK(hipEventElapsedTime(&f, e0, e1));
+    const double avg = (double)f / reps;
+    if (tiles && t < 4) tiles[t] = tile_epb(N, t);
+    if (ms && t < 4) ms[t] = avg;
+    if (avg < bestms) { bestms = avg; best = t; }
+  }
+  TCHK(hipMemcpyAsync(b.U, save, nv * 8, hipMemcpyDeviceToDevice, b.stream));
+  TCHK(hipMemcpyAsync(b.Uttmp, save + nv, nv * 8, hipMemcpyDeviceToDevice, b.stream));
+  TCHK(hipStreamSynchronize(b.stream));
+  TCHK(hipGetLastError());
+#undef TCHK
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  b.vol_tile = best;
+  tuned_tile(N, b.cfg.parabolic, b.cfg.precision) = best;
+  if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+  if (best_epb) *best_epb = tile_epb(N, best);
+  if (ntiles) *ntiles = nt;
+  return FX_OK;
+fail:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(b.stream);
+  cleanup();
+  return set_err(FX_EHIP, "tuning: %s", hipGetErrorString(err));
 }
 
 int fx_synchronize(fx_ctx *c)

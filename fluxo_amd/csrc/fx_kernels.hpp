@@ -29,6 +29,10 @@ struct Ops {
   double L0;                        // L_HatMinus0 = L_HatMinus(0) (dg.f90:228-229)
   float Ds32[MAXN + 1][MAXN + 1];   // FP32 copies for the split-precision volume integral
   float Dhat32[MAXN + 1][MAXN + 1];
+  // NFVSE shock capturing
+  double sVdm[MAXN + 1][MAXN + 1];  // nodal -> Legendre modal (BuildLegendreVdm basis.f90:196-252)
+  double Csub[MAXN + 1][MAXN + 1];  // Csub(f,m) = sum_{l<=f} w_l D(l,m): sub-cell face metrics
+  double sw[MAXN + 1];              // 1/w_i (sWGP, nfvse.f90:219)
 };
 
 template <typename VR> struct OpsOf;
@@ -93,16 +97,19 @@ __device__ __forceinline__ void vol_to_side(int loc, int flip, int i, int j, int
 }
 
 // -----------------------------------------------------------------------------------
-// Launch geometry of the volume kernel: EPB elements per block, W threads per
-// direction (a multiple of the 64-wide wavefront), 3*W threads per block.
-template <int N> struct VolGeom;
-template <> struct VolGeom<1> { static constexpr int EPB = 16, W = 64; };
-template <> struct VolGeom<2> { static constexpr int EPB = 7, W = 64; };
-template <> struct VolGeom<3> { static constexpr int EPB = 4, W = 64; };
-template <> struct VolGeom<4> { static constexpr int EPB = 5, W = 128; };
-template <> struct VolGeom<5> { static constexpr int EPB = 3, W = 128; };
-template <> struct VolGeom<6> { static constexpr int EPB = 1, W = 64; };
-template <> struct VolGeom<7> { static constexpr int EPB = 1, W = 64; };
+// Launch geometry of the volume kernel: EPB elements per block and W = EPB*(N+1)^2
+// threads per direction rounded up to a multiple of 32 (3*W threads per block).  Tiles[0]
+// is the default EPB of each N; the others are the candidates the volume-kernel tuner
+// times (fx_tune_volume) for the LSERK form with the Ranocha flux.
+template <int N> struct VolTiles;
+template <> struct VolTiles<2> { static constexpr int n = 3, T[3] = {7, 14, 3}; };
+template <> struct VolTiles<3> { static constexpr int n = 3, T[3] = {4, 2, 8}; };
+template <> struct VolTiles<4> { static constexpr int n = 3, T[3] = {5, 2, 3}; };
+template <> struct VolTiles<5> { static constexpr int n = 3, T[3] = {3, 1, 2}; };
+template <int N, int EPB_> struct VolGeom {
+  static constexpr int EPB = EPB_;
+  static constexpr int W = ((EPB * (N + 1) * (N + 1) + 31) / 32) * 32;
+};
 
 // LDS node index with the xi-rows padded to n+1 so xi-line owners (stride n) do not
 // hit one bank.
@@ -139,6 +146,10 @@ struct RKArgs {
   const int *eprob;   // element -> problem (nullptr: one problem)
   double A, b;        // RKA(stage) (0 for stage 1) and RKb(stage)
   int first;          // 1 on stage 1 (Ut_tmp = Ut)
+  // NFVSE blending of the volume integral (nullptr: off): alpha per element and the
+  // sub-cell finite-volume volume term Ffv (5 x (N+1)^3 per element, written for alpha > 0)
+  const double *alpha;
+  const double *Ffv;
 };
 
 // Element-centric volume kernel (VolInt_SplitForm volint.f90:173-218 + VolInt_visc
@@ -164,15 +175,15 @@ struct RKArgs {
 #ifndef FX_VOL_WAVES32
 #define FX_VOL_WAVES32 1
 #endif
-template <int N, int VF, bool PARA, int MODE, typename VR>
-__global__ void __launch_bounds__(3 * VolGeom<N>::W, sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES)
+template <int N, int VF, bool PARA, int MODE, typename VR, int EPB_>
+__global__ void __launch_bounds__((3 * VolGeom<N, EPB_>::W), (sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES))
 k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg,
          const VR *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
          const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
          double *__restrict__ Uttmp, RKArgs rk, int nElems, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-  constexpr int EPB = VolGeom<N>::EPB, W = VolGeom<N>::W, NP = Lds<N>::NP;
+  constexpr int EPB = EPB_, W = VolGeom<N, EPB_>::W, NP = Lds<N>::NP;
   constexpr int NFV = n_fields<VF>();
   constexpr int SLOTS = NFV > 5 ? NFV : 5;
   static_assert(EPB * n2 <= W, "volume launch geometry");
@@ -319,6 +330,15 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     double tot[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) tot[v] = double(SA(el2, v, l));
+    if (MODE != VMODE_VOLINT && rk.alpha) {
+      // VolInt_NFVSE (nfvse.f90:550): Ut = (1 - alpha) Ut + alpha F_FV on the volume part
+      const double a = rk.alpha[e2];
+      if (a != 0.0) {
+        const double *ff = rk.Ffv + ((size_t)e2 * n3 + node) * 5;
+#pragma unroll
+        for (int v = 0; v < 5; v++) tot[v] = (1.0 - a) * tot[v] + a * ff[v];
+      }
+    }
     if (MODE != VMODE_VOLINT) {
       // SurfInt gather: faces touched by node (i,j,k), in local side order
       int locs[3], nl = 0;

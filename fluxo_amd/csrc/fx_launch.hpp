@@ -13,6 +13,7 @@
 
 #include "../../include/fluxo_gpu.h"
 #include "fx_kernels.hpp"
+#include "fx_nfvse.hpp"
 
 namespace fx {
 
@@ -83,6 +84,42 @@ inline void lgl_nodes(int N, double *x, double *w)
   }
 }
 
+// Legendre polynomial of degree N normalised by sqrt(N+1/2) (LegendrePolynomialAndDerivative
+// basis.f90) and the Gauss nodes/weights (LegendreGaussNodesAndWeights basis.f90:430-497).
+inline double legendre_normalized(int N, double x)
+{
+  if (N == 0) return std::sqrt(0.5);
+  double Lm2 = 1.0, Lm1 = x, L = x;
+  for (int i = 2; i <= N; i++) {
+    L = ((double)(2 * i - 1) * x * Lm1 - (double)(i - 1) * Lm2) / (double)i;
+    Lm2 = Lm1; Lm1 = L;
+  }
+  return L * std::sqrt((double)N + 0.5);
+}
+
+inline void gauss_nodes(int N, double *x, double *w)
+{
+  const double pi = std::acos(-1.0);
+  const int n = N + 1;
+  for (int i = 0; i < n; i++) {
+    double xi = -std::cos((2.0 * i + 1.0) / (2.0 * N + 2.0) * pi), L = 0.0, Ld = 1.0;
+    for (int it = 0; it < 50; it++) {
+      double Lm2 = 1.0, Lm1 = xi, Ldm2 = 0.0, Ldm1 = 1.0;
+      L = xi; Ld = 1.0;
+      for (int k = 2; k <= n; k++) {
+        L = ((double)(2 * k - 1) * xi * Lm1 - (double)(k - 1) * Lm2) / (double)k;
+        Ld = Ldm2 + (double)(2 * k - 1) * Lm1;
+        Lm2 = Lm1; Lm1 = L; Ldm2 = Ldm1; Ldm1 = Ld;
+      }
+      const double dx = -L / Ld;
+      xi += dx;
+      if (std::fabs(dx) < 1e-15 * std::fabs(xi) + 1e-300) break;
+    }
+    x[i] = xi;
+    w[i] = 2.0 / ((1.0 - xi * xi) * Ld * Ld);
+  }
+}
+
 inline void make_ops(int N, Ops &o)
 {
   std::memset(&o, 0, sizeof(o));
@@ -116,6 +153,38 @@ inline void make_ops(int N, Ops &o)
       o.Ds32[i][j] = (float)o.Ds[i][j];
       o.Dhat32[i][j] = (float)o.Dhat[i][j];
     }
+  // NFVSE: nodal -> modal Legendre projection by Gauss quadrature with 3N/2+2 points
+  // (BuildLegendreVdm basis.f90:196-252), sub-cell face metric weights, 1/w
+  const int Ng = 3 * N / 2 + 1;
+  double xg[3 * MAXN / 2 + 2], wg[3 * MAXN / 2 + 2];
+  gauss_nodes(Ng, xg, wg);
+  for (int jm = 0; jm < n; jm++)
+    for (int c = 0; c < n; c++) {
+      double sum = 0.0;
+      for (int g = 0; g <= Ng; g++) {
+        // barycentric Lagrange polynomial of node c at xg (a node itself: unit vector)
+        int hit = -1;
+        for (int k = 0; k < n; k++)
+          if (std::fabs(xg[g] - x[k]) <= 4.5e-16 * std::fmax(1.0, std::fabs(x[k]))) hit = k;
+        double lc;
+        if (hit >= 0) {
+          lc = (hit == c) ? 1.0 : 0.0;
+        } else {
+          double den = 0.0;
+          for (int k = 0; k < n; k++) den += wb[k] / (xg[g] - x[k]);
+          lc = (wb[c] / (xg[g] - x[c])) / den;
+        }
+        sum += legendre_normalized(jm, xg[g]) * wg[g] * lc;
+      }
+      o.sVdm[jm][c] = sum;
+    }
+  for (int f = 0; f < n; f++)
+    for (int m = 0; m < n; m++) {
+      double c = 0.0;
+      for (int l = 0; l <= f; l++) c += w[l] * o.D[l][m];
+      o.Csub[f][m] = c;
+    }
+  for (int i = 0; i < n; i++) o.sw[i] = 1.0 / w[i];
 }
 
 // Operators of degree N into constant memory of the current device (once per process).
@@ -170,6 +239,25 @@ inline double dfl_number(int N, double DFLscale)
   return DFLscale * dfl_alpha(N) / ((2.0 * N + 1.0) * (2.0 * N + 1.0));
 }
 
+// Process-wide tile choice (index into VolTiles<N>::T) of the LSERK volume kernel for each
+// degree, Euler/Navier-Stokes and precision: set by the tuner, read when a context or server
+// pool is created.
+inline int &tuned_tile(int N, int parabolic, int precision)
+{
+  static int t[MAXN + 1][2][2] = {};
+  return t[N][parabolic ? 1 : 0][precision == FX_PREC_FP32_SPLIT ? 1 : 0];
+}
+inline int tile_epb(int N, int idx)
+{
+  switch (N) {
+  case 2: return VolTiles<2>::T[idx];
+  case 3: return VolTiles<3>::T[idx];
+  case 4: return VolTiles<4>::T[idx];
+  case 5: return VolTiles<5>::T[idx];
+  default: return 0;
+  }
+}
+
 // ---- batch ----------------------------------------------------------------------------
 // Device arrays of one launch domain.  cfg.nElems / cfg.nSides are the launch extents;
 // with eprob == nullptr all elements belong to problem 0 (st[0]), otherwise eprob maps
@@ -191,6 +279,10 @@ struct Batch {
   DtState *st = nullptr;
   int *rfirst = nullptr, *rcount = nullptr;
   int nProb = 1;
+  int vol_tile = 0;  // index into VolTiles<N>::T for the LSERK volume kernel
+  // NFVSE (cfg.nfvse): blending coefficient buffers (two for the propagation sweeps) and
+  // the sub-cell finite-volume volume term
+  double *alpha = nullptr, *alpha2 = nullptr, *Ffv = nullptr;
   double rkA[5], rkb[5], rkc[5];
 
   void set_degree(int N)
@@ -201,19 +293,33 @@ struct Batch {
   }
 };
 
+template <int N, int VF, bool PARA, int MODE, int EPB>
+void launch_volume_e(const Batch &b, RKArgs rk)
+{
+  using G = VolGeom<N, EPB>;
+  const int nb = (b.cfg.nElems + EPB - 1) / EPB;
+  if (nb <= 0) return;
+  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+                       b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.cfg.kappa);
+  else
+    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+                       b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.cfg.kappa);
+}
+
+// The LSERK form with the Ranocha flux has every tile of VolTiles<N>; other forms use the
+// default tile.
 template <int N, int VF, bool PARA, int MODE>
 void launch_volume_t(const Batch &b, RKArgs rk)
 {
-  const int nb = (b.cfg.nElems + VolGeom<N>::EPB - 1) / VolGeom<N>::EPB;
-  if (nb <= 0) return;
-  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
-    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
-                       b.stream, b.U, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp,
-                       rk, b.cfg.nElems, b.cfg.kappa);
-  else
-    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double>), dim3(nb), dim3(3 * VolGeom<N>::W), 0,
-                       b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk,
-                       b.cfg.nElems, b.cfg.kappa);
+  constexpr int T0 = VolTiles<N>::T[0], T1 = VolTiles<N>::T[1], T2 = VolTiles<N>::T[2];
+  if (MODE == VMODE_RK && VF == 32) {
+    if (b.vol_tile == 1) { launch_volume_e<N, VF, PARA, MODE, T1>(b, rk); return; }
+    if (b.vol_tile == 2) { launch_volume_e<N, VF, PARA, MODE, T2>(b, rk); return; }
+  }
+  launch_volume_e<N, VF, PARA, MODE, T0>(b, rk);
 }
 
 template <int N, int VF>
@@ -323,10 +429,71 @@ inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
   }
 }
 
+inline NfvseArgs nfvse_args(const fx_config &c)
+{
+  NfvseArgs a{};
+  a.computeAlpha = c.computeAlpha;
+  a.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow(c.N + 1.0, 0.25));
+  a.sharpness = std::log((1.0 - 1.e-4) / 1.e-4);
+  a.alpha_min = c.alpha_min;
+  a.alpha_max = c.alpha_max;
+  a.shockBlendCoef = c.shockBlendCoef;
+  return a;
+}
+
+template <int N>
+void launch_nfvse_alpha_n(const Batch &b)
+{
+  const int epb = LiftGeom<N>::EPB, nb = (b.cfg.nElems + epb - 1) / epb;
+  if (nb <= 0) return;
+  hipLaunchKernelGGL((k_nfvse_alpha<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.eprob, b.alpha2,
+                     b.cfg.nElems, b.cfg.kappa, nfvse_args(b.cfg));
+}
+
+template <int N, int RS>
+void launch_nfvse_fv_t(const Batch &b, const double *alpha)
+{
+  const int nb = (b.cfg.nElems + FvGeom<N>::EPB - 1) / FvGeom<N>::EPB;
+  if (nb <= 0) return;
+  hipLaunchKernelGGL((k_nfvse_fv<N, RS>), dim3(nb), dim3(3 * FvGeom<N>::W), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh,
+                     alpha, b.eprob, b.Ffv, b.cfg.nElems, b.cfg.kappa);
+}
+
+template <int N>
+void launch_nfvse_fv_n(const Batch &b, const double *alpha)
+{
+  switch (b.cfg.riemann) {
+  case 0: launch_nfvse_fv_t<N, 0>(b, alpha); break;
+  case 9: launch_nfvse_fv_t<N, 9>(b, alpha); break;
+  case 32: launch_nfvse_fv_t<N, 32>(b, alpha); break;
+  default: launch_nfvse_fv_t<N, 1>(b, alpha); break;
+  }
+}
+
+// CalcBlendingCoefficient + propagation sweeps; returns the buffer holding the final alpha.
+inline const double *q_nfvse_alpha(const Batch &b)
+{
+  FX_DISPATCH_N(b.cfg.N, launch_nfvse_alpha_n, b);
+  double *src = b.alpha2, *dst = b.alpha;
+  const int nb = (b.cfg.nElems + 255) / 256;
+  for (int sw = 0; sw < b.cfg.spacePropSweeps && nb > 0; sw++) {
+    hipLaunchKernelGGL(k_nfvse_prop, dim3(nb), dim3(256), 0, b.stream, src, dst, b.e2s, b.s2e, b.eprob,
+                       b.cfg.nElems, b.cfg.spacePropFactor);
+    double *t = src; src = dst; dst = t;
+  }
+  return src;
+}
+
 // one RHS evaluation, followed by the RK stage update in VMODE_RK
 inline void q_stage(const Batch &b, int mode, RKArgs rk)
 {
   if (b.cfg.parabolic) q_lifting(b, 1);
+  if (b.cfg.nfvse && mode != VMODE_VOLINT) {
+    const double *a = q_nfvse_alpha(b);
+    FX_DISPATCH_N(b.cfg.N, launch_nfvse_fv_n, b, a);
+    rk.alpha = a;
+    rk.Ffv = b.Ffv;
+  }
   q_surface(b);
   q_volume(b, mode, rk);
 }
@@ -341,6 +508,8 @@ inline void q_rk_step(const Batch &b)
     rk.A = b.rkA[s];
     rk.b = b.rkb[s];
     rk.first = (s == 0);
+    rk.alpha = nullptr;
+    rk.Ffv = nullptr;
     q_stage(b, VMODE_RK, rk);
   }
 }
@@ -379,6 +548,19 @@ inline int check_physics(int volumeFlux, int riemann, int precision, double kapp
     return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", precision);
   if (!(kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");
   if (parabolic && !(Pr > 0.0)) return set_err(FX_EINVAL, "Pr must be > 0");
+  return FX_OK;
+}
+
+inline int check_nfvse(const fx_config *c)
+{
+  if (!c->nfvse) return FX_OK;
+  if (c->nfvse != 1) return set_err(FX_EINVAL, "nfvse=%d (0 off, 1 first-order sub-cell FV)", c->nfvse);
+  if (c->parabolic) return set_err(FX_EINVAL, "NFVSE shock capturing is supported for Euler (parabolic=0)");
+  if (c->computeAlpha != 1 && c->computeAlpha != 3)
+    return set_err(FX_EINVAL, "ComputeAlpha=%d not supported (1 Persson-Peraire, 3 fixed)", c->computeAlpha);
+  if (c->spacePropSweeps < 0 || c->spacePropSweeps > 16) return set_err(FX_EINVAL, "SpacePropSweeps out of 0..16");
+  if (!(c->alpha_max > 0.0) || c->alpha_max > 1.0 || c->alpha_min < 0.0)
+    return set_err(FX_EINVAL, "alpha_max must be in (0,1], alpha_min >= 0");
   return FX_OK;
 }
 

@@ -211,6 +211,7 @@ inline int pool_init(Pool &p, const fx_server_config &c, int N, int par)
   b.cfg.parabolic = par; b.cfg.lifting = c.lifting; b.cfg.precision = c.precision;
   b.cfg.kappa = c.kappa; b.cfg.mu0 = c.mu0; b.cfg.Pr = c.Pr;
   b.set_degree(N);
+  b.vol_tile = tuned_tile(N, par, c.precision);
   b.cfg.nElems = 0; b.cfg.nSides = 0; b.nProb = 0;
   int rc = ensure_ops(N);
   if (rc) return rc;

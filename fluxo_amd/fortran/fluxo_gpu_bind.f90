@@ -14,6 +14,8 @@ module fluxo_gpu_bind
     integer(c_int) :: volumeFlux = 0, riemann = 1, parabolic = 0, lifting = 1
     integer(c_int) :: precision = 0, reserved = 0
     real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, reserved2 = 0.0d0
+    integer(c_int) :: nfvse = 0, computeAlpha = 1, spacePropSweeps = 1, reserved3 = 0
+    real(c_double) :: alpha_max = 0.5d0, alpha_min = 0.01d0, spacePropFactor = 0.5d0, shockBlendCoef = 0.0d0
   end type fx_config
 
   type, bind(C) :: fx_mesh
@@ -72,6 +74,11 @@ module fluxo_gpu_bind
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: gx(*), gy(*), gz(*)
     end function fx_lifting
+    integer(c_int) function fx_blending(ctx, alpha) bind(C, name="fx_blending")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: alpha(*)
+    end function fx_blending
     integer(c_int) function fx_calc_dt(ctx, cfl, dfl, dt, errType) bind(C, name="fx_calc_dt")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

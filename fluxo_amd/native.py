@@ -31,7 +31,10 @@ class fx_config(C.Structure):
                 ("volumeFlux", C.c_int), ("riemann", C.c_int), ("parabolic", C.c_int),
                 ("lifting", C.c_int), ("precision", C.c_int), ("reserved", C.c_int),
                 ("kappa", C.c_double), ("mu0", C.c_double), ("Pr", C.c_double),
-                ("reserved2", C.c_double)]
+                ("reserved2", C.c_double),
+                ("nfvse", C.c_int), ("computeAlpha", C.c_int), ("spacePropSweeps", C.c_int),
+                ("reserved3", C.c_int), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
+                ("spacePropFactor", C.c_double), ("shockBlendCoef", C.c_double)]
 
 
 class fx_mesh(C.Structure):
@@ -73,7 +76,8 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_lifting", "fx_calc_dt", "fx_rk_step", "fx_run", "fx_entropy", "fx_synchronize",
            "fx_ctx_bytes", "fx_profile_kernel", "fx_server_create", "fx_server_submit",
            "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
-           "fx_request_bytes", "fx_server_last_error"]
+           "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
+           "fx_use_volume_tile", "fx_blending"]
 
 FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS = range(5)
 
@@ -96,6 +100,7 @@ def lib():
         getattr(L, f).argtypes = [vp, vp]
     L.fx_rhs.argtypes = [vp, C.c_double, vp]
     L.fx_lifting.argtypes = [vp, vp, vp, vp]
+    L.fx_blending.argtypes = [vp, vp]
     L.fx_calc_dt.argtypes = [vp, C.c_double, C.c_double, dp, ip]
     L.fx_rk_step.argtypes = [vp, C.c_double, C.c_double]
     L.fx_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip]
@@ -113,6 +118,9 @@ def lib():
     L.fx_server_destroy.restype = None
     L.fx_request_bytes.argtypes = [C.POINTER(fx_server_config), C.c_int, C.c_int, C.c_int, C.c_int]
     L.fx_request_bytes.restype = C.c_longlong
+    L.fx_tune_volume.argtypes = [vp, C.c_int, ip, ip, dp, ip]
+    L.fx_volume_tile.argtypes = [C.c_int, C.c_int, C.c_int, ip]
+    L.fx_use_volume_tile.argtypes = [vp, C.c_int]
     L.fx_server_last_error.argtypes = [vp]
     L.fx_server_last_error.restype = C.c_char_p
     _LIB = L
@@ -129,8 +137,16 @@ def _ptr(a):
 
 
 def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, kappa=1.4,
-                mu0=0.0, Pr=0.72, precision=FX_PREC_FP64):
+                mu0=0.0, Pr=0.72, precision=FX_PREC_FP64, nfvse=False, alpha_max=0.5, alpha_min=0.01,
+                sweeps=1, prop_factor=0.5, fixed_alpha=None):
+    """Engine options; the NFVSE ones follow the reference's parameter names (alpha_max,
+    alpha_min, SpacePropSweeps, SpacePropFactor; fixed_alpha = ComputeAlpha 3 with that
+    ShockBlendCoef, None = Persson-Peraire indicator)."""
     c = fx_config()
+    c.nfvse, c.spacePropSweeps = int(bool(nfvse)), sweeps
+    c.computeAlpha = 1 if fixed_alpha is None else 3
+    c.alpha_max, c.alpha_min, c.spacePropFactor = alpha_max, alpha_min, prop_factor
+    c.shockBlendCoef = 0.0 if fixed_alpha is None else fixed_alpha
     c.N, c.nVar, c.nElems, c.nSides = N, 5, nElems, nSides
     c.volumeFlux, c.riemann, c.parabolic, c.lifting = volume_flux, riemann, int(bool(parabolic)), 1
     c.precision = precision
@@ -164,13 +180,13 @@ class Engine:
     """One device context: mesh, state and the RHS/RK entry points."""
 
     def __init__(self, mesh, volume_flux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0,
-                 Pr=0.72, precision=FX_PREC_FP64):
+                 Pr=0.72, precision=FX_PREC_FP64, **nfvse):
         L = lib()
         self.N = int(mesh["N"])
         self.nElems = int(mesh["nElems"])
         self.nSides = int(mesh["nSides"])
         self.cfg = make_config(self.N, self.nElems, self.nSides, volume_flux, riemann, parabolic,
-                               kappa, mu0, Pr, precision)
+                               kappa, mu0, Pr, precision, **nfvse)
         self._ctx = C.c_void_p()
         _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
         m, _keep = _mesh_struct(mesh)
@@ -216,6 +232,12 @@ class Engine:
         _chk(lib().fx_lifting(self._ctx, *(_ptr(a) for a in g)))
         return g
 
+    def blending(self):
+        """NFVSE blending coefficient of the current state (CalcBlendingCoefficient)."""
+        a = np.empty(self.nElems)
+        _chk(lib().fx_blending(self._ctx, _ptr(a)))
+        return a
+
     def calc_dt(self, cfl, dfl=0.0):
         dt, err = C.c_double(), C.c_int()
         _chk(lib().fx_calc_dt(self._ctx, cfl, dfl, C.byref(dt), C.byref(err)))
@@ -240,6 +262,17 @@ class Engine:
         _chk(lib().fx_profile_kernel(self._ctx, which, reps, C.byref(ms)))
         return ms.value
 
+    def tune_volume(self, reps=20):
+        """Time every tile of the LSERK volume kernel on this mesh/state (restored), keep the
+        fastest; returns (best elements per workgroup, {tile: ms})."""
+        best, n = C.c_int(0), C.c_int(0)
+        tiles, ms = (C.c_int * 4)(), (C.c_double * 4)()
+        _chk(lib().fx_tune_volume(self._ctx, reps, C.byref(best), tiles, ms, C.byref(n)))
+        return best.value, {tiles[i]: ms[i] for i in range(n.value)}
+
+    def use_volume_tile(self, epb):
+        _chk(lib().fx_use_volume_tile(self._ctx, epb))
+
     def sync(self):
         _chk(lib().fx_synchronize(self._ctx))
 
@@ -251,6 +284,13 @@ def server_config(volume_flux=32, riemann=1, precision=FX_PREC_FP64, kappa=1.4, 
     c.kappa, c.mu0, c.Pr, c.CFLscale, c.DFLscale = kappa, mu0, Pr, cfl, dfl
     c.hbmBudget, c.maxRequests = hbm_budget, max_requests
     return c
+
+
+def volume_tile(N, parabolic=False, precision=FX_PREC_FP64):
+    """Elements per workgroup of the LSERK volume kernel (process-wide choice)."""
+    e = C.c_int(0)
+    _chk(lib().fx_volume_tile(N, int(bool(parabolic)), precision, C.byref(e)))
+    return e.value
 
 
 def request_bytes(cfg, N, parabolic, nElems, nSides):

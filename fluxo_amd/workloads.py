@@ -84,3 +84,20 @@ def make_request(seed, s, cache, nsteps=50):
     U0 = density_wave(mesh["Elem_xGP"]) if ic == "wave" else taylor_green(mesh["Elem_xGP"])
     return dict(N=N, K=K, ic=ic, parabolic=par, mesh=mesh, U0=U0, nsteps=nsteps,
                 nodes=mesh["nElems"] * (N + 1) ** 3)
+
+
+SEDOV_SIGMA = 0.06
+SEDOV_E0 = 1.0
+
+
+def sedov_blast(xgp, kappa=1.4, sigma=SEDOV_SIGMA, E0=SEDOV_E0, p0=1.0e-5, center=(0.0, 0.0, 0.0)):
+    """3D Sedov-type blast: rho = 1, v = 0, p = p0 + (kappa-1) E0 G(r) with the normalised
+    Gaussian G(r) = exp(-r^2/(2 sigma^2)) / ((2 pi)^(3/2) sigma^3), so the pulse carries
+    total internal energy E0 (sigma = 0.06, E0 = 1, p0 = 1e-5 on [-1.5,1.5]^3)."""
+    r2 = sum((xgp[..., d] - center[d]) ** 2 for d in range(3))
+    g = np.exp(-r2 / (2.0 * sigma * sigma)) / ((2.0 * np.pi) ** 1.5 * sigma ** 3)
+    p = p0 + (kappa - 1.0) * E0 * g
+    U = np.zeros(xgp.shape[:-1] + (5,))
+    U[..., 0] = 1.0
+    U[..., 4] = p / (kappa - 1.0)
+    return U

@@ -50,6 +50,19 @@ typedef struct {
   double mu0;       /* 'mu0' constant viscosity (equation.f90:229-230) */
   double Pr;        /* 'Pr' (equation.f90:226) */
   double reserved2;
+  /* NFVSE shock capturing (SHOCK_NFVSE, CMakeLists.txt:316-331; parameters of InitNFVSE
+     shockcapturing/nfvse.f90:138-146), Euler only: sub-cell first-order finite volumes
+     (SubFVMethod=1) blended into the volume integral of each element with the coefficient
+     of CalcBlendingCoefficient (nfvse.f90:1537-1615) and TimeRelFactor=0. */
+  int nfvse;           /* 0 off, 1 on */
+  int computeAlpha;    /* 'ComputeAlpha': 1 Persson-Peraire indicator on rho*p
+                          (ModalThreshold=1), 3 fixed 'ShockBlendCoef' */
+  int spacePropSweeps; /* 'SpacePropSweeps' */
+  int reserved3;
+  double alpha_max;       /* 'alpha_max' */
+  double alpha_min;       /* 'alpha_min' */
+  double spacePropFactor; /* 'SpacePropFactor' */
+  double shockBlendCoef;  /* 'ShockBlendCoef' */
 } fx_config;
 
 /* Mesh data produced by the host's InitMesh/CalcMetrics (mesh.f90:100-300,
@@ -94,6 +107,10 @@ int fx_volint(fx_ctx *ctx, double *Ut_host);
    (nVar,0:N,0:N,0:N,nElems). */
 int fx_lifting(fx_ctx *ctx, double *gradPx, double *gradPy, double *gradPz);
 
+/* CalcBlendingCoefficient + PropagateBlendingCoeff (shockcapturing/nfvse.f90:1537-1615,
+   nfvse_mpi.f90:31-153) on the device state: alpha(nElems), as the RHS uses it. */
+int fx_blending(fx_ctx *ctx, double *alpha);
+
 /* CALCTIMESTEP (navierstokes/calctimestep.f90:39-170) with the CFL/DFL scaling of
    FillCFL_DFL (timedisc.f90:697-741): CFLscale/DFLscale are the parameter.ini
    values.  errType as in the reference: 1 NaN density, 2 p<=0, 3/4 NaN dt. */
@@ -124,6 +141,20 @@ int fx_synchronize(fx_ctx *ctx);
 enum { FX_KERNEL_VOLUME_RK = 0, FX_KERNEL_SURFACE = 1, FX_KERNEL_LIFTING = 2, FX_KERNEL_DT = 3,
        FX_KERNEL_VOLUME_RHS = 4 };
 int fx_profile_kernel(fx_ctx *ctx, int which, int reps, double *avg_ms);
+
+/* Tile tuner of the volume kernel (no counterpart in the reference).  Times each compiled
+   tile (elements per workgroup) of the LSERK volume kernel on this context's mesh and
+   state, restores U and Ut_tmp, and keeps the fastest for this context and as the
+   process-wide choice for its (N, parabolic, precision), used by contexts and server pools
+   created afterwards.
+   tiles[t] / ms[t] (room for 4) receive each candidate's elements per workgroup and average
+   launch time, *ntiles their number, *best_epb the choice.  Only the Ranocha (VolumeFlux
+   32) form has alternatives; other fluxes report their single tile. */
+int fx_tune_volume(fx_ctx *ctx, int reps, int *best_epb, int *tiles, double *ms, int *ntiles);
+/* The process-wide volume-kernel tile (elements per workgroup) for (N, parabolic, precision). */
+int fx_volume_tile(int N, int parabolic, int precision, int *epb);
+/* Use the tile with epb elements per workgroup for this context (FX_EINVAL if none). */
+int fx_use_volume_tile(fx_ctx *ctx, int epb);
 
 /* Device bytes held by a context with this configuration (for HBM admission). */
 long long fx_ctx_bytes(const fx_config *cfg);

@@ -72,6 +72,13 @@ typedef struct {
   /* mesh */
   const int *ElemToSide, *SideToElem;
   const double *Mf, *Mg, *Mh, *sJ, *NormVec, *TangVec1, *TangVec2, *SurfElem;
+  /* NFVSE shock capturing (SHOCK_NFVSE, SubFVMethod=1, ComputeAlpha=1, ModalThreshold=1,
+     TimeRelFactor=0, indicator rho*p): nfvse=0 off */
+  int nfvse, spacePropSweeps;
+  double alpha_max, alpha_min, spacePropFactor;
+  double *alpha;     /* optional out (nElems): blending coefficient of the last RHS */
+  int computeAlpha;  /* 1 Persson-Peraire indicator, 3 fixed alpha = shockBlendCoef */
+  double shockBlendCoef;
 } ora_problem;
 
 /* two-point and surface fluxes; U/aux layout per point: U(5), aux(6)=(1/rho,v1,v2,v3,p,|v|^2) */
@@ -103,6 +110,14 @@ void ora_calc_entropy(const ora_problem *P, const double *U, const double *Ut, d
    (src/equation/navierstokes/equation.f90:632-651, 802-811) */
 void ora_exactfunc(int which, double kappa, const double *advvel, double amp, double t,
                    int npts, const double *x, double *U);
+
+/* ---- NFVSE shock capturing (src/shockcapturing/nfvse.f90, src/indicators/indicators.f90) ---- */
+double ora_nfvse_threshold(int N);
+/* sub-cell face metrics of one element: [d][f][b][a][nv(3) t1(3) t2(3) norm], f = 0..N-1 */
+void ora_subcell_metrics(int N, const double *Mf, const double *Mg, const double *Mh, double *out);
+void ora_persson_peraire(const ora_problem *P, const double *U, double *eta);
+void ora_blending_coefficient(const ora_problem *P, const double *U, double *alpha);
+void ora_volint_nfvse(const ora_problem *P, const double *U, const double *alpha, double *Ut);
 
 /* Carpenter RK4-5 coefficients and the LGL CFL/DFL alpha tables
    (src/timedisc/timedisc_vars.f90:79-90, 204-233) */

@@ -540,7 +540,16 @@ void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, dou
   (void)t;
   size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
   memset(Ut, 0, sizeof(double) * NV * n3E);
+  double *alpha = NULL;
+  if (P->nfvse) {  /* CalcBlendingCoefficient before the volume integral (dg.f90:358-361) */
+    alpha = P->alpha ? P->alpha : malloc(sizeof(double) * P->nElems);
+    ora_blending_coefficient(P, U, alpha);
+  }
   ora_volint_splitform(P, U, Ut);
+  if (P->nfvse) {  /* VolInt_NFVSE blends the volume integral only (dg.f90:381-383) */
+    ora_volint_nfvse(P, U, alpha, Ut);
+    if (alpha != P->alpha) free(alpha);
+  }
   double *gx = NULL, *gy = NULL, *gz = NULL;
   if (P->parabolic) {
     gx = malloc(sizeof(double) * NV * n3E); gy = malloc(sizeof(double) * NV * n3E);

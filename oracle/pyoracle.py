@@ -58,6 +58,12 @@ def lib():
         L.ora_rk_coefs.argtypes = [_dp, _dp, _dp]
         L.ora_lgl_nodes.argtypes = [C.c_int, _dp, _dp]
         L.ora_dg_operators.argtypes = [C.c_int] + [_dp] * 7
+        L.ora_nfvse_threshold.restype = C.c_double
+        L.ora_nfvse_threshold.argtypes = [C.c_int]
+        L.ora_subcell_metrics.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.ora_persson_peraire.argtypes = [C.c_void_p, _dp, _dp]
+        L.ora_blending_coefficient.argtypes = [C.c_void_p, _dp, _dp]
+        L.ora_volint_nfvse.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -71,7 +77,10 @@ class _Problem(C.Structure):
                 ("ElemToSide", C.c_void_p), ("SideToElem", C.c_void_p),
                 ("Mf", C.c_void_p), ("Mg", C.c_void_p), ("Mh", C.c_void_p), ("sJ", C.c_void_p),
                 ("NormVec", C.c_void_p), ("TangVec1", C.c_void_p), ("TangVec2", C.c_void_p),
-                ("SurfElem", C.c_void_p)]
+                ("SurfElem", C.c_void_p),
+                ("nfvse", C.c_int), ("spacePropSweeps", C.c_int), ("alpha_max", C.c_double),
+                ("alpha_min", C.c_double), ("spacePropFactor", C.c_double), ("alpha", C.c_void_p),
+                ("computeAlpha", C.c_int), ("shockBlendCoef", C.c_double)]
 
 
 def box_mesh(K, N, x0=(0.0, 0.0, 0.0), length=(2.0, 2.0, 2.0), Ngeo=1, warp=False, amp=0.1):
@@ -106,7 +115,8 @@ class Problem:
     """An equation/mesh setup; keeps references to every array the C struct points to."""
 
     def __init__(self, mesh, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
-                 lifting=1):
+                 lifting=1, nfvse=False, alpha_max=0.5, alpha_min=0.01, sweeps=1, prop_factor=0.5,
+                 fixed_alpha=None):
         self.mesh = mesh
         self.N = mesh["N"]
         self.nElems = mesh["nElems"]
@@ -122,6 +132,12 @@ class Problem:
         for k in ("ElemToSide", "SideToElem", "Mf", "Mg", "Mh", "sJ", "NormVec", "TangVec1",
                   "TangVec2", "SurfElem"):
             setattr(p, k, mesh[k].ctypes.data)
+        p.nfvse, p.spacePropSweeps = int(bool(nfvse)), sweeps
+        p.alpha_max, p.alpha_min, p.spacePropFactor = alpha_max, alpha_min, prop_factor
+        p.computeAlpha = 1 if fixed_alpha is None else 3
+        p.shockBlendCoef = 0.0 if fixed_alpha is None else fixed_alpha
+        self.alpha = np.zeros(mesh["nElems"])
+        p.alpha = self.alpha.ctypes.data
         self._p = p
 
     @property
@@ -149,6 +165,16 @@ class Problem:
 
     def lserk_step(self, U, Ut_tmp, t, dt):
         lib().ora_lserk_step(self.ptr, U, Ut_tmp, t, dt)
+
+    def indicator(self, U):
+        eta = np.zeros(self.nElems)
+        lib().ora_persson_peraire(self.ptr, np.ascontiguousarray(U), eta)
+        return eta
+
+    def blending(self, U):
+        a = np.zeros(self.nElems)
+        lib().ora_blending_coefficient(self.ptr, np.ascontiguousarray(U), a)
+        return a
 
     def calc_dt(self, U, cfl, dfl=0.0):
         err = C.c_int(0)
@@ -211,3 +237,17 @@ def dg_operators(N):
     lib().ora_dg_operators(N, x, w, D, Dh, Ds, Lm, Lp)
     f = lambda a: a.reshape(n, n).T.copy()  # column-major -> A[r, c]
     return dict(x=x, w=w, D=f(D), D_hat=f(Dh), Dvolsurf=f(Ds), L_hatminus=Lm, L_hatplus=Lp)
+
+
+def subcell_metrics(mesh, e):
+    """Sub-cell face metrics of element e: (3, N, n, n, 10) = [d][f][b][a][nv t1 t2 norm]."""
+    N = mesh["N"]
+    n = N + 1
+    out = np.zeros((3, N, n, n, 10))
+    lib().ora_subcell_metrics(N, np.ascontiguousarray(mesh["Mf"][e]), np.ascontiguousarray(mesh["Mg"][e]),
+                              np.ascontiguousarray(mesh["Mh"][e]), out)
+    return out
+
+
+def nfvse_threshold(N):
+    return lib().ora_nfvse_threshold(N)

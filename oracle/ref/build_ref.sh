@@ -5,7 +5,8 @@
 # reported CPU baseline; the product never links or calls them.
 #
 # The reference's CMake build is not used.  The defines below are the ones its CMakeLists.txt
-# sets for the named configurations (LGL nodes, split form, no MPI/AMR/shock capturing).  The
+# sets for the named configurations (LGL nodes, split form, no MPI/AMR; shock capturing only in
+# the NFVSE variant).  The
 # two-point volume flux is fixed at compile time (FLUXO_EQN_VOLFLUX, CMakeLists.txt:242-245):
 # flang's preprocessor rejects the run-time default's "#if PP_VolFlux==-1".  Translation units
 # that need HDF5, the HOPR mesh reader, restart or the time-disc driver do not compile here and are not needed by the harness; the
@@ -18,7 +19,7 @@ OUT=$(cd "$HERE/.." && pwd)/_ref
 FC=${FC:-flang}
 [ -d "$REF" ] || { echo "reference sources not found at $REF; nothing built"; exit 0; }
 mkdir -p "$OUT"
-COMMON="-cpp -O2 -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0 -I$REF"
+COMMON="-cpp -O2 -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -I$REF"
 
 build_variant() {
   local name=$1; shift
@@ -49,6 +50,9 @@ build_variant() {
 }
 
 NS="-DPP_nVar=5 -Dnavierstokes -DPP_entropy_vars_exist -DPP_u_aux_exist"
-build_variant euler_vf32 $NS -DPP_VolFlux=32 -DPARABOLIC=0
-build_variant euler_vf5 $NS -DPP_VolFlux=5 -DPARABOLIC=0
-build_variant ns_br1_vf32 $NS -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
+NOSC="-DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0"
+build_variant euler_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=0
+build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
+build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
+# NFVSE shock capturing (FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE, CMakeLists.txt:316-331)
+build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0

@@ -40,6 +40,10 @@ program fluxo_ref_harness
   use MOD_Analyze_Vars,        only: wGPVol
   use MOD_AnalyzeEquation,     only: DefineParametersAnalyzeEquation, InitAnalyzeEquation, AnalyzeEquation
   use MOD_Vector,              only: VCopy, VAXPBY
+#if SHOCK_NFVSE
+  use MOD_ShockCapturing,      only: DefineParametersShockCapturing, InitShockCapturing
+  use MOD_NFVSE_Vars,          only: alpha
+#endif
   implicit none
   character(len=255) :: fpar
   character(len=512) :: fin, fout
@@ -60,6 +64,9 @@ program fluxo_ref_harness
   call DefineParametersAnalyzeEquation()
 #if PARABOLIC
   call DefineParametersLifting()
+#endif
+#if SHOCK_NFVSE
+  call DefineParametersShockCapturing()
 #endif
   call prms%read_options(fpar)
 
@@ -130,6 +137,9 @@ program fluxo_ref_harness
 #endif
   call InitTestcase()
   call InitAnalyzeEquation()
+#if SHOCK_NFVSE
+  call InitShockCapturing()
+#endif
   read(u_in) U
   close(u_in)
 
@@ -144,6 +154,9 @@ program fluxo_ref_harness
   write(u_out) Ut
 #if PARABOLIC
   write(u_out) gradPx, gradPy, gradPz
+#endif
+#if SHOCK_NFVSE
+  write(u_out) alpha
 #endif
   allocate(wGPVol(0:PP_N, 0:PP_N, 0:PP_N))
   do k = 0, PP_N; do j = 0, PP_N; do i = 0, PP_N

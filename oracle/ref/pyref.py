@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 
-def binary(parabolic, volflux):
-    name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}_vf{volflux}"
+def binary(parabolic, volflux, nfvse=False):
+    name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
     path = os.path.join(REF_DIR, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} not built (oracle/ref/build_ref.sh)")
@@ -24,18 +24,26 @@ def binary(parabolic, volflux):
 
 
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
-        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600):
+        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None):
     """Metrics, VolInt, RHS (+ gradients), entropy, dt and an nsteps trajectory computed by
-    the reference's code on the given mesh (connectivity + NodeCoords) and state."""
+    the reference's code on the given mesh (connectivity + NodeCoords) and state.
+    nfvse: None, or dict of NFVSE parameters (ComputeAlpha, ShockBlendCoef, alpha_max,
+    alpha_min, SpacePropSweeps, SpacePropFactor) for the shock-capturing build, which also
+    returns the blending coefficient alpha of the RHS evaluation."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
-    exe = binary(parabolic, volflux)
+    exe = binary(parabolic, volflux, nfvse is not None)
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
             f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = 1\n"
                     f"IniRefState = 1\nRefState = (/1.,0.,0.,0.,1./)\nkappa = {kappa!r}\n"
                     f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\n")
+            if nfvse is not None:
+                f.write("ShockIndicator = 1\nShockIndicatorQuantity = DensityTimesPressure\n"
+                        "SubFVMethod = 1\nModalThreshold = 1\nTimeRelFactor = 0.0\n")
+                for k, v in nfvse.items():
+                    f.write(f"{k} = {v!r}\n")
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
             np.array([N, Ngeo, E, S, nsteps, int(parabolic), 0, 0], np.int32).tofile(f)
@@ -68,6 +76,8 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     if parabolic:
         for k in ("gradPx", "gradPy", "gradPz"):
             take(k, (E, n, n, n, 5))
+    if nfvse is not None:
+        take("alpha", (E,))
     take("dt0", (2,))
     out["errType0"] = int(out["dt0"][1])
     out["dt0"] = float(out["dt0"][0])

@@ -19,7 +19,20 @@ def load(name):
         d[k] = int(d[k])
     for k in ("mu0", "Pr", "kappa", "cfl", "dfl", "dt0", "t_end", "Entropy", "dSdU_Ut"):
         d[k] = float(d[k])
+    d["nfvse"] = int(d.get("nfvse", 0))
+    if d["nfvse"]:
+        d["sweeps"] = int(d["sweeps"])
+        for k in ("alpha_max", "alpha_min", "prop_factor", "fixed_alpha"):
+            d[k] = float(d[k])
     return d
+
+
+def nfvse_args(g):
+    """Problem/Engine keyword arguments of a shock-capturing case (empty otherwise)."""
+    if not g["nfvse"]:
+        return {}
+    return dict(nfvse=True, alpha_max=g["alpha_max"], alpha_min=g["alpha_min"], sweeps=g["sweeps"],
+                prop_factor=g["prop_factor"], fixed_alpha=None if g["fixed_alpha"] < 0 else g["fixed_alpha"])
 
 
 def mesh_args(g):

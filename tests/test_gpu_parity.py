@@ -223,3 +223,26 @@ def test_c1_density_wave_100_steps():
     print(f"C1 100 steps t={tg:.6f} relL2={err:.3e}")
     assert err <= TRAJ_TOL
     assert tg == pytest.approx(t, rel=1e-12)
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_volume_tiles_give_identical_steps(N):
+    """Every tile of the LSERK volume kernel (elements per workgroup) gives bitwise the same
+    trajectory, and the tuner's choice is one of them."""
+    m = O.box_mesh(3, N, x0=(-np.pi,) * 3, length=(2 * np.pi,) * 3, Ngeo=3, warp=True)
+    U0 = tgv(m)
+    kw = dict(volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72)
+    e = engine(m, **kw)
+    e.upload(U0)
+    best, times = e.tune_volume(reps=3)
+    assert best in times and len(times) == 3
+    assert rel_l2(e.download(), U0) == 0.0          # the tuner restores the state
+    ref = None
+    for epb in times:
+        e.upload(U0)
+        e.use_volume_tile(epb)
+        e.run(3, 0.9, 0.9)
+        U = e.download()
+        if ref is None:
+            ref = U
+        assert np.array_equal(U, ref), epb

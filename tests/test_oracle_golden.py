@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from golden_cases import initial_state, load, mesh_args, names, rel_l2
+from golden_cases import initial_state, load, mesh_args, names, nfvse_args, rel_l2
 
 
 @pytest.mark.parametrize("name", names())
@@ -13,7 +13,7 @@ def test_oracle_matches_reference(name):
     m = O.box_mesh(g["K"], g["N"], **mesh_args(g))
     U = initial_state(g, O.exactfunc, m)
     P = O.Problem(m, volflux=g["volflux"], riemann=g["riemann"], parabolic=bool(g["parabolic"]),
-                  mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"])
+                  mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"], **nfvse_args(g))
     if "Mf" in g:
         assert np.abs(m["Mf"] - g["Mf"]).max() <= 1e-15 * np.abs(g["Mf"]).max()
         assert np.abs(m["sJ"] - g["sJ"]).max() <= 1e-15 * np.abs(g["sJ"]).max()
@@ -21,6 +21,8 @@ def test_oracle_matches_reference(name):
         assert rel_l2(P.volint(U), g["volint"]) <= 1e-14
     Ut = P.rhs(U)
     assert rel_l2(Ut, g["Ut"]) <= 1e-14
+    if g["nfvse"]:
+        assert np.array_equal(P.alpha, g["alpha"])
     if g["parabolic"]:
         for k, gr in zip(("gradPx", "gradPy", "gradPz"), P.lifting(U)):
             assert rel_l2(gr, g[k]) <= 1e-12
