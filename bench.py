@@ -219,6 +219,46 @@ def serving_leg(n_req, nsteps, seed, load):
     return out
 
 
+def sedov_leg(K=40, N=3, steps=300, cfl=0.5):
+    """Euler with NFVSE shock capturing: Gaussian pressure pulse (fluxo_amd.workloads.sedov_blast:
+    sigma 0.06, E0 1, rho 1, p0 1e-5) on a periodic Cartesian K^3 mesh of [-1.5,1.5]^3, N=3,
+    Ranocha + LLF, Persson-Peraire on rho*p, alpha_max 0.5, alpha_min 0.01, one propagation
+    sweep with factor 0.5, first-order sub-cell FV; `steps` LSERK steps at CFL `cfl` timed
+    from the initial state (a separate short run warms the kernels and the graph up)."""
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import Engine, FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV, FX_KERNEL_VOLUME_RK
+    from fluxo_amd.workloads import sedov_blast
+    m = box_mesh(K, N, x0=(-1.5,) * 3, length=(3.0,) * 3)
+    U0 = sedov_blast(m["Elem_xGP"])
+    kw = dict(volume_flux=32, riemann=1, nfvse=True, alpha_max=0.5, alpha_min=0.01, sweeps=1,
+              prop_factor=0.5)
+    warm = Engine(m, **kw)
+    warm.upload(U0)
+    warm.run(3, cfl)
+    warm.close()
+    e = Engine(m, **kw)
+    e.upload(U0)
+    e.sync()
+    t0 = time.perf_counter()
+    t_end, dt_last, err = e.run(steps, cfl)
+    e.sync()
+    el = time.perf_counter() - t0
+    a = e.blending()
+    U = e.download()
+    ms = {"nfvse_alpha": e.profile_kernel(FX_KERNEL_NFVSE_ALPHA, 10),
+          "nfvse_fv": e.profile_kernel(FX_KERNEL_NFVSE_FV, 10),
+          "volume": e.profile_kernel(FX_KERNEL_VOLUME_RK, 10)}
+    e.close()
+    nodes = m["nElems"] * (N + 1) ** 3
+    p = 0.4 * (U[..., 4] - 0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0])
+    return {"workload": f"Euler + NFVSE Sedov-type pulse, {K}^3 periodic Cartesian hexes of [-1.5,1.5]^3, "
+                        f"N={N}, {steps} steps CFL {cfl}",
+            "value": nodes * 5 * steps / el, "unit": "DOF-updates/s", "ms_per_step": el / steps * 1e3,
+            "errType": err, "t_end": t_end, "rho_min": float(U[..., 0].min()), "p_min": float(p.min()),
+            "elements_blended_at_end": int((a > 0).sum()), "elements": int(m["nElems"]),
+            "kernel_ms": {k: round(v, 4) for k, v in ms.items()}}
+
+
 def _sample_mesh_and_state(K):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -292,6 +332,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="Poisson arrival rate as a fraction of the offline request throughput")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-sedov", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU element partitioning is a design "
@@ -358,6 +399,7 @@ def main():
             "kernel": "k_volume<N=3,Ranocha,BR1,RK>", "launch_ms": round(ms["volume"], 4),
             "bytes_per_launch": vbytes}
     t_end, dt_last = r64["t_end"], r64["dt_last"]
+    sedov = None if args.no_sedov else sedov_leg()
     serving = None if args.no_serving else serving_leg(args.serve_requests, args.serve_steps, 2026,
                                                           args.serve_load)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
@@ -376,6 +418,7 @@ def main():
         "pid_s_per_dof_timestep": 5.0 / value,
         "fp32_split": fp32,
         "serving": serving,
+        "shock_capturing": sedov,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -358,7 +358,9 @@ int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
   if (rc) return rc;
   if (reps < 1 || !avg_ms) return set_err(FX_EINVAL, "reps < 1 or null avg_ms");
   if (which == FX_KERNEL_LIFTING && !c->b.cfg.parabolic) return set_err(FX_EINVAL, "lifting needs parabolic=1");
-  if (which < 0 || which > 4) return set_err(FX_EINVAL, "unknown kernel %d", which);
+  if (which < 0 || which > 6) return set_err(FX_EINVAL, "unknown kernel %d", which);
+  if ((which == FX_KERNEL_NFVSE_ALPHA || which == FX_KERNEL_NFVSE_FV) && !c->b.cfg.nfvse)
+    return set_err(FX_EINVAL, "NFVSE kernels need nfvse=1");
   // make the inputs of the timed kernel valid: viscous fluxes, face fluxes and a time step
   if (c->b.cfg.parabolic) q_lifting(c->b, 1);
   q_surface(c->b);
@@ -367,6 +369,11 @@ int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
   RKArgs rk;
   rk.st = c->b.st; rk.eprob = nullptr; rk.A = c->b.rkA[1]; rk.b = c->b.rkb[1]; rk.first = 0;
   rk.alpha = nullptr; rk.Ffv = nullptr;
+  if (which == FX_KERNEL_NFVSE_FV) {  // the FV kernel reads the blending coefficient
+    const double *a = q_nfvse_alpha(c->b);
+    if (a != c->b.alpha)
+      HIPCHK(hipMemcpyAsync(c->b.alpha, a, (size_t)c->b.cfg.nElems * 8, hipMemcpyDeviceToDevice, c->b.stream));
+  }
   if (which == FX_KERNEL_VOLUME_RK) {  // Ut_tmp must hold a previous stage: run stage 1 once
     RKArgs r1 = rk; r1.A = 0.0; r1.b = c->b.rkb[0]; r1.first = 1;
     q_volume(c->b, VMODE_RK, r1);
@@ -381,6 +388,8 @@ int fx_profile_kernel(fx_ctx *c, int which, int reps, double *avg_ms)
     case FX_KERNEL_SURFACE: q_surface(c->b); break;
     case FX_KERNEL_LIFTING: q_lifting(c->b, 1); break;
     case FX_KERNEL_DT: q_dt(c->b, 0.9, 0.9, 0); break;
+    case FX_KERNEL_NFVSE_ALPHA: (void)q_nfvse_alpha(c->b); break;
+    case FX_KERNEL_NFVSE_FV: FX_DISPATCH_N(c->b.cfg.N, launch_nfvse_fv_n, c->b, c->b.alpha); break;
     default: q_volume(c->b, VMODE_RHS, rk); break;
     }
   }

@@ -79,7 +79,8 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
            "fx_use_volume_tile", "fx_blending"]
 
-FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS = range(5)
+(FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS,
+ FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV) = range(7)
 
 
 def lib():

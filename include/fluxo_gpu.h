@@ -139,7 +139,7 @@ int fx_synchronize(fx_ctx *ctx);
    and leaves U advanced by `reps` partial stages.  No counterpart in the reference; it
    gives the benchmark per-kernel durations for the roofline figures. */
 enum { FX_KERNEL_VOLUME_RK = 0, FX_KERNEL_SURFACE = 1, FX_KERNEL_LIFTING = 2, FX_KERNEL_DT = 3,
-       FX_KERNEL_VOLUME_RHS = 4 };
+       FX_KERNEL_VOLUME_RHS = 4, FX_KERNEL_NFVSE_ALPHA = 5, FX_KERNEL_NFVSE_FV = 6 };
 int fx_profile_kernel(fx_ctx *ctx, int which, int reps, double *avg_ms);
 
 /* Tile tuner of the volume kernel (no counterpart in the reference).  Times each compiled

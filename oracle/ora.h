@@ -119,6 +119,22 @@ void ora_persson_peraire(const ora_problem *P, const double *U, double *eta);
 void ora_blending_coefficient(const ora_problem *P, const double *U, double *alpha);
 void ora_volint_nfvse(const ora_problem *P, const double *U, const double *alpha, double *Ut);
 
+/* ---- ideal GLM-MHD (src/equation/mhd), 9 variables (rho, rho v, E, B, psi) ---- */
+typedef struct {
+  const ora_problem *P;   /* mesh and N (nVar, volflux, riemann of P are not used) */
+  double kappa, mu_0, glm_scale;
+} ora_mhd;
+void ora_mhd_derigs(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *mL,
+                    const double *mR, double *F);
+void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *nv,
+                     const double *t1, const double *t2, double *F);
+void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut);
+double ora_mhd_glm_dtch1(const ora_mhd *M, double CFLscale);
+double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double glm_dtch1,
+                             double *glm_ch, int *errType);
+void ora_mhd_lserk_step(const ora_mhd *M, double *U, double *Ut_tmp, double glm_ch, double dt);
+void ora_mhd_orszag_tang(const ora_mhd *M, int npts, const double *x, double *U);
+
 /* Carpenter RK4-5 coefficients and the LGL CFL/DFL alpha tables
    (src/timedisc/timedisc_vars.f90:79-90, 204-233) */
 void ora_rk_coefs(double *A, double *b, double *c);

@@ -1,0 +1,334 @@
+/*
+ * CPU oracle: ideal GLM-MHD split-form DGSEM on LGL nodes (9 variables: rho, rho v, E, B,
+ * psi), with the Derigs et al. entropy- and kinetic-energy-conserving two-point flux
+ * (VolumeFlux=10), Powell non-conservative terms (NONCONS=1), the Rusanov surface flux
+ * (Riemann=1), CALCTIMESTEP with the GLM cleaning speed from the time step, and the
+ * Carpenter RK4-5 step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ora.h).  Restates src/equation/mhd/{flux_average,flux,
+ * riemann,equation_vars,calctimestep}.f90 and the NONCONS branches of src/dg/{fillflux,
+ * surfint,volint}.f90 for periodic conforming meshes.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ora.h"
+
+#define MV 9
+
+typedef struct { double sr, v[3], pt, v2, B2, vB; } aux_t;
+
+/* EvalUaux (mhd/flux_average.f90:223-259) */
+static void mhd_uaux(const ora_mhd *M, const double *U, aux_t *a)
+{
+  const double km1 = M->kappa - 1.0, km2 = M->kappa - 2.0, s2mu0 = 0.5 * (1.0 / M->mu_0);
+  a->sr = 1.0 / U[0];
+  for (int d = 0; d < 3; d++) a->v[d] = U[1 + d] * a->sr;
+  a->v2 = a->v[0] * a->v[0] + a->v[1] * a->v[1] + a->v[2] * a->v[2];
+  a->B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+  a->vB = a->v[0] * U[5] + a->v[1] * U[6] + a->v[2] * U[7];
+  a->pt = km1 * (U[4] - 0.5 * U[0] * a->v2 - s2mu0 * U[8] * U[8]) - km2 * s2mu0 * a->B2;
+}
+
+/* LN_MEAN of mhd/flux_average.f90:1091-1111 is the Navier-Stokes one (ora_ln_mean) */
+#define ln_mean ora_ln_mean
+
+/* EntropyAndKinEnergyConservingFluxVec_Derigs (mhd/flux_average.f90:761-848) */
+void ora_mhd_derigs(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *mL,
+                    const double *mR, double *F)
+{
+  aux_t aL, aR;
+  mhd_uaux(M, UL, &aL);
+  mhd_uaux(M, UR, &aR);
+  const double smu0 = 1.0 / M->mu_0, s2mu0 = 0.5 * smu0, skm1 = 1.0 / (M->kappa - 1.0);
+  double m[3];
+  for (int d = 0; d < 3; d++) m[d] = 0.5 * (mL[d] + mR[d]);
+  const double betaL = 0.5 * UL[0] / (aL.pt - s2mu0 * aL.B2), betaR = 0.5 * UR[0] / (aR.pt - s2mu0 * aR.B2);
+  const double rhoLN = ln_mean(UL[0], UR[0]), betaLN = ln_mean(betaL, betaR);
+  double vA[3], BA[3];
+  for (int d = 0; d < 3; d++) { vA[d] = 0.5 * (aL.v[d] + aR.v[d]); BA[d] = 0.5 * (UL[5 + d] + UR[5 + d]); }
+  const double pTilde = 0.5 * ((UL[0] + UR[0]) / (betaL + betaR) + smu0 * 0.5 * (aL.B2 + aR.B2));
+  const double vm = vA[0] * m[0] + vA[1] * m[1] + vA[2] * m[2];
+  const double Bm = BA[0] * m[0] + BA[1] * m[1] + BA[2] * m[2];
+  const double psiA = 0.5 * (UL[8] + UR[8]);
+  F[0] = rhoLN * vm;
+  for (int d = 0; d < 3; d++) F[1 + d] = F[0] * vA[d] - (smu0 * Bm) * BA[d] + pTilde * m[d];
+  for (int d = 0; d < 3; d++) F[5 + d] = vm * BA[d] - Bm * vA[d] + (glm_ch * psiA) * m[d];
+  F[8] = glm_ch * Bm;
+  double vF = vA[0] * F[1] + vA[1] * F[2] + vA[2] * F[3];
+  double BF = BA[0] * F[5] + BA[1] * F[6] + BA[2] * F[7];
+  double B2vm = 0.0, psiBm = 0.0;
+  for (int d = 0; d < 3; d++) {
+    B2vm += (aL.B2 * aL.v[d] + aR.B2 * aR.v[d]) * m[d];
+    psiBm += (UL[8] * UL[5 + d] + UR[8] * UR[5 + d]) * m[d];
+  }
+  F[4] = F[0] * 0.5 * (skm1 / betaLN - 0.5 * (aL.v2 + aR.v2)) + vF +
+         smu0 * (BF - 0.25 * B2vm + 0.5 * (aL.vB + aR.vB) * Bm + F[8] * psiA - (glm_ch * 0.5) * psiBm);
+}
+
+/* EvalAdvectionFlux1D (mhd/flux.f90:391-442), x-direction of a rotated state */
+static void adv_flux_1d(const ora_mhd *M, double glm_ch, const double *U, double *F)
+{
+  const double km1 = M->kappa - 1.0, km2 = M->kappa - 2.0, smu0 = 1.0 / M->mu_0, s2mu0 = 0.5 * smu0;
+  const double sr = 1.0 / U[0], v1 = U[1] * sr, v2 = U[2] * sr, v3 = U[3] * sr;
+  const double b1 = U[5], b2 = U[6], b3 = U[7];
+  const double Et = U[4] - s2mu0 * U[8] * U[8];
+  const double vb = b1 * v1 + b2 * v2 + b3 * v3;
+  const double pt = km1 * (Et - 0.5 * U[0] * (v1 * v1 + v2 * v2 + v3 * v3)) - km2 * s2mu0 * (b1 * b1 + b2 * b2 + b3 * b3);
+  F[0] = U[1];
+  F[1] = U[1] * v1 + pt - smu0 * b1 * b1;
+  F[2] = U[1] * v2 - smu0 * b1 * b2;
+  F[3] = U[1] * v3 - smu0 * b1 * b3;
+  F[4] = (Et + pt) * v1 - smu0 * b1 * vb;
+  F[5] = 0.0;
+  F[6] = v1 * b2 - b1 * v2;
+  F[7] = v1 * b3 - b1 * v3;
+  F[4] = F[4] + smu0 * glm_ch * b1 * U[8];
+  F[5] = F[5] + glm_ch * U[8];
+  F[8] = glm_ch * b1;
+}
+
+/* ConsToPrim (mhd/equation_vars.f90:229-251) */
+static void cons_to_prim(const ora_mhd *M, const double *c, double *p)
+{
+  const double sr = 1.0 / c[0];
+  p[0] = c[0];
+  for (int d = 1; d < 4; d++) p[d] = c[d] * sr;
+  double b2 = 0.0;
+  for (int v = 5; v < MV; v++) b2 += c[v] * c[v];
+  p[4] = (M->kappa - 1.0) * (c[4] - 0.5 * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]) - 0.5 * (1.0 / M->mu_0) * b2);
+  for (int v = 5; v < MV; v++) p[v] = c[v];
+}
+
+/* FastestWave1D (mhd/equation_vars.f90:663-684), rotated primitive state */
+static double fastest_wave_1d(const ora_mhd *M, const double *p)
+{
+  const double sr = 1.0 / p[0], smu0 = 1.0 / M->mu_0;
+  const double c2 = M->kappa * p[4] * sr;
+  const double ca2 = p[5] * p[5] * sr * smu0;
+  const double va2 = (p[6] * p[6] + p[7] * p[7]) * sr * smu0 + ca2;
+  const double astar = sqrt((c2 + va2) * (c2 + va2) - 4.0 * c2 * ca2);
+  return sqrt(0.5 * (c2 + va2 + astar));
+}
+
+/* AdvRiemann (mhd/riemann.f90:174-246) with RiemannSolverByRusanov (:273-308) */
+void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *nv,
+                     const double *t1, const double *t2, double *F)
+{
+  double cL[MV], cR[MV], FL[MV], FR[MV], pL[MV], pR[MV], f[MV];
+  memcpy(cL, UL, sizeof(cL));
+  memcpy(cR, UR, sizeof(cR));
+  for (int b = 1; b <= 5; b += 4) {  /* momentum (1:3) and B (5:7) */
+    cL[b] = UL[b] * nv[0] + UL[b + 1] * nv[1] + UL[b + 2] * nv[2];
+    cL[b + 1] = UL[b] * t1[0] + UL[b + 1] * t1[1] + UL[b + 2] * t1[2];
+    cL[b + 2] = UL[b] * t2[0] + UL[b + 1] * t2[1] + UL[b + 2] * t2[2];
+    cR[b] = UR[b] * nv[0] + UR[b + 1] * nv[1] + UR[b + 2] * nv[2];
+    cR[b + 1] = UR[b] * t1[0] + UR[b + 1] * t1[1] + UR[b + 2] * t1[2];
+    cR[b + 2] = UR[b] * t2[0] + UR[b + 1] * t2[1] + UR[b + 2] * t2[2];
+  }
+  adv_flux_1d(M, glm_ch, cL, FL);
+  adv_flux_1d(M, glm_ch, cR, FR);
+  cons_to_prim(M, cL, pL);
+  cons_to_prim(M, cR, pR);
+  const double cfL = fastest_wave_1d(M, pL), cfR = fastest_wave_1d(M, pR);
+  const double lam = fmax(fabs(pL[1]) + cfL, fabs(pR[1]) + cfR);
+  for (int v = 0; v < MV; v++) f[v] = 0.5 * ((FL[v] + FR[v]) - lam * (cR[v] - cL[v]));
+  for (int v = 0; v < MV; v++) F[v] = f[v];
+  for (int b = 1; b <= 5; b += 4)
+    for (int d = 0; d < 3; d++) F[b + d] = nv[d] * f[b] + t1[d] * f[b + 1] + t2[d] * f[b + 2];
+}
+
+/* AddNonConsFlux, Powell (mhd/riemann.f90:230-267): FL(2:8) += 1/2 (B_L+B_R).n (B_L, v_L.B_L, v_L) */
+static void add_noncons_surface(const double *UL, const double *UR, const double *nv, double *F)
+{
+  double vL[3];
+  for (int d = 0; d < 3; d++) vL[d] = UL[1 + d] / UL[0];
+  const double bn = 0.5 * ((UL[5] + UR[5]) * nv[0] + (UL[6] + UR[6]) * nv[1] + (UL[7] + UR[7]) * nv[2]);
+  const double phi[7] = {UL[5], UL[6], UL[7], UL[5] * vL[0] + UL[6] * vL[1] + UL[7] * vL[2], vL[0], vL[1], vL[2]};
+  for (int v = 0; v < 7; v++) F[1 + v] = F[1 + v] + bn * phi[v];
+}
+
+/* DGTimeDerivative_weakForm (dg.f90:282-452) for ideal GLM-MHD with NONCONS=1 on LGL
+   periodic conforming meshes: VolInt_SplitForm with EvalAdvFluxAverage3D +
+   AddNonConsFluxTilde3D, FillFlux (Riemann + AddNonConsFlux on both sides), SurfInt, -sJ. */
+void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
+{
+  const ora_problem *P = M->P;
+  const int N = P->N, n = N + 1, n3 = n * n * n;
+  double x[16], w[16], D[256], Dh[256], Ds[256], Lm[16], Lp[16];
+  ora_dg_operators(N, x, w, D, Dh, Ds, Lm, Lp);  /* Ds[i + n*l] = Dvolsurf(i,l) */
+  const size_t E = P->nElems;
+  memset(Ut, 0, sizeof(double) * MV * n3 * E);
+  double *ft = malloc(sizeof(double) * MV * n * n3);
+  const double *Met[3] = {P->Mf, P->Mg, P->Mh};
+  for (size_t e = 0; e < E; e++) {
+    const double *Ue = U + MV * n3 * e;
+    double *ute = Ut + MV * n3 * e;
+    for (int d = 0; d < 3; d++) {
+      const double *Me = Met[d] + 3 * n3 * e;
+      /* ft[(l, node)] = flux between node and the node l positions along direction d */
+      for (int node = 0; node < n3; node++) {
+        const int i = node % n, j = (node / n) % n, k = node / (n * n);
+        const int a = (d == 0) ? i : (d == 1 ? j : k);
+        for (int l = 0; l < n; l++) {
+          const int nl = (d == 0) ? (l + n * (j + n * k)) : (d == 1 ? (i + n * (l + n * k)) : (i + n * (j + n * l)));
+          double *f = ft + MV * (l + n * node);
+          /* the symmetric two-point flux is evaluated with the lower line index as left state
+             and copied to the other ordering, as EvalAdvFluxAverage3D does */
+          if (l == a) { for (int v = 0; v < MV; v++) f[v] = 0.0; }
+          else if (a < l) ora_mhd_derigs(M, glm_ch, Ue + MV * node, Ue + MV * nl, Me + 3 * node, Me + 3 * nl, f);
+          else ora_mhd_derigs(M, glm_ch, Ue + MV * nl, Ue + MV * node, Me + 3 * nl, Me + 3 * node, f);
+          /* AddNonConsFluxTilde3D (mhd/flux_average.f90:268-366), Powell, at node `node` */
+          const double *ui = Ue + MV * node, *ul = Ue + MV * nl, *mi = Me + 3 * node, *ml = Me + 3 * nl;
+          double vi[3];
+          const double sri = 1.0 / ui[0];
+          for (int c = 0; c < 3; c++) vi[c] = ui[1 + c] * sri;
+          const double phi[7] = {0.5 * ui[5], 0.5 * ui[6], 0.5 * ui[7],
+                                 0.5 * (vi[0] * ui[5] + vi[1] * ui[6] + vi[2] * ui[7]),
+                                 0.5 * vi[0], 0.5 * vi[1], 0.5 * vi[2]};
+          const double bhat = mi[0] * ui[5] + mi[1] * ui[6] + mi[2] * ui[7];
+          const double bl = 0.5 * (mi[0] + ml[0]) * ul[5] + 0.5 * (mi[1] + ml[1]) * ul[6] + 0.5 * (mi[2] + ml[2]) * ul[7];
+          for (int v = 0; v < 7; v++) f[1 + v] = f[1 + v] + phi[v] * (bhat + bl);
+        }
+      }
+      for (int node = 0; node < n3; node++) {
+        const int i = node % n, j = (node / n) % n, k = node / (n * n);
+        const int a = (d == 0) ? i : (d == 1 ? j : k);
+        for (int l = 0; l < n; l++)
+          for (int v = 0; v < MV; v++) ute[MV * node + v] += Ds[a + n * l] * ft[MV * (l + n * node) + v];
+      }
+    }
+  }
+  free(ft);
+  /* surface: FillFlux + SurfInt (LGL) */
+  const double L0 = Lm[0];
+  for (int s = 0; s < P->nSides; s++) {
+    const int *t = P->SideToElem + 5 * s;
+    const int em = t[0] - 1, es = t[1] - 1, lm = t[2], ls = t[3], fl = t[4];
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        int im[3], is[3];
+        ora_side_to_vol(N, 0, p, q, 0, lm, im);
+        ora_side_to_vol(N, 0, p, q, fl, ls, is);
+        const size_t om = MV * ((size_t)em * n3 + im[0] + n * (im[1] + n * im[2]));
+        const size_t os = MV * ((size_t)es * n3 + is[0] + n * (is[1] + n * is[2]));
+        const size_t f = (size_t)s * n * n + p + n * q;
+        const double *nv = P->NormVec + 3 * f, *t1 = P->TangVec1 + 3 * f, *t2 = P->TangVec2 + 3 * f;
+        double Fm[MV], Fs[MV];
+        ora_mhd_riemann(M, glm_ch, U + om, U + os, nv, t1, t2, Fm);
+        memcpy(Fs, Fm, sizeof(Fs));
+        add_noncons_surface(U + om, U + os, nv, Fm);
+        add_noncons_surface(U + os, U + om, nv, Fs);
+        for (int v = 0; v < MV; v++) {
+          Ut[om + v] += (Fm[v] * P->SurfElem[f]) * L0;
+          Ut[os + v] -= (Fs[v] * P->SurfElem[f]) * L0;
+        }
+      }
+  }
+  for (size_t p = 0; p < (size_t)n3 * E; p++) {
+    const double sj = -P->sJ[p];
+    for (int v = 0; v < MV; v++) Ut[MV * p + v] = Ut[MV * p + v] * sj;
+  }
+}
+
+/* InitTimeStep_GLM (mhd/calctimestep.f90:186-215): dt of a unit wave speed */
+double ora_mhd_glm_dtch1(const ora_mhd *M, double CFLscale)
+{
+  const ora_problem *P = M->P;
+  const int n = P->N + 1, n3 = n * n * n;
+  const double cfl = CFLscale * ora_cfl_alpha(P->N) / (2.0 * P->N + 1.0);
+  double dtch1 = 1.79e308;
+  const double *Met[3] = {P->Mf, P->Mg, P->Mh};
+  for (int e = 0; e < P->nElems; e++) {
+    double lam[3] = {0, 0, 0};
+    for (int p = 0; p < n3; p++) {
+      const size_t c = (size_t)e * n3 + p;
+      for (int d = 0; d < 3; d++) {
+        const double *m = Met[d] + 3 * c;
+        lam[d] = fmax(lam[d], P->sJ[c] * sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+      }
+    }
+    dtch1 = fmin(dtch1, cfl * 2.0 / (lam[0] + lam[1] + lam[2]));
+  }
+  return dtch1;
+}
+
+/* CALCTIMESTEP (mhd/calctimestep.f90:39-179), ideal MHD: returns dt; *glm_ch receives
+   GLM_scale * GLM_dtch1 / dt. */
+double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double glm_dtch1,
+                             double *glm_ch, int *errType)
+{
+  const ora_problem *P = M->P;
+  const int n = P->N + 1, n3 = n * n * n;
+  const double cfl = CFLscale * ora_cfl_alpha(P->N) / (2.0 * P->N + 1.0);
+  double dt = 1.79e308;
+  int err = 0;
+  const double *Met[3] = {P->Mf, P->Mg, P->Mh};
+  for (int e = 0; e < P->nElems && !err; e++) {
+    double lam[3] = {0, 0, 0};
+    for (int p = 0; p < n3; p++) {
+      const size_t c = (size_t)e * n3 + p;
+      const double *u = U + MV * c;
+      if (isnan(u[0])) err = 1;
+      double prim[MV];
+      cons_to_prim(M, u, prim);
+      if (prim[4] <= 0.0) err = 2;
+      const double cf = sqrt((M->kappa * prim[4] + (prim[5] * prim[5] + prim[6] * prim[6] + prim[7] * prim[7]) * (1.0 / M->mu_0)) / prim[0]);
+      for (int d = 0; d < 3; d++) {
+        const double *m = Met[d] + 3 * c;
+        lam[d] = fmax(lam[d], P->sJ[c] * (fabs(m[0] * prim[1] + m[1] * prim[2] + m[2] * prim[3]) +
+                                          cf * sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2])));
+      }
+    }
+    const double dte = cfl * 2.0 / (lam[0] + lam[1] + lam[2]);
+    dt = fmin(dt, dte);
+    if (isnan(dt)) err = 3;
+  }
+  if (errType) *errType = err;
+  if (glm_ch) *glm_ch = M->glm_scale * (glm_dtch1 / dt);
+  return dt;
+}
+
+/* TimeStepByLSERKW2 (timedisc.f90:424-480) with the cleaning speed of this step */
+void ora_mhd_lserk_step(const ora_mhd *M, double *U, double *Ut_tmp, double glm_ch, double dt)
+{
+  const ora_problem *P = M->P;
+  const size_t nt = (size_t)MV * (P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  double A[5], b[5], c[5];
+  ora_rk_coefs(A, b, c);
+  double *Ut = malloc(sizeof(double) * nt);
+  for (int s = 0; s < 5; s++) {
+    ora_mhd_rhs(M, U, glm_ch, Ut);
+    for (size_t i = 0; i < nt; i++) {
+      Ut_tmp[i] = (s == 0) ? Ut[i] : Ut_tmp[i] * (-A[s]) + Ut[i];
+      U[i] = U[i] + Ut_tmp[i] * (b[s] * dt);
+    }
+  }
+  free(Ut);
+}
+
+/* ExactFunc 312, 3D Orszag-Tang-type vortex (mhd/equation.f90:1257-1266) with PrimToCons
+   (mhd/equation_vars.f90:253-266) */
+void ora_mhd_orszag_tang(const ora_mhd *M, int npts, const double *x, double *U)
+{
+  const double pi = acos(-1.0), k = M->kappa;
+  for (int p = 0; p < npts; p++) {
+    const double *xp = x + 3 * p;
+    double prim[MV] = {0};
+    prim[0] = 1.0;
+    prim[1] = -(1.0 + 0.2 * sin(2.0 * pi * xp[2])) * sin(2.0 * pi * xp[1]);
+    prim[2] = (1.0 + 0.2 * sin(2.0 * pi * xp[2])) * sin(2.0 * pi * xp[0]);
+    prim[3] = 0.2 * sin(2.0 * pi * xp[2]);
+    prim[4] = 1.0 / k;
+    prim[5] = -sin(2.0 * pi * xp[1]) / k;
+    prim[6] = sin(4.0 * pi * xp[0]) / k;
+    double *c = U + MV * p;
+    c[0] = prim[0];
+    for (int d = 1; d < 4; d++) c[d] = prim[d] * prim[0];
+    double b2 = 0.0;
+    for (int v = 5; v < MV; v++) b2 += prim[v] * prim[v];
+    c[4] = (1.0 / (k - 1.0)) * prim[4] + 0.5 * (c[1] * prim[1] + c[2] * prim[2] + c[3] * prim[3]) + 0.5 * (1.0 / M->mu_0) * b2;
+    for (int v = 5; v < MV; v++) c[v] = prim[v];
+  }
+}

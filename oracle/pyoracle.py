@@ -58,6 +58,16 @@ def lib():
         L.ora_rk_coefs.argtypes = [_dp, _dp, _dp]
         L.ora_lgl_nodes.argtypes = [C.c_int, _dp, _dp]
         L.ora_dg_operators.argtypes = [C.c_int] + [_dp] * 7
+        L.ora_mhd_rhs.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ora_mhd_glm_dtch1.restype = C.c_double
+        L.ora_mhd_glm_dtch1.argtypes = [C.c_void_p, C.c_double]
+        L.ora_mhd_calc_timestep.restype = C.c_double
+        L.ora_mhd_calc_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.ora_mhd_lserk_step.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double]
+        L.ora_mhd_orszag_tang.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.ora_mhd_derigs.argtypes = [C.c_void_p, C.c_double] + [_dp] * 5
+        L.ora_mhd_riemann.argtypes = [C.c_void_p, C.c_double] + [_dp] * 6
         L.ora_nfvse_threshold.restype = C.c_double
         L.ora_nfvse_threshold.argtypes = [C.c_int]
         L.ora_subcell_metrics.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
@@ -251,3 +261,56 @@ def subcell_metrics(mesh, e):
 
 def nfvse_threshold(N):
     return lib().ora_nfvse_threshold(N)
+
+
+class _Mhd(C.Structure):
+    _fields_ = [("P", C.c_void_p), ("kappa", C.c_double), ("mu_0", C.c_double), ("glm_scale", C.c_double)]
+
+
+class MhdProblem:
+    """Ideal GLM-MHD setup (9 variables) on a mesh: Derigs volume flux, Rusanov surface
+    flux, Powell non-conservative terms.  The cleaning speed GLM_ch is explicit state: 0.5
+    before the first time step (the reference's initial value), then
+    glm_scale * dtch1 / dt from every CALCTIMESTEP."""
+
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5):
+        self.base = Problem(mesh, volflux=10, riemann=1, kappa=kappa)
+        self.mesh, self.N, self.nElems = mesh, mesh["N"], mesh["nElems"]
+        m = _Mhd()
+        m.P = C.cast(C.byref(self.base._p), C.c_void_p).value
+        m.kappa, m.mu_0, m.glm_scale = kappa, mu_0, glm_scale
+        self._m = m
+        self.glm_ch = 0.5
+        self.glm_dtch1 = None
+
+    @property
+    def ptr(self):
+        return C.byref(self._m)
+
+    def shape(self):
+        n = self.N + 1
+        return (self.nElems, n, n, n, 9)
+
+    def exact_orszag_tang(self):
+        x = np.ascontiguousarray(self.mesh["Elem_xGP"])
+        U = np.zeros(self.shape())
+        lib().ora_mhd_orszag_tang(self.ptr, x.size // 3, x, U)
+        return U
+
+    def rhs(self, U, glm_ch=None):
+        Ut = np.zeros(self.shape())
+        lib().ora_mhd_rhs(self.ptr, np.ascontiguousarray(U), self.glm_ch if glm_ch is None else glm_ch, Ut)
+        return Ut
+
+    def calc_dt(self, U, cfl):
+        """CALCTIMESTEP; updates glm_ch (and dtch1 on the first call) like the reference."""
+        if self.glm_dtch1 is None:
+            self.glm_dtch1 = lib().ora_mhd_glm_dtch1(self.ptr, cfl)
+        ch, err = C.c_double(0.0), C.c_int(0)
+        dt = lib().ora_mhd_calc_timestep(self.ptr, np.ascontiguousarray(U), cfl, self.glm_dtch1,
+                                         C.byref(ch), C.byref(err))
+        self.glm_ch = ch.value
+        return dt, err.value
+
+    def lserk_step(self, U, Ut_tmp, dt):
+        lib().ora_mhd_lserk_step(self.ptr, U, Ut_tmp, self.glm_ch, dt)

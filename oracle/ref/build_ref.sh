@@ -27,7 +27,11 @@ build_variant() {
   local obj="$OUT/obj_$name"
   mkdir -p "$obj"
   # the lifting directory not selected by PP_Lifting is left out, as the reference's CMake does
-  local exclude='/amr/|/equation/(mhd|maxwell|linearscalaradvection)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/br2/'
+  local eqn=navierstokes
+  case "$defs" in *-Dmhd*) eqn=mhd ;; esac
+  local others='mhd|maxwell|linearscalaradvection'
+  [ "$eqn" = mhd ] && others='navierstokes|maxwell|linearscalaradvection'
+  local exclude="/amr/|/equation/($others)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/br2/"
   find "$REF" -name "*.f90" | grep -v -E "$exclude" | sort > "$obj/todo.txt"
   local pass n f b
   for pass in $(seq 1 15); do
@@ -54,5 +58,8 @@ NOSC="-DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0"
 build_variant euler_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=0
 build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
 build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
+# ideal GLM-MHD with Powell non-conservative terms (FLUXO_EQNSYSNAME=mhd, FLUXO_EQN_GLM,
+# FLUXO_EQN_NONCONS=Powell; CMakeLists.txt:174-209), Derigs flux (VolumeFlux=10)
+build_variant mhd_vf10 -DPP_nVar=9 -Dmhd -DPP_entropy_vars_exist -DPP_u_aux_exist -DPP_GLM=T -DNONCONS=1 $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
 # NFVSE shock capturing (FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE, CMakeLists.txt:316-331)
 build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0

@@ -44,6 +44,9 @@ program fluxo_ref_harness
   use MOD_ShockCapturing,      only: DefineParametersShockCapturing, InitShockCapturing
   use MOD_NFVSE_Vars,          only: alpha
 #endif
+#ifdef PP_GLM
+  use MOD_Equation_Vars,       only: GLM_ch, GLM_dtch1
+#endif
   implicit none
   character(len=255) :: fpar
   character(len=512) :: fin, fout
@@ -173,6 +176,9 @@ program fluxo_ref_harness
   allocate(dtElem(nElems))
   dt = CALCTIMESTEP(errType)
   write(u_out) dt, real(errType)
+#ifdef PP_GLM
+  write(u_out) GLM_ch, GLM_dtch1
+#endif
   ! nSteps of TimeStepByLSERKW2 with CALCTIMESTEP before each step
   allocate(Utmp, mold=U)
   allocate(b_dt(nRKStages))

@@ -15,8 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 
-def binary(parabolic, volflux, nfvse=False):
-    name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
+def binary(parabolic, volflux, nfvse=False, mhd=False):
+    if mhd:
+        name = f"fluxo_ref_mhd_vf{volflux}"
+    else:
+        name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
     path = os.path.join(REF_DIR, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} not built (oracle/ref/build_ref.sh)")
@@ -89,4 +92,61 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     out["Entropy"] = float(m.group(1)) if m else None
     m = re.search(r"dSdU\*Ut\s*:\s*([-+0-9.Ee]+)", p.stdout)
     out["dSdU_Ut"] = float(m.group(1)) if m else None
+    return out
+
+
+def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600):
+    """Ideal GLM-MHD (Derigs volume flux, Rusanov surface flux, Powell non-conservative
+    terms) with the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before
+    the first CALCTIMESTEP), dt with GLM_ch and GLM_dtch1 after it, and an nsteps trajectory.
+    U: (nElems, n, n, n, 9)."""
+    N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
+    n = N + 1
+    exe = binary(False, 10, mhd=True)
+    with tempfile.TemporaryDirectory() as d:
+        par = os.path.join(d, "parameter.ini")
+        with open(par, "w") as f:
+            f.write(f"N = {N}\nRiemann = 1\nVolumeFlux = 10\nIniExactFunc = 1\nIniRefState = 1\n"
+                    f"RefState = (/1.,0.,0.,0.,1.,0.,0.,0./)\nkappa = {kappa!r}\nmu_0 = {mu_0!r}\n"
+                    f"GLM_scale = {glm_scale!r}\n")
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([N, Ngeo, E, S, nsteps, 0, 0, 0], np.int32).tofile(f)
+            np.array([cfl, 0.0, t0, 0.0], np.float64).tofile(f)
+            np.ascontiguousarray(mesh["NodeCoords"], np.float64).tofile(f)
+            np.ascontiguousarray(mesh["ElemToSide"], np.int32).tofile(f)
+            np.ascontiguousarray(mesh["SideToElem"], np.int32).tofile(f)
+            np.ascontiguousarray(U, np.float64).tofile(f)
+        p = subprocess.run([exe, par, fin, fout], cwd=d, capture_output=True, text=True, errors="replace",
+                           timeout=timeout)
+        if p.returncode != 0 or not os.path.exists(fout):
+            raise RuntimeError(f"reference harness failed ({p.returncode}):\n{p.stdout[-2000:]}\n{p.stderr[-2000:]}")
+        raw = np.fromfile(fout, np.float64)
+    out, o = {}, 0
+
+    def take(name, shape):
+        nonlocal o
+        cnt = int(np.prod(shape))
+        out[name] = raw[o:o + cnt].reshape(shape)
+        o += cnt
+
+    for k in ("Mf", "Mg", "Mh"):
+        take(k, (E, n, n, n, 3))
+    take("sJ", (E, n, n, n))
+    for k in ("NormVec", "TangVec1", "TangVec2"):
+        take(k, (S, n, n, 3))
+    take("SurfElem", (S, n, n))
+    take("volint", (E, n, n, n, 9))
+    take("Ut", (E, n, n, n, 9))
+    take("dt0", (2,))
+    out["errType0"] = int(out["dt0"][1])
+    out["dt0"] = float(out["dt0"][0])
+    take("glm", (2,))
+    out["glm_ch0"], out["glm_dtch1"] = float(out["glm"][0]), float(out["glm"][1])
+    del out["glm"]
+    take("U_end", (E, n, n, n, 9))
+    take("tail", (2,))
+    out["t_end"], out["dt_last"] = float(out["tail"][0]), float(out["tail"][1])
+    del out["tail"]
+    out["stdout"] = p.stdout
     return out
