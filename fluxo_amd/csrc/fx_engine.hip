@@ -49,7 +49,17 @@ int check_config(const fx_config *c)
 {
   if (!c) return set_err(FX_EINVAL, "null config");
   if (c->N < 2 || c->N > 5) return set_err(FX_EINVAL, "N=%d not supported (2..5)", c->N);
-  if (c->nVar != 5) return set_err(FX_EINVAL, "nVar=%d not supported (5)", c->nVar);
+  if (c->nVar != 5 && c->nVar != 9) return set_err(FX_EINVAL, "nVar=%d not supported (5, 9 GLM-MHD)", c->nVar);
+  if (c->nVar == 9) {
+    if (c->volumeFlux != 10 || c->riemann != 1)
+      return set_err(FX_EINVAL, "GLM-MHD: VolumeFlux=10 (Derigs) and Riemann=1 (Rusanov) only");
+    if (c->parabolic || c->nfvse) return set_err(FX_EINVAL, "GLM-MHD: ideal (parabolic=0), no shock capturing");
+    if (!(c->kappa > 1.0) || !(c->mu_0 > 0.0) || !(c->GLM_scale > 0.0))
+      return set_err(FX_EINVAL, "GLM-MHD: kappa > 1, mu_0 > 0, GLM_scale > 0");
+    if (c->precision != FX_PREC_FP64 && c->precision != FX_PREC_FP32_SPLIT)
+      return set_err(FX_EINVAL, "precision %d not supported", c->precision);
+    return FX_OK;
+  }
   if (c->nElems < 1 || c->nSides < 1) return set_err(FX_EINVAL, "empty mesh");
   if (c->parabolic && c->lifting != 1) return set_err(FX_EINVAL, "only BR1 lifting (lifting=1)");
   const int rc = check_physics(c->volumeFlux, c->riemann, c->precision, c->kappa, c->Pr, c->parabolic);
@@ -76,6 +86,7 @@ long long fx_ctx_bytes(const fx_config *cfg)
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
   if (cfg->parabolic) b += 12LL * n3 * E * 8 + 15LL * n3 * E * 8;
   if (cfg->nfvse) b += 5LL * n3 * E * 8 + 2LL * E * 8;  // Ffv, alpha buffers
+  if (cfg->nVar == 9) b += 4LL * 3 * n3 * E * 8 + (9LL + 4) * n * n * S * 8 * 2 + E * 8;  // MHD extra
   b += 2LL * ((n3 * E + 255) / 256 + 1) * 8 + 64;
   return b;
 }
@@ -118,7 +129,11 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
     ALLOC(b.Mg32, 3 * n3E * 4);
     ALLOC(b.Mh32, 3 * n3E * 4);
   }
-  ALLOC(b.Flux, 5 * nf * 8);
+  ALLOC(b.Flux, (size_t)cfg->nVar * nf * 8);
+  if (cfg->nVar == 9) {
+    ALLOC(b.FluxS, (size_t)cfg->nVar * nf * 8);
+    ALLOC(b.dtch1Elem, (size_t)cfg->nElems * 8);
+  }
   ALLOC(b.nvse, nf * sizeof(double4));
   ALLOC(b.e2s, (size_t)12 * cfg->nElems * 4);
   ALLOC(b.s2e, (size_t)5 * cfg->nSides * 4);
@@ -140,6 +155,7 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
 #undef ALLOC
   DtState z{};
   z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
+  z.glm_init = 0; z.glm_ch = 0.5; z.glm_dtch1 = 0.0;
   if ((e = hipMemcpy(b.st, &z, sizeof(z), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "init");
   *out = c;
   return FX_OK;
@@ -153,7 +169,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Fv, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
-                  b.Ffv};
+                  b.Ffv, b.FluxS, b.dtch1Elem};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -338,10 +354,23 @@ int fx_run(fx_ctx *c, int nSteps, double CFLscale, double DFLscale, double *t, d
   return FX_OK;
 }
 
+int fx_glm(fx_ctx *c, double *glm_ch, double *glm_dtch1)
+{
+  if (!c) return set_err(FX_EINVAL, "null context");
+  if (!c->b.mhd()) return set_err(FX_ESTATE, "GLM state needs nVar=9");
+  DtState h;
+  HIPCHK(hipMemcpyAsync(&h, c->b.st, sizeof(h), hipMemcpyDeviceToHost, c->b.stream));
+  HIPCHK(hipStreamSynchronize(c->b.stream));
+  if (glm_ch) *glm_ch = h.glm_ch;
+  if (glm_dtch1) *glm_dtch1 = h.glm_dtch1;
+  return FX_OK;
+}
+
 int fx_entropy(fx_ctx *c, double *S, double *dSdU_Ut)
 {
   int rc = ready(c);
   if (rc) return rc;
+  if (c->b.mhd()) return set_err(FX_EINVAL, "CalcEntropy is implemented for Euler/Navier-Stokes");
   q_entropy(c);
   HIPCHK(hipGetLastError());
   double h[2];

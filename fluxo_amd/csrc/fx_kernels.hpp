@@ -30,6 +30,7 @@ struct Ops {
   float Ds32[MAXN + 1][MAXN + 1];   // FP32 copies for the split-precision volume integral
   float Dhat32[MAXN + 1][MAXN + 1];
   // NFVSE shock capturing
+  double DsRow[MAXN + 1];           // sum_l Dvolsurf(i,l) (non-conservative MHD terms)
   double sVdm[MAXN + 1][MAXN + 1];  // nodal -> Legendre modal (BuildLegendreVdm basis.f90:196-252)
   double Csub[MAXN + 1][MAXN + 1];  // Csub(f,m) = sum_{l<=f} w_l D(l,m): sub-cell face metrics
   double sw[MAXN + 1];              // 1/w_i (sWGP, nfvse.f90:219)
@@ -133,7 +134,9 @@ struct DtState {
   double dt;                   // dt of the current step
   double t;                    // simulation time
   int step;                    // steps done
-  int pad;
+  int glm_init;                // GLM-MHD: dtch1 computed (InitTimeStep_GLM done)
+  double glm_ch;               // GLM-MHD cleaning speed (0.5 before the first CALCTIMESTEP)
+  double glm_dtch1;            // GLM-MHD time step of a unit wave speed
 };
 
 // Element -> problem map of a batch of independent meshes sharing one launch: eprob[e] is

@@ -14,6 +14,7 @@
 #include "../../include/fluxo_gpu.h"
 #include "fx_kernels.hpp"
 #include "fx_nfvse.hpp"
+#include "fx_mhd.hpp"
 
 namespace fx {
 
@@ -185,6 +186,11 @@ inline void make_ops(int N, Ops &o)
       o.Csub[f][m] = c;
     }
   for (int i = 0; i < n; i++) o.sw[i] = 1.0 / w[i];
+  for (int i = 0; i < n; i++) {
+    double r = 0.0;
+    for (int l = 0; l < n; l++) r += o.Ds[i][l];
+    o.DsRow[i] = r;
+  }
 }
 
 // Operators of degree N into constant memory of the current device (once per process).
@@ -283,6 +289,18 @@ struct Batch {
   // NFVSE (cfg.nfvse): blending coefficient buffers (two for the propagation sweeps) and
   // the sub-cell finite-volume volume term
   double *alpha = nullptr, *alpha2 = nullptr, *Ffv = nullptr;
+  // GLM-MHD (cfg.nVar == 9): slave-side face fluxes (Flux holds the master side) and the
+  // per-element dt of a unit wave speed
+  double *FluxS = nullptr, *dtch1Elem = nullptr;
+
+  bool mhd() const { return cfg.nVar == 9; }
+  MhdArgs mhd_args() const
+  {
+    MhdArgs a;
+    a.kappa = cfg.kappa;
+    a.smu0 = 1.0 / cfg.mu_0;
+    return a;
+  }
   double rkA[5], rkb[5], rkc[5];
 
   void set_degree(int N)
@@ -402,6 +420,47 @@ void launch_dt_n(const Batch &b, double cfl, double dfl)
                      b.cfg.parabolic);
 }
 
+template <int N, int MODE>
+void launch_mhd_volume_t(const Batch &b, RKArgs rk)
+{
+  constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W;
+  const int nb = (b.cfg.nElems + EPB - 1) / EPB;
+  if (nb <= 0) return;
+  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
+    hipLaunchKernelGGL((k_mhd_volume<N, MODE, float>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf32, b.Mg32,
+                       b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(), b.st);
+  else
+    hipLaunchKernelGGL((k_mhd_volume<N, MODE, double>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh,
+                       b.sJ, b.e2s, b.Flux, b.FluxS, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(), b.st);
+}
+
+template <int N>
+void launch_mhd_volume_n(const Batch &b, int mode, RKArgs rk)
+{
+  if (mode == VMODE_RK) launch_mhd_volume_t<N, VMODE_RK>(b, rk);
+  else if (mode == VMODE_VOLINT) launch_mhd_volume_t<N, VMODE_VOLINT>(b, rk);
+  else launch_mhd_volume_t<N, VMODE_RHS>(b, rk);
+}
+
+template <int N>
+void launch_mhd_surface_n(const Batch &b)
+{
+  const long long tot = (long long)b.cfg.nSides * b.n2;
+  const int nb = (int)((tot + 255) / 256);
+  if (nb <= 0) return;
+  hipLaunchKernelGGL((k_mhd_surface<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.s2e, b.nvse, b.Flux, b.FluxS,
+                     b.cfg.nSides, b.mhd_args(), b.st);
+}
+
+template <int N>
+void launch_mhd_dt_n(const Batch &b, double cfl)
+{
+  const int epb = DtGeom<N>::EPB, nb = (b.cfg.nElems + epb - 1) / epb;
+  if (nb <= 0) return;
+  hipLaunchKernelGGL((k_mhd_calc_dt<N>), dim3(nb), dim3(DtGeom<N>::T), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                     b.dtElem, b.dtch1Elem, b.errElem, b.cfg.nElems, b.mhd_args(), cfl);
+}
+
 #define FX_DISPATCH_N(N_, fn, ...)             \
   switch (N_) {                                \
   case 2: fn<2>(__VA_ARGS__); break;           \
@@ -411,13 +470,27 @@ void launch_dt_n(const Batch &b, double cfl, double dfl)
   default: break;                              \
   }
 
-inline void q_volume(const Batch &b, int mode, RKArgs rk) { FX_DISPATCH_N(b.cfg.N, launch_volume_n, b, mode, rk); }
-inline void q_surface(const Batch &b) { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
+inline void q_volume(const Batch &b, int mode, RKArgs rk)
+{
+  if (b.mhd()) { FX_DISPATCH_N(b.cfg.N, launch_mhd_volume_n, b, mode, rk); }
+  else { FX_DISPATCH_N(b.cfg.N, launch_volume_n, b, mode, rk); }
+}
+inline void q_surface(const Batch &b)
+{
+  if (b.mhd()) { FX_DISPATCH_N(b.cfg.N, launch_mhd_surface_n, b); }
+  else { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
+}
 inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out); }
 
 // CALCTIMESTEP into the problem states (advance=1 also moves t and the step counter)
 inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
 {
+  if (b.mhd()) {  // single problem only
+    FX_DISPATCH_N(b.cfg.N, launch_mhd_dt_n, b, cfl);
+    hipLaunchKernelGGL(k_mhd_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.dtch1Elem, b.errElem,
+                       b.cfg.nElems, b.st, advance, b.cfg.GLM_scale);
+    return;
+  }
   FX_DISPATCH_N(b.cfg.N, launch_dt_n, b, cfl, dfl);
   if (b.eprob) {
     if (b.nProb > 0)

@@ -1,0 +1,412 @@
+// Ideal GLM-MHD (9 variables: rho, rho v, E, B, psi) on gfx950: the Derigs et al. entropy-
+// and kinetic-energy-conserving two-point flux, Powell non-conservative terms, the Rusanov
+// surface flux, CALCTIMESTEP with the GLM cleaning speed from the time step, and the LSERK
+// stage update (src/equation/mhd/{flux_average,flux,riemann,equation_vars,calctimestep}.f90,
+// NONCONS branches of src/dg/{volint,fillflux,surfint}.f90).
+//
+// Kernels per RK stage: k_mhd_surface (side-centric: conservative flux plus the
+// non-conservative term of each side -> FluxM, FluxS) and k_mhd_volume (element-centric like
+// k_volume: node fields in LDS, one thread per (direction, element, line) evaluating each
+// pair's two-point flux once, the Powell terms from per-node line sums, direction sum in
+// LDS, surface gather +FluxM L0 / -FluxS L0, -sJ, RK update).  Reformulations exact in real
+// arithmetic: per-node logs for LN_MEAN, the Cartesian frame for the surface flux (the
+// reference rotates momentum and B into (n,t1,t2) and back), the Powell volume term at node
+// i as 1/2 Phi_i (B_i.m_i sum_l Dvolsurf(i,l) + sum_l Dvolsurf(i,l) B_l.{{m}}_il).
+#pragma once
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+constexpr int MV = 9;
+
+struct MhdArgs {
+  double kappa, smu0;   // kappa, 1/mu_0
+};
+
+// fields staged per node (EvalUaux mhd/flux_average.f90:223-259 plus beta and the logs)
+enum { MF_RHO = 0, MF_V1, MF_V2, MF_V3, MF_B1, MF_B2, MF_B3, MF_PSI, MF_BETA, MF_LNRHO, MF_LNBETA,
+       MF_VV, MF_BB, MF_VB, MF_COUNT };
+
+__device__ __forceinline__ void mhd_node_fields(const double *u, const MhdArgs &a, double *f)
+{
+  const double km1 = a.kappa - 1.0, km2 = a.kappa - 2.0, s2mu0 = 0.5 * a.smu0;
+  const double sr = 1.0 / u[0];
+  const double v1 = u[1] * sr, v2 = u[2] * sr, v3 = u[3] * sr;
+  const double vv = v1 * v1 + v2 * v2 + v3 * v3;
+  const double BB = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+  const double pt = km1 * (u[4] - 0.5 * u[0] * vv - s2mu0 * u[8] * u[8]) - km2 * s2mu0 * BB;
+  const double beta = 0.5 * u[0] / (pt - s2mu0 * BB);
+  f[MF_RHO] = u[0]; f[MF_V1] = v1; f[MF_V2] = v2; f[MF_V3] = v3;
+  f[MF_B1] = u[5]; f[MF_B2] = u[6]; f[MF_B3] = u[7]; f[MF_PSI] = u[8];
+  f[MF_BETA] = beta; f[MF_LNRHO] = log(u[0]); f[MF_LNBETA] = log(beta);
+  f[MF_VV] = vv; f[MF_BB] = BB; f[MF_VB] = v1 * u[5] + v2 * u[6] + v3 * u[7];
+}
+
+// EntropyAndKinEnergyConservingFluxVec_Derigs (mhd/flux_average.f90:761-848)
+template <typename R>
+__device__ __forceinline__ void derigs_flux(const R *L, const R *Rt, const R *mL, const R *mR, R ch,
+                                            R smu0, R skm1, R *F)
+{
+  const R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+  const R rhoLN = ln_mean(L[MF_RHO], Rt[MF_RHO], L[MF_LNRHO], Rt[MF_LNRHO]);
+  const R sbetaLN = inv_ln_mean(L[MF_BETA], Rt[MF_BETA], L[MF_LNBETA], Rt[MF_LNBETA]);
+  const R vA0 = R(0.5) * (L[MF_V1] + Rt[MF_V1]), vA1 = R(0.5) * (L[MF_V2] + Rt[MF_V2]),
+          vA2 = R(0.5) * (L[MF_V3] + Rt[MF_V3]);
+  const R BA0 = R(0.5) * (L[MF_B1] + Rt[MF_B1]), BA1 = R(0.5) * (L[MF_B2] + Rt[MF_B2]),
+          BA2 = R(0.5) * (L[MF_B3] + Rt[MF_B3]);
+  const R pTilde = R(0.5) * ((L[MF_RHO] + Rt[MF_RHO]) * rcp_nr(L[MF_BETA] + Rt[MF_BETA]) +
+                             smu0 * R(0.5) * (L[MF_BB] + Rt[MF_BB]));
+  const R vm = vA0 * m0 + vA1 * m1 + vA2 * m2;
+  const R Bm = BA0 * m0 + BA1 * m1 + BA2 * m2;
+  const R psiA = R(0.5) * (L[MF_PSI] + Rt[MF_PSI]);
+  F[0] = rhoLN * vm;
+  F[1] = F[0] * vA0 - (smu0 * Bm) * BA0 + pTilde * m0;
+  F[2] = F[0] * vA1 - (smu0 * Bm) * BA1 + pTilde * m1;
+  F[3] = F[0] * vA2 - (smu0 * Bm) * BA2 + pTilde * m2;
+  F[5] = vm * BA0 - Bm * vA0 + (ch * psiA) * m0;
+  F[6] = vm * BA1 - Bm * vA1 + (ch * psiA) * m1;
+  F[7] = vm * BA2 - Bm * vA2 + (ch * psiA) * m2;
+  F[8] = ch * Bm;
+  const R B2vm = (L[MF_BB] * L[MF_V1] + Rt[MF_BB] * Rt[MF_V1]) * m0 +
+                 (L[MF_BB] * L[MF_V2] + Rt[MF_BB] * Rt[MF_V2]) * m1 +
+                 (L[MF_BB] * L[MF_V3] + Rt[MF_BB] * Rt[MF_V3]) * m2;
+  const R psiBm = (L[MF_PSI] * L[MF_B1] + Rt[MF_PSI] * Rt[MF_B1]) * m0 +
+                  (L[MF_PSI] * L[MF_B2] + Rt[MF_PSI] * Rt[MF_B2]) * m1 +
+                  (L[MF_PSI] * L[MF_B3] + Rt[MF_PSI] * Rt[MF_B3]) * m2;
+  F[4] = F[0] * R(0.5) * (skm1 * sbetaLN - R(0.5) * (L[MF_VV] + Rt[MF_VV])) + (vA0 * F[1] + vA1 * F[2] + vA2 * F[3]) +
+         smu0 * ((BA0 * F[5] + BA1 * F[6] + BA2 * F[7]) - R(0.25) * B2vm + R(0.5) * (L[MF_VB] + Rt[MF_VB]) * Bm +
+                 F[8] * psiA - (ch * R(0.5)) * psiBm);
+}
+
+// physical GLM-MHD flux along the unit normal n (EvalAdvectionFlux1D mhd/flux.f90:391-442 in
+// the Cartesian frame) and the fast magnetosonic speed along n (FastestWave1D
+// mhd/equation_vars.f90:663-684)
+__device__ __forceinline__ void mhd_flux_n(const double *U, const double *n, const MhdArgs &a, double ch,
+                                           double *F, double &vn, double &cf)
+{
+  const double km1 = a.kappa - 1.0, km2 = a.kappa - 2.0, smu0 = a.smu0, s2mu0 = 0.5 * smu0;
+  const double sr = 1.0 / U[0];
+  const double v1 = U[1] * sr, v2 = U[2] * sr, v3 = U[3] * sr;
+  const double Et = U[4] - s2mu0 * U[8] * U[8];
+  const double BB = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+  const double pt = km1 * (Et - 0.5 * U[0] * (v1 * v1 + v2 * v2 + v3 * v3)) - km2 * s2mu0 * BB;
+  vn = v1 * n[0] + v2 * n[1] + v3 * n[2];
+  const double Bn = U[5] * n[0] + U[6] * n[1] + U[7] * n[2];
+  const double vb = U[5] * v1 + U[6] * v2 + U[7] * v3;
+  F[0] = U[0] * vn;
+  F[1] = U[1] * vn + pt * n[0] - smu0 * Bn * U[5];
+  F[2] = U[2] * vn + pt * n[1] - smu0 * Bn * U[6];
+  F[3] = U[3] * vn + pt * n[2] - smu0 * Bn * U[7];
+  F[4] = (Et + pt) * vn - smu0 * Bn * vb + smu0 * ch * Bn * U[8];
+  F[5] = vn * U[5] - Bn * v1 + ch * U[8] * n[0];
+  F[6] = vn * U[6] - Bn * v2 + ch * U[8] * n[1];
+  F[7] = vn * U[7] - Bn * v3 + ch * U[8] * n[2];
+  F[8] = ch * Bn;
+  // ConsToPrim pressure (mhd/equation_vars.f90:248) and FastestWave1D with B_n, |B_t|^2
+  const double p = km1 * (U[4] - 0.5 * (U[1] * v1 + U[2] * v2 + U[3] * v3) - s2mu0 * (BB + U[8] * U[8]));
+  const double c2 = a.kappa * p * sr, ca2 = Bn * Bn * sr * smu0, va2 = (BB - Bn * Bn) * sr * smu0 + ca2;
+  const double astar = sqrt((c2 + va2) * (c2 + va2) - 4.0 * c2 * ca2);
+  cf = sqrt(0.5 * (c2 + va2 + astar));
+}
+
+// Powell non-conservative surface term of the left state (AddNonConsFlux mhd/riemann.f90:230-267)
+__device__ __forceinline__ void mhd_noncons_surface(const double *UL, const double *UR, const double *n, double *F)
+{
+  const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0], vL2 = UL[3] / UL[0];
+  const double bn = 0.5 * ((UL[5] + UR[5]) * n[0] + (UL[6] + UR[6]) * n[1] + (UL[7] + UR[7]) * n[2]);
+  F[1] += bn * UL[5]; F[2] += bn * UL[6]; F[3] += bn * UL[7];
+  F[4] += bn * (UL[5] * vL0 + UL[6] * vL1 + UL[7] * vL2);
+  F[5] += bn * vL0; F[6] += bn * vL1; F[7] += bn * vL2;
+}
+
+// FillFlux for GLM-MHD (fillflux.f90:41-110 with NONCONS): Rusanov (RiemannSolverByRusanov
+// mhd/riemann.f90:273-308) plus each side's Powell term, times SurfElem.
+template <int N>
+__global__ void __launch_bounds__(256)
+k_mhd_surface(const double *__restrict__ U, const int *__restrict__ s2e, const double4 *__restrict__ nvse,
+              double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides, MhdArgs a,
+              const DtState *__restrict__ st)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nSides * n2) return;
+  const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
+  const int em = s2e[5 * s] - 1, es = s2e[5 * s + 1] - 1;
+  if (em < 0) return;
+  const int vm = side_to_vol<N>(s2e[5 * s + 2], 0, p, q), vs = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
+  const double *um = U + ((size_t)em * n3 + vm) * MV, *us = U + ((size_t)es * n3 + vs) * MV;
+  double UL[MV], UR[MV];
+#pragma unroll
+  for (int v = 0; v < MV; v++) { UL[v] = um[v]; UR[v] = us[v]; }
+  const double4 g = nvse[gid];
+  const double nv[3] = {g.x, g.y, g.z};
+  const double ch = st->glm_ch;
+  double FL[MV], FR[MV], vnL, vnR, cfL, cfR;
+  mhd_flux_n(UL, nv, a, ch, FL, vnL, cfL);
+  mhd_flux_n(UR, nv, a, ch, FR, vnR, cfR);
+  const double lam = fmax(fabs(vnL) + cfL, fabs(vnR) + cfR);
+  double Fm[MV], Fs[MV];
+#pragma unroll
+  for (int v = 0; v < MV; v++) { Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v])); Fs[v] = Fm[v]; }
+  mhd_noncons_surface(UL, UR, nv, Fm);
+  mhd_noncons_surface(UR, UL, nv, Fs);
+  double *om = FluxM + (size_t)gid * MV, *os = FluxS + (size_t)gid * MV;
+#pragma unroll
+  for (int v = 0; v < MV; v++) { om[v] = Fm[v] * g.w; os[v] = Fs[v] * g.w; }
+}
+
+// Element-centric GLM-MHD volume kernel (same phases as k_volume).
+template <int N> struct MhdGeom;
+template <> struct MhdGeom<2> { static constexpr int EPB = 7, W = 64; };
+template <> struct MhdGeom<3> { static constexpr int EPB = 4, W = 64; };
+template <> struct MhdGeom<4> { static constexpr int EPB = 2, W = 64; };
+template <> struct MhdGeom<5> { static constexpr int EPB = 1, W = 64; };
+
+template <int N, int MODE, typename VR>
+__global__ void __launch_bounds__(3 * MhdGeom<N>::W)
+k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
+             const double *__restrict__ sJ, const int *__restrict__ e2s, const double *__restrict__ FluxM,
+             const double *__restrict__ FluxS, double *__restrict__ Ut, double *__restrict__ Uttmp, RKArgs rk,
+             int nElems, MhdArgs a, const DtState *__restrict__ st)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W, NP = Lds<N>::NP;
+  static_assert(EPB * n2 <= W, "MHD volume launch geometry");
+  __shared__ VR s_f[EPB][MF_COUNT * NP];
+  __shared__ VR s_acc[EPB][MV * NP];
+  const Ops &op = c_ops[N];
+  const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
+  const VR ch = VR(st->glm_ch), smu0 = VR(a.smu0), skm1 = VR(1.0 / (a.kappa - 1.0));
+
+  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+    const int el = idx / n3, node = idx - el * n3, e = e0 + el;
+    if (e < nElems && elem_problem(rk.eprob, e) >= 0) {
+      const double *u = U + ((size_t)e * n3 + node) * MV;
+      double uu[MV], f[MF_COUNT];
+#pragma unroll
+      for (int v = 0; v < MV; v++) uu[v] = u[v];
+      mhd_node_fields(uu, a, f);
+      const int l = Lds<N>::li(node);
+#pragma unroll
+      for (int c = 0; c < MF_COUNT; c++) s_f[el][c * NP + l] = VR(f[c]);
+    }
+  }
+  __syncthreads();
+
+  const int dir = tid / W, lt = tid - dir * W;
+  const int el = lt / n2, line = lt - el * n2, e = e0 + el;
+  const bool active = (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
+  const int aa = line % n, bb = line / n;
+  int nodes[n], lds[n];
+#pragma unroll
+  for (int l = 0; l < n; l++) {
+    const int i = (dir == 0) ? l : aa, j = (dir == 1) ? l : (dir == 0 ? aa : bb), k = (dir == 2) ? l : bb;
+    nodes[l] = i + n * (j + n * k);
+    lds[l] = Lds<N>::li(i, j, k);
+  }
+  VR acc[n][MV], snc[n];
+#pragma unroll
+  for (int i = 0; i < n; i++) {
+    snc[i] = VR(0);
+#pragma unroll
+    for (int v = 0; v < MV; v++) acc[i][v] = VR(0);
+  }
+  if (active) {
+    const VR *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
+    VR m[n][3];
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+      const VR *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
+      m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
+    }
+#pragma unroll
+    for (int i = 0; i < n; i++)
+#pragma unroll
+      for (int l = i + 1; l < n; l++) {
+        VR fi[MF_COUNT], fl[MF_COUNT], F[MV];
+#pragma unroll
+        for (int c = 0; c < MF_COUNT; c++) { fi[c] = s_f[el][c * NP + lds[i]]; fl[c] = s_f[el][c * NP + lds[l]]; }
+        derigs_flux<VR>(fi, fl, m[i], m[l], ch, smu0, skm1, F);
+        const VR dil = OpsOf<VR>::ds(op, i, l), dli = OpsOf<VR>::ds(op, l, i);
+#pragma unroll
+        for (int v = 0; v < MV; v++) {
+          acc[i][v] += dil * F[v];
+          acc[l][v] += dli * F[v];
+        }
+        // Powell terms: B_l.{{m}} towards node i, B_i.{{m}} towards node l
+        const VR mb0 = VR(0.5) * (m[i][0] + m[l][0]), mb1 = VR(0.5) * (m[i][1] + m[l][1]),
+                 mb2 = VR(0.5) * (m[i][2] + m[l][2]);
+        snc[i] += dil * (mb0 * fl[MF_B1] + mb1 * fl[MF_B2] + mb2 * fl[MF_B3]);
+        snc[l] += dli * (mb0 * fi[MF_B1] + mb1 * fi[MF_B2] + mb2 * fi[MF_B3]);
+      }
+#pragma unroll
+    for (int i = 0; i < n; i++) {
+      const VR b1 = s_f[el][MF_B1 * NP + lds[i]], b2 = s_f[el][MF_B2 * NP + lds[i]], b3 = s_f[el][MF_B3 * NP + lds[i]];
+      const VR bhat = m[i][0] * b1 + m[i][1] * b2 + m[i][2] * b3;
+      const VR t = VR(0.5) * (bhat * VR(op.DsRow[i]) + snc[i]);
+      acc[i][1] += t * b1; acc[i][2] += t * b2; acc[i][3] += t * b3;
+      acc[i][4] += t * s_f[el][MF_VB * NP + lds[i]];
+      acc[i][5] += t * s_f[el][MF_V1 * NP + lds[i]];
+      acc[i][6] += t * s_f[el][MF_V2 * NP + lds[i]];
+      acc[i][7] += t * s_f[el][MF_V3 * NP + lds[i]];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (active && dir == d) {
+#pragma unroll
+      for (int l = 0; l < n; l++)
+#pragma unroll
+        for (int v = 0; v < MV; v++)
+          s_acc[el][v * NP + lds[l]] = (d == 0) ? acc[l][v] : s_acc[el][v * NP + lds[l]] + acc[l][v];
+    }
+    __syncthreads();
+  }
+
+  const double L0 = op.L0;
+  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+    const int el2 = idx / n3, node = idx - el2 * n3, e2 = e0 + el2;
+    if (e2 >= nElems) continue;
+    const int prob = elem_problem(rk.eprob, e2);
+    if (prob < 0) continue;
+    const int i = node % n, j = (node / n) % n, k = node / n2, l = Lds<N>::li(node);
+    double tot[MV];
+#pragma unroll
+    for (int v = 0; v < MV; v++) tot[v] = double(s_acc[el2][v * NP + l]);
+    if (MODE != VMODE_VOLINT) {
+      int locs[3], nl = 0;
+      if (k == 0) locs[nl++] = ZETA_MINUS;
+      if (j == 0) locs[nl++] = ETA_MINUS;
+      if (i == N) locs[nl++] = XI_PLUS;
+      if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
+      if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
+      if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+      for (int t = 0; t < nl; t++) {
+        const int loc = locs[t];
+        const int s = e2s[(e2 * 6 + loc - 1) * 2] - 1, fl = e2s[(e2 * 6 + loc - 1) * 2 + 1];
+        int p, q;
+        vol_to_side<N>(loc, fl, i, j, k, p, q);
+        const size_t f = ((size_t)(s * n + q) * n + p) * MV;
+        if (fl == 0) {
+#pragma unroll
+          for (int v = 0; v < MV; v++) tot[v] = tot[v] + FluxM[f + v] * L0;
+        } else {
+#pragma unroll
+          for (int v = 0; v < MV; v++) tot[v] = tot[v] - FluxS[f + v] * L0;
+        }
+      }
+      const double msj = -sJ[(size_t)e2 * n3 + node];
+#pragma unroll
+      for (int v = 0; v < MV; v++) tot[v] = tot[v] * msj;
+    }
+    const size_t o = ((size_t)e2 * n3 + node) * MV;
+    if (MODE == VMODE_RK) {
+      const double bdt = rk.b * rk.st[prob].dt;
+#pragma unroll
+      for (int v = 0; v < MV; v++) {
+        const double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
+        Uttmp[o + v] = ut;
+        U[o + v] = U[o + v] + ut * bdt;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < MV; v++) Ut[o + v] = tot[v];
+    }
+  }
+}
+
+// CALCTIMESTEP for ideal GLM-MHD (mhd/calctimestep.f90:39-179) and InitTimeStep_GLM
+// (:186-215): per element the convective dt (FastestWave3D) and the dt of a unit wave speed.
+template <int N>
+__global__ void __launch_bounds__(DtGeom<N>::T)
+k_mhd_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
+              const double *__restrict__ Mh, const double *__restrict__ sJ, double *__restrict__ dtElem,
+              double *__restrict__ dtch1Elem, int *__restrict__ errElem, int nElems, MhdArgs a, double cfl)
+{
+  constexpr int n3 = DtGeom<N>::n3, G = DtGeom<N>::G, EPB = DtGeom<N>::EPB, T = DtGeom<N>::T;
+  __shared__ double s_l[6][T];
+  __shared__ int s_err[T];
+  const int tid = threadIdx.x, el = tid / G, node = tid - el * G, e = blockIdx.x * EPB + el;
+  double l[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int err = 0;
+  if (node < n3 && e < nElems) {
+    const size_t c = (size_t)e * n3 + node;
+    const double *u = U + MV * c;
+    if (isnan(u[0])) err = 1;
+    const double sr = 1.0 / u[0];
+    const double v0 = u[1] * sr, v1 = u[2] * sr, v2 = u[3] * sr;
+    double b2 = 0.0;
+#pragma unroll
+    for (int v = 5; v < MV; v++) b2 += u[v] * u[v];
+    const double p = (a.kappa - 1.0) * (u[4] - 0.5 * (u[1] * v0 + u[2] * v1 + u[3] * v2) - 0.5 * a.smu0 * b2);
+    if (p <= 0.0) err = 2;
+    const double cf = sqrt((a.kappa * p + (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) * a.smu0) / u[0]);
+    const double sj = sJ[c];
+    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+    const double nf = sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]);
+    const double ng = sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]);
+    const double nh = sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]);
+    l[0] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cf * nf);
+    l[1] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cf * ng);
+    l[2] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cf * nh);
+    l[3] = sj * nf; l[4] = sj * ng; l[5] = sj * nh;
+  }
+#pragma unroll
+  for (int d = 0; d < 6; d++) s_l[d][tid] = l[d];
+  s_err[tid] = err;
+  __syncthreads();
+  for (int stp = G / 2; stp > 0; stp >>= 1) {
+    if (node < stp) {
+#pragma unroll
+      for (int d = 0; d < 6; d++) s_l[d][tid] = fmax(s_l[d][tid], s_l[d][tid + stp]);
+      s_err[tid] = max(s_err[tid], s_err[tid + stp]);
+    }
+    __syncthreads();
+  }
+  if (node == 0 && e < nElems) {
+    int er = s_err[tid];
+    const double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
+    if (isnan(dte)) er = max(er, 3);
+    dtElem[e] = dte;
+    dtch1Elem[e] = cfl * 2.0 / (s_l[3][tid] + s_l[4][tid] + s_l[5][tid]);
+    errElem[e] = er;
+  }
+}
+
+// minimum dt (and, on the first call, the unit-speed dt), GLM_ch = scale * dtch1 / dt
+__global__ void __launch_bounds__(1024)
+k_mhd_dt_reduce(const double *__restrict__ dtElem, const double *__restrict__ dtch1Elem,
+                const int *__restrict__ errElem, int nElems, DtState *st, int advance, double glm_scale)
+{
+  __shared__ double s_dt[1024], s_c[1024];
+  __shared__ int s_er[1024];
+  const int tid = threadIdx.x;
+  double m = 1.79e308, mc = 1.79e308;
+  int er = 0;
+  for (int e = tid; e < nElems; e += 1024) {
+    const double d = dtElem[e], dc = dtch1Elem[e];
+    if (d < m) m = d;
+    if (dc < mc) mc = dc;
+    er = max(er, errElem[e]);
+  }
+  s_dt[tid] = m; s_c[tid] = mc; s_er[tid] = er;
+  __syncthreads();
+  for (int stp = 512; stp > 0; stp >>= 1) {
+    if (tid < stp) {
+      s_dt[tid] = fmin(s_dt[tid], s_dt[tid + stp]);
+      s_c[tid] = fmin(s_c[tid], s_c[tid + stp]);
+      s_er[tid] = max(s_er[tid], s_er[tid + stp]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st->dt = s_dt[0];
+    if (!st->glm_init) { st->glm_dtch1 = s_c[0]; st->glm_init = 1; }
+    st->glm_ch = glm_scale * (st->glm_dtch1 / st->dt);
+    st->errType = max(st->errType, s_er[0]);
+    if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
+    if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+  }
+}
+
+}  // namespace fx

@@ -16,6 +16,7 @@ module fluxo_gpu_bind
     real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, reserved2 = 0.0d0
     integer(c_int) :: nfvse = 0, computeAlpha = 1, spacePropSweeps = 1, reserved3 = 0
     real(c_double) :: alpha_max = 0.5d0, alpha_min = 0.01d0, spacePropFactor = 0.5d0, shockBlendCoef = 0.0d0
+    real(c_double) :: mu_0 = 1.0d0, GLM_scale = 0.5d0
   end type fx_config
 
   type, bind(C) :: fx_mesh
@@ -105,6 +106,11 @@ module fluxo_gpu_bind
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: S, dSdU_Ut
     end function fx_entropy
+    integer(c_int) function fx_glm(ctx, glm_ch, glm_dtch1) bind(C, name="fx_glm")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: glm_ch, glm_dtch1
+    end function fx_glm
     integer(c_int) function fx_synchronize(ctx) bind(C, name="fx_synchronize")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

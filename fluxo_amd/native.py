@@ -34,7 +34,8 @@ class fx_config(C.Structure):
                 ("reserved2", C.c_double),
                 ("nfvse", C.c_int), ("computeAlpha", C.c_int), ("spacePropSweeps", C.c_int),
                 ("reserved3", C.c_int), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
-                ("spacePropFactor", C.c_double), ("shockBlendCoef", C.c_double)]
+                ("spacePropFactor", C.c_double), ("shockBlendCoef", C.c_double),
+                ("mu_0", C.c_double), ("GLM_scale", C.c_double)]
 
 
 class fx_mesh(C.Structure):
@@ -77,7 +78,7 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_ctx_bytes", "fx_profile_kernel", "fx_server_create", "fx_server_submit",
            "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
            "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
-           "fx_use_volume_tile", "fx_blending"]
+           "fx_use_volume_tile", "fx_blending", "fx_glm"]
 
 (FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS,
  FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV) = range(7)
@@ -102,6 +103,7 @@ def lib():
     L.fx_rhs.argtypes = [vp, C.c_double, vp]
     L.fx_lifting.argtypes = [vp, vp, vp, vp]
     L.fx_blending.argtypes = [vp, vp]
+    L.fx_glm.argtypes = [vp, dp, dp]
     L.fx_calc_dt.argtypes = [vp, C.c_double, C.c_double, dp, ip]
     L.fx_rk_step.argtypes = [vp, C.c_double, C.c_double]
     L.fx_run.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp, dp, ip]
@@ -144,6 +146,7 @@ def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, k
     alpha_min, SpacePropSweeps, SpacePropFactor; fixed_alpha = ComputeAlpha 3 with that
     ShockBlendCoef, None = Persson-Peraire indicator)."""
     c = fx_config()
+    c.mu_0, c.GLM_scale = 1.0, 0.5
     c.nfvse, c.spacePropSweeps = int(bool(nfvse)), sweeps
     c.computeAlpha = 1 if fixed_alpha is None else 3
     c.alpha_max, c.alpha_min, c.spacePropFactor = alpha_max, alpha_min, prop_factor
@@ -206,7 +209,7 @@ class Engine:
 
     def shape(self):
         n = self.N + 1
-        return (self.nElems, n, n, n, 5)
+        return (self.nElems, n, n, n, int(self.cfg.nVar))
 
     def upload(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
@@ -232,6 +235,12 @@ class Engine:
         g = [np.empty(self.shape()) for _ in range(3)]
         _chk(lib().fx_lifting(self._ctx, *(_ptr(a) for a in g)))
         return g
+
+    def glm(self):
+        """(GLM_ch, GLM_dtch1) of a GLM-MHD context."""
+        ch, d1 = C.c_double(0.0), C.c_double(0.0)
+        _chk(lib().fx_glm(self._ctx, C.byref(ch), C.byref(d1)))
+        return ch.value, d1.value
 
     def blending(self):
         """NFVSE blending coefficient of the current state (CalcBlendingCoefficient)."""
@@ -353,3 +362,20 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class MhdEngine(Engine):
+    """A GLM-MHD context (9 variables; Derigs volume flux, Rusanov surface flux, Powell
+    terms): U is (nElems, n, n, n, 9)."""
+
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64):
+        L = lib()
+        self.N = int(mesh["N"])
+        self.nElems = int(mesh["nElems"])
+        self.nSides = int(mesh["nSides"])
+        self.cfg = make_config(self.N, self.nElems, self.nSides, 10, 1, False, kappa, 0.0, 0.72, precision)
+        self.cfg.nVar, self.cfg.mu_0, self.cfg.GLM_scale = 9, mu_0, glm_scale
+        self._ctx = C.c_void_p()
+        _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
+        m, _keep = _mesh_struct(mesh)
+        _chk(L.fx_mesh_upload(self._ctx, C.byref(m)))

@@ -37,10 +37,10 @@ enum { FX_PREC_FP64 = 0, FX_PREC_FP32_SPLIT = 1 };
 /* Options that the reference reads from parameter.ini / CMake defines. */
 typedef struct {
   int N;            /* polynomial degree, 'N' (interpolation.f90:125-126); 1..7 */
-  int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) */
+  int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) or 9 (ideal GLM-MHD) */
   int nElems;       /* local elements */
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
-  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32 */
+  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32; MHD 10 */
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:103-133): 0, 1, 9, 32 */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler, 1 Navier-Stokes */
   int lifting;      /* PP_Lifting: 1 = BR1 with entropy-variable lifting */
@@ -63,6 +63,11 @@ typedef struct {
   double alpha_min;       /* 'alpha_min' */
   double spacePropFactor; /* 'SpacePropFactor' */
   double shockBlendCoef;  /* 'ShockBlendCoef' */
+  /* ideal GLM-MHD (nVar = 9: FLUXO_EQNSYSNAME=mhd with FLUXO_EQN_GLM and Powell
+     non-conservative terms, CMakeLists.txt:174-209; mhd/equation.f90:174-208):
+     VolumeFlux 10 (Derigs), Riemann 1 (Rusanov), Euler only, no shock capturing. */
+  double mu_0;            /* 'mu_0' */
+  double GLM_scale;       /* 'GLM_scale' */
 } fx_config;
 
 /* Mesh data produced by the host's InitMesh/CalcMetrics (mesh.f90:100-300,
@@ -129,6 +134,10 @@ int fx_run(fx_ctx *ctx, int nSteps, double CFLscale, double DFLscale, double *t,
 /* CalcEntropy (navierstokes/analyze_equation.f90:549-599) on the device state and
    the last Ut: S = int -rho s/(kappa-1), dSdU_Ut = int dS/dU . Ut. */
 int fx_entropy(fx_ctx *ctx, double *S, double *dSdU_Ut);
+
+/* GLM-MHD cleaning state: GLM_ch (0.5 until the first CALCTIMESTEP, then
+   GLM_scale*GLM_dtch1/dt of the last one, mhd/calctimestep.f90:175-177) and GLM_dtch1. */
+int fx_glm(fx_ctx *ctx, double *glm_ch, double *glm_dtch1);
 
 /* Wait for all work queued on the context's stream. */
 int fx_synchronize(fx_ctx *ctx);

@@ -1,0 +1,106 @@
+"""Ideal GLM-MHD on the GPU against the CPU oracle and the reference's own outputs
+(tests/golden/mhd), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per RHS, <= 1e-10
+after RK steps; dt and the cleaning speed GLM_ch to 1e-14.  The FP32-split variant states
+its error against the FP64 oracle."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mhd")
+NAMES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz")))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def state(m, seed=3):
+    P = O.MhdProblem(m)
+    U = P.exact_orszag_tang()
+    rng = np.random.default_rng(seed)
+    U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
+    U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
+    U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
+    return P, U
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_rhs_matches_oracle(N):
+    from fluxo_amd.native import MhdEngine
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    P, U = state(m)
+    e = MhdEngine(m)
+    e.upload(U)
+    assert e.glm()[0] == 0.5
+    err = rel_l2(e.rhs(), P.rhs(U, glm_ch=0.5))
+    print(f"MHD N={N}: RHS relL2 {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_engine_matches_reference_mhd(name):
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import MhdEngine
+    with np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    m = box_mesh(int(g["K"]), int(g["N"]), x0=tuple(g["x0"]), length=tuple(g["L"]), Ngeo=int(g["Ngeo"]),
+                 warp=bool(g["warp"]))
+    e = MhdEngine(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]))
+    e.upload(g["U"])
+    err = rel_l2(e.rhs(), g["Ut"])
+    print(f"{name}: RHS relL2 vs reference {err:.3e}")
+    assert err <= 1e-12
+    dt, errt = e.calc_dt(float(g["cfl"]))
+    ch, dtch1 = e.glm()
+    assert errt == int(g["errType0"]) and dt == pytest.approx(float(g["dt0"]), rel=1e-14)
+    assert dtch1 == pytest.approx(float(g["glm_dtch1"]), rel=1e-14)
+    assert ch == pytest.approx(float(g["glm_ch0"]), rel=1e-14)
+    e2 = MhdEngine(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]))
+    e2.upload(g["U"])
+    t, _, errt = e2.run(int(g["nsteps"]), float(g["cfl"]))
+    assert errt == 0
+    traj = rel_l2(e2.download(), g["U_end"])
+    print(f"{name}: {int(g['nsteps'])} steps relL2 vs reference {traj:.3e}")
+    assert traj <= 1e-10
+    assert t == pytest.approx(float(g["t_end"]), rel=1e-12)
+
+
+def test_free_stream_warped():
+    from fluxo_amd.native import MhdEngine
+    m = O.box_mesh(3, 4, Ngeo=3, warp=True)
+    e = MhdEngine(m)
+    U = np.zeros(e.shape())
+    U[...] = [1.1, 0.2, -0.1, 0.3, 4.0, 0.5, -0.4, 0.2, 0.0]
+    e.upload(U)
+    assert np.abs(e.rhs()).max() < 1e-11
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_fp32split_error_vs_fp64(N):
+    """FP32 flux differencing for GLM-MHD: RHS and a 5-step trajectory against the FP64
+    oracle (relative L2 / Linf printed; bounds about 3x the measured values)."""
+    from fluxo_amd.native import MhdEngine, FX_PREC_FP32_SPLIT
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    P, U = state(m)
+    e = MhdEngine(m, precision=FX_PREC_FP32_SPLIT)
+    e.upload(U)
+    ref = P.rhs(U, glm_ch=0.5)
+    got = e.rhs()
+    l2 = rel_l2(got, ref)
+    li = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    Ue, Utmp = U.copy(), np.zeros_like(U)
+    for _ in range(5):
+        d, _ = P.calc_dt(Ue, 0.9)
+        P.lserk_step(Ue, Utmp, d)
+    e2 = MhdEngine(m, precision=FX_PREC_FP32_SPLIT)
+    e2.upload(U)
+    e2.run(5, 0.9)
+    tl2 = rel_l2(e2.download(), Ue)
+    print(f"MHD FP32-split N={N}: RHS relL2 {l2:.3e} relLinf {li:.3e}; 5 steps relL2 {tl2:.3e}")
+    assert l2 <= 1e-4 and tl2 <= 1e-5
