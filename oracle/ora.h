@@ -97,7 +97,9 @@ void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, dou
 void ora_volint_splitform(const ora_problem *P, const double *U, double *Ut);
 void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
                           const double *gz, double *Ut);
-/* BR1/BR2 lifting: primitive gradients gradPx/y/z(5,n,n,n,E) */
+/* BR1 lifting of nvar lifting variables w (volume nodes): physical gradients of w */
+void ora_lift(const ora_problem *P, int nvar, const double *w, double *gx, double *gy, double *gz);
+/* BR1 lifting of the Navier-Stokes entropy variables: primitive gradients gradPx/y/z(5,n,n,n,E) */
 void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz);
 /* Carpenter-Kennedy RK4(5) 2N low-storage step (src/timedisc/timedisc.f90:424-480) */
 void ora_lserk_step(const ora_problem *P, double *U, double *Ut_tmp, double t, double dt);
@@ -119,19 +121,26 @@ void ora_persson_peraire(const ora_problem *P, const double *U, double *eta);
 void ora_blending_coefficient(const ora_problem *P, const double *U, double *alpha);
 void ora_volint_nfvse(const ora_problem *P, const double *U, const double *alpha, double *Ut);
 
-/* ---- ideal GLM-MHD (src/equation/mhd), 9 variables (rho, rho v, E, B, psi) ---- */
+/* ---- GLM-MHD (src/equation/mhd), 9 variables (rho, rho v, E, B, psi): ideal, or resistive
+   (viscosity mu, resistivity eta, heat conduction mu kappa/Pr) with BR1 lifting of the
+   entropy variables ---- */
 typedef struct {
   const ora_problem *P;   /* mesh and N (nVar, volflux, riemann of P are not used) */
   double kappa, mu_0, glm_scale;
+  int parabolic, reserved;
+  double mu, eta, Pr;
 } ora_mhd;
+/* primitive gradients (rho, v, p, B, psi) of the resistive case, gradPx/y/z(9,n,n,n,E) */
+void ora_mhd_lifting(const ora_mhd *M, const double *U, double *gx, double *gy, double *gz);
 void ora_mhd_derigs(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *mL,
                     const double *mR, double *F);
 void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double *F);
 void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut);
 double ora_mhd_glm_dtch1(const ora_mhd *M, double CFLscale);
-double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double glm_dtch1,
-                             double *glm_ch, int *errType);
+/* CALCTIMESTEP; DFLscale sets the viscous/resistive limit of the resistive case */
+double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double DFLscale,
+                             double glm_dtch1, double *glm_ch, int *errType);
 void ora_mhd_lserk_step(const ora_mhd *M, double *U, double *Ut_tmp, double glm_ch, double dt);
 void ora_mhd_orszag_tang(const ora_mhd *M, int npts, const double *x, double *U);
 

@@ -1,9 +1,11 @@
 /*
- * CPU oracle: ideal GLM-MHD split-form DGSEM on LGL nodes (9 variables: rho, rho v, E, B,
- * psi), with the Derigs et al. entropy- and kinetic-energy-conserving two-point flux
+ * CPU oracle: GLM-MHD split-form DGSEM on LGL nodes (9 variables: rho, rho v, E, B, psi),
+ * with the Derigs et al. entropy- and kinetic-energy-conserving two-point flux
  * (VolumeFlux=10), Powell non-conservative terms (NONCONS=1), the Rusanov surface flux
  * (Riemann=1), CALCTIMESTEP with the GLM cleaning speed from the time step, and the
- * Carpenter RK4-5 step.
+ * Carpenter RK4-5 step.  Resistive case (PARABOLIC): BR1 lifting of the entropy variables,
+ * isotropic viscosity, resistivity and heat conduction (EvalDiffFlux3D), the weak-form
+ * VolInt_visc and the BR1 average of the diffusive fluxes at faces.
  *
  * TEST INFRASTRUCTURE ONLY (see ora.h).  Restates src/equation/mhd/{flux_average,flux,
  * riemann,equation_vars,calctimestep}.f90 and the NONCONS branches of src/dg/{fillflux,
@@ -140,6 +142,144 @@ void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const do
 }
 
 /* AddNonConsFlux, Powell (mhd/riemann.f90:230-267): FL(2:8) += 1/2 (B_L+B_R).n (B_L, v_L.B_L, v_L) */
+/* ConsToEntropy (mhd/equation_vars.f90:330-361) */
+static void mhd_cons_to_entropy(const ora_mhd *M, const double *c, double *w)
+{
+  const double km1 = M->kappa - 1.0, s2mu0 = 0.5 * (1.0 / M->mu_0);
+  const double srho = 1.0 / c[0];
+  const double u = c[1] * srho, v = c[2] * srho, ww = c[3] * srho;
+  const double v2s2 = 0.5 * (u * u + v * v + ww * ww);
+  double b2 = 0.0;
+  for (int i = 5; i < MV; i++) b2 += c[i] * c[i];
+  const double rho_sp = c[0] / (km1 * (c[4] - c[0] * v2s2 - s2mu0 * b2));
+  const double s = -log(rho_sp * pow(c[0], km1));
+  w[0] = (M->kappa - s) * (1.0 / km1) - rho_sp * v2s2;
+  w[1] = rho_sp * u;
+  w[2] = rho_sp * v;
+  w[3] = rho_sp * ww;
+  w[4] = -rho_sp;
+  for (int i = 5; i < MV; i++) w[i] = rho_sp * c[i];
+}
+
+/* ConvertToGradPrimVec, entropy-variable lifting (mhd/equation_vars.f90:547-628): gradient of
+   the entropy variables in, gradient of (rho, v, p, B, psi) out */
+static void mhd_grad_prim(const ora_mhd *M, const double *c, double *g)
+{
+  const double km1 = M->kappa - 1.0, s2mu0 = 0.5 * (1.0 / M->mu_0);
+  const double srho = 1.0 / c[0];
+  const double u = c[1] * srho, v = c[2] * srho, w = c[3] * srho;
+  const double ekin = 0.5 * (c[1] * u + c[2] * v + c[3] * w);
+  double b2 = 0.0;
+  for (int i = 5; i < MV; i++) b2 += c[i] * c[i];
+  const double p = km1 * (c[4] - ekin - s2mu0 * b2);
+  const double rho_sp = c[0] / p, p_srho = p * srho;
+  const double g1 = g[0], g5 = g[4];
+  const double gu = p_srho * (g[1] + u * g5), gv = p_srho * (g[2] + v * g5), gw = p_srho * (g[3] + w * g5);
+  g[0] = c[0] * g1 - g5 * (ekin - p * (1.0 / km1)) + rho_sp * (c[1] * gu + c[2] * gv + c[3] * gw);
+  g[1] = gu;
+  g[2] = gv;
+  g[3] = gw;
+  g[4] = p_srho * (g[0] + p * g5);
+  for (int i = 5; i < MV; i++) g[i] = p_srho * (g[i] + c[i] * g5);
+}
+
+/* EvalDiffFlux3D, isotropic heat flux (mhd/flux.f90:449-573), one point: Cartesian diffusive
+   fluxes f, g, h from the state and its primitive gradients */
+static void mhd_diff_flux(const ora_mhd *M, const double *c, const double *gx, const double *gy,
+                          const double *gz, double *f, double *g, double *h)
+{
+  const double km1 = M->kappa - 1.0, skm1 = 1.0 / km1, smu0 = 1.0 / M->mu_0, s2mu0 = 0.5 * smu0;
+  const double s23 = 2.0 / 3.0, mu = M->mu, etasmu0 = M->eta * smu0;
+  const double srho = 1.0 / c[0];
+  const double v1 = c[1] * srho, v2 = c[2] * srho, v3 = c[3] * srho;
+  double b2 = 0.0;
+  for (int i = 5; i < MV; i++) b2 += c[i] * c[i];
+  const double p = km1 * (c[4] - 0.5 * (c[0] * (v1 * v1 + v2 * v2 + v3 * v3)) - s2mu0 * b2);
+  const double b1 = c[5], bb2 = c[6], b3 = c[7];
+  const double divv = gx[1] + gy[2] + gz[3];
+  const double tx = skm1 * srho * (gx[4] - srho * p * gx[0]);
+  const double ty = skm1 * srho * (gy[4] - srho * p * gy[0]);
+  const double tz = skm1 * srho * (gz[4] - srho * p * gz[0]);
+  const double lambda = mu * (M->kappa / M->Pr);
+  const double qx = lambda * tx, qy = lambda * ty, qz = lambda * tz;
+  f[0] = 0.0;
+  f[1] = -mu * (2 * gx[1] - s23 * divv);
+  f[2] = -mu * (gx[2] + gy[1]);
+  f[3] = -mu * (gx[3] + gz[1]);
+  f[5] = 0.0;
+  f[6] = -etasmu0 * (gx[6] - gy[5]);
+  f[7] = -etasmu0 * (gx[7] - gz[5]);
+  f[4] = v1 * f[1] + v2 * f[2] + v3 * f[3] + smu0 * (b1 * f[5] + bb2 * f[6] + b3 * f[7]) - qx;
+  g[0] = 0.0;
+  g[1] = f[2];
+  g[2] = -mu * (2 * gy[2] - s23 * divv);
+  g[3] = -mu * (gy[3] + gz[2]);
+  g[5] = -f[6];
+  g[6] = 0.0;
+  g[7] = -etasmu0 * (gy[7] - gz[6]);
+  g[4] = v1 * g[1] + v2 * g[2] + v3 * g[3] + smu0 * (b1 * g[5] + bb2 * g[6] + b3 * g[7]) - qy;
+  h[0] = 0.0;
+  h[1] = f[3];
+  h[2] = g[3];
+  h[3] = -mu * (2 * gz[3] - s23 * divv);
+  h[5] = -f[7];
+  h[6] = -g[7];
+  h[7] = 0.0;
+  h[4] = v1 * h[1] + v2 * h[2] + v3 * h[3] + smu0 * (b1 * h[5] + bb2 * h[6] + b3 * h[7]) - qz;
+  f[8] = g[8] = h[8] = 0.0;
+}
+
+/* BR1 lifting of the entropy variables (br1.f90:154-364 with mhd/flux.f90:718-793) followed
+   by ConvertToGradPrimVec */
+void ora_mhd_lifting(const ora_mhd *M, const double *U, double *gx, double *gy, double *gz)
+{
+  const ora_problem *P = M->P;
+  const size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  double *w = calloc((size_t)MV * n3E, sizeof(double));
+  for (size_t p = 0; p < n3E; p++) mhd_cons_to_entropy(M, U + MV * p, w + MV * p);
+  ora_lift(P, MV, w, gx, gy, gz);
+  for (size_t p = 0; p < n3E; p++) {
+    mhd_grad_prim(M, U + MV * p, gx + MV * p);
+    mhd_grad_prim(M, U + MV * p, gy + MV * p);
+    mhd_grad_prim(M, U + MV * p, gz + MV * p);
+  }
+  free(w);
+}
+
+/* VolInt_visc (volint.f90:228-271) with EvalDiffFluxTilde3D (mhd/flux.f90:579-712) */
+static void mhd_volint_visc(const ora_mhd *M, const double *U, const double *gx, const double *gy,
+                            const double *gz, const double *Dh, double *Ut)
+{
+  const ora_problem *P = M->P;
+  const int n = P->N + 1, n3 = n * n * n;
+  double *ft = malloc(sizeof(double) * MV * n3 * 3);
+  double *gt = ft + MV * n3, *ht = gt + MV * n3;
+  for (int e = 0; e < P->nElems; e++) {
+    for (int p = 0; p < n3; p++) {
+      const size_t c = (size_t)n3 * e + p;
+      double f[MV], g[MV], h[MV];
+      mhd_diff_flux(M, U + MV * c, gx + MV * c, gy + MV * c, gz + MV * c, f, g, h);
+      const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
+      for (int v = 0; v < MV; v++) {
+        ft[MV * p + v] = f[v] * mf[0] + g[v] * mf[1] + h[v] * mf[2];
+        gt[MV * p + v] = f[v] * mg[0] + g[v] * mg[1] + h[v] * mg[2];
+        ht[MV * p + v] = f[v] * mh[0] + g[v] * mh[1] + h[v] * mh[2];
+      }
+    }
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double *ut = Ut + MV * ((size_t)n3 * e + i + n * (j + n * k));
+          for (int l = 0; l < n; l++)
+            for (int v = 0; v < MV; v++)
+              ut[v] = ut[v] + Dh[i + n * l] * ft[MV * (l + n * (j + n * k)) + v]
+                            + Dh[j + n * l] * gt[MV * (i + n * (l + n * k)) + v]
+                            + Dh[k + n * l] * ht[MV * (i + n * (j + n * l)) + v];
+        }
+  }
+  free(ft);
+}
+
 static void add_noncons_surface(const double *UL, const double *UR, const double *nv, double *F)
 {
   double vL[3];
@@ -201,6 +341,13 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
     }
   }
   free(ft);
+  double *gx = NULL, *gy = NULL, *gz = NULL;
+  if (M->parabolic) {
+    const size_t nt = (size_t)MV * n3 * E;
+    gx = malloc(sizeof(double) * nt); gy = malloc(sizeof(double) * nt); gz = malloc(sizeof(double) * nt);
+    ora_mhd_lifting(M, U, gx, gy, gz);
+    mhd_volint_visc(M, U, gx, gy, gz, Dh, Ut);
+  }
   /* surface: FillFlux + SurfInt (LGL) */
   const double L0 = Lm[0];
   for (int s = 0; s < P->nSides; s++) {
@@ -217,6 +364,13 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
         const double *nv = P->NormVec + 3 * f, *t1 = P->TangVec1 + 3 * f, *t2 = P->TangVec2 + 3 * f;
         double Fm[MV], Fs[MV];
         ora_mhd_riemann(M, glm_ch, U + om, U + os, nv, t1, t2, Fm);
+        if (M->parabolic) {  /* BR1 average of the diffusive fluxes (mhd/riemann.f90:116-138) */
+          double kL[MV], gL[MV], jL[MV], kR[MV], gR[MV], jR[MV];
+          mhd_diff_flux(M, U + om, gx + om, gy + om, gz + om, kL, gL, jL);
+          mhd_diff_flux(M, U + os, gx + os, gy + os, gz + os, kR, gR, jR);
+          for (int v = 1; v < MV; v++)
+            Fm[v] = Fm[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
+        }
         memcpy(Fs, Fm, sizeof(Fs));
         add_noncons_surface(U + om, U + os, nv, Fm);
         add_noncons_surface(U + os, U + om, nv, Fs);
@@ -230,6 +384,7 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
     const double sj = -P->sJ[p];
     for (int v = 0; v < MV; v++) Ut[MV * p + v] = Ut[MV * p + v] * sj;
   }
+  free(gx); free(gy); free(gz);
 }
 
 /* InitTimeStep_GLM (mhd/calctimestep.f90:186-215): dt of a unit wave speed */
@@ -254,23 +409,35 @@ double ora_mhd_glm_dtch1(const ora_mhd *M, double CFLscale)
   return dtch1;
 }
 
-/* CALCTIMESTEP (mhd/calctimestep.f90:39-179), ideal MHD: returns dt; *glm_ch receives
-   GLM_scale * GLM_dtch1 / dt. */
-double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double glm_dtch1,
-                             double *glm_ch, int *errType)
+/* CALCTIMESTEP (mhd/calctimestep.f90:39-179): returns dt; *glm_ch receives
+   GLM_scale * GLM_dtch1 / dt.  Resistive case: the element's dt is also limited by
+   DFL*4/max_node(max(eta/mu_0, mu max(4/3, kappa/Pr)/rho) sum_d |sJ m_d|^2). */
+double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double DFLscale,
+                             double glm_dtch1, double *glm_ch, int *errType)
 {
   const ora_problem *P = M->P;
   const int n = P->N + 1, n3 = n * n * n;
   const double cfl = CFLscale * ora_cfl_alpha(P->N) / (2.0 * P->N + 1.0);
-  double dt = 1.79e308;
+  const double dfl = DFLscale * ora_dfl_alpha(P->N) / ((2.0 * P->N + 1.0) * (2.0 * P->N + 1.0));
+  const double muk = M->mu * fmax(4.0 / 3.0, M->kappa / M->Pr), diffc = M->eta * (1.0 / M->mu_0);
+  double dt = 1.79e308, dtv = 1.79e308;
   int err = 0;
   const double *Met[3] = {P->Mf, P->Mg, P->Mh};
   for (int e = 0; e < P->nElems && !err; e++) {
-    double lam[3] = {0, 0, 0};
+    double lam[3] = {0, 0, 0}, lamv = 0.0;
     for (int p = 0; p < n3; p++) {
       const size_t c = (size_t)e * n3 + p;
       const double *u = U + MV * c;
       if (isnan(u[0])) err = 1;
+      if (M->parabolic) {
+        double lv = 0.0;
+        for (int d = 0; d < 3; d++) {
+          const double *m = Met[d] + 3 * c;
+          const double a = m[0] * P->sJ[c], b = m[1] * P->sJ[c], cc = m[2] * P->sJ[c];
+          lv += a * a + b * b + cc * cc;
+        }
+        lamv = fmax(lamv, fmax(diffc, (1.0 / u[0]) * muk) * lv);
+      }
       double prim[MV];
       cons_to_prim(M, u, prim);
       if (prim[4] <= 0.0) err = 2;
@@ -284,7 +451,12 @@ double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale,
     const double dte = cfl * 2.0 / (lam[0] + lam[1] + lam[2]);
     dt = fmin(dt, dte);
     if (isnan(dt)) err = 3;
+    if (M->parabolic) {
+      if (lamv > 0.0) dtv = fmin(dtv, dfl * 4.0 / lamv);
+      if (isnan(dtv)) err = 4;
+    }
   }
+  dt = fmin(dt, dtv);
   if (errType) *errType = err;
   if (glm_ch) *glm_ch = M->glm_scale * (glm_dtch1 / dt);
   return dt;

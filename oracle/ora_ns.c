@@ -385,18 +385,17 @@ static void surf_int(const ora_problem *P, const ops_t *o, int nvar, const doubl
 }
 
 /* BR1 lifting (br1.f90:154-364, br1_volint.f90:45-98, br1_fillflux.f90:49-91,
-   br1_surfint.f90:52-131).
-   Output: primitive gradients at volume nodes. */
-void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz)
+   br1_surfint.f90:52-131) of nvar lifting variables w given at the volume nodes: returns the
+   physical gradients of w (Lifting_VolInt + strong surface flux 1/2(w_s-w_m) SurfElem nv on
+   both sides + V2D_M_V1D(sJ)); the equation converts them to primitive gradients. */
+void ora_lift(const ora_problem *P, int nvar, const double *w, double *gx, double *gy, double *gz)
 {
   ops_t o; make_ops(P, &o);
   int n = o.n, N = o.N;
-  size_t nvol = (size_t)NV * o.n3 * o.E, nface = (size_t)NV * n * n * o.S;
-  double *w = malloc(sizeof(double) * nvol);
+  size_t nvol = (size_t)nvar * o.n3 * o.E, nface = (size_t)nvar * n * n * o.S;
   double *wm = malloc(sizeof(double) * nface), *ws = malloc(sizeof(double) * nface);
   double *fx = malloc(sizeof(double) * nface), *fy = malloc(sizeof(double) * nface),
          *fz = malloc(sizeof(double) * nface);
-  for (size_t p = 0; p < (size_t)o.n3 * o.E; p++) ora_cons_to_entropy(P->kappa, U + NV * p, w + NV * p);
   memset(gx, 0, sizeof(double) * nvol);
   memset(gy, 0, sizeof(double) * nvol);
   memset(gz, 0, sizeof(double) * nvol);
@@ -405,57 +404,70 @@ void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, 
     for (int k = 0; k < n; k++)
       for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) {
-          double gxi[5], geta[5], gzeta[5];
-          for (int v = 0; v < NV; v++) {
-            gxi[v] = o.D[i + n * 0] * w[NV * VOL(e, 0, j, k) + v];
-            geta[v] = o.D[j + n * 0] * w[NV * VOL(e, i, 0, k) + v];
-            gzeta[v] = o.D[k + n * 0] * w[NV * VOL(e, i, j, 0) + v];
+          double gxi[16], geta[16], gzeta[16];
+          for (int v = 0; v < nvar; v++) {
+            gxi[v] = o.D[i + n * 0] * w[nvar * VOL(e, 0, j, k) + v];
+            geta[v] = o.D[j + n * 0] * w[nvar * VOL(e, i, 0, k) + v];
+            gzeta[v] = o.D[k + n * 0] * w[nvar * VOL(e, i, j, 0) + v];
           }
           for (int l = 1; l <= N; l++)
-            for (int v = 0; v < NV; v++) {
-              gxi[v] = gxi[v] + o.D[i + n * l] * w[NV * VOL(e, l, j, k) + v];
-              geta[v] = geta[v] + o.D[j + n * l] * w[NV * VOL(e, i, l, k) + v];
-              gzeta[v] = gzeta[v] + o.D[k + n * l] * w[NV * VOL(e, i, j, l) + v];
+            for (int v = 0; v < nvar; v++) {
+              gxi[v] = gxi[v] + o.D[i + n * l] * w[nvar * VOL(e, l, j, k) + v];
+              geta[v] = geta[v] + o.D[j + n * l] * w[nvar * VOL(e, i, l, k) + v];
+              gzeta[v] = gzeta[v] + o.D[k + n * l] * w[nvar * VOL(e, i, j, l) + v];
             }
           size_t c = VOL(e, i, j, k);
           const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
-          for (int v = 0; v < NV; v++) {
-            gx[NV * c + v] = gx[NV * c + v] + (mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v]);
-            gy[NV * c + v] = gy[NV * c + v] + (mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v]);
-            gz[NV * c + v] = gz[NV * c + v] + (mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v]);
+          for (int v = 0; v < nvar; v++) {
+            gx[nvar * c + v] = gx[nvar * c + v] + (mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v]);
+            gy[nvar * c + v] = gy[nvar * c + v] + (mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v]);
+            gz[nvar * c + v] = gz[nvar * c + v] + (mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v]);
           }
         }
   /* Lifting_FillFlux: strong flux 1/2(w_s-w_m)*SurfElem*nv (flux.f90:677-716) */
-  prolong_to_face(P, &o, NV, w, wm, ws);
+  prolong_to_face(P, &o, nvar, w, wm, ws);
   for (int s = 0; s < o.S; s++)
     for (int q = 0; q < n; q++)
       for (int p = 0; p < n; p++) {
         size_t f = FACE(s, p, q);
         const double *nv = P->NormVec + 3 * f;
         double se = P->SurfElem[f];
-        for (int v = 0; v < NV; v++) {
-          double fl = 0.5 * (ws[NV * f + v] - wm[NV * f + v]) * se;
-          fx[NV * f + v] = fl * nv[0];
-          fy[NV * f + v] = fl * nv[1];
-          fz[NV * f + v] = fl * nv[2];
+        for (int v = 0; v < nvar; v++) {
+          double fl = 0.5 * (ws[nvar * f + v] - wm[nvar * f + v]) * se;
+          fx[nvar * f + v] = fl * nv[0];
+          fy[nvar * f + v] = fl * nv[1];
+          fz[nvar * f + v] = fl * nv[2];
         }
       }
   /* Lifting_SurfInt: both sides add +Flux*L_hatMinus0 */
-  surf_int(P, &o, NV, fx, fx, 1.0, gx);
-  surf_int(P, &o, NV, fy, fy, 1.0, gy);
-  surf_int(P, &o, NV, fz, fz, 1.0, gz);
+  surf_int(P, &o, nvar, fx, fx, 1.0, gx);
+  surf_int(P, &o, nvar, fy, fy, 1.0, gy);
+  surf_int(P, &o, nvar, fz, fz, 1.0, gz);
   for (size_t p = 0; p < (size_t)o.n3 * o.E; p++) {
     double sj = P->sJ[p];
-    for (int v = 0; v < NV; v++) {
-      gx[NV * p + v] = gx[NV * p + v] * sj;
-      gy[NV * p + v] = gy[NV * p + v] * sj;
-      gz[NV * p + v] = gz[NV * p + v] * sj;
+    for (int v = 0; v < nvar; v++) {
+      gx[nvar * p + v] = gx[nvar * p + v] * sj;
+      gy[nvar * p + v] = gy[nvar * p + v] * sj;
+      gz[nvar * p + v] = gz[nvar * p + v] * sj;
     }
+  }
+  free(wm); free(ws); free(fx); free(fy); free(fz);
+}
+
+/* BR1 lifting of the Navier-Stokes entropy variables; output: primitive gradients at the
+   volume nodes. */
+void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz)
+{
+  size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  double *w = calloc((size_t)NV * n3E, sizeof(double));
+  for (size_t p = 0; p < n3E; p++) ora_cons_to_entropy(P->kappa, U + NV * p, w + NV * p);
+  ora_lift(P, NV, w, gx, gy, gz);
+  for (size_t p = 0; p < n3E; p++) {
     convert_to_grad_prim(P->kappa, U + NV * p, gx + NV * p);
     convert_to_grad_prim(P->kappa, U + NV * p, gy + NV * p);
     convert_to_grad_prim(P->kappa, U + NV * p, gz + NV * p);
   }
-  free(w); free(wm); free(ws); free(fx); free(fy); free(fz);
+  free(w);
 }
 
 /* FillFlux + SurfInt (fillflux.f90:41-110, riemann.f90:104-169, surfint.f90) */

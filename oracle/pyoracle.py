@@ -62,8 +62,9 @@ def lib():
         L.ora_mhd_glm_dtch1.restype = C.c_double
         L.ora_mhd_glm_dtch1.argtypes = [C.c_void_p, C.c_double]
         L.ora_mhd_calc_timestep.restype = C.c_double
-        L.ora_mhd_calc_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double,
+        L.ora_mhd_calc_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.ora_mhd_lifting.argtypes = [C.c_void_p] + [_dp] * 4
         L.ora_mhd_lserk_step.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double]
         L.ora_mhd_orszag_tang.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.ora_mhd_derigs.argtypes = [C.c_void_p, C.c_double] + [_dp] * 5
@@ -264,21 +265,29 @@ def nfvse_threshold(N):
 
 
 class _Mhd(C.Structure):
-    _fields_ = [("P", C.c_void_p), ("kappa", C.c_double), ("mu_0", C.c_double), ("glm_scale", C.c_double)]
+    _fields_ = [("P", C.c_void_p), ("kappa", C.c_double), ("mu_0", C.c_double), ("glm_scale", C.c_double),
+                ("parabolic", C.c_int), ("reserved", C.c_int), ("mu", C.c_double), ("eta", C.c_double),
+                ("Pr", C.c_double)]
 
 
 class MhdProblem:
-    """Ideal GLM-MHD setup (9 variables) on a mesh: Derigs volume flux, Rusanov surface
-    flux, Powell non-conservative terms.  The cleaning speed GLM_ch is explicit state: 0.5
-    before the first time step (the reference's initial value), then
-    glm_scale * dtch1 / dt from every CALCTIMESTEP."""
+    """GLM-MHD setup (9 variables) on a mesh: Derigs volume flux, Rusanov surface flux,
+    Powell non-conservative terms; resistive (BR1 lifting of the entropy variables, viscosity
+    mu, resistivity eta, heat conduction mu kappa/Pr) when `resistive` is given as
+    dict(mu=, eta=, Pr=).  The cleaning speed GLM_ch is explicit state: 0.5 before the first
+    time step (the reference's initial value), then glm_scale * dtch1 / dt from every
+    CALCTIMESTEP."""
 
-    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5):
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, resistive=None):
         self.base = Problem(mesh, volflux=10, riemann=1, kappa=kappa)
         self.mesh, self.N, self.nElems = mesh, mesh["N"], mesh["nElems"]
         m = _Mhd()
         m.P = C.cast(C.byref(self.base._p), C.c_void_p).value
         m.kappa, m.mu_0, m.glm_scale = kappa, mu_0, glm_scale
+        if resistive is not None:
+            m.parabolic = 1
+            m.mu, m.eta, m.Pr = float(resistive["mu"]), float(resistive["eta"]), float(resistive.get("Pr", 0.72))
+        self.parabolic = resistive is not None
         self._m = m
         self.glm_ch = 0.5
         self.glm_dtch1 = None
@@ -302,12 +311,18 @@ class MhdProblem:
         lib().ora_mhd_rhs(self.ptr, np.ascontiguousarray(U), self.glm_ch if glm_ch is None else glm_ch, Ut)
         return Ut
 
-    def calc_dt(self, U, cfl):
+    def lifting(self, U):
+        """Primitive gradients (rho, v, p, B, psi) x, y, z of the resistive case."""
+        g = [np.zeros(self.shape()) for _ in range(3)]
+        lib().ora_mhd_lifting(self.ptr, np.ascontiguousarray(U), *g)
+        return g
+
+    def calc_dt(self, U, cfl, dfl=0.9):
         """CALCTIMESTEP; updates glm_ch (and dtch1 on the first call) like the reference."""
         if self.glm_dtch1 is None:
             self.glm_dtch1 = lib().ora_mhd_glm_dtch1(self.ptr, cfl)
         ch, err = C.c_double(0.0), C.c_int(0)
-        dt = lib().ora_mhd_calc_timestep(self.ptr, np.ascontiguousarray(U), cfl, self.glm_dtch1,
+        dt = lib().ora_mhd_calc_timestep(self.ptr, np.ascontiguousarray(U), cfl, dfl, self.glm_dtch1,
                                          C.byref(ch), C.byref(err))
         self.glm_ch = ch.value
         return dt, err.value

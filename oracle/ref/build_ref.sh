@@ -60,6 +60,10 @@ build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
 build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
 # ideal GLM-MHD with Powell non-conservative terms (FLUXO_EQNSYSNAME=mhd, FLUXO_EQN_GLM,
 # FLUXO_EQN_NONCONS=Powell; CMakeLists.txt:174-209), Derigs flux (VolumeFlux=10)
-build_variant mhd_vf10 -DPP_nVar=9 -Dmhd -DPP_entropy_vars_exist -DPP_u_aux_exist -DPP_GLM=T -DNONCONS=1 $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
+MHD="-DPP_nVar=9 -Dmhd -DPP_entropy_vars_exist -DPP_u_aux_exist -DPP_GLM=T -DNONCONS=1"
+build_variant mhd_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
+# resistive GLM-MHD: viscosity, resistivity and heat conduction through BR1 lifting of the
+# entropy variables (FLUXO_PARABOLIC, CMakeLists.txt:253-283)
+build_variant mhd_br1_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3
 # NFVSE shock capturing (FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE, CMakeLists.txt:316-331)
 build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0

@@ -17,7 +17,7 @@ REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 def binary(parabolic, volflux, nfvse=False, mhd=False):
     if mhd:
-        name = f"fluxo_ref_mhd_vf{volflux}"
+        name = f"fluxo_ref_mhd{'_br1' if parabolic else ''}_vf{volflux}"
     else:
         name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
     path = os.path.join(REF_DIR, name)
@@ -95,24 +95,31 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     return out
 
 
-def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600):
-    """Ideal GLM-MHD (Derigs volume flux, Rusanov surface flux, Powell non-conservative
-    terms) with the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before
-    the first CALCTIMESTEP), dt with GLM_ch and GLM_dtch1 after it, and an nsteps trajectory.
+def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600,
+            resistive=None, dfl=0.9):
+    """GLM-MHD (Derigs volume flux, Rusanov surface flux, Powell non-conservative terms) with
+    the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before the first
+    CALCTIMESTEP), dt with GLM_ch and GLM_dtch1 after it, and an nsteps trajectory.
+    resistive: None (ideal), or dict(mu=, eta=, Pr=) for the BR1 build with viscosity,
+    resistivity and heat conduction, which also returns the primitive gradients.
     U: (nElems, n, n, n, 9)."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
-    exe = binary(False, 10, mhd=True)
+    par_on = resistive is not None
+    exe = binary(par_on, 10, mhd=True)
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
             f.write(f"N = {N}\nRiemann = 1\nVolumeFlux = 10\nIniExactFunc = 1\nIniRefState = 1\n"
                     f"RefState = (/1.,0.,0.,0.,1.,0.,0.,0./)\nkappa = {kappa!r}\nmu_0 = {mu_0!r}\n"
                     f"GLM_scale = {glm_scale!r}\n")
+            if par_on:
+                f.write(f"mu = {float(resistive['mu'])!r}\neta = {float(resistive['eta'])!r}\n"
+                        f"Pr = {float(resistive.get('Pr', 0.72))!r}\n")
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
-            np.array([N, Ngeo, E, S, nsteps, 0, 0, 0], np.int32).tofile(f)
-            np.array([cfl, 0.0, t0, 0.0], np.float64).tofile(f)
+            np.array([N, Ngeo, E, S, nsteps, int(par_on), 0, 0], np.int32).tofile(f)
+            np.array([cfl, dfl if par_on else 0.0, t0, 0.0], np.float64).tofile(f)
             np.ascontiguousarray(mesh["NodeCoords"], np.float64).tofile(f)
             np.ascontiguousarray(mesh["ElemToSide"], np.int32).tofile(f)
             np.ascontiguousarray(mesh["SideToElem"], np.int32).tofile(f)
@@ -138,6 +145,9 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
     take("SurfElem", (S, n, n))
     take("volint", (E, n, n, n, 9))
     take("Ut", (E, n, n, n, 9))
+    if par_on:
+        for k in ("gradPx", "gradPy", "gradPz"):
+            take(k, (E, n, n, n, 9))
     take("dt0", (2,))
     out["errType0"] = int(out["dt0"][1])
     out["dt0"] = float(out["dt0"][0])

@@ -1,6 +1,6 @@
 """The oracle's ideal GLM-MHD (Derigs volume flux, Rusanov surface flux, Powell
 non-conservative terms, CALCTIMESTEP with the GLM cleaning speed) against the reference's
-own outputs (tests/golden/mhd, made by the reference built for mhd), and checks the
+own outputs (tests/golden/mhd, made by the reference built for mhd, ideal and resistive), and checks the
 mathematics fixes: consistency (closed-form physical flux) and symmetry of the two-point
 flux, discrete entropy conservation / stability, rotational invariance of the surface flux,
 free-stream preservation."""
@@ -25,6 +25,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+def resistive_of(g):
+    return dict(mu=float(g["mu"]), eta=float(g["eta"]), Pr=float(g["Pr"])) if "mu" in g else None
+
+
 def mesh_of(g):
     return O.box_mesh(int(g["K"]), int(g["N"]), x0=tuple(g["x0"]), length=tuple(g["L"]), Ngeo=int(g["Ngeo"]),
                       warp=bool(g["warp"]))
@@ -34,17 +38,24 @@ def mesh_of(g):
 def test_oracle_matches_reference_mhd(name):
     g = load(name)
     m = mesh_of(g)
-    P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]))
+    res = resistive_of(g)
+    dfl = float(g["dfl"]) if res else 0.0
+    P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
+                     resistive=res)
     U = g["U"]
+    if res:
+        for a, k in zip(P.lifting(U), ("gradPx", "gradPy", "gradPz")):
+            assert rel_l2(a, g[k]) <= 1e-14
     assert rel_l2(P.rhs(U, glm_ch=0.5), g["Ut"]) <= 1e-14
-    dt, err = P.calc_dt(U, float(g["cfl"]))
+    dt, err = P.calc_dt(U, float(g["cfl"]), dfl)
     assert err == int(g["errType0"]) and dt == pytest.approx(float(g["dt0"]), rel=1e-15)
     assert P.glm_dtch1 == pytest.approx(float(g["glm_dtch1"]), rel=1e-15)
     assert P.glm_ch == pytest.approx(float(g["glm_ch0"]), rel=1e-15)
-    P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]))
+    P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
+                     resistive=res)
     Ue, Utmp, t = U.copy(), np.zeros_like(U), 0.0
     for _ in range(int(g["nsteps"])):
-        d, _ = P.calc_dt(Ue, float(g["cfl"]))
+        d, _ = P.calc_dt(Ue, float(g["cfl"]), dfl)
         P.lserk_step(Ue, Utmp, d)
         t += d
     assert rel_l2(Ue, g["U_end"]) <= 1e-13
@@ -171,3 +182,28 @@ def test_free_stream_warped():
     U = np.zeros(P.shape())
     U[...] = [1.1, 0.2, -0.1, 0.3, 4.0, 0.5, -0.4, 0.2, 0.0]
     assert np.abs(P.rhs(U, glm_ch=0.9)).max() < 1e-11
+
+
+def test_resistive_terms_linear_and_dissipative():
+    """Resistive GLM-MHD on a warped mesh, continuous data with psi = 0: the diffusive part
+    Ut(mu, eta) - Ut(0, 0) is linear in (mu, eta) at fixed Prandtl number, and BR1 lifting of
+    the entropy variables makes its entropy production strictly negative for viscosity alone
+    and for resistivity alone (the ideal part produces none on continuous data)."""
+    m = O.box_mesh(3, 3, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    U = O.MhdProblem(m).exact_orszag_tang()
+    U[..., 8] = 0.0
+    base = O.MhdProblem(m, resistive=dict(mu=0.0, eta=0.0, Pr=0.72)).rhs(U, glm_ch=1.0)
+    ideal = O.MhdProblem(m).rhs(U, glm_ch=1.0)
+    assert rel_l2(base, ideal) <= 1e-15
+
+    def diff(mu, eta):
+        return O.MhdProblem(m, resistive=dict(mu=mu, eta=eta, Pr=0.72)).rhs(U, glm_ch=1.0) - base
+
+    d1, d2 = diff(1e-3, 2e-3), diff(2e-3, 4e-3)
+    assert rel_l2(d2, 2.0 * d1) <= 1e-10
+    assert rel_l2(diff(1e-3, 0.0) + diff(0.0, 2e-3), d1) <= 1e-10
+    prod0, scale0 = entropy_production(m, U, base)
+    assert abs(prod0) <= 1e-13 * scale0
+    for mu, eta in ((1e-3, 0.0), (0.0, 1e-3)):
+        prod, scale = entropy_production(m, U, base + diff(mu, eta))
+        assert prod < -1e-3 * scale
