@@ -53,9 +53,11 @@ int check_config(const fx_config *c)
   if (c->nVar == 9) {
     if (c->volumeFlux != 10 || c->riemann != 1)
       return set_err(FX_EINVAL, "GLM-MHD: VolumeFlux=10 (Derigs) and Riemann=1 (Rusanov) only");
-    if (c->parabolic || c->nfvse) return set_err(FX_EINVAL, "GLM-MHD: ideal (parabolic=0), no shock capturing");
+    if (c->nfvse) return set_err(FX_EINVAL, "GLM-MHD: no shock capturing");
     if (!(c->kappa > 1.0) || !(c->mu_0 > 0.0) || !(c->GLM_scale > 0.0))
       return set_err(FX_EINVAL, "GLM-MHD: kappa > 1, mu_0 > 0, GLM_scale > 0");
+    if (c->parabolic && (c->lifting != 1 || !(c->Pr > 0.0) || !(c->mu0 >= 0.0) || !(c->eta >= 0.0)))
+      return set_err(FX_EINVAL, "resistive GLM-MHD: BR1 lifting (lifting=1), Pr > 0, mu >= 0, eta >= 0");
     if (c->precision != FX_PREC_FP64 && c->precision != FX_PREC_FP32_SPLIT)
       return set_err(FX_EINVAL, "precision %d not supported", c->precision);
     return FX_OK;
@@ -84,7 +86,7 @@ long long fx_ctx_bytes(const fx_config *cfg)
   b += 5LL * n * n * S * 8;          // Flux
   b += 4LL * n * n * S * 8;          // nvse
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
-  if (cfg->parabolic) b += 12LL * n3 * E * 8 + 15LL * n3 * E * 8;
+  if (cfg->parabolic) b += 12LL * n3 * E * 8 + 3LL * cfg->nVar * n3 * E * 8;  // Fv, gradients
   if (cfg->nfvse) b += 5LL * n3 * E * 8 + 2LL * E * 8;  // Ffv, alpha buffers
   if (cfg->nVar == 9) b += 4LL * 3 * n3 * E * 8 + (9LL + 4) * n * n * S * 8 * 2 + E * 8;  // MHD extra
   b += 2LL * ((n3 * E + 255) / 256 + 1) * 8 + 64;

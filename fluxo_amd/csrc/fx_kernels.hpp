@@ -8,7 +8,8 @@
 //   Fv            : 12 x (N+1)^3 x nElems, Cartesian viscous fluxes (f,g,h; momentum+energy)
 //
 // Kernel roles per RHS (one LSERK stage):
-//   k_lifting  (Navier-Stokes only) element-centric BR1 gradients -> viscous fluxes Fv
+//   k_lifting  element-centric BR1 gradients -> viscous fluxes Fv (Navier-Stokes,
+//              resistive GLM-MHD)
 //   k_surface  side-centric numerical flux at every face point  -> Flux
 //   k_volume   element-centric flux differencing + viscous volume term + surface
 //              gather + Jacobian + (optionally) the low-storage RK stage update.
@@ -428,41 +429,57 @@ k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int
 // flux.f90:677-716, Lifting_SurfInt br1_surfint.f90:52-131, V2D_M_V1D(sJ),
 // ConvertToGradPrimVec).  Element-centric: the surface term of every face is formed from
 // the neighbour's face node directly.  OUT=0 writes the primitive gradients (gradPx/y/z),
-// OUT=1 writes the Cartesian viscous fluxes (EvalDiffFlux3D) consumed by the surface and
-// volume kernels.
+// OUT=1 writes the 12 independent Cartesian viscous flux values per node (EvalDiffFlux3D)
+// consumed by the surface and volume kernels.  EQ supplies the equation's entropy
+// variables, gradient conversion and diffusive fluxes (NsLift; MhdLift for resistive
+// GLM-MHD).
 template <int N> struct LiftGeom { static constexpr int EPB = (256 / ((N + 1) * (N + 1) * (N + 1))) > 0 ? (256 / ((N + 1) * (N + 1) * (N + 1))) : 1; };
 
-template <int N, int OUT>
+// Navier-Stokes lifting physics: entropy variables (ConsToEntropy), the primitive-gradient
+// conversion and the 12 viscous flux values per node (f, g, h of momentum and energy).
+struct NsLift {
+  static constexpr int NV = 5;
+  double kappa, mu0, KappasPr;
+  __device__ __forceinline__ void entropy(const double *u, double *w) const { cons_to_entropy(u, kappa, w); }
+  __device__ __forceinline__ void grad_prim(const double *u, double *g) const { grad_entropy_to_prim(u, kappa, g); }
+  __device__ __forceinline__ void fluxes(const double *u, const double *gx, const double *gy, const double *gz,
+                                         double *fv) const
+  {
+    diff_flux(u, gx, gy, gz, kappa, mu0, KappasPr, fv);
+  }
+};
+
+template <int N, int OUT, class EQ>
 __global__ void __launch_bounds__(256)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
-          int nElems, double kappa, double mu0, double KappasPr)
+          int nElems, EQ eq)
 {
-  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV;
   constexpr int EPB = LiftGeom<N>::EPB;
-  __shared__ double s_w[EPB][5][n3];
+  __shared__ double s_w[EPB][NV][n3];
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
   const int el = tid / n3, node = tid - el * n3;
   const int e = blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (e < nElems) && elem_problem(eprob, e) >= 0;
-  double u[5], w[5];
+  double u[NV], w[NV];
   if (active) {
-    const double *up = U + ((size_t)e * n3 + node) * 5;
+    const double *up = U + ((size_t)e * n3 + node) * NV;
 #pragma unroll
-    for (int v = 0; v < 5; v++) u[v] = up[v];
-    cons_to_entropy(u, kappa, w);
+    for (int v = 0; v < NV; v++) u[v] = up[v];
+    eq.entropy(u, w);
 #pragma unroll
-    for (int v = 0; v < 5; v++) s_w[el][v][node] = w[v];
+    for (int v = 0; v < NV; v++) s_w[el][v][node] = w[v];
   }
   __syncthreads();
   if (!active) return;
   const int i = node % n, j = (node / n) % n, k = node / n2;
-  double gxi[5], geta[5], gzeta[5];
+  double gxi[NV], geta[NV], gzeta[NV];
 #pragma unroll
-  for (int v = 0; v < 5; v++) {
+  for (int v = 0; v < NV; v++) {
     gxi[v] = op.D[i][0] * s_w[el][v][0 + n * (j + n * k)];
     geta[v] = op.D[j][0] * s_w[el][v][i + n * (0 + n * k)];
     gzeta[v] = op.D[k][0] * s_w[el][v][i + n * (j + n * 0)];
@@ -470,16 +487,16 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 #pragma unroll
   for (int l = 1; l < n; l++)
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
       gxi[v] = gxi[v] + op.D[i][l] * s_w[el][v][l + n * (j + n * k)];
       geta[v] = geta[v] + op.D[j][l] * s_w[el][v][i + n * (l + n * k)];
       gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
     }
   const size_t c = (size_t)e * n3 + node;
   const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-  double gx[5], gy[5], gz[5];
+  double gx[NV], gy[NV], gz[NV];
 #pragma unroll
-  for (int v = 0; v < 5; v++) {
+  for (int v = 0; v < NV; v++) {
     gx[v] = mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v];
     gy[v] = mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v];
     gz[v] = mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v];
@@ -506,12 +523,14 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       enb = s2e[5 * s] - 1;
       vnb = side_to_vol<N>(s2e[5 * s + 2], 0, p, q);
     }
-    const double *unbp = U + ((size_t)enb * n3 + vnb) * 5;
-    double unb[5] = {unbp[0], unbp[1], unbp[2], unbp[3], unbp[4]}, wnb[5];
-    cons_to_entropy(unb, kappa, wnb);
+    const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
+    double unb[NV], wnb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) unb[v] = unbp[v];
+    eq.entropy(unb, wnb);
     const double4 g = nvse[(size_t)(s * n + q) * n + p];
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
       const double ws = (fl == 0) ? wnb[v] : w[v];
       const double wm = (fl == 0) ? w[v] : wnb[v];
       const double f = 0.5 * (ws - wm) * g.w;
@@ -522,16 +541,16 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   }
   const double sj = sJ[c];
 #pragma unroll
-  for (int v = 0; v < 5; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
-  grad_entropy_to_prim(u, kappa, gx);
-  grad_entropy_to_prim(u, kappa, gy);
-  grad_entropy_to_prim(u, kappa, gz);
+  for (int v = 0; v < NV; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+  eq.grad_prim(u, gx);
+  eq.grad_prim(u, gy);
+  eq.grad_prim(u, gz);
   if (OUT == 0) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) { out0[5 * c + v] = gx[v]; out1[5 * c + v] = gy[v]; out2[5 * c + v] = gz[v]; }
+    for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
   } else {
     double fv[12];
-    diff_flux(u, gx, gy, gz, kappa, mu0, KappasPr, fv);
+    eq.fluxes(u, gx, gy, gz, fv);
 #pragma unroll
     for (int m = 0; m < 12; m++) out0[12 * c + m] = fv[m];
   }

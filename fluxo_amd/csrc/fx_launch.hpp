@@ -389,22 +389,39 @@ void launch_surface_n(const Batch &b)
   }
 }
 
-template <int N>
-void launch_lifting_n(const Batch &b, int out)
+template <int N, class EQ>
+void launch_lifting_e(const Batch &b, int out, const EQ &eq)
 {
   const int epb = LiftGeom<N>::EPB;
   const int nb = (b.cfg.nElems + epb - 1) / epb;
   if (nb <= 0) return;
-  const double KsPr = b.cfg.kappa / b.cfg.Pr;
-  const size_t nv = (size_t)5 * b.n3 * b.cfg.nElems;
+  const size_t nv = (size_t)EQ::NV * b.n3 * b.cfg.nElems;
   if (out == 0)
-    hipLaunchKernelGGL((k_lifting<N, 0>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems,
-                       b.cfg.kappa, b.cfg.mu0, KsPr);
+    hipLaunchKernelGGL((k_lifting<N, 0, EQ>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq);
   else
-    hipLaunchKernelGGL((k_lifting<N, 1>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.Fv, nullptr, nullptr, b.eprob, b.cfg.nElems, b.cfg.kappa,
-                       b.cfg.mu0, KsPr);
+    hipLaunchKernelGGL((k_lifting<N, 1, EQ>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.Fv, nullptr, nullptr, b.eprob, b.cfg.nElems, eq);
+}
+
+template <int N>
+void launch_lifting_n(const Batch &b, int out)
+{
+  if (b.mhd()) {
+    MhdLift eq;
+    eq.kappa = b.cfg.kappa;
+    eq.smu0 = 1.0 / b.cfg.mu_0;
+    eq.mu = b.cfg.mu0;
+    eq.etasmu0 = b.cfg.eta * eq.smu0;
+    eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
+    launch_lifting_e<N>(b, out, eq);
+  } else {
+    NsLift eq;
+    eq.kappa = b.cfg.kappa;
+    eq.mu0 = b.cfg.mu0;
+    eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
+    launch_lifting_e<N>(b, out, eq);
+  }
 }
 
 template <int N>
@@ -420,25 +437,34 @@ void launch_dt_n(const Batch &b, double cfl, double dfl)
                      b.cfg.parabolic);
 }
 
-template <int N, int MODE>
-void launch_mhd_volume_t(const Batch &b, RKArgs rk)
+template <int N, int MODE, bool PARA>
+void launch_mhd_volume_p(const Batch &b, RKArgs rk)
 {
   constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W;
   const int nb = (b.cfg.nElems + EPB - 1) / EPB;
   if (nb <= 0) return;
   if (b.cfg.precision == FX_PREC_FP32_SPLIT)
-    hipLaunchKernelGGL((k_mhd_volume<N, MODE, float>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf32, b.Mg32,
-                       b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(), b.st);
+    hipLaunchKernelGGL((k_mhd_volume<N, MODE, float, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf32,
+                       b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.mhd_args(), b.st);
   else
-    hipLaunchKernelGGL((k_mhd_volume<N, MODE, double>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh,
-                       b.sJ, b.e2s, b.Flux, b.FluxS, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(), b.st);
+    hipLaunchKernelGGL((k_mhd_volume<N, MODE, double, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf, b.Mg,
+                       b.Mh, b.sJ, b.e2s, b.Flux, b.FluxS, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(),
+                       b.st);
+}
+
+template <int N, int MODE>
+void launch_mhd_volume_t(const Batch &b, RKArgs rk)
+{
+  if (b.cfg.parabolic) launch_mhd_volume_p<N, MODE, true>(b, rk);
+  else launch_mhd_volume_p<N, MODE, false>(b, rk);
 }
 
 template <int N>
 void launch_mhd_volume_n(const Batch &b, int mode, RKArgs rk)
 {
   if (mode == VMODE_RK) launch_mhd_volume_t<N, VMODE_RK>(b, rk);
-  else if (mode == VMODE_VOLINT) launch_mhd_volume_t<N, VMODE_VOLINT>(b, rk);
+  else if (mode == VMODE_VOLINT) launch_mhd_volume_p<N, VMODE_VOLINT, false>(b, rk);
   else launch_mhd_volume_t<N, VMODE_RHS>(b, rk);
 }
 
@@ -448,17 +474,28 @@ void launch_mhd_surface_n(const Batch &b)
   const long long tot = (long long)b.cfg.nSides * b.n2;
   const int nb = (int)((tot + 255) / 256);
   if (nb <= 0) return;
-  hipLaunchKernelGGL((k_mhd_surface<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.s2e, b.nvse, b.Flux, b.FluxS,
-                     b.cfg.nSides, b.mhd_args(), b.st);
+  if (b.cfg.parabolic)
+    hipLaunchKernelGGL((k_mhd_surface<N, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e, b.nvse, b.Flux,
+                       b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
+  else
+    hipLaunchKernelGGL((k_mhd_surface<N, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e, b.nvse,
+                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
 }
 
 template <int N>
-void launch_mhd_dt_n(const Batch &b, double cfl)
+void launch_mhd_dt_n(const Batch &b, double cfl, double dfl)
 {
   const int epb = DtGeom<N>::EPB, nb = (b.cfg.nElems + epb - 1) / epb;
   if (nb <= 0) return;
+  MhdDtArgs da;
+  da.cfl = cfl;
+  da.dfl = dfl;
+  const double KsPr = b.cfg.kappa / b.cfg.Pr;
+  da.muk = b.cfg.mu0 * ((4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr);
+  da.diffc = b.cfg.eta * (1.0 / b.cfg.mu_0);
+  da.parabolic = b.cfg.parabolic;
   hipLaunchKernelGGL((k_mhd_calc_dt<N>), dim3(nb), dim3(DtGeom<N>::T), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                     b.dtElem, b.dtch1Elem, b.errElem, b.cfg.nElems, b.mhd_args(), cfl);
+                     b.dtElem, b.dtch1Elem, b.errElem, b.cfg.nElems, b.mhd_args(), da);
 }
 
 #define FX_DISPATCH_N(N_, fn, ...)             \
@@ -486,7 +523,7 @@ inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_l
 inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
 {
   if (b.mhd()) {  // single problem only
-    FX_DISPATCH_N(b.cfg.N, launch_mhd_dt_n, b, cfl);
+    FX_DISPATCH_N(b.cfg.N, launch_mhd_dt_n, b, cfl, dfl);
     hipLaunchKernelGGL(k_mhd_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.dtch1Elem, b.errElem,
                        b.cfg.nElems, b.st, advance, b.cfg.GLM_scale);
     return;

@@ -1,8 +1,11 @@
-// Ideal GLM-MHD (9 variables: rho, rho v, E, B, psi) on gfx950: the Derigs et al. entropy-
-// and kinetic-energy-conserving two-point flux, Powell non-conservative terms, the Rusanov
+// GLM-MHD (9 variables: rho, rho v, E, B, psi) on gfx950: the Derigs et al. entropy- and
+// kinetic-energy-conserving two-point flux, Powell non-conservative terms, the Rusanov
 // surface flux, CALCTIMESTEP with the GLM cleaning speed from the time step, and the LSERK
 // stage update (src/equation/mhd/{flux_average,flux,riemann,equation_vars,calctimestep}.f90,
-// NONCONS branches of src/dg/{volint,fillflux,surfint}.f90).
+// NONCONS branches of src/dg/{volint,fillflux,surfint}.f90).  Resistive case (PARABOLIC):
+// k_lifting with MhdLift (BR1 lifting of the entropy variables) writes 12 independent
+// diffusive flux values per node; the surface kernel adds their BR1 face average, the
+// volume kernel the weak-form D_hat term (VolInt_visc), the time step the viscous limit.
 //
 // Kernels per RK stage: k_mhd_surface (side-centric: conservative flux plus the
 // non-conservative term of each side -> FluxM, FluxS) and k_mhd_volume (element-centric like
@@ -109,6 +112,94 @@ __device__ __forceinline__ void mhd_flux_n(const double *U, const double *n, con
   cf = sqrt(0.5 * (c2 + va2 + astar));
 }
 
+// Resistive GLM-MHD lifting physics: entropy variables (ConsToEntropy
+// mhd/equation_vars.f90:330-361), the conversion of their gradients to gradients of
+// (rho, v, p, B, psi) (ConvertToGradPrimVec, :547-628) and the isotropic diffusive fluxes
+// (EvalDiffFlux3D mhd/flux.f90:449-573).  The 18 nonzero Cartesian components of f, g, h
+// hold 12 independent values, written per node as
+//   s = (f1, f2, f3, g2, g3, h3, f6, f7, g7, f4, g4, h4)
+// (f1..f3 the viscous stress row x, f6/f7/g7 the resistive terms, f4/g4/h4 the energy
+// fluxes); g1 = f2, h1 = f3, h2 = g3, g5 = -f6, h5 = -f7, h6 = -g7, the rest vanish.
+struct MhdLift {
+  static constexpr int NV = 9;
+  double kappa, smu0, mu, etasmu0, KappasPr;
+  __device__ __forceinline__ void entropy(const double *c, double *w) const
+  {
+    const double km1 = kappa - 1.0, s2mu0 = 0.5 * smu0;
+    const double srho = 1.0 / c[0];
+    const double u = c[1] * srho, v = c[2] * srho, ww = c[3] * srho;
+    const double v2s2 = 0.5 * (u * u + v * v + ww * ww);
+    const double b2 = c[5] * c[5] + c[6] * c[6] + c[7] * c[7] + c[8] * c[8];
+    const double rho_sp = c[0] / (km1 * (c[4] - c[0] * v2s2 - s2mu0 * b2));
+    const double s = -log(rho_sp * pow(c[0], km1));
+    w[0] = (kappa - s) * (1.0 / km1) - rho_sp * v2s2;
+    w[1] = rho_sp * u;
+    w[2] = rho_sp * v;
+    w[3] = rho_sp * ww;
+    w[4] = -rho_sp;
+#pragma unroll
+    for (int i = 5; i < NV; i++) w[i] = rho_sp * c[i];
+  }
+  __device__ __forceinline__ void grad_prim(const double *c, double *g) const
+  {
+    const double km1 = kappa - 1.0, s2mu0 = 0.5 * smu0;
+    const double srho = 1.0 / c[0];
+    const double u = c[1] * srho, v = c[2] * srho, w = c[3] * srho;
+    const double ekin = 0.5 * (c[1] * u + c[2] * v + c[3] * w);
+    const double b2 = c[5] * c[5] + c[6] * c[6] + c[7] * c[7] + c[8] * c[8];
+    const double p = km1 * (c[4] - ekin - s2mu0 * b2);
+    const double rho_sp = c[0] / p, p_srho = p * srho;
+    const double g1 = g[0], g5 = g[4];
+    const double gu = p_srho * (g[1] + u * g5), gv = p_srho * (g[2] + v * g5), gw = p_srho * (g[3] + w * g5);
+    g[0] = c[0] * g1 - g5 * (ekin - p * (1.0 / km1)) + rho_sp * (c[1] * gu + c[2] * gv + c[3] * gw);
+    g[1] = gu;
+    g[2] = gv;
+    g[3] = gw;
+    g[4] = p_srho * (g[0] + p * g5);
+#pragma unroll
+    for (int i = 5; i < NV; i++) g[i] = p_srho * (g[i] + c[i] * g5);
+  }
+  __device__ __forceinline__ void fluxes(const double *c, const double *gx, const double *gy, const double *gz,
+                                         double *sv) const
+  {
+    const double km1 = kappa - 1.0, skm1 = 1.0 / km1, s2mu0 = 0.5 * smu0, s23 = 2.0 / 3.0;
+    const double srho = 1.0 / c[0];
+    const double v1 = c[1] * srho, v2 = c[2] * srho, v3 = c[3] * srho;
+    const double b2 = c[5] * c[5] + c[6] * c[6] + c[7] * c[7] + c[8] * c[8];
+    const double p = km1 * (c[4] - 0.5 * (c[0] * (v1 * v1 + v2 * v2 + v3 * v3)) - s2mu0 * b2);
+    const double b1 = c[5], bb2 = c[6], b3 = c[7];
+    const double divv = gx[1] + gy[2] + gz[3];
+    const double lambda = mu * KappasPr;
+    const double qx = lambda * (skm1 * srho * (gx[4] - srho * p * gx[0]));
+    const double qy = lambda * (skm1 * srho * (gy[4] - srho * p * gy[0]));
+    const double qz = lambda * (skm1 * srho * (gz[4] - srho * p * gz[0]));
+    const double f1 = -mu * (2 * gx[1] - s23 * divv), f2 = -mu * (gx[2] + gy[1]), f3 = -mu * (gx[3] + gz[1]);
+    const double g2 = -mu * (2 * gy[2] - s23 * divv), g3 = -mu * (gy[3] + gz[2]);
+    const double h3 = -mu * (2 * gz[3] - s23 * divv);
+    const double f6 = -etasmu0 * (gx[6] - gy[5]), f7 = -etasmu0 * (gx[7] - gz[5]);
+    const double g7 = -etasmu0 * (gy[7] - gz[6]);
+    sv[0] = f1; sv[1] = f2; sv[2] = f3; sv[3] = g2; sv[4] = g3; sv[5] = h3;
+    sv[6] = f6; sv[7] = f7; sv[8] = g7;
+    sv[9] = v1 * f1 + v2 * f2 + v3 * f3 + smu0 * (bb2 * f6 + b3 * f7) - qx;
+    sv[10] = v1 * f2 + v2 * g2 + v3 * g3 + smu0 * (b1 * (-f6) + b3 * g7) - qy;
+    sv[11] = v1 * f3 + v2 * g3 + v3 * h3 + smu0 * (b1 * (-f7) + bb2 * (-g7)) - qz;
+  }
+};
+
+// Components 1..7 of f*a0 + g*a1 + h*a2 for the packed diffusive values s (MhdLift); the
+// mass and psi components vanish.
+template <typename R>
+__device__ __forceinline__ void mhd_visc_dir(const R *s, const R *a, R *o)
+{
+  o[1] = s[0] * a[0] + s[1] * a[1] + s[2] * a[2];
+  o[2] = s[1] * a[0] + s[3] * a[1] + s[4] * a[2];
+  o[3] = s[2] * a[0] + s[4] * a[1] + s[5] * a[2];
+  o[4] = s[9] * a[0] + s[10] * a[1] + s[11] * a[2];
+  o[5] = -(s[6] * a[1]) - s[7] * a[2];
+  o[6] = s[6] * a[0] - s[8] * a[2];
+  o[7] = s[7] * a[0] + s[8] * a[1];
+}
+
 // Powell non-conservative surface term of the left state (AddNonConsFlux mhd/riemann.f90:230-267)
 __device__ __forceinline__ void mhd_noncons_surface(const double *UL, const double *UR, const double *n, double *F)
 {
@@ -120,12 +211,13 @@ __device__ __forceinline__ void mhd_noncons_surface(const double *UL, const doub
 }
 
 // FillFlux for GLM-MHD (fillflux.f90:41-110 with NONCONS): Rusanov (RiemannSolverByRusanov
-// mhd/riemann.f90:273-308) plus each side's Powell term, times SurfElem.
-template <int N>
+// mhd/riemann.f90:273-308), in the resistive case plus the BR1 average of the diffusive
+// fluxes (Riemann mhd/riemann.f90:83-142), then each side's Powell term, times SurfElem.
+template <int N, bool PARA>
 __global__ void __launch_bounds__(256)
-k_mhd_surface(const double *__restrict__ U, const int *__restrict__ s2e, const double4 *__restrict__ nvse,
-              double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides, MhdArgs a,
-              const DtState *__restrict__ st)
+k_mhd_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int *__restrict__ s2e,
+              const double4 *__restrict__ nvse, double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides,
+              MhdArgs a, const DtState *__restrict__ st)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,7 +239,18 @@ k_mhd_surface(const double *__restrict__ U, const int *__restrict__ s2e, const d
   const double lam = fmax(fabs(vnL) + cfL, fabs(vnR) + cfR);
   double Fm[MV], Fs[MV];
 #pragma unroll
-  for (int v = 0; v < MV; v++) { Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v])); Fs[v] = Fm[v]; }
+  for (int v = 0; v < MV; v++) Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
+  if (PARA) {
+    const double *sL = Fv + ((size_t)em * n3 + vm) * 12, *sR = Fv + ((size_t)es * n3 + vs) * 12;
+    double sv[12], d[8];
+#pragma unroll
+    for (int c = 0; c < 12; c++) sv[c] = sL[c] + sR[c];
+    mhd_visc_dir<double>(sv, nv, d);
+#pragma unroll
+    for (int v = 1; v < 8; v++) Fm[v] = Fm[v] + 0.5 * d[v];
+  }
+#pragma unroll
+  for (int v = 0; v < MV; v++) Fs[v] = Fm[v];
   mhd_noncons_surface(UL, UR, nv, Fm);
   mhd_noncons_surface(UR, UL, nv, Fs);
   double *om = FluxM + (size_t)gid * MV, *os = FluxS + (size_t)gid * MV;
@@ -162,12 +265,12 @@ template <> struct MhdGeom<3> { static constexpr int EPB = 4, W = 64; };
 template <> struct MhdGeom<4> { static constexpr int EPB = 2, W = 64; };
 template <> struct MhdGeom<5> { static constexpr int EPB = 1, W = 64; };
 
-template <int N, int MODE, typename VR>
+template <int N, int MODE, typename VR, bool PARA>
 __global__ void __launch_bounds__(3 * MhdGeom<N>::W)
 k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
              const double *__restrict__ sJ, const int *__restrict__ e2s, const double *__restrict__ FluxM,
-             const double *__restrict__ FluxS, double *__restrict__ Ut, double *__restrict__ Uttmp, RKArgs rk,
-             int nElems, MhdArgs a, const DtState *__restrict__ st)
+             const double *__restrict__ FluxS, const double *__restrict__ Fv, double *__restrict__ Ut,
+             double *__restrict__ Uttmp, RKArgs rk, int nElems, MhdArgs a, const DtState *__restrict__ st)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W, NP = Lds<N>::NP;
@@ -250,6 +353,24 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
       acc[i][6] += t * s_f[el][MF_V2 * NP + lds[i]];
       acc[i][7] += t * s_f[el][MF_V3 * NP + lds[i]];
     }
+    if (PARA && MODE != VMODE_VOLINT) {
+      // VolInt_visc (volint.f90:228-271): D_hat contraction of the transformed diffusive
+      // flux f*M1 + g*M2 + h*M3 along the line
+#pragma unroll
+      for (int l = 0; l < n; l++) {
+        const double *fv = Fv + ((size_t)e * n3 + nodes[l]) * 12;
+        VR sv[12], ft[8];
+#pragma unroll
+        for (int c = 0; c < 12; c++) sv[c] = VR(fv[c]);
+        mhd_visc_dir<VR>(sv, m[l], ft);
+#pragma unroll
+        for (int i = 0; i < n; i++) {
+          const VR dh = OpsOf<VR>::dhat(op, i, l);
+#pragma unroll
+          for (int v = 1; v < 8; v++) acc[i][v] += dh * ft[v];
+        }
+      }
+    }
   }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -315,19 +436,27 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
   }
 }
 
-// CALCTIMESTEP for ideal GLM-MHD (mhd/calctimestep.f90:39-179) and InitTimeStep_GLM
-// (:186-215): per element the convective dt (FastestWave3D) and the dt of a unit wave speed.
+// CALCTIMESTEP for GLM-MHD (mhd/calctimestep.f90:39-179) and InitTimeStep_GLM (:186-215):
+// per element the convective dt (FastestWave3D), in the resistive case limited by
+// DFL*4/max_node(max(eta/mu_0, mu max(4/3, kappa/Pr)/rho) sum_d |sJ m_d|^2), and the dt of a
+// unit wave speed.
+struct MhdDtArgs {
+  double cfl, dfl, muk, diffc;  // CFLScale, DFLScale, mu*max(4/3, kappa/Pr), eta/mu_0
+  int parabolic;
+};
+
 template <int N>
 __global__ void __launch_bounds__(DtGeom<N>::T)
 k_mhd_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
               const double *__restrict__ Mh, const double *__restrict__ sJ, double *__restrict__ dtElem,
-              double *__restrict__ dtch1Elem, int *__restrict__ errElem, int nElems, MhdArgs a, double cfl)
+              double *__restrict__ dtch1Elem, int *__restrict__ errElem, int nElems, MhdArgs a, MhdDtArgs da)
 {
   constexpr int n3 = DtGeom<N>::n3, G = DtGeom<N>::G, EPB = DtGeom<N>::EPB, T = DtGeom<N>::T;
-  __shared__ double s_l[6][T];
+  __shared__ double s_l[7][T];
   __shared__ int s_err[T];
   const int tid = threadIdx.x, el = tid / G, node = tid - el * G, e = blockIdx.x * EPB + el;
-  double l[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double cfl = da.cfl;
+  double l[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int err = 0;
   if (node < n3 && e < nElems) {
     const size_t c = (size_t)e * n3 + node;
@@ -350,23 +479,30 @@ k_mhd_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const
     l[1] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cf * ng);
     l[2] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cf * nh);
     l[3] = sj * nf; l[4] = sj * ng; l[5] = sj * nh;
+    if (da.parabolic) {
+      const double lf = (mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj);
+      const double lg = (mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj);
+      const double lh = (mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj);
+      l[6] = fmax(da.diffc, sr * da.muk) * (lf + lg + lh);
+    }
   }
 #pragma unroll
-  for (int d = 0; d < 6; d++) s_l[d][tid] = l[d];
+  for (int d = 0; d < 7; d++) s_l[d][tid] = l[d];
   s_err[tid] = err;
   __syncthreads();
   for (int stp = G / 2; stp > 0; stp >>= 1) {
     if (node < stp) {
 #pragma unroll
-      for (int d = 0; d < 6; d++) s_l[d][tid] = fmax(s_l[d][tid], s_l[d][tid + stp]);
+      for (int d = 0; d < 7; d++) s_l[d][tid] = fmax(s_l[d][tid], s_l[d][tid + stp]);
       s_err[tid] = max(s_err[tid], s_err[tid + stp]);
     }
     __syncthreads();
   }
   if (node == 0 && e < nElems) {
     int er = s_err[tid];
-    const double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
+    double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
     if (isnan(dte)) er = max(er, 3);
+    if (da.parabolic && s_l[6][tid] > 0.0) dte = fmin(dte, da.dfl * 4.0 / s_l[6][tid]);
     dtElem[e] = dte;
     dtch1Elem[e] = cfl * 2.0 / (s_l[3][tid] + s_l[4][tid] + s_l[5][tid]);
     errElem[e] = er;

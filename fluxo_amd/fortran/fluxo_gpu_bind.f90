@@ -16,7 +16,7 @@ module fluxo_gpu_bind
     real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, reserved2 = 0.0d0
     integer(c_int) :: nfvse = 0, computeAlpha = 1, spacePropSweeps = 1, reserved3 = 0
     real(c_double) :: alpha_max = 0.5d0, alpha_min = 0.01d0, spacePropFactor = 0.5d0, shockBlendCoef = 0.0d0
-    real(c_double) :: mu_0 = 1.0d0, GLM_scale = 0.5d0
+    real(c_double) :: mu_0 = 1.0d0, GLM_scale = 0.5d0, eta = 0.0d0, reserved4 = 0.0d0
   end type fx_config
 
   type, bind(C) :: fx_mesh

@@ -35,7 +35,7 @@ class fx_config(C.Structure):
                 ("nfvse", C.c_int), ("computeAlpha", C.c_int), ("spacePropSweeps", C.c_int),
                 ("reserved3", C.c_int), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
                 ("spacePropFactor", C.c_double), ("shockBlendCoef", C.c_double),
-                ("mu_0", C.c_double), ("GLM_scale", C.c_double)]
+                ("mu_0", C.c_double), ("GLM_scale", C.c_double), ("eta", C.c_double), ("reserved4", C.c_double)]
 
 
 class fx_mesh(C.Structure):
@@ -366,15 +366,19 @@ class Server:
 
 class MhdEngine(Engine):
     """A GLM-MHD context (9 variables; Derigs volume flux, Rusanov surface flux, Powell
-    terms): U is (nElems, n, n, n, 9)."""
+    terms): U is (nElems, n, n, n, 9).  resistive=dict(mu=, eta=, Pr=) selects the resistive
+    equations (BR1 lifting of the entropy variables)."""
 
-    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64):
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64, resistive=None):
         L = lib()
         self.N = int(mesh["N"])
         self.nElems = int(mesh["nElems"])
         self.nSides = int(mesh["nSides"])
-        self.cfg = make_config(self.N, self.nElems, self.nSides, 10, 1, False, kappa, 0.0, 0.72, precision)
+        res = resistive or {}
+        self.cfg = make_config(self.N, self.nElems, self.nSides, 10, 1, resistive is not None, kappa,
+                               float(res.get("mu", 0.0)), float(res.get("Pr", 0.72)), precision)
         self.cfg.nVar, self.cfg.mu_0, self.cfg.GLM_scale = 9, mu_0, glm_scale
+        self.cfg.eta = float(res.get("eta", 0.0))
         self._ctx = C.c_void_p()
         _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
         m, _keep = _mesh_struct(mesh)

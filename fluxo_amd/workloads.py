@@ -101,3 +101,35 @@ def sedov_blast(xgp, kappa=1.4, sigma=SEDOV_SIGMA, E0=SEDOV_E0, p0=1.0e-5, cente
     U[..., 0] = 1.0
     U[..., 4] = p / (kappa - 1.0)
     return U
+
+
+def orszag_tang(xgp, kappa=5.0 / 3.0, mu_0=1.0):
+    """3D Orszag-Tang-type vortex of the reference's ExactFunc 312 (mhd/equation.f90:1257-1266),
+    converted with PrimToCons (mhd/equation_vars.f90:258-278): rho = 1, p = 1/kappa,
+    v = (-(1 + 0.2 sin 2 pi z) sin 2 pi y, (1 + 0.2 sin 2 pi z) sin 2 pi x, 0.2 sin 2 pi z),
+    B = (-sin 2 pi y, sin 4 pi x, 0) / kappa, psi = 0.  Returns (..., 9) conservative."""
+    x, y, z = xgp[..., 0], xgp[..., 1], xgp[..., 2]
+    tp = 2.0 * np.pi
+    a = 1.0 + 0.2 * np.sin(tp * z)
+    v = np.stack([-a * np.sin(tp * y), a * np.sin(tp * x), 0.2 * np.sin(tp * z)], -1)
+    B = np.stack([-np.sin(tp * y) / kappa, np.sin(2.0 * tp * x) / kappa, np.zeros_like(x)], -1)
+    U = np.zeros(xgp.shape[:-1] + (9,))
+    U[..., 0] = 1.0
+    U[..., 1:4] = v
+    U[..., 5:8] = B
+    U[..., 4] = (1.0 / (kappa - 1.0)) * (1.0 / kappa) + 0.5 * (v * v).sum(-1) + 0.5 * (1.0 / mu_0) * (B * B).sum(-1)
+    return U
+
+
+def perturbed_orszag_tang(xgp, seed, amp=1.0e-3, kappa=5.0 / 3.0, mu_0=1.0):
+    """orszag_tang with rho and B multiplied by 1 + amp*uniform(-1,1) (numpy default_rng(seed)),
+    the total energy recomputed for the unchanged pressure and velocity."""
+    U = orszag_tang(xgp, kappa, mu_0)
+    rng = np.random.default_rng(seed)
+    v = U[..., 1:4] / U[..., :1]
+    p = (kappa - 1.0) * (U[..., 4] - 0.5 * U[..., 0] * (v * v).sum(-1) - 0.5 / mu_0 * (U[..., 5:8] ** 2).sum(-1))
+    U[..., 0] *= 1.0 + amp * rng.uniform(-1, 1, U[..., 0].shape)
+    U[..., 5:8] *= 1.0 + amp * rng.uniform(-1, 1, U[..., 5:8].shape)
+    U[..., 1:4] = U[..., :1] * v
+    U[..., 4] = p / (kappa - 1.0) + 0.5 * U[..., 0] * (v * v).sum(-1) + 0.5 / mu_0 * (U[..., 5:8] ** 2).sum(-1)
+    return U

@@ -42,12 +42,14 @@ typedef struct {
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
   int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32; MHD 10 */
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:103-133): 0, 1, 9, 32 */
-  int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler, 1 Navier-Stokes */
+  int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
+                       resistive MHD */
   int lifting;      /* PP_Lifting: 1 = BR1 with entropy-variable lifting */
   int precision;    /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
   int reserved;
   double kappa;     /* 'kappa' (equation.f90:218) */
-  double mu0;       /* 'mu0' constant viscosity (equation.f90:229-230) */
+  double mu0;       /* constant dynamic viscosity: 'mu0' (navierstokes/equation.f90:229-230),
+                       'mu' for MHD (mhd/equation.f90:190) */
   double Pr;        /* 'Pr' (equation.f90:226) */
   double reserved2;
   /* NFVSE shock capturing (SHOCK_NFVSE, CMakeLists.txt:316-331; parameters of InitNFVSE
@@ -63,11 +65,15 @@ typedef struct {
   double alpha_min;       /* 'alpha_min' */
   double spacePropFactor; /* 'SpacePropFactor' */
   double shockBlendCoef;  /* 'ShockBlendCoef' */
-  /* ideal GLM-MHD (nVar = 9: FLUXO_EQNSYSNAME=mhd with FLUXO_EQN_GLM and Powell
+  /* GLM-MHD (nVar = 9: FLUXO_EQNSYSNAME=mhd with FLUXO_EQN_GLM and Powell
      non-conservative terms, CMakeLists.txt:174-209; mhd/equation.f90:174-208):
-     VolumeFlux 10 (Derigs), Riemann 1 (Rusanov), Euler only, no shock capturing. */
-  double mu_0;            /* 'mu_0' */
+     VolumeFlux 10 (Derigs), Riemann 1 (Rusanov), no shock capturing; parabolic = 1 adds
+     viscosity mu0, resistivity eta and heat conduction mu0*kappa/Pr with BR1 lifting of
+     the entropy variables (mhd/flux.f90:449-573). */
+  double mu_0;            /* 'mu_0' magnetic permeability */
   double GLM_scale;       /* 'GLM_scale' */
+  double eta;             /* 'eta' resistivity (mhd/equation.f90:187) */
+  double reserved4;
 } fx_config;
 
 /* Mesh data produced by the host's InitMesh/CalcMetrics (mesh.f90:100-300,

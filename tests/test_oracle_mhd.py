@@ -207,3 +207,22 @@ def test_resistive_terms_linear_and_dissipative():
     for mu, eta in ((1e-3, 0.0), (0.0, 1e-3)):
         prod, scale = entropy_production(m, U, base + diff(mu, eta))
         assert prod < -1e-3 * scale
+
+
+def test_workload_orszag_tang_matches_oracle():
+    """fluxo_amd.workloads.orszag_tang (the bench's initial state) equals the oracle's
+    ExactFunc 312; the perturbed variant keeps p and v and perturbs rho and B by <= 1e-3."""
+    from fluxo_amd.workloads import orszag_tang, perturbed_orszag_tang
+    m = O.box_mesh(3, 4, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    U = O.MhdProblem(m).exact_orszag_tang()
+    assert np.allclose(orszag_tang(m["Elem_xGP"]), U, rtol=1e-15, atol=1e-15)
+    Up = perturbed_orszag_tang(m["Elem_xGP"], 7)
+
+    def prim(W):
+        v = W[..., 1:4] / W[..., :1]
+        return v, (2.0 / 3.0) * (W[..., 4] - 0.5 * W[..., 0] * (v * v).sum(-1) - 0.5 * (W[..., 5:8] ** 2).sum(-1))
+
+    (v0, p0), (v1, p1) = prim(U), prim(Up)
+    assert np.allclose(v0, v1, atol=1e-14) and np.allclose(p0, p1, atol=1e-13)
+    r = Up[..., 0] / U[..., 0] - 1.0
+    assert 5e-4 < np.abs(r).max() <= 1e-3
