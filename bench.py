@@ -73,6 +73,69 @@ def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
     return per_elem * nElems
 
 
+def mhd_volume_bytes(N, nElems, parabolic, metric_bytes=8):
+    """Algorithmic HBM bytes of one launch of the GLM-MHD volume kernel in its RK-stage form:
+    U (9 variables) read + write, Ut_tmp read + write, metrics, sJ, the master- and
+    slave-side face fluxes (2 x 9 values per face point, 3 sides per element) and, in the
+    resistive case, the 12 packed diffusive flux values per node."""
+    n = N + 1
+    n3, n2 = n ** 3, n ** 2
+    per_elem = (8 * (9 * n3 * 4 + n3 + 3 * n2 * 9 * 2 + (12 * n3 if parabolic else 0))
+                + metric_bytes * 9 * n3)
+    return per_elem * nElems
+
+
+def mhd_leg(K=24, N=4, steps=100, warmup=5, cfl=0.9, dfl=0.9, eta=1.0e-3, mu=0.0, seed=2026):
+    """Resistive GLM-MHD (9 variables): the 3D Orszag-Tang vortex (ExactFunc 312) with a seeded
+    1e-3 uniform perturbation of rho and B (fluxo_amd.workloads.perturbed_orszag_tang) on a
+    periodic K^3 mesh of warped hexes (Ngeo=3) of [0,1]^3, N=4, Derigs EC volume flux +
+    Rusanov with Powell non-conservative terms, GLM_scale 0.5, resistivity eta, viscosity mu,
+    BR1 entropy-variable lifting.  FP64 and FP32-split each run `warmup` + `steps` LSERK steps
+    (CFL/DFL before each) from the same state; the `steps` are timed; the FP32-split state is
+    compared with the FP64 one at the end."""
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import (MhdEngine, FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING,
+                                  FX_KERNEL_DT, FX_PREC_FP64, FX_PREC_FP32_SPLIT)
+    from fluxo_amd.workloads import perturbed_orszag_tang
+    m = box_mesh(K, N, x0=(0.0,) * 3, length=(1.0,) * 3, Ngeo=3, warp=True)
+    U0 = perturbed_orszag_tang(m["Elem_xGP"], seed)
+    res = dict(mu=mu, eta=eta, Pr=0.72)
+    nodes = m["nElems"] * (N + 1) ** 3
+    out = {}
+    for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
+        e = MhdEngine(m, precision=prec, resistive=res)
+        e.upload(U0)
+        e.run(warmup, cfl, dfl)
+        e.sync()
+        t0 = time.perf_counter()
+        t_end, dt_last, err = e.run(steps, cfl, dfl)
+        e.sync()
+        el = time.perf_counter() - t0
+        U = e.download()
+        reps = 10
+        ms = {"volume": e.profile_kernel(FX_KERNEL_VOLUME_RK, reps),
+              "surface": e.profile_kernel(FX_KERNEL_SURFACE, reps),
+              "lifting": e.profile_kernel(FX_KERNEL_LIFTING, reps),
+              "calc_dt": e.profile_kernel(FX_KERNEL_DT, reps)}
+        ch, _ = e.glm()
+        e.close()
+        vb = mhd_volume_bytes(N, m["nElems"], True, 8 if prec == FX_PREC_FP64 else 4)
+        out[name] = {"value": nodes * 9 * steps / el, "unit": "DOF-updates/s", "ms_per_step": el / steps * 1e3,
+                     "errType": err, "t_end": t_end, "dt_last": dt_last, "GLM_ch": ch,
+                     "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+                     "volume_kernel_GBps": round(vb / (ms["volume"] * 1e-3) / 1e9, 1), "U": U}
+    U64, U32 = out["fp64"].pop("U"), out["fp32_split"].pop("U")
+    d = U32 - U64
+    out["fp32_split"]["speedup_vs_fp64"] = out["fp32_split"]["value"] / out["fp64"]["value"]
+    out["fp32_split"]["relL2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(U64))
+    out["fp32_split"]["relLinf_vs_fp64"] = float(np.max(np.abs(d)) / np.max(np.abs(U64)))
+    out["workload"] = (f"resistive GLM-MHD, perturbed 3D Orszag-Tang, {K}^3 warped periodic hexes of [0,1]^3, "
+                       f"N={N}, eta={eta}, mu={mu}, Derigs EC + Rusanov + Powell, BR1, {warmup}+{steps} steps "
+                       f"CFL/DFL {cfl}/{dfl}")
+    out["nElems"], out["nodes"] = int(m["nElems"]), int(nodes)
+    return out
+
+
 # CFLscale/DFLscale of the serving stream: at 0.9 the N=5 density wave (Ranocha + LLF,
 # Carpenter RK4-5) loses positivity after ~33 steps, in the CPU oracle as on the GPU.
 SERVE_CFL = 0.5
@@ -333,6 +396,8 @@ def main():
                     help="Poisson arrival rate as a fraction of the offline request throughput")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-sedov", action="store_true")
+    ap.add_argument("--no-mhd", action="store_true")
+    ap.add_argument("--mhd-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU element partitioning is a design "
@@ -400,6 +465,7 @@ def main():
             "bytes_per_launch": vbytes}
     t_end, dt_last = r64["t_end"], r64["dt_last"]
     sedov = None if args.no_sedov else sedov_leg()
+    mhd = None if args.no_mhd else mhd_leg(steps=args.mhd_steps, warmup=min(5, args.mhd_steps))
     serving = None if args.no_serving else serving_leg(args.serve_requests, args.serve_steps, 2026,
                                                           args.serve_load)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
@@ -419,6 +485,7 @@ def main():
         "fp32_split": fp32,
         "serving": serving,
         "shock_capturing": sedov,
+        "mhd_resistive": mhd,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
