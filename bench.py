@@ -64,11 +64,11 @@ def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
     """Algorithmic HBM bytes of one launch of the volume kernel in its RK-stage form:
     U read + write, Ut_tmp read + write, metrics (9 per node, FP64 or the FP32 copies of
     the split-precision path), sJ (1), the face fluxes of the element's 3 owned sides
-    (each side's flux is read once per side), and for Navier-Stokes the 12 Cartesian
-    viscous flux components per node."""
+    (each side's flux is read once per side), and for Navier-Stokes the 4 components of
+    the weak-form viscous volume term per node (formed by the lifting kernel)."""
     n = N + 1
     n3, n2 = n ** 3, n ** 2
-    per_elem = (8 * (5 * n3 * 4 + n3 + 3 * n2 * 5 + (12 * n3 if parabolic else 0))
+    per_elem = (8 * (5 * n3 * 4 + n3 + 3 * n2 * 5 + (4 * n3 if parabolic else 0))
                 + metric_bytes * 9 * n3)
     return per_elem * nElems
 
@@ -77,22 +77,25 @@ def mhd_volume_bytes(N, nElems, parabolic, metric_bytes=8):
     """Algorithmic HBM bytes of one launch of the GLM-MHD volume kernel in its RK-stage form:
     U (9 variables) read + write, Ut_tmp read + write, metrics, sJ, the master- and
     slave-side face fluxes (2 x 9 values per face point, 3 sides per element) and, in the
-    resistive case, the 12 packed diffusive flux values per node."""
+    resistive case, the 7 components of the weak-form viscous volume term per node."""
     n = N + 1
     n3, n2 = n ** 3, n ** 2
-    per_elem = (8 * (9 * n3 * 4 + n3 + 3 * n2 * 9 * 2 + (12 * n3 if parabolic else 0))
+    per_elem = (8 * (9 * n3 * 4 + n3 + 3 * n2 * 9 * 2 + (7 * n3 if parabolic else 0))
                 + metric_bytes * 9 * n3)
     return per_elem * nElems
 
 
-def mhd_leg(K=24, N=4, steps=100, warmup=5, cfl=0.9, dfl=0.9, eta=1.0e-3, mu=0.0, seed=2026):
+def mhd_leg(K=24, N=4, steps=100, warmup=5, cfl=0.5, dfl=0.5, eta=1.0e-3, mu=0.0, seed=2026):
     """Resistive GLM-MHD (9 variables): the 3D Orszag-Tang vortex (ExactFunc 312) with a seeded
     1e-3 uniform perturbation of rho and B (fluxo_amd.workloads.perturbed_orszag_tang) on a
     periodic K^3 mesh of warped hexes (Ngeo=3) of [0,1]^3, N=4, Derigs EC volume flux +
     Rusanov with Powell non-conservative terms, GLM_scale 0.5, resistivity eta, viscosity mu,
     BR1 entropy-variable lifting.  FP64 and FP32-split each run `warmup` + `steps` LSERK steps
     (CFL/DFL before each) from the same state; the `steps` are timed; the FP32-split state is
-    compared with the FP64 one at the end."""
+    compared with the FP64 one at the end.  CFLscale 0.5: at 0.9 the grid-scale perturbation
+    grows until the density is NaN (the reference's own code does the same on a 12^3 mesh
+    after ~35 steps, as does the oracle; the 24^3 mesh fails within 10 steps at 0.9 and runs
+    past 200 steps at 0.5)."""
     from fluxo_amd.mesh import box_mesh
     from fluxo_amd.native import (MhdEngine, FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING,
                                   FX_KERNEL_DT, FX_PREC_FP64, FX_PREC_FP32_SPLIT)

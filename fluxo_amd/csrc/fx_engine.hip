@@ -86,7 +86,10 @@ long long fx_ctx_bytes(const fx_config *cfg)
   b += 5LL * n * n * S * 8;          // Flux
   b += 4LL * n * n * S * 8;          // nvse
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
-  if (cfg->parabolic) b += 12LL * n3 * E * 8 + 3LL * cfg->nVar * n3 * E * 8;  // Fv, gradients
+  if (cfg->parabolic) {  // Uv, FvN, gradients
+    const long long nvv = cfg->nVar == 9 ? 7 : 4;
+    b += nvv * n3 * E * 8 + 2 * nvv * n * n * S * 8 + 3LL * cfg->nVar * n3 * E * 8;
+  }
   if (cfg->nfvse) b += 5LL * n3 * E * 8 + 2LL * E * 8;  // Ffv, alpha buffers
   if (cfg->nVar == 9) b += 4LL * 3 * n3 * E * 8 + (9LL + 4) * n * n * S * 8 * 2 + E * 8;  // MHD extra
   b += 2LL * ((n3 * E + 255) / 256 + 1) * 8 + 64;
@@ -140,7 +143,9 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   ALLOC(b.e2s, (size_t)12 * cfg->nElems * 4);
   ALLOC(b.s2e, (size_t)5 * cfg->nSides * 4);
   if (cfg->parabolic) {
-    ALLOC(b.Fv, 12 * n3E * 8);
+    const size_t nvv = cfg->nVar == 9 ? 7 : 4;  // nonzero viscous components
+    ALLOC(b.Uv, nvv * n3E * 8);
+    ALLOC(b.FvN, 2 * nvv * nf * 8);
     ALLOC(b.grad, 3 * nv * 8);
   }
   if (cfg->nfvse) {
@@ -169,7 +174,7 @@ void fx_ctx_destroy(fx_ctx *c)
   Batch &b = c->b;
   if (b.stream) (void)hipStreamSynchronize(b.stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Fv, b.grad,
+  void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
                   b.Ffv, b.FluxS, b.dtch1Elem};
   for (void *p : ptrs)

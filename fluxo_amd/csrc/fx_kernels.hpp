@@ -5,10 +5,11 @@
 //   Mf, Mg, Mh    : 3 x (N+1)^3 x nElems       sJ : (N+1)^3 x nElems
 //   nvse          : per side face point (nx, ny, nz, SurfElem)
 //   Flux          : nVar x (N+1)^2 x nSides (Flux_master * SurfElem; slave uses -Flux)
-//   Fv            : 12 x (N+1)^3 x nElems, Cartesian viscous fluxes (f,g,h; momentum+energy)
+//   Uv            : NVV x (N+1)^3 x nElems, weak-form viscous volume term (components 1..NVV)
+//   FvN           : NVV x 2 x (N+1)^2 x nSides, each side's face-normal viscous flux
 //
 // Kernel roles per RHS (one LSERK stage):
-//   k_lifting  element-centric BR1 gradients -> viscous fluxes Fv (Navier-Stokes,
+//   k_lifting  element-centric BR1 gradients -> viscous fluxes -> Uv, FvN (Navier-Stokes,
 //              resistive GLM-MHD)
 //   k_surface  side-centric numerical flux at every face point  -> Flux
 //   k_volume   element-centric flux differencing + viscous volume term + surface
@@ -161,15 +162,16 @@ struct RKArgs {
 // the LSERK stage update timedisc.f90:454-470).
 //   phase 1  one thread per node stages the flux fields of EPB elements in LDS
 //   phase 2  one thread per (direction, element, line) evaluates the n(n-1)/2 symmetric
-//            two-point fluxes of its line (each pair once) and the weak-form viscous
-//            term, accumulating both Dvolsurf contributions in registers
+//            two-point fluxes of its line (each pair once), accumulating both Dvolsurf
+//            contributions in registers
 //   phase 3  the three direction partial sums are added in LDS (reusing the field
-//            storage), then one thread per node adds the surface fluxes, applies -sJ and
-//            writes Ut or performs the RK stage update (coalesced by node).
+//            storage), then one thread per node adds the weak-form viscous volume term
+//            (formed by the lifting kernel), the surface fluxes, applies -sJ and writes
+//            Ut or performs the RK stage update (coalesced by node).
 // VR is the arithmetic type of phases 1-3a (the flux differencing and its accumulation):
 // double for the FP64 path, float for the split-precision path, whose metrics are FP32
-// copies.  The per-node fields are formed in FP64 and rounded once; the surface gather,
-// the Jacobian and the RK update are FP64 in both.
+// copies.  The per-node fields are formed in FP64 and rounded once; the viscous term, the
+// surface gather, the Jacobian and the RK update are FP64 in both.
 #ifndef FX_VOL_WAVES
 #define FX_VOL_WAVES 1
 #endif
@@ -183,7 +185,7 @@ template <int N, int VF, bool PARA, int MODE, typename VR, int EPB_>
 __global__ void __launch_bounds__((3 * VolGeom<N, EPB_>::W), (sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES))
 k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg,
          const VR *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
-         const double *__restrict__ Flux, const double *__restrict__ Fv, double *__restrict__ Ut,
+         const double *__restrict__ Flux, const double *__restrict__ Uv, double *__restrict__ Ut,
          double *__restrict__ Uttmp, RKArgs rk, int nElems, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -288,25 +290,6 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
           acc[l][v] += dli * F[v];
         }
       }
-    if (PARA && MODE != VMODE_VOLINT) {
-      // transformed viscous flux along the line: ft(l) = f*M1 + g*M2 + h*M3
-      VR ft[n][4];
-#pragma unroll
-      for (int l = 0; l < n; l++) {
-        const double *fv = Fv + ((size_t)e * n3 + nodes[l]) * 12;
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-          ft[l][c] = VR(fv[c]) * m[l][0] + VR(fv[4 + c]) * m[l][1] + VR(fv[8 + c]) * m[l][2];
-      }
-#pragma unroll
-      for (int i = 0; i < n; i++)
-#pragma unroll
-        for (int l = 0; l < n; l++) {
-          const VR dh = OpsOf<VR>::dhat(op, i, l);
-#pragma unroll
-          for (int c = 0; c < 4; c++) acc[i][c + 1] += dh * ft[l][c];
-        }
-    }
   }
   // phase 3a: xi + eta + zeta partial sums, in this order, into LDS
 #pragma unroll
@@ -342,6 +325,11 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 #pragma unroll
         for (int v = 0; v < 5; v++) tot[v] = (1.0 - a) * tot[v] + a * ff[v];
       }
+    }
+    if (PARA && MODE != VMODE_VOLINT) {  // VolInt_visc, formed by the lifting kernel
+      const double *uv = Uv + ((size_t)e2 * n3 + node) * 4;
+#pragma unroll
+      for (int c = 0; c < 4; c++) tot[c + 1] = tot[c + 1] + uv[c];
     }
     if (MODE != VMODE_VOLINT) {
       // SurfInt gather: faces touched by node (i,j,k), in local side order
@@ -394,7 +382,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 // riemann.f90:104-169).
 template <int N, int RS, bool PARA>
 __global__ void __launch_bounds__(256)
-k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int *__restrict__ s2e,
+k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
           const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -412,12 +400,10 @@ k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int
   const double nv[3] = {g.x, g.y, g.z};
   double F[5];
   riemann<RS>(UL, UR, nv, kappa, F);
-  if (PARA) {
-    const double *fL = Fv + ((size_t)em * n3 + vm) * 12, *fR = Fv + ((size_t)es * n3 + vs) * 12;
+  if (PARA) {  // BR1: mean of the two sides' normal viscous fluxes
+    const double *fn = FvN + (size_t)gid * 8;
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      F[c + 1] = F[c + 1] + 0.5 * (nv[0] * (fL[c] + fR[c]) + nv[1] * (fL[4 + c] + fR[4 + c]) +
-                                   nv[2] * (fL[8 + c] + fR[8 + c]));
+    for (int c = 0; c < 4; c++) F[c + 1] = F[c + 1] + 0.5 * (fn[c] + fn[4 + c]);
   }
   double *o = Flux + (size_t)gid * 5;
 #pragma unroll
@@ -428,17 +414,18 @@ k_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int
 // br1_volint.f90:45-98, Lifting_FillFlux br1_fillflux.f90:49-91 with EvalLiftingSurfFlux
 // flux.f90:677-716, Lifting_SurfInt br1_surfint.f90:52-131, V2D_M_V1D(sJ),
 // ConvertToGradPrimVec).  Element-centric: the surface term of every face is formed from
-// the neighbour's face node directly.  OUT=0 writes the primitive gradients (gradPx/y/z),
-// OUT=1 writes the 12 independent Cartesian viscous flux values per node (EvalDiffFlux3D)
-// consumed by the surface and volume kernels.  EQ supplies the equation's entropy
-// variables, gradient conversion and diffusive fluxes (NsLift; MhdLift for resistive
-// GLM-MHD).
+// the neighbour's face node directly.  OUT=0 writes the primitive gradients (gradPx/y/z);
+// OUT=1 forms the Cartesian viscous fluxes (EvalDiffFlux3D) and writes what the volume and
+// surface kernels need of them: the weak-form viscous volume term Uv and the face-normal
+// fluxes FvN.  EQ supplies the equation's entropy variables, gradient conversion and
+// diffusive fluxes (NsLift; MhdLift for resistive GLM-MHD).
 template <int N> struct LiftGeom { static constexpr int EPB = (256 / ((N + 1) * (N + 1) * (N + 1))) > 0 ? (256 / ((N + 1) * (N + 1) * (N + 1))) : 1; };
 
 // Navier-Stokes lifting physics: entropy variables (ConsToEntropy), the primitive-gradient
-// conversion and the 12 viscous flux values per node (f, g, h of momentum and energy).
+// conversion and the 12 viscous flux values per node (f, g, h of momentum and energy);
+// vis_dir gives the NVV = 4 nonzero components (momentum, energy) of f a0 + g a1 + h a2.
 struct NsLift {
-  static constexpr int NV = 5;
+  static constexpr int NV = 5, NVV = 4;
   double kappa, mu0, KappasPr;
   __device__ __forceinline__ void entropy(const double *u, double *w) const { cons_to_entropy(u, kappa, w); }
   __device__ __forceinline__ void grad_prim(const double *u, double *g) const { grad_entropy_to_prim(u, kappa, g); }
@@ -447,27 +434,50 @@ struct NsLift {
   {
     diff_flux(u, gx, gy, gz, kappa, mu0, KappasPr, fv);
   }
+  __device__ __forceinline__ static void vis_dir(const double *s, double a0, double a1, double a2, double *o)
+  {
+#pragma unroll
+    for (int c = 0; c < 4; c++) o[c] = s[c] * a0 + s[4 + c] * a1 + s[8 + c] * a2;
+  }
 };
 
+// Outputs of the lifting kernel's viscous form (OUT=1), per element and side:
+//   Uv(NVV, n^3, E)      weak-form viscous volume term sum_l D_hat(i,l) (f~, g~, h~)(l) of
+//                        VolInt_visc (volint.f90:228-271), components 1..NVV;
+//   FvN(NVV, 2, n^2, S)  n.(f, g, h) of each side's element at every face point (master
+//                        normal; [0] master, [1] slave), whose mean is the BR1 viscous
+//                        surface flux (riemann.f90:155-168).
+// The element's transformed fluxes pass through LDS: s_ft holds 3 NVV values per node.
+#ifndef FX_LIFT_WAVES
+#define FX_LIFT_WAVES 1
+#endif
 template <int N, int OUT, class EQ>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
           int nElems, EQ eq)
 {
-  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV;
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
   constexpr int EPB = LiftGeom<N>::EPB;
   __shared__ double s_w[EPB][NV][n3];
+  // viscous form: the node's 12 Cartesian flux values, then (same slots, same thread) its
+  // transformed fluxes f~ (NVV), g~ (NVV), h~ (NVV)
+  constexpr int FTW = 3 * NVV > 12 ? 3 * NVV : 12;
+  __shared__ double s_ft[OUT == 1 ? EPB : 1][FTW][OUT == 1 ? n3 : 1];
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
   const int el = tid / n3, node = tid - el * n3;
   const int e = blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (e < nElems) && elem_problem(eprob, e) >= 0;
+  const int i = node % n, j = (node / n) % n, k = node / n2;
+  const size_t c = (size_t)e * n3 + node;
   double u[NV], w[NV];
+  int nl = 0, fpt[3] = {0, 0, 0};  // faces of this node: face point (s n + q) n + p, master?
+  bool fmaster[3] = {false, false, false};
   if (active) {
-    const double *up = U + ((size_t)e * n3 + node) * NV;
+    const double *up = U + c * NV;
 #pragma unroll
     for (int v = 0; v < NV; v++) u[v] = up[v];
     eq.entropy(u, w);
@@ -475,84 +485,133 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     for (int v = 0; v < NV; v++) s_w[el][v][node] = w[v];
   }
   __syncthreads();
-  if (!active) return;
-  const int i = node % n, j = (node / n) % n, k = node / n2;
-  double gxi[NV], geta[NV], gzeta[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    gxi[v] = op.D[i][0] * s_w[el][v][0 + n * (j + n * k)];
-    geta[v] = op.D[j][0] * s_w[el][v][i + n * (0 + n * k)];
-    gzeta[v] = op.D[k][0] * s_w[el][v][i + n * (j + n * 0)];
-  }
-#pragma unroll
-  for (int l = 1; l < n; l++)
+  if (active) {
+    double gxi[NV], geta[NV], gzeta[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      gxi[v] = gxi[v] + op.D[i][l] * s_w[el][v][l + n * (j + n * k)];
-      geta[v] = geta[v] + op.D[j][l] * s_w[el][v][i + n * (l + n * k)];
-      gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
+      gxi[v] = op.D[i][0] * s_w[el][v][0 + n * (j + n * k)];
+      geta[v] = op.D[j][0] * s_w[el][v][i + n * (0 + n * k)];
+      gzeta[v] = op.D[k][0] * s_w[el][v][i + n * (j + n * 0)];
     }
-  const size_t c = (size_t)e * n3 + node;
-  const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-  double gx[NV], gy[NV], gz[NV];
+    // not unrolled: all n NV LDS operands in flight at once would cost an occupancy level
+#pragma unroll 1
+    for (int l = 1; l < n; l++)
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    gx[v] = mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v];
-    gy[v] = mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v];
-    gz[v] = mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v];
-  }
-  // surface terms: +1/2 (w_slave - w_master) SurfElem n_master L_HatMinus0 on both sides
-  int locs[3], nl = 0;
-  if (k == 0) locs[nl++] = ZETA_MINUS;
-  if (j == 0) locs[nl++] = ETA_MINUS;
-  if (i == N) locs[nl++] = XI_PLUS;
-  if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
-  if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
-  if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
-  const double L0 = op.L0;
-  for (int t = 0; t < nl; t++) {
-    const int loc = locs[t];
-    const int s = e2s[(e * 6 + loc - 1) * 2] - 1, fl = e2s[(e * 6 + loc - 1) * 2 + 1];
-    int p, q;
-    vol_to_side<N>(loc, fl, i, j, k, p, q);
-    int enb, vnb;
-    if (fl == 0) {
-      enb = s2e[5 * s + 1] - 1;
-      vnb = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
+      for (int v = 0; v < NV; v++) {
+        gxi[v] = gxi[v] + op.D[i][l] * s_w[el][v][l + n * (j + n * k)];
+        geta[v] = geta[v] + op.D[j][l] * s_w[el][v][i + n * (l + n * k)];
+        gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
+      }
+    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+    const double mf0 = mf[0], mf1 = mf[1], mf2 = mf[2], mg0 = mg[0], mg1 = mg[1], mg2 = mg[2];
+    const double mh0 = mh[0], mh1 = mh[1], mh2 = mh[2];
+    double gx[NV], gy[NV], gz[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      gx[v] = mf0 * gxi[v] + mg0 * geta[v] + mh0 * gzeta[v];
+      gy[v] = mf1 * gxi[v] + mg1 * geta[v] + mh1 * gzeta[v];
+      gz[v] = mf2 * gxi[v] + mg2 * geta[v] + mh2 * gzeta[v];
+    }
+    // surface terms: +1/2 (w_slave - w_master) SurfElem n_master L_HatMinus0 on both sides
+    int locs[3];
+    if (k == 0) locs[nl++] = ZETA_MINUS;
+    if (j == 0) locs[nl++] = ETA_MINUS;
+    if (i == N) locs[nl++] = XI_PLUS;
+    if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
+    if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
+    if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+    const double L0 = op.L0;
+    for (int t = 0; t < nl; t++) {
+      const int loc = locs[t];
+      const int s = e2s[(e * 6 + loc - 1) * 2] - 1, fl = e2s[(e * 6 + loc - 1) * 2 + 1];
+      int p, q;
+      vol_to_side<N>(loc, fl, i, j, k, p, q);
+      int enb, vnb;
+      if (fl == 0) {
+        enb = s2e[5 * s + 1] - 1;
+        vnb = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
+      } else {
+        enb = s2e[5 * s] - 1;
+        vnb = side_to_vol<N>(s2e[5 * s + 2], 0, p, q);
+      }
+      const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
+      double unb[NV], wnb[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) unb[v] = unbp[v];
+      eq.entropy(unb, wnb);
+      fpt[t] = (s * n + q) * n + p;
+      fmaster[t] = (fl == 0);
+      const double4 g = nvse[fpt[t]];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double ws = (fl == 0) ? wnb[v] : w[v];
+        const double wm = (fl == 0) ? w[v] : wnb[v];
+        const double f = 0.5 * (ws - wm) * g.w;
+        gx[v] = gx[v] + (f * g.x) * L0;
+        gy[v] = gy[v] + (f * g.y) * L0;
+        gz[v] = gz[v] + (f * g.z) * L0;
+      }
+    }
+    const double sj = sJ[c];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+    eq.grad_prim(u, gx);
+    eq.grad_prim(u, gy);
+    eq.grad_prim(u, gz);
+    if (OUT == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
     } else {
-      enb = s2e[5 * s] - 1;
-      vnb = side_to_vol<N>(s2e[5 * s + 2], 0, p, q);
-    }
-    const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
-    double unb[NV], wnb[NV];
+      // the 12 Cartesian viscous flux values of this node into its own LDS slots (the
+      // gradients die here, which keeps the register count of the first part)
+      double fv[12];
+      eq.fluxes(u, gx, gy, gz, fv);
 #pragma unroll
-    for (int v = 0; v < NV; v++) unb[v] = unbp[v];
-    eq.entropy(unb, wnb);
-    const double4 g = nvse[(size_t)(s * n + q) * n + p];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const double ws = (fl == 0) ? wnb[v] : w[v];
-      const double wm = (fl == 0) ? w[v] : wnb[v];
-      const double f = 0.5 * (ws - wm) * g.w;
-      gx[v] = gx[v] + (f * g.x) * L0;
-      gy[v] = gy[v] + (f * g.y) * L0;
-      gz[v] = gz[v] + (f * g.z) * L0;
+      for (int m = 0; m < 12; m++) s_ft[el][m][node] = fv[m];
     }
   }
-  const double sj = sJ[c];
+  if (OUT == 1) __syncthreads();
+  if (OUT == 1 && active) {
+    double fv[12], o[NVV];
 #pragma unroll
-  for (int v = 0; v < NV; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
-  eq.grad_prim(u, gx);
-  eq.grad_prim(u, gy);
-  eq.grad_prim(u, gz);
-  if (OUT == 0) {
+    for (int m = 0; m < 12; m++) fv[m] = s_ft[el][m][node];
+    // face-normal viscous flux of this element at each of its face points
+    for (int t = 0; t < nl; t++) {
+      const double4 g = nvse[fpt[t]];
+      EQ::vis_dir(fv, g.x, g.y, g.z, o);
+      double *dst = out1 + ((size_t)fpt[t] * 2 + (fmaster[t] ? 0 : 1)) * NVV;
 #pragma unroll
-    for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
-  } else {
-    double fv[12];
-    eq.fluxes(u, gx, gy, gz, fv);
+      for (int v = 0; v < NVV; v++) dst[v] = o[v];
+    }
+    // transformed fluxes f~, g~, h~ (EvalDiffFluxTilde3D) over the node's own slots
+    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+    EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
 #pragma unroll
-    for (int m = 0; m < 12; m++) out0[12 * c + m] = fv[m];
+    for (int v = 0; v < NVV; v++) s_ft[el][v][node] = o[v];
+    EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
+#pragma unroll
+    for (int v = 0; v < NVV; v++) s_ft[el][NVV + v][node] = o[v];
+    EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
+#pragma unroll
+    for (int v = 0; v < NVV; v++) s_ft[el][2 * NVV + v][node] = o[v];
+  }
+  if (OUT == 1) {
+    __syncthreads();
+    if (active) {
+      double uv[NVV];
+#pragma unroll
+      for (int v = 0; v < NVV; v++) uv[v] = 0.0;
+#pragma unroll 1
+      for (int l = 0; l < n; l++) {
+        const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
+#pragma unroll
+        for (int v = 0; v < NVV; v++)
+          uv[v] = uv[v] + di * s_ft[el][v][l + n * (j + n * k)] + dj * s_ft[el][NVV + v][i + n * (l + n * k)] +
+                  dk * s_ft[el][2 * NVV + v][i + n * (j + n * l)];
+      }
+#pragma unroll
+      for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
+    }
   }
 }
 

@@ -276,7 +276,8 @@ struct Batch {
   double *U = nullptr, *Ut = nullptr, *Uttmp = nullptr;
   double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr;
   float *Mf32 = nullptr, *Mg32 = nullptr, *Mh32 = nullptr;  // split precision only
-  double *Flux = nullptr, *Fv = nullptr, *grad = nullptr;
+  // viscous terms formed by the lifting kernel: Uv (NVV per node), FvN (2 NVV per face point)
+  double *Flux = nullptr, *Uv = nullptr, *FvN = nullptr, *grad = nullptr;
   double4 *nvse = nullptr;
   int *e2s = nullptr, *s2e = nullptr;
   int *eprob = nullptr;
@@ -319,11 +320,11 @@ void launch_volume_e(const Batch &b, RKArgs rk)
   if (nb <= 0) return;
   if (b.cfg.precision == FX_PREC_FP32_SPLIT)
     hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
-                       b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                        b.cfg.kappa);
   else
     hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
-                       b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                        b.cfg.kappa);
 }
 
@@ -371,10 +372,10 @@ void launch_surface_t(const Batch &b)
   const int nb = (int)((tot + 255) / 256);
   if (nb <= 0) return;
   if (b.cfg.parabolic)
-    hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e,
+    hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e,
                        b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
   else
-    hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e,
+    hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e,
                        b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
 }
 
@@ -401,7 +402,7 @@ void launch_lifting_e(const Batch &b, int out, const EQ &eq)
                        b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq);
   else
     hipLaunchKernelGGL((k_lifting<N, 1, EQ>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.Fv, nullptr, nullptr, b.eprob, b.cfg.nElems, eq);
+                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq);
 }
 
 template <int N>
@@ -445,11 +446,11 @@ void launch_mhd_volume_p(const Batch &b, RKArgs rk)
   if (nb <= 0) return;
   if (b.cfg.precision == FX_PREC_FP32_SPLIT)
     hipLaunchKernelGGL((k_mhd_volume<N, MODE, float, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf32,
-                       b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                       b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                        b.mhd_args(), b.st);
   else
     hipLaunchKernelGGL((k_mhd_volume<N, MODE, double, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf, b.Mg,
-                       b.Mh, b.sJ, b.e2s, b.Flux, b.FluxS, b.Fv, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(),
+                       b.Mh, b.sJ, b.e2s, b.Flux, b.FluxS, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(),
                        b.st);
 }
 
@@ -475,10 +476,10 @@ void launch_mhd_surface_n(const Batch &b)
   const int nb = (int)((tot + 255) / 256);
   if (nb <= 0) return;
   if (b.cfg.parabolic)
-    hipLaunchKernelGGL((k_mhd_surface<N, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e, b.nvse, b.Flux,
+    hipLaunchKernelGGL((k_mhd_surface<N, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse, b.Flux,
                        b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
   else
-    hipLaunchKernelGGL((k_mhd_surface<N, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Fv, b.s2e, b.nvse,
+    hipLaunchKernelGGL((k_mhd_surface<N, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
                        b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
 }
 

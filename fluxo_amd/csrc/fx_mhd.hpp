@@ -3,9 +3,10 @@
 // surface flux, CALCTIMESTEP with the GLM cleaning speed from the time step, and the LSERK
 // stage update (src/equation/mhd/{flux_average,flux,riemann,equation_vars,calctimestep}.f90,
 // NONCONS branches of src/dg/{volint,fillflux,surfint}.f90).  Resistive case (PARABOLIC):
-// k_lifting with MhdLift (BR1 lifting of the entropy variables) writes 12 independent
-// diffusive flux values per node; the surface kernel adds their BR1 face average, the
-// volume kernel the weak-form D_hat term (VolInt_visc), the time step the viscous limit.
+// k_lifting with MhdLift (BR1 lifting of the entropy variables) forms the diffusive fluxes
+// and writes the weak-form viscous volume term (VolInt_visc) and each side's face-normal
+// diffusive flux; the surface kernel adds their BR1 mean, the volume kernel the volume
+// term, the time step the viscous limit.
 //
 // Kernels per RK stage: k_mhd_surface (side-centric: conservative flux plus the
 // non-conservative term of each side -> FluxM, FluxS) and k_mhd_volume (element-centric like
@@ -112,6 +113,20 @@ __device__ __forceinline__ void mhd_flux_n(const double *U, const double *n, con
   cf = sqrt(0.5 * (c2 + va2 + astar));
 }
 
+// Components 1..7 of f*a0 + g*a1 + h*a2 for the packed diffusive values s (MhdLift); the
+// mass and psi components vanish.
+template <typename R>
+__device__ __forceinline__ void mhd_visc_dir(const R *s, const R *a, R *o)
+{
+  o[1] = s[0] * a[0] + s[1] * a[1] + s[2] * a[2];
+  o[2] = s[1] * a[0] + s[3] * a[1] + s[4] * a[2];
+  o[3] = s[2] * a[0] + s[4] * a[1] + s[5] * a[2];
+  o[4] = s[9] * a[0] + s[10] * a[1] + s[11] * a[2];
+  o[5] = -(s[6] * a[1]) - s[7] * a[2];
+  o[6] = s[6] * a[0] - s[8] * a[2];
+  o[7] = s[7] * a[0] + s[8] * a[1];
+}
+
 // Resistive GLM-MHD lifting physics: entropy variables (ConsToEntropy
 // mhd/equation_vars.f90:330-361), the conversion of their gradients to gradients of
 // (rho, v, p, B, psi) (ConvertToGradPrimVec, :547-628) and the isotropic diffusive fluxes
@@ -121,8 +136,17 @@ __device__ __forceinline__ void mhd_flux_n(const double *U, const double *n, con
 // (f1..f3 the viscous stress row x, f6/f7/g7 the resistive terms, f4/g4/h4 the energy
 // fluxes); g1 = f2, h1 = f3, h2 = g3, g5 = -f6, h5 = -f7, h6 = -g7, the rest vanish.
 struct MhdLift {
-  static constexpr int NV = 9;
+  static constexpr int NV = 9, NVV = 7;
   double kappa, smu0, mu, etasmu0, KappasPr;
+  // components 1..7 of f a0 + g a1 + h a2
+  __device__ __forceinline__ static void vis_dir(const double *s, double a0, double a1, double a2, double *o)
+  {
+    const double a[3] = {a0, a1, a2};
+    double t[8];
+    mhd_visc_dir<double>(s, a, t);
+#pragma unroll
+    for (int v = 0; v < 7; v++) o[v] = t[v + 1];
+  }
   __device__ __forceinline__ void entropy(const double *c, double *w) const
   {
     const double km1 = kappa - 1.0, s2mu0 = 0.5 * smu0;
@@ -186,20 +210,6 @@ struct MhdLift {
   }
 };
 
-// Components 1..7 of f*a0 + g*a1 + h*a2 for the packed diffusive values s (MhdLift); the
-// mass and psi components vanish.
-template <typename R>
-__device__ __forceinline__ void mhd_visc_dir(const R *s, const R *a, R *o)
-{
-  o[1] = s[0] * a[0] + s[1] * a[1] + s[2] * a[2];
-  o[2] = s[1] * a[0] + s[3] * a[1] + s[4] * a[2];
-  o[3] = s[2] * a[0] + s[4] * a[1] + s[5] * a[2];
-  o[4] = s[9] * a[0] + s[10] * a[1] + s[11] * a[2];
-  o[5] = -(s[6] * a[1]) - s[7] * a[2];
-  o[6] = s[6] * a[0] - s[8] * a[2];
-  o[7] = s[7] * a[0] + s[8] * a[1];
-}
-
 // Powell non-conservative surface term of the left state (AddNonConsFlux mhd/riemann.f90:230-267)
 __device__ __forceinline__ void mhd_noncons_surface(const double *UL, const double *UR, const double *n, double *F)
 {
@@ -215,7 +225,7 @@ __device__ __forceinline__ void mhd_noncons_surface(const double *UL, const doub
 // fluxes (Riemann mhd/riemann.f90:83-142), then each side's Powell term, times SurfElem.
 template <int N, bool PARA>
 __global__ void __launch_bounds__(256)
-k_mhd_surface(const double *__restrict__ U, const double *__restrict__ Fv, const int *__restrict__ s2e,
+k_mhd_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
               const double4 *__restrict__ nvse, double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides,
               MhdArgs a, const DtState *__restrict__ st)
 {
@@ -240,14 +250,10 @@ k_mhd_surface(const double *__restrict__ U, const double *__restrict__ Fv, const
   double Fm[MV], Fs[MV];
 #pragma unroll
   for (int v = 0; v < MV; v++) Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
-  if (PARA) {
-    const double *sL = Fv + ((size_t)em * n3 + vm) * 12, *sR = Fv + ((size_t)es * n3 + vs) * 12;
-    double sv[12], d[8];
+  if (PARA) {  // BR1: mean of the two sides' normal diffusive fluxes
+    const double *fn = FvN + (size_t)gid * 14;
 #pragma unroll
-    for (int c = 0; c < 12; c++) sv[c] = sL[c] + sR[c];
-    mhd_visc_dir<double>(sv, nv, d);
-#pragma unroll
-    for (int v = 1; v < 8; v++) Fm[v] = Fm[v] + 0.5 * d[v];
+    for (int v = 1; v < 8; v++) Fm[v] = Fm[v] + 0.5 * (fn[v - 1] + fn[6 + v]);
   }
 #pragma unroll
   for (int v = 0; v < MV; v++) Fs[v] = Fm[v];
@@ -269,7 +275,7 @@ template <int N, int MODE, typename VR, bool PARA>
 __global__ void __launch_bounds__(3 * MhdGeom<N>::W)
 k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
              const double *__restrict__ sJ, const int *__restrict__ e2s, const double *__restrict__ FluxM,
-             const double *__restrict__ FluxS, const double *__restrict__ Fv, double *__restrict__ Ut,
+             const double *__restrict__ FluxS, const double *__restrict__ Uv, double *__restrict__ Ut,
              double *__restrict__ Uttmp, RKArgs rk, int nElems, MhdArgs a, const DtState *__restrict__ st)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -353,24 +359,6 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
       acc[i][6] += t * s_f[el][MF_V2 * NP + lds[i]];
       acc[i][7] += t * s_f[el][MF_V3 * NP + lds[i]];
     }
-    if (PARA && MODE != VMODE_VOLINT) {
-      // VolInt_visc (volint.f90:228-271): D_hat contraction of the transformed diffusive
-      // flux f*M1 + g*M2 + h*M3 along the line
-#pragma unroll
-      for (int l = 0; l < n; l++) {
-        const double *fv = Fv + ((size_t)e * n3 + nodes[l]) * 12;
-        VR sv[12], ft[8];
-#pragma unroll
-        for (int c = 0; c < 12; c++) sv[c] = VR(fv[c]);
-        mhd_visc_dir<VR>(sv, m[l], ft);
-#pragma unroll
-        for (int i = 0; i < n; i++) {
-          const VR dh = OpsOf<VR>::dhat(op, i, l);
-#pragma unroll
-          for (int v = 1; v < 8; v++) acc[i][v] += dh * ft[v];
-        }
-      }
-    }
   }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -394,6 +382,11 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
     double tot[MV];
 #pragma unroll
     for (int v = 0; v < MV; v++) tot[v] = double(s_acc[el2][v * NP + l]);
+    if (PARA && MODE != VMODE_VOLINT) {  // VolInt_visc (volint.f90:228-271), formed by the lifting kernel
+      const double *uv = Uv + ((size_t)e2 * n3 + node) * 7;
+#pragma unroll
+      for (int v = 1; v < 8; v++) tot[v] = tot[v] + uv[v - 1];
+    }
     if (MODE != VMODE_VOLINT) {
       int locs[3], nl = 0;
       if (k == 0) locs[nl++] = ZETA_MINUS;

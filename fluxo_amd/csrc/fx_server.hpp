@@ -127,8 +127,8 @@ inline long long request_bytes(const fx_server_config &c, int N, int par, long l
   const long long n = N + 1, n3 = n * n * n, n2 = n * n;
   long long per_e = 2 * 5 * n3 * 8 + 9 * n3 * 8 + n3 * 8 + 12 * 4 + 4 + 8 + 4;
   if (c.precision == FX_PREC_FP32_SPLIT) per_e += 9 * n3 * 4;
-  if (par) per_e += 12 * n3 * 8;
-  const long long per_s = 5 * n2 * 8 + n2 * (long long)sizeof(double4) + 5 * 4;
+  if (par) per_e += 4 * n3 * 8;
+  const long long per_s = 5 * n2 * 8 + n2 * (long long)sizeof(double4) + 5 * 4 + (par ? 8 * n2 * 8 : 0);
   return per_e * E + per_s * S + (long long)sizeof(DtState) + 8;
 }
 
@@ -167,13 +167,14 @@ inline int pool_realloc(Pool &p, const fx_server_config &c, int Ecap, int Scap, 
     GROW(b.Mg32, 3 * n3, oE, Ecap, -1);
     GROW(b.Mh32, 3 * n3, oE, Ecap, -1);
   }
-  if (p.parabolic) GROW(b.Fv, 12 * n3, oE, Ecap, -1);
+  if (p.parabolic) GROW(b.Uv, 4 * n3, oE, Ecap, -1);
   GROW(b.e2s, 12, oE, Ecap, 0);
   GROW(b.eprob, 1, oE, Ecap, 0xFF);  // -1: unused
   GROW(b.dtElem, 1, oE, Ecap, -1);
   GROW(b.errElem, 1, oE, Ecap, 0);
   GROW(b.Flux, 5 * n2, oS, Scap, -1);
   GROW(b.nvse, n2, oS, Scap, -1);
+  if (p.parabolic) GROW(b.FvN, 8 * n2, oS, Scap, -1);
   GROW(b.s2e, 5, oS, Scap, 0);       // zero SideToElem: unused side slot
   GROW(b.st, 1, oR, Rcap, 0);
   GROW(b.rfirst, 1, oR, Rcap, 0);
@@ -194,7 +195,7 @@ inline void pool_free(Pool &p)
   if (b.stream) (void)hipStreamSynchronize(b.stream);
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
   p.graph = nullptr;
-  void *ptrs[] = {b.U, b.Uttmp, b.Mf, b.Mg, b.Mh, b.sJ, b.Mf32, b.Mg32, b.Mh32, b.Fv, b.e2s, b.eprob,
+  void *ptrs[] = {b.U, b.Uttmp, b.Mf, b.Mg, b.Mh, b.sJ, b.Mf32, b.Mg32, b.Mh32, b.Uv, b.FvN, b.e2s, b.eprob,
                   b.dtElem, b.errElem, b.Flux, b.nvse, b.s2e, b.st, b.rfirst, b.rcount};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
