@@ -73,6 +73,25 @@ def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
     return per_elem * nElems
 
 
+def lifting_kernel_bytes(N, nElems):
+    """Algorithmic HBM bytes of one launch of the Navier-Stokes lifting kernel (viscous
+    form): U, metrics and sJ of every node; at each of the element's 6 faces the
+    neighbour's face-node state (5) and the face normal with SurfElem (4); written: the
+    4-component viscous volume term per node and the 4-component normal viscous flux at
+    each of the 6 faces."""
+    n = N + 1
+    n3, n2 = n ** 3, n ** 2
+    return 8 * (5 * n3 + 9 * n3 + n3 + 6 * n2 * 5 + 6 * n2 * 4 + 4 * n3 + 6 * n2 * 4) * nElems
+
+
+def surface_kernel_bytes(N, nSides, parabolic):
+    """Algorithmic HBM bytes of one launch of the surface kernel: per face point both
+    sides' states, normal + SurfElem, (Navier-Stokes) both sides' normal viscous fluxes,
+    the numerical flux written; SideToElem per side."""
+    n2 = (N + 1) ** 2
+    return (8 * n2 * (2 * 5 + 4 + (8 if parabolic else 0) + 5) + 20) * nSides
+
+
 def mhd_volume_bytes(N, nElems, parabolic, metric_bytes=8):
     """Algorithmic HBM bytes of one launch of the GLM-MHD volume kernel in its RK-stage form:
     U (9 variables) read + write, Ut_tmp read + write, metrics, sJ, the master- and
@@ -459,13 +478,30 @@ def main():
                                         / (r32["ms"]["volume"] * 1e-3) / 1e9, 1),
             "note": "state after warmup+steps RK steps vs the FP64 run from the same initial state"}
 
-    # dominant kernel: the volume kernel in its RK-stage form, timed with HIP events
-    vbytes = volume_kernel_bytes(N, mesh["nElems"], True)
-    achieved = vbytes / (ms["volume"] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_volume<N=3,Ranocha,BR1,RK>", "launch_ms": round(ms["volume"], 4),
-            "bytes_per_launch": vbytes}
+    # per-kernel rooflines (durations from HIP events on the engine stream); the dominant
+    # kernel is the one with the longest launch; HBM traffic from the committed PMC summary
+    # of the same kernel and workload (profiles/pmc_traffic.json), when there is one
+    traffic = {}
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+            traffic = json.load(f)
+    except (OSError, ValueError):
+        pass
+    tile = r64["tiles"]["chosen_elems_per_workgroup"]
+    kern = {"volume": ("k_volume<3, 32, true, 1, double, %d>" % tile, volume_kernel_bytes(N, mesh["nElems"], True)),
+            "lifting": ("k_lifting<3, 1, fx::NsLift>", lifting_kernel_bytes(N, mesh["nElems"])),
+            "surface": ("k_surface<3, 1, true>", surface_kernel_bytes(N, mesh["nSides"], True))}
+    roofs = {}
+    for k, (kname, nbytes) in kern.items():
+        ach = nbytes / (ms[k] * 1e-3) / 1e9
+        tr = traffic.get(kname, {}).get("hbm_bytes_per_launch")
+        roofs[k] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": kname,
+                    "launch_ms": round(ms[k], 4), "bytes_per_launch": nbytes}
+    dom = max(roofs, key=lambda k: roofs[k]["launch_ms"])
+    roof = dict(roofs[dom])
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = "profiles/pmc_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
     t_end, dt_last = r64["t_end"], r64["dt_last"]
     sedov = None if args.no_sedov else sedov_leg()
     mhd = None if args.no_mhd else mhd_leg(steps=args.mhd_steps, warmup=min(5, args.mhd_steps))
@@ -490,6 +526,7 @@ def main():
         "shock_capturing": sedov,
         "mhd_resistive": mhd,
         "roofline": roof,
+        "roofline_per_kernel": roofs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
