@@ -63,7 +63,9 @@ int check_config(const fx_config *c)
     return FX_OK;
   }
   if (c->nElems < 1 || c->nSides < 1) return set_err(FX_EINVAL, "empty mesh");
-  if (c->parabolic && c->lifting != 1) return set_err(FX_EINVAL, "only BR1 lifting (lifting=1)");
+  if (c->parabolic && c->lifting != 1 && c->lifting != 2)
+    return set_err(FX_EINVAL, "lifting=%d not supported (1 BR1, 2 BR2)", c->lifting);
+  if (c->parabolic && c->lifting == 2 && !(c->etaBR2 > 0.0)) return set_err(FX_EINVAL, "BR2 needs etaBR2 > 0");
   const int rc = check_physics(c->volumeFlux, c->riemann, c->precision, c->kappa, c->Pr, c->parabolic);
   return rc ? rc : check_nfvse(c);
 }

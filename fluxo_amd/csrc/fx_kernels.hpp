@@ -458,13 +458,17 @@ struct NsLift {
 #ifndef FX_LIFT_WAVES
 #define FX_LIFT_WAVES 1
 #endif
-template <int N, int OUT, class EQ>
+// LIFT 2 is BR2 (br2.f90:167-336, br2_volint.f90:45-98, br2_surfint.f90:52-131): sJ is
+// applied inside the volume integral and to every surface term, and the gradient each side
+// sees at its face is the node's volume-only gradient plus etaBR2 times that face's own
+// surface term (instead of the full gradient of BR1); the volume gradient is the same sum.
+template <int N, int OUT, class EQ, int LIFT = 1>
 __global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
-          int nElems, EQ eq)
+          int nElems, EQ eq, double etaBR2)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
   constexpr int EPB = LiftGeom<N>::EPB;
@@ -512,12 +516,19 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
     const double mf0 = mf[0], mf1 = mf[1], mf2 = mf[2], mg0 = mg[0], mg1 = mg[1], mg2 = mg[2];
     const double mh0 = mh[0], mh1 = mh[1], mh2 = mh[2];
+    const double sj = sJ[c];
     double gx[NV], gy[NV], gz[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       gx[v] = mf0 * gxi[v] + mg0 * geta[v] + mh0 * gzeta[v];
       gy[v] = mf1 * gxi[v] + mg1 * geta[v] + mh1 * gzeta[v];
       gz[v] = mf2 * gxi[v] + mg2 * geta[v] + mh2 * gzeta[v];
+      if (LIFT == 2) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+    }
+    // BR2: the volume-only gradient, the base of every face gradient (in gxi.. storage)
+    if (LIFT == 2) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) { gxi[v] = gx[v]; geta[v] = gy[v]; gzeta[v] = gz[v]; }
     }
     // surface terms: +1/2 (w_slave - w_master) SurfElem n_master L_HatMinus0 on both sides
     int locs[3];
@@ -549,19 +560,51 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       fpt[t] = (s * n + q) * n + p;
       fmaster[t] = (fl == 0);
       const double4 g = nvse[fpt[t]];
+      if (LIFT == 1) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const double ws = (fl == 0) ? wnb[v] : w[v];
-        const double wm = (fl == 0) ? w[v] : wnb[v];
-        const double f = 0.5 * (ws - wm) * g.w;
-        gx[v] = gx[v] + (f * g.x) * L0;
-        gy[v] = gy[v] + (f * g.y) * L0;
-        gz[v] = gz[v] + (f * g.z) * L0;
+        for (int v = 0; v < NV; v++) {
+          const double ws = (fl == 0) ? wnb[v] : w[v];
+          const double wm = (fl == 0) ? w[v] : wnb[v];
+          const double f = 0.5 * (ws - wm) * g.w;
+          gx[v] = gx[v] + (f * g.x) * L0;
+          gy[v] = gy[v] + (f * g.y) * L0;
+          gz[v] = gz[v] + (f * g.z) * L0;
+        }
+      } else {
+        // F_loc = sJ L_HatMinus0 Flux: into the volume gradient, and etaBR2 F_loc on the
+        // volume-only gradient is this face's gradient
+        const double sl = sj * L0;
+        double fx[NV], fy[NV], fz[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double ws = (fl == 0) ? wnb[v] : w[v];
+          const double wm = (fl == 0) ? w[v] : wnb[v];
+          const double f = 0.5 * (ws - wm) * g.w;
+          const double lx = sl * (f * g.x), ly = sl * (f * g.y), lz = sl * (f * g.z);
+          gx[v] = gx[v] + lx;
+          gy[v] = gy[v] + ly;
+          gz[v] = gz[v] + lz;
+          fx[v] = gxi[v] + etaBR2 * lx;
+          fy[v] = geta[v] + etaBR2 * ly;
+          fz[v] = gzeta[v] + etaBR2 * lz;
+        }
+        if (OUT == 1) {
+          double fv[12], o[NVV];
+          eq.grad_prim(u, fx);
+          eq.grad_prim(u, fy);
+          eq.grad_prim(u, fz);
+          eq.fluxes(u, fx, fy, fz, fv);
+          EQ::vis_dir(fv, g.x, g.y, g.z, o);
+          double *dst = out1 + ((size_t)fpt[t] * 2 + (fl == 0 ? 0 : 1)) * NVV;
+#pragma unroll
+          for (int v = 0; v < NVV; v++) dst[v] = o[v];
+        }
       }
     }
-    const double sj = sJ[c];
+    if (LIFT == 1) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+      for (int v = 0; v < NV; v++) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
+    }
     eq.grad_prim(u, gx);
     eq.grad_prim(u, gy);
     eq.grad_prim(u, gz);
@@ -582,8 +625,9 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     double fv[12], o[NVV];
 #pragma unroll
     for (int m = 0; m < 12; m++) fv[m] = s_ft[el][m][node];
-    // face-normal viscous flux of this element at each of its face points
-    for (int t = 0; t < nl; t++) {
+    // face-normal viscous flux of this element at each of its face points (BR1: from the
+    // node's full gradient; BR2 wrote them from the face gradients above)
+    for (int t = 0; t < (LIFT == 1 ? nl : 0); t++) {
       const double4 g = nvse[fpt[t]];
       EQ::vis_dir(fv, g.x, g.y, g.z, o);
       double *dst = out1 + ((size_t)fpt[t] * 2 + (fmaster[t] ? 0 : 1)) * NVV;

@@ -390,19 +390,27 @@ void launch_surface_n(const Batch &b)
   }
 }
 
-template <int N, class EQ>
-void launch_lifting_e(const Batch &b, int out, const EQ &eq)
+template <int N, class EQ, int LIFT>
+void launch_lifting_l(const Batch &b, int out, const EQ &eq)
 {
   const int epb = LiftGeom<N>::EPB;
   const int nb = (b.cfg.nElems + epb - 1) / epb;
   if (nb <= 0) return;
   const size_t nv = (size_t)EQ::NV * b.n3 * b.cfg.nElems;
   if (out == 0)
-    hipLaunchKernelGGL((k_lifting<N, 0, EQ>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq);
+    hipLaunchKernelGGL((k_lifting<N, 0, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq,
+                       b.cfg.etaBR2);
   else
-    hipLaunchKernelGGL((k_lifting<N, 1, EQ>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq);
+    hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
+                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2);
+}
+
+template <int N, class EQ>
+void launch_lifting_e(const Batch &b, int out, const EQ &eq)
+{
+  if (b.cfg.lifting == 2) launch_lifting_l<N, EQ, 2>(b, out, eq);
+  else launch_lifting_l<N, EQ, 1>(b, out, eq);
 }
 
 template <int N>

@@ -13,7 +13,7 @@ module fluxo_gpu_bind
     integer(c_int) :: N = 3, nVar = 5, nElems = 0, nSides = 0
     integer(c_int) :: volumeFlux = 0, riemann = 1, parabolic = 0, lifting = 1
     integer(c_int) :: precision = 0, reserved = 0
-    real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, reserved2 = 0.0d0
+    real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, etaBR2 = 2.0d0
     integer(c_int) :: nfvse = 0, computeAlpha = 1, spacePropSweeps = 1, reserved3 = 0
     real(c_double) :: alpha_max = 0.5d0, alpha_min = 0.01d0, spacePropFactor = 0.5d0, shockBlendCoef = 0.0d0
     real(c_double) :: mu_0 = 1.0d0, GLM_scale = 0.5d0, eta = 0.0d0, reserved4 = 0.0d0

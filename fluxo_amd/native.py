@@ -31,7 +31,7 @@ class fx_config(C.Structure):
                 ("volumeFlux", C.c_int), ("riemann", C.c_int), ("parabolic", C.c_int),
                 ("lifting", C.c_int), ("precision", C.c_int), ("reserved", C.c_int),
                 ("kappa", C.c_double), ("mu0", C.c_double), ("Pr", C.c_double),
-                ("reserved2", C.c_double),
+                ("etaBR2", C.c_double),
                 ("nfvse", C.c_int), ("computeAlpha", C.c_int), ("spacePropSweeps", C.c_int),
                 ("reserved3", C.c_int), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
                 ("spacePropFactor", C.c_double), ("shockBlendCoef", C.c_double),
@@ -141,10 +141,11 @@ def _ptr(a):
 
 def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, kappa=1.4,
                 mu0=0.0, Pr=0.72, precision=FX_PREC_FP64, nfvse=False, alpha_max=0.5, alpha_min=0.01,
-                sweeps=1, prop_factor=0.5, fixed_alpha=None):
+                sweeps=1, prop_factor=0.5, fixed_alpha=None, lifting=1, etaBR2=2.0):
     """Engine options; the NFVSE ones follow the reference's parameter names (alpha_max,
     alpha_min, SpacePropSweeps, SpacePropFactor; fixed_alpha = ComputeAlpha 3 with that
-    ShockBlendCoef, None = Persson-Peraire indicator)."""
+    ShockBlendCoef, None = Persson-Peraire indicator); lifting 1 BR1, 2 BR2 (penalty
+    etaBR2)."""
     c = fx_config()
     c.mu_0, c.GLM_scale = 1.0, 0.5
     c.nfvse, c.spacePropSweeps = int(bool(nfvse)), sweeps
@@ -152,7 +153,8 @@ def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, k
     c.alpha_max, c.alpha_min, c.spacePropFactor = alpha_max, alpha_min, prop_factor
     c.shockBlendCoef = 0.0 if fixed_alpha is None else fixed_alpha
     c.N, c.nVar, c.nElems, c.nSides = N, 5, nElems, nSides
-    c.volumeFlux, c.riemann, c.parabolic, c.lifting = volume_flux, riemann, int(bool(parabolic)), 1
+    c.volumeFlux, c.riemann, c.parabolic, c.lifting = volume_flux, riemann, int(bool(parabolic)), lifting
+    c.etaBR2 = etaBR2
     c.precision = precision
     c.kappa, c.mu0, c.Pr = kappa, mu0, Pr
     return c
@@ -181,16 +183,17 @@ def _mesh_struct(mesh):
 
 
 class Engine:
-    """One device context: mesh, state and the RHS/RK entry points."""
+    """One device context: mesh, state and the RHS/RK entry points.  `options` are the
+    further make_config arguments (NFVSE shock capturing, lifting/etaBR2)."""
 
     def __init__(self, mesh, volume_flux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0,
-                 Pr=0.72, precision=FX_PREC_FP64, **nfvse):
+                 Pr=0.72, precision=FX_PREC_FP64, **options):
         L = lib()
         self.N = int(mesh["N"])
         self.nElems = int(mesh["nElems"])
         self.nSides = int(mesh["nSides"])
         self.cfg = make_config(self.N, self.nElems, self.nSides, volume_flux, riemann, parabolic,
-                               kappa, mu0, Pr, precision, **nfvse)
+                               kappa, mu0, Pr, precision, **options)
         self._ctx = C.c_void_p()
         _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
         m, _keep = _mesh_struct(mesh)

@@ -44,14 +44,15 @@ typedef struct {
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:103-133): 0, 1, 9, 32 */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
                        resistive MHD */
-  int lifting;      /* PP_Lifting: 1 = BR1 with entropy-variable lifting */
+  int lifting;      /* PP_Lifting with entropy-variable lifting: 1 = BR1, 2 = BR2
+                       (Navier-Stokes; src/dg/br1, src/dg/br2) */
   int precision;    /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
   int reserved;
   double kappa;     /* 'kappa' (equation.f90:218) */
   double mu0;       /* constant dynamic viscosity: 'mu0' (navierstokes/equation.f90:229-230),
                        'mu' for MHD (mhd/equation.f90:190) */
   double Pr;        /* 'Pr' (equation.f90:226) */
-  double reserved2;
+  double etaBR2;    /* 'etaBR2' BR2 penalty (br2.f90:116, default 2) */
   /* NFVSE shock capturing (SHOCK_NFVSE, CMakeLists.txt:316-331; parameters of InitNFVSE
      shockcapturing/nfvse.f90:138-146), Euler only: sub-cell first-order finite volumes
      (SubFVMethod=1) blended into the volume integral of each element with the coefficient

@@ -99,7 +99,12 @@ void ora_surface_integral(const ora_problem *P, const double *U, const double *g
                           const double *gz, double *Ut);
 /* BR1 lifting of nvar lifting variables w (volume nodes): physical gradients of w */
 void ora_lift(const ora_problem *P, int nvar, const double *w, double *gx, double *gy, double *gz);
-/* BR1 lifting of the Navier-Stokes entropy variables: primitive gradients gradPx/y/z(5,n,n,n,E) */
+/* BR2 lifting of nvar lifting variables w: volume gradients and each side's master/slave
+   face gradients (nvar,0:N,0:N,nSides) */
+void ora_lift_br2(const ora_problem *P, int nvar, const double *w, double eta, double *gx, double *gy,
+                  double *gz, double *fxm, double *fym, double *fzm, double *fxs, double *fys, double *fzs);
+/* BR1 (lifting 1) or BR2 (lifting 2, volume gradients) lifting of the Navier-Stokes entropy
+   variables: primitive gradients gradPx/y/z(5,n,n,n,E) */
 void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz);
 /* Carpenter-Kennedy RK4(5) 2N low-storage step (src/timedisc/timedisc.f90:424-480) */
 void ora_lserk_step(const ora_problem *P, double *U, double *Ut_tmp, double t, double dt);

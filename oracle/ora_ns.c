@@ -454,6 +454,94 @@ void ora_lift(const ora_problem *P, int nvar, const double *w, double *gx, doubl
   free(wm); free(ws); free(fx); free(fy); free(fz);
 }
 
+/* BR2 lifting (br2.f90:167-336, br2_volint.f90:45-98, br2_fillflux.f90:49-91,
+   br2_surfint.f90:52-131) of nvar lifting variables w: volume gradients g* (sJ applied inside
+   the volume integral and to every surface term) and, per side, the face gradients of the
+   master (fm*) and slave (fs*) element: the prolonged volume-only gradient plus etaBR2 times
+   that face's own surface term.  Face arrays (nvar,0:N,0:N,nSides). */
+void ora_lift_br2(const ora_problem *P, int nvar, const double *w, double eta, double *gx, double *gy,
+                  double *gz, double *fxm, double *fym, double *fzm, double *fxs, double *fys, double *fzs)
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n, N = o.N;
+  size_t nvol = (size_t)nvar * o.n3 * o.E, nface = (size_t)nvar * n * n * o.S;
+  double *wm = malloc(sizeof(double) * nface), *ws = malloc(sizeof(double) * nface);
+  double *fl[3] = {malloc(sizeof(double) * nface), malloc(sizeof(double) * nface), malloc(sizeof(double) * nface)};
+  double *g[3] = {gx, gy, gz}, *fmst[3] = {fxm, fym, fzm}, *fslv[3] = {fxs, fys, fzs};
+  for (int d = 0; d < 3; d++) memset(g[d], 0, sizeof(double) * nvol);
+  /* Lifting_FillFlux: strong flux 1/2(w_s-w_m)*SurfElem*nv */
+  prolong_to_face(P, &o, nvar, w, wm, ws);
+  for (int s = 0; s < o.S; s++)
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        size_t f = FACE(s, p, q);
+        const double *nv = P->NormVec + 3 * f;
+        double se = P->SurfElem[f];
+        for (int v = 0; v < nvar; v++) {
+          double flx = 0.5 * (ws[nvar * f + v] - wm[nvar * f + v]) * se;
+          for (int d = 0; d < 3; d++) fl[d][nvar * f + v] = flx * nv[d];
+        }
+      }
+  /* Lifting_VolInt with sJ inside */
+  for (int e = 0; e < o.E; e++)
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          double gxi[16], geta[16], gzeta[16];
+          for (int v = 0; v < nvar; v++) {
+            gxi[v] = o.D[i + n * 0] * w[nvar * VOL(e, 0, j, k) + v];
+            geta[v] = o.D[j + n * 0] * w[nvar * VOL(e, i, 0, k) + v];
+            gzeta[v] = o.D[k + n * 0] * w[nvar * VOL(e, i, j, 0) + v];
+          }
+          for (int l = 1; l <= N; l++)
+            for (int v = 0; v < nvar; v++) {
+              gxi[v] = gxi[v] + o.D[i + n * l] * w[nvar * VOL(e, l, j, k) + v];
+              geta[v] = geta[v] + o.D[j + n * l] * w[nvar * VOL(e, i, l, k) + v];
+              gzeta[v] = gzeta[v] + o.D[k + n * l] * w[nvar * VOL(e, i, j, l) + v];
+            }
+          size_t c = VOL(e, i, j, k);
+          const double *mf = P->Mf + 3 * c, *mg = P->Mg + 3 * c, *mh = P->Mh + 3 * c;
+          const double sj = P->sJ[c];
+          for (int d = 0; d < 3; d++)
+            for (int v = 0; v < nvar; v++)
+              g[d][nvar * c + v] = g[d][nvar * c + v] + (mf[d] * gxi[v] + mg[d] * geta[v] + mh[d] * gzeta[v]) * sj;
+        }
+  /* ProlongToFace of the volume gradients, then BR2_SurfInt per direction */
+  for (int d = 0; d < 3; d++) {
+    prolong_to_face(P, &o, nvar, g[d], fmst[d], fslv[d]);
+    const double l0 = o.Lhm[0];
+    for (int s = 0; s < o.S; s++) {
+      const int *se = P->SideToElem + 5 * s;
+      int em = se[0] - 1, es = se[1] - 1, lm = se[2], ls = se[3], flp = se[4];
+      for (int q = 0; q < n; q++)
+        for (int p = 0; p < n; p++) {
+          int ijk[3];
+          size_t f = FACE(s, p, q);
+          ora_side_to_vol(N, 0, p, q, 0, lm, ijk);
+          size_t cm = VOL(em, ijk[0], ijk[1], ijk[2]);
+          for (int v = 0; v < nvar; v++) {
+            double floc = P->sJ[cm] * l0 * fl[d][nvar * f + v];
+            g[d][nvar * cm + v] = g[d][nvar * cm + v] + floc;
+            fmst[d][nvar * f + v] = fmst[d][nvar * f + v] + eta * floc;
+          }
+        }
+      for (int q = 0; q < n; q++)
+        for (int p = 0; p < n; p++) {
+          int ijk[3];
+          size_t f = FACE(s, p, q);
+          ora_side_to_vol(N, 0, p, q, flp, ls, ijk);
+          size_t cs = VOL(es, ijk[0], ijk[1], ijk[2]);
+          for (int v = 0; v < nvar; v++) {
+            double floc = P->sJ[cs] * l0 * fl[d][nvar * f + v];
+            g[d][nvar * cs + v] = g[d][nvar * cs + v] + floc;
+            fslv[d][nvar * f + v] = fslv[d][nvar * f + v] + eta * floc;
+          }
+        }
+    }
+  }
+  free(wm); free(ws); free(fl[0]); free(fl[1]); free(fl[2]);
+}
+
 /* BR1 lifting of the Navier-Stokes entropy variables; output: primitive gradients at the
    volume nodes. */
 void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, double *gz)
@@ -461,7 +549,14 @@ void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, 
   size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
   double *w = calloc((size_t)NV * n3E, sizeof(double));
   for (size_t p = 0; p < n3E; p++) ora_cons_to_entropy(P->kappa, U + NV * p, w + NV * p);
-  ora_lift(P, NV, w, gx, gy, gz);
+  if (P->lifting == 2) {  /* BR2: volume gradients; the face gradients are not returned */
+    size_t nf = (size_t)NV * (P->N + 1) * (P->N + 1) * P->nSides;
+    double *fa = malloc(sizeof(double) * 6 * nf);
+    ora_lift_br2(P, NV, w, P->etaBR2, gx, gy, gz, fa, fa + nf, fa + 2 * nf, fa + 3 * nf, fa + 4 * nf, fa + 5 * nf);
+    free(fa);
+  } else {
+    ora_lift(P, NV, w, gx, gy, gz);
+  }
   for (size_t p = 0; p < n3E; p++) {
     convert_to_grad_prim(P->kappa, U + NV * p, gx + NV * p);
     convert_to_grad_prim(P->kappa, U + NV * p, gy + NV * p);
@@ -470,25 +565,16 @@ void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, 
   free(w);
 }
 
-/* FillFlux + SurfInt (fillflux.f90:41-110, riemann.f90:104-169, surfint.f90) */
-void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
-                          const double *gz, double *Ut)
+/* FillFlux + SurfInt (fillflux.f90:41-110, riemann.f90:104-169, surfint.f90) with the face
+   gradients (primitive, (5,0:N,0:N,nSides) master/slave) given, or NULL for Euler */
+static void surface_integral_faces(const ora_problem *P, const double *U, double *const fg[6], double *Ut)
 {
   ops_t o; make_ops(P, &o);
   int n = o.n;
   size_t nface = (size_t)NV * n * n * o.S;
   double *Um = malloc(sizeof(double) * nface), *Us = malloc(sizeof(double) * nface);
   double *Fm = malloc(sizeof(double) * nface);
-  double *gxm = NULL, *gxs = NULL, *gym = NULL, *gys = NULL, *gzm = NULL, *gzs = NULL;
   prolong_to_face(P, &o, NV, U, Um, Us);
-  if (P->parabolic) {
-    gxm = malloc(sizeof(double) * nface); gxs = malloc(sizeof(double) * nface);
-    gym = malloc(sizeof(double) * nface); gys = malloc(sizeof(double) * nface);
-    gzm = malloc(sizeof(double) * nface); gzs = malloc(sizeof(double) * nface);
-    prolong_to_face(P, &o, NV, gx, gxm, gxs);
-    prolong_to_face(P, &o, NV, gy, gym, gys);
-    prolong_to_face(P, &o, NV, gz, gzm, gzs);
-  }
   for (int s = 0; s < o.S; s++)
     for (int q = 0; q < n; q++)
       for (int p = 0; p < n; p++) {
@@ -499,8 +585,8 @@ void ora_surface_integral(const ora_problem *P, const double *U, const double *g
         if (P->parabolic) {
           double kL[5], gL[5], jL[5], kR[5], gR[5], jR[5];
           const double *nv = P->NormVec + 3 * f;
-          diff_flux_3d(P, Um + NV * f, gxm + NV * f, gym + NV * f, gzm + NV * f, kL, gL, jL);
-          diff_flux_3d(P, Us + NV * f, gxs + NV * f, gys + NV * f, gzs + NV * f, kR, gR, jR);
+          diff_flux_3d(P, Um + NV * f, fg[0] + NV * f, fg[2] + NV * f, fg[4] + NV * f, kL, gL, jL);
+          diff_flux_3d(P, Us + NV * f, fg[1] + NV * f, fg[3] + NV * f, fg[5] + NV * f, kR, gR, jR);
           for (int v = 1; v < NV; v++)
             F[v] = F[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
         }
@@ -509,7 +595,53 @@ void ora_surface_integral(const ora_problem *P, const double *U, const double *g
       }
   surf_int(P, &o, NV, Fm, Fm, -1.0, Ut);
   free(Um); free(Us); free(Fm);
-  if (P->parabolic) { free(gxm); free(gxs); free(gym); free(gys); free(gzm); free(gzs); }
+}
+
+/* FillFlux + SurfInt with BR1 face gradients: the volume gradients prolonged to the faces
+   (br1.f90:154-364) */
+void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
+                          const double *gz, double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  size_t nface = (size_t)NV * o.n * o.n * o.S;
+  double *fg[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
+  if (P->parabolic) {
+    const double *g[3] = {gx, gy, gz};
+    for (int d = 0; d < 3; d++) {
+      fg[2 * d] = malloc(sizeof(double) * nface);
+      fg[2 * d + 1] = malloc(sizeof(double) * nface);
+      prolong_to_face(P, &o, NV, g[d], fg[2 * d], fg[2 * d + 1]);
+    }
+  }
+  surface_integral_faces(P, U, fg, Ut);
+  for (int d = 0; d < 6; d++) free(fg[d]);
+}
+
+/* BR2 lifting of the Navier-Stokes entropy variables with the conversion to primitive
+   gradients of the volume gradients (with U) and the face gradients (with the face states,
+   the volume nodes for LGL) (br2.f90:264-300) */
+static void br2_gradients(const ora_problem *P, const double *U, double *gx, double *gy, double *gz,
+                          double *const fg[6])
+{
+  ops_t o; make_ops(P, &o);
+  int n = o.n;
+  size_t n3E = (size_t)o.n3 * o.E, nface = (size_t)NV * n * n * o.S;
+  double *w = calloc((size_t)NV * n3E, sizeof(double));
+  for (size_t p = 0; p < n3E; p++) ora_cons_to_entropy(P->kappa, U + NV * p, w + NV * p);
+  ora_lift_br2(P, NV, w, P->etaBR2, gx, gy, gz, fg[0], fg[2], fg[4], fg[1], fg[3], fg[5]);
+  for (size_t p = 0; p < n3E; p++) {
+    convert_to_grad_prim(P->kappa, U + NV * p, gx + NV * p);
+    convert_to_grad_prim(P->kappa, U + NV * p, gy + NV * p);
+    convert_to_grad_prim(P->kappa, U + NV * p, gz + NV * p);
+  }
+  double *Um = malloc(sizeof(double) * nface), *Us = malloc(sizeof(double) * nface);
+  prolong_to_face(P, &o, NV, U, Um, Us);
+  for (size_t f = 0; f < (size_t)n * n * o.S; f++)
+    for (int d = 0; d < 3; d++) {
+      convert_to_grad_prim(P->kappa, Um + NV * f, fg[2 * d] + NV * f);
+      convert_to_grad_prim(P->kappa, Us + NV * f, fg[2 * d + 1] + NV * f);
+    }
+  free(Um); free(Us); free(w);
 }
 
 /* VolInt_visc + EvalDiffFluxTilde3D (volint.f90:228-271, flux.f90:523-635) */
@@ -566,10 +698,22 @@ void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, dou
   if (P->parabolic) {
     gx = malloc(sizeof(double) * NV * n3E); gy = malloc(sizeof(double) * NV * n3E);
     gz = malloc(sizeof(double) * NV * n3E);
-    ora_lifting(P, U, gx, gy, gz);
-    volint_visc(P, U, gx, gy, gz, Ut);
   }
-  ora_surface_integral(P, U, gx, gy, gz, Ut);
+  if (P->parabolic && P->lifting == 2) {
+    size_t nface = (size_t)NV * (P->N + 1) * (P->N + 1) * P->nSides;
+    double *fg[6];
+    for (int d = 0; d < 6; d++) fg[d] = malloc(sizeof(double) * nface);
+    br2_gradients(P, U, gx, gy, gz, fg);
+    volint_visc(P, U, gx, gy, gz, Ut);
+    surface_integral_faces(P, U, fg, Ut);
+    for (int d = 0; d < 6; d++) free(fg[d]);
+  } else {
+    if (P->parabolic) {
+      ora_lifting(P, U, gx, gy, gz);
+      volint_visc(P, U, gx, gy, gz, Ut);
+    }
+    ora_surface_integral(P, U, gx, gy, gz, Ut);
+  }
   for (size_t p = 0; p < n3E; p++) {
     double s = -P->sJ[p];
     for (int v = 0; v < NV; v++) Ut[NV * p + v] = Ut[NV * p + v] * s;

@@ -127,7 +127,7 @@ class Problem:
 
     def __init__(self, mesh, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
                  lifting=1, nfvse=False, alpha_max=0.5, alpha_min=0.01, sweeps=1, prop_factor=0.5,
-                 fixed_alpha=None):
+                 fixed_alpha=None, etaBR2=2.0):
         self.mesh = mesh
         self.N = mesh["N"]
         self.nElems = mesh["nElems"]
@@ -139,7 +139,7 @@ class Problem:
         p = _Problem()
         p.N, p.nVar, p.nElems, p.nSides = self.N, 5, mesh["nElems"], mesh["nSides"]
         p.volflux, p.riemann, p.parabolic, p.lifting = volflux, riemann, int(bool(parabolic)), lifting
-        p.kappa, p.mu0, p.Pr, p.etaBR2 = kappa, mu0, Pr, 2.0
+        p.kappa, p.mu0, p.Pr, p.etaBR2 = kappa, mu0, Pr, etaBR2
         for k in ("ElemToSide", "SideToElem", "Mf", "Mg", "Mh", "sJ", "NormVec", "TangVec1",
                   "TangVec2", "SurfElem"):
             setattr(p, k, mesh[k].ctypes.data)

@@ -31,7 +31,9 @@ build_variant() {
   case "$defs" in *-Dmhd*) eqn=mhd ;; esac
   local others='mhd|maxwell|linearscalaradvection'
   [ "$eqn" = mhd ] && others='navierstokes|maxwell|linearscalaradvection'
-  local exclude="/amr/|/equation/($others)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/br2/"
+  local lift=br2
+  case "$defs" in *-DPP_Lifting=2*) lift=br1 ;; esac
+  local exclude="/amr/|/equation/($others)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/$lift/"
   find "$REF" -name "*.f90" | grep -v -E "$exclude" | sort > "$obj/todo.txt"
   local pass n f b
   for pass in $(seq 1 15); do
@@ -58,6 +60,8 @@ NOSC="-DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0"
 build_variant euler_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=0
 build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
 build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
+# BR2 lifting (FLUXO_PARABOLIC_LIFTING=br2, CMakeLists.txt:278-283)
+build_variant ns_br2_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=2 -DPP_Lifting_Var=3 -DPP_VISC=0
 # ideal GLM-MHD with Powell non-conservative terms (FLUXO_EQNSYSNAME=mhd, FLUXO_EQN_GLM,
 # FLUXO_EQN_NONCONS=Powell; CMakeLists.txt:174-209), Derigs flux (VolumeFlux=10)
 MHD="-DPP_nVar=9 -Dmhd -DPP_entropy_vars_exist -DPP_u_aux_exist -DPP_GLM=T -DNONCONS=1"

@@ -15,11 +15,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 
-def binary(parabolic, volflux, nfvse=False, mhd=False):
+def binary(parabolic, volflux, nfvse=False, mhd=False, lifting=1):
     if mhd:
         name = f"fluxo_ref_mhd{'_br1' if parabolic else ''}_vf{volflux}"
     else:
-        name = f"fluxo_ref_{'ns_br1' if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
+        ns = f"ns_br{lifting}"
+        name = f"fluxo_ref_{ns if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
     path = os.path.join(REF_DIR, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} not built (oracle/ref/build_ref.sh)")
@@ -27,21 +28,24 @@ def binary(parabolic, volflux, nfvse=False, mhd=False):
 
 
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
-        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None):
+        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0):
     """Metrics, VolInt, RHS (+ gradients), entropy, dt and an nsteps trajectory computed by
     the reference's code on the given mesh (connectivity + NodeCoords) and state.
     nfvse: None, or dict of NFVSE parameters (ComputeAlpha, ShockBlendCoef, alpha_max,
     alpha_min, SpacePropSweeps, SpacePropFactor) for the shock-capturing build, which also
-    returns the blending coefficient alpha of the RHS evaluation."""
+    returns the blending coefficient alpha of the RHS evaluation.  lifting 2: the BR2 build
+    with penalty etaBR2 (gradients returned are the volume gradients)."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
-    exe = binary(parabolic, volflux, nfvse is not None)
+    exe = binary(parabolic, volflux, nfvse is not None, lifting=lifting)
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
             f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = 1\n"
                     f"IniRefState = 1\nRefState = (/1.,0.,0.,0.,1./)\nkappa = {kappa!r}\n"
                     f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\n")
+            if lifting == 2:
+                f.write(f"etaBR2 = {etaBR2!r}\n")
             if nfvse is not None:
                 f.write("ShockIndicator = 1\nShockIndicatorQuantity = DensityTimesPressure\n"
                         "SubFVMethod = 1\nModalThreshold = 1\nTimeRelFactor = 0.0\n")

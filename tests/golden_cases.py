@@ -20,6 +20,8 @@ def load(name):
     for k in ("mu0", "Pr", "kappa", "cfl", "dfl", "dt0", "t_end", "Entropy", "dSdU_Ut"):
         d[k] = float(d[k])
     d["nfvse"] = int(d.get("nfvse", 0))
+    d["lifting"] = int(d.get("lifting", 1))
+    d["etaBR2"] = float(d.get("etaBR2", 2.0))
     if d["nfvse"]:
         d["sweeps"] = int(d["sweeps"])
         for k in ("alpha_max", "alpha_min", "prop_factor", "fixed_alpha"):
@@ -33,6 +35,11 @@ def nfvse_args(g):
         return {}
     return dict(nfvse=True, alpha_max=g["alpha_max"], alpha_min=g["alpha_min"], sweeps=g["sweeps"],
                 prop_factor=g["prop_factor"], fixed_alpha=None if g["fixed_alpha"] < 0 else g["fixed_alpha"])
+
+
+def lifting_args(g):
+    """Problem/Engine keyword arguments of the lifting (BR1 unless the case says BR2)."""
+    return dict(lifting=g["lifting"], etaBR2=g["etaBR2"]) if g["parabolic"] else {}
 
 
 def mesh_args(g):

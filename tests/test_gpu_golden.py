@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from golden_cases import initial_state, load, mesh_args, names, nfvse_args, rel_l2
+from golden_cases import initial_state, lifting_args, load, mesh_args, names, nfvse_args, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -18,7 +18,7 @@ def test_engine_matches_reference(name):
     m = box_mesh(g["K"], g["N"], **mesh_args(g))
     U = initial_state(g, O.exactfunc, m)
     e = Engine(m, volume_flux=g["volflux"], riemann=g["riemann"], parabolic=bool(g["parabolic"]),
-               mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"], **nfvse_args(g))
+               mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"], **nfvse_args(g), **lifting_args(g))
     e.upload(U)
     if g["nfvse"]:
         da = np.max(np.abs(e.blending() - g["alpha"]))

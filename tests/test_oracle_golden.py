@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from golden_cases import initial_state, load, mesh_args, names, nfvse_args, rel_l2
+from golden_cases import initial_state, lifting_args, load, mesh_args, names, nfvse_args, rel_l2
 
 
 @pytest.mark.parametrize("name", names())
@@ -13,7 +13,7 @@ def test_oracle_matches_reference(name):
     m = O.box_mesh(g["K"], g["N"], **mesh_args(g))
     U = initial_state(g, O.exactfunc, m)
     P = O.Problem(m, volflux=g["volflux"], riemann=g["riemann"], parabolic=bool(g["parabolic"]),
-                  mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"], **nfvse_args(g))
+                  mu0=g["mu0"], Pr=g["Pr"], kappa=g["kappa"], **nfvse_args(g), **lifting_args(g))
     if "Mf" in g:
         assert np.abs(m["Mf"] - g["Mf"]).max() <= 1e-15 * np.abs(g["Mf"]).max()
         assert np.abs(m["sJ"] - g["sJ"]).max() <= 1e-15 * np.abs(g["sJ"]).max()
