@@ -6,9 +6,12 @@
 !> IniAmplitude, kappa, mu0, Pr, CFLscale, DFLscale, tEnd, maxIter, Analyze_dt.
 !> Mesh keys (this driver's periodic box generator replaces MeshFile): BoxElems (3 ints),
 !> BoxCorner, BoxLength (3 reals each), Ngeo, meshdeform (logical), meshdeformAmp.
-!> Parabolic (logical) selects Navier-Stokes with BR1 lifting at run time.
+!> Parabolic (logical) selects Navier-Stokes at run time, with BR1 lifting or, with Lifting = 2
+!> (the reference's compile-time PP_Lifting), BR2 with penalty etaBR2 (br2.f90:116, default 2).
 !> DeviceResident (logical, default T): T keeps U on the device and runs whole steps there;
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
+!> StateFile (optional): U(5,0:N,0:N,0:N,nElems) at the end, raw FP64 in Fortran order (the
+!> place of the reference's WriteState, hdf5_output.f90:74-165).
 !==================================================================================================
 program fluxo_gpu
   use, intrinsic :: iso_c_binding
@@ -19,6 +22,8 @@ program fluxo_gpu
   use fxh_dg
   implicit none
   character(len=512) :: fname
+  character(len=256) :: sfile
+  integer :: u_out
   type(fx_config) :: c
   integer :: KB(3), Ngeo, E, S, ierr, errType, iter, maxIter, nChunk, n
   real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(5), Linf(5)
@@ -44,6 +49,8 @@ program fluxo_gpu
   c%kappa = GETREAL('kappa', '1.4')
   c%mu0 = GETREAL('mu0', '0.')
   c%Pr = GETREAL('Pr', '0.72')
+  c%lifting = GETINT('Lifting', '1')
+  c%etaBR2 = GETREAL('etaBR2', '2.')
   IniExactFunc = GETINT('IniExactFunc')
   AdvVel = GETREALARRAY('AdvVel', 3, '1.,1.,1.')
   IniAmplitude = GETREAL('IniAmplitude', '0.1')
@@ -139,6 +146,12 @@ program fluxo_gpu
     write(*, '(A,5ES14.6)') ' L_inf =', Linf
   end if
   write(*, '(A,ES14.6,A,ES14.6)') ' DOF-updates/s=', rate, '  PID [s/(DOF*timestep)]=', 5.0_rk/rate
+  if (has_key('StateFile')) then
+    sfile = GETSTR('StateFile')
+    open(newunit=u_out, file=trim(sfile), access='stream', form='unformatted', status='replace')
+    write(u_out) U
+    close(u_out)
+  end if
   write(*, '(A)') ' FLUXO_GPU FINISHED'
   call FinalizeDG_GPU()
 end program fluxo_gpu
