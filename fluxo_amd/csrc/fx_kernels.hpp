@@ -387,26 +387,50 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 // Side-centric numerical flux: Flux(:,p,q,s) = F*(U_master, U_slave; n) * SurfElem, with
 // the BR1 viscous average for Navier-Stokes (FillFlux fillflux.f90:41-110, Riemann
 // riemann.f90:104-169).
+// Launch geometry of the surface kernel: whole sides per block (SPB sides of n^2 points), so
+// that a face-wide maximum (Riemann 16) is a block-local reduction.
+template <int N> struct SurfGeom {
+  static constexpr int n2 = (N + 1) * (N + 1), SPB = 256 / n2, T = SPB * n2;
+};
+
 template <int N, int RS, bool PARA>
 __global__ void __launch_bounds__(256)
 k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
           const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nSides * n2) return;
+  constexpr bool ES = (RS == 4 || RS == 16 || RS == 33 || RS == 44);
+  const int gid = blockIdx.x * SurfGeom<N>::T + threadIdx.x;
   const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
-  const int em = s2e[5 * s] - 1, es = s2e[5 * s + 1] - 1;
-  if (em < 0) return;  // unused side slot of a batch (SideToElem zeroed)
-  const int lm = s2e[5 * s + 2], ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
-  const int vm = side_to_vol<N>(lm, 0, p, q), vs = side_to_vol<N>(ls, fl, p, q);
-  const double *um = U + ((size_t)em * n3 + vm) * 5, *us = U + ((size_t)es * n3 + vs) * 5;
-  double UL[5] = {um[0], um[1], um[2], um[3], um[4]};
-  double UR[5] = {us[0], us[1], us[2], us[3], us[4]};
-  const double4 g = nvse[gid];
-  const double nv[3] = {g.x, g.y, g.z};
+  const bool valid = threadIdx.x < SurfGeom<N>::T && gid < nSides * n2;
+  const int em = valid ? s2e[5 * s] - 1 : -1;
+  const bool used = valid && em >= 0;  // unused side slots of a batch have SideToElem zeroed
+  double UL[5], UR[5], nv[3] = {0.0, 0.0, 0.0}, se = 0.0;
+  if (used) {
+    const int es = s2e[5 * s + 1] - 1;
+    const int lm = s2e[5 * s + 2], ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
+    const int vm = side_to_vol<N>(lm, 0, p, q), vs = side_to_vol<N>(ls, fl, p, q);
+    const double *um = U + ((size_t)em * n3 + vm) * 5, *us = U + ((size_t)es * n3 + vs) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; v++) { UL[v] = um[v]; UR[v] = us[v]; }
+    const double4 g = nvse[gid];
+    nv[0] = g.x; nv[1] = g.y; nv[2] = g.z; se = g.w;
+  }
+  double lam = 0.0;
+  if (RS == 16) {  // LambdaMaxMax of the face
+    __shared__ double s_lam[SurfGeom<N>::T];
+    if (threadIdx.x < SurfGeom<N>::T) s_lam[threadIdx.x] = used ? face_lambda(UL, UR, nv, kappa) : 0.0;
+    __syncthreads();
+    if (used) {
+      const int b0 = threadIdx.x - pq;
+#pragma unroll 4
+      for (int k = 0; k < n2; k++) lam = fmax(lam, s_lam[b0 + k]);
+    }
+  }
+  if (!used) return;
   double F[5];
-  riemann<RS>(UL, UR, nv, kappa, F);
+  if (ES) riemann_es<ES ? RS : 33>(UL, UR, nv, kappa, lam, F);
+  else riemann<RS>(UL, UR, nv, kappa, F);
   if (PARA) {  // BR1: mean of the two sides' normal viscous fluxes
     const double *fn = FvN + (size_t)gid * 8;
 #pragma unroll
@@ -414,7 +438,7 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
   }
   double *o = Flux + (size_t)gid * 5;
 #pragma unroll
-  for (int v = 0; v < 5; v++) o[v] = F[v] * g.w;
+  for (int v = 0; v < 5; v++) o[v] = F[v] * se;
 }
 
 // BR1 lifting with entropy variables (Lifting br1.f90:154-364: Lifting_VolInt

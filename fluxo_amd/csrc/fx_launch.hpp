@@ -368,14 +368,15 @@ void launch_volume_n(const Batch &b, int mode, RKArgs rk)
 template <int N, int RS>
 void launch_surface_t(const Batch &b)
 {
+  constexpr int T = SurfGeom<N>::T;
   const long long tot = (long long)b.cfg.nSides * b.n2;
-  const int nb = (int)((tot + 255) / 256);
+  const int nb = (int)((tot + T - 1) / T);
   if (nb <= 0) return;
   if (b.cfg.parabolic)
-    hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e,
+    hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(T), 0, b.stream, b.U, b.FvN, b.s2e,
                        b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
   else
-    hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e,
+    hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(T), 0, b.stream, b.U, b.FvN, b.s2e,
                        b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
 }
 
@@ -384,8 +385,12 @@ void launch_surface_n(const Batch &b)
 {
   switch (b.cfg.riemann) {
   case 0: launch_surface_t<N, 0>(b); break;
+  case 4: launch_surface_t<N, 4>(b); break;
   case 9: launch_surface_t<N, 9>(b); break;
+  case 16: launch_surface_t<N, 16>(b); break;
   case 32: launch_surface_t<N, 32>(b); break;
+  case 33: launch_surface_t<N, 33>(b); break;
+  case 44: launch_surface_t<N, 44>(b); break;
   default: launch_surface_t<N, 1>(b); break;
   }
 }
@@ -661,8 +666,9 @@ inline int check_physics(int volumeFlux, int riemann, int precision, double kapp
 {
   if (volumeFlux != 0 && volumeFlux != 5 && volumeFlux != 32)
     return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0,5,32)", volumeFlux);
-  if (riemann != 0 && riemann != 1 && riemann != 9 && riemann != 32)
-    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,9,32)", riemann);
+  if (riemann != 0 && riemann != 1 && riemann != 4 && riemann != 9 && riemann != 16 && riemann != 32 &&
+      riemann != 33 && riemann != 44)
+    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4,9,16,32,33,44)", riemann);
   if (precision != FX_PREC_FP64 && precision != FX_PREC_FP32_SPLIT)
     return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", precision);
   if (!(kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");
@@ -675,6 +681,8 @@ inline int check_nfvse(const fx_config *c)
   if (!c->nfvse) return FX_OK;
   if (c->nfvse != 1) return set_err(FX_EINVAL, "nfvse=%d (0 off, 1 first-order sub-cell FV)", c->nfvse);
   if (c->parabolic) return set_err(FX_EINVAL, "NFVSE shock capturing is supported for Euler (parabolic=0)");
+  if (c->riemann != 0 && c->riemann != 1 && c->riemann != 9 && c->riemann != 32)
+    return set_err(FX_EINVAL, "NFVSE sub-cell fluxes: Riemann 0, 1, 9, 32");
   if (c->computeAlpha != 1 && c->computeAlpha != 3)
     return set_err(FX_EINVAL, "ComputeAlpha=%d not supported (1 Persson-Peraire, 3 fixed)", c->computeAlpha);
   if (c->spacePropSweeps < 0 || c->spacePropSweeps > 16) return set_err(FX_EINVAL, "SpacePropSweeps out of 0..16");

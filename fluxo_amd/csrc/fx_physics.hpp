@@ -245,6 +245,135 @@ __device__ __forceinline__ void riemann(const double *UL, const double *UR, cons
   }
 }
 
+// Entropy-stable surface fluxes in the Cartesian frame.  The reference rotates the states
+// into (n, t1, t2) (riemann.f90:174-246) and applies, at one face point,
+//   RS=4, 44, 33: RiemannSolver_EntropyStable (riemann.f90:874-975): the Ismail-Roe
+//     (TwoPointEntropyConservingFlux flux_average.f90:500-572), Chandrashekar
+//     (EntropyAndEnergyConservingFlux :1214-1269) or Ranocha (RanochaFlux :1806-1866) flux
+//     minus 1/2 R D R^T [v] (entropy variables v, right eigenvectors R, D of the averaged
+//     state: 1/2|u-a| rho/kappa, |u|(kappa-1)/kappa rho, |u| p1 (twice), 1/2|u+a| rho/kappa);
+//   RS=16: RiemannSolver_ECKEP_LLF (riemann.f90:1157-1209): the Chandrashekar flux minus
+//     1/2 lam times the entropy-consistent jump, lam the face's largest wave speed.
+// Here the averaged velocity stays a Cartesian vector vh (u = vh.n is the normal part), the
+// eigenvectors of the two tangential waves enter only through their sum, which is the
+// projection onto the face plane; the result equals the rotated-back reference flux in
+// exact arithmetic.
+__device__ __forceinline__ double ln_mean2(double a, double b) { return ln_mean(a, b, log(a), log(b)); }
+
+template <int RS>
+__device__ __forceinline__ void riemann_es(const double *UL, const double *UR, const double *n, double kappa,
+                                           double lam_face, double *F)
+{
+  const double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  const double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  const double vL[3] = {UL[1] * sL, UL[2] * sL, UL[3] * sL}, vR[3] = {UR[1] * sR, UR[2] * sR, UR[3] * sR};
+  const double pL = km1 * (UL[4] - 0.5 * (UL[1] * vL[0] + UL[2] * vL[1] + UL[3] * vL[2]));
+  const double pR = km1 * (UR[4] - 0.5 * (UR[1] * vR[0] + UR[2] * vR[1] + UR[3] * vR[2]));
+  const double vnL = vL[0] * n[0] + vL[1] * n[1] + vL[2] * n[2];
+  const double vnR = vR[0] * n[0] + vR[1] * n[1] + vR[2] * n[2];
+  double rhoH, vh[3], p1H, aH, HH, Fc[5];
+  if (RS == 4) {  // Ismail-Roe
+    const double z1L = sqrt(UL[0] / pL), z1R = sqrt(UR[0] / pR);
+    const double z1M = 0.5 * (z1L + z1R), z1LN = ln_mean2(z1L, z1R);
+    const double z5L = pL * z1L, z5R = pR * z1R;
+    const double z5M = 0.5 * (z5L + z5R), z5LN = ln_mean2(z5L, z5R);
+    rhoH = z1M * z5LN;
+    const double sz1 = 1.0 / z1M;
+#pragma unroll
+    for (int d = 0; d < 3; d++) vh[d] = (0.5 * (z1L * vL[d] + z1R * vR[d])) * sz1;
+    p1H = z5M * sz1;
+    const double kp2 = 0.5 * ((kappa + 1.0) * (z5LN / z1LN) + km1 * p1H);
+    aH = sqrt(kp2 / rhoH);
+    HH = kp2 / (rhoH * km1) + 0.5 * (vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]);
+    const double un = vh[0] * n[0] + vh[1] * n[1] + vh[2] * n[2];
+    Fc[0] = rhoH * un;
+#pragma unroll
+    for (int d = 0; d < 3; d++) Fc[1 + d] = Fc[0] * vh[d] + p1H * n[d];
+    Fc[4] = Fc[0] * HH;
+  } else if (RS == 44 || RS == 16) {  // Chandrashekar
+    const double v2s2L = 0.5 * (vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2]);
+    const double v2s2R = 0.5 * (vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]);
+    const double bL = 0.5 * UL[0] / pL, bR = 0.5 * UR[0] / pR;
+    rhoH = ln_mean2(UL[0], UR[0]);
+    const double bH = ln_mean2(bL, bR);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vh[d] = 0.5 * (vL[d] + vR[d]);
+    aH = sqrt(kappa * 0.5 * (pL + pR) / rhoH);
+    HH = kappa / (2.0 * km1 * bH) + 0.5 * (vL[0] * vR[0] + vL[1] * vR[1] + vL[2] * vR[2]);
+    p1H = 0.5 * (0.5 * (UL[0] + UR[0])) / (0.5 * (bL + bR));
+    const double un = vh[0] * n[0] + vh[1] * n[1] + vh[2] * n[2];
+    Fc[0] = rhoH * un;
+#pragma unroll
+    for (int d = 0; d < 3; d++) Fc[1 + d] = Fc[0] * vh[d] + p1H * n[d];
+    Fc[4] = Fc[0] * 0.5 * (skm1 / bH - (v2s2L + v2s2R)) + (vh[0] * Fc[1] + vh[1] * Fc[2] + vh[2] * Fc[3]);
+    if (RS == 16) {
+      const double rhoa = 0.5 * (UR[0] + UL[0]);
+      const double jump = (0.5 / (km1 * bH) + 0.5 * (vL[0] * vR[0] + vL[1] * vR[1] + vL[2] * vR[2])) * (UR[0] - UL[0]) +
+                          rhoa * (vh[0] * (vR[0] - vL[0]) + vh[1] * (vR[1] - vL[1]) + vh[2] * (vR[2] - vL[2]) +
+                                  0.5 / km1 * (1.0 / bR - 1.0 / bL));
+#pragma unroll
+      for (int v = 0; v < 4; v++) F[v] = Fc[v] - 0.5 * lam_face * (UR[v] - UL[v]);
+      F[4] = Fc[4] - 0.5 * lam_face * jump;
+      return;
+    }
+  } else {  // Ranocha (33)
+    rhoH = ln_mean2(UL[0], UR[0]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vh[d] = 0.5 * (vL[d] + vR[d]);
+    const double v2zip = 0.5 * (vL[0] * vR[0] + vL[1] * vR[1] + vL[2] * vR[2]);
+    p1H = 0.5 * (pL + pR);
+    aH = sqrt(kappa * p1H / rhoH);
+    HH = kappa * p1H * skm1 / rhoH + 0.5 * (vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]);
+    const double rspM = ln_mean2(UL[0] / pL, UR[0] / pR);
+    const double un = vh[0] * n[0] + vh[1] * n[1] + vh[2] * n[2];
+    Fc[0] = rhoH * un;
+#pragma unroll
+    for (int d = 0; d < 3; d++) Fc[1 + d] = Fc[0] * vh[d] + p1H * n[d];
+    Fc[4] = Fc[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (pL * vnR + pR * vnL);
+  }
+  // 1/2 R D R^T [v]
+  const double un = vh[0] * n[0] + vh[1] * n[1] + vh[2] * n[2];
+  const double D1 = 0.5 * fabs(un - aH) * rhoH / kappa, D2 = fabs(un) * (km1 / kappa) * rhoH;
+  const double D3 = fabs(un) * p1H, D5 = 0.5 * fabs(un + aH) * rhoH / kappa;
+  const double rpL = UL[0] / pL, rpR = UR[0] / pR;
+  const double sEL = log(pL) - kappa * log(UL[0]), sER = log(pR) - kappa * log(UR[0]);
+  const double wv2L = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
+  const double wv2R = vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2];
+  const double dv0 = ((kappa - sER) / km1 - 0.5 * rpR * wv2R) - ((kappa - sEL) / km1 - 0.5 * rpL * wv2L);
+  double dm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) dm[d] = rpR * vR[d] - rpL * vL[d];
+  const double dv4 = rpL - rpR;  // -rho/p
+  const double dmn = dm[0] * n[0] + dm[1] * n[1] + dm[2] * n[2];
+  const double dmv = dm[0] * vh[0] + dm[1] * vh[1] + dm[2] * vh[2];
+  const double vh2 = vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2];
+  const double c1 = D1 * (dv0 + dmv - aH * dmn + (HH - un * aH) * dv4);
+  const double c2 = D2 * (dv0 + dmv + 0.5 * vh2 * dv4);
+  const double c5 = D5 * (dv0 + dmv + aH * dmn + (HH + un * aH) * dv4);
+  double wt[3], vt[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    vt[d] = vh[d] - un * n[d];
+    wt[d] = (dm[d] - dmn * n[d]) + dv4 * vt[d];
+  }
+  const double vtw = vt[0] * wt[0] + vt[1] * wt[1] + vt[2] * wt[2];
+  F[0] = Fc[0] - 0.5 * (c1 + c2 + c5);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    F[1 + d] = Fc[1 + d] - 0.5 * (c1 * (vh[d] - aH * n[d]) + c2 * vh[d] + c5 * (vh[d] + aH * n[d]) + D3 * wt[d]);
+  F[4] = Fc[4] - 0.5 * (c1 * (HH - un * aH) + c2 * 0.5 * vh2 + c5 * (HH + un * aH) + D3 * vtw);
+}
+
+// The largest wave speed of one face point (the LambdaMaxMax term of riemann.f90:1176-1181).
+__device__ __forceinline__ double face_lambda(const double *UL, const double *UR, const double *n, double kappa)
+{
+  const double km1 = kappa - 1.0;
+  const double mL = UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2], mR = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
+  const double c2L = kappa * km1 * (UL[4] * UL[0] - 0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3])) / (UL[0] * UL[0]);
+  const double c2R = kappa * km1 * (UR[4] * UR[0] - 0.5 * (UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3])) / (UR[0] * UR[0]);
+  return fmax(fabs(mL / UL[0]), fabs(mR / UR[0])) + sqrt(fmax(c2L, c2R));
+}
+
 // ConsToEntropy (navierstokes/equation_vars.f90:362-391); s = -ln(rho_sp*rho^(kappa-1))
 // is evaluated as -(ln rho_sp + (kappa-1) ln rho).
 __device__ __forceinline__ void cons_to_entropy(const double *U, double kappa, double *w)

@@ -41,7 +41,10 @@ typedef struct {
   int nElems;       /* local elements */
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
   int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32; MHD 10 */
-  int riemann;      /* 'Riemann' (navierstokes/equation.f90:103-133): 0, 1, 9, 32 */
+  int riemann;      /* 'Riemann' (navierstokes/equation.f90:291-404): 0 central, 1 LLF,
+                       9 / 32 EC (Chandrashekar / Ranocha), 4 / 44 / 33 entropy stable
+                       (Ismail-Roe / Chandrashekar / Ranocha + matrix dissipation),
+                       16 EC-KEP + face LLF */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
                        resistive MHD */
   int lifting;      /* PP_Lifting with entropy-variable lifting: 1 = BR1, 2 = BR2

@@ -86,8 +86,13 @@ double ora_ln_mean(double aL, double aR);
 void ora_eval_uaux(double kappa, const double *U, double *aux);
 void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
                  const double *auxR, const double *mL, const double *mR, double *F);
+/* numerical flux at one face point (Riemann 0, 1, 4, 9, 16, 32, 33, 44); Riemann 16 takes the
+   face's largest wave speed lam_face (ora_riemann uses the point's own) */
 void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
                  const double *t1, const double *t2, double *F);
+void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR, const double *nv,
+                     const double *t1, const double *t2, double lam_face, double *F);
+double ora_face_lambda(double kappa, const double *UL, const double *UR, const double *nv);
 void ora_cons_to_entropy(double kappa, const double *U, double *w);
 
 /* ---- DG operator ---- */

@@ -163,11 +163,153 @@ static void flux1d_ranocha(double kappa, const double *UL, const double *UR, dou
   F[4] = F[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (pL * vR[0] + pR * vL[0]);
 }
 
+/* The 1D EC fluxes with the averaged quantities the entropy-stable dissipation uses
+   (uHat, vHat, wHat, aHat, HHat, p1Hat, rhoHat as in the reference's signatures):
+   TwoPointEntropyConservingFlux / Ismail-Roe (flux_average.f90:500-572),
+   EntropyAndEnergyConservingFlux / Chandrashekar (:1214-1269), RanochaFlux (:1806-1866). */
+typedef struct { double u, v, w, a, H, p1, rho; } hat_t;
+
+static void flux1d_hat(int which, double kappa, const double *UL, const double *UR, double *F, hat_t *h)
+{
+  const double km1 = kappa - 1.0, skm1 = 1.0 / km1;
+  const double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  const double uL = sL * UL[1], vL = sL * UL[2], wL = sL * UL[3];
+  const double uR = sR * UR[1], vR = sR * UR[2], wR = sR * UR[3];
+  if (which == 4) {  /* Ismail-Roe */
+    const double pL = km1 * (UL[4] - 0.5 * (UL[1] * uL + UL[2] * vL + UL[3] * wL));
+    const double pR = km1 * (UR[4] - 0.5 * (UR[1] * uR + UR[2] * vR + UR[3] * wR));
+    const double z1L = sqrt(UL[0] / pL), z1R = sqrt(UR[0] / pR);
+    const double z1M = 0.5 * (z1L + z1R), z1LN = ora_ln_mean(z1L, z1R);
+    const double z2M = 0.5 * (z1L * uL + z1R * uR), z3M = 0.5 * (z1L * vL + z1R * vR), z4M = 0.5 * (z1L * wL + z1R * wR);
+    const double z5L = pL * z1L, z5R = pR * z1R;
+    const double z5M = 0.5 * (z5L + z5R), z5LN = ora_ln_mean(z5L, z5R);
+    h->rho = z1M * z5LN;
+    const double sz1 = 1.0 / z1M;
+    h->u = z2M * sz1; h->v = z3M * sz1; h->w = z4M * sz1;
+    h->p1 = z5M * sz1;
+    const double kp2 = 0.5 * ((kappa + 1.0) * (z5LN / z1LN) + km1 * h->p1);
+    h->a = sqrt(kp2 / h->rho);
+    h->H = kp2 / (h->rho * km1) + 0.5 * (h->u * h->u + h->v * h->v + h->w * h->w);
+    F[0] = h->rho * h->u;
+    F[1] = F[0] * h->u + h->p1;
+    F[2] = F[0] * h->v;
+    F[3] = F[0] * h->w;
+    F[4] = F[0] * h->H;
+  } else if (which == 44 || which == 16) {  /* Chandrashekar */
+    const double v2s2L = 0.5 * (uL * uL + vL * vL + wL * wL), v2s2R = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double pL = km1 * (UL[4] - UL[0] * v2s2L), pR = km1 * (UR[4] - UR[0] * v2s2R);
+    const double bL = 0.5 * UL[0] / pL, bR = 0.5 * UR[0] / pR;
+    const double rho_mean = 0.5 * (UL[0] + UR[0]);
+    h->rho = ora_ln_mean(UL[0], UR[0]);
+    const double beta_mean = 0.5 * (bL + bR), beta_hat = ora_ln_mean(bL, bR);
+    h->u = 0.5 * (uL + uR); h->v = 0.5 * (vL + vR); h->w = 0.5 * (wL + wR);
+    h->a = sqrt(kappa * 0.5 * (pL + pR) / h->rho);
+    h->H = kappa / (2.0 * km1 * beta_hat) + 0.5 * (uL * uR + vL * vR + wL * wR);
+    h->p1 = 0.5 * rho_mean / beta_mean;
+    F[0] = h->rho * h->u;
+    F[1] = F[0] * h->u + h->p1;
+    F[2] = F[0] * h->v;
+    F[3] = F[0] * h->w;
+    F[4] = F[0] * 0.5 * (skm1 / beta_hat - (v2s2L + v2s2R)) + h->u * F[1] + h->v * F[2] + h->w * F[3];
+  } else {  /* Ranocha (33) */
+    const double v2L = uL * uL + vL * vL + wL * wL, v2R = uR * uR + vR * vR + wR * wR;
+    const double pL = km1 * (UL[4] - 0.5 * UL[0] * v2L), pR = km1 * (UR[4] - 0.5 * UR[0] * v2R);
+    h->rho = ora_ln_mean(UL[0], UR[0]);
+    h->u = 0.5 * (uL + uR); h->v = 0.5 * (vL + vR); h->w = 0.5 * (wL + wR);
+    const double v2zip = 0.5 * (uL * uR + vL * vR + wL * wR);
+    h->p1 = 0.5 * (pL + pR);
+    h->a = sqrt(kappa * h->p1 / h->rho);
+    h->H = kappa * h->p1 * skm1 / h->rho + 0.5 * (h->u * h->u + h->v * h->v + h->w * h->w);
+    const double rspM = ora_ln_mean(UL[0] / pL, UR[0] / pR);
+    F[0] = h->rho * h->u;
+    F[1] = F[0] * h->u + h->p1;
+    F[2] = F[0] * h->v;
+    F[3] = F[0] * h->w;
+    F[4] = F[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (pL * uR + pR * uL);
+  }
+}
+
+/* RiemannSolver_EntropyStable (riemann.f90:874-931) with
+   RiemannSolver_EntropyStable_VolFluxAndDissipMatrices (:936-975): EC flux minus
+   1/2 R D R^T [v], v the entropy variables, R the right eigenvectors, D the scaled
+   eigenvalues; rotated frame. */
+static void riemann_es(int which, double kappa, const double *uLL, const double *uRR, double *f)
+{
+  const double km1 = kappa - 1.0;
+  double Fc[5];
+  hat_t h;
+  flux1d_hat(which, kappa, uLL, uRR, Fc, &h);
+  double R[5][5] = {{1.0, 1.0, 0.0, 0.0, 1.0},
+                    {h.u - h.a, h.u, 0.0, 0.0, h.u + h.a},
+                    {h.v, h.v, 1.0, 0.0, h.v},
+                    {h.w, h.w, 0.0, 1.0, h.w},
+                    {h.H - h.u * h.a, 0.5 * (h.u * h.u + h.v * h.v + h.w * h.w), h.v, h.w, h.H + h.u * h.a}};
+  double D[5];
+  D[0] = 0.5 * fabs(h.u - h.a) * h.rho / kappa;
+  D[1] = fabs(h.u) * (km1 / kappa) * h.rho;
+  D[2] = fabs(h.u) * h.p1;
+  D[3] = D[2];
+  D[4] = 0.5 * fabs(h.u + h.a) * h.rho / kappa;
+  const double sL = 1.0 / uLL[0], sR = 1.0 / uRR[0];
+  const double VL[3] = {uLL[1] * sL, uLL[2] * sL, uLL[3] * sL}, VR[3] = {uRR[1] * sR, uRR[2] * sR, uRR[3] * sR};
+  const double pL = km1 * (uLL[4] - 0.5 * (uLL[1] * VL[0] + uLL[2] * VL[1] + uLL[3] * VL[2]));
+  const double pR = km1 * (uRR[4] - 0.5 * (uRR[1] * VR[0] + uRR[2] * VR[1] + uRR[3] * VR[2]));
+  const double sRs = log(pR) - kappa * log(uRR[0]), sLs = log(pL) - kappa * log(uLL[0]);
+  const double rpR = uRR[0] / pR, rpL = uLL[0] / pL;
+  double vL[5], vR[5], dv[5], d[5], diss[5];
+  vL[0] = (kappa - sLs) / km1 - 0.5 * rpL * (VL[0] * VL[0] + VL[1] * VL[1] + VL[2] * VL[2]);
+  vR[0] = (kappa - sRs) / km1 - 0.5 * rpR * (VR[0] * VR[0] + VR[1] * VR[1] + VR[2] * VR[2]);
+  for (int c = 0; c < 3; c++) { vL[1 + c] = rpL * VL[c]; vR[1 + c] = rpR * VR[c]; }
+  vR[4] = -rpR;
+  vL[4] = -rpL;
+  for (int c = 0; c < 5; c++) dv[c] = vR[c] - vL[c];
+  for (int c = 0; c < 5; c++)  /* R^T [v] */
+    d[c] = R[0][c] * dv[0] + R[1][c] * dv[1] + R[2][c] * dv[2] + R[3][c] * dv[3] + R[4][c] * dv[4];
+  for (int c = 0; c < 5; c++) d[c] = D[c] * d[c];
+  for (int r = 0; r < 5; r++)
+    diss[r] = R[r][0] * d[0] + R[r][1] * d[1] + R[r][2] * d[2] + R[r][3] * d[3] + R[r][4] * d[4];
+  for (int c = 0; c < 5; c++) f[c] = Fc[c] - 0.5 * diss[c];
+}
+
+/* RiemannSolver_ECKEP_LLF (riemann.f90:1157-1209): Chandrashekar flux with a local
+   Lax-Friedrichs dissipation whose energy part is consistent with the entropy variables;
+   lam is the largest wave speed over the whole face (LambdaMaxMax). */
+static void riemann_eckep_llf(double kappa, const double *uLL, const double *uRR, double lam, double *f)
+{
+  const double km1 = kappa - 1.0;
+  double Fc[5];
+  hat_t h;
+  flux1d_hat(16, kappa, uLL, uRR, Fc, &h);
+  const double sL = 1.0 / uLL[0], sR = 1.0 / uRR[0];
+  const double VL[3] = {uLL[1] * sL, uLL[2] * sL, uLL[3] * sL}, VR[3] = {uRR[1] * sR, uRR[2] * sR, uRR[3] * sR};
+  const double pL = km1 * (uLL[4] - 0.5 * (uLL[1] * VL[0] + uLL[2] * VL[1] + uLL[3] * VL[2]));
+  const double pR = km1 * (uRR[4] - 0.5 * (uRR[1] * VR[0] + uRR[2] * VR[1] + uRR[3] * VR[2]));
+  const double bL = 0.5 * uLL[0] / pL, bR = 0.5 * uRR[0] / pR;
+  const double rhoa = 0.5 * (uRR[0] + uLL[0]);
+  const double bLN = ora_ln_mean(bL, bR);
+  for (int c = 0; c < 4; c++) f[c] = Fc[c] - 0.5 * lam * (uRR[c] - uLL[c]);
+  f[4] = Fc[4] - 0.5 * lam * ((0.5 / (km1 * bLN) + 0.5 * (VL[0] * VR[0] + VL[1] * VR[1] + VL[2] * VR[2])) * (uRR[0] - uLL[0]) +
+                              rhoa * (h.u * (VR[0] - VL[0]) + h.v * (VR[1] - VL[1]) + h.w * (VR[2] - VL[2]) +
+                                      0.5 / km1 * (1.0 / bR - 1.0 / bL)));
+}
+
+/* the largest wave speed of one face point in the rotated frame (the LambdaMaxMax term) */
+double ora_face_lambda(double kappa, const double *UL, const double *UR, const double *nv)
+{
+  const double mL = UL[1] * nv[0] + UL[2] * nv[1] + UL[3] * nv[2];
+  const double mR = UR[1] * nv[0] + UR[2] * nv[1] + UR[3] * nv[2];
+  const double a = fabs(mL / UL[0]), b = fabs(mR / UR[0]);
+  const double c2L = sound_speed2(kappa, UL), c2R = sound_speed2(kappa, UR);
+  return (a > b ? a : b) + sqrt(c2L > c2R ? c2L : c2R);
+}
+
 /* Riemann -> AdvRiemann: rotate, solve, rotate back (riemann.f90:174-246); solvers:
    0 central (:251-272), 1 Rusanov/LLF (:279-313), 9 EC-KEP and 32 Ranocha through
-   RiemannSolver_VolumeFluxAverage (:980-1005). Point-wise (one face node). */
-void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
-                 const double *t1, const double *t2, double *F)
+   RiemannSolver_VolumeFluxAverage (:980-1005), 4 / 44 / 33 entropy stable with the
+   Ismail-Roe / Chandrashekar / Ranocha flux (:874-975), 16 EC-KEP + LLF (:1157-1209) with the
+   face's LambdaMaxMax in lam_face. Point-wise (one face node). */
+void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR, const double *nv,
+                     const double *t1, const double *t2, double lam_face, double *F)
 {
   double uLL[5], uRR[5];
   uLL[0] = UL[0]; uRR[0] = UR[0];
@@ -193,6 +335,10 @@ void ora_riemann(int which, double kappa, const double *UL, const double *UR, co
     }
   } else if (which == 9) {
     flux1d_chandrashekar(kappa, uLL, uRR, f);
+  } else if (which == 4 || which == 33 || which == 44) {
+    riemann_es(which, kappa, uLL, uRR, f);
+  } else if (which == 16) {
+    riemann_eckep_llf(kappa, uLL, uRR, lam_face, f);
   } else {
     flux1d_ranocha(kappa, uLL, uRR, f);
   }
@@ -201,6 +347,12 @@ void ora_riemann(int which, double kappa, const double *UL, const double *UR, co
   F[2] = nv[1] * f[1] + t1[1] * f[2] + t2[1] * f[3];
   F[3] = nv[2] * f[1] + t1[2] * f[2] + t2[2] * f[3];
   F[4] = f[4];
+}
+
+void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
+                 const double *t1, const double *t2, double *F)
+{
+  ora_riemann_lam(which, kappa, UL, UR, nv, t1, t2, ora_face_lambda(kappa, UL, UR, nv), F);
 }
 
 /* ConsToEntropy (equation_vars.f90:362-391) */
@@ -575,13 +727,18 @@ static void surface_integral_faces(const ora_problem *P, const double *U, double
   double *Um = malloc(sizeof(double) * nface), *Us = malloc(sizeof(double) * nface);
   double *Fm = malloc(sizeof(double) * nface);
   prolong_to_face(P, &o, NV, U, Um, Us);
-  for (int s = 0; s < o.S; s++)
+  for (int s = 0; s < o.S; s++) {
+    double lam = 0.0;  /* LambdaMaxMax of the face (Riemann 16) */
+    for (int f = s * n * n; f < (s + 1) * n * n; f++) {
+      const double l = ora_face_lambda(P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f);
+      lam = lam > l ? lam : l;
+    }
     for (int q = 0; q < n; q++)
       for (int p = 0; p < n; p++) {
         size_t f = FACE(s, p, q);
         double *F = Fm + NV * f;
-        ora_riemann(P->riemann, P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f,
-                    P->TangVec1 + 3 * f, P->TangVec2 + 3 * f, F);
+        ora_riemann_lam(P->riemann, P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f,
+                        P->TangVec1 + 3 * f, P->TangVec2 + 3 * f, lam, F);
         if (P->parabolic) {
           double kL[5], gL[5], jL[5], kR[5], gR[5], jR[5];
           const double *nv = P->NormVec + 3 * f;
@@ -593,6 +750,7 @@ static void surface_integral_faces(const ora_problem *P, const double *U, double
         double se = P->SurfElem[f];
         for (int v = 0; v < NV; v++) F[v] = F[v] * se;
       }
+  }
   surf_int(P, &o, NV, Fm, Fm, -1.0, Ut);
   free(Um); free(Us); free(Fm);
 }

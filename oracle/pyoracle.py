@@ -54,6 +54,7 @@ def lib():
                                     _dp, _dp]
         L.ora_volflux.argtypes = [C.c_int, C.c_double] + [_dp] * 7
         L.ora_riemann.argtypes = [C.c_int, C.c_double] + [_dp] * 6
+        L.ora_riemann_lam.argtypes = [C.c_int, C.c_double] + [_dp] * 5 + [C.c_double, _dp]
         L.ora_cons_to_entropy.argtypes = [C.c_double, _dp, _dp]
         L.ora_rk_coefs.argtypes = [_dp, _dp, _dp]
         L.ora_lgl_nodes.argtypes = [C.c_int, _dp, _dp]

@@ -1,0 +1,65 @@
+"""Surface fluxes of the oracle beyond the golden vectors (euler_*_es*, *_eckep_llf16_*):
+for random state pairs and normals, the entropy-stable solvers (Riemann 4, 33, 44) and the
+EC-KEP + LLF solver (16) dissipate entropy, [v].F <= [psi] with psi = rho v.n (Tadmor's
+condition holds with equality for the EC fluxes 9 and 32), and for equal states every solver
+returns the physical Euler flux."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+KAPPA = 1.4
+
+
+def random_state(rng):
+    rho, p = rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)
+    v = rng.uniform(-1, 1, 3)
+    return np.array([rho, *(rho * v), p / (KAPPA - 1) + 0.5 * rho * v @ v])
+
+
+def frame(rng):
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    a = rng.normal(size=3)
+    t1 = a - (a @ n) * n
+    t1 /= np.linalg.norm(t1)
+    return n, t1, np.cross(n, t1)
+
+
+def entropy_vars(U):
+    w = np.zeros(5)
+    O.lib().ora_cons_to_entropy(KAPPA, np.ascontiguousarray(U), w)
+    return w
+
+
+def flux(rs, UL, UR, n, t1, t2, lam=None):
+    F = np.zeros(5)
+    L = O.lib()
+    if lam is None:
+        L.ora_riemann(rs, KAPPA, UL, UR, n, t1, t2, F)
+    else:
+        L.ora_riemann_lam(rs, KAPPA, UL, UR, n, t1, t2, lam, F)
+    return F
+
+
+@pytest.mark.parametrize("rs", [4, 9, 16, 32, 33, 44])
+def test_entropy_condition_and_consistency(rs):
+    rng = np.random.default_rng(rs)
+    for _ in range(30):
+        UL, UR = random_state(rng), random_state(rng)
+        n, t1, t2 = frame(rng)
+        F = flux(rs, UL, UR, n, t1, t2, lam=2.5 if rs == 16 else None)
+        dv = entropy_vars(UR) - entropy_vars(UL)
+        dpsi = UR[1:4] @ n - UL[1:4] @ n
+        prod = dv @ F - dpsi
+        if rs in (9, 32):
+            assert abs(prod) <= 1e-12 * (1 + abs(dpsi))
+        else:
+            assert prod < -1e-8
+        # equal states: the physical flux
+        Fe = flux(rs, UL, UL, n, t1, t2, lam=2.5 if rs == 16 else None)
+        v = UL[1:4] / UL[0]
+        p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
+        vn = v @ n
+        Fp = np.array([UL[0] * vn, *(UL[1:4] * vn + p * n), (UL[4] + p) * vn])
+        assert np.allclose(Fe, Fp, rtol=1e-13, atol=1e-13)
