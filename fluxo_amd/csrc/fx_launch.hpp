@@ -360,6 +360,7 @@ void launch_volume_n(const Batch &b, int mode, RKArgs rk)
 {
   switch (b.cfg.volumeFlux) {
   case 5: launch_volume_v<N, 5>(b, mode, rk); break;
+  case 10: launch_volume_v<N, 10>(b, mode, rk); break;
   case 32: launch_volume_v<N, 32>(b, mode, rk); break;
   default: launch_volume_v<N, 0>(b, mode, rk); break;
   }
@@ -664,8 +665,8 @@ inline int check_connectivity(const int *ElemToSide, const int *SideToElem, int 
 // Physics options shared by contexts and servers.
 inline int check_physics(int volumeFlux, int riemann, int precision, double kappa, double Pr, int parabolic)
 {
-  if (volumeFlux != 0 && volumeFlux != 5 && volumeFlux != 32)
-    return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0,5,32)", volumeFlux);
+  if (volumeFlux != 0 && volumeFlux != 5 && volumeFlux != 10 && volumeFlux != 32)
+    return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0,5,10,32)", volumeFlux);
   if (riemann != 0 && riemann != 1 && riemann != 4 && riemann != 9 && riemann != 16 && riemann != 32 &&
       riemann != 33 && riemann != 44)
     return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4,9,16,32,33,44)", riemann);

@@ -92,14 +92,16 @@ __device__ __forceinline__ float inv_ln_mean(float a, float b, float la, float l
 // Per-node quantities staged for the two-point fluxes.
 //   rho, v1, v2, v3, p, ln(rho), X, ln(X), |v|^2, rhoE
 // X = rho/p for Ranocha (RanochaFluxVec), X = beta = rho/(2p) for Chandrashekar
-// (EntropyAndEnergyConservingFluxVec); EvalUaux (flux_average.f90:305-332).
+// (EntropyAndEnergyConservingFluxVec); EvalUaux (flux_average.f90:305-332).  For the
+// Ismail-Roe flux (VF=10) the slots RHO/LNRHO/X/LNX hold z1 = sqrt(rho/p), ln z1, z5 = z1 p,
+// ln z5 (TwoPointEntropyConservingFluxVec flux_average.f90:574-673).
 enum { NF_RHO = 0, NF_V1, NF_V2, NF_V3, NF_P, NF_LNRHO, NF_X, NF_LNX, NF_VV, NF_E, NF_COUNT };
 
 // Fields a volume flux reads (the others are never staged).
 template <int VF>
 __host__ __device__ constexpr bool uses_field(int f)
 {
-  return VF == 32 ? (f <= NF_LNX) : (VF == 5 ? (f <= NF_VV) : (f <= NF_P || f == NF_E));
+  return (VF == 32 || VF == 10) ? (f <= NF_LNX) : (VF == 5 ? (f <= NF_VV) : (f <= NF_P || f == NF_E));
 }
 template <int VF>
 __host__ __device__ constexpr int n_fields()
@@ -132,6 +134,9 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
   } else if (VF == 5) {
     double X = 0.5 * u[0] / p;
     f[NF_LNRHO] = log(u[0]); f[NF_X] = X; f[NF_LNX] = log(X);
+  } else if (VF == 10) {
+    double z1 = sqrt(u[0] / p), z5 = z1 * p;
+    f[NF_RHO] = z1; f[NF_LNRHO] = log(z1); f[NF_X] = z5; f[NF_LNX] = log(z5);
   } else {
     f[NF_LNRHO] = 0.0; f[NF_X] = 0.0; f[NF_LNX] = 0.0;
   }
@@ -140,6 +145,7 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
 // Metric-dealiased two-point volume flux Fstar(U_L, U_R, {{m}}).
 //  VF=32: RanochaFluxVec (flux_average.f90:1876-1929)
 //  VF=5 : EntropyAndEnergyConservingFluxVec (flux_average.f90:1277-1356)
+//  VF=10: TwoPointEntropyConservingFluxVec, Ismail-Roe (flux_average.f90:574-673)
 //  VF=0 : StandardDGFluxVec (flux_average.f90:394-439)
 template <int VF, typename R>
 __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL, const R *mR,
@@ -178,6 +184,23 @@ __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL
     F[3] = F[0] * wH + m2 * p1;
     F[4] = F[0] * R(0.5) * (skappaM1 * sbeta_hat - R(0.5) * (L[NF_VV] + Rt[NF_VV])) + uH * F[1] +
            vH * F[2] + wH * F[3];
+  } else if (VF == 10) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    const R z1L = L[NF_RHO], z1R = Rt[NF_RHO];
+    R z1M = R(0.5) * (z1L + z1R);
+    R sz1LN = inv_ln_mean(z1L, z1R, L[NF_LNRHO], Rt[NF_LNRHO]);
+    R z5LN = ln_mean(L[NF_X], Rt[NF_X], L[NF_LNX], Rt[NF_LNX]);
+    R sz1 = R(1) / z1M;
+    R uH = R(0.5) * (z1L * L[NF_V1] + z1R * Rt[NF_V1]) * sz1;
+    R vH = R(0.5) * (z1L * L[NF_V2] + z1R * Rt[NF_V2]) * sz1;
+    R wH = R(0.5) * (z1L * L[NF_V3] + z1R * Rt[NF_V3]) * sz1;
+    R p1 = R(0.5) * (L[NF_X] + Rt[NF_X]) * sz1;
+    R qH = uH * m0 + vH * m1 + wH * m2;
+    F[0] = z1M * z5LN * qH;
+    F[1] = F[0] * uH + m0 * p1;
+    F[2] = F[0] * vH + m1 * p1;
+    F[3] = F[0] * wH + m2 * p1;
+    F[4] = R(0.5) * (qH * (R(1) + R(2) * skappaM1) * (z5LN * sz1LN) + uH * F[1] + vH * F[2] + wH * F[3]);
   } else {
     R qL = L[NF_V1] * mL[0] + L[NF_V2] * mL[1] + L[NF_V3] * mL[2];
     R qR = Rt[NF_V1] * mR[0] + Rt[NF_V2] * mR[1] + Rt[NF_V3] * mR[2];

@@ -40,7 +40,8 @@ typedef struct {
   int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) or 9 (ideal GLM-MHD) */
   int nElems;       /* local elements */
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
-  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:134-148): 0, 5, 32; MHD 10 */
+  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:414-460): 0 standard DG, 5
+                       Chandrashekar, 10 Ismail-Roe, 32 Ranocha; MHD 10 (Derigs) */
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:291-404): 0 central, 1 LLF,
                        9 / 32 EC (Chandrashekar / Ranocha), 4 / 44 / 33 entropy stable
                        (Ismail-Roe / Chandrashekar / Ranocha + matrix dissipation),

@@ -64,7 +64,8 @@ int  ora_calc_metrics(int N, int Ngeo, int nElems, int nSides, const double *Nod
 /* ---- equation options (ints and reals), mirrors parameter.ini keys ---- */
 typedef struct {
   int N, nVar, nElems, nSides;
-  int volflux;       /* VolumeFlux: 0 standard DG, 5 Chandrashekar EC-KEP, 32 Ranocha EC-KEP */
+  int volflux;       /* VolumeFlux: 0 standard DG, 5 Chandrashekar EC-KEP, 10 Ismail-Roe EC,
+                        32 Ranocha EC-KEP */
   int riemann;       /* Riemann: 1 LLF (Rusanov), 5 EC Chandrashekar surface, 32 EC Ranocha surface */
   int parabolic;     /* 0 Euler, 1 Navier-Stokes */
   int lifting;       /* 1 BR1, 2 BR2 */

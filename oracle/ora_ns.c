@@ -89,11 +89,35 @@ static void flux_ranocha(double kappa, const double *UL, const double *UR, const
   F[4] = F[0] * (v2zip + 1.0 / rspM * skm1) + 0.5 * (aL[4] * vmR + aR[4] * vmL);
 }
 
+/* TwoPointEntropyConservingFluxVec, Ismail-Roe (flux_average.f90:574-673) */
+static void flux_ismail_roe(double kappa, const double *UL, const double *UR, const double *aL,
+                            const double *aR, const double *mL, const double *mR, double *F)
+{
+  const double skm1 = 1.0 / (kappa - 1.0), kp1 = kappa + 1.0;
+  double m[3] = {0.5 * (mL[0] + mR[0]), 0.5 * (mL[1] + mR[1]), 0.5 * (mL[2] + mR[2])};
+  const double z1L = sqrt(UL[0] / aL[4]), z1R = sqrt(UR[0] / aR[4]);
+  const double z1M = 0.5 * (z1L + z1R), z1LN = ora_ln_mean(z1L, z1R);
+  const double z2M = 0.5 * (z1L * aL[1] + z1R * aR[1]);
+  const double z3M = 0.5 * (z1L * aL[2] + z1R * aR[2]);
+  const double z4M = 0.5 * (z1L * aL[3] + z1R * aR[3]);
+  const double z5L = z1L * aL[4], z5R = z1R * aR[4];
+  const double z5M = 0.5 * (z5L + z5R), z5LN = ora_ln_mean(z5L, z5R);
+  const double rhoHat = z1M * z5LN, sz1 = 1.0 / z1M;
+  const double uH = z2M * sz1, vH = z3M * sz1, wH = z4M * sz1, p1 = z5M * sz1;
+  const double qH = uH * m[0] + vH * m[1] + wH * m[2];
+  F[0] = rhoHat * qH;
+  F[1] = F[0] * uH + m[0] * p1;
+  F[2] = F[0] * vH + m[1] * p1;
+  F[3] = F[0] * wH + m[2] * p1;
+  F[4] = 0.5 * (qH * kp1 * skm1 * (z5LN / z1LN) + uH * F[1] + vH * F[2] + wH * F[3]);
+}
+
 void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
                  const double *auxR, const double *mL, const double *mR, double *F)
 {
   switch (which) {
   case 5: flux_chandrashekar(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 10: flux_ismail_roe(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
   case 32: flux_ranocha(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
   default: flux_standard(UL, UR, auxL, auxR, mL, mR, F); break;
   }

@@ -59,6 +59,9 @@ NS="-DPP_nVar=5 -Dnavierstokes -DPP_entropy_vars_exist -DPP_u_aux_exist"
 NOSC="-DSHOCKCAPTURE=0 -DSHOCK_NFVSE=0"
 build_variant euler_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=0
 build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
+# Ismail-Roe two-point flux (VolumeFlux=10)
+build_variant euler_vf10 $NS $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
+build_variant ns_br1_vf10 $NS $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
 build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
 # BR2 lifting (FLUXO_PARABOLIC_LIFTING=br2, CMakeLists.txt:278-283)
 build_variant ns_br2_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=2 -DPP_Lifting_Var=3 -DPP_VISC=0

@@ -33,6 +33,10 @@ CASES = [
     ("ns_n3_tgv_br1_warped", 3, 3, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 0.0, 5),
     ("ns_n4_tgv_br1_warped", 2, 4, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 0.0, 3),
     ("euler_c1_density_wave_100steps", 8, 3, 1, False, (0, 0, 0), (2, 2, 2), 32, 1, False, 2, 0.0, 100),
+    # Ismail-Roe volume flux (VolumeFlux=10), with LLF and with the Ismail-Roe ES surface flux
+    ("euler_n3_ismailroe_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 10, 1, False, 2, 0.05, 3),
+    ("euler_n4_ismailroe_es4_warped", 2, 4, 3, True, (0, 0, 0), (2, 2, 2), 10, 4, False, 2, 0.05, 3),
+    ("ns_n3_tgv_br1_ismailroe_warped", 3, 3, 3, True, (-PI,) * 3, (2 * PI,) * 3, 10, 1, True, 7, 0.01, 3),
     # entropy-stable surface fluxes (Riemann 4 / 44 / 33: Ismail-Roe / Chandrashekar / Ranocha
     # + full-wave matrix dissipation) and EC-KEP + face LLF (16)
     ("euler_n3_ranocha_es33_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 32, 33, False, 2, 0.05, 3),

@@ -1,8 +1,9 @@
-"""Surface fluxes on the GPU against the oracle: the entropy-stable solvers (Riemann 4, 33,
-44, in the Cartesian frame) and EC-KEP + face LLF (16, a face-wide maximum wave speed), on
-warped meshes with perturbed states, N = 2..5 (relative L2 <= 1e-12 per RHS), Navier-Stokes
-with 33, and a 10-step trajectory (<= 1e-10).  The reference's own outputs are checked in
-test_gpu_golden (euler_*_es*, *_eckep_llf16_*, ns_*_es33_*)."""
+"""Surface fluxes and the Ismail-Roe volume flux on the GPU against the oracle: the
+entropy-stable solvers (Riemann 4, 33, 44, in the Cartesian frame) and EC-KEP + face LLF (16,
+a face-wide maximum wave speed), on warped meshes with perturbed states, N = 2..5 (relative
+L2 <= 1e-12 per RHS), Navier-Stokes with 33, a 10-step trajectory (<= 1e-10), and VolumeFlux
+10.  The reference's own outputs are checked in test_gpu_golden (euler_*_es*,
+*_eckep_llf16_*, ns_*_es33_*, *_ismailroe_*)."""
 import math
 
 import numpy as np
@@ -65,3 +66,25 @@ def test_navier_stokes_es33():
     err = rel_l2(e.rhs(), O.Problem(m, volflux=32, riemann=33, **kw).rhs(U))
     print(f"NS Riemann 33: RHS relL2 {err:.3e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_ismail_roe_volume_flux(N):
+    """VolumeFlux 10 (Ismail-Roe two-point flux) with LLF and with the Ismail-Roe ES surface
+    flux (Riemann 4) against the oracle; the FP32-split variant's error stated at N=3."""
+    from fluxo_amd.native import Engine, FX_PREC_FP32_SPLIT
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True)
+    U = state(m)
+    for rs in (1, 4):
+        e = Engine(m, volume_flux=10, riemann=rs)
+        e.upload(U)
+        ref = O.Problem(m, volflux=10, riemann=rs).rhs(U)
+        err = rel_l2(e.rhs(), ref)
+        print(f"VolumeFlux 10 Riemann {rs} N={N}: RHS relL2 {err:.3e}")
+        assert err <= 1e-12
+    if N == 3:
+        e = Engine(m, volume_flux=10, riemann=1, precision=FX_PREC_FP32_SPLIT)
+        e.upload(U)
+        err = rel_l2(e.rhs(), O.Problem(m, volflux=10, riemann=1).rhs(U))
+        print(f"VolumeFlux 10 FP32-split N=3: RHS relL2 {err:.3e}")
+        assert err <= 1e-4

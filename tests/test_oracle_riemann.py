@@ -63,3 +63,24 @@ def test_entropy_condition_and_consistency(rs):
         vn = v @ n
         Fp = np.array([UL[0] * vn, *(UL[1:4] * vn + p * n), (UL[4] + p) * vn])
         assert np.allclose(Fe, Fp, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("vf", [5, 10, 32])
+def test_volume_flux_tadmor_and_consistency(vf):
+    """Two-point volume fluxes (Chandrashekar 5, Ismail-Roe 10, Ranocha 32) with a metric
+    m (the same on both sides): Tadmor's condition [v].F#(m) = [rho v.m], symmetry, and
+    F#(U,U) = f(U).m."""
+    rng = np.random.default_rng(vf)
+    for _ in range(30):
+        UL, UR = random_state(rng), random_state(rng)
+        m = rng.uniform(-1, 1, 3)
+        F, Fs = O.volflux(vf, UL, UR, m, m, KAPPA), O.volflux(vf, UR, UL, m, m, KAPPA)
+        dv = entropy_vars(UR) - entropy_vars(UL)
+        dpsi = UR[1:4] @ m - UL[1:4] @ m
+        assert abs(dv @ F - dpsi) <= 1e-12 * (1 + abs(dpsi))
+        assert np.allclose(F, Fs, rtol=1e-13, atol=1e-13)
+        Fe = O.volflux(vf, UL, UL, m, m, KAPPA)
+        v = UL[1:4] / UL[0]
+        p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
+        vm = v @ m
+        assert np.allclose(Fe, [UL[0] * vm, *(UL[1:4] * vm + p * m), (UL[4] + p) * vm], rtol=1e-12, atol=1e-12)
