@@ -486,6 +486,51 @@ struct NsLift {
 // applied inside the volume integral and to every surface term, and the gradient each side
 // sees at its face is the node's volume-only gradient plus etaBR2 times that face's own
 // surface term (instead of the full gradient of BR1); the volume gradient is the same sum.
+// Node (i + n(j + n k)) at element-local face coordinates (a, b) of local side loc: xi sides
+// (a, b) = (j, k), eta sides (i, k), zeta sides (i, j).
+template <int N>
+__device__ __forceinline__ int face_node(int loc, int a, int b)
+{
+  constexpr int n = N + 1;
+  switch (loc) {
+  case XI_MINUS: return 0 + n * (a + n * b);
+  case XI_PLUS: return N + n * (a + n * b);
+  case ETA_MINUS: return a + n * (0 + n * b);
+  case ETA_PLUS: return a + n * (N + n * b);
+  case ZETA_MINUS: return a + n * (b + n * 0);
+  default: return a + n * (b + n * N);
+  }
+}
+
+// The faces of node (i,j,k) in the reference's order of SurfInt (ZETA_MINUS, ETA_MINUS,
+// XI_PLUS, ETA_PLUS, XI_MINUS, ZETA_PLUS; at most 3), each as the face-point slot
+// (locSide - 1) n^2 + a + n b of the element, -1 where there is none.  Kept in scalars (a
+// dynamically indexed array would live in scratch memory).
+struct NodeFaces {
+  int r0 = -1, r1 = -1, r2 = -1;
+  __device__ __forceinline__ void push(int r)
+  {
+    if (r0 < 0) r0 = r;
+    else if (r1 < 0) r1 = r;
+    else r2 = r;
+  }
+  __device__ __forceinline__ int get(int t) const { return t == 0 ? r0 : (t == 1 ? r1 : r2); }
+};
+
+template <int N>
+__device__ __forceinline__ NodeFaces node_faces(int i, int j, int k)
+{
+  constexpr int n = N + 1, n2 = n * n;
+  NodeFaces f;
+  if (k == 0) f.push((ZETA_MINUS - 1) * n2 + i + n * j);
+  if (j == 0) f.push((ETA_MINUS - 1) * n2 + i + n * k);
+  if (i == N) f.push((XI_PLUS - 1) * n2 + j + n * k);
+  if (j == N) f.push((ETA_PLUS - 1) * n2 + i + n * k);
+  if (i == 0) f.push((XI_MINUS - 1) * n2 + j + n * k);
+  if (k == N) f.push((ZETA_PLUS - 1) * n2 + i + n * j);
+  return f;
+}
+
 template <int N, int OUT, class EQ, int LIFT = 1>
 __global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
@@ -495,12 +540,15 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
           int nElems, EQ eq, double etaBR2)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
-  constexpr int EPB = LiftGeom<N>::EPB;
-  __shared__ double s_w[EPB][NV][n3];
-  // viscous form: the node's 12 Cartesian flux values, then (same slots, same thread) its
-  // transformed fluxes f~ (NVV), g~ (NVV), h~ (NVV)
+  constexpr int EPB = LiftGeom<N>::EPB, NF = 6 * n2;  // face points per element
+  // per element: the face data of phase F ((NV + 3) per face point: the BR1 lifting flux
+  // 1/2 (w_slave - w_master) SurfElem and the normal), later (same memory, after a barrier)
+  // the node's 12 Cartesian viscous flux values, then its transformed fluxes (3 NVV)
   constexpr int FTW = 3 * NVV > 12 ? 3 * NVV : 12;
-  __shared__ double s_ft[OUT == 1 ? EPB : 1][FTW][OUT == 1 ? n3 : 1];
+  constexpr int SLOT = (NV + 3) * NF > FTW * n3 ? (NV + 3) * NF : FTW * n3;
+  __shared__ double s_w[EPB][NV][n3];
+  __shared__ double s_buf[EPB][SLOT];
+  __shared__ int s_fpt[EPB][NF];  // 2 * face point ((s n + q) n + p) + (0 master, 1 slave)
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
   const int el = tid / n3, node = tid - el * n3;
@@ -509,8 +557,6 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   const int i = node % n, j = (node / n) % n, k = node / n2;
   const size_t c = (size_t)e * n3 + node;
   double u[NV], w[NV];
-  int nl = 0, fpt[3] = {0, 0, 0};  // faces of this node: face point (s n + q) n + p, master?
-  bool fmaster[3] = {false, false, false};
   if (active) {
     const double *up = U + c * NV;
 #pragma unroll
@@ -520,6 +566,46 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     for (int v = 0; v < NV; v++) s_w[el][v][node] = w[v];
   }
   __syncthreads();
+  // phase F: one work item per face point of the block's elements (the element-local
+  // coordinates keep a wavefront on few sides): neighbour state, its entropy variables and
+  // the strong lifting flux, Lifting_FillFlux (br1_fillflux.f90:49-91)
+  for (int it = tid; it < EPB * NF; it += 256) {
+    const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
+    const int ee = blockIdx.x * EPB + fe;
+    if (ee >= nElems || elem_problem(eprob, ee) < 0) continue;
+    const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+    const int fi = fnode % n, fj = (fnode / n) % n, fk = fnode / n2;
+    const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+    int p, q;
+    vol_to_side<N>(loc, fl, fi, fj, fk, p, q);
+    int enb, vnb;
+    if (fl == 0) {
+      enb = s2e[5 * sd + 1] - 1;
+      vnb = side_to_vol<N>(s2e[5 * sd + 3], s2e[5 * sd + 4], p, q);
+    } else {
+      enb = s2e[5 * sd] - 1;
+      vnb = side_to_vol<N>(s2e[5 * sd + 2], 0, p, q);
+    }
+    const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
+    double unb[NV], wnb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) unb[v] = unbp[v];
+    eq.entropy(unb, wnb);
+    const int fpt = (sd * n + q) * n + p;
+    const double4 g = nvse[fpt];
+    double *d = &s_buf[fe][r * (NV + 3)];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double wo = s_w[fe][v][fnode];
+      const double ws = (fl == 0) ? wnb[v] : wo;
+      const double wm = (fl == 0) ? wo : wnb[v];
+      d[v] = 0.5 * (ws - wm) * g.w;
+    }
+    d[NV] = g.x; d[NV + 1] = g.y; d[NV + 2] = g.z;
+    s_fpt[fe][r] = 2 * fpt + (fl == 0 ? 0 : 1);
+  }
+  __syncthreads();
+  double gx[NV], gy[NV], gz[NV];
   if (active) {
     double gxi[NV], geta[NV], gzeta[NV];
 #pragma unroll
@@ -538,15 +624,12 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
         gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
       }
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-    const double mf0 = mf[0], mf1 = mf[1], mf2 = mf[2], mg0 = mg[0], mg1 = mg[1], mg2 = mg[2];
-    const double mh0 = mh[0], mh1 = mh[1], mh2 = mh[2];
     const double sj = sJ[c];
-    double gx[NV], gy[NV], gz[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      gx[v] = mf0 * gxi[v] + mg0 * geta[v] + mh0 * gzeta[v];
-      gy[v] = mf1 * gxi[v] + mg1 * geta[v] + mh1 * gzeta[v];
-      gz[v] = mf2 * gxi[v] + mg2 * geta[v] + mh2 * gzeta[v];
+      gx[v] = mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v];
+      gy[v] = mf[1] * gxi[v] + mg[1] * geta[v] + mh[1] * gzeta[v];
+      gz[v] = mf[2] * gxi[v] + mg[2] * geta[v] + mh[2] * gzeta[v];
       if (LIFT == 2) { gx[v] = gx[v] * sj; gy[v] = gy[v] * sj; gz[v] = gz[v] * sj; }
     }
     // BR2: the volume-only gradient, the base of every face gradient (in gxi.. storage)
@@ -554,45 +637,22 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 #pragma unroll
       for (int v = 0; v < NV; v++) { gxi[v] = gx[v]; geta[v] = gy[v]; gzeta[v] = gz[v]; }
     }
-    // surface terms: +1/2 (w_slave - w_master) SurfElem n_master L_HatMinus0 on both sides
-    int locs[3];
-    if (k == 0) locs[nl++] = ZETA_MINUS;
-    if (j == 0) locs[nl++] = ETA_MINUS;
-    if (i == N) locs[nl++] = XI_PLUS;
-    if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
-    if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
-    if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
+    // Lifting_SurfInt: + flux n L_HatMinus0 of every face of the node (br1_surfint.f90:52-131)
+    const NodeFaces nf = node_faces<N>(i, j, k);
     const double L0 = op.L0;
-    for (int t = 0; t < nl; t++) {
-      const int loc = locs[t];
-      const int s = e2s[(e * 6 + loc - 1) * 2] - 1, fl = e2s[(e * 6 + loc - 1) * 2 + 1];
-      int p, q;
-      vol_to_side<N>(loc, fl, i, j, k, p, q);
-      int enb, vnb;
-      if (fl == 0) {
-        enb = s2e[5 * s + 1] - 1;
-        vnb = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
-      } else {
-        enb = s2e[5 * s] - 1;
-        vnb = side_to_vol<N>(s2e[5 * s + 2], 0, p, q);
-      }
-      const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
-      double unb[NV], wnb[NV];
 #pragma unroll
-      for (int v = 0; v < NV; v++) unb[v] = unbp[v];
-      eq.entropy(unb, wnb);
-      fpt[t] = (s * n + q) * n + p;
-      fmaster[t] = (fl == 0);
-      const double4 g = nvse[fpt[t]];
+    for (int t = 0; t < 3; t++) {
+      const int r = nf.get(t);
+      if (r < 0) continue;
+      const double *d = &s_buf[el][r * (NV + 3)];
+      const double nx = d[NV], ny = d[NV + 1], nz = d[NV + 2];
       if (LIFT == 1) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          const double ws = (fl == 0) ? wnb[v] : w[v];
-          const double wm = (fl == 0) ? w[v] : wnb[v];
-          const double f = 0.5 * (ws - wm) * g.w;
-          gx[v] = gx[v] + (f * g.x) * L0;
-          gy[v] = gy[v] + (f * g.y) * L0;
-          gz[v] = gz[v] + (f * g.z) * L0;
+          const double f = d[v];
+          gx[v] = gx[v] + (f * nx) * L0;
+          gy[v] = gy[v] + (f * ny) * L0;
+          gz[v] = gz[v] + (f * nz) * L0;
         }
       } else {
         // F_loc = sJ L_HatMinus0 Flux: into the volume gradient, and etaBR2 F_loc on the
@@ -601,10 +661,8 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
         double fx[NV], fy[NV], fz[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          const double ws = (fl == 0) ? wnb[v] : w[v];
-          const double wm = (fl == 0) ? w[v] : wnb[v];
-          const double f = 0.5 * (ws - wm) * g.w;
-          const double lx = sl * (f * g.x), ly = sl * (f * g.y), lz = sl * (f * g.z);
+          const double f = d[v];
+          const double lx = sl * (f * nx), ly = sl * (f * ny), lz = sl * (f * nz);
           gx[v] = gx[v] + lx;
           gy[v] = gy[v] + ly;
           gz[v] = gz[v] + lz;
@@ -618,8 +676,8 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
           eq.grad_prim(u, fy);
           eq.grad_prim(u, fz);
           eq.fluxes(u, fx, fy, fz, fv);
-          EQ::vis_dir(fv, g.x, g.y, g.z, o);
-          double *dst = out1 + ((size_t)fpt[t] * 2 + (fl == 0 ? 0 : 1)) * NVV;
+          EQ::vis_dir(fv, nx, ny, nz, o);
+          double *dst = out1 + (size_t)s_fpt[el][r] * NVV;
 #pragma unroll
           for (int v = 0; v < NVV; v++) dst[v] = o[v];
         }
@@ -635,58 +693,66 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     if (OUT == 0) {
 #pragma unroll
       for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
-    } else {
-      // the 12 Cartesian viscous flux values of this node into its own LDS slots (the
-      // gradients die here, which keeps the register count of the first part)
-      double fv[12];
-      eq.fluxes(u, gx, gy, gz, fv);
-#pragma unroll
-      for (int m = 0; m < 12; m++) s_ft[el][m][node] = fv[m];
     }
   }
-  if (OUT == 1) __syncthreads();
-  if (OUT == 1 && active) {
+  if (OUT == 0) return;
+  __syncthreads();  // the face data is no longer read; its memory takes the fluxes
+  if (active) {
+    // the 12 Cartesian viscous flux values of this node into its own slots
+    double fv[12];
+    eq.fluxes(u, gx, gy, gz, fv);
+#pragma unroll
+    for (int m = 0; m < 12; m++) s_buf[el][m * n3 + node] = fv[m];
+  }
+  __syncthreads();
+  if (active) {
     double fv[12], o[NVV];
 #pragma unroll
-    for (int m = 0; m < 12; m++) fv[m] = s_ft[el][m][node];
+    for (int m = 0; m < 12; m++) fv[m] = s_buf[el][m * n3 + node];
     // face-normal viscous flux of this element at each of its face points (BR1: from the
     // node's full gradient; BR2 wrote them from the face gradients above)
-    for (int t = 0; t < (LIFT == 1 ? nl : 0); t++) {
-      const double4 g = nvse[fpt[t]];
-      EQ::vis_dir(fv, g.x, g.y, g.z, o);
-      double *dst = out1 + ((size_t)fpt[t] * 2 + (fmaster[t] ? 0 : 1)) * NVV;
+    if (LIFT == 1) {
+      const NodeFaces nf = node_faces<N>(i, j, k);
 #pragma unroll
-      for (int v = 0; v < NVV; v++) dst[v] = o[v];
+      for (int t = 0; t < 3; t++) {
+        const int r = nf.get(t);
+        if (r < 0) continue;
+        const int code = s_fpt[el][r];
+        const double4 g = nvse[code >> 1];
+        EQ::vis_dir(fv, g.x, g.y, g.z, o);
+        double *dst = out1 + (size_t)code * NVV;
+#pragma unroll
+        for (int v = 0; v < NVV; v++) dst[v] = o[v];
+      }
     }
     // transformed fluxes f~, g~, h~ (EvalDiffFluxTilde3D) over the node's own slots
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
     EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_ft[el][v][node] = o[v];
+    for (int v = 0; v < NVV; v++) s_buf[el][v * n3 + node] = o[v];
     EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_ft[el][NVV + v][node] = o[v];
+    for (int v = 0; v < NVV; v++) s_buf[el][(NVV + v) * n3 + node] = o[v];
     EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_ft[el][2 * NVV + v][node] = o[v];
+    for (int v = 0; v < NVV; v++) s_buf[el][(2 * NVV + v) * n3 + node] = o[v];
   }
-  if (OUT == 1) {
-    __syncthreads();
-    if (active) {
-      double uv[NVV];
+  __syncthreads();
+  if (active) {
+    double uv[NVV];
 #pragma unroll
-      for (int v = 0; v < NVV; v++) uv[v] = 0.0;
+    for (int v = 0; v < NVV; v++) uv[v] = 0.0;
 #pragma unroll 1
-      for (int l = 0; l < n; l++) {
-        const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
+    for (int l = 0; l < n; l++) {
+      const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
 #pragma unroll
-        for (int v = 0; v < NVV; v++)
-          uv[v] = uv[v] + di * s_ft[el][v][l + n * (j + n * k)] + dj * s_ft[el][NVV + v][i + n * (l + n * k)] +
-                  dk * s_ft[el][2 * NVV + v][i + n * (j + n * l)];
-      }
-#pragma unroll
-      for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
+      for (int v = 0; v < NVV; v++)
+        uv[v] = uv[v] + di * s_buf[el][v * n3 + l + n * (j + n * k)] +
+                dj * s_buf[el][(NVV + v) * n3 + i + n * (l + n * k)] +
+                dk * s_buf[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
     }
+#pragma unroll
+    for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
   }
 }
 
