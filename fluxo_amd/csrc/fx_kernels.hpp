@@ -99,6 +99,51 @@ __device__ __forceinline__ void vol_to_side(int loc, int flip, int i, int j, int
   flip_pq(N, flip, p, q);
 }
 
+// Node (i + n(j + n k)) at element-local face coordinates (a, b) of local side loc: xi sides
+// (a, b) = (j, k), eta sides (i, k), zeta sides (i, j).
+template <int N>
+__device__ __forceinline__ int face_node(int loc, int a, int b)
+{
+  constexpr int n = N + 1;
+  switch (loc) {
+  case XI_MINUS: return 0 + n * (a + n * b);
+  case XI_PLUS: return N + n * (a + n * b);
+  case ETA_MINUS: return a + n * (0 + n * b);
+  case ETA_PLUS: return a + n * (N + n * b);
+  case ZETA_MINUS: return a + n * (b + n * 0);
+  default: return a + n * (b + n * N);
+  }
+}
+
+// The faces of node (i,j,k) in the reference's order of SurfInt (ZETA_MINUS, ETA_MINUS,
+// XI_PLUS, ETA_PLUS, XI_MINUS, ZETA_PLUS; at most 3), each as the face-point slot
+// (locSide - 1) n^2 + a + n b of the element, -1 where there is none.  Kept in scalars (a
+// dynamically indexed array would live in scratch memory).
+struct NodeFaces {
+  int r0 = -1, r1 = -1, r2 = -1;
+  __device__ __forceinline__ void push(int r)
+  {
+    if (r0 < 0) r0 = r;
+    else if (r1 < 0) r1 = r;
+    else r2 = r;
+  }
+  __device__ __forceinline__ int get(int t) const { return t == 0 ? r0 : (t == 1 ? r1 : r2); }
+};
+
+template <int N>
+__device__ __forceinline__ NodeFaces node_faces(int i, int j, int k)
+{
+  constexpr int n = N + 1, n2 = n * n;
+  NodeFaces f;
+  if (k == 0) f.push((ZETA_MINUS - 1) * n2 + i + n * j);
+  if (j == 0) f.push((ETA_MINUS - 1) * n2 + i + n * k);
+  if (i == N) f.push((XI_PLUS - 1) * n2 + j + n * k);
+  if (j == N) f.push((ETA_PLUS - 1) * n2 + i + n * k);
+  if (i == 0) f.push((XI_MINUS - 1) * n2 + j + n * k);
+  if (k == N) f.push((ZETA_PLUS - 1) * n2 + i + n * j);
+  return f;
+}
+
 // -----------------------------------------------------------------------------------
 // Launch geometry of the volume kernel: EPB elements per block and W = EPB*(N+1)^2
 // threads per direction rounded up to a multiple of 32 (3*W threads per block).  Tiles[0]
@@ -311,8 +356,30 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     __syncthreads();
   }
 
-  // phase 3b: one thread per node
+  // phase 3b: one work item per face point of the block's elements gathers the element's
+  // SurfInt term +-Flux L_HatMinus0 (master +, slave -; surfint.f90:177-184, 253-259) into
+  // LDS in element-local face coordinates, so that the node pass reads it without side and
+  // flip index arithmetic
   const double L0 = op.L0;
+  constexpr int NF = 6 * n2;
+  __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * 5 : 1];
+  if (MODE != VMODE_VOLINT) {
+    for (int it = tid; it < EPB * NF; it += 3 * W) {
+      const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
+      const int ee = e0 + fe;
+      if (ee >= nElems || elem_problem(rk.eprob, ee) < 0) continue;
+      const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+      const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+      int p, q;
+      vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
+      const double *F = Flux + ((size_t)(sd * n + q) * n + p) * 5;
+      const double sg = (fl == 0) ? L0 : -L0;
+#pragma unroll
+      for (int v = 0; v < 5; v++) s_face[fe][r * 5 + v] = F[v] * sg;
+    }
+    __syncthreads();
+  }
+  // phase 3c: one thread per node
   for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
     const int el2 = idx / n3, node = idx - el2 * n3;
     const int e2 = e0 + el2;
@@ -339,27 +406,14 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
       for (int c = 0; c < 4; c++) tot[c + 1] = tot[c + 1] + uv[c];
     }
     if (MODE != VMODE_VOLINT) {
-      // SurfInt gather: faces touched by node (i,j,k), in local side order
-      int locs[3], nl = 0;
-      if (k == 0) locs[nl++] = ZETA_MINUS;
-      if (j == 0) locs[nl++] = ETA_MINUS;
-      if (i == N) locs[nl++] = XI_PLUS;
-      if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
-      if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
-      if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
-      for (int t = 0; t < nl; t++) {
-        const int loc = locs[t];
-        const int s = e2s[(e2 * 6 + loc - 1) * 2] - 1, fl = e2s[(e2 * 6 + loc - 1) * 2 + 1];
-        int p, q;
-        vol_to_side<N>(loc, fl, i, j, k, p, q);
-        const double *F = Flux + ((size_t)(s * n + q) * n + p) * 5;
-        if (fl == 0) {
+      // SurfInt: the faces touched by node (i,j,k), in local side order
+      const NodeFaces nf = node_faces<N>(i, j, k);
 #pragma unroll
-          for (int v = 0; v < 5; v++) tot[v] = tot[v] + F[v] * L0;
-        } else {
+      for (int t = 0; t < 3; t++) {
+        const int r = nf.get(t);
+        if (r < 0) continue;
 #pragma unroll
-          for (int v = 0; v < 5; v++) tot[v] = tot[v] - F[v] * L0;
-        }
+        for (int v = 0; v < 5; v++) tot[v] = tot[v] + s_face[el2][r * 5 + v];
       }
       const double msj = -sJ[(size_t)e2 * n3 + node];
 #pragma unroll
@@ -486,51 +540,6 @@ struct NsLift {
 // applied inside the volume integral and to every surface term, and the gradient each side
 // sees at its face is the node's volume-only gradient plus etaBR2 times that face's own
 // surface term (instead of the full gradient of BR1); the volume gradient is the same sum.
-// Node (i + n(j + n k)) at element-local face coordinates (a, b) of local side loc: xi sides
-// (a, b) = (j, k), eta sides (i, k), zeta sides (i, j).
-template <int N>
-__device__ __forceinline__ int face_node(int loc, int a, int b)
-{
-  constexpr int n = N + 1;
-  switch (loc) {
-  case XI_MINUS: return 0 + n * (a + n * b);
-  case XI_PLUS: return N + n * (a + n * b);
-  case ETA_MINUS: return a + n * (0 + n * b);
-  case ETA_PLUS: return a + n * (N + n * b);
-  case ZETA_MINUS: return a + n * (b + n * 0);
-  default: return a + n * (b + n * N);
-  }
-}
-
-// The faces of node (i,j,k) in the reference's order of SurfInt (ZETA_MINUS, ETA_MINUS,
-// XI_PLUS, ETA_PLUS, XI_MINUS, ZETA_PLUS; at most 3), each as the face-point slot
-// (locSide - 1) n^2 + a + n b of the element, -1 where there is none.  Kept in scalars (a
-// dynamically indexed array would live in scratch memory).
-struct NodeFaces {
-  int r0 = -1, r1 = -1, r2 = -1;
-  __device__ __forceinline__ void push(int r)
-  {
-    if (r0 < 0) r0 = r;
-    else if (r1 < 0) r1 = r;
-    else r2 = r;
-  }
-  __device__ __forceinline__ int get(int t) const { return t == 0 ? r0 : (t == 1 ? r1 : r2); }
-};
-
-template <int N>
-__device__ __forceinline__ NodeFaces node_faces(int i, int j, int k)
-{
-  constexpr int n = N + 1, n2 = n * n;
-  NodeFaces f;
-  if (k == 0) f.push((ZETA_MINUS - 1) * n2 + i + n * j);
-  if (j == 0) f.push((ETA_MINUS - 1) * n2 + i + n * k);
-  if (i == N) f.push((XI_PLUS - 1) * n2 + j + n * k);
-  if (j == N) f.push((ETA_PLUS - 1) * n2 + i + n * k);
-  if (i == 0) f.push((XI_MINUS - 1) * n2 + j + n * k);
-  if (k == N) f.push((ZETA_PLUS - 1) * n2 + i + n * j);
-  return f;
-}
-
 template <int N, int OUT, class EQ, int LIFT = 1>
 __global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
