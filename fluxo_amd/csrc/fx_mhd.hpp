@@ -372,7 +372,28 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
     __syncthreads();
   }
 
+  // the element's SurfInt terms +FluxM L0 (master) / -FluxS L0 (slave) into LDS in
+  // element-local face coordinates, one work item per face point
   const double L0 = op.L0;
+  constexpr int NF = 6 * n2;
+  __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * MV : 1];
+  if (MODE != VMODE_VOLINT) {
+    for (int it = tid; it < EPB * NF; it += 3 * W) {
+      const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
+      const int ee = e0 + fe;
+      if (ee >= nElems || elem_problem(rk.eprob, ee) < 0) continue;
+      const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+      const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+      int p, q;
+      vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
+      const size_t f = ((size_t)(sd * n + q) * n + p) * MV;
+      const double *F = (fl == 0) ? FluxM + f : FluxS + f;
+      const double sg = (fl == 0) ? L0 : -L0;
+#pragma unroll
+      for (int v = 0; v < MV; v++) s_face[fe][r * MV + v] = F[v] * sg;
+    }
+    __syncthreads();
+  }
   for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
     const int el2 = idx / n3, node = idx - el2 * n3, e2 = e0 + el2;
     if (e2 >= nElems) continue;
@@ -388,26 +409,13 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
       for (int v = 1; v < 8; v++) tot[v] = tot[v] + uv[v - 1];
     }
     if (MODE != VMODE_VOLINT) {
-      int locs[3], nl = 0;
-      if (k == 0) locs[nl++] = ZETA_MINUS;
-      if (j == 0) locs[nl++] = ETA_MINUS;
-      if (i == N) locs[nl++] = XI_PLUS;
-      if (j == N && nl < 3) locs[nl++] = ETA_PLUS;
-      if (i == 0 && nl < 3) locs[nl++] = XI_MINUS;
-      if (k == N && nl < 3) locs[nl++] = ZETA_PLUS;
-      for (int t = 0; t < nl; t++) {
-        const int loc = locs[t];
-        const int s = e2s[(e2 * 6 + loc - 1) * 2] - 1, fl = e2s[(e2 * 6 + loc - 1) * 2 + 1];
-        int p, q;
-        vol_to_side<N>(loc, fl, i, j, k, p, q);
-        const size_t f = ((size_t)(s * n + q) * n + p) * MV;
-        if (fl == 0) {
+      const NodeFaces nf = node_faces<N>(i, j, k);
 #pragma unroll
-          for (int v = 0; v < MV; v++) tot[v] = tot[v] + FluxM[f + v] * L0;
-        } else {
+      for (int t = 0; t < 3; t++) {
+        const int r = nf.get(t);
+        if (r < 0) continue;
 #pragma unroll
-          for (int v = 0; v < MV; v++) tot[v] = tot[v] - FluxS[f + v] * L0;
-        }
+        for (int v = 0; v < MV; v++) tot[v] = tot[v] + s_face[el2][r * MV + v];
       }
       const double msj = -sJ[(size_t)e2 * n3 + node];
 #pragma unroll
