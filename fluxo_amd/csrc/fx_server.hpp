@@ -11,11 +11,12 @@
 // kernel skips.  A request never references another request's slots (its mesh is
 // periodic and self-contained).
 //
-// The worker thread is the only thread that calls HIP.  Each iteration it admits queued
-// requests (HBM budget, pool slots; pools grow by reallocation), launches one step graph
-// per pool with active requests (each pool on its own stream, so pools overlap), waits,
-// then counts steps, reads every active problem's DtState, and retires finished requests
-// (result copied to the caller's U_out, slots freed).
+// The worker thread is the only thread that calls HIP.  Each pool steps on its own stream,
+// independently of the others: when a pool's step completes (its event), the worker counts
+// the step, reads every active problem's DtState (pinned read-back queued behind the step),
+// retires finished requests (result copied to the caller's U_out, slots freed), admits queued
+// requests into it (HBM budget, pool slots; pools grow by reallocation) and launches its
+// next step graph.
 #pragma once
 #include <algorithm>
 #include <chrono>
@@ -93,7 +94,10 @@ struct Pool {
   hipGraphExec_t graph = nullptr;
   int gE = -1, gS = -1, gR = -1;  // extents the graph was captured with
   bool ptrs_changed = true;
-  std::vector<DtState> st_host;
+  DtState *st_host = nullptr;     // pinned: the per-problem step state read back after each step
+  int st_cap = 0;
+  hipEvent_t done = nullptr;      // recorded after a step's graph and state read-back
+  bool inflight = false;          // a step is queued on the stream and not yet seen complete
   std::vector<int> active_ids;
   long long bytes_alloc = 0;
 };
@@ -183,7 +187,13 @@ inline int pool_realloc(Pool &p, const fx_server_config &c, int Ecap, int Scap, 
   if (Ecap > (int)oE) p.ea.grow(Ecap);
   if (Scap > (int)oS) p.sa.grow(Scap);
   if (Rcap > (int)oR) p.ra.grow(Rcap);
-  p.st_host.resize(Rcap);
+  if (Rcap > p.st_cap) {
+    DtState *h = nullptr;
+    HIPCHK(hipHostMalloc((void **)&h, sizeof(DtState) * Rcap, hipHostMallocDefault));
+    if (p.st_host) (void)hipHostFree(p.st_host);
+    p.st_host = h;
+    p.st_cap = Rcap;
+  }
   p.bytes_alloc = bytes;
   p.ptrs_changed = true;
   return FX_OK;
@@ -200,6 +210,9 @@ inline void pool_free(Pool &p)
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (b.stream) (void)hipStreamDestroy(b.stream);
+  if (p.st_host) (void)hipHostFree(p.st_host);
+  if (p.done) (void)hipEventDestroy(p.done);
+  p.st_host = nullptr; p.st_cap = 0; p.done = nullptr; p.inflight = false;
   b = Batch();
 }
 
@@ -217,6 +230,7 @@ inline int pool_init(Pool &p, const fx_server_config &c, int N, int par)
   int rc = ensure_ops(N);
   if (rc) return rc;
   HIPCHK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
   return FX_OK;
 }
 
@@ -350,24 +364,88 @@ inline void finish(fx_server *s, Request &q, int state)
   s->nActive--;
 }
 
+// Bookkeeping after a pool's step has completed (called with the server lock held; the lock
+// is released around device copies).  Requests that reached nSteps or failed are finished.
+inline void complete_step(fx_server *s, Pool &p, std::unique_lock<std::mutex> &lk)
+{
+  s->steps_launched++;
+  for (int id : std::vector<int>(p.active_ids)) {
+    Request &q = s->reqs[id];
+    const DtState st = p.st_host[q.r];
+    q.steps++;
+    q.errType = st.errType;
+    q.t_end = st.t;
+    q.dt_last = st.dt;
+    if (q.steps < q.req.nSteps && st.errType == 0) continue;
+    lk.unlock();
+    int rc2 = FX_OK;
+    if (st.errType == 0 && q.req.U_out) {
+      hipError_t e = hipMemcpyAsync(q.req.U_out, p.b.U + (size_t)5 * p.b.n3 * q.e0,
+                                    (size_t)5 * p.b.n3 * q.req.nElems * 8, hipMemcpyDeviceToHost, p.b.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(p.b.stream);
+      if (e != hipSuccess) rc2 = set_err(FX_EHIP, "result copy: %s", hipGetErrorString(e));
+    }
+    int rc3 = release_request(p, q);
+    lk.lock();
+    finish(s, q, (st.errType == 0 && rc2 == FX_OK && rc3 == FX_OK) ? FX_REQ_DONE : FX_REQ_FAILED);
+  }
+}
+
+// A device failure fails every running request; the pools are drained and left as they are.
+inline void fail_running(fx_server *s, std::unique_lock<std::mutex> &lk)
+{
+  snprintf(s->worker_err, sizeof(s->worker_err), "%s", err_buf());
+  for (auto &p : s->pools) {
+    if (p.b.stream) (void)hipStreamSynchronize(p.b.stream);
+    p.inflight = false;
+    for (int id : std::vector<int>(p.active_ids)) {
+      Request &q = s->reqs[id];
+      lk.unlock();
+      (void)release_request(p, q);
+      lk.lock();
+      finish(s, q, FX_REQ_FAILED);
+    }
+  }
+  s->cv_done.notify_all();
+}
+
+// The scheduler.  Every pool (one per polynomial degree and equation) steps on its own stream
+// as soon as its previous step has completed, so a pool of small requests is not held back by
+// a pool of large ones.  Admission into a pool, request completion and graph re-capture
+// happen only between that pool's steps.
 inline void worker_loop(fx_server *s)
 {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    // admission in FIFO order; a request that does not fit the budget waits, later
-    // smaller ones may pass it.  Submissions may arrive while the lock is released.
+    bool progressed = false;
+    int rc = FX_OK;
+    for (auto &p : s->pools) {
+      if (!p.inflight) continue;
+      hipError_t e = hipEventQuery(p.done);
+      if (e == hipErrorNotReady) continue;
+      p.inflight = false;
+      progressed = true;
+      if (e != hipSuccess) { rc = set_err(FX_EHIP, "step: %s", hipGetErrorString(e)); break; }
+      complete_step(s, p, lk);
+    }
+    if (rc != FX_OK) { fail_running(s, lk); continue; }
+    if (progressed) s->cv_done.notify_all();
+    // admission in FIFO order; a request that does not fit the budget, or whose pool is
+    // between launch and completion of a step, waits; later ones may pass it.  Submissions
+    // may arrive while the lock is released.
     const std::vector<int> waiting(s->queue.begin(), s->queue.end());
     for (int qid : waiting) {
       Request &q = s->reqs[qid];
       if (s->bytes_in_use + q.bytes > s->cfg.hbmBudget) continue;
+      if (s->pools[2 * q.req.N + (q.req.parabolic ? 1 : 0)].inflight) continue;
       lk.unlock();
-      int rc = admit(s, q);
+      int rc2 = admit(s, q);
       lk.lock();
       s->queue.erase(std::find(s->queue.begin(), s->queue.end(), qid));
       long long a = 0;
       for (auto &p : s->pools) a += p.bytes_alloc;
       s->bytes_allocated = a;
-      if (rc != FX_OK) {
+      if (rc2 != FX_OK) {
         q.state = FX_REQ_FAILED; q.t_done = seconds_since(s); s->nFailed++;
         snprintf(s->worker_err, sizeof(s->worker_err), "%s", err_buf());
         s->cv_done.notify_all();
@@ -379,66 +457,34 @@ inline void worker_loop(fx_server *s)
       s->bytes_in_use += q.bytes;
       s->bytes_peak = std::max(s->bytes_peak, s->bytes_in_use);
     }
-    if (s->nActive == 0) {
+    if (s->nActive == 0) {  // no pool holds a request, so none is in flight
       if (s->stop) break;
       s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
       continue;
     }
-    if (s->stop) break;
+    if (s->stop) {
+      for (auto &p : s->pools)
+        if (p.inflight) { (void)hipStreamSynchronize(p.b.stream); p.inflight = false; }
+      break;
+    }
     lk.unlock();
-    // one step of every pool with active requests, pools overlapping on their streams
-    int rc = FX_OK;
+    // next step of every pool that holds requests and has no step outstanding
+    bool launched = false;
     for (auto &p : s->pools) {
-      if (p.nActive == 0) continue;
+      if (p.nActive == 0 || p.inflight) continue;
       if ((rc = pool_graph(s, p)) != FX_OK) break;
       hipError_t e = hipGraphLaunch(p.graph, p.b.stream);
       if (e == hipSuccess)
-        e = hipMemcpyAsync(p.st_host.data(), p.b.st, sizeof(DtState) * p.gR, hipMemcpyDeviceToHost, p.b.stream);
+        e = hipMemcpyAsync(p.st_host, p.b.st, sizeof(DtState) * p.gR, hipMemcpyDeviceToHost, p.b.stream);
+      if (e == hipSuccess) e = hipEventRecord(p.done, p.b.stream);
       if (e != hipSuccess) { rc = set_err(FX_EHIP, "step launch: %s", hipGetErrorString(e)); break; }
+      p.inflight = true;
+      launched = true;
     }
-    for (auto &p : s->pools)
-      if (p.nActive > 0) (void)hipStreamSynchronize(p.b.stream);
+    // nothing to do until a step completes: poll at a period well below one step
+    if (rc == FX_OK && !progressed && !launched) std::this_thread::sleep_for(std::chrono::microseconds(10));
     lk.lock();
-    if (rc != FX_OK) {
-      // a launch failure fails every running request; the pools are left as they are
-      snprintf(s->worker_err, sizeof(s->worker_err), "%s", err_buf());
-      for (auto &p : s->pools) {
-        for (int id : std::vector<int>(p.active_ids)) {
-          Request &q = s->reqs[id];
-          lk.unlock();
-          (void)release_request(p, q);
-          lk.lock();
-          finish(s, q, FX_REQ_FAILED);
-        }
-      }
-      s->cv_done.notify_all();
-      continue;
-    }
-    for (auto &p : s->pools) {
-      if (p.nActive == 0) continue;
-      s->steps_launched++;
-      for (int id : std::vector<int>(p.active_ids)) {
-        Request &q = s->reqs[id];
-        const DtState &st = p.st_host[q.r];
-        q.steps++;
-        q.errType = st.errType;
-        q.t_end = st.t;
-        q.dt_last = st.dt;
-        if (q.steps < q.req.nSteps && st.errType == 0) continue;
-        lk.unlock();
-        int rc2 = FX_OK;
-        if (st.errType == 0 && q.req.U_out) {
-          hipError_t e = hipMemcpyAsync(q.req.U_out, p.b.U + (size_t)5 * p.b.n3 * q.e0,
-                                        (size_t)5 * p.b.n3 * q.req.nElems * 8, hipMemcpyDeviceToHost, p.b.stream);
-          if (e == hipSuccess) e = hipStreamSynchronize(p.b.stream);
-          if (e != hipSuccess) rc2 = set_err(FX_EHIP, "result copy: %s", hipGetErrorString(e));
-        }
-        int rc3 = release_request(p, q);
-        lk.lock();
-        finish(s, q, (st.errType == 0 && rc2 == FX_OK && rc3 == FX_OK) ? FX_REQ_DONE : FX_REQ_FAILED);
-      }
-    }
-    s->cv_done.notify_all();
+    if (rc != FX_OK) fail_running(s, lk);
   }
   // stopping: whatever has not completed fails
   for (auto &q : s->reqs)
