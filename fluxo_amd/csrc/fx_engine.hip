@@ -51,8 +51,10 @@ int check_config(const fx_config *c)
   if (c->N < 2 || c->N > 5) return set_err(FX_EINVAL, "N=%d not supported (2..5)", c->N);
   if (c->nVar != 5 && c->nVar != 9) return set_err(FX_EINVAL, "nVar=%d not supported (5, 9 GLM-MHD)", c->nVar);
   if (c->nVar == 9) {
-    if (c->volumeFlux != 10 || c->riemann != 1)
-      return set_err(FX_EINVAL, "GLM-MHD: VolumeFlux=10 (Derigs) and Riemann=1 (Rusanov) only");
+    if (c->volumeFlux != 10)
+      return set_err(FX_EINVAL, "GLM-MHD: VolumeFlux=10 (Derigs) only");
+    if (c->riemann != 0 && c->riemann != 1 && c->riemann != 10 && c->riemann != 11)
+      return set_err(FX_EINVAL, "GLM-MHD: Riemann=%d not supported (0, 1, 10, 11)", c->riemann);
     if (c->nfvse) return set_err(FX_EINVAL, "GLM-MHD: no shock capturing");
     if (!(c->kappa > 1.0) || !(c->mu_0 > 0.0) || !(c->GLM_scale > 0.0))
       return set_err(FX_EINVAL, "GLM-MHD: kappa > 1, mu_0 > 0, GLM_scale > 0");

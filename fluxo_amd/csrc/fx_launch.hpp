@@ -483,18 +483,29 @@ void launch_mhd_volume_n(const Batch &b, int mode, RKArgs rk)
   else launch_mhd_volume_t<N, VMODE_RHS>(b, rk);
 }
 
-template <int N>
-void launch_mhd_surface_n(const Batch &b)
+template <int N, int RS>
+void launch_mhd_surface_r(const Batch &b)
 {
   const long long tot = (long long)b.cfg.nSides * b.n2;
   const int nb = (int)((tot + 255) / 256);
   if (nb <= 0) return;
   if (b.cfg.parabolic)
-    hipLaunchKernelGGL((k_mhd_surface<N, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse, b.Flux,
-                       b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
-  else
-    hipLaunchKernelGGL((k_mhd_surface<N, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
+    hipLaunchKernelGGL((k_mhd_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
                        b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
+  else
+    hipLaunchKernelGGL((k_mhd_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
+                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
+}
+
+template <int N>
+void launch_mhd_surface_n(const Batch &b)
+{
+  switch (b.cfg.riemann) {
+    case 0: launch_mhd_surface_r<N, 0>(b); break;
+    case 10: launch_mhd_surface_r<N, 10>(b); break;
+    case 11: launch_mhd_surface_r<N, 11>(b); break;
+    default: launch_mhd_surface_r<N, 1>(b); break;
+  }
 }
 
 template <int N>

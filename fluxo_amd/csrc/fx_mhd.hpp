@@ -220,10 +220,73 @@ __device__ __forceinline__ void mhd_noncons_surface(const double *UL, const doub
   F[5] += bn * vL0; F[6] += bn * vL1; F[7] += bn * vL2;
 }
 
-// FillFlux for GLM-MHD (fillflux.f90:41-110 with NONCONS): Rusanov (RiemannSolverByRusanov
-// mhd/riemann.f90:273-308), in the resistive case plus the BR1 average of the diffusive
-// fluxes (Riemann mhd/riemann.f90:83-142), then each side's Powell term, times SurfElem.
-template <int N, bool PARA>
+// Entropy-conserving surface fluxes along the unit normal n: the Derigs flux
+// (EntropyAndKinEnergyConservingFlux_Derigs mhd/flux_average.f90:660-751; Riemann=11) and
+// EntropyStableByLLF (mhd/riemann.f90:1027-1211; Riemann=10), which subtracts
+// LambdaMax_s2 H [v_R - v_L] with the jump of the entropy variables and the reference's
+// symmetric H (Euler block in the log-mean averages, tau = 1/(beta_L+beta_R) on B and psi).
+// The reference evaluates both on states rotated to (n, t1, t2) and rotates the flux back; here
+// the same quantities are formed in the Cartesian frame: scalars and the products u_avg.n,
+// B_avg.n are rotation invariant, the momentum row of H is u_avg (rhoLN v1 + rhoLN u_avg.vv +
+// (Eavg+pAvg) v5) + pAvg vv, which is what rotating the rotated-frame rows back gives.
+template <int RS>
+__device__ __forceinline__ void mhd_riemann_ec(const double *UL, const double *UR, const double *n, const MhdArgs &a,
+                                               double ch, double *F)
+{
+  const double km1 = a.kappa - 1.0, skm1 = 1.0 / km1;
+  double fL[MF_COUNT], fR[MF_COUNT];
+  mhd_node_fields(UL, a, fL);
+  mhd_node_fields(UR, a, fR);
+  derigs_flux<double>(fL, fR, n, n, ch, a.smu0, skm1, F);
+  if (RS != 10) return;
+  const double rhoL = fL[MF_RHO], rhoR = fR[MF_RHO], bL = fL[MF_BETA], bR = fR[MF_BETA];
+  const double rhoLN = ln_mean(rhoL, rhoR, fL[MF_LNRHO], fR[MF_LNRHO]);
+  const double sbetaLN = inv_ln_mean(bL, bR, fL[MF_LNBETA], fR[MF_LNBETA]);
+  const double tau = 1.0 / (bL + bR);
+  const double pAvg = 0.5 * (rhoL + rhoR) * tau;
+  const double pLN = 0.5 * rhoLN * sbetaLN, srhoLN = 1.0 / rhoLN;
+  const double uA[3] = {0.5 * (fL[MF_V1] + fR[MF_V1]), 0.5 * (fL[MF_V2] + fR[MF_V2]), 0.5 * (fL[MF_V3] + fR[MF_V3])};
+  const double BA[3] = {0.5 * (fL[MF_B1] + fR[MF_B1]), 0.5 * (fL[MF_B2] + fR[MF_B2]), 0.5 * (fL[MF_B3] + fR[MF_B3])};
+  const double psiA = 0.5 * (fL[MF_PSI] + fR[MF_PSI]);
+  const double u2Avg = 0.5 * (fL[MF_VV] + fR[MF_VV]), B2Avg = 0.5 * (fL[MF_BB] + fR[MF_BB]);
+  const double un = uA[0] * n[0] + uA[1] * n[1] + uA[2] * n[2];
+  const double Bn = BA[0] * n[0] + BA[1] * n[1] + BA[2] * n[2];
+  // wave speed estimate (the reference's va2Avg carries no 1/mu_0)
+  const double a2 = a.kappa * pAvg * srhoLN, va2 = B2Avg * srhoLN, ca2 = Bn * Bn * srhoLN;
+  const double cf = sqrt(0.5 * ((a2 + va2) + sqrt((a2 + va2) * (a2 + va2) - 4.0 * a2 * ca2)));
+  const double lam = 0.5 * (fabs(un) + cf);
+  const double uA2 = uA[0] * uA[0] + uA[1] * uA[1] + uA[2] * uA[2];
+  const double Eavg = pLN * skm1 + 0.5 * rhoLN * (2.0 * uA2 - u2Avg);
+  const double H55 = (pLN * pLN * skm1 + Eavg * Eavg) * srhoLN + uA2 * pAvg +
+                     tau * ((BA[0] * BA[0] + BA[1] * BA[1] + BA[2] * BA[2]) + psiA * psiA);
+  // jump of the entropy variables (ConsToEntropy, logs of the ratios as in the reference)
+  const double pL = 0.5 * rhoL / bL, pR = 0.5 * rhoR / bR;
+  const double j1 = (a.kappa * log(rhoR * (1.0 / rhoL)) - log(pR / pL)) * skm1 - (bR * fR[MF_VV] - bL * fL[MF_VV]);
+  const double jv[3] = {2.0 * (bR * fR[MF_V1] - bL * fL[MF_V1]), 2.0 * (bR * fR[MF_V2] - bL * fL[MF_V2]),
+                        2.0 * (bR * fR[MF_V3] - bL * fL[MF_V3])};
+  const double j5 = -2.0 * (bR - bL);
+  const double jB[3] = {2.0 * (bR * fR[MF_B1] - bL * fL[MF_B1]), 2.0 * (bR * fR[MF_B2] - bL * fL[MF_B2]),
+                        2.0 * (bR * fR[MF_B3] - bL * fL[MF_B3])};
+  const double j9 = 2.0 * (bR * fR[MF_PSI] - bL * fL[MF_PSI]);
+  const double ujv = uA[0] * jv[0] + uA[1] * jv[1] + uA[2] * jv[2];
+  const double Ep = Eavg + pAvg;
+  F[0] -= lam * (rhoLN * j1 + rhoLN * ujv + Eavg * j5);
+  const double cm = rhoLN * j1 + rhoLN * ujv + Ep * j5;
+#pragma unroll
+  for (int d = 0; d < 3; d++) F[1 + d] -= lam * (uA[d] * cm + pAvg * jv[d]);
+  F[4] -= lam * (Eavg * j1 + Ep * ujv + H55 * j5 + tau * ((BA[0] * jB[0] + BA[1] * jB[1] + BA[2] * jB[2]) + psiA * j9));
+  const double lt = lam * tau;
+#pragma unroll
+  for (int d = 0; d < 3; d++) F[5 + d] -= lt * (BA[d] * j5 + jB[d]);
+  F[8] -= lt * (psiA * j5 + j9);
+}
+
+// FillFlux for GLM-MHD (fillflux.f90:41-110 with NONCONS): the surface flux RS -- standard
+// DG average 0 (StandardDGFlux mhd/flux_average.f90:421-484), Rusanov 1
+// (RiemannSolverByRusanov mhd/riemann.f90:273-308), entropy-stable LLF 10, Derigs EC 11 --
+// in the resistive case plus the BR1 average of the diffusive fluxes (Riemann
+// mhd/riemann.f90:83-142), then each side's Powell term, times SurfElem.
+template <int N, int RS, bool PARA>
 __global__ void __launch_bounds__(256)
 k_mhd_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
               const double4 *__restrict__ nvse, double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides,
@@ -243,13 +306,17 @@ k_mhd_surface(const double *__restrict__ U, const double *__restrict__ FvN, cons
   const double4 g = nvse[gid];
   const double nv[3] = {g.x, g.y, g.z};
   const double ch = st->glm_ch;
-  double FL[MV], FR[MV], vnL, vnR, cfL, cfR;
-  mhd_flux_n(UL, nv, a, ch, FL, vnL, cfL);
-  mhd_flux_n(UR, nv, a, ch, FR, vnR, cfR);
-  const double lam = fmax(fabs(vnL) + cfL, fabs(vnR) + cfR);
   double Fm[MV], Fs[MV];
+  if (RS == 0 || RS == 1) {
+    double FL[MV], FR[MV], vnL, vnR, cfL, cfR;
+    mhd_flux_n(UL, nv, a, ch, FL, vnL, cfL);
+    mhd_flux_n(UR, nv, a, ch, FR, vnR, cfR);
+    const double lam = RS == 1 ? fmax(fabs(vnL) + cfL, fabs(vnR) + cfR) : 0.0;
 #pragma unroll
-  for (int v = 0; v < MV; v++) Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
+    for (int v = 0; v < MV; v++) Fm[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
+  } else {
+    mhd_riemann_ec<RS>(UL, UR, nv, a, ch, Fm);
+  }
   if (PARA) {  // BR1: mean of the two sides' normal diffusive fluxes
     const double *fn = FvN + (size_t)gid * 14;
 #pragma unroll

@@ -368,17 +368,19 @@ class Server:
 
 
 class MhdEngine(Engine):
-    """A GLM-MHD context (9 variables; Derigs volume flux, Rusanov surface flux, Powell
-    terms): U is (nElems, n, n, n, 9).  resistive=dict(mu=, eta=, Pr=) selects the resistive
-    equations (BR1 lifting of the entropy variables)."""
+    """A GLM-MHD context (9 variables; Derigs volume flux, Powell terms; surface flux
+    `riemann` 0 central, 1 Rusanov, 10 Derigs + entropy-stable LLF, 11 Derigs): U is
+    (nElems, n, n, n, 9).  resistive=dict(mu=, eta=, Pr=) selects the resistive equations
+    (BR1 lifting of the entropy variables)."""
 
-    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64, resistive=None):
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64, resistive=None,
+                 riemann=1):
         L = lib()
         self.N = int(mesh["N"])
         self.nElems = int(mesh["nElems"])
         self.nSides = int(mesh["nSides"])
         res = resistive or {}
-        self.cfg = make_config(self.N, self.nElems, self.nSides, 10, 1, resistive is not None, kappa,
+        self.cfg = make_config(self.N, self.nElems, self.nSides, 10, riemann, resistive is not None, kappa,
                                float(res.get("mu", 0.0)), float(res.get("Pr", 0.72)), precision)
         self.cfg.nVar, self.cfg.mu_0, self.cfg.GLM_scale = 9, mu_0, glm_scale
         self.cfg.eta = float(res.get("eta", 0.0))

@@ -45,7 +45,8 @@ typedef struct {
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:291-404): 0 central, 1 LLF,
                        9 / 32 EC (Chandrashekar / Ranocha), 4 / 44 / 33 entropy stable
                        (Ismail-Roe / Chandrashekar / Ranocha + matrix dissipation),
-                       16 EC-KEP + face LLF */
+                       16 EC-KEP + face LLF; MHD (mhd/equation.f90:327-386): 0 central,
+                       1 Rusanov, 10 Derigs EC + entropy-stable LLF, 11 Derigs EC */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
                        resistive MHD */
   int lifting;      /* PP_Lifting with entropy-variable lifting: 1 = BR1, 2 = BR2
@@ -72,7 +73,7 @@ typedef struct {
   double shockBlendCoef;  /* 'ShockBlendCoef' */
   /* GLM-MHD (nVar = 9: FLUXO_EQNSYSNAME=mhd with FLUXO_EQN_GLM and Powell
      non-conservative terms, CMakeLists.txt:174-209; mhd/equation.f90:174-208):
-     VolumeFlux 10 (Derigs), Riemann 1 (Rusanov), no shock capturing; parabolic = 1 adds
+     VolumeFlux 10 (Derigs), Riemann 0, 1, 10 or 11, no shock capturing; parabolic = 1 adds
      viscosity mu0, resistivity eta and heat conduction mu0*kappa/Pr with BR1 lifting of
      the entropy variables (mhd/flux.f90:449-573). */
   double mu_0;            /* 'mu_0' magnetic permeability */

@@ -138,7 +138,8 @@ void ora_volint_nfvse(const ora_problem *P, const double *U, const double *alpha
 typedef struct {
   const ora_problem *P;   /* mesh and N (nVar, volflux, riemann of P are not used) */
   double kappa, mu_0, glm_scale;
-  int parabolic, reserved;
+  int parabolic;
+  int riemann;            /* surface flux: 0 central, 1 Rusanov, 10 Derigs + ES-LLF, 11 Derigs */
   double mu, eta, Pr;
 } ora_mhd;
 /* primitive gradients (rho, v, p, B, psi) of the resistive case, gradPx/y/z(9,n,n,n,E) */

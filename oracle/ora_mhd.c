@@ -1,8 +1,8 @@
 /*
  * CPU oracle: GLM-MHD split-form DGSEM on LGL nodes (9 variables: rho, rho v, E, B, psi),
  * with the Derigs et al. entropy- and kinetic-energy-conserving two-point flux
- * (VolumeFlux=10), Powell non-conservative terms (NONCONS=1), the Rusanov surface flux
- * (Riemann=1), CALCTIMESTEP with the GLM cleaning speed from the time step, and the
+ * (VolumeFlux=10), Powell non-conservative terms (NONCONS=1), the central, Rusanov,
+ * Derigs entropy-stable LLF or Derigs EC surface flux (Riemann=0, 1, 10, 11), CALCTIMESTEP with the GLM cleaning speed from the time step, and the
  * Carpenter RK4-5 step.  Resistive case (PARABOLIC): BR1 lifting of the entropy variables,
  * isotropic viscosity, resistivity and heat conduction (EvalDiffFlux3D), the weak-form
  * VolInt_visc and the BR1 average of the diffusive fluxes at faces.
@@ -114,7 +114,98 @@ static double fastest_wave_1d(const ora_mhd *M, const double *p)
   return sqrt(0.5 * (c2 + va2 + astar));
 }
 
-/* AdvRiemann (mhd/riemann.f90:174-246) with RiemannSolverByRusanov (:273-308) */
+/* EntropyAndKinEnergyConservingFlux_Derigs (mhd/flux_average.f90:660-751): x-direction of a
+   rotated state */
+static void derigs_1d(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, double *F)
+{
+  const double smu0 = 1.0 / M->mu_0, km1 = M->kappa - 1.0, skm1 = 1.0 / km1;
+  const double EL = UL[4] - 0.5 * smu0 * UL[8] * UL[8], ER = UR[4] - 0.5 * smu0 * UR[8] * UR[8];
+  double vL[3], vR[3], vA[3], BA[3];
+  for (int d = 0; d < 3; d++) { vL[d] = UL[1 + d] / UL[0]; vR[d] = UR[1 + d] / UR[0]; }
+  const double v2L = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
+  const double v2R = vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2];
+  const double B2L = UL[5] * UL[5] + UL[6] * UL[6] + UL[7] * UL[7];
+  const double B2R = UR[5] * UR[5] + UR[6] * UR[6] + UR[7] * UR[7];
+  const double pL = km1 * (EL - 0.5 * (UL[0] * v2L + smu0 * B2L));
+  const double pR = km1 * (ER - 0.5 * (UR[0] * v2R + smu0 * B2R));
+  const double betaL = 0.5 * UL[0] / pL, betaR = 0.5 * UR[0] / pR;
+  const double rhoLN = ln_mean(UL[0], UR[0]), betaLN = ln_mean(betaL, betaR);
+  for (int d = 0; d < 3; d++) { vA[d] = 0.5 * (vL[d] + vR[d]); BA[d] = 0.5 * (UL[5 + d] + UR[5 + d]); }
+  const double v1B2 = 0.5 * (vL[0] * B2L + vR[0] * B2R);
+  const double vB = 0.5 * ((vL[0] * UL[5] + vL[1] * UL[6] + vL[2] * UL[7]) +
+                           (vR[0] * UR[5] + vR[1] * UR[6] + vR[2] * UR[7]));
+  const double pTilde = 0.5 * ((UL[0] + UR[0]) / (betaL + betaR) + smu0 * 0.5 * (B2L + B2R));
+  const double psiA = 0.5 * (UL[8] + UR[8]);
+  F[0] = rhoLN * vA[0];
+  F[1] = F[0] * vA[0] - smu0 * BA[0] * BA[0] + pTilde;
+  F[2] = F[0] * vA[1] - smu0 * BA[0] * BA[1];
+  F[3] = F[0] * vA[2] - smu0 * BA[0] * BA[2];
+  F[6] = vA[0] * BA[1] - BA[0] * vA[1];
+  F[7] = vA[0] * BA[2] - BA[0] * vA[2];
+  F[5] = glm_ch * psiA;
+  F[8] = glm_ch * BA[0];
+  F[4] = F[0] * 0.5 * (skm1 / betaLN - 0.5 * (v2L + v2R)) + (vA[0] * F[1] + vA[1] * F[2] + vA[2] * F[3]) +
+         smu0 * ((BA[0] * F[5] + BA[1] * F[6] + BA[2] * F[7]) - 0.5 * v1B2 + BA[0] * vB + F[8] * psiA -
+                 glm_ch * 0.5 * (UL[8] * UL[5] + UR[8] * UR[5]));
+}
+
+/* EntropyStableByLLF (mhd/riemann.f90:1027-1211), rotated states: the Derigs flux minus
+   LambdaMax_s2 H (v_R - v_L), H the symmetric matrix of the reference (Euler block, tau on
+   B and psi), written out as the full 9x9 product (the reference skips its zero blocks) */
+static void es_llf_1d(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, double *F)
+{
+  const double smu0 = 1.0 / M->mu_0, km1 = M->kappa - 1.0, skm1 = 1.0 / km1, kappa = M->kappa;
+  derigs_1d(M, glm_ch, UL, UR, F);
+  const double EL = UL[4] - 0.5 * smu0 * UL[8] * UL[8], ER = UR[4] - 0.5 * smu0 * UR[8] * UR[8];
+  const double srL = 1.0 / UL[0], srR = 1.0 / UR[0];
+  double uL[3], uR[3], uA[3], BA[3];
+  for (int d = 0; d < 3; d++) { uL[d] = UL[1 + d] * srL; uR[d] = UR[1 + d] * srR; }
+  const double u2L = uL[0] * uL[0] + uL[1] * uL[1] + uL[2] * uL[2];
+  const double u2R = uR[0] * uR[0] + uR[1] * uR[1] + uR[2] * uR[2];
+  const double B2L = UL[5] * UL[5] + UL[6] * UL[6] + UL[7] * UL[7];
+  const double B2R = UR[5] * UR[5] + UR[6] * UR[6] + UR[7] * UR[7];
+  const double pL = km1 * (EL - 0.5 * (UL[0] * u2L + smu0 * B2L));
+  const double pR = km1 * (ER - 0.5 * (UR[0] * u2R + smu0 * B2R));
+  const double bL = 0.5 * UL[0] / pL, bR = 0.5 * UR[0] / pR;
+  const double rhoLN = ln_mean(UL[0], UR[0]), sbetaLN = 1.0 / ln_mean(bL, bR);
+  for (int d = 0; d < 3; d++) { uA[d] = 0.5 * (uL[d] + uR[d]); BA[d] = 0.5 * (UL[5 + d] + UR[5 + d]); }
+  const double u2Avg = 0.5 * (u2L + u2R), B2Avg = 0.5 * (B2L + B2R);
+  const double pAvg = 0.5 * (UL[0] + UR[0]) / (bL + bR);
+  const double psiA = 0.5 * (UL[8] + UR[8]);
+  const double pLN = 0.5 * rhoLN * sbetaLN, srhoLN = 1.0 / rhoLN;
+  const double a2 = kappa * pAvg * srhoLN, va2 = B2Avg * srhoLN, ca2 = BA[0] * BA[0] * srhoLN;
+  const double cf = sqrt(0.5 * ((a2 + va2) + sqrt((a2 + va2) * (a2 + va2) - 4.0 * a2 * ca2)));
+  const double lam = 0.5 * (fabs(uA[0]) + cf);
+  const double uA2 = uA[0] * uA[0] + uA[1] * uA[1] + uA[2] * uA[2];
+  const double Eavg = pLN * skm1 + 0.5 * rhoLN * (2.0 * uA2 - u2Avg);
+  const double tau = 1.0 / (bL + bR);
+  double H[9][9] = {{0}};
+  H[0][0] = rhoLN;
+  for (int d = 0; d < 3; d++) { H[1 + d][0] = H[0][1 + d] = rhoLN * uA[d]; }
+  H[4][0] = H[0][4] = Eavg;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) H[1 + r][1 + c] = rhoLN * uA[r] * uA[c] + (r == c ? pAvg : 0.0);
+  for (int d = 0; d < 3; d++) H[1 + d][4] = H[4][1 + d] = (Eavg + pAvg) * uA[d];
+  H[4][4] = (pLN * pLN * skm1 + Eavg * Eavg) * srhoLN + uA2 * pAvg +
+            tau * (BA[0] * BA[0] + BA[1] * BA[1] + BA[2] * BA[2] + psiA * psiA);
+  for (int d = 0; d < 3; d++) { H[5 + d][4] = H[4][5 + d] = tau * BA[d]; H[5 + d][5 + d] = tau; }
+  H[8][4] = H[4][8] = tau * psiA;
+  H[8][8] = tau;
+  double V[9];
+  V[0] = (kappa * log(UR[0] * srL) - log(pR / pL)) * skm1 - (bR * u2R - bL * u2L);
+  for (int d = 0; d < 3; d++) V[1 + d] = 2.0 * (bR * uR[d] - bL * uL[d]);
+  V[4] = -2.0 * (bR - bL);
+  for (int v = 5; v < MV; v++) V[v] = 2.0 * (bR * UR[v] - bL * UL[v]);
+  for (int r = 0; r < MV; r++) {
+    double s = 0.0;
+    for (int c = 0; c < MV; c++) s += H[r][c] * V[c];
+    F[r] -= lam * s;
+  }
+}
+
+/* AdvRiemann (mhd/riemann.f90:174-246) with the surface flux M->riemann: StandardDGFlux
+   (mhd/flux_average.f90:421-484; 0), RiemannSolverByRusanov (:273-308; 1), EntropyStableByLLF
+   (10), EntropyAndKinEnergyConservingFlux_Derigs (11) -- mhd/equation.f90:327-386 */
 void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double *F)
 {
@@ -129,13 +220,19 @@ void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const do
     cR[b + 1] = UR[b] * t1[0] + UR[b + 1] * t1[1] + UR[b + 2] * t1[2];
     cR[b + 2] = UR[b] * t2[0] + UR[b + 1] * t2[1] + UR[b + 2] * t2[2];
   }
-  adv_flux_1d(M, glm_ch, cL, FL);
-  adv_flux_1d(M, glm_ch, cR, FR);
-  cons_to_prim(M, cL, pL);
-  cons_to_prim(M, cR, pR);
-  const double cfL = fastest_wave_1d(M, pL), cfR = fastest_wave_1d(M, pR);
-  const double lam = fmax(fabs(pL[1]) + cfL, fabs(pR[1]) + cfR);
-  for (int v = 0; v < MV; v++) f[v] = 0.5 * ((FL[v] + FR[v]) - lam * (cR[v] - cL[v]));
+  if (M->riemann == 10) {
+    es_llf_1d(M, glm_ch, cL, cR, f);
+  } else if (M->riemann == 11) {
+    derigs_1d(M, glm_ch, cL, cR, f);
+  } else {
+    adv_flux_1d(M, glm_ch, cL, FL);
+    adv_flux_1d(M, glm_ch, cR, FR);
+    cons_to_prim(M, cL, pL);
+    cons_to_prim(M, cR, pR);
+    const double cfL = fastest_wave_1d(M, pL), cfR = fastest_wave_1d(M, pR);
+    const double lam = M->riemann == 1 ? fmax(fabs(pL[1]) + cfL, fabs(pR[1]) + cfR) : 0.0;
+    for (int v = 0; v < MV; v++) f[v] = 0.5 * ((FL[v] + FR[v]) - lam * (cR[v] - cL[v]));
+  }
   for (int v = 0; v < MV; v++) F[v] = f[v];
   for (int b = 1; b <= 5; b += 4)
     for (int d = 0; d < 3; d++) F[b + d] = nv[d] * f[b] + t1[d] * f[b + 1] + t2[d] * f[b + 2];

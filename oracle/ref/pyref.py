@@ -100,8 +100,9 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
 
 
 def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600,
-            resistive=None, dfl=0.9):
-    """GLM-MHD (Derigs volume flux, Rusanov surface flux, Powell non-conservative terms) with
+            resistive=None, dfl=0.9, riemann=1):
+    """GLM-MHD (Derigs volume flux, surface flux `riemann` -- 0 central, 1 Rusanov, 10 Derigs +
+    entropy-stable LLF, 11 Derigs -- Powell non-conservative terms) with
     the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before the first
     CALCTIMESTEP), dt with GLM_ch and GLM_dtch1 after it, and an nsteps trajectory.
     resistive: None (ideal), or dict(mu=, eta=, Pr=) for the BR1 build with viscosity,
@@ -114,7 +115,7 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
-            f.write(f"N = {N}\nRiemann = 1\nVolumeFlux = 10\nIniExactFunc = 1\nIniRefState = 1\n"
+            f.write(f"N = {N}\nRiemann = {int(riemann)}\nVolumeFlux = 10\nIniExactFunc = 1\nIniRefState = 1\n"
                     f"RefState = (/1.,0.,0.,0.,1.,0.,0.,0./)\nkappa = {kappa!r}\nmu_0 = {mu_0!r}\n"
                     f"GLM_scale = {glm_scale!r}\n")
             if par_on:

@@ -106,8 +106,18 @@ MHD_RES_CASES = [
     ("mhd_resistive_n3_viscous_limited", 3, 3, 3, True, 3, 0.05, 0.0, 0.9),
 ]
 
+# other GLM-MHD surface fluxes (Riemann 0 central, 10 Derigs + entropy-stable LLF, 11 Derigs EC):
+# name, K, N, Ngeo, warp, nsteps, riemann, resistive (mu, eta, Pr) or None
+MHD_RIEMANN_CASES = [
+    ("mhd_n3_es_llf_warped", 3, 3, 3, True, 3, 10, None),
+    ("mhd_n4_es_llf_warped", 2, 4, 3, True, 3, 10, None),
+    ("mhd_n3_derigs_ec_warped", 3, 3, 3, True, 3, 11, None),
+    ("mhd_n2_central_warped", 3, 2, 3, True, 3, 0, None),
+    ("mhd_resistive_n3_es_llf_warped", 2, 3, 3, True, 3, 10, (0.01, 0.02, 0.72)),
+]
 
-def make_mhd(case):
+
+def make_mhd(case, riemann=1):
     name, K, N, Ngeo, warp, nsteps = case[:6]
     res = dict(mu=case[6], eta=case[7], Pr=case[8]) if len(case) > 6 else None
     m = O.box_mesh(K, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=Ngeo, warp=warp)
@@ -117,8 +127,9 @@ def make_mhd(case):
     U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
     U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
     U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
-    r = R.run_mhd(m, U, cfl=0.9, nsteps=nsteps, resistive=res, dfl=0.9)
+    r = R.run_mhd(m, U, cfl=0.9, nsteps=nsteps, resistive=res, dfl=0.9, riemann=riemann)
     out = dict(K=K, N=N, Ngeo=Ngeo, warp=int(warp), x0=np.zeros(3), L=np.ones(3), kappa=5.0 / 3.0, mu_0=1.0,
+               riemann=riemann,
                glm_scale=0.5, cfl=0.9, nsteps=nsteps, U=U, Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"],
                errType0=r["errType0"], glm_ch0=r["glm_ch0"], glm_dtch1=r["glm_dtch1"], U_end=r["U_end"],
                t_end=r["t_end"], sJ=r["sJ"], Mf=r["Mf"], SurfElem=r["SurfElem"])
@@ -159,3 +170,5 @@ if __name__ == "__main__":
         make_nfvse(c)
     for c in MHD_CASES + MHD_RES_CASES:
         make_mhd(c)
+    for name, K, N, Ngeo, warp, nsteps, rs, res in MHD_RIEMANN_CASES:
+        make_mhd((name, K, N, Ngeo, warp, nsteps) + (tuple(res) if res else ()), riemann=rs)

@@ -29,6 +29,10 @@ def resistive_of(g):
     return dict(mu=float(g["mu"]), eta=float(g["eta"]), Pr=float(g["Pr"])) if "mu" in g else None
 
 
+def riemann_of(g):
+    return int(g["riemann"]) if "riemann" in g else 1
+
+
 def mesh_of(g):
     return O.box_mesh(int(g["K"]), int(g["N"]), x0=tuple(g["x0"]), length=tuple(g["L"]), Ngeo=int(g["Ngeo"]),
                       warp=bool(g["warp"]))
@@ -41,7 +45,7 @@ def test_oracle_matches_reference_mhd(name):
     res = resistive_of(g)
     dfl = float(g["dfl"]) if res else 0.0
     P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
-                     resistive=res)
+                     resistive=res, riemann=riemann_of(g))
     U = g["U"]
     if res:
         for a, k in zip(P.lifting(U), ("gradPx", "gradPy", "gradPz")):
@@ -52,7 +56,7 @@ def test_oracle_matches_reference_mhd(name):
     assert P.glm_dtch1 == pytest.approx(float(g["glm_dtch1"]), rel=1e-15)
     assert P.glm_ch == pytest.approx(float(g["glm_ch0"]), rel=1e-15)
     P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
-                     resistive=res)
+                     resistive=res, riemann=riemann_of(g))
     Ue, Utmp, t = U.copy(), np.zeros_like(U), 0.0
     for _ in range(int(g["nsteps"])):
         d, _ = P.calc_dt(Ue, float(g["cfl"]), dfl)
@@ -148,6 +152,55 @@ def test_semidiscrete_entropy_conservation_and_stability():
     U[..., 5:8] *= 1.0 + 1e-2 * rng.uniform(-1, 1, U[..., 5:8].shape)
     prod, scale = entropy_production(m, U, P.rhs(U, glm_ch=1.0))
     assert prod < -1e-6 * scale
+
+
+@pytest.mark.parametrize("riemann", [11, 10])
+def test_ec_and_es_surface_fluxes_entropy_balance(riemann):
+    """The reference's softBlast entropyCons / entropyStab checks (tests/job_def.py:519-548,
+    Riemann 11 / 10 with VolumeFlux 10): on data with jumps at every face (psi = 0), the
+    Derigs EC surface flux gives zero entropy production to round-off and the
+    entropy-stable LLF flux a strictly negative one."""
+    m = O.box_mesh(3, 3, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    P = O.MhdProblem(m, riemann=riemann)
+    U = P.exact_orszag_tang()
+    U[..., 8] = 0.0
+    rng = np.random.default_rng(6)
+    U[..., 0] *= 1.0 + 1e-2 * rng.uniform(-1, 1, U[..., 0].shape)
+    U[..., 5:8] *= 1.0 + 1e-2 * rng.uniform(-1, 1, U[..., 5:8].shape)
+    prod, scale = entropy_production(m, U, P.rhs(U, glm_ch=1.0))
+    if riemann == 11:
+        assert abs(prod) <= 1e-13 * scale
+    else:
+        assert prod < -1e-6 * scale
+
+
+def test_es_llf_dissipation_is_negative_definite_and_rotation_invariant():
+    """Per face: [w] . (F_ES - F_EC) = -LambdaMax_s2 [w]^T H [w] < 0 for random state pairs
+    (H symmetric positive definite), F_ES(u, u) = F_EC(u, u), and the Cartesian flux does
+    not depend on the choice of tangent vectors."""
+    rng = np.random.default_rng(8)
+    m = O.box_mesh(1, 2)
+    ES, EC = O.MhdProblem(m, riemann=10), O.MhdProblem(m, riemann=11)
+    lib = O.lib()
+    for _ in range(20):
+        UL, UR = random_state(rng), random_state(rng)
+        nv = rng.normal(size=3)
+        nv /= np.linalg.norm(nv)
+        a = rng.normal(size=3)
+        t1 = a - (a @ nv) * nv
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(nv, t1)
+        Fes, Fec, F2 = np.zeros(9), np.zeros(9), np.zeros(9)
+        lib.ora_mhd_riemann(ES.ptr, 0.7, UL, UR, nv, t1, t2, Fes)
+        lib.ora_mhd_riemann(EC.ptr, 0.7, UL, UR, nv, t1, t2, Fec)
+        wj = entropy_vars(UR)[0] - entropy_vars(UL)[0]
+        assert wj @ (Fes - Fec) < -1e-8 * np.abs(wj).sum() * np.abs(Fes - Fec).sum()
+        c, s = np.cos(0.4), np.sin(0.4)
+        lib.ora_mhd_riemann(ES.ptr, 0.7, UL, UR, nv, c * t1 + s * t2, -s * t1 + c * t2, F2)
+        assert np.allclose(Fes, F2, rtol=1e-13, atol=1e-13)
+        lib.ora_mhd_riemann(ES.ptr, 0.7, UL, UL, nv, t1, t2, Fes)
+        lib.ora_mhd_riemann(EC.ptr, 0.7, UL, UL, nv, t1, t2, Fec)
+        assert np.allclose(Fes, Fec, rtol=1e-13, atol=1e-13)
 
 
 def test_rusanov_rotation_invariance_and_consistency():
