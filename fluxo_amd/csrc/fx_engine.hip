@@ -11,12 +11,16 @@ __constant__ Ops c_ops[MAXN + 1];
 }
 
 #include "fx_launch.hpp"
+#include "fx_analyze.hpp"
 
 using namespace fx;
 
 struct fx_ctx {
   Batch b;                       // nElems/nSides of the mesh, eprob == nullptr
   double *part = nullptr, *red = nullptr;
+  double *xgp = nullptr;                                   // Elem_xGP (fx_elem_xgp_upload)
+  double *ana_part = nullptr, *ana_ops = nullptr, *ana_out = nullptr;  // fx_analyze, on first use
+  int ana_na = -1;
   bool mesh_ok = false, state_ok = false;
   hipGraphExec_t graph = nullptr;
   double graph_cfl = -1.0, graph_dfl = -1.0;
@@ -75,6 +79,32 @@ int check_config(const fx_config *c)
 }  // namespace
 
 // ======================================================================================
+template <int N, int NV>
+static void launch_analyze_nv(fx_ctx *c, const AnaArgs &a)
+{
+  const Batch &b = c->b;
+  constexpr int n = N + 1;
+  const size_t lds = (size_t)(NV + 4) * a.na * n * n * 8;
+  const double *g = b.cfg.parabolic ? b.grad : nullptr;
+  const size_t nv = (size_t)NV * b.n3 * b.cfg.nElems;
+  hipLaunchKernelGGL((k_analyze_nodes<N, NV>), dim3(b.cfg.nElems), dim3(256), 0, b.stream, b.U, b.Ut, b.sJ,
+                     c->xgp, g, g ? g + nv : nullptr, g ? g + 2 * nv : nullptr, a, c->ana_part);
+  if (a.exactFunc > 0)
+    hipLaunchKernelGGL((k_analyze_points<N, NV>), dim3(b.cfg.nElems), dim3(256), lds, b.stream, b.U, b.sJ, c->xgp,
+                       c->ana_ops, c->ana_ops + ANA_MAXNA * (MAXN + 1), a, c->ana_part);
+  else  // no error norms: zero the L2 and LINF columns
+    hipLaunchKernelGGL((k_analyze_zero_norms<NV>), dim3((b.cfg.nElems + 255) / 256), dim3(256), 0, b.stream,
+                       c->ana_part, b.cfg.nElems);
+  hipLaunchKernelGGL((k_analyze_reduce<NV>), dim3(AnaRec<NV>::COUNT), dim3(256), 0, b.stream, c->ana_part,
+                     b.cfg.nElems, c->ana_out);
+}
+template <int N>
+static void launch_analyze_n(fx_ctx *c, const AnaArgs &a)
+{
+  if (c->b.mhd()) launch_analyze_nv<N, 9>(c, a);
+  else launch_analyze_nv<N, 5>(c, a);
+}
+
 extern "C" {
 
 const char *fx_last_error(void) { return err_buf(); }
@@ -180,7 +210,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
-                  b.Ffv, b.FluxS, b.dtch1Elem};
+                  b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -527,6 +557,93 @@ fail:
   (void)hipStreamSynchronize(b.stream);
   cleanup();
   return set_err(FX_EHIP, "tuning: %s", hipGetErrorString(err));
+}
+
+int fx_elem_xgp_upload(fx_ctx *c, const double *Elem_xGP)
+{
+  if (!c || !Elem_xGP) return set_err(FX_EINVAL, "null argument");
+  const size_t bytes = (size_t)3 * c->b.n3 * c->b.cfg.nElems * 8;
+  if (!c->xgp) HIPCHK(hipMalloc(&c->xgp, bytes));
+  HIPCHK(hipMemcpyAsync(c->xgp, Elem_xGP, bytes, hipMemcpyHostToDevice, c->b.stream));
+  HIPCHK(hipStreamSynchronize(c->b.stream));
+  return FX_OK;
+}
+
+int fx_analyze(fx_ctx *c, double t, const fx_analyze_config *cfg, fx_analysis *out)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (!cfg || !out) return set_err(FX_EINVAL, "null argument");
+  Batch &b = c->b;
+  const int N = b.cfg.N, NV = b.cfg.nVar, n = N + 1;
+  if (cfg->exactFunc < 0 || cfg->exactFunc > 2 || (cfg->exactFunc == 2 && NV != 5))
+    return set_err(FX_EINVAL, "AnalyzeExactFunc %d not supported (0, 1; 2 for 5 variables)", cfg->exactFunc);
+  if (cfg->exactFunc > 0 && !c->xgp) return set_err(FX_ESTATE, "error norms need fx_elem_xgp_upload");
+  const int NA = cfg->nAnalyze > 0 ? cfg->nAnalyze : 2 * (N + 1);
+  if (NA > ANA_MAXNA - 1) return set_err(FX_EINVAL, "NAnalyze=%d above 16", NA);
+  const int L = NV == 9 ? AnaRec<9>::COUNT : AnaRec<5>::COUNT;
+  if ((size_t)(NV + 4) * (NA + 1) * n * n * 8 + 4 * 2 * NV * 8 > 65536)
+    return set_err(FX_EINVAL, "NAnalyze=%d needs more than 64 KiB of LDS at N=%d", NA, N);
+  if (!c->ana_part) {
+    HIPCHK(hipMalloc(&c->ana_part, (size_t)b.cfg.nElems * L * 8));
+    HIPCHK(hipMalloc(&c->ana_ops, (size_t)(ANA_MAXNA * (MAXN + 1) + ANA_MAXNA) * 8));
+    HIPCHK(hipMalloc(&c->ana_out, (size_t)L * 8));
+  }
+  if (c->ana_na != NA) {
+    std::vector<double> h(ANA_MAXNA * (MAXN + 1) + ANA_MAXNA, 0.0);
+    analyze_basis(N, NA, h.data(), h.data() + ANA_MAXNA * (MAXN + 1));
+    HIPCHK(hipMemcpyAsync(c->ana_ops, h.data(), h.size() * 8, hipMemcpyHostToDevice, b.stream));
+    HIPCHK(hipStreamSynchronize(b.stream));
+    c->ana_na = NA;
+  }
+  AnaArgs a{};
+  a.exactFunc = cfg->exactFunc;
+  a.na = NA + 1;
+  for (int v = 0; v < 9; v++) a.ref[v] = cfg->refState[v];
+  for (int d = 0; d < 3; d++) { a.adv[d] = cfg->advVel[d]; a.wave[d] = cfg->waveNumber[d]; }
+  a.amp = cfg->amplitude;
+  a.t = t;
+  a.kappa = b.cfg.kappa;
+  a.s2mu0 = NV == 9 ? 0.5 / b.cfg.mu_0 : 0.0;
+  a.R = cfg->R;
+  // DGTimeDerivative of the state (Ut), and the primitive gradients when parabolic
+  RKArgs rk{};
+  q_stage(b, VMODE_RHS, rk);
+  if (b.cfg.parabolic) q_lifting(b, 0);
+  FX_DISPATCH_N(N, launch_analyze_n, c, a);
+  HIPCHK(hipGetLastError());
+  double h[AnaRec<9>::COUNT];
+  HIPCHK(hipMemcpyAsync(h, c->ana_out, (size_t)L * 8, hipMemcpyDeviceToHost, b.stream));
+  HIPCHK(hipStreamSynchronize(b.stream));
+  // normalisation of CalcErrorNorms and AnalyzeTestCase
+  std::memset(out, 0, sizeof(*out));
+  const int L2 = 1, L2C = 1 + NV, LINF = 1 + 2 * NV, BULK = 1 + 3 * NV, BULKT = 1 + 4 * NV, MAXUT = 1 + 5 * NV,
+            TC = 1 + 6 * NV;
+  const double vol = h[0], rho0 = 1.0;
+  out->vol = vol;
+  for (int v = 0; v < NV; v++) {
+    out->L2[v] = std::sqrt(h[L2 + v] / vol);
+    out->L2colloc[v] = std::sqrt(h[L2C + v] / vol);
+    out->Linf[v] = h[LINF + v];
+    out->bulk[v] = h[BULK + v];
+    out->bulk_t[v] = h[BULKT + v];
+    out->maxabs_Ut[v] = h[MAXUT + v];
+  }
+  out->uprime = std::sqrt(h[TC + 0] / vol * 2.0 / 3.0);
+  out->Ekin = h[TC + 0] / vol;
+  out->Ekin_comp = h[TC + 1] / (rho0 * vol);
+  out->Emag_comp = h[TC + 2] / (rho0 * vol);
+  out->Enstrophy_comp = h[TC + 3] / (rho0 * vol);
+  out->Mag_Enstrophy = h[TC + 4] / (rho0 * vol);
+  out->Entropy_comp = h[TC + 5] / (rho0 * vol);
+  out->DR_u = h[TC + 6] * cfg->dissFactor / vol;
+  out->DR_S = h[TC + 7] * 2.0 * cfg->dissFactor / (rho0 * vol);
+  out->DR_Sd = h[TC + 8] * 2.0 * cfg->dissFactor / (rho0 * vol);
+  out->DR_p = -h[TC + 9] / (rho0 * vol);
+  out->mean_temperature = h[TC + 10] / vol;
+  out->max_Vorticity = h[TC + 11];
+  out->max_Current = h[TC + 12];
+  return FX_OK;
 }
 
 int fx_synchronize(fx_ctx *c)

@@ -70,6 +70,23 @@ class fx_server_stats(C.Structure):
                 ("done", C.c_int), ("failed", C.c_int)]
 
 
+class fx_analyze_config(C.Structure):
+    _fields_ = [("exactFunc", C.c_int), ("nAnalyze", C.c_int), ("refState", C.c_double * 9),
+                ("advVel", C.c_double * 3), ("waveNumber", C.c_double * 3), ("amplitude", C.c_double),
+                ("R", C.c_double), ("dissFactor", C.c_double)]
+
+
+ANALYSIS_VECTORS = ("L2", "L2colloc", "Linf", "bulk", "bulk_t", "maxabs_Ut")
+ANALYSIS_SCALARS = ("Ekin", "Ekin_comp", "Emag_comp", "Enstrophy_comp", "Mag_Enstrophy", "Entropy_comp",
+                    "DR_u", "DR_S", "DR_Sd", "DR_p", "max_Vorticity", "max_Current", "mean_temperature",
+                    "uprime")
+
+
+class fx_analysis(C.Structure):
+    _fields_ = ([("vol", C.c_double)] + [(k, C.c_double * 9) for k in ANALYSIS_VECTORS] +
+                [(k, C.c_double) for k in ANALYSIS_SCALARS])
+
+
 FX_REQ_QUEUED, FX_REQ_RUNNING, FX_REQ_DONE, FX_REQ_FAILED = range(4)
 
 EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
@@ -78,7 +95,7 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_ctx_bytes", "fx_profile_kernel", "fx_server_create", "fx_server_submit",
            "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
            "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
-           "fx_use_volume_tile", "fx_blending", "fx_glm"]
+           "fx_use_volume_tile", "fx_blending", "fx_glm", "fx_elem_xgp_upload", "fx_analyze"]
 
 (FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS,
  FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV) = range(7)
@@ -124,6 +141,8 @@ def lib():
     L.fx_tune_volume.argtypes = [vp, C.c_int, ip, ip, dp, ip]
     L.fx_volume_tile.argtypes = [C.c_int, C.c_int, C.c_int, ip]
     L.fx_use_volume_tile.argtypes = [vp, C.c_int]
+    L.fx_elem_xgp_upload.argtypes = [vp, vp]
+    L.fx_analyze.argtypes = [vp, C.c_double, C.POINTER(fx_analyze_config), C.POINTER(fx_analysis)]
     L.fx_server_last_error.argtypes = [vp]
     L.fx_server_last_error.restype = C.c_char_p
     _LIB = L
@@ -268,6 +287,33 @@ class Engine:
         s, d = C.c_double(), C.c_double()
         _chk(lib().fx_entropy(self._ctx, C.byref(s), C.byref(d)))
         return s.value, d.value
+
+    def upload_coords(self, Elem_xGP):
+        """Elem_xGP (nElems, n, n, n, 3), for the exact functions of analyze()."""
+        x = np.ascontiguousarray(Elem_xGP, dtype=np.float64)
+        _chk(lib().fx_elem_xgp_upload(self._ctx, _ptr(x)))
+
+    def analyze(self, t=0.0, exact_func=0, n_analyze=0, ref_state=None, adv_vel=(1.0, 1.0, 1.0),
+                wave_number=(1.0, 1.0, 1.0), amplitude=0.1, R=287.058, diss_factor=1.0):
+        """The reference's Analyze of the device state at time t (fx_analyze): error norms
+        against exact_func (needs upload_coords), volume, bulk integrals of U and of its RHS,
+        Taylor-Green vortex quantities.  Returns a dict of the fx_analysis fields."""
+        c = fx_analyze_config()
+        c.exactFunc, c.nAnalyze = exact_func, n_analyze
+        for i, v in enumerate(ref_state or ()):
+            c.refState[i] = v
+        for i in range(3):
+            c.advVel[i], c.waveNumber[i] = adv_vel[i], wave_number[i]
+        c.amplitude, c.R, c.dissFactor = amplitude, R, diss_factor
+        a = fx_analysis()
+        _chk(lib().fx_analyze(self._ctx, t, C.byref(c), C.byref(a)))
+        nv = self.shape()[-1]
+        out = {"vol": a.vol}
+        for k in ANALYSIS_VECTORS:
+            out[k] = np.array(getattr(a, k)[:nv])
+        for k in ANALYSIS_SCALARS:
+            out[k] = getattr(a, k)
+        return out
 
     def profile_kernel(self, which, reps=20):
         """Average ms per launch of one engine kernel (HIP events on the engine stream)."""

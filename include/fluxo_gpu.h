@@ -147,6 +147,44 @@ int fx_run(fx_ctx *ctx, int nSteps, double CFLscale, double DFLscale, double *t,
    the last Ut: S = int -rho s/(kappa-1), dSdU_Ut = int dS/dU . Ut. */
 int fx_entropy(fx_ctx *ctx, double *S, double *dSdU_Ut);
 
+/* Analysis of the device state (Analyze, src/analyze/analyze.f90:370-540).  Options follow
+   the reference's parameters. */
+typedef struct {
+  int exactFunc;          /* 'AnalyzeExactFunc' (analyze.f90:113): 0 no error norms, 1 the
+                             constant refState, 2 density wave (ExactFunc 2 of
+                             navierstokes/equation.f90:632-651; 5 variables only) */
+  int nAnalyze;           /* 'NAnalyze' (analyze.f90:122); 0 = the default 2(N+1); at most 16 */
+  double refState[9];     /* ExactFunc 1: conservative state (RefStateCons) */
+  double advVel[3];       /* 'AdvVel' */
+  double waveNumber[3];   /* 'IniWaveNumber' */
+  double amplitude;       /* 'IniAmplitude' */
+  double R;               /* 'R' gas constant, for the mean temperature (reference 287.058) */
+  double dissFactor;      /* factor of DR_u, DR_S, DR_Sd (the reference's MHD Taylor-Green
+                             testcase multiplies by mu_0) */
+} fx_analyze_config;
+
+/* Results, normalised as the reference prints them: CalcErrorNorms (analyze.f90:557-628: L2
+   on the NAnalyze Gauss-Lobatto points, L2 on the collocation points, Linf; zero when
+   exactFunc = 0), the domain volume, CalcBulk (:633-675), and the Taylor-Green vortex
+   testcase analysis (testcase/taylorgreenvortex/testcase_analyze.f90:127-401; magnetic
+   terms zero for 5 variables; gradient terms zero without PARABOLIC). */
+typedef struct {
+  double vol;
+  double L2[9], L2colloc[9], Linf[9];
+  double bulk[9], bulk_t[9], maxabs_Ut[9];
+  double Ekin, Ekin_comp, Emag_comp, Enstrophy_comp, Mag_Enstrophy, Entropy_comp;
+  double DR_u, DR_S, DR_Sd, DR_p, max_Vorticity, max_Current, mean_temperature, uprime;
+} fx_analysis;
+
+/* Elem_xGP(3,0:N,0:N,0:N,nElems) of CalcMetrics (metrics.f90), kept on the device for the
+   exact functions of fx_analyze. */
+int fx_elem_xgp_upload(fx_ctx *ctx, const double *Elem_xGP);
+
+/* Analyze (analyze.f90:370-540) at time t: evaluates the right-hand side of the device state
+   (DGTimeDerivative, and the lifted gradients when parabolic) and reduces everything on the
+   device; only the fx_analysis record crosses to the host. */
+int fx_analyze(fx_ctx *ctx, double t, const fx_analyze_config *cfg, fx_analysis *out);
+
 /* GLM-MHD cleaning state: GLM_ch (0.5 until the first CALCTIMESTEP, then
    GLM_scale*GLM_dtch1/dt of the last one, mhd/calctimestep.f90:175-177) and GLM_dtch1. */
 int fx_glm(fx_ctx *ctx, double *glm_ch, double *glm_dtch1);

@@ -156,6 +156,30 @@ double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale,
 void ora_mhd_lserk_step(const ora_mhd *M, double *U, double *Ut_tmp, double glm_ch, double dt);
 void ora_mhd_orszag_tang(const ora_mhd *M, int npts, const double *x, double *U);
 
+/* ---- analysis of a state (src/analyze/analyze.f90:557-675, src/testcase/taylorgreenvortex/
+   testcase_analyze.f90:127-401) ---- */
+typedef struct {
+  int exactFunc;          /* AnalyzeExactFunc: 0 none (no error norms), 1 constant refState,
+                             2 density wave (5 variables) */
+  int nAnalyze;           /* NAnalyze; 0 = the default 2(N+1) */
+  double refState[9];     /* conservative state of ExactFunc 1 */
+  double advVel[3], waveNumber[3], amplitude;   /* AdvVel, IniWaveNumber, IniAmplitude */
+  double R;               /* gas constant 'R' (mean temperature) */
+  double dissFactor;      /* factor of DR_u, DR_S, DR_Sd (the MHD testcase uses mu_0) */
+} ora_analyze_cfg;
+typedef struct {
+  double vol;
+  double L2[9], L2colloc[9], Linf[9];
+  double bulk[9], bulk_t[9], maxabsUt[9];
+  double Ekin, Ekin_comp, Emag_comp, Enstrophy_comp, Mag_Enstrophy, Entropy_comp;
+  double DR_u, DR_S, DR_Sd, DR_p, max_Vorticity, max_Current, mean_temperature, uprime;
+} ora_analysis;
+/* nVar 5 (Euler / Navier-Stokes) or 9 (GLM-MHD, mu_0 its permeability); gx/gy/gz the lifted
+   primitive gradients or NULL (zero velocity and B gradients, as without PARABOLIC) */
+void ora_analyze(int N, int nVar, int nElems, const double *Elem_xGP, const double *sJ, const double *U,
+                 const double *Ut, const double *gx, const double *gy, const double *gz, double kappa,
+                 double mu_0, double t, const ora_analyze_cfg *c, ora_analysis *o);
+
 /* Carpenter RK4-5 coefficients and the LGL CFL/DFL alpha tables
    (src/timedisc/timedisc_vars.f90:79-90, 204-233) */
 void ora_rk_coefs(double *A, double *b, double *c);

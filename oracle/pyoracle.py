@@ -76,6 +76,8 @@ def lib():
         L.ora_persson_peraire.argtypes = [C.c_void_p, _dp, _dp]
         L.ora_blending_coefficient.argtypes = [C.c_void_p, _dp, _dp]
         L.ora_volint_nfvse.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ora_analyze.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -198,6 +200,60 @@ class Problem:
         lib().ora_calc_entropy(self.ptr, np.ascontiguousarray(U), np.ascontiguousarray(Ut),
                                C.byref(s), C.byref(d))
         return s.value, d.value
+
+
+class AnalyzeCfg(C.Structure):
+    _fields_ = [("exactFunc", C.c_int), ("nAnalyze", C.c_int), ("refState", C.c_double * 9),
+                ("advVel", C.c_double * 3), ("waveNumber", C.c_double * 3), ("amplitude", C.c_double),
+                ("R", C.c_double), ("dissFactor", C.c_double)]
+
+
+ANALYSIS_SCALARS = ("Ekin", "Ekin_comp", "Emag_comp", "Enstrophy_comp", "Mag_Enstrophy", "Entropy_comp",
+                    "DR_u", "DR_S", "DR_Sd", "DR_p", "max_Vorticity", "max_Current", "mean_temperature",
+                    "uprime")
+ANALYSIS_VECTORS = ("L2", "L2colloc", "Linf", "bulk", "bulk_t", "maxabsUt")
+
+
+class Analysis(C.Structure):
+    _fields_ = ([("vol", C.c_double)] + [(k, C.c_double * 9) for k in ANALYSIS_VECTORS] +
+                [(k, C.c_double) for k in ANALYSIS_SCALARS])
+
+
+def analyze_cfg(exact_func=0, n_analyze=0, ref_state=None, adv_vel=(1.0, 1.0, 1.0), wave_number=(1.0, 1.0, 1.0),
+                amplitude=0.1, R=287.058, diss_factor=1.0):
+    c = AnalyzeCfg()
+    c.exactFunc, c.nAnalyze = exact_func, n_analyze
+    if ref_state is not None:
+        for i, v in enumerate(ref_state):
+            c.refState[i] = v
+    for i in range(3):
+        c.advVel[i], c.waveNumber[i] = adv_vel[i], wave_number[i]
+    c.amplitude, c.R, c.dissFactor = amplitude, R, diss_factor
+    return c
+
+
+def analysis_dict(a, nVar):
+    out = {"vol": a.vol}
+    for k in ANALYSIS_VECTORS:
+        out[k] = np.array(getattr(a, k)[:nVar])
+    for k in ANALYSIS_SCALARS:
+        out[k] = getattr(a, k)
+    return out
+
+
+def analyze(mesh, U, Ut, grads=None, kappa=1.4, mu_0=1.0, t=0.0, **cfg):
+    """The reference's analysis of state U (with its RHS Ut and lifted primitive gradients
+    grads = (gx, gy, gz) or None): error norms against cfg's exact function, bulk integrals,
+    the Taylor-Green vortex quantities.  nVar from U's last axis (5 or 9)."""
+    U, Ut = np.ascontiguousarray(U), np.ascontiguousarray(Ut)
+    nVar = U.shape[-1]
+    keep = [np.ascontiguousarray(g) for g in grads] if grads is not None else None
+    gp = [g.ctypes.data for g in keep] if keep else [None, None, None]
+    c, a = analyze_cfg(**cfg), Analysis()
+    lib().ora_analyze(mesh["N"], nVar, mesh["nElems"], np.ascontiguousarray(mesh["Elem_xGP"]),
+                      np.ascontiguousarray(mesh["sJ"]), U, Ut, gp[0], gp[1], gp[2], kappa, mu_0, t,
+                      C.byref(c), C.byref(a))
+    return analysis_dict(a, nVar)
 
 
 def exactfunc(which, x, kappa=1.4, advvel=(1.0, 1.0, 1.0), amp=0.1, t=0.0):

@@ -33,14 +33,23 @@ build_variant() {
   [ "$eqn" = mhd ] && others='navierstokes|maxwell|linearscalaradvection'
   local lift=br2
   case "$defs" in *-DPP_Lifting=2*) lift=br1 ;; esac
-  local exclude="/amr/|/equation/($others)/|/testcase/(taylorgreenvortex|mhd_equilibrium|ns_angularmomentum)/|fluxo_post|/fluxo\.f90|/dg/$lift/"
+  # FLUXO_TESTCASE (CMakeLists.txt:101-103): the testcase directory TC (default: default)
+  local tc=${TC:-default}
+  local tcs
+  tcs=$(printf '%s\n' default taylorgreenvortex mhd_equilibrium ns_angularmomentum | grep -v -x "$tc" | paste -sd'|')
+  local exclude="/amr/|/equation/($others)/|/testcase/($tcs)/|fluxo_post|/fluxo\.f90|/dg/$lift/"
   find "$REF" -name "*.f90" | grep -v -E "$exclude" | sort > "$obj/todo.txt"
-  local pass n f b
+  local pass n f b extra
   for pass in $(seq 1 15); do
     : > "$obj/next.txt"; n=0
     while read -r f; do
       b=$(basename "$f" .f90)
-      if [ -f "$obj/$b.o" ] || $FC $COMMON $defs -module-dir "$obj" -c "$f" -o "$obj/$b.o" 2> "$obj/$b.err"; then
+      # taylorgreenvortex/testcase_analyze.f90 tests '#ifdef MPI' where the rest of the code
+      # tests '#if MPI'; with the serial build's MPI=0 it would take its MPI branch, so it is
+      # compiled with MPI undefined, which selects its serial branch
+      extra=""
+      case "$f" in */taylorgreenvortex/testcase_analyze.f90) extra="-UMPI" ;; esac
+      if [ -f "$obj/$b.o" ] || $FC $COMMON $defs $extra -module-dir "$obj" -c "$f" -o "$obj/$b.o" 2> "$obj/$b.err"; then
         n=$((n+1))
       else
         echo "$f" >> "$obj/next.txt"
@@ -72,5 +81,9 @@ build_variant mhd_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
 # resistive GLM-MHD: viscosity, resistivity and heat conduction through BR1 lifting of the
 # entropy variables (FLUXO_PARABOLIC, CMakeLists.txt:253-283)
 build_variant mhd_br1_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3
+# the same with the Taylor-Green vortex testcase (FLUXO_TESTCASE=taylorgreenvortex), whose
+# AnalyzeTestcase integrates kinetic and magnetic energy, enstrophy and dissipation rates
+TC=taylorgreenvortex build_variant mhd_br1_vf10_tgv $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 \
+  -DPP_Lifting_Var=3
 # NFVSE shock capturing (FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE, CMakeLists.txt:316-331)
 build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0

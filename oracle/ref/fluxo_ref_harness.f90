@@ -7,6 +7,10 @@
 !> HOPR file -- side counts and ranges, ElemToSide/SideToElem, buildMappings, metric array
 !> allocation and CalcMetrics -- and the TimeStepByLSERKW2 loop (src/timedisc/timedisc.f90:
 !> 424-480, with CALCTIMESTEP before every step); everything else is the reference's code.
+!> The reference's Analyze (src/analyze/analyze.f90:370-540: CalcErrorNorms, CalcBulk,
+!> AnalyzeEquation, AnalyzeTestcase) runs once on the input state after its right-hand side;
+!> the values it collects for AnalyzeToFile are written as 'name value' lines to analysis.txt
+!> in the working directory.
 !>
 !> usage: fluxo_ref_harness parameter.ini input.bin output.bin
 !==================================================================================================
@@ -18,7 +22,7 @@ program fluxo_ref_harness
   use MOD_Output,              only: DefineParametersOutput
   use MOD_ReadInTools,         only: prms
   use MOD_Interpolation,       only: DefineParametersInterpolation, InitInterpolation, GetVandermonde
-  use MOD_Interpolation_Vars,  only: NodeType, NodeTypeGL, NodeTypeEQ, wGP
+  use MOD_Interpolation_Vars,  only: NodeType, NodeTypeGL, NodeTypeEQ
   use MOD_Mortar,              only: DefineParametersMortar, InitMortarBase, InitMortar
   use MOD_Equation,            only: DefineParametersEquation, InitEquation
   use MOD_Testcase,            only: DefineParametersTestcase, InitTestcase
@@ -37,8 +41,8 @@ program fluxo_ref_harness
   use MOD_TimeDisc_Vars,       only: DFLScale, DFLScaleAlpha, RelativeDFL
 #endif
   use MOD_CalcTimeStep,        only: CALCTIMESTEP
-  use MOD_Analyze_Vars,        only: wGPVol
-  use MOD_AnalyzeEquation,     only: DefineParametersAnalyzeEquation, InitAnalyzeEquation, AnalyzeEquation
+  use MOD_Analyze,             only: DefineParametersAnalyze, InitAnalyze, Analyze
+  use MOD_Analyze_Vars,        only: A2F_nVars, A2F_VarNames, A2F_Data
   use MOD_Vector,              only: VCopy, VAXPBY
 #if SHOCK_NFVSE
   use MOD_ShockCapturing,      only: DefineParametersShockCapturing, InitShockCapturing
@@ -50,7 +54,7 @@ program fluxo_ref_harness
   implicit none
   character(len=255) :: fpar
   character(len=512) :: fin, fout
-  integer :: hdr(8), nSteps, iStep, iStage, errType, i, j, k, u_in, u_out
+  integer :: hdr(8), nSteps, iStep, iStage, errType, u_in, u_out, u_an, iv
   real :: rhdr(4), cflIn, dflIn, t, dt, tStage
   real, allocatable :: Utmp(:, :, :, :, :), b_dt(:)
 
@@ -64,7 +68,7 @@ program fluxo_ref_harness
   call DefineParametersMortar()
   call DefineParametersEquation()
   call DefineParametersTestcase()
-  call DefineParametersAnalyzeEquation()
+  call DefineParametersAnalyze()
 #if PARABOLIC
   call DefineParametersLifting()
 #endif
@@ -139,7 +143,7 @@ program fluxo_ref_harness
   call InitLifting()
 #endif
   call InitTestcase()
-  call InitAnalyzeEquation()
+  call InitAnalyze()
 #if SHOCK_NFVSE
   call InitShockCapturing()
 #endif
@@ -161,12 +165,14 @@ program fluxo_ref_harness
 #if SHOCK_NFVSE
   write(u_out) alpha
 #endif
-  allocate(wGPVol(0:PP_N, 0:PP_N, 0:PP_N))
-  do k = 0, PP_N; do j = 0, PP_N; do i = 0, PP_N
-    wGPVol(i, j, k) = wGP(i)*wGP(j)*wGP(k)
-  end do; end do; end do
-  ! prints ' Entropy      :' and ' dSdU*Ut         :' on stdout (CalcEntropy is module-private)
-  call AnalyzeEquation(t)
+  ! error norms, bulk, entropy (prints ' Entropy      :' and ' dSdU*Ut         :' on stdout;
+  ! CalcEntropy is module-private) and the testcase analysis of this state
+  call Analyze(t, 0_8)
+  open(newunit=u_an, file='analysis.txt', status='replace')
+  do iv = 5, A2F_nVars
+    write(u_an, '(A,1X,ES25.17)') trim(A2F_VarNames(iv)), A2F_Data(iv)
+  end do
+  close(u_an)
   ! CFL/DFL scaling of FillCFL_DFL (timedisc.f90:697-741) with the RK4-5 tables
   call SetTimeDiscCoefs('carpenterrk4-5')
   CFLScale = cflIn*CFLScaleAlpha(MIN(15, PP_N))/(2.*PP_N + 1.)

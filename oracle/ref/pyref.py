@@ -27,23 +27,51 @@ def binary(parabolic, volflux, nfvse=False, mhd=False, lifting=1):
     return path
 
 
+def analyze_lines(analyze):
+    """parameter.ini lines of the reference's Analyze: bulk integrals and AnalyzeToFile on,
+    plus the given options (AnalyzeExactFunc, NAnalyze, AdvVel, IniAmplitude, ...)."""
+    lines = "CalcBulk = T\nAnalyzeToFile = T\n"
+    for k, v in (analyze or {}).items():
+        if isinstance(v, (tuple, list)):
+            v = "(/" + ",".join(repr(float(x)) for x in v) + "/)"
+        lines += f"{k} = {v}\n"
+    return lines
+
+
+def read_analysis(d):
+    """The 'name value' lines the harness writes from the reference's AnalyzeToFile data."""
+    out = {}
+    path = os.path.join(d, "analysis.txt")
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                k, v = line.rsplit(None, 1)
+                out[k.strip().strip('"')] = float(v)
+    return out
+
+
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
-        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0):
+        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0, analyze=None):
     """Metrics, VolInt, RHS (+ gradients), entropy, dt and an nsteps trajectory computed by
     the reference's code on the given mesh (connectivity + NodeCoords) and state.
     nfvse: None, or dict of NFVSE parameters (ComputeAlpha, ShockBlendCoef, alpha_max,
     alpha_min, SpacePropSweeps, SpacePropFactor) for the shock-capturing build, which also
     returns the blending coefficient alpha of the RHS evaluation.  lifting 2: the BR2 build
-    with penalty etaBR2 (gradients returned are the volume gradients)."""
+    with penalty etaBR2 (gradients returned are the volume gradients).  out["analysis"]
+    holds what the reference's Analyze collects for this state at time t0 (error norms
+    against AnalyzeExactFunc, bulk integrals, entropy); `analyze` adds parameters to it."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
     exe = binary(parabolic, volflux, nfvse is not None, lifting=lifting)
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
-            f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = 1\n"
+            analyze = dict(analyze or {})
+            ini = analyze.pop("IniExactFunc", 1)
+            f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = {ini}\n"
                     f"IniRefState = 1\nRefState = (/1.,0.,0.,0.,1./)\nkappa = {kappa!r}\n"
                     f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\n")
+            f.write(analyze_lines(analyze))
             if lifting == 2:
                 f.write(f"etaBR2 = {etaBR2!r}\n")
             if nfvse is not None:
@@ -64,7 +92,8 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
         if p.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"reference harness failed ({p.returncode}):\n{p.stdout[-2000:]}\n{p.stderr[-2000:]}")
         raw = np.fromfile(fout, np.float64)
-    out, o = {}, 0
+        analysis = read_analysis(d)
+    out, o = {"analysis": analysis}, 0
 
     def take(name, shape):
         nonlocal o
@@ -100,24 +129,33 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
 
 
 def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600,
-            resistive=None, dfl=0.9, riemann=1):
+            resistive=None, dfl=0.9, riemann=1, analyze=None, tgv=False):
     """GLM-MHD (Derigs volume flux, surface flux `riemann` -- 0 central, 1 Rusanov, 10 Derigs +
     entropy-stable LLF, 11 Derigs -- Powell non-conservative terms) with
     the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before the first
     CALCTIMESTEP), dt with GLM_ch and GLM_dtch1 after it, and an nsteps trajectory.
     resistive: None (ideal), or dict(mu=, eta=, Pr=) for the BR1 build with viscosity,
     resistivity and heat conduction, which also returns the primitive gradients.
-    U: (nElems, n, n, n, 9)."""
+    U: (nElems, n, n, n, 9).  out["analysis"] as in run(); tgv=True runs the build with the
+    Taylor-Green vortex testcase (resistive only), whose AnalyzeTestcase adds energies,
+    enstrophies and dissipation rates."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
     par_on = resistive is not None
     exe = binary(par_on, 10, mhd=True)
+    if tgv:
+        exe = exe + "_tgv"
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
-            f.write(f"N = {N}\nRiemann = {int(riemann)}\nVolumeFlux = 10\nIniExactFunc = 1\nIniRefState = 1\n"
+            analyze = dict(analyze or {})
+            ini = analyze.pop("IniExactFunc", 1)
+            f.write(f"N = {N}\nRiemann = {int(riemann)}\nVolumeFlux = 10\nIniExactFunc = {ini}\nIniRefState = 1\n"
                     f"RefState = (/1.,0.,0.,0.,1.,0.,0.,0./)\nkappa = {kappa!r}\nmu_0 = {mu_0!r}\n"
                     f"GLM_scale = {glm_scale!r}\n")
+            f.write(analyze_lines(analyze))
+            if tgv:
+                f.write("doTCanalyze = T\n")
             if par_on:
                 f.write(f"mu = {float(resistive['mu'])!r}\neta = {float(resistive['eta'])!r}\n"
                         f"Pr = {float(resistive.get('Pr', 0.72))!r}\n")
@@ -134,7 +172,8 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
         if p.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"reference harness failed ({p.returncode}):\n{p.stdout[-2000:]}\n{p.stderr[-2000:]}")
         raw = np.fromfile(fout, np.float64)
-    out, o = {}, 0
+        analysis = read_analysis(d)
+    out, o = {"analysis": analysis}, 0
 
     def take(name, shape):
         nonlocal o

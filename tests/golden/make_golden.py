@@ -141,6 +141,60 @@ def make_mhd(case, riemann=1):
     print(f"{name}: dt0={r['dt0']:.6e} GLM_ch={r['glm_ch0']:.6f}")
 
 
+# The reference's Analyze of a state (CalcErrorNorms, CalcBulk, AnalyzeEquation and, for the
+# taylorgreenvortex build, AnalyzeTestcase): name, equations, K, N, warp, t0, options.
+# Euler/NS: box [0,2]^3 (density wave) or [-pi,pi]^3 (Taylor-Green), Ngeo 3; the state is the
+# initial condition ic with relative noise, exact function `exact` (2 density wave, 1 the
+# constant RefState (1,0,0,0,1) in primitive variables).  MHD: perturbed Orszag-Tang on [0,1]^3,
+# resistive (mu, eta, Pr), against the constant RefState (1,0,0,0,1,0,0,0).
+ANALYZE_CASES = [
+    ("euler_n3_density_wave_t025", "ns", 3, 3, True, 0.25, dict(ic=2, noise=1e-3, exact=2)),
+    ("euler_n2_density_wave_nanalyze5", "ns", 3, 2, True, 0.0, dict(ic=2, noise=1e-3, exact=2, nanalyze=5)),
+    ("ns_n3_tgv_br1_constant_ref", "ns", 2, 3, True, 0.0, dict(ic=7, noise=1e-3, exact=1, par=True)),
+    ("mhd_resistive_n3_tgv_analysis", "mhd", 2, 3, True, 0.0, dict(res=(0.01, 0.02, 0.72))),
+    ("mhd_resistive_n4_tgv_analysis", "mhd", 2, 4, True, 0.1, dict(res=(0.05, 0.01, 0.9))),
+]
+
+
+def make_analyze(case):
+    name, eq, K, N, warp, t0, o = case
+    rng = np.random.default_rng(20261203)
+    if eq == "ns":
+        tgv = o["ic"] == 7
+        x0, L = ((-math.pi,) * 3, (2 * math.pi,) * 3) if tgv else ((0.0,) * 3, (2.0,) * 3)
+        m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=warp)
+        U = O.exactfunc(o["ic"], m["Elem_xGP"])
+        U = U * (1.0 + o["noise"] * rng.uniform(-1.0, 1.0, U.shape))
+        par = o.get("par", False)
+        an = dict(AnalyzeExactFunc=o["exact"])
+        if o["exact"] == 2:
+            an.update(IniExactFunc=2, AdvVel=(1.0, 1.0, 1.0), IniAmplitude=0.1)
+        if "nanalyze" in o:
+            an["NAnalyze"] = o["nanalyze"]
+        mu0 = 1.0 / 1600 if par else 0.0
+        r = R.run(m, U, volflux=32, riemann=1, parabolic=par, mu0=mu0, t0=t0, analyze=an)
+        out = dict(eq="ns", parabolic=int(par), mu0=mu0, Pr=0.72, kappa=1.4, exact=o["exact"],
+                   nanalyze=o.get("nanalyze", 0))
+    else:
+        x0, L = (0.0,) * 3, (1.0,) * 3
+        m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=warp)
+        U = O.MhdProblem(m).exact_orszag_tang()
+        U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
+        U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
+        U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
+        mu, eta, Pr = o["res"]
+        r = R.run_mhd(m, U, resistive=dict(mu=mu, eta=eta, Pr=Pr), t0=t0, tgv=True,
+                      analyze=dict(AnalyzeExactFunc=1))
+        out = dict(eq="mhd", parabolic=1, mu=mu, eta=eta, Pr=Pr, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5,
+                   exact=1, nanalyze=0)
+    names = [k for k in r["analysis"] if k not in ("DOF", "nElems", "nRanks", "-dEkin/dt", "-dEmag/dt")]
+    out.update(K=K, N=N, Ngeo=3, warp=int(warp), x0=np.array(x0, float), L=np.array(L, float), t0=t0, U=U,
+               Ut=r["Ut"], names=np.array(names), values=np.array([r["analysis"][k] for k in names]))
+    os.makedirs(os.path.join(HERE, "analyze"), exist_ok=True)
+    np.savez(os.path.join(HERE, "analyze", name + ".npz"), **out)
+    print(f"{name}: {len(names)} analysis values")
+
+
 def make_nfvse(case):
     from fluxo_amd.workloads import sedov_blast
     name, K, N, sigma, p0, amax, fixed, nsteps = case
@@ -172,3 +226,5 @@ if __name__ == "__main__":
         make_mhd(c)
     for name, K, N, Ngeo, warp, nsteps, rs, res in MHD_RIEMANN_CASES:
         make_mhd((name, K, N, Ngeo, warp, nsteps) + (tuple(res) if res else ()), riemann=rs)
+    for c in ANALYZE_CASES:
+        make_analyze(c)
