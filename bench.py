@@ -452,13 +452,23 @@ def main():
         if err != 0:
             raise RuntimeError(f"CALCTIMESTEP reported errType={err}")
         U_end = eng.download()
+        # the reference's TGV analysis of the final state on the device (outside the timed steps)
+        ta = time.perf_counter()
+        an = eng.analyze(t_end, diss_factor=1.0 / 1600)
+        eng.sync()
+        an_ms = (time.perf_counter() - ta) * 1e3
+        analysis = {k: an[k] for k in ("Ekin", "Ekin_comp", "Enstrophy_comp", "DR_S", "DR_Sd", "DR_p",
+                                       "max_Vorticity", "Entropy_comp")}
+        analysis.update(vol=an["vol"], wall_ms=round(an_ms, 3),
+                        note="fx_analyze at t_end: RHS + lifting + testcase integrals on the device, "
+                             "dissipation rates with mu0")
         reps = 20
         ms = {"volume": eng.profile_kernel(FX_KERNEL_VOLUME_RK, reps),
               "surface": eng.profile_kernel(FX_KERNEL_SURFACE, reps),
               "lifting": eng.profile_kernel(FX_KERNEL_LIFTING, reps),
               "calc_dt": eng.profile_kernel(FX_KERNEL_DT, reps)}
         eng.close()
-        return dict(el=el, t_end=t_end, dt_last=dt_last, U=U_end, ms=ms,
+        return dict(el=el, t_end=t_end, dt_last=dt_last, U=U_end, ms=ms, analysis=analysis,
                     value=nodes * 5 * args.steps / el,
                     tiles={"chosen_elems_per_workgroup": tile,
                            "ms_by_tile": {str(k): round(v, 4) for k, v in tile_ms.items()}})
@@ -519,6 +529,7 @@ def main():
                    "nElems": mesh["nElems"], "nodes": nodes, "N": N, "precision": "FP64",
                    "t_end": t_end, "dt_last": dt_last},
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "tgv_analysis": r64["analysis"],
         "volume_tile": r64["tiles"],
         "pid_s_per_dof_timestep": 5.0 / value,
         "fp32_split": fp32,

@@ -26,7 +26,32 @@ module fluxo_gpu_bind
     type(c_ptr) :: SurfElem = c_null_ptr
   end type fx_mesh
 
+  type, bind(C) :: fx_analyze_config
+    integer(c_int) :: exactFunc = 0, nAnalyze = 0
+    real(c_double) :: refState(9) = 0.0d0, advVel(3) = 1.0d0, waveNumber(3) = 1.0d0
+    real(c_double) :: amplitude = 0.1d0, R = 287.058d0, dissFactor = 1.0d0
+  end type fx_analyze_config
+
+  type, bind(C) :: fx_analysis
+    real(c_double) :: vol
+    real(c_double) :: L2(9), L2colloc(9), Linf(9), bulk(9), bulk_t(9), maxabs_Ut(9)
+    real(c_double) :: Ekin, Ekin_comp, Emag_comp, Enstrophy_comp, Mag_Enstrophy, Entropy_comp
+    real(c_double) :: DR_u, DR_S, DR_Sd, DR_p, max_Vorticity, max_Current, mean_temperature, uprime
+  end type fx_analysis
+
   interface
+    integer(c_int) function fx_elem_xgp_upload(ctx, xgp) bind(C, name="fx_elem_xgp_upload")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: xgp(*)
+    end function fx_elem_xgp_upload
+    integer(c_int) function fx_analyze(ctx, t, cfg, res) bind(C, name="fx_analyze")
+      import :: c_int, c_ptr, c_double, fx_analyze_config, fx_analysis
+      type(c_ptr), value :: ctx
+      real(c_double), value :: t
+      type(fx_analyze_config), intent(in) :: cfg
+      type(fx_analysis), intent(out) :: res
+    end function fx_analyze
     integer(c_int) function fx_ctx_create(cfg, ctx) bind(C, name="fx_ctx_create")
       import :: c_int, c_ptr, fx_config
       type(fx_config), intent(in) :: cfg

@@ -26,7 +26,7 @@ program fluxo_gpu
   integer :: u_out
   type(fx_config) :: c
   integer :: KB(3), Ngeo, E, S, ierr, errType, iter, maxIter, nChunk, n
-  real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(5), Linf(5)
+  real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(5), L2c(5), Linf(5)
   real(rk) :: xg(0:15), wg(0:15), t0, t1, rate, A(5), b(5), RKc(5)
   integer(c_int), allocatable :: e2s(:, :, :), s2e(:, :)
   real(rk), allocatable :: NC(:, :, :, :, :), Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
@@ -141,9 +141,10 @@ program fluxo_gpu
   write(*, '(A,I0,A,ES22.14,A,ES14.6)') ' iter=', iter, ' t=', t, ' dt=', dt
   write(*, '(A,ES22.14,A,ES22.14)') ' Entropy=', Ssum, ' dSdU*Ut=', dSdU
   if (IniExactFunc == 2) then
-    call CalcErrorNorms(t, wg(0:c%N), L2, Linf)
-    write(*, '(A,5ES14.6)') ' L_2   =', L2
-    write(*, '(A,5ES14.6)') ' L_inf =', Linf
+    call CalcErrorNorms(t, L2, L2c, Linf)
+    write(*, '(A14,5ES21.12)') ' L_2        : ', L2
+    write(*, '(A14,5ES21.12)') ' L_2_colloc : ', L2c
+    write(*, '(A14,5ES21.12)') ' L_inf      : ', Linf
   end if
   write(*, '(A,ES14.6,A,ES14.6)') ' DOF-updates/s=', rate, '  PID [s/(DOF*timestep)]=', 5.0_rk/rate
   if (has_key('StateFile')) then

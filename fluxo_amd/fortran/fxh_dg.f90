@@ -10,6 +10,7 @@
 !>   TimeStepByLSERKW2(t)            src/timedisc/timedisc.f90:424-480 (device-resident U)
 !>   CalcEntropy(S,dSdU_Ut)          src/equation/navierstokes/analyze_equation.f90:549-599
 !>   ExactFunc(2|7)                  src/equation/navierstokes/equation.f90:632-651, 802-811
+!>   CalcErrorNorms(t,L2,L2c,Linf)   src/analyze/analyze.f90:557-628 (on the device, fx_analyze)
 !> Module variables play the role of MOD_DG_Vars / MOD_Mesh_Vars / MOD_TimeDisc_Vars.
 !==================================================================================================
 module fxh_dg
@@ -64,6 +65,7 @@ contains
     allocate(U(5, 0:PP_N, 0:PP_N, 0:PP_N, nElems), Ut(5, 0:PP_N, 0:PP_N, 0:PP_N, nElems))
     allocate(Elem_xGP(3, 0:PP_N, 0:PP_N, 0:PP_N, nElems), sJ(0:PP_N, 0:PP_N, 0:PP_N, nElems))
     Elem_xGP = xGP; sJ = sJin
+    call chk(fx_elem_xgp_upload(ctx, Elem_xGP), 'fx_elem_xgp_upload')
     U = 0.0_rk; Ut = 0.0_rk
   end subroutine InitDG_GPU
 
@@ -186,28 +188,19 @@ contains
     end do
   end subroutine FillIni
 
-  !> L2 / Linf errors of the host U against ExactFunc at time t, on the volume nodes with the
-  !> LGL quadrature (the reference's CalcErrorNorms interpolates to NAnalyze first).
-  subroutine CalcErrorNorms(t, w, L2, Linf)
-    real(rk), intent(in) :: t, w(0:PP_N)
-    real(rk), intent(out) :: L2(5), Linf(5)
-    real(rk) :: ex(5), vol, wv
-    integer :: i, j, k, e
-    L2 = 0.0_rk; Linf = 0.0_rk; vol = 0.0_rk
-    do e = 1, nElems
-      do k = 0, PP_N
-        do j = 0, PP_N
-          do i = 0, PP_N
-            call ExactFunc(IniExactFunc, t, Elem_xGP(:, i, j, k, e), ex)
-            wv = w(i)*w(j)*w(k)/sJ(i, j, k, e)
-            L2 = L2 + (U(:, i, j, k, e) - ex)**2*wv
-            Linf = max(Linf, abs(U(:, i, j, k, e) - ex))
-            vol = vol + wv
-          end do
-        end do
-      end do
-    end do
-    L2 = sqrt(L2/vol)
+  !> CalcErrorNorms of the device state against ExactFunc IniExactFunc (2, the density wave)
+  !> at time t: L2 on NAnalyze = 2(N+1) Gauss-Lobatto points, L2 on the collocation points,
+  !> Linf; computed on the device (fx_analyze), only the norms come back.
+  subroutine CalcErrorNorms(t, L2, L2colloc, Linf)
+    real(rk), intent(in) :: t
+    real(rk), intent(out) :: L2(5), L2colloc(5), Linf(5)
+    type(fx_analyze_config) :: ac
+    type(fx_analysis) :: res
+    ac%exactFunc = IniExactFunc
+    ac%advVel = AdvVel
+    ac%amplitude = IniAmplitude
+    call chk(fx_analyze(ctx, t, ac, res), 'fx_analyze')
+    L2 = res%L2(1:5); L2colloc = res%L2colloc(1:5); Linf = res%Linf(1:5)
   end subroutine CalcErrorNorms
 
 end module fxh_dg

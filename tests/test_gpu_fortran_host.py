@@ -49,3 +49,38 @@ def test_fortran_host_matches_oracle(tmp_path, resident, lifting):
     print(f"  {STEPS} steps relL2 vs oracle {err:.3e}")
     assert err <= 1e-10
     assert f"iter={STEPS}" in p.stdout.replace(" ", "")
+
+
+def test_fortran_host_error_norms(tmp_path):
+    """Euler density wave (IniExactFunc 2) for 10 steps: the driver prints the reference's
+    CalcErrorNorms (L2 on NAnalyze points, L2 on the nodes, Linf; fx_analyze on the device)
+    of the final state; they match the oracle's analysis of its own trajectory to 1e-9."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    Kd, Nd, steps = 3, 4, 10
+    ini = tmp_path / "parameter.ini"
+    ini.write_text(
+        f"N = {Nd}\nVolumeFlux = 32\nRiemann = 1\nkappa = 1.4\nIniExactFunc = 2\nAdvVel = (/1.,1.,1./)\n"
+        f"IniAmplitude = 0.1\nCFLscale = 0.9\nmaxIter = {steps}\nBoxElems = (/{Kd},{Kd},{Kd}/)\n"
+        "BoxCorner = (/0.,0.,0./)\nBoxLength = (/2.,2.,2./)\nNgeo = 3\nmeshdeform = T\nmeshdeformAmp = 0.1\n")
+    print("fluxo_gpu density wave error norms", flush=True)
+    p = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    got = {}
+    for line in p.stdout.splitlines():
+        for key in ("L_2_colloc", "L_2", "L_inf"):
+            if line.strip().startswith(key + " "):
+                got[key] = np.array([float(x) for x in line.split(":")[1].split()])
+    m = O.box_mesh(Kd, Nd, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True, amp=0.1)
+    P = O.Problem(m, volflux=32, riemann=1)
+    Ue = O.exactfunc(2, m["Elem_xGP"])
+    Utmp, t = np.zeros_like(Ue), 0.0
+    for _ in range(steps):
+        d, _ = P.calc_dt(Ue, 0.9, 0.9)
+        P.lserk_step(Ue, Utmp, t, d)
+        t += d
+    a = O.analyze(m, Ue, P.rhs(Ue), exact_func=2, t=t)
+    for key, field in (("L_2", "L2"), ("L_2_colloc", "L2colloc"), ("L_inf", "Linf")):
+        err = np.abs(got[key] - a[field]) / np.abs(a[field])
+        print(f"  {key}: {got[key][0]:.6e} (oracle {a[field][0]:.6e}), max rel diff {err.max():.1e}")
+        assert np.all(err <= 1e-9)
