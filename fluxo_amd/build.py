@@ -54,7 +54,23 @@ def build_host(verbose=False):
     os.replace(exe + ".tmp", exe)
 
 
+HDF5_PREFIX = os.environ.get("HDF5_PREFIX", "/opt/conda")
+
+
+def build_io(verbose=False):
+    """libfluxo_io.so: HOPR mesh and state files over the HDF5 C library (host code)."""
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"HDF5 headers not found under {inc} (set HDF5_PREFIX)")
+    out = os.path.join(HERE, "libfluxo_io.so")
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I" + inc, os.path.join(HERE, "io", "fx_io.cpp"),
+          "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib, "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(verbose=False):
     build_engine(verbose)
+    build_io(verbose)
     build_host(verbose)
     build_oracle(verbose)

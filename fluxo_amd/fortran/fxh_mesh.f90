@@ -153,8 +153,9 @@ contains
 
   !> CalcMetrics + CalcSurfMetrics; ierr=1 for non-positive or badly scaled Jacobians.
   subroutine calc_metrics(N, Ngeo, nElems, nSides, NodeCoords, e2s, Mf, Mg, Mh, sJ, NormVec, &
-                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr)
+                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr, nodesGL)
     integer, intent(in) :: N, Ngeo, nElems, nSides
+    logical, intent(in), optional :: nodesGL   ! NodeCoords on Gauss-Lobatto instead of equidistant nodes
     real(rk), intent(in) :: NodeCoords(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, nElems)
     integer, intent(in) :: e2s(2, 6, nElems)
     real(rk), intent(out), dimension(3, 0:N, 0:N, 0:N, nElems) :: Mf, Mg, Mh, Elem_xGP
@@ -178,6 +179,9 @@ contains
     allocate(dXref(3, 3, 0:NgeoRef, 0:NgeoRef, 0:NgeoRef), detRef(1, 0:NgeoRef, 0:NgeoRef, 0:NgeoRef))
     allocate(sl_ref(3, 0:NgeoRef, 0:NgeoRef, 0:NgeoRef))
     call vandermonde(Ngeo, NODE_EQUI, Ngeo, NODE_GL, .false., V_eq_gl)
+    if (present(nodesGL)) then
+      if (nodesGL) call vandermonde(Ngeo, NODE_GL, Ngeo, NODE_GL, .false., V_eq_gl)
+    end if
     call nodes_of(Ngeo, NODE_GL, xg)
     call derivative_matrix(Ngeo, xg, DGLgeo)
     call vandermonde(Ngeo, NODE_GL, NgeoRef, NODE_GL, .false., V_gl_ref)
@@ -335,5 +339,26 @@ contains
                       SurfElem, Elem_xGP, ie)
     ierr = ie
   end subroutine fxh_box_mesh
+
+  !> C entry for a mesh read from a file (HOPR reader, include/fluxo_io.h): the metric arrays
+  !> of CalcMetrics / CalcSurfMetrics for given connectivity and node coordinates.
+  subroutine fxh_metrics(N, Ngeo, nElems, nodesGL, e2s, NodeCoords, Mf, Mg, Mh, sJ, NormVec, TangVec1, &
+                         TangVec2, SurfElem, Elem_xGP, ierr) bind(C, name="fxh_metrics")
+    integer(c_int), value :: N, Ngeo, nElems, nodesGL
+    integer(c_int), intent(in) :: e2s(2, 6, nElems)
+    real(c_double), intent(in) :: NodeCoords(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, nElems)
+    real(c_double), intent(out), dimension(3, 0:N, 0:N, 0:N, nElems) :: Mf, Mg, Mh, Elem_xGP
+    real(c_double), intent(out) :: sJ(0:N, 0:N, 0:N, nElems)
+    real(c_double), intent(out), dimension(3, 0:N, 0:N, 3*nElems) :: NormVec, TangVec1, TangVec2
+    real(c_double), intent(out) :: SurfElem(0:N, 0:N, 3*nElems)
+    integer(c_int), intent(out) :: ierr
+    integer :: ie
+    if (N < 1 .or. Ngeo < 1 .or. nElems < 1) then
+      ierr = -1; return
+    end if
+    call calc_metrics(N, Ngeo, nElems, 3*nElems, NodeCoords, e2s, Mf, Mg, Mh, sJ, NormVec, TangVec1, &
+                      TangVec2, SurfElem, Elem_xGP, ie, nodesGL /= 0)
+    ierr = ie
+  end subroutine fxh_metrics
 
 end module fxh_mesh

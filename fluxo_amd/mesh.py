@@ -1,4 +1,5 @@
-"""Periodic box meshes built by the Fortran host (libfluxo_host.so, fxh_box_mesh).
+"""Meshes for the engine: periodic box meshes built by the Fortran host (libfluxo_host.so,
+fxh_box_mesh), and HOPR mesh files (fluxo_amd.io) with the host's metrics (fxh_metrics).
 
 Returns the arrays the engine's fx_mesh_upload takes, in the reference layout (numpy C
 order with the Fortran index order reversed), plus NodeCoords and the volume node
@@ -26,6 +27,8 @@ def _lib():
         L.fxh_box_mesh.argtypes = [ci, ci, ci, ci, ci, dp, dp, ci, cd, ip, ip] + [dp] * 10 + [
             C.POINTER(ci)]
         L.fxh_box_mesh.restype = None
+        L.fxh_metrics.argtypes = [ci, ci, ci, ci, ip, dp] + [dp] * 9 + [C.POINTER(ci)]
+        L.fxh_metrics.restype = None
         _LIB = L
     return _LIB
 
@@ -51,4 +54,27 @@ def box_mesh(K, N, x0=(0.0, 0.0, 0.0), length=(2.0, 2.0, 2.0), Ngeo=1, warp=Fals
                         m["TangVec2"], m["SurfElem"], m["Elem_xGP"], C.byref(ierr))
     if ierr.value != 0:
         raise ValueError(f"box mesh failed (ierr={ierr.value}): non-positive or badly scaled Jacobian")
+    return m
+
+
+def hopr_mesh(path, N):
+    """A conforming, periodic HOPR mesh file (the reference's MeshFile) for polynomial degree
+    N: the reader's connectivity and node coordinates, the host's CalcMetrics/CalcSurfMetrics."""
+    from fluxo_amd import io
+    r = io.read_mesh(path)
+    E, S, Ngeo, n = r["nElems"], r["nSides"], r["Ngeo"], N + 1
+    m = dict(N=N, Ngeo=Ngeo, nElems=E, nSides=S, MeshFile=os.fspath(path))
+    m.update(ElemToSide=r["ElemToSide"], SideToElem=r["SideToElem"], NodeCoords=r["NodeCoords"])
+    for k in ("Mf", "Mg", "Mh", "Elem_xGP"):
+        m[k] = np.zeros((E, n, n, n, 3))
+    m["sJ"] = np.zeros((E, n, n, n))
+    for k in ("NormVec", "TangVec1", "TangVec2"):
+        m[k] = np.zeros((S, n, n, 3))
+    m["SurfElem"] = np.zeros((S, n, n))
+    ierr = C.c_int(0)
+    _lib().fxh_metrics(N, Ngeo, E, int(r["nodes_gl"]), m["ElemToSide"], np.ascontiguousarray(m["NodeCoords"]),
+                       m["Mf"], m["Mg"], m["Mh"], m["sJ"], m["NormVec"], m["TangVec1"], m["TangVec2"],
+                       m["SurfElem"], m["Elem_xGP"], C.byref(ierr))
+    if ierr.value != 0:
+        raise ValueError(f"{path}: metrics failed (ierr={ierr.value}): non-positive or badly scaled Jacobian")
     return m

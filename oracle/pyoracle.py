@@ -103,15 +103,26 @@ def box_mesh(K, N, x0=(0.0, 0.0, 0.0), length=(2.0, 2.0, 2.0), Ngeo=1, warp=Fals
     Kx, Ky, Kz = (K, K, K) if np.isscalar(K) else K
     E = Kx * Ky * Kz
     S = 3 * E
-    n, ng = N + 1, Ngeo + 1
+    ng = Ngeo + 1
     e2s = np.zeros((E, 6, 2), np.int32)
     s2e = np.zeros((S, 5), np.int32)
     assert L.ora_box_connectivity(Kx, Ky, Kz, e2s, s2e) == E
     X = np.zeros((E, ng, ng, ng, 3))
     L.ora_box_nodecoords(Kx, Ky, Kz, np.array(x0, float), np.array(length, float), Ngeo,
                          int(bool(warp)), amp, X)
-    m = dict(N=N, Ngeo=Ngeo, nElems=E, nSides=S, K=(Kx, Ky, Kz), ElemToSide=e2s, SideToElem=s2e,
-             NodeCoords=X)
+    m = mesh_from(N, Ngeo, e2s, s2e, X)
+    m["K"] = (Kx, Ky, Kz)
+    return m
+
+
+def mesh_from(N, Ngeo, e2s, s2e, X):
+    """FLUXO metrics (the oracle's CalcMetrics) of a conforming mesh given by its connectivity
+    and equidistant node coordinates, e.g. a HOPR file's."""
+    L = lib()
+    E, S, n = e2s.shape[0], s2e.shape[0], N + 1
+    e2s, s2e = np.ascontiguousarray(e2s, np.int32), np.ascontiguousarray(s2e, np.int32)
+    X = np.ascontiguousarray(X, np.float64)
+    m = dict(N=N, Ngeo=Ngeo, nElems=E, nSides=S, ElemToSide=e2s, SideToElem=s2e, NodeCoords=X)
     m["Mf"] = np.zeros((E, n, n, n, 3)); m["Mg"] = np.zeros((E, n, n, n, 3))
     m["Mh"] = np.zeros((E, n, n, n, 3)); m["sJ"] = np.zeros((E, n, n, n))
     for k in ("NormVec", "TangVec1", "TangVec2"):

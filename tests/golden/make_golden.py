@@ -195,6 +195,38 @@ def make_analyze(case):
     print(f"{name}: {len(names)} analysis values")
 
 
+# The reference's own HOPR meshes (copied as data to tests/golden/io): connectivity from
+# fluxo_amd.io's reader, metrics and everything else from the reference's code through the
+# harness.  name, mesh file, N, equations (volflux, riemann, parabolic), initial state, steps.
+HOPR_CASES = [
+    ("hopr_unstruct_n3_euler_dw", "ConformBoxUnstructPeriodicHeavilyWarped_Ngeo2_mesh.h5", 3, (32, 1, False), "dw", 5),
+    ("hopr_unstruct_n4_ns_dw", "ConformBoxUnstructPeriodicHeavilyWarped_Ngeo2_mesh.h5", 4, (32, 1, True), "dw", 3),
+    ("hopr_unstruct_n3_ec", "ConformBoxUnstructPeriodicHeavilyWarped_Ngeo2_mesh.h5", 3, (32, 32, False), "dw", 3),
+    ("hopr_warped_n3_euler_dw", "ConformBoxHeavilyWarped_Ngeo3__03_03_03_mesh.h5", 3, (5, 9, False), "dw", 3),
+    ("hopr_tgv_n3_ns", "TGV_004_004_004_mesh.h5", 3, (32, 1, True), "tgv", 3),
+]
+
+
+def make_hopr(case):
+    from fluxo_amd import io
+    name, mesh_file, N, (vf, rs, par), ic, nsteps = case
+    r = io.read_mesh(os.path.join(HERE, "io", mesh_file))
+    m = O.mesh_from(N, r["Ngeo"], r["ElemToSide"], r["SideToElem"], r["NodeCoords"])
+    U = O.exactfunc(2 if ic == "dw" else 7, m["Elem_xGP"])
+    U = U * (1.0 + 1e-3 * np.random.default_rng(20261204).uniform(-1.0, 1.0, U.shape))
+    mu0 = 1e-3 if par else 0.0
+    rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=par, mu0=mu0, Pr=0.72, cfl=0.9, dfl=0.9, nsteps=nsteps)
+    out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, parabolic=int(par), mu0=mu0, Pr=0.72, kappa=1.4,
+               cfl=0.9, dfl=0.9, nsteps=nsteps, U=U, Ut=rr["Ut"], dt0=rr["dt0"], errType0=rr["errType0"],
+               U_end=rr["U_end"], t_end=rr["t_end"], sJ=rr["sJ"], Mf=rr["Mf"], NormVec=rr["NormVec"],
+               SurfElem=rr["SurfElem"], Entropy=rr["Entropy"], dSdU_Ut=rr["dSdU_Ut"])
+    if par:
+        out.update(gradPx=rr["gradPx"], gradPy=rr["gradPy"], gradPz=rr["gradPz"])
+    os.makedirs(os.path.join(HERE, "hopr"), exist_ok=True)
+    np.savez(os.path.join(HERE, "hopr", name + ".npz"), **out)
+    print(f"{name}: dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} dSdU_Ut={rr['dSdU_Ut']}")
+
+
 def make_nfvse(case):
     from fluxo_amd.workloads import sedov_blast
     name, K, N, sigma, p0, amax, fixed, nsteps = case
@@ -228,3 +260,5 @@ if __name__ == "__main__":
         make_mhd((name, K, N, Ngeo, warp, nsteps) + (tuple(res) if res else ()), riemann=rs)
     for c in ANALYZE_CASES:
         make_analyze(c)
+    for c in HOPR_CASES:
+        make_hopr(c)
