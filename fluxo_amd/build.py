@@ -50,7 +50,7 @@ def build_host(verbose=False):
     _run([FLANG, "-shared", "-o", so + ".tmp", *objs[:-1], "-L" + HERE, "-lfluxo_gpu", rpath], verbose)
     os.replace(so + ".tmp", so)
     exe = os.path.join(HERE, "fluxo_gpu")
-    _run([FLANG, "-o", exe + ".tmp", *objs, "-L" + HERE, "-lfluxo_gpu", rpath], verbose)
+    _run([FLANG, "-o", exe + ".tmp", *objs, "-L" + HERE, "-lfluxo_gpu", "-lfluxo_io", rpath], verbose)
     os.replace(exe + ".tmp", exe)
 
 

@@ -39,6 +39,45 @@ module fluxo_gpu_bind
     real(c_double) :: DR_u, DR_S, DR_Sd, DR_p, max_Vorticity, max_Current, mean_temperature, uprime
   end type fx_analysis
 
+  ! libfluxo_io (include/fluxo_io.h): HOPR mesh files and state files
+  interface
+    integer(c_int) function fxio_mesh_info(file, nElems, nSides, Ngeo, nodeTypeGL) bind(C, name="fxio_mesh_info")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: file(*)
+      integer(c_int), intent(out) :: nElems, nSides, Ngeo, nodeTypeGL
+    end function fxio_mesh_info
+    integer(c_int) function fxio_mesh_read(file, e2s, s2e, NodeCoords) bind(C, name="fxio_mesh_read")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: file(*)
+      integer(c_int), intent(out) :: e2s(*), s2e(*)
+      real(c_double), intent(out) :: NodeCoords(*)
+    end function fxio_mesh_read
+    integer(c_int) function fxio_state_write(file, project, meshFile, N, nVar, nElems, time, varNames, U) &
+        bind(C, name="fxio_state_write")
+      import :: c_int, c_char, c_double, c_ptr
+      character(kind=c_char), intent(in) :: file(*), project(*), meshFile(*)
+      integer(c_int), value :: N, nVar, nElems
+      real(c_double), value :: time
+      type(c_ptr), intent(in) :: varNames(*)
+      real(c_double), intent(in) :: U(*)
+    end function fxio_state_write
+    integer(c_int) function fxio_state_info(file, N, nVar, nElems, time) bind(C, name="fxio_state_info")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: file(*)
+      integer(c_int), intent(out) :: N, nVar, nElems
+      real(c_double), intent(out) :: time
+    end function fxio_state_info
+    integer(c_int) function fxio_state_read(file, N, nVar, nElems, U) bind(C, name="fxio_state_read")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: file(*)
+      integer(c_int), value :: N, nVar, nElems
+      real(c_double), intent(out) :: U(*)
+    end function fxio_state_read
+    type(c_ptr) function fxio_last_error() bind(C, name="fxio_last_error")
+      import :: c_ptr
+    end function fxio_last_error
+  end interface
+
   interface
     integer(c_int) function fx_elem_xgp_upload(ctx, xgp) bind(C, name="fx_elem_xgp_upload")
       import :: c_int, c_ptr, c_double
@@ -160,5 +199,17 @@ contains
       msg(i:i) = p(i)
     end do
   end function fx_error_text
+
+  function fxio_error_text() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(fxio_last_error(), p, [512])
+    do i = 1, 512
+      if (p(i) == c_null_char) exit
+      msg(i:i) = p(i)
+    end do
+  end function fxio_error_text
 
 end module fluxo_gpu_bind

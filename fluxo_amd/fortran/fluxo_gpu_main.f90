@@ -2,28 +2,37 @@
 !> fluxo_gpu: parameter.ini-driven time loop on the MI355X engine (the reference's PROGRAM Fluxo,
 !> src/fluxo.f90, restricted to periodic box meshes, with TimeDisc timedisc.f90:144-415).
 !>
-!> Keys (reference meaning unless noted): N, Riemann, VolumeFlux, IniExactFunc, AdvVel,
-!> IniAmplitude, kappa, mu0, Pr, CFLscale, DFLscale, tEnd, maxIter, Analyze_dt.
+!> Keys (reference meaning unless noted): N, Riemann, VolumeFlux, IniExactFunc (1, 2, 7),
+!> RefState, AdvVel, IniAmplitude, kappa, mu0, Pr, CFLscale, DFLscale, tEnd, maxIter,
+!> Analyze_dt.
 !> Mesh keys (this driver's periodic box generator replaces MeshFile): BoxElems (3 ints),
 !> BoxCorner, BoxLength (3 reals each), Ngeo, meshdeform (logical), meshdeformAmp.
 !> Parabolic (logical) selects Navier-Stokes at run time, with BR1 lifting or, with Lifting = 2
 !> (the reference's compile-time PP_Lifting), BR2 with penalty etaBR2 (br2.f90:116, default 2).
 !> DeviceResident (logical, default T): T keeps U on the device and runs whole steps there;
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
-!> StateFile (optional): U(5,0:N,0:N,0:N,nElems) at the end, raw FP64 in Fortran order (the
-!> place of the reference's WriteState, hdf5_output.f90:74-165).
+!> MeshFile (optional): a HOPR mesh file (conforming, periodic boundaries; libfluxo_io) instead
+!> of the box generator.  RestartFile (optional): initial state from a state file (Restart,
+!> restart.f90:147-232; same N) instead of IniExactFunc, its Time as the start time.
+!> ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
+!> <ProjectName>_State_<time>.h5.  StateFile (optional): the final U(5,0:N,0:N,0:N,nElems) as
+!> raw FP64 in Fortran order.
 !==================================================================================================
 program fluxo_gpu
   use, intrinsic :: iso_c_binding
   use fxh_basis
   use fxh_mesh
   use fxh_readini
-  use fluxo_gpu_bind, only: fx_config
+  use fluxo_gpu_bind, only: fx_config, fxio_mesh_info, fxio_mesh_read, fxio_state_write, fxio_state_info, &
+                            fxio_state_read, fxio_error_text
   use fxh_dg
   implicit none
   character(len=512) :: fname
-  character(len=256) :: sfile
-  integer :: u_out
+  character(len=256) :: sfile, meshFile, project
+  character(len=32) :: tstamp
+  character(len=40), target :: vnames(5)
+  type(c_ptr) :: vptr(5)
+  integer :: u_out, nodeGL, rN, rVar, rE, iv
   type(fx_config) :: c
   integer :: KB(3), Ngeo, E, S, ierr, errType, iter, maxIter, nChunk, n
   real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(5), L2c(5), Linf(5)
@@ -54,38 +63,60 @@ program fluxo_gpu
   IniExactFunc = GETINT('IniExactFunc')
   AdvVel = GETREALARRAY('AdvVel', 3, '1.,1.,1.')
   IniAmplitude = GETREAL('IniAmplitude', '0.1')
+  if (has_key('RefState')) then  ! primitive (rho, v, p) -> RefStateCons (equation.f90:236-250)
+    A = GETREALARRAY('RefState', 5)
+    RefStateCons(1) = A(1); RefStateCons(2:4) = A(1)*A(2:4)
+    RefStateCons(5) = A(5)/(c%kappa - 1.0_rk) + 0.5_rk*A(1)*sum(A(2:4)**2)
+  end if
   CFLScale = GETREAL('CFLscale')
   DFLScale = GETREAL('DFLscale', '0.9')
   tEnd = GETREAL('tEnd', '1.0e30')
   maxIter = GETINT('maxIter', '100')
   Analyze_dt = GETREAL('Analyze_dt', '1.0e30')
   resident = GETLOGICAL('DeviceResident', 'T')
-  KB = GETINTARRAY('BoxElems', 3)
-  x0 = GETREALARRAY('BoxCorner', 3, '0.,0.,0.')
-  L = GETREALARRAY('BoxLength', 3, '2.,2.,2.')
-  Ngeo = GETINT('Ngeo', '1')
-  warp = GETLOGICAL('meshdeform', 'F')
-  amp = GETREAL('meshdeformAmp', '0.1')
-
-  E = KB(1)*KB(2)*KB(3); S = 3*E; n = c%N
+  n = c%N
+  nodeGL = 0
+  if (has_key('MeshFile')) then
+    ! ReadMesh / setLocalSideIDs / fillMeshInfo of a HOPR file (libfluxo_io)
+    meshFile = GETSTR('MeshFile')
+    call io_chk(fxio_mesh_info(trim(meshFile)//c_null_char, E, S, Ngeo, nodeGL), 'fxio_mesh_info')
+    allocate(e2s(2, 6, E), s2e(5, S), NC(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, E))
+    call io_chk(fxio_mesh_read(trim(meshFile)//c_null_char, e2s, s2e, NC), 'fxio_mesh_read')
+  else
+    meshFile = 'box'
+    KB = GETINTARRAY('BoxElems', 3)
+    x0 = GETREALARRAY('BoxCorner', 3, '0.,0.,0.')
+    L = GETREALARRAY('BoxLength', 3, '2.,2.,2.')
+    Ngeo = GETINT('Ngeo', '1')
+    warp = GETLOGICAL('meshdeform', 'F')
+    amp = GETREAL('meshdeformAmp', '0.1')
+    E = KB(1)*KB(2)*KB(3); S = 3*E
+    allocate(e2s(2, 6, E), s2e(5, S), NC(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, E))
+    call box_connectivity(KB, e2s, s2e)
+    call box_nodecoords(KB, x0, L, Ngeo, warp, amp, NC)
+  end if
   c%nElems = E; c%nSides = S
-  allocate(e2s(2, 6, E), s2e(5, S), NC(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, E))
   allocate(Mf(3, 0:n, 0:n, 0:n, E), Mg(3, 0:n, 0:n, 0:n, E), Mh(3, 0:n, 0:n, 0:n, E), xGP(3, 0:n, 0:n, 0:n, E))
   allocate(sJm(0:n, 0:n, 0:n, E), nv(3, 0:n, 0:n, S), t1v(3, 0:n, 0:n, S), t2v(3, 0:n, 0:n, S), se(0:n, 0:n, S))
-  call box_connectivity(KB, e2s, s2e)
-  call box_nodecoords(KB, x0, L, Ngeo, warp, amp, NC)
-  call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr)
+  call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0)
   if (ierr /= 0) then
     write(*, '(A)') ' ERROR: negative or badly scaled Jacobian in the generated mesh'
     stop 2
   end if
   call InitDG_GPU(c, e2s, s2e, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP)
   call lgl_nodes(n, xg(0:n), wg(0:n))
-  call FillIni()
+  t = 0.0_rk
+  if (has_key('RestartFile')) then
+    sfile = GETSTR('RestartFile')
+    call io_chk(fxio_state_info(trim(sfile)//c_null_char, rN, rVar, rE, t), 'fxio_state_info')
+    call io_chk(fxio_state_read(trim(sfile)//c_null_char, n, 5, E, U), 'fxio_state_read')
+    write(*, '(A,A,A,ES14.6)') ' restart from ', trim(sfile), ' at t=', t
+  else
+    call FillIni()
+  end if
   call StateToDevice()
   write(*, '(A,I0,A,I0,A,I0,A,I0)') ' fluxo_gpu: nElems=', E, ' N=', n, ' VolumeFlux=', c%volumeFlux, &
       ' Riemann=', c%riemann
-  t = 0.0_rk
   call CalcEntropy(Ssum, dSdU)
   write(*, '(A,ES22.14,A,ES22.14)') ' t=0 Entropy=', Ssum, ' dSdU*Ut=', dSdU
   tAnalyze = Analyze_dt
@@ -93,7 +124,7 @@ program fluxo_gpu
   call system_clock(c0, cr)
   if (resident) then
     do while (iter < maxIter .and. t < tEnd)
-      nChunk = maxIter - iter
+      nChunk = min(maxIter - iter, 100)  ! tEnd is checked between chunks (the last chunk may pass it)
       call TimeDiscRun(nChunk, t, errType)
       iter = iter + nChunk
       if (errType /= 0) then
@@ -140,13 +171,30 @@ program fluxo_gpu
   call CalcEntropy(Ssum, dSdU)
   write(*, '(A,I0,A,ES22.14,A,ES14.6)') ' iter=', iter, ' t=', t, ' dt=', dt
   write(*, '(A,ES22.14,A,ES22.14)') ' Entropy=', Ssum, ' dSdU*Ut=', dSdU
-  if (IniExactFunc == 2) then
+  if (IniExactFunc == 1 .or. IniExactFunc == 2) then
     call CalcErrorNorms(t, L2, L2c, Linf)
     write(*, '(A14,5ES21.12)') ' L_2        : ', L2
     write(*, '(A14,5ES21.12)') ' L_2_colloc : ', L2c
     write(*, '(A14,5ES21.12)') ' L_inf      : ', Linf
   end if
   write(*, '(A,ES14.6,A,ES14.6)') ' DOF-updates/s=', rate, '  PID [s/(DOF*timestep)]=', 5.0_rk/rate
+  if (has_key('ProjectName')) then
+    ! WriteState: <ProjectName>_State_<TIMESTAMP>.h5 (hdf5_output.f90:74-165, globals.f90:285-303)
+    project = GETSTR('ProjectName')
+    write(tstamp, '(F17.9)') t
+    do iv = 1, len_trim(tstamp)
+      if (tstamp(iv:iv) == ' ') tstamp(iv:iv) = '0'
+    end do
+    sfile = trim(project)//'_State_'//trim(tstamp)//'.h5'
+    vnames = [character(len=40) :: 'Density', 'MomentumX', 'MomentumY', 'MomentumZ', 'EnergyStagnationDensity']
+    do iv = 1, 5
+      vnames(iv) = trim(vnames(iv))//c_null_char
+      vptr(iv) = c_loc(vnames(iv))
+    end do
+    call io_chk(fxio_state_write(trim(sfile)//c_null_char, trim(project)//c_null_char, &
+                                 trim(meshFile)//c_null_char, n, 5, E, t, vptr, U), 'fxio_state_write')
+    write(*, '(A,A)') ' WRITE STATE TO HDF5 FILE: ', trim(sfile)
+  end if
   if (has_key('StateFile')) then
     sfile = GETSTR('StateFile')
     open(newunit=u_out, file=trim(sfile), access='stream', form='unformatted', status='replace')
@@ -155,4 +203,15 @@ program fluxo_gpu
   end if
   write(*, '(A)') ' FLUXO_GPU FINISHED'
   call FinalizeDG_GPU()
+
+contains
+
+  subroutine io_chk(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= 0) then
+      write(*, '(A,I0,A)') ' ERROR in '//where//' (code ', rc, '): '//trim(fxio_error_text())
+      stop 2
+    end if
+  end subroutine io_chk
 end program fluxo_gpu

@@ -23,7 +23,7 @@ module fxh_dg
   public :: CALCTIMESTEP, TimeStepByLSERKW2, TimeDiscRun, CalcEntropy, ExactFunc, FillIni
   public :: StateToHost, StateToDevice, CalcErrorNorms
   public :: U, Ut, gradPx, gradPy, gradPz, Elem_xGP, sJ, PP_N, nElems, nSides, dt, CFLScale, DFLScale
-  public :: Kappa, AdvVel, IniAmplitude, IniExactFunc, cfg
+  public :: Kappa, AdvVel, IniAmplitude, IniExactFunc, cfg, RefStateCons
 
   integer :: PP_N = 3, nElems = 0, nSides = 0, IniExactFunc = 2
   real(rk), allocatable, target :: U(:, :, :, :, :), Ut(:, :, :, :, :)
@@ -31,6 +31,7 @@ module fxh_dg
   real(rk), allocatable, target :: Elem_xGP(:, :, :, :, :), sJ(:, :, :, :)
   real(rk) :: dt = 0.0_rk, CFLScale = 0.9_rk, DFLScale = 0.9_rk
   real(rk) :: Kappa = 1.4_rk, AdvVel(3) = 1.0_rk, IniAmplitude = 0.1_rk
+  real(rk) :: RefStateCons(5) = [1.0_rk, 0.0_rk, 0.0_rk, 0.0_rk, 2.5_rk]   ! ExactFunc 1
   type(fx_config) :: cfg
   type(c_ptr) :: ctx = c_null_ptr
 
@@ -153,6 +154,8 @@ contains
     pi = acos(-1.0_rk)
     skm1 = 1.0_rk/(Kappa - 1.0_rk)
     select case (ExactFunction)
+    case (1)  ! constant RefState
+      Resu = RefStateCons
     case (7)  ! Taylor-Green vortex
       A = 1.0_rk; Ms = 0.1_rk
       prim(1) = 1.0_rk
@@ -188,15 +191,17 @@ contains
     end do
   end subroutine FillIni
 
-  !> CalcErrorNorms of the device state against ExactFunc IniExactFunc (2, the density wave)
-  !> at time t: L2 on NAnalyze = 2(N+1) Gauss-Lobatto points, L2 on the collocation points,
-  !> Linf; computed on the device (fx_analyze), only the norms come back.
+  !> CalcErrorNorms of the device state against ExactFunc IniExactFunc (1 the constant
+  !> RefState, 2 the density wave) at time t: L2 on NAnalyze = 2(N+1) Gauss-Lobatto points, L2
+  !> on the collocation points, Linf; computed on the device (fx_analyze), only the norms come
+  !> back.
   subroutine CalcErrorNorms(t, L2, L2colloc, Linf)
     real(rk), intent(in) :: t
     real(rk), intent(out) :: L2(5), L2colloc(5), Linf(5)
     type(fx_analyze_config) :: ac
     type(fx_analysis) :: res
     ac%exactFunc = IniExactFunc
+    ac%refState(1:5) = RefStateCons
     ac%advVel = AdvVel
     ac%amplitude = IniAmplitude
     call chk(fx_analyze(ctx, t, ac, res), 'fx_analyze')

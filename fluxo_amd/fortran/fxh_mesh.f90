@@ -18,7 +18,7 @@ module fxh_mesh
   use fxh_basis
   implicit none
   private
-  public :: box_connectivity, box_nodecoords, calc_metrics, fxh_box_mesh
+  public :: box_connectivity, box_nodecoords, calc_metrics, fxh_box_mesh, fxh_metrics
 
   integer, parameter :: ZETA_MINUS = 1, ETA_MINUS = 2, XI_PLUS = 3, ETA_PLUS = 4, XI_MINUS = 5, ZETA_PLUS = 6
   integer, parameter :: plus_side(3) = [XI_PLUS, ETA_PLUS, ZETA_PLUS]

@@ -84,3 +84,36 @@ def test_fortran_host_error_norms(tmp_path):
         err = np.abs(got[key] - a[field]) / np.abs(a[field])
         print(f"  {key}: {got[key][0]:.6e} (oracle {a[field][0]:.6e}), max rel diff {err.max():.1e}")
         assert np.all(err <= 1e-9)
+
+
+def test_fortran_host_runs_the_reference_free_stream_input(tmp_path):
+    """The reference's navst freestream/conforming input for the driver: its HOPR mesh
+    (MeshFile), its restart state (RestartFile), RefState and tend 20; the driver writes the
+    final state in the reference's state file format (ProjectName) and prints CalcErrorNorms,
+    whose L_inf must be below the reference test's 1e-11 (VolumeFlux 32 in place of the
+    reference run's 1, which this engine does not build)."""
+    from fluxo_amd import io
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "io")
+    ini = tmp_path / "parameter.ini"
+    ini.write_text(
+        "ProjectName = NAVIERSTOKES_FREESTREAM\nN = 3\nRiemann = 1\nVolumeFlux = 32\nParabolic = T\n"
+        f"MeshFile = {gold}/ConformBoxHeavilyWarped_Ngeo3__03_03_03_mesh.h5\n"
+        f"RestartFile = {gold}/NAVIERSTOKES_FREESTREAM_State_0000000.000000000.h5\n"
+        "IniExactFunc = 1\nIniRefState = 1\nRefState = (/1.0, 0.1, 0.2, -0.3, 20. /)\nmu0 = 1.0E-06\n"
+        "kappa = 1.4\nPr = 0.72\ntEnd = 20.0\nmaxIter = 100000\nCFLscale = 0.9\nDFLscale = 0.5\n")
+    print("fluxo_gpu reference freestream input", flush=True)
+    p = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    linf = None
+    for line in p.stdout.splitlines():
+        if line.strip().startswith("L_inf "):
+            linf = np.array([float(x) for x in line.split(":")[1].split()])
+    print(f"  L_inf {linf}")
+    assert linf is not None and np.all(linf <= 1e-11)
+    files = sorted(tmp_path.glob("NAVIERSTOKES_FREESTREAM_State_*.h5"))
+    assert len(files) == 1
+    U, t = io.read_state(files[0])
+    assert t >= 20.0 and U.shape == (27, 4, 4, 4, 5)
+    assert np.abs(U - [1.0, 0.1, 0.2, -0.3, 50.07]).max() <= 1e-11
