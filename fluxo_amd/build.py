@@ -47,7 +47,7 @@ def build_host(verbose=False):
         objs.append(o)
     rpath = "-Wl,-rpath,$ORIGIN"
     so = os.path.join(HERE, "libfluxo_host.so")
-    _run([FLANG, "-shared", "-o", so + ".tmp", *objs[:-1], "-L" + HERE, "-lfluxo_gpu", rpath], verbose)
+    _run([FLANG, "-shared", "-o", so + ".tmp", *objs[:-1], "-L" + HERE, "-lfluxo_gpu", "-lfluxo_io", rpath], verbose)
     os.replace(so + ".tmp", so)
     exe = os.path.join(HERE, "fluxo_gpu")
     _run([FLANG, "-o", exe + ".tmp", *objs, "-L" + HERE, "-lfluxo_gpu", "-lfluxo_io", rpath], verbose)

@@ -62,8 +62,10 @@ int check_config(const fx_config *c)
     if (c->nfvse) return set_err(FX_EINVAL, "GLM-MHD: no shock capturing");
     if (!(c->kappa > 1.0) || !(c->mu_0 > 0.0) || !(c->GLM_scale > 0.0))
       return set_err(FX_EINVAL, "GLM-MHD: kappa > 1, mu_0 > 0, GLM_scale > 0");
-    if (c->parabolic && (c->lifting != 1 || !(c->Pr > 0.0) || !(c->mu0 >= 0.0) || !(c->eta >= 0.0)))
-      return set_err(FX_EINVAL, "resistive GLM-MHD: BR1 lifting (lifting=1), Pr > 0, mu >= 0, eta >= 0");
+    if (c->parabolic && ((c->lifting != 1 && c->lifting != 2) || !(c->Pr > 0.0) || !(c->mu0 >= 0.0) ||
+                         !(c->eta >= 0.0)))
+      return set_err(FX_EINVAL, "resistive GLM-MHD: lifting 1 (BR1) or 2 (BR2), Pr > 0, mu >= 0, eta >= 0");
+    if (c->parabolic && c->lifting == 2 && !(c->etaBR2 > 0.0)) return set_err(FX_EINVAL, "BR2 needs etaBR2 > 0");
     if (c->precision != FX_PREC_FP64 && c->precision != FX_PREC_FP32_SPLIT)
       return set_err(FX_EINVAL, "precision %d not supported", c->precision);
     return FX_OK;

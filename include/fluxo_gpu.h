@@ -50,7 +50,7 @@ typedef struct {
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
                        resistive MHD */
   int lifting;      /* PP_Lifting with entropy-variable lifting: 1 = BR1, 2 = BR2
-                       (Navier-Stokes; src/dg/br1, src/dg/br2) */
+                       (src/dg/br1, src/dg/br2; Navier-Stokes and resistive GLM-MHD) */
   int precision;    /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
   int reserved;
   double kappa;     /* 'kappa' (equation.f90:218) */
@@ -74,8 +74,8 @@ typedef struct {
   /* GLM-MHD (nVar = 9: FLUXO_EQNSYSNAME=mhd with FLUXO_EQN_GLM and Powell
      non-conservative terms, CMakeLists.txt:174-209; mhd/equation.f90:174-208):
      VolumeFlux 10 (Derigs), Riemann 0, 1, 10 or 11, no shock capturing; parabolic = 1 adds
-     viscosity mu0, resistivity eta and heat conduction mu0*kappa/Pr with BR1 lifting of
-     the entropy variables (mhd/flux.f90:449-573). */
+     viscosity mu0, resistivity eta and heat conduction mu0*kappa/Pr with BR1 or BR2 lifting
+     of the entropy variables (mhd/flux.f90:449-573). */
   double mu_0;            /* 'mu_0' magnetic permeability */
   double GLM_scale;       /* 'GLM_scale' */
   double eta;             /* 'eta' resistivity (mhd/equation.f90:187) */

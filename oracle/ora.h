@@ -141,6 +141,8 @@ typedef struct {
   int parabolic;
   int riemann;            /* surface flux: 0 central, 1 Rusanov, 10 Derigs + ES-LLF, 11 Derigs */
   double mu, eta, Pr;
+  int lifting, reserved2; /* resistive case: 1 BR1 (default when 0), 2 BR2 with etaBR2 */
+  double etaBR2;
 } ora_mhd;
 /* primitive gradients (rho, v, p, B, psi) of the resistive case, gradPx/y/z(9,n,n,n,E) */
 void ora_mhd_lifting(const ora_mhd *M, const double *U, double *gx, double *gy, double *gz);

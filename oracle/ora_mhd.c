@@ -328,10 +328,54 @@ static void mhd_diff_flux(const ora_mhd *M, const double *c, const double *gx, c
 
 /* BR1 lifting of the entropy variables (br1.f90:154-364 with mhd/flux.f90:718-793) followed
    by ConvertToGradPrimVec */
+/* BR2 (br2.f90:167-336): volume gradients and, when fg is given, each side's master (fg[0],
+   fg[2], fg[4]) and slave (fg[1], fg[3], fg[5]) face gradients (MV,0:N,0:N,nSides), all
+   converted to primitive gradients with the node / face state */
+static void mhd_br2_gradients(const ora_mhd *M, const double *U, double *gx, double *gy, double *gz,
+                              double *const fg[6])
+{
+  const ora_problem *P = M->P;
+  const int N = P->N, n = N + 1, n3 = n * n * n;
+  const size_t n3E = (size_t)n3 * P->nElems, nface = (size_t)MV * n * n * P->nSides;
+  double *w = calloc((size_t)MV * n3E, sizeof(double));
+  double *f[6];
+  for (int d = 0; d < 6; d++) f[d] = fg ? fg[d] : malloc(sizeof(double) * nface);
+  for (size_t p = 0; p < n3E; p++) mhd_cons_to_entropy(M, U + MV * p, w + MV * p);
+  ora_lift_br2(P, MV, w, M->etaBR2, gx, gy, gz, f[0], f[2], f[4], f[1], f[3], f[5]);
+  for (size_t p = 0; p < n3E; p++) {
+    mhd_grad_prim(M, U + MV * p, gx + MV * p);
+    mhd_grad_prim(M, U + MV * p, gy + MV * p);
+    mhd_grad_prim(M, U + MV * p, gz + MV * p);
+  }
+  for (int s = 0; s < P->nSides; s++) {  /* face states of LGL: the sides' own volume nodes */
+    const int *t = P->SideToElem + 5 * s;
+    for (int q = 0; q < n; q++)
+      for (int pp = 0; pp < n; pp++) {
+        int im[3], is[3];
+        ora_side_to_vol(N, 0, pp, q, 0, t[2], im);
+        ora_side_to_vol(N, 0, pp, q, t[4], t[3], is);
+        const double *um = U + MV * ((size_t)(t[0] - 1) * n3 + im[0] + n * (im[1] + n * im[2]));
+        const double *us = U + MV * ((size_t)(t[1] - 1) * n3 + is[0] + n * (is[1] + n * is[2]));
+        const size_t fi = MV * ((size_t)s * n * n + pp + n * q);
+        for (int d = 0; d < 3; d++) {
+          mhd_grad_prim(M, um, f[2 * d] + fi);
+          mhd_grad_prim(M, us, f[2 * d + 1] + fi);
+        }
+      }
+  }
+  if (!fg)
+    for (int d = 0; d < 6; d++) free(f[d]);
+  free(w);
+}
+
 void ora_mhd_lifting(const ora_mhd *M, const double *U, double *gx, double *gy, double *gz)
 {
   const ora_problem *P = M->P;
   const size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  if (M->lifting == 2) {  /* BR2: the volume gradients */
+    mhd_br2_gradients(M, U, gx, gy, gz, NULL);
+    return;
+  }
   double *w = calloc((size_t)MV * n3E, sizeof(double));
   for (size_t p = 0; p < n3E; p++) mhd_cons_to_entropy(M, U + MV * p, w + MV * p);
   ora_lift(P, MV, w, gx, gy, gz);
@@ -438,11 +482,17 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
     }
   }
   free(ft);
-  double *gx = NULL, *gy = NULL, *gz = NULL;
+  double *gx = NULL, *gy = NULL, *gz = NULL, *fg[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
+  const int br2 = M->parabolic && M->lifting == 2;
   if (M->parabolic) {
     const size_t nt = (size_t)MV * n3 * E;
     gx = malloc(sizeof(double) * nt); gy = malloc(sizeof(double) * nt); gz = malloc(sizeof(double) * nt);
-    ora_mhd_lifting(M, U, gx, gy, gz);
+    if (br2) {
+      for (int d = 0; d < 6; d++) fg[d] = malloc(sizeof(double) * MV * n * n * P->nSides);
+      mhd_br2_gradients(M, U, gx, gy, gz, fg);
+    } else {
+      ora_mhd_lifting(M, U, gx, gy, gz);
+    }
     mhd_volint_visc(M, U, gx, gy, gz, Dh, Ut);
   }
   /* surface: FillFlux + SurfInt (LGL) */
@@ -461,10 +511,17 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
         const double *nv = P->NormVec + 3 * f, *t1 = P->TangVec1 + 3 * f, *t2 = P->TangVec2 + 3 * f;
         double Fm[MV], Fs[MV];
         ora_mhd_riemann(M, glm_ch, U + om, U + os, nv, t1, t2, Fm);
-        if (M->parabolic) {  /* BR1 average of the diffusive fluxes (mhd/riemann.f90:116-138) */
+        if (M->parabolic) {  /* mean of the two sides' diffusive fluxes (mhd/riemann.f90:116-138)
+                                with the volume (BR1) or the sides' face gradients (BR2) */
           double kL[MV], gL[MV], jL[MV], kR[MV], gR[MV], jR[MV];
-          mhd_diff_flux(M, U + om, gx + om, gy + om, gz + om, kL, gL, jL);
-          mhd_diff_flux(M, U + os, gx + os, gy + os, gz + os, kR, gR, jR);
+          if (br2) {
+            const size_t fo = MV * f;
+            mhd_diff_flux(M, U + om, fg[0] + fo, fg[2] + fo, fg[4] + fo, kL, gL, jL);
+            mhd_diff_flux(M, U + os, fg[1] + fo, fg[3] + fo, fg[5] + fo, kR, gR, jR);
+          } else {
+            mhd_diff_flux(M, U + om, gx + om, gy + om, gz + om, kL, gL, jL);
+            mhd_diff_flux(M, U + os, gx + os, gy + os, gz + os, kR, gR, jR);
+          }
           for (int v = 1; v < MV; v++)
             Fm[v] = Fm[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
         }
@@ -482,6 +539,7 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
     for (int v = 0; v < MV; v++) Ut[MV * p + v] = Ut[MV * p + v] * sj;
   }
   free(gx); free(gy); free(gz);
+  for (int d = 0; d < 6; d++) free(fg[d]);
 }
 
 /* InitTimeStep_GLM (mhd/calctimestep.f90:186-215): dt of a unit wave speed */

@@ -335,25 +335,28 @@ def nfvse_threshold(N):
 class _Mhd(C.Structure):
     _fields_ = [("P", C.c_void_p), ("kappa", C.c_double), ("mu_0", C.c_double), ("glm_scale", C.c_double),
                 ("parabolic", C.c_int), ("riemann", C.c_int), ("mu", C.c_double), ("eta", C.c_double),
-                ("Pr", C.c_double)]
+                ("Pr", C.c_double), ("lifting", C.c_int), ("reserved2", C.c_int), ("etaBR2", C.c_double)]
 
 
 class MhdProblem:
     """GLM-MHD setup (9 variables) on a mesh: Derigs volume flux, surface flux `riemann`
     (0 central, 1 Rusanov, 10 Derigs + entropy-stable LLF, 11 Derigs), Powell
     non-conservative terms; resistive (BR1 lifting of the entropy variables, viscosity
-    mu, resistivity eta, heat conduction mu kappa/Pr) when `resistive` is given as
+    mu, resistivity eta, heat conduction mu kappa/Pr; lifting 2 = BR2 with etaBR2) when
+    `resistive` is given as
     dict(mu=, eta=, Pr=).  The cleaning speed GLM_ch is explicit state: 0.5 before the first
     time step (the reference's initial value), then glm_scale * dtch1 / dt from every
     CALCTIMESTEP."""
 
-    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, resistive=None, riemann=1):
+    def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, resistive=None, riemann=1, lifting=1,
+                 etaBR2=2.0):
         self.base = Problem(mesh, volflux=10, riemann=1, kappa=kappa)
         self.mesh, self.N, self.nElems = mesh, mesh["N"], mesh["nElems"]
         m = _Mhd()
         m.P = C.cast(C.byref(self.base._p), C.c_void_p).value
         m.kappa, m.mu_0, m.glm_scale = kappa, mu_0, glm_scale
         m.riemann = riemann
+        m.lifting, m.etaBR2 = lifting, etaBR2
         if resistive is not None:
             m.parabolic = 1
             m.mu, m.eta, m.Pr = float(resistive["mu"]), float(resistive["eta"]), float(resistive.get("Pr", 0.72))

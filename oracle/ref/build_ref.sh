@@ -81,6 +81,7 @@ build_variant mhd_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
 # resistive GLM-MHD: viscosity, resistivity and heat conduction through BR1 lifting of the
 # entropy variables (FLUXO_PARABOLIC, CMakeLists.txt:253-283)
 build_variant mhd_br1_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3
+build_variant mhd_br2_vf10 $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=2 -DPP_Lifting_Var=3
 # the same with the Taylor-Green vortex testcase (FLUXO_TESTCASE=taylorgreenvortex), whose
 # AnalyzeTestcase integrates kinetic and magnetic energy, enstrophy and dissipation rates
 TC=taylorgreenvortex build_variant mhd_br1_vf10_tgv $MHD $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 \

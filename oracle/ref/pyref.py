@@ -17,7 +17,7 @@ REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 def binary(parabolic, volflux, nfvse=False, mhd=False, lifting=1):
     if mhd:
-        name = f"fluxo_ref_mhd{'_br1' if parabolic else ''}_vf{volflux}"
+        name = f"fluxo_ref_mhd{f'_br{lifting}' if parabolic else ''}_vf{volflux}"
     else:
         ns = f"ns_br{lifting}"
         name = f"fluxo_ref_{ns if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
@@ -129,7 +129,7 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
 
 
 def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600,
-            resistive=None, dfl=0.9, riemann=1, analyze=None, tgv=False):
+            resistive=None, dfl=0.9, riemann=1, analyze=None, tgv=False, lifting=1, etaBR2=2.0):
     """GLM-MHD (Derigs volume flux, surface flux `riemann` -- 0 central, 1 Rusanov, 10 Derigs +
     entropy-stable LLF, 11 Derigs -- Powell non-conservative terms) with
     the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before the first
@@ -142,7 +142,7 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
     par_on = resistive is not None
-    exe = binary(par_on, 10, mhd=True)
+    exe = binary(par_on, 10, mhd=True, lifting=lifting)
     if tgv:
         exe = exe + "_tgv"
     with tempfile.TemporaryDirectory() as d:
@@ -156,6 +156,8 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
             f.write(analyze_lines(analyze))
             if tgv:
                 f.write("doTCanalyze = T\n")
+            if par_on and lifting == 2:
+                f.write(f"etaBR2 = {etaBR2!r}\n")
             if par_on:
                 f.write(f"mu = {float(resistive['mu'])!r}\neta = {float(resistive['eta'])!r}\n"
                         f"Pr = {float(resistive.get('Pr', 0.72))!r}\n")

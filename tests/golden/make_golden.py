@@ -115,9 +115,14 @@ MHD_RIEMANN_CASES = [
     ("mhd_n2_central_warped", 3, 2, 3, True, 3, 0, None),
     ("mhd_resistive_n3_es_llf_warped", 2, 3, 3, True, 3, 10, (0.01, 0.02, 0.72)),
 ]
+# resistive GLM-MHD with BR2 lifting (PP_Lifting=2): name, K, N, Ngeo, warp, nsteps, mu, eta, Pr, etaBR2
+MHD_BR2_CASES = [
+    ("mhd_resistive_br2_n3_warped", 2, 3, 3, True, 3, 0.01, 0.02, 0.72, 2.0),
+    ("mhd_resistive_br2_n4_eta25", 2, 4, 3, True, 3, 0.005, 0.005, 0.9, 2.5),
+]
 
 
-def make_mhd(case, riemann=1):
+def make_mhd(case, riemann=1, lifting=1, etaBR2=2.0):
     name, K, N, Ngeo, warp, nsteps = case[:6]
     res = dict(mu=case[6], eta=case[7], Pr=case[8]) if len(case) > 6 else None
     m = O.box_mesh(K, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=Ngeo, warp=warp)
@@ -127,9 +132,10 @@ def make_mhd(case, riemann=1):
     U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
     U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
     U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
-    r = R.run_mhd(m, U, cfl=0.9, nsteps=nsteps, resistive=res, dfl=0.9, riemann=riemann)
+    r = R.run_mhd(m, U, cfl=0.9, nsteps=nsteps, resistive=res, dfl=0.9, riemann=riemann, lifting=lifting,
+                  etaBR2=etaBR2)
     out = dict(K=K, N=N, Ngeo=Ngeo, warp=int(warp), x0=np.zeros(3), L=np.ones(3), kappa=5.0 / 3.0, mu_0=1.0,
-               riemann=riemann,
+               riemann=riemann, lifting=lifting, etaBR2=etaBR2,
                glm_scale=0.5, cfl=0.9, nsteps=nsteps, U=U, Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"],
                errType0=r["errType0"], glm_ch0=r["glm_ch0"], glm_dtch1=r["glm_dtch1"], U_end=r["U_end"],
                t_end=r["t_end"], sJ=r["sJ"], Mf=r["Mf"], SurfElem=r["SurfElem"])
@@ -258,6 +264,8 @@ if __name__ == "__main__":
         make_mhd(c)
     for name, K, N, Ngeo, warp, nsteps, rs, res in MHD_RIEMANN_CASES:
         make_mhd((name, K, N, Ngeo, warp, nsteps) + (tuple(res) if res else ()), riemann=rs)
+    for c in MHD_BR2_CASES:
+        make_mhd(c[:9], lifting=2, etaBR2=c[9])
     for c in ANALYZE_CASES:
         make_analyze(c)
     for c in HOPR_CASES:

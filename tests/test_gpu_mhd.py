@@ -99,6 +99,21 @@ def test_entropy_balance_ec_and_es_surface_fluxes(riemann):
 
 
 @pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_resistive_br2_matches_oracle(N):
+    """BR2 lifting (etaBR2 = 2.5): volume gradients and RHS against the oracle."""
+    from fluxo_amd.native import MhdEngine
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    _, U = state(m, resistive=RES)
+    P = O.MhdProblem(m, resistive=RES, lifting=2, etaBR2=2.5)
+    e = MhdEngine(m, resistive=RES, lifting=2, etaBR2=2.5)
+    e.upload(U)
+    gerr = max(rel_l2(a, b) for a, b in zip(e.lifting(), P.lifting(U)))
+    err = rel_l2(e.rhs(), P.rhs(U, glm_ch=0.5))
+    print(f"resistive MHD BR2 N={N}: gradients relL2 {gerr:.3e}, RHS relL2 {err:.3e}")
+    assert gerr <= 1e-12 and err <= 1e-12
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
 def test_resistive_rhs_and_gradients_match_oracle(N):
     from fluxo_amd.native import MhdEngine
     m = O.box_mesh(3, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
@@ -122,7 +137,9 @@ def test_engine_matches_reference_mhd(name):
     res = dict(mu=float(g["mu"]), eta=float(g["eta"]), Pr=float(g["Pr"])) if "mu" in g else None
     dfl = float(g["dfl"]) if res else 0.0
     kw = dict(kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]), resistive=res,
-              riemann=int(g["riemann"]) if "riemann" in g else 1)
+              riemann=int(g["riemann"]) if "riemann" in g else 1,
+              lifting=int(g["lifting"]) if "lifting" in g else 1,
+              etaBR2=float(g["etaBR2"]) if "etaBR2" in g else 2.0)
     e = MhdEngine(m, **kw)
     e.upload(g["U"])
     if res:

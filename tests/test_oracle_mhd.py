@@ -33,6 +33,11 @@ def riemann_of(g):
     return int(g["riemann"]) if "riemann" in g else 1
 
 
+def lifting_of(g):
+    return dict(lifting=int(g["lifting"]) if "lifting" in g else 1,
+                etaBR2=float(g["etaBR2"]) if "etaBR2" in g else 2.0)
+
+
 def mesh_of(g):
     return O.box_mesh(int(g["K"]), int(g["N"]), x0=tuple(g["x0"]), length=tuple(g["L"]), Ngeo=int(g["Ngeo"]),
                       warp=bool(g["warp"]))
@@ -45,7 +50,7 @@ def test_oracle_matches_reference_mhd(name):
     res = resistive_of(g)
     dfl = float(g["dfl"]) if res else 0.0
     P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
-                     resistive=res, riemann=riemann_of(g))
+                     resistive=res, riemann=riemann_of(g), **lifting_of(g))
     U = g["U"]
     if res:
         for a, k in zip(P.lifting(U), ("gradPx", "gradPy", "gradPz")):
@@ -56,7 +61,7 @@ def test_oracle_matches_reference_mhd(name):
     assert P.glm_dtch1 == pytest.approx(float(g["glm_dtch1"]), rel=1e-15)
     assert P.glm_ch == pytest.approx(float(g["glm_ch0"]), rel=1e-15)
     P = O.MhdProblem(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
-                     resistive=res, riemann=riemann_of(g))
+                     resistive=res, riemann=riemann_of(g), **lifting_of(g))
     Ue, Utmp, t = U.copy(), np.zeros_like(U), 0.0
     for _ in range(int(g["nsteps"])):
         d, _ = P.calc_dt(Ue, float(g["cfl"]), dfl)
