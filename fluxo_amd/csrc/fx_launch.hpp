@@ -692,7 +692,6 @@ inline int check_nfvse(const fx_config *c)
 {
   if (!c->nfvse) return FX_OK;
   if (c->nfvse != 1) return set_err(FX_EINVAL, "nfvse=%d (0 off, 1 first-order sub-cell FV)", c->nfvse);
-  if (c->parabolic) return set_err(FX_EINVAL, "NFVSE shock capturing is supported for Euler (parabolic=0)");
   if (c->riemann != 0 && c->riemann != 1 && c->riemann != 9 && c->riemann != 32)
     return set_err(FX_EINVAL, "NFVSE sub-cell fluxes: Riemann 0, 1, 9, 32");
   if (c->computeAlpha != 1 && c->computeAlpha != 3)

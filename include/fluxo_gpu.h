@@ -59,9 +59,10 @@ typedef struct {
   double Pr;        /* 'Pr' (equation.f90:226) */
   double etaBR2;    /* 'etaBR2' BR2 penalty (br2.f90:116, default 2) */
   /* NFVSE shock capturing (SHOCK_NFVSE, CMakeLists.txt:316-331; parameters of InitNFVSE
-     shockcapturing/nfvse.f90:138-146), Euler only: sub-cell first-order finite volumes
+     shockcapturing/nfvse.f90:138-146), Euler or Navier-Stokes: sub-cell first-order finite volumes
      (SubFVMethod=1) blended into the volume integral of each element with the coefficient
-     of CalcBlendingCoefficient (nfvse.f90:1537-1615) and TimeRelFactor=0. */
+     of CalcBlendingCoefficient (nfvse.f90:1537-1615) and TimeRelFactor=0; with
+     parabolic = 1 the viscous volume term is added unblended (dg.f90:368-398). */
   int nfvse;           /* 0 off, 1 on */
   int computeAlpha;    /* 'ComputeAlpha': 1 Persson-Peraire indicator on rho*p
                           (ModalThreshold=1), 3 fixed 'ShockBlendCoef' */

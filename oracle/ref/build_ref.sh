@@ -88,3 +88,6 @@ TC=taylorgreenvortex build_variant mhd_br1_vf10_tgv $MHD $NOSC -DPP_VolFlux=10 -
   -DPP_Lifting_Var=3
 # NFVSE shock capturing (FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE, CMakeLists.txt:316-331)
 build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0
+# Navier-Stokes (BR1) with NFVSE: the blended advective volume integral plus VolInt_visc (dg.f90:368-398)
+build_variant ns_br1_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 \
+  -DPP_Lifting_Var=3 -DPP_VISC=0

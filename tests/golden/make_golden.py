@@ -233,21 +233,32 @@ def make_hopr(case):
     print(f"{name}: dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} dSdU_Ut={rr['dSdU_Ut']}")
 
 
-def make_nfvse(case):
+# Navier-Stokes (BR1, mu0) with NFVSE: name, K, N, sigma, p0, alpha_max, fixed alpha, steps, mu0
+NFVSE_NS_CASES = [
+    ("ns_nfvse_n3_pulse_warped", 5, 3, 0.25, 1e-2, 0.5, None, 3, 1e-3),
+    ("ns_nfvse_n4_fixed_alpha_warped", 4, 4, 0.25, 1e-2, 1.0, 0.4, 2, 1e-3),
+]
+
+
+def make_nfvse(case, mu0=None):
     from fluxo_amd.workloads import sedov_blast
-    name, K, N, sigma, p0, amax, fixed, nsteps = case
+    name, K, N, sigma, p0, amax, fixed, nsteps = case[:8]
+    par = mu0 is not None
     x0, L = (-1.5,) * 3, (3.0,) * 3
     m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=True)
     U = sedov_blast(m["Elem_xGP"], sigma=sigma, p0=p0)
-    par = dict(ComputeAlpha=1 if fixed is None else 3, ShockBlendCoef=float(fixed or 0.0),
-               alpha_max=amax, alpha_min=0.01, SpacePropSweeps=1, SpacePropFactor=0.5)
-    r = R.run(m, U, volflux=32, riemann=1, cfl=0.5, dfl=0.5, nsteps=nsteps, nfvse=par)
+    par_nfvse = dict(ComputeAlpha=1 if fixed is None else 3, ShockBlendCoef=float(fixed or 0.0),
+                     alpha_max=amax, alpha_min=0.01, SpacePropSweeps=1, SpacePropFactor=0.5)
+    r = R.run(m, U, volflux=32, riemann=1, cfl=0.5, dfl=0.5, nsteps=nsteps, nfvse=par_nfvse, parabolic=par,
+              mu0=mu0 or 0.0, Pr=0.72)
     out = dict(K=K, N=N, Ngeo=3, warp=1, x0=np.array(x0), L=np.array(L), volflux=32, riemann=1,
-               parabolic=0, mu0=0.0, Pr=0.72, kappa=1.4, cfl=0.5, dfl=0.5, nsteps=nsteps, U=U,
+               parabolic=int(par), mu0=mu0 or 0.0, Pr=0.72, kappa=1.4, cfl=0.5, dfl=0.5, nsteps=nsteps, U=U,
                Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"], errType0=r["errType0"], U_end=r["U_end"],
                t_end=r["t_end"], Entropy=r["Entropy"], dSdU_Ut=r["dSdU_Ut"], sJ=r["sJ"], Mf=r["Mf"],
                SurfElem=r["SurfElem"], nfvse=1, alpha=r["alpha"], alpha_max=amax, alpha_min=0.01,
                sweeps=1, prop_factor=0.5, fixed_alpha=-1.0 if fixed is None else fixed)
+    if par:
+        out.update(gradPx=r["gradPx"], gradPy=r["gradPy"], gradPz=r["gradPz"], lifting=1, etaBR2=2.0)
     np.savez(os.path.join(HERE, name + ".npz"), **out)
     a = r["alpha"]
     print(f"{name}: alpha zero {np.sum(a == 0)} max {np.sum(a == amax)} between "
@@ -260,6 +271,8 @@ if __name__ == "__main__":
         make(c)
     for c in NFVSE_CASES:
         make_nfvse(c)
+    for c in NFVSE_NS_CASES:
+        make_nfvse(c[:8], mu0=c[8])
     for c in MHD_CASES + MHD_RES_CASES:
         make_mhd(c)
     for name, K, N, Ngeo, warp, nsteps, rs, res in MHD_RIEMANN_CASES:
