@@ -318,12 +318,15 @@ void launch_volume_e(const Batch &b, RKArgs rk)
   using G = VolGeom<N, EPB>;
   const int nb = (b.cfg.nElems + EPB - 1) / EPB;
   if (nb <= 0) return;
-  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
-    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
-                       b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
-                       b.cfg.kappa);
-  else
-    hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+  if constexpr (has_fp32(VF)) {
+    if (b.cfg.precision == FX_PREC_FP32_SPLIT) {
+      hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+                         b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
+                         b.cfg.kappa);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
                        b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                        b.cfg.kappa);
 }
@@ -359,7 +362,15 @@ template <int N>
 void launch_volume_n(const Batch &b, int mode, RKArgs rk)
 {
   switch (b.cfg.volumeFlux) {
+  case 1: launch_volume_v<N, 1>(b, mode, rk); break;
+  case 2: launch_volume_v<N, 2>(b, mode, rk); break;
+  case 3: launch_volume_v<N, 3>(b, mode, rk); break;
+  case 4: launch_volume_v<N, 4>(b, mode, rk); break;
   case 5: launch_volume_v<N, 5>(b, mode, rk); break;
+  case 6: launch_volume_v<N, 6>(b, mode, rk); break;
+  case 7: launch_volume_v<N, 7>(b, mode, rk); break;
+  case 8: launch_volume_v<N, 8>(b, mode, rk); break;
+  case 9: launch_volume_v<N, 9>(b, mode, rk); break;
   case 10: launch_volume_v<N, 10>(b, mode, rk); break;
   case 32: launch_volume_v<N, 32>(b, mode, rk); break;
   default: launch_volume_v<N, 0>(b, mode, rk); break;
@@ -676,8 +687,10 @@ inline int check_connectivity(const int *ElemToSide, const int *SideToElem, int 
 // Physics options shared by contexts and servers.
 inline int check_physics(int volumeFlux, int riemann, int precision, double kappa, double Pr, int parabolic)
 {
-  if (volumeFlux != 0 && volumeFlux != 5 && volumeFlux != 10 && volumeFlux != 32)
-    return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0,5,10,32)", volumeFlux);
+  if (volumeFlux < 0 || (volumeFlux > 10 && volumeFlux != 32))
+    return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0..10, 32)", volumeFlux);
+  if (precision == FX_PREC_FP32_SPLIT && !has_fp32(volumeFlux))
+    return set_err(FX_EINVAL, "VolumeFlux=%d runs in FP64 only (FP32-split: 0, 5, 10, 32)", volumeFlux);
   if (riemann != 0 && riemann != 1 && riemann != 4 && riemann != 9 && riemann != 16 && riemann != 32 &&
       riemann != 33 && riemann != 44)
     return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4,9,16,32,33,44)", riemann);

@@ -98,11 +98,21 @@ __device__ __forceinline__ float inv_ln_mean(float a, float b, float la, float l
 enum { NF_RHO = 0, NF_V1, NF_V2, NF_V3, NF_P, NF_LNRHO, NF_X, NF_LNX, NF_VV, NF_E, NF_COUNT };
 
 // Fields a volume flux reads (the others are never staged).
+// NF_E holds rhoE (VF 0, 1, 3), e = rhoE/rho (VF 2), H = (rhoE+p)/rho (VF 8, 9); for VF 9
+// NF_RHO holds sqrt(rho); X = beta = rho/(2p) for VF 5, 6, 7.
 template <int VF>
 __host__ __device__ constexpr bool uses_field(int f)
 {
-  return (VF == 32 || VF == 10) ? (f <= NF_LNX) : (VF == 5 ? (f <= NF_VV) : (f <= NF_P || f == NF_E));
+  return (VF == 32 || VF == 10) ? (f <= NF_LNX)
+       : VF == 5 ? (f <= NF_VV)
+       : VF == 4 ? (f <= NF_P || f == NF_VV)
+       : VF == 6 ? (f <= NF_V3 || f == NF_X || f == NF_VV)
+       : VF == 7 ? (f <= NF_P || f == NF_X || f == NF_LNX || f == NF_VV)
+       : VF == 9 ? (f <= NF_V3 || f == NF_E)
+       : (f <= NF_P || f == NF_E);
 }
+// Volume fluxes with an FP32-split instantiation; the others run in FP64 only.
+__host__ __device__ constexpr bool has_fp32(int VF) { return VF == 0 || VF == 5 || VF == 10 || VF == 32; }
 template <int VF>
 __host__ __device__ constexpr int n_fields()
 {
@@ -137,6 +147,13 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
   } else if (VF == 10) {
     double z1 = sqrt(u[0] / p), z5 = z1 * p;
     f[NF_RHO] = z1; f[NF_LNRHO] = log(z1); f[NF_X] = z5; f[NF_LNX] = log(z5);
+  } else if (VF == 6 || VF == 7) {
+    double X = 0.5 * u[0] / p;
+    f[NF_LNRHO] = 0.0; f[NF_X] = X; f[NF_LNX] = VF == 7 ? log(X) : 0.0;
+  } else if (VF == 2 || VF == 8 || VF == 9) {
+    f[NF_E] = VF == 2 ? srho * u[4] : (u[4] + p) * srho;
+    if (VF == 9) f[NF_RHO] = sqrt(u[0]);
+    f[NF_LNRHO] = 0.0; f[NF_X] = 0.0; f[NF_LNX] = 0.0;
   } else {
     f[NF_LNRHO] = 0.0; f[NF_X] = 0.0; f[NF_LNX] = 0.0;
   }
@@ -146,7 +163,17 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
 //  VF=32: RanochaFluxVec (flux_average.f90:1876-1929)
 //  VF=5 : EntropyAndEnergyConservingFluxVec (flux_average.f90:1277-1356)
 //  VF=10: TwoPointEntropyConservingFluxVec, Ismail-Roe (flux_average.f90:574-673)
+//  VF=1 : StandardDGFluxDealiasedMetricVec (flux_average.f90:446-494)
+//  VF=2 : KennedyAndGruberFluxVec1 (flux_average.f90:731-803)
+//  VF=8 : KennedyAndGruberFluxVec2 (flux_average.f90:857-929)
+//  VF=3 : DucrosFluxVec (flux_average.f90:986-1059)
+//  VF=4 : MorinishiFluxVec (flux_average.f90:1121-1205), kappa/(kappa-1) = 1 + 1/(kappa-1)
+//  VF=6 : EntropyAndEnergyConservingFluxVec2 (flux_average.f90:1427-1504)
+//  VF=7 : ggFluxVec (flux_average.f90:1580-1653)
+//  VF=9 : GassnerWintersWalchFluxVec (flux_average.f90:1724-1799), (kappa-1)/kappa =
+//         1/(1 + 1/(kappa-1))
 //  VF=0 : StandardDGFluxVec (flux_average.f90:394-439)
+// Momenta rhoU of the reference enter as rho*v (one rounding apart).
 template <int VF, typename R>
 __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL, const R *mR,
                                             R skappaM1, R *F)
@@ -201,6 +228,87 @@ __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL
     F[2] = F[0] * vH + m1 * p1;
     F[3] = F[0] * wH + m2 * p1;
     F[4] = R(0.5) * (qH * (R(1) + R(2) * skappaM1) * (z5LN * sz1LN) + uH * F[1] + vH * F[2] + wH * F[3]);
+  } else if (VF == 1) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R qL = L[NF_V1] * m0 + L[NF_V2] * m1 + L[NF_V3] * m2;
+    R qR = Rt[NF_V1] * m0 + Rt[NF_V2] * m1 + Rt[NF_V3] * m2;
+    R rqL = L[NF_RHO] * qL, rqR = Rt[NF_RHO] * qR, ps = L[NF_P] + Rt[NF_P];
+    F[0] = R(0.5) * (rqL + rqR);
+    F[1] = R(0.5) * (rqL * L[NF_V1] + rqR * Rt[NF_V1] + m0 * ps);
+    F[2] = R(0.5) * (rqL * L[NF_V2] + rqR * Rt[NF_V2] + m1 * ps);
+    F[3] = R(0.5) * (rqL * L[NF_V3] + rqR * Rt[NF_V3] + m2 * ps);
+    F[4] = R(0.5) * ((L[NF_E] + L[NF_P]) * qL + (Rt[NF_E] + Rt[NF_P]) * qR);
+  } else if (VF == 2 || VF == 8 || VF == 3) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R uH = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
+    R vH = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
+    R wH = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
+    R p1 = R(0.5) * (L[NF_P] + Rt[NF_P]);
+    R qH = uH * m0 + vH * m1 + wH * m2;
+    F[0] = R(0.5) * (L[NF_RHO] + Rt[NF_RHO]) * qH;
+    if (VF == 3) {
+      F[1] = R(0.5) * (L[NF_RHO] * L[NF_V1] + Rt[NF_RHO] * Rt[NF_V1]) * qH + m0 * p1;
+      F[2] = R(0.5) * (L[NF_RHO] * L[NF_V2] + Rt[NF_RHO] * Rt[NF_V2]) * qH + m1 * p1;
+      F[3] = R(0.5) * (L[NF_RHO] * L[NF_V3] + Rt[NF_RHO] * Rt[NF_V3]) * qH + m2 * p1;
+      F[4] = (R(0.5) * (L[NF_E] + Rt[NF_E]) + p1) * qH;
+    } else {
+      F[1] = F[0] * uH + m0 * p1;
+      F[2] = F[0] * vH + m1 * p1;
+      F[3] = F[0] * wH + m2 * p1;
+      F[4] = F[0] * (R(0.5) * (L[NF_E] + Rt[NF_E])) + (VF == 2 ? p1 * qH : R(0));
+    }
+  } else if (VF == 4) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R uH = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
+    R vH = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
+    R wH = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
+    R p1 = R(0.5) * (L[NF_P] + Rt[NF_P]);
+    R qL = L[NF_V1] * m0 + L[NF_V2] * m1 + L[NF_V3] * m2;
+    R qR = Rt[NF_V1] * m0 + Rt[NF_V2] * m1 + Rt[NF_V3] * m2;
+    R rqL = R(0.5) * L[NF_RHO] * qL, rqR = R(0.5) * Rt[NF_RHO] * qR;
+    F[0] = rqL + rqR;
+    F[1] = F[0] * uH + m0 * p1;
+    F[2] = F[0] * vH + m1 * p1;
+    F[3] = F[0] * wH + m2 * p1;
+    F[4] = R(0.5) * ((R(1) + skappaM1) * (L[NF_P] * qL + Rt[NF_P] * qR) - (rqL * L[NF_VV] + rqR * Rt[NF_VV])) +
+           rqL * (L[NF_V1] * uH + L[NF_V2] * vH + L[NF_V3] * wH) +
+           rqR * (Rt[NF_V1] * uH + Rt[NF_V2] * vH + Rt[NF_V3] * wH);
+  } else if (VF == 6 || VF == 7) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    R rho_mean = R(0.5) * (L[NF_RHO] + Rt[NF_RHO]);
+    R uH = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
+    R vH = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
+    R wH = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
+    R sbeta, p1;
+    if (VF == 6) {
+      sbeta = R(2) * rcp_nr(L[NF_X] + Rt[NF_X]);
+      p1 = R(0.5) * rho_mean * sbeta;
+    } else {
+      sbeta = inv_ln_mean(L[NF_X], Rt[NF_X], L[NF_LNX], Rt[NF_LNX]);
+      p1 = R(0.5) * (L[NF_P] + Rt[NF_P]);
+    }
+    R qH = uH * m0 + vH * m1 + wH * m2;
+    F[0] = rho_mean * qH;
+    F[1] = F[0] * uH + m0 * p1;
+    F[2] = F[0] * vH + m1 * p1;
+    F[3] = F[0] * wH + m2 * p1;
+    F[4] = F[0] * R(0.5) * (skappaM1 * sbeta - R(0.5) * (L[NF_VV] + Rt[NF_VV])) + uH * F[1] + vH * F[2] +
+           wH * F[3];
+  } else if (VF == 9) {
+    R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
+    const R sL = L[NF_RHO], sR = Rt[NF_RHO];
+    R rhoH = R(0.5) * (sL + sR);
+    R uH = R(0.5) * (sL * L[NF_V1] + sR * Rt[NF_V1]);
+    R vH = R(0.5) * (sL * L[NF_V2] + sR * Rt[NF_V2]);
+    R wH = R(0.5) * (sL * L[NF_V3] + sR * Rt[NF_V3]);
+    R HH = R(0.5) * (sL * L[NF_E] + sR * Rt[NF_E]);
+    R p1 = (rhoH * HH - R(0.5) * (uH * uH + vH * vH + wH * wH)) * rcp_nr(R(1) + skappaM1);
+    R qH = uH * m0 + vH * m1 + wH * m2;
+    F[0] = rhoH * qH;
+    F[1] = qH * uH + m0 * p1;
+    F[2] = qH * vH + m1 * p1;
+    F[3] = qH * wH + m2 * p1;
+    F[4] = qH * HH;
   } else {
     R qL = L[NF_V1] * mL[0] + L[NF_V2] * mL[1] + L[NF_V3] * mL[2];
     R qR = Rt[NF_V1] * mR[0] + Rt[NF_V2] * mR[1] + Rt[NF_V3] * mR[2];

@@ -40,8 +40,11 @@ typedef struct {
   int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) or 9 (ideal GLM-MHD) */
   int nElems;       /* local elements */
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
-  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:414-460): 0 standard DG, 5
-                       Chandrashekar, 10 Ismail-Roe, 32 Ranocha; MHD 10 (Derigs) */
+  int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:414-460): 0 standard DG, 1
+                       standard DG with averaged metrics, 2 / 8 Kennedy-Gruber, 3 Ducros,
+                       4 Morinishi, 5 Chandrashekar, 6 EC with arithmetic means, 7 gg,
+                       9 Gassner-Winters-Walch, 10 Ismail-Roe, 32 Ranocha (FP32-split:
+                       0, 5, 10, 32 only); MHD 10 (Derigs) */
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:291-404): 0 central, 1 LLF,
                        9 / 32 EC (Chandrashekar / Ranocha), 4 / 44 / 33 entropy stable
                        (Ismail-Roe / Chandrashekar / Ranocha + matrix dissipation),

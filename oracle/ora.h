@@ -64,8 +64,10 @@ int  ora_calc_metrics(int N, int Ngeo, int nElems, int nSides, const double *Nod
 /* ---- equation options (ints and reals), mirrors parameter.ini keys ---- */
 typedef struct {
   int N, nVar, nElems, nSides;
-  int volflux;       /* VolumeFlux: 0 standard DG, 5 Chandrashekar EC-KEP, 10 Ismail-Roe EC,
-                        32 Ranocha EC-KEP */
+  int volflux;       /* VolumeFlux: 0 standard DG, 1 standard DG with averaged metrics,
+                        2/8 Kennedy-Gruber, 3 Ducros, 4 Morinishi, 5 Chandrashekar EC-KEP,
+                        6 EC with arithmetic means, 7 gg, 9 Gassner-Winters-Walch,
+                        10 Ismail-Roe EC, 32 Ranocha EC-KEP */
   int riemann;       /* Riemann: 1 LLF (Rusanov), 5 EC Chandrashekar surface, 32 EC Ranocha surface */
   int parabolic;     /* 0 Euler, 1 Navier-Stokes */
   int lifting;       /* 1 BR1, 2 BR2 */

@@ -112,10 +112,127 @@ static void flux_ismail_roe(double kappa, const double *UL, const double *UR, co
   F[4] = 0.5 * (qH * kp1 * skm1 * (z5LN / z1LN) + uH * F[1] + vH * F[2] + wH * F[3]);
 }
 
+/* The averaged metric {{m}} and the contravariant velocity {{v}}.{{m}} shared by the fluxes
+   below (each starts with metric = 0.5*(metric_L+metric_R)). */
+static void mean_metric(const double *mL, const double *mR, const double *aL, const double *aR, double *m,
+                        double *vh, double *qh)
+{
+  for (int d = 0; d < 3; d++) {
+    m[d] = 0.5 * (mL[d] + mR[d]);
+    vh[d] = 0.5 * (aL[1 + d] + aR[1 + d]);
+  }
+  *qh = vh[0] * m[0] + vh[1] * m[1] + vh[2] * m[2];
+}
+
+/* StandardDGFluxDealiasedMetricVec (flux_average.f90:446-494) */
+static void flux_standard_dealiased(const double *UL, const double *UR, const double *aL, const double *aR,
+                                    const double *mL, const double *mR, double *F)
+{
+  double m[3] = {0.5 * (mL[0] + mR[0]), 0.5 * (mL[1] + mR[1]), 0.5 * (mL[2] + mR[2])};
+  double qL = aL[1] * m[0] + aL[2] * m[1] + aL[3] * m[2];
+  double qR = aR[1] * m[0] + aR[2] * m[1] + aR[3] * m[2];
+  F[0] = 0.5 * (UL[0] * qL + UR[0] * qR);
+  for (int d = 0; d < 3; d++) F[1 + d] = 0.5 * (UL[1 + d] * qL + UR[1 + d] * qR + m[d] * (aL[4] + aR[4]));
+  F[4] = 0.5 * ((UL[4] + aL[4]) * qL + (UR[4] + aR[4]) * qR);
+}
+
+/* KennedyAndGruberFluxVec1 (flux_average.f90:731-803) and KennedyAndGruberFluxVec2
+   (flux_average.f90:857-929): the energy flux with {{e}} = {{rhoE/rho}} (KG1) or the
+   enthalpy {{H}} = {{(rhoE+p)/rho}} (KG2). */
+static void flux_kennedy_gruber(int which, const double *UL, const double *UR, const double *aL,
+                                const double *aR, const double *mL, const double *mR, double *F)
+{
+  double m[3], vh[3], qh;
+  mean_metric(mL, mR, aL, aR, m, vh, &qh);
+  double rhoHat = 0.5 * (UL[0] + UR[0]), p1 = 0.5 * (aL[4] + aR[4]);
+  F[0] = rhoHat * qh;
+  for (int d = 0; d < 3; d++) F[1 + d] = F[0] * vh[d] + m[d] * p1;
+  if (which == 2)
+    F[4] = F[0] * (0.5 * (aL[0] * UL[4] + aR[0] * UR[4])) + p1 * qh;
+  else
+    F[4] = F[0] * (0.5 * ((UL[4] + aL[4]) * aL[0] + (UR[4] + aR[4]) * aR[0]));
+}
+
+/* DucrosFluxVec (flux_average.f90:986-1059) */
+static void flux_ducros(const double *UL, const double *UR, const double *aL, const double *aR, const double *mL,
+                        const double *mR, double *F)
+{
+  double m[3], vh[3], qh;
+  mean_metric(mL, mR, aL, aR, m, vh, &qh);
+  double p1 = 0.5 * (aL[4] + aR[4]);
+  F[0] = 0.5 * (UL[0] + UR[0]) * qh;
+  for (int d = 0; d < 3; d++) F[1 + d] = 0.5 * (UL[1 + d] + UR[1 + d]) * qh + m[d] * p1;
+  F[4] = (0.5 * (UL[4] + UR[4]) + p1) * qh;
+}
+
+/* MorinishiFluxVec (flux_average.f90:1121-1205) */
+static void flux_morinishi(double kappa, const double *UL, const double *UR, const double *aL, const double *aR,
+                           const double *mL, const double *mR, double *F)
+{
+  double m[3], vh[3], qh;
+  mean_metric(mL, mR, aL, aR, m, vh, &qh);
+  double p1 = 0.5 * (aL[4] + aR[4]);
+  double qL = aL[1] * m[0] + aL[2] * m[1] + aL[3] * m[2];
+  double qR = aR[1] * m[0] + aR[2] * m[1] + aR[3] * m[2];
+  double rqL = 0.5 * UL[0] * qL, rqR = 0.5 * UR[0] * qR;
+  F[0] = rqL + rqR;
+  for (int d = 0; d < 3; d++) F[1 + d] = F[0] * vh[d] + m[d] * p1;
+  F[4] = 0.5 * (kappa / (kappa - 1.0) * (aL[4] * qL + aR[4] * qR) - (rqL * aL[5] + rqR * aR[5])) +
+         rqL * (aL[1] * vh[0] + aL[2] * vh[1] + aL[3] * vh[2]) + rqR * (aR[1] * vh[0] + aR[2] * vh[1] + aR[3] * vh[2]);
+}
+
+/* EntropyAndEnergyConservingFluxVec2 (flux_average.f90:1427-1504) with arithmetic means of rho
+   and beta = rho/(2p), and ggFluxVec (flux_average.f90:1580-1653) with the log mean of beta. */
+static void flux_ec2_gg(int which, double kappa, const double *UL, const double *UR, const double *aL,
+                        const double *aR, const double *mL, const double *mR, double *F)
+{
+  double m[3], vh[3], qh;
+  mean_metric(mL, mR, aL, aR, m, vh, &qh);
+  const double skm1 = 1.0 / (kappa - 1.0);
+  double bL = 0.5 * UL[0] / aL[4], bR = 0.5 * UR[0] / aR[4];
+  double rho_mean = 0.5 * (UL[0] + UR[0]);
+  double sbeta, p1;
+  if (which == 6) {
+    sbeta = 2.0 / (bL + bR);
+    p1 = 0.5 * rho_mean * sbeta;
+  } else {
+    sbeta = 1.0 / ora_ln_mean(bL, bR);
+    p1 = 0.5 * (aL[4] + aR[4]);
+  }
+  F[0] = rho_mean * qh;
+  for (int d = 0; d < 3; d++) F[1 + d] = F[0] * vh[d] + m[d] * p1;
+  F[4] = F[0] * 0.5 * (skm1 * sbeta - 0.5 * (aL[5] + aR[5])) + vh[0] * F[1] + vh[1] * F[2] + vh[2] * F[3];
+}
+
+/* GassnerWintersWalchFluxVec (flux_average.f90:1724-1799): averages of sqrt(rho)-weighted
+   variables */
+static void flux_gww(double kappa, const double *UL, const double *UR, const double *aL, const double *aR,
+                     const double *mL, const double *mR, double *F)
+{
+  double m[3] = {0.5 * (mL[0] + mR[0]), 0.5 * (mL[1] + mR[1]), 0.5 * (mL[2] + mR[2])};
+  double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+  double rhoHat = 0.5 * (sL + sR), h[3];
+  for (int d = 0; d < 3; d++) h[d] = 0.5 * (sL * aL[1 + d] + sR * aR[1 + d]);
+  double Hh = 0.5 * (sL * aL[0] * (UL[4] + aL[4]) + sR * aR[0] * (UR[4] + aR[4]));
+  double p1 = (kappa - 1.0) / kappa * (rhoHat * Hh - 0.5 * (h[0] * h[0] + h[1] * h[1] + h[2] * h[2]));
+  double qh = h[0] * m[0] + h[1] * m[1] + h[2] * m[2];
+  F[0] = rhoHat * qh;
+  for (int d = 0; d < 3; d++) F[1 + d] = qh * h[d] + m[d] * p1;
+  F[4] = qh * Hh;
+}
+
 void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
                  const double *auxR, const double *mL, const double *mR, double *F)
 {
   switch (which) {
+  case 1: flux_standard_dealiased(UL, UR, auxL, auxR, mL, mR, F); break;
+  case 2: flux_kennedy_gruber(2, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 8: flux_kennedy_gruber(8, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 3: flux_ducros(UL, UR, auxL, auxR, mL, mR, F); break;
+  case 4: flux_morinishi(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 6: flux_ec2_gg(6, kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 7: flux_ec2_gg(7, kappa, UL, UR, auxL, auxR, mL, mR, F); break;
+  case 9: flux_gww(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
   case 5: flux_chandrashekar(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
   case 10: flux_ismail_roe(kappa, UL, UR, auxL, auxR, mL, mR, F); break;
   case 32: flux_ranocha(kappa, UL, UR, auxL, auxR, mL, mR, F); break;

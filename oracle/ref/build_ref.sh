@@ -21,8 +21,10 @@ FC=${FC:-flang}
 mkdir -p "$OUT"
 COMMON="-cpp -O2 -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -I$REF"
 
+# VARIANTS="a b ..." builds only the named variants (all when unset)
 build_variant() {
   local name=$1; shift
+  case " ${VARIANTS:-$name} " in *" $name "*) ;; *) return 0 ;; esac
   local defs="$*"
   local obj="$OUT/obj_$name"
   mkdir -p "$obj"
@@ -70,6 +72,10 @@ build_variant euler_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=0
 build_variant euler_vf5 $NS $NOSC -DPP_VolFlux=5 -DPARABOLIC=0
 # Ismail-Roe two-point flux (VolumeFlux=10)
 build_variant euler_vf10 $NS $NOSC -DPP_VolFlux=10 -DPARABOLIC=0
+# the other split-form volume fluxes (VolumeFlux 1-4 and 6-9, equation.f90:414-455)
+for vf in 1 2 3 4 6 7 8 9; do
+  build_variant euler_vf$vf $NS $NOSC -DPP_VolFlux=$vf -DPARABOLIC=0
+done
 build_variant ns_br1_vf10 $NS $NOSC -DPP_VolFlux=10 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
 build_variant ns_br1_vf32 $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 -DPP_Lifting_Var=3 -DPP_VISC=0
 # BR2 lifting (FLUXO_PARABOLIC_LIFTING=br2, CMakeLists.txt:278-283)

@@ -48,6 +48,17 @@ CASES = [
     ("ns_n3_tgv_br2_warped", 3, 3, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 0.01, 3, 2, 2.0),
     ("ns_n4_tgv_br2_eta3p5_warped", 2, 4, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 0.01, 3, 2, 3.5),
     ("ns_n2_tgv_br2_warped", 3, 2, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 0.01, 3, 2, 2.0),
+    # the other volume fluxes (VolumeFlux 1: standard DG with averaged metrics, 2/8:
+    # Kennedy-Gruber, 3: Ducros, 4: Morinishi, 6: EC with arithmetic means, 7: gg,
+    # 9: Gassner-Winters-Walch) with LLF
+    ("euler_n3_vf1_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 1, 1, False, 2, 0.05, 3),
+    ("euler_n4_vf2_llf_warped", 2, 4, 3, True, (0, 0, 0), (2, 2, 2), 2, 1, False, 2, 0.05, 3),
+    ("euler_n2_vf3_llf_warped", 3, 2, 3, True, (0, 0, 0), (2, 2, 2), 3, 1, False, 2, 0.05, 3),
+    ("euler_n3_vf4_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 4, 1, False, 2, 0.05, 3),
+    ("euler_n4_vf6_llf_warped", 2, 4, 3, True, (0, 0, 0), (2, 2, 2), 6, 1, False, 2, 0.05, 3),
+    ("euler_n3_vf7_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 7, 1, False, 2, 0.05, 3),
+    ("euler_n5_vf8_llf_warped", 2, 5, 3, True, (0, 0, 0), (2, 2, 2), 8, 1, False, 2, 0.05, 3),
+    ("euler_n3_vf9_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 9, 1, False, 2, 0.05, 3),
 ]
 
 # NFVSE shock capturing (the reference built with SHOCK_NFVSE): a Gaussian pressure pulse
@@ -267,6 +278,11 @@ def make_nfvse(case, mu0=None):
 
 if __name__ == "__main__":
     sys.path.insert(0, ROOT)
+    if len(sys.argv) > 1:  # only the named cases of CASES
+        for c in CASES:
+            if c[0] in sys.argv[1:]:
+                make(c)
+        sys.exit(0)
     for c in CASES:
         make(c)
     for c in NFVSE_CASES:

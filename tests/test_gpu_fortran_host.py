@@ -90,15 +90,16 @@ def test_fortran_host_runs_the_reference_free_stream_input(tmp_path):
     """The reference's navst freestream/conforming input for the driver: its HOPR mesh
     (MeshFile), its restart state (RestartFile), RefState and tend 20; the driver writes the
     final state in the reference's state file format (ProjectName) and prints CalcErrorNorms,
-    whose L_inf must be below the reference test's 1e-11 (VolumeFlux 32 in place of the
-    reference run's 1, which this engine does not build)."""
+    whose L_inf must be below the reference test's 1e-11.  The equation keys are the
+    reference's parameter.ini's: Riemann 1, VolumeFlux 1 (standard DG with averaged
+    metrics), mu0 1e-6, CFL 0.9, DFL 0.5."""
     from fluxo_amd import io
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "io")
     ini = tmp_path / "parameter.ini"
     ini.write_text(
-        "ProjectName = NAVIERSTOKES_FREESTREAM\nN = 3\nRiemann = 1\nVolumeFlux = 32\nParabolic = T\n"
+        "ProjectName = NAVIERSTOKES_FREESTREAM\nN = 3\nRiemann = 1\nVolumeFlux = 1\nParabolic = T\n"
         f"MeshFile = {gold}/ConformBoxHeavilyWarped_Ngeo3__03_03_03_mesh.h5\n"
         f"RestartFile = {gold}/NAVIERSTOKES_FREESTREAM_State_0000000.000000000.h5\n"
         "IniExactFunc = 1\nIniRefState = 1\nRefState = (/1.0, 0.1, 0.2, -0.3, 20. /)\nmu0 = 1.0E-06\n"

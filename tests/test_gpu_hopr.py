@@ -51,18 +51,19 @@ def test_engine_matches_reference_on_hopr_meshes(name):
     assert traj <= 1e-10 and t == pytest.approx(float(g["t_end"]), rel=1e-12)
 
 
-def test_reference_navier_stokes_free_stream_test():
+@pytest.mark.parametrize("volume_flux", [1, 32])
+def test_reference_navier_stokes_free_stream_test(volume_flux):
     """The reference's navst freestream/conforming test: restart from its state file on its
     heavily warped Ngeo=3 mesh, N=3, mu0 = 1e-6, CFL 0.9, DFL 0.5, run to t >= 20, then the
     L_inf error of CalcErrorNorms against the constant RefState must stay below 1e-11 (its
-    err_tol).  The reference's run uses VolumeFlux 1 (standard DG with metric dealiasing);
-    here the Ranocha split form, which preserves the free stream the same way."""
+    err_tol).  VolumeFlux 1 (standard DG with averaged metrics, Riemann 1) is the reference
+    run's setting; the Ranocha split form preserves the free stream the same way."""
     from fluxo_amd import io
     from fluxo_amd.mesh import hopr_mesh
     from fluxo_amd.native import Engine
     U0, t0 = io.read_state(STATE)
     m = hopr_mesh(os.path.join(IO_DIR, "ConformBoxHeavilyWarped_Ngeo3__03_03_03_mesh.h5"), 3)
-    e = Engine(m, volume_flux=32, riemann=1, parabolic=True, mu0=1e-6, Pr=0.72)
+    e = Engine(m, volume_flux=volume_flux, riemann=1, parabolic=True, mu0=1e-6, Pr=0.72)
     e.upload(U0)
     e.upload_coords(m["Elem_xGP"])
     t, steps = t0, 0
@@ -71,7 +72,7 @@ def test_reference_navier_stokes_free_stream_test():
         assert err == 0
         steps += 500
     a = e.analyze(t, exact_func=1, ref_state=tuple(U0[0, 0, 0, 0]))
-    print(f"free stream: t = {t:.3f} after {steps} steps, L_inf {a['Linf']}")
+    print(f"free stream VolumeFlux {volume_flux}: t = {t:.3f} after {steps} steps, L_inf {a['Linf']}")
     assert t >= 20.0
     assert np.all(a["Linf"] <= 1e-11)
 

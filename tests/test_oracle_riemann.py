@@ -84,3 +84,31 @@ def test_volume_flux_tadmor_and_consistency(vf):
         p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
         vm = v @ m
         assert np.allclose(Fe, [UL[0] * vm, *(UL[1:4] * vm + p * m), (UL[4] + p) * vm], rtol=1e-12, atol=1e-12)
+
+
+KEP = {2: True, 3: False, 4: True, 6: True, 7: True, 8: True, 9: False, 1: False}
+
+
+@pytest.mark.parametrize("vf", [1, 2, 3, 4, 6, 7, 8, 9])
+def test_other_volume_fluxes_consistency_symmetry_kep(vf):
+    """The other two-point volume fluxes (golden vectors euler_*_vf*_llf_*): symmetric under
+    exchanging the sides with their metrics, F#(U,U,mL,mR) = f(U).{{m}}, and for the kinetic
+    energy preserving ones (Kennedy-Gruber, Morinishi, EC with arithmetic means, gg) the
+    momentum flux is F#_rho {{v}} + p~ {{m}} with one pressure average p~ for all three
+    components."""
+    rng = np.random.default_rng(100 + vf)
+    for _ in range(30):
+        UL, UR = random_state(rng), random_state(rng)
+        mL, mR = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        m = 0.5 * (mL + mR)
+        F = O.volflux(vf, UL, UR, mL, mR, KAPPA)
+        assert np.allclose(F, O.volflux(vf, UR, UL, mR, mL, KAPPA), rtol=1e-13, atol=1e-13)
+        Fe = O.volflux(vf, UL, UL, mL, mR, KAPPA)
+        v = UL[1:4] / UL[0]
+        p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
+        vm = v @ m
+        assert np.allclose(Fe, [UL[0] * vm, *(UL[1:4] * vm + p * m), (UL[4] + p) * vm], rtol=1e-12, atol=1e-12)
+        if KEP[vf]:
+            va = 0.5 * (UL[1:4] / UL[0] + UR[1:4] / UR[0])
+            pt = (F[1:4] - F[0] * va) / m
+            assert np.ptp(pt) <= 1e-12 * np.abs(pt).max(), pt
