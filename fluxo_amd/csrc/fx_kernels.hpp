@@ -442,7 +442,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 // the BR1 viscous average for Navier-Stokes (FillFlux fillflux.f90:41-110, Riemann
 // riemann.f90:104-169).
 // Launch geometry of the surface kernel: whole sides per block (SPB sides of n^2 points), so
-// that a face-wide maximum (Riemann 16) is a block-local reduction.
+// that a face-wide maximum (Riemann 13-19, 21) is a block-local reduction.
 template <int N> struct SurfGeom {
   static constexpr int n2 = (N + 1) * (N + 1), SPB = 256 / n2, T = SPB * n2;
 };
@@ -453,7 +453,7 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
           const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-  constexpr bool ES = (RS == 4 || RS == 16 || RS == 33 || RS == 44);
+  constexpr bool ES = (RS == 4 || RS == 5 || RS == 13 || RS == 16 || RS == 33 || RS == 44);
   const int gid = blockIdx.x * SurfGeom<N>::T + threadIdx.x;
   const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
   const bool valid = threadIdx.x < SurfGeom<N>::T && gid < nSides * n2;
@@ -471,7 +471,7 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
     nv[0] = g.x; nv[1] = g.y; nv[2] = g.z; se = g.w;
   }
   double lam = 0.0;
-  if (RS == 16) {  // LambdaMaxMax of the face
+  if (riemann_face_lambda(RS)) {  // LambdaMaxMax of the face
     __shared__ double s_lam[SurfGeom<N>::T];
     if (threadIdx.x < SurfGeom<N>::T) s_lam[threadIdx.x] = used ? face_lambda(UL, UR, nv, kappa) : 0.0;
     __syncthreads();
@@ -484,7 +484,7 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
   if (!used) return;
   double F[5];
   if (ES) riemann_es<ES ? RS : 33>(UL, UR, nv, kappa, lam, F);
-  else riemann<RS>(UL, UR, nv, kappa, F);
+  else riemann<RS>(UL, UR, nv, kappa, F, lam);
   if (PARA) {  // BR1: mean of the two sides' normal viscous fluxes
     const double *fn = FvN + (size_t)gid * 8;
 #pragma unroll

@@ -398,6 +398,21 @@ void launch_surface_n(const Batch &b)
   switch (b.cfg.riemann) {
   case 0: launch_surface_t<N, 0>(b); break;
   case 4: launch_surface_t<N, 4>(b); break;
+  case 5: launch_surface_t<N, 5>(b); break;
+  case 6: launch_surface_t<N, 6>(b); break;
+  case 7: launch_surface_t<N, 7>(b); break;
+  case 8: launch_surface_t<N, 8>(b); break;
+  case 10: launch_surface_t<N, 10>(b); break;
+  case 11: launch_surface_t<N, 11>(b); break;
+  case 12: launch_surface_t<N, 12>(b); break;
+  case 13: launch_surface_t<N, 13>(b); break;
+  case 14: launch_surface_t<N, 14>(b); break;
+  case 15: launch_surface_t<N, 15>(b); break;
+  case 17: launch_surface_t<N, 17>(b); break;
+  case 18: launch_surface_t<N, 18>(b); break;
+  case 19: launch_surface_t<N, 19>(b); break;
+  case 20: launch_surface_t<N, 20>(b); break;
+  case 21: launch_surface_t<N, 21>(b); break;
   case 9: launch_surface_t<N, 9>(b); break;
   case 16: launch_surface_t<N, 16>(b); break;
   case 32: launch_surface_t<N, 32>(b); break;
@@ -691,9 +706,8 @@ inline int check_physics(int volumeFlux, int riemann, int precision, double kapp
     return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0..10, 32)", volumeFlux);
   if (precision == FX_PREC_FP32_SPLIT && !has_fp32(volumeFlux))
     return set_err(FX_EINVAL, "VolumeFlux=%d runs in FP64 only (FP32-split: 0, 5, 10, 32)", volumeFlux);
-  if (riemann != 0 && riemann != 1 && riemann != 4 && riemann != 9 && riemann != 16 && riemann != 32 &&
-      riemann != 33 && riemann != 44)
-    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4,9,16,32,33,44)", riemann);
+  if (riemann < 0 || riemann == 2 || riemann == 3 || (riemann > 21 && riemann != 32 && riemann != 33 && riemann != 44))
+    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4-21,32,33,44)", riemann);
   if (precision != FX_PREC_FP64 && precision != FX_PREC_FP32_SPLIT)
     return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", precision);
   if (!(kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");

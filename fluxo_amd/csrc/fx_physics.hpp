@@ -108,6 +108,7 @@ __host__ __device__ constexpr bool uses_field(int f)
        : VF == 4 ? (f <= NF_P || f == NF_VV)
        : VF == 6 ? (f <= NF_V3 || f == NF_X || f == NF_VV)
        : VF == 7 ? (f <= NF_P || f == NF_X || f == NF_LNX || f == NF_VV)
+       : VF == 107 ? (f <= NF_LNX || f == NF_VV)
        : VF == 9 ? (f <= NF_V3 || f == NF_E)
        : (f <= NF_P || f == NF_E);
 }
@@ -147,9 +148,9 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
   } else if (VF == 10) {
     double z1 = sqrt(u[0] / p), z5 = z1 * p;
     f[NF_RHO] = z1; f[NF_LNRHO] = log(z1); f[NF_X] = z5; f[NF_LNX] = log(z5);
-  } else if (VF == 6 || VF == 7) {
+  } else if (VF == 6 || VF == 7 || VF == 107) {
     double X = 0.5 * u[0] / p;
-    f[NF_LNRHO] = 0.0; f[NF_X] = X; f[NF_LNX] = VF == 7 ? log(X) : 0.0;
+    f[NF_LNRHO] = VF == 107 ? log(u[0]) : 0.0; f[NF_X] = X; f[NF_LNX] = VF != 6 ? log(X) : 0.0;
   } else if (VF == 2 || VF == 8 || VF == 9) {
     f[NF_E] = VF == 2 ? srho * u[4] : (u[4] + p) * srho;
     if (VF == 9) f[NF_RHO] = sqrt(u[0]);
@@ -169,7 +170,8 @@ __device__ __forceinline__ void node_fields(const double *u, double kappaM1, dou
 //  VF=3 : DucrosFluxVec (flux_average.f90:986-1059)
 //  VF=4 : MorinishiFluxVec (flux_average.f90:1121-1205), kappa/(kappa-1) = 1 + 1/(kappa-1)
 //  VF=6 : EntropyAndEnergyConservingFluxVec2 (flux_average.f90:1427-1504)
-//  VF=7 : ggFluxVec (flux_average.f90:1580-1653)
+//  VF=7 : ggFluxVec (flux_average.f90:1580-1653); VF=107 its surface form ggflux
+//         (flux_average.f90:1513-1574), whose mass flux takes the log mean of rho
 //  VF=9 : GassnerWintersWalchFluxVec (flux_average.f90:1724-1799), (kappa-1)/kappa =
 //         1/(1 + 1/(kappa-1))
 //  VF=0 : StandardDGFluxVec (flux_average.f90:394-439)
@@ -273,9 +275,10 @@ __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL
     F[4] = R(0.5) * ((R(1) + skappaM1) * (L[NF_P] * qL + Rt[NF_P] * qR) - (rqL * L[NF_VV] + rqR * Rt[NF_VV])) +
            rqL * (L[NF_V1] * uH + L[NF_V2] * vH + L[NF_V3] * wH) +
            rqR * (Rt[NF_V1] * uH + Rt[NF_V2] * vH + Rt[NF_V3] * wH);
-  } else if (VF == 6 || VF == 7) {
+  } else if (VF == 6 || VF == 7 || VF == 107) {
     R m0 = R(0.5) * (mL[0] + mR[0]), m1 = R(0.5) * (mL[1] + mR[1]), m2 = R(0.5) * (mL[2] + mR[2]);
-    R rho_mean = R(0.5) * (L[NF_RHO] + Rt[NF_RHO]);
+    R rho_mean = VF == 107 ? ln_mean(L[NF_RHO], Rt[NF_RHO], L[NF_LNRHO], Rt[NF_LNRHO])
+                           : R(0.5) * (L[NF_RHO] + Rt[NF_RHO]);
     R uH = R(0.5) * (L[NF_V1] + Rt[NF_V1]);
     R vH = R(0.5) * (L[NF_V2] + Rt[NF_V2]);
     R wH = R(0.5) * (L[NF_V3] + Rt[NF_V3]);
@@ -327,9 +330,24 @@ __device__ __forceinline__ void volume_flux(const R *L, const R *Rt, const R *mL
 //         flux_average.f90:1806-1866)
 //  RS=9 : RiemannSolver_VolumeFluxAverage + EntropyAndEnergyConservingFlux
 //         (flux_average.f90:1214-1270)
+// Two-point flux of the volume-flux-average surface fluxes (equation.f90:292-404): Riemann
+// 6 / 14 Kennedy-Gruber 1, 7 / 15 Ducros, 8 / 19 Morinishi, 10 EC with arithmetic means,
+// 11 / 17 gg (surface form), 12 / 18 Kennedy-Gruber 2, 20 / 21 Gassner-Winters-Walch; -1 for
+// the others.  Riemann 5 (Ismail-Roe) runs through riemann_es.
+__host__ __device__ constexpr int riemann_volflux(int RS)
+{
+  return (RS == 6 || RS == 14) ? 2 : (RS == 7 || RS == 15) ? 3 : (RS == 8 || RS == 19) ? 4 : RS == 10 ? 6
+       : (RS == 11 || RS == 17) ? 107 : (RS == 12 || RS == 18) ? 8 : (RS == 20 || RS == 21) ? 9 : -1;
+}
+// Surface fluxes that take the face-wide largest wave speed LambdaMaxMax (riemann.f90:1011-1209).
+__host__ __device__ constexpr bool riemann_face_lambda(int RS)
+{
+  return RS == 13 || RS == 14 || RS == 15 || RS == 16 || RS == 17 || RS == 18 || RS == 19 || RS == 21;
+}
+
 template <int RS>
 __device__ __forceinline__ void riemann(const double *UL, const double *UR, const double *nv,
-                                        double kappa, double *F)
+                                        double kappa, double *F, double lam_face = 0.0)
 {
   const double km1 = kappa - 1.0;
   double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
@@ -358,6 +376,18 @@ __device__ __forceinline__ void riemann(const double *UL, const double *UR, cons
       double lam = fmax(fabs(mnL / UL[0]), fabs(mnR / UR[0])) + sqrt(fmax(c2L, c2R));
 #pragma unroll
       for (int v = 0; v < 5; v++) F[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
+    }
+  } else if constexpr (riemann_volflux(RS) >= 0) {
+    // RiemannSolver_VolumeFluxAverage (riemann.f90:980-1005) and, with the face's
+    // LambdaMaxMax, RiemannSolver_VolumeFluxAverage_LLF (:1118-1151)
+    constexpr int VFS = riemann_volflux(RS);
+    double fL[NF_COUNT], fR[NF_COUNT];
+    node_fields<VFS>(UL, km1, fL);
+    node_fields<VFS>(UR, km1, fR);
+    volume_flux<VFS, double>(fL, fR, nv, nv, 1.0 / km1, F);
+    if (riemann_face_lambda(RS)) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = F[v] - 0.5 * lam_face * (UR[v] - UL[v]);
     }
   } else {
     double vv_L = vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2];
@@ -403,7 +433,7 @@ __device__ __forceinline__ void riemann_es(const double *UL, const double *UR, c
   const double vnL = vL[0] * n[0] + vL[1] * n[1] + vL[2] * n[2];
   const double vnR = vR[0] * n[0] + vR[1] * n[1] + vR[2] * n[2];
   double rhoH, vh[3], p1H, aH, HH, Fc[5];
-  if (RS == 4) {  // Ismail-Roe
+  if (RS == 4 || RS == 5 || RS == 13) {  // Ismail-Roe
     const double z1L = sqrt(UL[0] / pL), z1R = sqrt(UR[0] / pR);
     const double z1M = 0.5 * (z1L + z1R), z1LN = ln_mean2(z1L, z1R);
     const double z5L = pL * z1L, z5R = pR * z1R;
@@ -421,6 +451,11 @@ __device__ __forceinline__ void riemann_es(const double *UL, const double *UR, c
 #pragma unroll
     for (int d = 0; d < 3; d++) Fc[1 + d] = Fc[0] * vh[d] + p1H * n[d];
     Fc[4] = Fc[0] * HH;
+    if (RS == 5) {  // RiemannSolver_VolumeFluxAverage with TwoPointEntropyConservingFlux
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = Fc[v];
+      return;
+    }
   } else if (RS == 44 || RS == 16) {  // Chandrashekar
     const double v2s2L = 0.5 * (vL[0] * vL[0] + vL[1] * vL[1] + vL[2] * vL[2]);
     const double v2s2R = 0.5 * (vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]);
@@ -478,6 +513,18 @@ __device__ __forceinline__ void riemann_es(const double *UL, const double *UR, c
   const double dmn = dm[0] * n[0] + dm[1] * n[1] + dm[2] * n[2];
   const double dmv = dm[0] * vh[0] + dm[1] * vh[1] + dm[2] * vh[2];
   const double vh2 = vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2];
+  if (RS == 13) {
+    // RiemannSolver_EC_LLF (riemann.f90:1011-1112): Fc - 1/2 lam H [v] with H = dU/dv at the
+    // Ismail-Roe averages (a Cartesian-frame form of the reference's rotated 5x5 matrix)
+    const double rhoea = p1H * skm1 + 0.5 * rhoH * vh2;
+    const double h0 = rhoH * dv0 + rhoH * dmv + rhoea * dv4;
+    F[0] = Fc[0] - 0.5 * lam_face * h0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      F[1 + d] = Fc[1 + d] - 0.5 * lam_face * (rhoH * vh[d] * (dv0 + dmv + HH * dv4) + p1H * dm[d]);
+    F[4] = Fc[4] - 0.5 * lam_face * (rhoea * dv0 + rhoH * HH * dmv + (rhoH * HH * HH - aH * aH * p1H * skm1) * dv4);
+    return;
+  }
   const double c1 = D1 * (dv0 + dmv - aH * dmn + (HH - un * aH) * dv4);
   const double c2 = D2 * (dv0 + dmv + 0.5 * vh2 * dv4);
   const double c5 = D5 * (dv0 + dmv + aH * dmn + (HH + un * aH) * dv4);

@@ -48,7 +48,12 @@ typedef struct {
   int riemann;      /* 'Riemann' (navierstokes/equation.f90:291-404): 0 central, 1 LLF,
                        9 / 32 EC (Chandrashekar / Ranocha), 4 / 44 / 33 entropy stable
                        (Ismail-Roe / Chandrashekar / Ranocha + matrix dissipation),
-                       16 EC-KEP + face LLF; MHD (mhd/equation.f90:327-386): 0 central,
+                       16 EC-KEP + face LLF, 5-8 / 10-12 / 20 a two-point flux in the
+                       normal direction (Ismail-Roe, Kennedy-Gruber 1, Ducros, Morinishi,
+                       EC2, gg, Kennedy-Gruber 2, Gassner-Winters-Walch), 14 / 15 / 17 / 18 /
+                       19 / 21 the same + face LLF, 13 Ismail-Roe + entropy-Jacobian face
+                       LLF (2, 3, 22-24 HLL-type and Roe are not built); MHD
+                       (mhd/equation.f90:327-386): 0 central,
                        1 Rusanov, 10 Derigs EC + entropy-stable LLF, 11 Derigs EC */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /
                        resistive MHD */

@@ -434,6 +434,96 @@ static void riemann_eckep_llf(double kappa, const double *uLL, const double *uRR
                                       0.5 / km1 * (1.0 / bR - 1.0 / bL)));
 }
 
+/* The 1D flux of RiemannSolver_VolumeFluxAverage (riemann.f90:980-1005) and
+   RiemannSolver_VolumeFluxAverage_LLF (:1118-1151) in the rotated frame, selected as
+   equation.f90:292-404 does: 5 Ismail-Roe (TwoPointEntropyConservingFlux), 6 / 14
+   KennedyAndGruberFlux1, 7 / 15 DucrosFlux, 8 / 19 MorinishiFlux, 10
+   EntropyAndEnergyConservingFlux2, 11 / 17 ggflux, 12 / 18 KennedyAndGruberFlux2, 20 / 21
+   GassnerWintersWalchFlux.  These equal the metric forms of the volume fluxes with the
+   metric (1,0,0), except ggflux (flux_average.f90:1513-1574), whose mass flux uses the log
+   mean of rho where ggFluxVec uses the arithmetic mean. */
+static int riemann_volflux_of(int which)
+{
+  switch (which) {
+  case 5: return 10;
+  case 6: case 14: return 2;
+  case 7: case 15: return 3;
+  case 8: case 19: return 4;
+  case 10: return 6;
+  case 11: case 17: return 7;
+  case 12: case 18: return 8;
+  case 20: case 21: return 9;
+  default: return -1;
+  }
+}
+
+static void riemann_volflux_avg(int which, double kappa, const double *uLL, const double *uRR, double lam,
+                                double *f)
+{
+  const double e1[3] = {1.0, 0.0, 0.0};
+  double aL[6], aR[6];
+  ora_eval_uaux(kappa, uLL, aL);
+  ora_eval_uaux(kappa, uRR, aR);
+  if (which == 5) {  /* TwoPointEntropyConservingFlux with its energy flux F1*HHat */
+    hat_t h;
+    flux1d_hat(4, kappa, uLL, uRR, f, &h);
+    return;
+  }
+  ora_volflux(riemann_volflux_of(which), kappa, uLL, uRR, aL, aR, e1, e1, f);
+  if (which == 11 || which == 17) {  /* ggflux: F1 = LN_MEAN(rho) uHat */
+    const double uH = 0.5 * (aL[1] + aR[1]), vH = 0.5 * (aL[2] + aR[2]), wH = 0.5 * (aL[3] + aR[3]);
+    const double bL = 0.5 * uLL[0] / aL[4], bR = 0.5 * uRR[0] / aR[4];
+    const double p1 = 0.5 * (aL[4] + aR[4]);
+    f[0] = ora_ln_mean(uLL[0], uRR[0]) * uH;
+    f[1] = f[0] * uH + p1;
+    f[2] = f[0] * vH;
+    f[3] = f[0] * wH;
+    f[4] = f[0] * 0.5 * (1.0 / ((kappa - 1.0) * ora_ln_mean(bL, bR)) - 0.5 * (aL[5] + aR[5])) + uH * f[1] +
+           vH * f[2] + wH * f[3];
+  }
+  if (which == 14 || which == 15 || which == 17 || which == 18 || which == 19 || which == 21)
+    for (int v = 0; v < 5; v++) f[v] = f[v] - 0.5 * lam * (uRR[v] - uLL[v]);
+}
+
+/* RiemannSolver_EC_LLF (riemann.f90:1011-1112): the Ismail-Roe flux minus 1/2 lam H [v],
+   H = dU/dv at the Ismail-Roe averages, [v] the jump of the entropy variables, lam the
+   face's LambdaMaxMax; rotated frame. */
+static void riemann_ec_llf(double kappa, const double *uLL, const double *uRR, double lam, double *f)
+{
+  const double km1 = kappa - 1.0;
+  double Fc[5], dv[5], H[5][5];
+  hat_t h;
+  flux1d_hat(4, kappa, uLL, uRR, Fc, &h);
+  /* [v] with the reference's evaluation order: s = ln p - kappa ln rho */
+  const double *U2[2] = {uLL, uRR};
+  for (int c = 0; c < 5; c++) dv[c] = 0.0;
+  for (int k = 0; k < 2; k++) {
+    const double *u = U2[k];
+    const double sr = 1.0 / u[0], vel[3] = {u[1] * sr, u[2] * sr, u[3] * sr};
+    const double p = km1 * (u[4] - 0.5 * (u[1] * vel[0] + u[2] * vel[1] + u[3] * vel[2]));
+    const double se = log(p) - kappa * log(u[0]);
+    const double w1 = (kappa - se) / km1 - 0.5 * u[0] * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) / p;
+    dv[0] = k == 0 ? -w1 : dv[0] + w1;
+    for (int d = 0; d < 3; d++) dv[1 + d] = k == 0 ? -u[0] * vel[d] / p : dv[1 + d] + u[0] * vel[d] / p;
+    dv[4] = k == 0 ? u[0] / p : dv[4] - u[0] / p;
+  }
+  const double va[3] = {h.u, h.v, h.w};
+  const double rhoea = h.p1 / km1 + 0.5 * h.rho * (h.u * h.u + h.v * h.v + h.w * h.w);
+  H[0][0] = h.rho;
+  H[4][4] = h.rho * h.H * h.H - h.a * h.a * h.p1 / km1;
+  H[0][4] = H[4][0] = rhoea;
+  for (int i = 0; i < 3; i++) {
+    H[0][1 + i] = H[1 + i][0] = h.rho * va[i];
+    H[4][1 + i] = H[1 + i][4] = h.rho * va[i] * h.H;
+    for (int j = 0; j < 3; j++) H[1 + i][1 + j] = h.rho * va[i] * va[j] + (i == j ? h.p1 : 0.0);
+  }
+  for (int r = 0; r < 5; r++) {
+    double d = 0.0;
+    for (int c = 0; c < 5; c++) d += H[r][c] * dv[c];
+    f[r] = Fc[r] - 0.5 * lam * d;
+  }
+}
+
 /* the largest wave speed of one face point in the rotated frame (the LambdaMaxMax term) */
 double ora_face_lambda(double kappa, const double *UL, const double *UR, const double *nv)
 {
@@ -447,8 +537,10 @@ double ora_face_lambda(double kappa, const double *UL, const double *UR, const d
 /* Riemann -> AdvRiemann: rotate, solve, rotate back (riemann.f90:174-246); solvers:
    0 central (:251-272), 1 Rusanov/LLF (:279-313), 9 EC-KEP and 32 Ranocha through
    RiemannSolver_VolumeFluxAverage (:980-1005), 4 / 44 / 33 entropy stable with the
-   Ismail-Roe / Chandrashekar / Ranocha flux (:874-975), 16 EC-KEP + LLF (:1157-1209) with the
-   face's LambdaMaxMax in lam_face. Point-wise (one face node). */
+   Ismail-Roe / Chandrashekar / Ranocha flux (:874-975), 16 EC-KEP + LLF (:1157-1209) and
+   13 Ismail-Roe + LLF (:1011-1112) with the face's LambdaMaxMax in lam_face, and the
+   volume-flux averages 5-8, 10-12, 20 and their LLF forms 14, 15, 17-19, 21.  Point-wise
+   (one face node). */
 void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double lam_face, double *F)
 {
@@ -480,6 +572,10 @@ void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR
     riemann_es(which, kappa, uLL, uRR, f);
   } else if (which == 16) {
     riemann_eckep_llf(kappa, uLL, uRR, lam_face, f);
+  } else if (which == 13) {
+    riemann_ec_llf(kappa, uLL, uRR, lam_face, f);
+  } else if (riemann_volflux_of(which) >= 0) {
+    riemann_volflux_avg(which, kappa, uLL, uRR, lam_face, f);
   } else {
     flux1d_ranocha(kappa, uLL, uRR, f);
   }

@@ -59,7 +59,13 @@ CASES = [
     ("euler_n3_vf7_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 7, 1, False, 2, 0.05, 3),
     ("euler_n5_vf8_llf_warped", 2, 5, 3, True, (0, 0, 0), (2, 2, 2), 8, 1, False, 2, 0.05, 3),
     ("euler_n3_vf9_llf_warped", 3, 3, 3, True, (0, 0, 0), (2, 2, 2), 9, 1, False, 2, 0.05, 3),
-]
+    # the other surface fluxes: volume-flux averages in the normal direction (Riemann 5-8,
+    # 10-12, 20), their LLF forms with the face's largest wave speed (14, 15, 17-19, 21) and
+    # Ismail-Roe + entropy-Jacobian LLF (13)
+] + [(f"euler_n{N}_riemann{rs}_warped", 2, N, 3, True, (0, 0, 0), (2, 2, 2), vf, rs, False, 2, 0.05, 3)
+     for rs, N, vf in ((5, 3, 10), (6, 2, 32), (7, 3, 32), (8, 4, 32), (10, 3, 32), (11, 2, 32), (12, 3, 32),
+                       (13, 3, 10), (14, 4, 32), (15, 2, 32), (17, 3, 32), (18, 4, 32), (19, 3, 32), (20, 2, 32),
+                       (21, 3, 32))]
 
 # NFVSE shock capturing (the reference built with SHOCK_NFVSE): a Gaussian pressure pulse
 # (fluxo_amd.workloads.sedov_blast with the width and background pressure given) on warped

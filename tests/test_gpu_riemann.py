@@ -1,8 +1,8 @@
 """Surface fluxes and the Ismail-Roe volume flux on the GPU against the oracle: the
 entropy-stable solvers (Riemann 4, 33, 44, in the Cartesian frame) and EC-KEP + face LLF (16,
 a face-wide maximum wave speed), on warped meshes with perturbed states, N = 2..5 (relative
-L2 <= 1e-12 per RHS), Navier-Stokes with 33, a 10-step trajectory (<= 1e-10), and VolumeFlux
-10.  The reference's own outputs are checked in test_gpu_golden (euler_*_es*,
+L2 <= 1e-12 per RHS), Navier-Stokes with 33, a 10-step trajectory (<= 1e-10), VolumeFlux
+10, and the other surface fluxes 5-8, 10-15, 17-21.  The reference's own outputs are checked in test_gpu_golden (euler_*_es*,
 *_eckep_llf16_*, ns_*_es33_*, *_ismailroe_*)."""
 import math
 
@@ -34,6 +34,39 @@ def test_surface_flux_rhs(rs, N):
     e.upload(U)
     err = rel_l2(e.rhs(), O.Problem(m, volflux=32, riemann=rs).rhs(U))
     print(f"Riemann {rs} N={N}: RHS relL2 {err:.3e}")
+    assert err <= 1e-12
+
+
+NEW_RS = [5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21]
+
+
+@pytest.mark.parametrize("rs", NEW_RS)
+@pytest.mark.parametrize("N", [2, 5])
+def test_other_surface_fluxes_rhs(rs, N):
+    """Riemann 5-8, 10-15, 17-21 (volume-flux averages in the normal direction, their LLF
+    forms with the face's largest wave speed, Ismail-Roe + entropy-Jacobian LLF) against the
+    oracle; N = 3, 4 are in the golden vectors (euler_*_riemann*)."""
+    from fluxo_amd.native import Engine
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True)
+    U = state(m)
+    vf = 10 if rs in (5, 13) else 32
+    e = Engine(m, volume_flux=vf, riemann=rs)
+    e.upload(U)
+    err = rel_l2(e.rhs(), O.Problem(m, volflux=vf, riemann=rs).rhs(U))
+    print(f"Riemann {rs} N={N}: RHS relL2 {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("rs", [13, 17])
+def test_other_surface_fluxes_navier_stokes(rs):
+    from fluxo_amd.native import Engine
+    m = O.box_mesh(3, 3, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3, Ngeo=3, warp=True)
+    U = O.exactfunc(7, m["Elem_xGP"])
+    kw = dict(parabolic=True, mu0=1e-3, Pr=0.72)
+    e = Engine(m, volume_flux=32, riemann=rs, **kw)
+    e.upload(U)
+    err = rel_l2(e.rhs(), O.Problem(m, volflux=32, riemann=rs, **kw).rhs(U))
+    print(f"Navier-Stokes Riemann {rs}: RHS relL2 {err:.3e}")
     assert err <= 1e-12
 
 

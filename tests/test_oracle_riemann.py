@@ -42,22 +42,22 @@ def flux(rs, UL, UR, n, t1, t2, lam=None):
     return F
 
 
-@pytest.mark.parametrize("rs", [4, 9, 16, 32, 33, 44])
+@pytest.mark.parametrize("rs", [4, 5, 9, 13, 16, 32, 33, 44])
 def test_entropy_condition_and_consistency(rs):
     rng = np.random.default_rng(rs)
     for _ in range(30):
         UL, UR = random_state(rng), random_state(rng)
         n, t1, t2 = frame(rng)
-        F = flux(rs, UL, UR, n, t1, t2, lam=2.5 if rs == 16 else None)
+        F = flux(rs, UL, UR, n, t1, t2, lam=2.5 if rs in (13, 16) else None)
         dv = entropy_vars(UR) - entropy_vars(UL)
         dpsi = UR[1:4] @ n - UL[1:4] @ n
         prod = dv @ F - dpsi
-        if rs in (9, 32):
+        if rs in (5, 9, 32):
             assert abs(prod) <= 1e-12 * (1 + abs(dpsi))
         else:
             assert prod < -1e-8
         # equal states: the physical flux
-        Fe = flux(rs, UL, UL, n, t1, t2, lam=2.5 if rs == 16 else None)
+        Fe = flux(rs, UL, UL, n, t1, t2, lam=2.5 if rs in (13, 16) else None)
         v = UL[1:4] / UL[0]
         p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
         vn = v @ n
@@ -112,3 +112,33 @@ def test_other_volume_fluxes_consistency_symmetry_kep(vf):
             va = 0.5 * (UL[1:4] / UL[0] + UR[1:4] / UR[0])
             pt = (F[1:4] - F[0] * va) / m
             assert np.ptp(pt) <= 1e-12 * np.abs(pt).max(), pt
+
+
+LLF_OF = {14: 6, 15: 7, 17: 11, 18: 12, 19: 8, 21: 20}
+
+
+@pytest.mark.parametrize("rs", [6, 7, 8, 10, 11, 12, 20, 14, 15, 17, 18, 19, 21])
+def test_volume_flux_average_solvers(rs):
+    """Riemann 6-8, 10-12, 20 (a two-point flux in the normal direction) and their LLF forms
+    14, 15, 17-19, 21: the physical flux for equal states, independent of the choice of the
+    tangents, symmetric for the pure averages (F(L,R;n) = F(R,L;n)), and the LLF form is the
+    average minus lam/2 [U] with the face's lam."""
+    rng = np.random.default_rng(200 + rs)
+    for _ in range(20):
+        UL, UR = random_state(rng), random_state(rng)
+        n, t1, t2 = frame(rng)
+        F = flux(rs, UL, UR, n, t1, t2, lam=1.7)
+        _, s1, s2 = frame(rng)
+        s1 = s1 - (s1 @ n) * n
+        s1 /= np.linalg.norm(s1)
+        assert np.allclose(F, flux(rs, UL, UR, n, s1, np.cross(n, s1), lam=1.7), rtol=1e-12, atol=1e-12)
+        if rs in LLF_OF:
+            Fa = flux(LLF_OF[rs], UL, UR, n, t1, t2)
+            assert np.allclose(F, Fa - 0.85 * (UR - UL), rtol=1e-13, atol=1e-13)
+        else:
+            assert np.allclose(F, flux(rs, UR, UL, n, t1, t2), rtol=1e-13, atol=1e-13)
+        Fe = flux(rs, UL, UL, n, t1, t2, lam=1.7)
+        v = UL[1:4] / UL[0]
+        p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
+        vn = v @ n
+        assert np.allclose(Fe, [UL[0] * vn, *(UL[1:4] * vn + p * n), (UL[4] + p) * vn], rtol=1e-12, atol=1e-12)
