@@ -398,6 +398,18 @@ void launch_surface_n(const Batch &b)
   switch (b.cfg.riemann) {
   case 0: launch_surface_t<N, 0>(b); break;
   case 4: launch_surface_t<N, 4>(b); break;
+  case 2: launch_surface_t<N, 2>(b); break;
+  case 22: launch_surface_t<N, 22>(b); break;
+  case 23: launch_surface_t<N, 23>(b); break;
+  case 24: launch_surface_t<N, 24>(b); break;
+  case 3:  // Roe's central part follows the volume flux (riemann.f90:836-856)
+    switch (b.cfg.volumeFlux) {
+    case 2: launch_surface_t<N, 302>(b); break;
+    case 3: launch_surface_t<N, 303>(b); break;
+    case 4: launch_surface_t<N, 308>(b); break;
+    default: launch_surface_t<N, 3>(b); break;
+    }
+    break;
   case 5: launch_surface_t<N, 5>(b); break;
   case 6: launch_surface_t<N, 6>(b); break;
   case 7: launch_surface_t<N, 7>(b); break;
@@ -706,8 +718,8 @@ inline int check_physics(int volumeFlux, int riemann, int precision, double kapp
     return set_err(FX_EINVAL, "VolumeFlux=%d not supported (0..10, 32)", volumeFlux);
   if (precision == FX_PREC_FP32_SPLIT && !has_fp32(volumeFlux))
     return set_err(FX_EINVAL, "VolumeFlux=%d runs in FP64 only (FP32-split: 0, 5, 10, 32)", volumeFlux);
-  if (riemann < 0 || riemann == 2 || riemann == 3 || (riemann > 21 && riemann != 32 && riemann != 33 && riemann != 44))
-    return set_err(FX_EINVAL, "Riemann=%d not supported (0,1,4-21,32,33,44)", riemann);
+  if (riemann < 0 || (riemann > 24 && riemann != 32 && riemann != 33 && riemann != 44))
+    return set_err(FX_EINVAL, "Riemann=%d not supported (0-24,32,33,44)", riemann);
   if (precision != FX_PREC_FP64 && precision != FX_PREC_FP32_SPLIT)
     return set_err(FX_EINVAL, "precision %d not supported (0 FP64, 1 FP32-split)", precision);
   if (!(kappa > 1.0)) return set_err(FX_EINVAL, "kappa must be > 1");

@@ -345,6 +345,157 @@ __host__ __device__ constexpr bool riemann_face_lambda(int RS)
   return RS == 13 || RS == 14 || RS == 15 || RS == 16 || RS == 17 || RS == 18 || RS == 19 || RS == 21;
 }
 
+// HLL-type and Roe solvers in the Cartesian frame (the reference rotates into (n, t1, t2),
+// riemann.f90:174-246; every formula below is the rotated one written with n and the
+// tangential projection v - (v.n) n):
+//   RS=2  : RiemannSolverByHLLC (riemann.f90:552-615)
+//   RS=22 : RiemannSolverByHLL (:320-388), its wave speeds from the Roe-averaged normal
+//           velocity alone
+//   RS=23 : RiemannSolverByHLLE (:396-462); RS=24 HLLEM (:468-547) with the anti-diffusion
+//           of the contact and shear waves
+//   RS=3  : RiemannSolverByRoe (:746-869) with the average of the Euler fluxes as central
+//           part; 302 / 303 / 308 with the Kennedy-Gruber 1 / Ducros / Kennedy-Gruber 2
+//           two-point flux, the reference's choice for VolumeFlux 2 / 3 / 4 (split form),
+//           302 and 308 with max(|u-c|,|u+c|) on both acoustic waves.
+template <int RS>
+__device__ __forceinline__ void riemann_hll_roe(const double *UL, const double *UR, const double *n, double kappa,
+                                                double *F)
+{
+  const double km1 = kappa - 1.0;
+  const double sL = 1.0 / UL[0], sR = 1.0 / UR[0];
+  const double vL[3] = {UL[1] * sL, UL[2] * sL, UL[3] * sL}, vR[3] = {UR[1] * sR, UR[2] * sR, UR[3] * sR};
+  const double pL = km1 * (UL[4] - 0.5 * (UL[1] * vL[0] + UL[2] * vL[1] + UL[3] * vL[2]));
+  const double pR = km1 * (UR[4] - 0.5 * (UR[1] * vR[0] + UR[2] * vR[1] + UR[3] * vR[2]));
+  const double unL = vL[0] * n[0] + vL[1] * n[1] + vL[2] * n[2], unR = vR[0] * n[0] + vR[1] * n[1] + vR[2] * n[2];
+  double FL[5], FR[5];
+  FL[0] = UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2];
+  FR[0] = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    FL[1 + d] = UL[1 + d] * unL + pL * n[d];
+    FR[1 + d] = UR[1 + d] * unR + pR * n[d];
+  }
+  FL[4] = (UL[4] + pL) * unL;
+  FR[4] = (UR[4] + pR) * unR;
+  if constexpr (RS == 2) {
+    const double Ssl = unL - sqrt(kappa * pL * sL), Ssr = unR + sqrt(kappa * pR * sR);
+    if (Ssl >= 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FL[v];
+      return;
+    }
+    if (Ssr <= 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FR[v];
+      return;
+    }
+    const double sMuL = Ssl - unL, sMuR = Ssr - unR;
+    const double SStar = (pR - pL + FL[0] * sMuL - FR[0] * sMuR) / (UL[0] * sMuL - UR[0] * sMuR);
+    const bool left = (Ssl <= 0.0) && (SStar >= 0.0);
+    const double *u = left ? UL : UR, *Fk = left ? FL : FR, *vk = left ? vL : vR;
+    const double sk = left ? sL : sR, sMu = left ? sMuL : sMuR, Ss = left ? Ssl : Ssr, pk = left ? pL : pR;
+    const double unk = left ? unL : unR;
+    const double c = u[0] * sMu / (Ss - SStar);
+    F[0] = Fk[0] + Ss * (c - u[0]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) F[1 + d] = Fk[1 + d] + Ss * (c * (SStar * n[d] + (vk[d] - unk * n[d])) - u[1 + d]);
+    F[4] = Fk[4] + Ss * (c * (u[4] * sk + (SStar - unk) * (SStar + pk * sk / sMu)) - u[4]);
+    return;
+  }
+  const double HL = (UL[4] + pL) * sL, HR = (UR[4] + pR) * sR;
+  const double qL = sqrt(UL[0]), qR = sqrt(UR[0]);
+  if constexpr (RS == 22) {
+    const double Ht = (qL * HL + qR * HR) / (qL + qR), vt = (qL * unL + qR * unR) / (qL + qR);
+    const double at = sqrt(km1 * (Ht - 0.5 * vt * vt));
+    const double Ssl = vt - at, Ssr = vt + at;
+    if (Ssl >= 0.0 && Ssr > 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FL[v];
+    } else if (Ssr <= 0.0 && Ssl < 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FR[v];
+    } else {
+      const double sd = 1.0 / (Ssr - Ssl);
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = (Ssr * FL[v] - Ssl * FR[v] + Ssl * Ssr * (UR[v] - UL[v])) * sd;
+    }
+    return;
+  }
+  const double sq = 1.0 / (qL + qR);
+  double rv[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) rv[d] = (qR * vR[d] + qL * vL[d]) * sq;
+  const double rH = (qR * HR + qL * HL) * sq, av = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+  const double rc = sqrt(km1 * (rH - 0.5 * av));
+  const double un = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+  if constexpr (RS == 23 || RS == 24) {
+    const double beta = sqrt(0.5 * km1 / kappa);
+    const double aL = sqrt(kappa * pL * sL), aR = sqrt(kappa * pR * sR);
+    const double Ssl = fmin(fmin(un - rc, unL - beta * aL), 0.0);
+    const double Ssr = fmax(fmax(un + rc, unR + beta * aR), 0.0);
+    if (Ssl >= 0.0 && Ssr > 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FL[v];
+      return;
+    }
+    if (Ssr <= 0.0 && Ssl < 0.0) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) F[v] = FR[v];
+      return;
+    }
+    double anti[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (RS == 24) {
+      const double delta = rc / (rc + fabs(0.5 * (Ssl + Ssr)));
+      const double rd = sqrt(UL[0] * UR[0]);
+      const double A2 = (UR[0] - UL[0]) - (pR - pL) / (rc * rc);
+      double dvt[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) dvt[d] = rd * ((vR[d] - vL[d]) - ((unR - unL) * n[d]));
+      anti[0] = delta * A2;
+#pragma unroll
+      for (int d = 0; d < 3; d++) anti[1 + d] = delta * (rv[d] * A2 + dvt[d]);
+      anti[4] = delta * (0.5 * av * A2 + rv[0] * dvt[0] + rv[1] * dvt[1] + rv[2] * dvt[2]);
+    }
+    const double sd = 1.0 / (Ssr - Ssl);
+#pragma unroll
+    for (int v = 0; v < 5; v++) F[v] = (Ssr * FL[v] - Ssl * FR[v] + Ssl * Ssr * (UR[v] - UL[v] - anti[v])) * sd;
+    return;
+  }
+  // Roe: [U], the tangential momentum jump relative to the Roe velocity, the wave strengths
+  double dU[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) dU[v] = UR[v] - UL[v];
+  const double dmn = dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2];
+  double At[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) At[d] = (dU[1 + d] - rv[d] * dU[0]) - (dmn - un * dU[0]) * n[d];
+  const double dU6 = dU[4] - (At[0] * rv[0] + At[1] * rv[1] + At[2] * rv[2]);
+  const double A2 = km1 / (rc * rc) * (dU[0] * (rH - un * un) - dU6 + un * dmn);
+  const double A1 = 0.5 / rc * (dU[0] * (un + rc) - dmn - rc * A2);
+  const double A5 = dU[0] - A1 - A2;
+  double a1 = fabs(un - rc), a5 = fabs(un + rc);
+  const double a2 = fabs(un);
+  double Fc[5];
+  if constexpr (RS == 302 || RS == 303 || RS == 308) {
+    constexpr int VFC = RS - 300;
+    double fL[NF_COUNT], fR[NF_COUNT];
+    node_fields<VFC>(UL, km1, fL);
+    node_fields<VFC>(UR, km1, fR);
+    volume_flux<VFC, double>(fL, fR, n, n, 1.0 / km1, Fc);
+    if (RS != 303) { a1 = fmax(a1, a5); a5 = a1; }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; v++) Fc[v] = 0.5 * (FL[v] + FR[v]);
+  }
+  const double w1 = A1 * a1, w2 = A2 * a2, w5 = A5 * a5;
+  F[0] = Fc[0] - 0.5 * (w1 + w2 + w5);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    F[1 + d] = Fc[1 + d] - 0.5 * (w1 * (rv[d] - rc * n[d]) + w2 * rv[d] + a2 * At[d] + w5 * (rv[d] + rc * n[d]));
+  F[4] = Fc[4] - 0.5 * (w1 * (rH - un * rc) + w2 * 0.5 * av + a2 * (At[0] * rv[0] + At[1] * rv[1] + At[2] * rv[2]) +
+                        w5 * (rH + un * rc));
+}
+
 template <int RS>
 __device__ __forceinline__ void riemann(const double *UL, const double *UR, const double *nv,
                                         double kappa, double *F, double lam_face = 0.0)
@@ -377,6 +528,9 @@ __device__ __forceinline__ void riemann(const double *UL, const double *UR, cons
 #pragma unroll
       for (int v = 0; v < 5; v++) F[v] = 0.5 * ((FL[v] + FR[v]) - lam * (UR[v] - UL[v]));
     }
+  } else if constexpr (RS == 2 || RS == 22 || RS == 23 || RS == 24 || RS == 3 || RS == 302 || RS == 303 ||
+                       RS == 308) {
+    riemann_hll_roe<RS>(UL, UR, nv, kappa, F);
   } else if constexpr (riemann_volflux(RS) >= 0) {
     // RiemannSolver_VolumeFluxAverage (riemann.f90:980-1005) and, with the face's
     // LambdaMaxMax, RiemannSolver_VolumeFluxAverage_LLF (:1118-1151)

@@ -52,7 +52,8 @@ typedef struct {
                        normal direction (Ismail-Roe, Kennedy-Gruber 1, Ducros, Morinishi,
                        EC2, gg, Kennedy-Gruber 2, Gassner-Winters-Walch), 14 / 15 / 17 / 18 /
                        19 / 21 the same + face LLF, 13 Ismail-Roe + entropy-Jacobian face
-                       LLF (2, 3, 22-24 HLL-type and Roe are not built); MHD
+                       LLF, 2 HLLC, 22 HLL, 23 HLLE, 24 HLLEM, 3 Roe (central part from
+                       VolumeFlux 2 / 3 / 4 as the reference's split form does); MHD
                        (mhd/equation.f90:327-386): 0 central,
                        1 Rusanov, 10 Derigs EC + entropy-stable LLF, 11 Derigs EC */
   int parabolic;    /* PARABOLIC (CMakeLists.txt:253): 0 Euler / ideal MHD, 1 Navier-Stokes /

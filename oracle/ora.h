@@ -68,7 +68,7 @@ typedef struct {
                         2/8 Kennedy-Gruber, 3 Ducros, 4 Morinishi, 5 Chandrashekar EC-KEP,
                         6 EC with arithmetic means, 7 gg, 9 Gassner-Winters-Walch,
                         10 Ismail-Roe EC, 32 Ranocha EC-KEP */
-  int riemann;       /* Riemann: 1 LLF (Rusanov), 5 EC Chandrashekar surface, 32 EC Ranocha surface */
+  int riemann;       /* Riemann: 0-24, 32, 33, 44 (navierstokes/equation.f90:292-404) */
   int parabolic;     /* 0 Euler, 1 Navier-Stokes */
   int lifting;       /* 1 BR1, 2 BR2 */
   double kappa, mu0, Pr, etaBR2;
@@ -89,12 +89,15 @@ double ora_ln_mean(double aL, double aR);
 void ora_eval_uaux(double kappa, const double *U, double *aux);
 void ora_volflux(int which, double kappa, const double *UL, const double *UR, const double *auxL,
                  const double *auxR, const double *mL, const double *mR, double *F);
-/* numerical flux at one face point (Riemann 0, 1, 4, 9, 16, 32, 33, 44); Riemann 16 takes the
-   face's largest wave speed lam_face (ora_riemann uses the point's own) */
+/* numerical flux at one face point (Riemann 0-24, 32, 33, 44); Riemann 13-19 and 21 take the
+   face's largest wave speed lam_face (ora_riemann uses the point's own); Roe (3) takes its
+   central part from volflux (ora_riemann_lam_vf; the others pass 0) */
 void ora_riemann(int which, double kappa, const double *UL, const double *UR, const double *nv,
                  const double *t1, const double *t2, double *F);
 void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double lam_face, double *F);
+void ora_riemann_lam_vf(int which, int volflux, double kappa, const double *UL, const double *UR,
+                        const double *nv, const double *t1, const double *t2, double lam_face, double *F);
 double ora_face_lambda(double kappa, const double *UL, const double *UR, const double *nv);
 void ora_cons_to_entropy(double kappa, const double *U, double *w);
 

@@ -524,6 +524,110 @@ static void riemann_ec_llf(double kappa, const double *uLL, const double *uRR, d
   }
 }
 
+/* HLL-type and Roe solvers in the rotated frame: RiemannSolverByHLL (riemann.f90:320-388),
+   ByHLLE (:396-462), ByHLLEM (:468-547), ByHLLC (:552-615) and ByRoe (:746-869), whose central
+   part follows the volume flux (PP_DiscType 2: VolumeFlux 2 KennedyAndGruberFlux1, 3
+   DucrosFlux, 4 KennedyAndGruberFlux2 (sic), otherwise the average of the Euler fluxes; with
+   2 and 4 the acoustic waves take max(|u-c|,|u+c|)). */
+static void riemann_hll_roe(int which, int volflux, double kappa, const double *uLL, const double *uRR, double *f)
+{
+  const double km1 = kappa - 1.0;
+  double FL[5], FR[5];
+  euler_flux_1d(kappa, uLL, FL);
+  euler_flux_1d(kappa, uRR, FR);
+  const double sL = 1.0 / uLL[0], sR = 1.0 / uRR[0];
+  const double vL[3] = {uLL[1] * sL, uLL[2] * sL, uLL[3] * sL}, vR[3] = {uRR[1] * sR, uRR[2] * sR, uRR[3] * sR};
+  const double pL = km1 * (uLL[4] - 0.5 * (uLL[1] * vL[0] + uLL[2] * vL[1] + uLL[3] * vL[2]));
+  const double pR = km1 * (uRR[4] - 0.5 * (uRR[1] * vR[0] + uRR[2] * vR[1] + uRR[3] * vR[2]));
+  if (which == 2) {  /* HLLC */
+    const double Ssl = vL[0] - sqrt(kappa * pL * sL), Ssr = vR[0] + sqrt(kappa * pR * sR);
+    if (Ssl >= 0.0) { for (int v = 0; v < 5; v++) f[v] = FL[v]; return; }
+    if (Ssr <= 0.0) { for (int v = 0; v < 5; v++) f[v] = FR[v]; return; }
+    const double sMuL = Ssl - vL[0], sMuR = Ssr - vR[0];
+    const double SStar = (pR - pL + uLL[1] * sMuL - uRR[1] * sMuR) / (uLL[0] * sMuL - uRR[0] * sMuR);
+    const int left = (Ssl <= 0.0) && (SStar >= 0.0);
+    const double *u = left ? uLL : uRR, *Fk = left ? FL : FR, *vk = left ? vL : vR;
+    const double sk = left ? sL : sR, sMu = left ? sMuL : sMuR, Ss = left ? Ssl : Ssr, pk = left ? pL : pR;
+    const double c = u[0] * sMu / (Ss - SStar);
+    const double Us[5] = {c, c * SStar, c * u[2] * sk, c * u[3] * sk,
+                          c * (u[4] * sk + (SStar - vk[0]) * (SStar + pk * sk / sMu))};
+    for (int v = 0; v < 5; v++) f[v] = Fk[v] + Ss * (Us[v] - u[v]);
+    return;
+  }
+  const double HL = (uLL[4] + pL) * sL, HR = (uRR[4] + pR) * sR;
+  const double qL = sqrt(uLL[0]), qR = sqrt(uRR[0]);
+  if (which == 22) {  /* HLL */
+    const double Ht = (qL * HL + qR * HR) / (qL + qR), vt = (qL * vL[0] + qR * vR[0]) / (qL + qR);
+    const double at = sqrt((kappa - 1.0) * (Ht - 0.5 * vt * vt));
+    const double Ssl = vt - at, Ssr = vt + at;
+    if (Ssl >= 0.0 && Ssr > 0.0) { for (int v = 0; v < 5; v++) f[v] = FL[v]; return; }
+    if (Ssr <= 0.0 && Ssl < 0.0) { for (int v = 0; v < 5; v++) f[v] = FR[v]; return; }
+    for (int v = 0; v < 5; v++) f[v] = (Ssr * FL[v] - Ssl * FR[v] + Ssl * Ssr * (uRR[v] - uLL[v])) / (Ssr - Ssl);
+    return;
+  }
+  const double sq = 1.0 / (qL + qR);
+  double rv[3];
+  for (int d = 0; d < 3; d++) rv[d] = (qR * vR[d] + qL * vL[d]) * sq;
+  const double rH = (qR * HR + qL * HL) * sq, av = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+  const double rc = sqrt(km1 * (rH - 0.5 * av));
+  if (which == 23 || which == 24) {  /* HLLE, HLLEM */
+    const double beta = sqrt(0.5 * km1 / kappa);
+    const double aL = sqrt(kappa * pL * sL), aR = sqrt(kappa * pR * sR);
+    double Ssl = rv[0] - rc, Ssr = rv[0] + rc;
+    if (vL[0] - beta * aL < Ssl) Ssl = vL[0] - beta * aL;
+    if (Ssl > 0.0) Ssl = 0.0;
+    if (vR[0] + beta * aR > Ssr) Ssr = vR[0] + beta * aR;
+    if (Ssr < 0.0) Ssr = 0.0;
+    if (Ssl >= 0.0 && Ssr > 0.0) { for (int v = 0; v < 5; v++) f[v] = FL[v]; return; }
+    if (Ssr <= 0.0 && Ssl < 0.0) { for (int v = 0; v < 5; v++) f[v] = FR[v]; return; }
+    double anti[5] = {0, 0, 0, 0, 0};
+    if (which == 24) {
+      const double delta = rc / (rc + fabs(0.5 * (Ssl + Ssr)));
+      const double rd = sqrt(uLL[0] * uRR[0]);
+      const double A2 = (uRR[0] - uLL[0]) - (pR - pL) / (rc * rc);
+      const double A3 = rd * (vR[1] - vL[1]), A4 = rd * (vR[2] - vL[2]);
+      const double r2[5] = {1.0, rv[0], rv[1], rv[2], 0.5 * av};
+      const double r3[5] = {0.0, 0.0, 1.0, 0.0, rv[1]}, r4[5] = {0.0, 0.0, 0.0, 1.0, rv[2]};
+      for (int v = 0; v < 5; v++) anti[v] = delta * (r2[v] * A2 + r3[v] * A3 + r4[v] * A4);
+    }
+    for (int v = 0; v < 5; v++)
+      f[v] = (Ssr * FL[v] - Ssl * FR[v] + Ssl * Ssr * (uRR[v] - uLL[v] - anti[v])) / (Ssr - Ssl);
+    return;
+  }
+  /* Roe */
+  double a1 = rv[0] - rc, a5 = rv[0] + rc;
+  const double a2 = rv[0];
+  const double r1[5] = {1.0, a1, rv[1], rv[2], rH - rv[0] * rc};
+  const double r2[5] = {1.0, rv[0], rv[1], rv[2], 0.5 * av};
+  const double r3[5] = {0.0, 0.0, 1.0, 0.0, rv[1]}, r4[5] = {0.0, 0.0, 0.0, 1.0, rv[2]};
+  const double r5[5] = {1.0, a5, rv[1], rv[2], rH + rv[0] * rc};
+  double dU[6];
+  for (int v = 0; v < 5; v++) dU[v] = uRR[v] - uLL[v];
+  dU[5] = dU[4] - (dU[2] - rv[1] * dU[0]) * rv[1] - (dU[3] - rv[2] * dU[0]) * rv[2];
+  const double A3 = dU[2] - rv[1] * dU[0], A4 = dU[3] - rv[2] * dU[0];
+  const double A2 = km1 / (rc * rc) * (dU[0] * (rH - rv[0] * rv[0]) - dU[5] + rv[0] * dU[1]);
+  const double A1 = 0.5 / rc * (dU[0] * (rv[0] + rc) - dU[1] - rc * A2);
+  const double A5 = dU[0] - A1 - A2;
+  double Fc[5];
+  if (volflux == 2 || volflux == 3 || volflux == 4) {
+    const double e1[3] = {1.0, 0.0, 0.0};
+    double xL[6], xR[6];
+    ora_eval_uaux(kappa, uLL, xL);
+    ora_eval_uaux(kappa, uRR, xR);
+    ora_volflux(volflux == 4 ? 8 : volflux, kappa, uLL, uRR, xL, xR, e1, e1, Fc);
+    if (volflux != 3) {
+      const double m = fabs(rv[0] - rc) > fabs(rv[0] + rc) ? fabs(rv[0] - rc) : fabs(rv[0] + rc);
+      a1 = m;
+      a5 = m;
+    }
+  } else {
+    for (int v = 0; v < 5; v++) Fc[v] = 0.5 * (FL[v] + FR[v]);
+  }
+  for (int v = 0; v < 5; v++)
+    f[v] = Fc[v] - 0.5 * (A1 * fabs(a1) * r1[v] + A2 * fabs(a2) * r2[v] + A3 * fabs(a2) * r3[v] + A4 * fabs(a2) * r4[v] +
+                          A5 * fabs(a5) * r5[v]);
+}
+
 /* the largest wave speed of one face point in the rotated frame (the LambdaMaxMax term) */
 double ora_face_lambda(double kappa, const double *UL, const double *UR, const double *nv)
 {
@@ -543,6 +647,13 @@ double ora_face_lambda(double kappa, const double *UL, const double *UR, const d
    (one face node). */
 void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double lam_face, double *F)
+{
+  ora_riemann_lam_vf(which, 0, kappa, UL, UR, nv, t1, t2, lam_face, F);
+}
+
+/* the same with the compile-time VolumeFlux, which selects the central part of Roe (3) */
+void ora_riemann_lam_vf(int which, int volflux, double kappa, const double *UL, const double *UR, const double *nv,
+                        const double *t1, const double *t2, double lam_face, double *F)
 {
   double uLL[5], uRR[5];
   uLL[0] = UL[0]; uRR[0] = UR[0];
@@ -574,6 +685,8 @@ void ora_riemann_lam(int which, double kappa, const double *UL, const double *UR
     riemann_eckep_llf(kappa, uLL, uRR, lam_face, f);
   } else if (which == 13) {
     riemann_ec_llf(kappa, uLL, uRR, lam_face, f);
+  } else if (which == 2 || which == 3 || (which >= 22 && which <= 24)) {
+    riemann_hll_roe(which, volflux, kappa, uLL, uRR, f);
   } else if (riemann_volflux_of(which) >= 0) {
     riemann_volflux_avg(which, kappa, uLL, uRR, lam_face, f);
   } else {
@@ -974,8 +1087,8 @@ static void surface_integral_faces(const ora_problem *P, const double *U, double
       for (int p = 0; p < n; p++) {
         size_t f = FACE(s, p, q);
         double *F = Fm + NV * f;
-        ora_riemann_lam(P->riemann, P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f,
-                        P->TangVec1 + 3 * f, P->TangVec2 + 3 * f, lam, F);
+        ora_riemann_lam_vf(P->riemann, P->volflux, P->kappa, Um + NV * f, Us + NV * f, P->NormVec + 3 * f,
+                           P->TangVec1 + 3 * f, P->TangVec2 + 3 * f, lam, F);
         if (P->parabolic) {
           double kL[5], gL[5], jL[5], kR[5], gR[5], jR[5];
           const double *nv = P->NormVec + 3 * f;

@@ -65,7 +65,12 @@ CASES = [
 ] + [(f"euler_n{N}_riemann{rs}_warped", 2, N, 3, True, (0, 0, 0), (2, 2, 2), vf, rs, False, 2, 0.05, 3)
      for rs, N, vf in ((5, 3, 10), (6, 2, 32), (7, 3, 32), (8, 4, 32), (10, 3, 32), (11, 2, 32), (12, 3, 32),
                        (13, 3, 10), (14, 4, 32), (15, 2, 32), (17, 3, 32), (18, 4, 32), (19, 3, 32), (20, 2, 32),
-                       (21, 3, 32))]
+                       (21, 3, 32))] + [
+    # HLLC (2), HLL (22), HLLE (23), HLLEM (24) and Roe (3), whose central part follows the
+    # volume flux (2: Kennedy-Gruber 1, 3: Ducros, 4: Kennedy-Gruber 2, else the average)
+    (f"euler_n{N}_riemann{rs}_vf{vf}_warped", 2, N, 3, True, (0, 0, 0), (2, 2, 2), vf, rs, False, 2, 0.05, 3)
+    for rs, N, vf in ((2, 3, 32), (22, 3, 32), (23, 4, 32), (24, 3, 32), (3, 3, 32), (3, 2, 2), (3, 4, 3),
+                      (3, 3, 4))]
 
 # NFVSE shock capturing (the reference built with SHOCK_NFVSE): a Gaussian pressure pulse
 # (fluxo_amd.workloads.sedov_blast with the width and background pressure given) on warped

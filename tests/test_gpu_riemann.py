@@ -2,7 +2,7 @@
 entropy-stable solvers (Riemann 4, 33, 44, in the Cartesian frame) and EC-KEP + face LLF (16,
 a face-wide maximum wave speed), on warped meshes with perturbed states, N = 2..5 (relative
 L2 <= 1e-12 per RHS), Navier-Stokes with 33, a 10-step trajectory (<= 1e-10), VolumeFlux
-10, and the other surface fluxes 5-8, 10-15, 17-21.  The reference's own outputs are checked in test_gpu_golden (euler_*_es*,
+10, and the other surface fluxes 2, 3, 5-8, 10-15, 17-24.  The reference's own outputs are checked in test_gpu_golden (euler_*_es*,
 *_eckep_llf16_*, ns_*_es33_*, *_ismailroe_*)."""
 import math
 
@@ -37,15 +37,30 @@ def test_surface_flux_rhs(rs, N):
     assert err <= 1e-12
 
 
-NEW_RS = [5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21]
+NEW_RS = [2, 3, 5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21, 22, 23, 24]
+
+
+@pytest.mark.parametrize("vf", [2, 3, 4])
+@pytest.mark.parametrize("N", [2, 5])
+def test_roe_with_split_form_central_part(vf, N):
+    """Roe (3) whose central part follows VolumeFlux 2 / 3 / 4 (riemann.f90:836-856)."""
+    from fluxo_amd.native import Engine
+    m = O.box_mesh(3, N, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True)
+    U = state(m)
+    e = Engine(m, volume_flux=vf, riemann=3)
+    e.upload(U)
+    err = rel_l2(e.rhs(), O.Problem(m, volflux=vf, riemann=3).rhs(U))
+    print(f"Roe VolumeFlux {vf} N={N}: RHS relL2 {err:.3e}")
+    assert err <= 1e-12
 
 
 @pytest.mark.parametrize("rs", NEW_RS)
 @pytest.mark.parametrize("N", [2, 5])
 def test_other_surface_fluxes_rhs(rs, N):
-    """Riemann 5-8, 10-15, 17-21 (volume-flux averages in the normal direction, their LLF
-    forms with the face's largest wave speed, Ismail-Roe + entropy-Jacobian LLF) against the
-    oracle; N = 3, 4 are in the golden vectors (euler_*_riemann*)."""
+    """Riemann 2-24 beyond 4, 9, 16 (HLLC, Roe, HLL, HLLE, HLLEM, volume-flux averages in
+    the normal direction, their LLF forms with the face's largest wave speed, Ismail-Roe +
+    entropy-Jacobian LLF) against the oracle; N = 3, 4 are in the golden vectors
+    (euler_*_riemann*)."""
     from fluxo_amd.native import Engine
     m = O.box_mesh(3, N, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True)
     U = state(m)

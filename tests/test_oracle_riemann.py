@@ -142,3 +142,30 @@ def test_volume_flux_average_solvers(rs):
         p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
         vn = v @ n
         assert np.allclose(Fe, [UL[0] * vn, *(UL[1:4] * vn + p * n), (UL[4] + p) * vn], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("rs", [2, 3, 22, 23, 24])
+def test_hll_and_roe_solvers(rs):
+    """HLLC (2), Roe (3), HLL (22), HLLE (23), HLLEM (24): the physical flux for equal states,
+    independence of the tangent choice, and for the upwind cases (supersonic flow through the
+    face) the upwind side's physical flux for the HLL family."""
+    rng = np.random.default_rng(300 + rs)
+    for _ in range(20):
+        UL, UR = random_state(rng), random_state(rng)
+        n, t1, t2 = frame(rng)
+        F = flux(rs, UL, UR, n, t1, t2)
+        _, s1, _ = frame(rng)
+        s1 = s1 - (s1 @ n) * n
+        s1 /= np.linalg.norm(s1)
+        assert np.allclose(F, flux(rs, UL, UR, n, s1, np.cross(n, s1)), rtol=1e-12, atol=1e-12)
+        Fe = flux(rs, UL, UL, n, t1, t2)
+        v = UL[1:4] / UL[0]
+        p = (KAPPA - 1) * (UL[4] - 0.5 * UL[0] * v @ v)
+        vn = v @ n
+        assert np.allclose(Fe, [UL[0] * vn, *(UL[1:4] * vn + p * n), (UL[4] + p) * vn], rtol=1e-12, atol=1e-12)
+    if rs != 3:  # supersonic along n: every wave moves right, the flux is the left state's
+        UL = np.array([1.0, 5.0 * n[0], 5.0 * n[1], 5.0 * n[2], 1.0 / (KAPPA - 1) + 12.5])
+        UR = np.array([1.1, 5.2 * n[0], 5.2 * n[1], 5.2 * n[2], 1.1 / (KAPPA - 1) + 0.55 * 27.04])
+        vn = 5.0
+        FL = [UL[0] * vn, *(UL[1:4] * vn + 1.0 * n), (UL[4] + 1.0) * vn]
+        assert np.allclose(flux(rs, UL, UR, n, t1, t2), FL, rtol=1e-12, atol=1e-12)
