@@ -638,6 +638,29 @@ void launch_nfvse_fv_n(const Batch &b, const double *alpha)
 {
   switch (b.cfg.riemann) {
   case 0: launch_nfvse_fv_t<N, 0>(b, alpha); break;
+  case 2: launch_nfvse_fv_t<N, 2>(b, alpha); break;
+  case 3:
+    switch (b.cfg.volumeFlux) {
+    case 2: launch_nfvse_fv_t<N, 302>(b, alpha); break;
+    case 3: launch_nfvse_fv_t<N, 303>(b, alpha); break;
+    case 4: launch_nfvse_fv_t<N, 308>(b, alpha); break;
+    default: launch_nfvse_fv_t<N, 3>(b, alpha); break;
+    }
+    break;
+  case 4: launch_nfvse_fv_t<N, 4>(b, alpha); break;
+  case 5: launch_nfvse_fv_t<N, 5>(b, alpha); break;
+  case 6: launch_nfvse_fv_t<N, 6>(b, alpha); break;
+  case 7: launch_nfvse_fv_t<N, 7>(b, alpha); break;
+  case 8: launch_nfvse_fv_t<N, 8>(b, alpha); break;
+  case 10: launch_nfvse_fv_t<N, 10>(b, alpha); break;
+  case 11: launch_nfvse_fv_t<N, 11>(b, alpha); break;
+  case 12: launch_nfvse_fv_t<N, 12>(b, alpha); break;
+  case 20: launch_nfvse_fv_t<N, 20>(b, alpha); break;
+  case 22: launch_nfvse_fv_t<N, 22>(b, alpha); break;
+  case 23: launch_nfvse_fv_t<N, 23>(b, alpha); break;
+  case 24: launch_nfvse_fv_t<N, 24>(b, alpha); break;
+  case 33: launch_nfvse_fv_t<N, 33>(b, alpha); break;
+  case 44: launch_nfvse_fv_t<N, 44>(b, alpha); break;
   case 9: launch_nfvse_fv_t<N, 9>(b, alpha); break;
   case 32: launch_nfvse_fv_t<N, 32>(b, alpha); break;
   default: launch_nfvse_fv_t<N, 1>(b, alpha); break;
@@ -731,8 +754,10 @@ inline int check_nfvse(const fx_config *c)
 {
   if (!c->nfvse) return FX_OK;
   if (c->nfvse != 1) return set_err(FX_EINVAL, "nfvse=%d (0 off, 1 first-order sub-cell FV)", c->nfvse);
-  if (c->riemann != 0 && c->riemann != 1 && c->riemann != 9 && c->riemann != 32)
-    return set_err(FX_EINVAL, "NFVSE sub-cell fluxes: Riemann 0, 1, 9, 32");
+  // the sub-cell interfaces take point-wise solvers; those with a face-wide wave speed
+  // (13-19, 21) would need it over each sub-cell interface plane
+  if (c->riemann >= 13 && c->riemann <= 21 && c->riemann != 20)
+    return set_err(FX_EINVAL, "NFVSE sub-cell fluxes: Riemann 0-12, 20, 22-24, 32, 33, 44 (not 13-19, 21)");
   if (c->computeAlpha != 1 && c->computeAlpha != 3)
     return set_err(FX_EINVAL, "ComputeAlpha=%d not supported (1 Persson-Peraire, 3 fixed)", c->computeAlpha);
   if (c->spacePropSweeps < 0 || c->spacePropSweeps > 16) return set_err(FX_EINVAL, "SpacePropSweeps out of 0..16");

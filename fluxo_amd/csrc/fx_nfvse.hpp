@@ -174,7 +174,8 @@ k_nfvse_fv(const double *__restrict__ U, const double *__restrict__ Mf, const do
       nv[0] = nv[0] / norm; nv[1] = nv[1] / norm; nv[2] = nv[2] / norm;
       const double *ur = U + ((size_t)e * n3 + nodes[f + 1]) * 5;
       double UR[5] = {ur[0], ur[1], ur[2], ur[3], ur[4]}, F[5];
-      riemann<RS>(UL, UR, nv, kappa, F);
+      if constexpr (RS == 4 || RS == 5 || RS == 33 || RS == 44) riemann_es<RS>(UL, UR, nv, kappa, 0.0, F);
+      else riemann<RS>(UL, UR, nv, kappa, F);
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         F[v] = F[v] * norm;

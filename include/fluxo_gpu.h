@@ -71,7 +71,9 @@ typedef struct {
      shockcapturing/nfvse.f90:138-146), Euler or Navier-Stokes: sub-cell first-order finite volumes
      (SubFVMethod=1) blended into the volume integral of each element with the coefficient
      of CalcBlendingCoefficient (nfvse.f90:1537-1615) and TimeRelFactor=0; with
-     parabolic = 1 the viscous volume term is added unblended (dg.f90:368-398). */
+     parabolic = 1 the viscous volume term is added unblended (dg.f90:368-398).  The sub-cell
+     interfaces take any point-wise Riemann solver (not 13-19, 21, whose face-wide wave speed
+     would span a sub-cell interface plane). */
   int nfvse;           /* 0 off, 1 on */
   int computeAlpha;    /* 'ComputeAlpha': 1 Persson-Peraire indicator on rho*p
                           (ModalThreshold=1), 3 fixed 'ShockBlendCoef' */

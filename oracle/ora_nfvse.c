@@ -156,8 +156,9 @@ void ora_volint_nfvse(const ora_problem *P, const double *U, const double *alpha
 #define NODE(m) (d == 0 ? ((m) + n * ((a) + n * (b))) : (d == 1 ? ((a) + n * ((m) + n * (b))) : ((a) + n * ((b) + n * (m)))))
             const double *g = scm + 10 * (a + n * (b + n * (f + N * d)));
             double F[5];
-            ora_riemann(P->riemann, P->kappa, U + NV * (eo + NODE(f)), U + NV * (eo + NODE(f + 1)), g, g + 3,
-                        g + 6, F);
+            /* point-wise solvers only (no face-wide wave speed on sub-cell interfaces) */
+            ora_riemann_lam_vf(P->riemann, P->volflux, P->kappa, U + NV * (eo + NODE(f)), U + NV * (eo + NODE(f + 1)),
+                               g, g + 3, g + 6, 0.0, F);
 #undef NODE
             double *o = fl + NV * (a + n * (b + n * ((f + 1) + (n + 1) * d)));
             for (int v = 0; v < NV; v++) o[v] = F[v] * g[9];

@@ -256,13 +256,23 @@ def make_hopr(case):
 
 
 # Navier-Stokes (BR1, mu0) with NFVSE: name, K, N, sigma, p0, alpha_max, fixed alpha, steps, mu0
+# NFVSE with other point-wise surface fluxes at the sub-cell interfaces (Riemann at run time)
+NFVSE_RIEMANN_CASES = [
+    # name, K, N, sigma, p0, alpha_max, fixed, nsteps, riemann
+    ("euler_nfvse_n3_hllc_warped", 4, 3, 0.2, 1e-2, 0.5, None, 2, 2),
+    ("euler_nfvse_n3_roe_warped", 4, 3, 0.2, 1e-2, 0.5, None, 2, 3),
+    ("euler_nfvse_n3_es33_warped", 4, 3, 0.3, 1e-1, 0.5, None, 2, 33),  # the 0.2 / 1e-2 pulse fails in the reference too
+    ("euler_nfvse_n4_hllem_warped", 3, 4, 0.2, 1e-2, 0.5, None, 2, 24),
+    ("euler_nfvse_n2_ismailroe5_warped", 5, 2, 0.2, 1e-2, 0.5, None, 2, 5),
+]
+
 NFVSE_NS_CASES = [
     ("ns_nfvse_n3_pulse_warped", 5, 3, 0.25, 1e-2, 0.5, None, 3, 1e-3),
     ("ns_nfvse_n4_fixed_alpha_warped", 4, 4, 0.25, 1e-2, 1.0, 0.4, 2, 1e-3),
 ]
 
 
-def make_nfvse(case, mu0=None):
+def make_nfvse(case, mu0=None, riemann=1):
     from fluxo_amd.workloads import sedov_blast
     name, K, N, sigma, p0, amax, fixed, nsteps = case[:8]
     par = mu0 is not None
@@ -271,9 +281,9 @@ def make_nfvse(case, mu0=None):
     U = sedov_blast(m["Elem_xGP"], sigma=sigma, p0=p0)
     par_nfvse = dict(ComputeAlpha=1 if fixed is None else 3, ShockBlendCoef=float(fixed or 0.0),
                      alpha_max=amax, alpha_min=0.01, SpacePropSweeps=1, SpacePropFactor=0.5)
-    r = R.run(m, U, volflux=32, riemann=1, cfl=0.5, dfl=0.5, nsteps=nsteps, nfvse=par_nfvse, parabolic=par,
+    r = R.run(m, U, volflux=32, riemann=riemann, cfl=0.5, dfl=0.5, nsteps=nsteps, nfvse=par_nfvse, parabolic=par,
               mu0=mu0 or 0.0, Pr=0.72)
-    out = dict(K=K, N=N, Ngeo=3, warp=1, x0=np.array(x0), L=np.array(L), volflux=32, riemann=1,
+    out = dict(K=K, N=N, Ngeo=3, warp=1, x0=np.array(x0), L=np.array(L), volflux=32, riemann=riemann,
                parabolic=int(par), mu0=mu0 or 0.0, Pr=0.72, kappa=1.4, cfl=0.5, dfl=0.5, nsteps=nsteps, U=U,
                Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"], errType0=r["errType0"], U_end=r["U_end"],
                t_end=r["t_end"], Entropy=r["Entropy"], dSdU_Ut=r["dSdU_Ut"], sJ=r["sJ"], Mf=r["Mf"],
@@ -289,15 +299,20 @@ def make_nfvse(case, mu0=None):
 
 if __name__ == "__main__":
     sys.path.insert(0, ROOT)
-    if len(sys.argv) > 1:  # only the named cases of CASES
+    if len(sys.argv) > 1:  # only the named cases of CASES and NFVSE_RIEMANN_CASES
         for c in CASES:
             if c[0] in sys.argv[1:]:
                 make(c)
+        for c in NFVSE_RIEMANN_CASES:
+            if c[0] in sys.argv[1:]:
+                make_nfvse(c[:8], riemann=c[8])
         sys.exit(0)
     for c in CASES:
         make(c)
     for c in NFVSE_CASES:
         make_nfvse(c)
+    for c in NFVSE_RIEMANN_CASES:
+        make_nfvse(c[:8], riemann=c[8])
     for c in NFVSE_NS_CASES:
         make_nfvse(c[:8], mu0=c[8])
     for c in MHD_CASES + MHD_RES_CASES:
