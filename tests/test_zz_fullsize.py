@@ -1,0 +1,94 @@
+"""Size-independent properties at the benchmark's full size, where the oracle would take
+minutes per RHS: the Navier-Stokes TGV workload mesh (32^3 warped periodic hexes of
+[-pi,pi]^3, Ngeo=3, N=3, 2.1 M nodes) and the Cartesian 32^3 box.
+
+  * free-stream preservation on the warped mesh (the reference's L_inf < 1e-11 check,
+    tests/job_def.py:1099-1103, on O(1) elements; here scaled by the largest 1/J), Euler and
+    Navier-Stokes;
+  * discrete entropy conservation with the EC volume and surface fluxes (Ranocha 32/32):
+    dS/dU.Ut is round-off, many orders below the entropy production of the same state with
+    the dissipative LLF surface flux (job_def.py:1159-1164 checks |dS/dU.Ut| < 1e-13 on small
+    meshes);
+  * shift equivariance on the Cartesian box: the RHS of a state moved by one element in x is
+    the RHS moved by one element, which exercises every side's connectivity and orientation
+    at full size."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+K, N = 32, 3
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def tgv_mesh(warp):
+    from fluxo_amd.mesh import box_mesh
+    return box_mesh(K, N, x0=(-math.pi,) * 3, length=(2 * math.pi,) * 3, Ngeo=3 if warp else 1, warp=warp)
+
+
+def noisy_tgv(m, seed=5):
+    from fluxo_amd.workloads import taylor_green
+    U = taylor_green(m["Elem_xGP"])
+    return U * (1.0 + 0.02 * np.random.default_rng(seed).uniform(-1.0, 1.0, U.shape))
+
+
+def test_free_stream_full_size():
+    from fluxo_amd.native import Engine
+    m = tgv_mesh(True)
+    U = np.empty(m["Elem_xGP"].shape[:-1] + (5,))
+    U[...] = [1.2, 0.3, -0.2, 0.1, 2.5]
+    print(f"full size free stream: {m['nElems']} elements, N={N}", flush=True)
+    for kw in (dict(), dict(parabolic=True, mu0=1.0 / 1600, Pr=0.72)):
+        e = Engine(m, volume_flux=32, riemann=1, **kw)
+        e.upload(U)
+        linf = float(np.abs(e.rhs()).max())
+        # Ut carries 1/J (about 1e3 on these small elements): the bound is round-off of the
+        # state times the largest sJ, as in the reference's check on O(1) elements
+        bound = 1e-12 * np.abs(U).max() * np.abs(m["sJ"]).max()
+        print(f"  parabolic={bool(kw)}: max|Ut| = {linf:.3e} (bound {bound:.3e})")
+        assert linf < bound
+
+
+def test_entropy_conservation_full_size():
+    from fluxo_amd.native import Engine
+    m = tgv_mesh(True)
+    U = noisy_tgv(m)
+    e = Engine(m, volume_flux=32, riemann=32)
+    e.upload(U)
+    e.rhs()
+    S, dS = e.entropy()
+    d = Engine(m, volume_flux=32, riemann=1)
+    d.upload(U)
+    d.rhs()
+    _, dS_llf = d.entropy()
+    print(f"full size entropy: S = {S:.12e}, dS/dU.Ut EC {dS:.3e}, LLF {dS_llf:.3e}")
+    assert dS_llf < 0.0
+    assert abs(dS) <= 1e-9 * abs(dS_llf)
+
+
+def test_shift_equivariance_full_size():
+    from fluxo_amd.native import Engine
+    m = tgv_mesh(False)
+    h = 2 * math.pi / K
+    c = m["Elem_xGP"].reshape(m["nElems"], -1, 3).mean(axis=1)
+    ijk = np.rint((c + math.pi) / h - 0.5).astype(np.int64)
+    assert np.abs((c + math.pi) / h - 0.5 - ijk).max() < 1e-9
+    cell = {tuple(t): e for e, t in enumerate(ijk)}
+    # nxt[e] is the element one step in +x of e (periodic)
+    nxt = np.array([cell[((t[0] + 1) % K, t[1], t[2])] for t in ijk])
+    U = noisy_tgv(m)
+    e = Engine(m, volume_flux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72)
+    e.upload(U)
+    R = e.rhs()
+    Us = np.empty_like(U)
+    Us[nxt] = U  # the state of e moves to the element in +x
+    e.upload(Us)
+    Rs = e.rhs()
+    err = rel_l2(Rs[nxt], R)
+    print(f"full size shift equivariance: relL2 {err:.3e}")
+    assert err <= 1e-12
