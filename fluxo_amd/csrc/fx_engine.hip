@@ -2,6 +2,7 @@
 // in include/fluxo_gpu.h.  This file is the library's one device-code unit: the launch
 // layer (fx_launch.hpp) and the batched server (fx_server.hpp) are compiled with it.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <vector>
 
 #include "fx_kernels.hpp"
@@ -652,6 +653,178 @@ int fx_synchronize(fx_ctx *c)
 {
   if (!c) return set_err(FX_EINVAL, "null context");
   HIPCHK(hipStreamSynchronize(c->b.stream));
+  return FX_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================================
+// fx_rhs_noalloc: the right-hand side of fx_rhs for callers that cannot allocate device
+// memory.  The face fluxes, packed normals, viscous terms and NFVSE buffers that a context
+// keeps live in `scratch`, one file-scope device array per GPU, held until the process
+// ends.  Host code never passes `scratch` itself: its name on the host is a stand-in, so
+// each device's address is looked up once (hipGetSymbolAddress) and kept in g_scratch.
+static __device__ __align__(256) unsigned char scratch[FX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAXDEV = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[NOALLOC_MAXDEV] = {};
+
+// The current device's address of `scratch` (looked up on its first call on that device).
+int scratch_ptr(unsigned char **out)
+{
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAXDEV) return set_err(FX_ENODEV, "device %d beyond %d", dev, NOALLOC_MAXDEV);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(FX_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = static_cast<unsigned char *>(g_scratch[dev]);
+  return FX_OK;
+}
+
+// Scratch layout of one call, each part 256-byte aligned; returns the bytes needed.
+struct NoallocLayout {
+  size_t nvse = 0, flux = 0, uv = 0, fvn = 0, alpha = 0, alpha2 = 0, ffv = 0, flag = 0, total = 0;
+};
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+NoallocLayout noalloc_layout(const fx_config &c)
+{
+  const size_t n = c.N + 1, n3E = n * n * n * c.nElems, nf = n * n * c.nSides;
+  NoallocLayout L;
+  size_t o = 0;
+  auto take = [&](size_t &at, size_t bytes) { at = o; o += align256(bytes); };
+  take(L.nvse, nf * sizeof(double4));
+  take(L.flux, 5 * nf * 8);
+  if (c.parabolic) {
+    take(L.uv, 4 * n3E * 8);
+    take(L.fvn, 8 * nf * 8);
+  }
+  if (c.nfvse) {
+    take(L.alpha, (size_t)c.nElems * 8);
+    take(L.alpha2, (size_t)c.nElems * 8);
+    take(L.ffv, 5 * n3E * 8);
+  }
+  take(L.flag, sizeof(int));
+  L.total = o;
+  return L;
+}
+
+__global__ void k_noalloc_flag_clear(int *flag) { flag[0] = 0; }
+
+// Device-side form of check_connectivity: every index the kernels derive from the caller's
+// ElemToSide / SideToElem must stay inside the mesh; any violation sets *flag.
+__global__ void k_noalloc_check(const int *__restrict__ e2s, const int *__restrict__ s2e, int E, int S,
+                                int *__restrict__ flag)
+{
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int bad = 0;
+  if (g < S) {
+    const int s = (int)g;
+    const int *t = s2e + 5 * (size_t)s;
+    if (t[0] < 1 || t[0] > E || t[1] < 1 || t[1] > E || t[2] < 1 || t[2] > 6 || t[3] < 1 || t[3] > 6 ||
+        t[4] < 1 || t[4] > 4) {
+      bad = 1;
+    } else {
+      const int *am = e2s + 2 * (6 * (size_t)(t[0] - 1) + t[2] - 1);
+      const int *as = e2s + 2 * (6 * (size_t)(t[1] - 1) + t[3] - 1);
+      if (am[0] != s + 1 || am[1] != 0 || as[0] != s + 1 || as[1] != t[4]) bad = 1;
+    }
+  } else if (g < (long long)S + 6LL * E) {
+    const int *a = e2s + 2 * (size_t)(g - S);
+    if (a[0] < 1 || a[0] > S || a[1] < 0 || a[1] > 4) bad = 1;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+// (nx, ny, nz, SurfElem) per face point, the layout the kernels read
+__global__ void k_noalloc_pack_nvse(const double *__restrict__ nv, const double *__restrict__ se,
+                                    double4 *__restrict__ out, long long nf)
+{
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nf) out[f] = make_double4(nv[3 * f], nv[3 * f + 1], nv[3 * f + 2], se[f]);
+}
+
+}  // namespace
+
+extern "C" {
+
+long long fx_rhs_noalloc_bytes(const fx_config *cfg)
+{
+  if (!cfg || cfg->N < 2 || cfg->N > 5 || cfg->nElems < 1 || cfg->nSides < 1) return -1;
+  return (long long)noalloc_layout(*cfg).total;
+}
+
+int fx_rhs_noalloc(const fx_config *cfg, const fx_mesh *mesh, const double *U, double *Ut, void *stream)
+{
+  if (!cfg || !mesh || !U || !Ut) return set_err(FX_EINVAL, "null argument");
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  if (cfg->nVar != 5)
+    return set_err(FX_EINVAL, "fx_rhs_noalloc: Euler / Navier-Stokes (nVar = 5) only; GLM-MHD keeps GLM_ch "
+                              "state between calls and needs a context");
+  if (cfg->precision != FX_PREC_FP64)
+    return set_err(FX_EINVAL, "fx_rhs_noalloc: FP64 only (FP32-split needs FP32 metric copies)");
+  if (!mesh->ElemToSide || !mesh->SideToElem || !mesh->Metrics_fTilde || !mesh->Metrics_gTilde ||
+      !mesh->Metrics_hTilde || !mesh->sJ || !mesh->NormVec || !mesh->SurfElem)
+    return set_err(FX_EINVAL, "missing mesh array");
+  const NoallocLayout L = noalloc_layout(*cfg);
+  if (L.total > (size_t)FX_NOALLOC_SCRATCH_BYTES)
+    return set_err(FX_EINVAL, "fx_rhs_noalloc: N=%d, %d elements, %d sides need %zu bytes of scratch, above "
+                              "the maximum %lld", cfg->N, cfg->nElems, cfg->nSides, L.total,
+                   (long long)FX_NOALLOC_SCRATCH_BYTES);
+  unsigned char *sp = nullptr;
+  if ((rc = scratch_ptr(&sp))) return rc;
+  if ((rc = ensure_ops(cfg->N))) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int *flag = reinterpret_cast<int *>(sp + L.flag);
+  // connectivity check on the device, read back before any kernel indexes through it
+  hipLaunchKernelGGL(k_noalloc_flag_clear, dim3(1), dim3(1), 0, st, flag);
+  const long long nchk = (long long)cfg->nSides + 6LL * cfg->nElems;
+  hipLaunchKernelGGL(k_noalloc_check, dim3((unsigned)((nchk + 255) / 256)), dim3(256), 0, st,
+                     (const int *)mesh->ElemToSide, (const int *)mesh->SideToElem, cfg->nElems, cfg->nSides, flag);
+  HIPCHK(hipGetLastError());
+  int hflag = 0;
+  HIPCHK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hflag) return set_err(FX_EINVAL, "fx_rhs_noalloc: ElemToSide / SideToElem invalid (only inner conforming "
+                                       "sides are supported)");
+  Batch b;
+  b.cfg = *cfg;
+  b.set_degree(cfg->N);
+  b.stream = st;
+  b.U = const_cast<double *>(U);  // read only in the right-hand-side form
+  b.Ut = Ut;
+  b.Mf = const_cast<double *>(mesh->Metrics_fTilde);
+  b.Mg = const_cast<double *>(mesh->Metrics_gTilde);
+  b.Mh = const_cast<double *>(mesh->Metrics_hTilde);
+  b.sJ = const_cast<double *>(mesh->sJ);
+  b.e2s = const_cast<int *>(mesh->ElemToSide);
+  b.s2e = const_cast<int *>(mesh->SideToElem);
+  b.nvse = reinterpret_cast<double4 *>(sp + L.nvse);
+  b.Flux = reinterpret_cast<double *>(sp + L.flux);
+  if (cfg->parabolic) {
+    b.Uv = reinterpret_cast<double *>(sp + L.uv);
+    b.FvN = reinterpret_cast<double *>(sp + L.fvn);
+  }
+  if (cfg->nfvse) {
+    b.alpha = reinterpret_cast<double *>(sp + L.alpha);
+    b.alpha2 = reinterpret_cast<double *>(sp + L.alpha2);
+    b.Ffv = reinterpret_cast<double *>(sp + L.ffv);
+  }
+  const long long nf = (long long)b.n2 * cfg->nSides;
+  hipLaunchKernelGGL(k_noalloc_pack_nvse, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, mesh->NormVec,
+                     mesh->SurfElem, b.nvse, nf);
+  RKArgs rk{};
+  q_stage(b, VMODE_RHS, rk);
+  HIPCHK(hipGetLastError());
   return FX_OK;
 }
 

@@ -95,7 +95,8 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_ctx_bytes", "fx_profile_kernel", "fx_server_create", "fx_server_submit",
            "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
            "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
-           "fx_use_volume_tile", "fx_blending", "fx_glm", "fx_elem_xgp_upload", "fx_analyze"]
+           "fx_use_volume_tile", "fx_blending", "fx_glm", "fx_elem_xgp_upload", "fx_analyze",
+           "fx_rhs_noalloc", "fx_rhs_noalloc_bytes"]
 
 (FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS,
  FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV) = range(7)
@@ -145,6 +146,9 @@ def lib():
     L.fx_analyze.argtypes = [vp, C.c_double, C.POINTER(fx_analyze_config), C.POINTER(fx_analysis)]
     L.fx_server_last_error.argtypes = [vp]
     L.fx_server_last_error.restype = C.c_char_p
+    L.fx_rhs_noalloc.argtypes = [C.POINTER(fx_config), C.POINTER(fx_mesh), vp, vp, vp]
+    L.fx_rhs_noalloc_bytes.argtypes = [C.POINTER(fx_config)]
+    L.fx_rhs_noalloc_bytes.restype = C.c_longlong
     _LIB = L
     return L
 

@@ -230,6 +230,25 @@ int fx_use_volume_tile(fx_ctx *ctx, int epb);
 /* Device bytes held by a context with this configuration (for HBM admission). */
 long long fx_ctx_bytes(const fx_config *cfg);
 
+/* ---- Right-hand side without allocation ---------------------------------------------
+   No counterpart in the reference.  For callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+   right-hand side of fx_rhs (DGTimeDerivative_weakForm, dg.f90:282-452) with every input and
+   output a DEVICE pointer owned by the caller -- the fx_mesh arrays (TangVec1/2 unused), U and
+   Ut, in the layouts above -- and no hipMalloc, hipFree or other allocation.  The face
+   fluxes, packed normals, viscous terms and NFVSE buffers live in one library-owned static
+   device array of FX_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; configurations
+   that need more (fx_rhs_noalloc_bytes) are rejected with FX_EINVAL.  Every call on one GPU
+   shares that array and overwrites it: calls that overlap on one GPU (other streams or host
+   threads) race.  Euler / Navier-Stokes (nVar 5), FP64, with or without NFVSE; GLM-MHD and
+   FP32-split are rejected.  Work is queued on `stream` (a hipStream_t; NULL the default
+   stream); the call waits once on that stream for the device-side connectivity check, then
+   returns with the right-hand side queued. */
+#define FX_NOALLOC_SCRATCH_BYTES (64LL * 1024 * 1024)
+long long fx_rhs_noalloc_bytes(const fx_config *cfg);
+int fx_rhs_noalloc(const fx_config *cfg, const fx_mesh *mesh_dev, const double *U_dev, double *Ut_dev,
+                   void *stream);
+
 /* ---- Batched serving ---------------------------------------------------------------
    No counterpart in the reference, which runs one solve per process (fluxo.f90:180 ->
    TimeDisc, timedisc.f90:144-415); a request here is that whole run: nSteps steps of

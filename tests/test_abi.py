@@ -56,3 +56,25 @@ def test_libraries_load_alone_with_all_symbols_resolved():
         code = f"import ctypes, os; ctypes.CDLL({path!r}, mode=os.RTLD_NOW)"
         p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
         assert p.returncode == 0, (so, p.stderr[-500:])
+
+
+def test_noalloc_rejects_sizes_beyond_its_scratch_on_the_host():
+    """fx_rhs_noalloc checks the configuration and the scratch it needs before any HIP call:
+    a mesh too large for the static scratch array is refused with FX_EINVAL."""
+    L = native.lib()
+    small = native.make_config(3, 8, 24, parabolic=True)
+    big = native.make_config(3, 40 ** 3, 3 * 40 ** 3, parabolic=True)
+    limit = 64 * 1024 * 1024
+    assert 0 < L.fx_rhs_noalloc_bytes(ctypes.byref(small)) < limit < L.fx_rhs_noalloc_bytes(ctypes.byref(big))
+    dummy = (ctypes.c_double * 8)()
+    mesh = native.fx_mesh()
+    for field, _ in native._MESH_KEYS:
+        setattr(mesh, field, ctypes.addressof(dummy))
+    rc = L.fx_rhs_noalloc(ctypes.byref(big), ctypes.byref(mesh), ctypes.addressof(dummy), ctypes.addressof(dummy),
+                          None)
+    assert rc == -1 and b"maximum" in L.fx_last_error()
+    mhd = native.make_config(3, 8, 24)
+    mhd.nVar, mhd.volumeFlux = 9, 10
+    rc = L.fx_rhs_noalloc(ctypes.byref(mhd), ctypes.byref(mesh), ctypes.addressof(dummy), ctypes.addressof(dummy),
+                          None)
+    assert rc == -1 and b"nVar = 5" in L.fx_last_error()
