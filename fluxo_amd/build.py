@@ -15,10 +15,37 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build_engine(verbose=False):
+# per-degree kernel units (fx_tu.hip): degrees 2..5 x kernel groups 0..3
+TU_DEGREES = (2, 3, 4, 5)
+TU_GROUPS = (0, 1, 2, 3)
+
+
+def build_engine(verbose=False, jobs=None):
+    """libfluxo_gpu.so from fx_engine.hip (context, C ABI, server, non-template kernels) and
+    one object per (degree, kernel group) of fx_tu.hip, compiled in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+    csrc = os.path.join(HERE, "csrc")
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    units = [(os.path.join(csrc, "fx_engine.hip"), os.path.join(bdir, "fx_engine.o"), [])]
+    for N in TU_DEGREES:
+        for g in TU_GROUPS:
+            units.append((os.path.join(csrc, "fx_tu.hip"), os.path.join(bdir, f"fx_tu_n{N}_g{g}.o"),
+                          [f"-DFX_TU_N={N}", f"-DFX_TU_GROUP={g}"]))
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(u):
+        src, obj, defs = u
+        _run([HIPCC, *flags, *defs, "-c", src, "-o", obj + ".tmp"], verbose)
+        os.replace(obj + ".tmp", obj)
+        return obj
+
+    jobs = jobs or max(1, min(len(units), os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, units))
     out = os.path.join(HERE, "libfluxo_gpu.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "fx_engine.hip")], verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, out)
     return out
 

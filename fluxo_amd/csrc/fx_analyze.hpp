@@ -250,7 +250,9 @@ k_analyze_points(const double *__restrict__ U, const double *__restrict__ sJ, co
   }
 }
 
-template <int NV>
+// (these two are also templated on the caller's degree N, so that each compilation unit
+// instantiates its own)
+template <int N, int NV>
 __global__ void k_analyze_zero_norms(double *__restrict__ part, int nElems)
 {
   using Rc = AnaRec<NV>;
@@ -260,7 +262,7 @@ __global__ void k_analyze_zero_norms(double *__restrict__ part, int nElems)
 }
 
 // column c of the element records: sum or max in a fixed order (one workgroup per column)
-template <int NV>
+template <int N, int NV>
 __global__ void __launch_bounds__(256) k_analyze_reduce(const double *__restrict__ part, int nElems, double *__restrict__ out)
 {
   using Rc = AnaRec<NV>;
@@ -307,5 +309,11 @@ inline void analyze_basis(int N, int NA, double *vdm, double *wA)
     for (int i = 0; i <= N; i++) vdm[q * (N + 1) + i] = wb[i] / (xa[q] - x[i]) / den;
   }
 }
+
+// Analysis kernels of degree N (fx_launch_n.hpp): per-element records into ana_part, their
+// reduction into ana_out.
+template <int N>
+void launch_analyze_n(const Batch &b, const AnaArgs &a, const double *xgp, const double *ana_ops, double *ana_part,
+                      double *ana_out);
 
 }  // namespace fx

@@ -2,17 +2,42 @@
 // in include/fluxo_gpu.h.  This file is the library's one device-code unit: the launch
 // layer (fx_launch.hpp) and the batched server (fx_server.hpp) are compiled with it.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
+// The non-template kernels (CFL reductions, NFVSE propagation, the no-allocation helpers) and
+// their launches are compiled here; the per-degree kernels are in fx_tu.hip units.
+#define FX_KERNEL_HOME 1
 #include "fx_kernels.hpp"
-
-namespace fx {
-__constant__ Ops c_ops[MAXN + 1];
-}
-
 #include "fx_launch.hpp"
 #include "fx_analyze.hpp"
+
+namespace fx {
+
+// Load-time registry of the units' constant-memory uploaders (see ensure_ops).
+static std::vector<OpsUploader> &ops_uploaders()
+{
+  static std::vector<OpsUploader> v;
+  return v;
+}
+int register_ops_uploader(OpsUploader f)
+{
+  ops_uploaders().push_back(f);
+  return 1;
+}
+int upload_ops_everywhere(const Ops &o, int N)
+{
+  if (ops_uploaders().empty()) return set_err(FX_EHIP, "no kernel units registered");
+  for (OpsUploader f : ops_uploaders()) {
+    const int rc = f(o, N);
+    if (rc) return rc;
+  }
+  return FX_OK;
+}
+
+}  // namespace fx
 
 using namespace fx;
 
@@ -21,6 +46,7 @@ struct fx_ctx {
   double *part = nullptr, *red = nullptr;
   double *xgp = nullptr;                                   // Elem_xGP (fx_elem_xgp_upload)
   double *ana_part = nullptr, *ana_ops = nullptr, *ana_out = nullptr;  // fx_analyze, on first use
+  double *api_fs = nullptr;                                // Flux_slave of fx_surfint (nVar 5), on first use
   int ana_na = -1;
   bool mesh_ok = false, state_ok = false;
   hipGraphExec_t graph = nullptr;
@@ -31,24 +57,18 @@ namespace {
 
 size_t vol_doubles(const fx_config &c) { size_t n = c.N + 1; return (size_t)c.nVar * n * n * n * c.nElems; }
 
-template <int N>
-void launch_entropy_n(fx_ctx *c)
-{
-  const Batch &b = c->b;
-  const long long nn = (long long)b.n3 * b.cfg.nElems;
-  const int nb = (int)((nn + 255) / 256);
-  hipLaunchKernelGGL((k_entropy<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Ut, b.sJ, c->part, nn,
-                     b.cfg.kappa);
-  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, b.stream, c->part, nb, c->red);
-}
-void q_entropy(fx_ctx *c) { FX_DISPATCH_N(c->b.cfg.N, launch_entropy_n, c); }
+void q_entropy(fx_ctx *c) { FX_DISPATCH_N(c->b.cfg.N, launch_entropy_n, c->b, c->part, c->red); }
 
-__global__ void k_set_dt(DtState *st, double dt, double t, int resetErr)
+__global__ void k_set_dt(DtState *st, double dt, double t, int resetErr, double tEnd)
 {
   st->dt = dt;
   st->t = t;
+  st->tEnd = tEnd;
+  st->done = 0;
   if (resetErr) { st->errType = 0; st->errStep = -1; st->step = 0; }
 }
+
+constexpr double NO_END_TIME = 1.0e300;
 
 int check_config(const fx_config *c)
 {
@@ -80,33 +100,6 @@ int check_config(const fx_config *c)
 }
 
 }  // namespace
-
-// ======================================================================================
-template <int N, int NV>
-static void launch_analyze_nv(fx_ctx *c, const AnaArgs &a)
-{
-  const Batch &b = c->b;
-  constexpr int n = N + 1;
-  const size_t lds = (size_t)(NV + 4) * a.na * n * n * 8;
-  const double *g = b.cfg.parabolic ? b.grad : nullptr;
-  const size_t nv = (size_t)NV * b.n3 * b.cfg.nElems;
-  hipLaunchKernelGGL((k_analyze_nodes<N, NV>), dim3(b.cfg.nElems), dim3(256), 0, b.stream, b.U, b.Ut, b.sJ,
-                     c->xgp, g, g ? g + nv : nullptr, g ? g + 2 * nv : nullptr, a, c->ana_part);
-  if (a.exactFunc > 0)
-    hipLaunchKernelGGL((k_analyze_points<N, NV>), dim3(b.cfg.nElems), dim3(256), lds, b.stream, b.U, b.sJ, c->xgp,
-                       c->ana_ops, c->ana_ops + ANA_MAXNA * (MAXN + 1), a, c->ana_part);
-  else  // no error norms: zero the L2 and LINF columns
-    hipLaunchKernelGGL((k_analyze_zero_norms<NV>), dim3((b.cfg.nElems + 255) / 256), dim3(256), 0, b.stream,
-                       c->ana_part, b.cfg.nElems);
-  hipLaunchKernelGGL((k_analyze_reduce<NV>), dim3(AnaRec<NV>::COUNT), dim3(256), 0, b.stream, c->ana_part,
-                     b.cfg.nElems, c->ana_out);
-}
-template <int N>
-static void launch_analyze_n(fx_ctx *c, const AnaArgs &a)
-{
-  if (c->b.mhd()) launch_analyze_nv<N, 9>(c, a);
-  else launch_analyze_nv<N, 5>(c, a);
-}
 
 extern "C" {
 
@@ -200,6 +193,7 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   DtState z{};
   z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
   z.glm_init = 0; z.glm_ch = 0.5; z.glm_dtch1 = 0.0;
+  z.tEnd = 1.0e300; z.done = 0; z.pad = 0;
   if ((e = hipMemcpy(b.st, &z, sizeof(z), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "init");
   *out = c;
   return FX_OK;
@@ -213,7 +207,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
-                  b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out};
+                  b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -327,6 +321,59 @@ int fx_lifting(fx_ctx *c, double *gx, double *gy, double *gz)
   return FX_OK;
 }
 
+int fx_fillflux(fx_ctx *c, double *Flux_master, double *Flux_slave)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  Batch &b = c->b;
+  if (b.cfg.parabolic) q_lifting(b, 1);  // the sides' face-normal viscous fluxes
+  q_surface(b);
+  HIPCHK(hipGetLastError());
+  const size_t nf = (size_t)b.cfg.nVar * b.n2 * b.cfg.nSides;
+  if (Flux_master) HIPCHK(hipMemcpyAsync(Flux_master, b.Flux, nf * 8, hipMemcpyDeviceToHost, b.stream));
+  if (Flux_slave)  // conforming sides: Flux_slave = Flux_master, plus the slave's Powell term for MHD
+    HIPCHK(hipMemcpyAsync(Flux_slave, b.mhd() ? b.FluxS : b.Flux, nf * 8, hipMemcpyDeviceToHost, b.stream));
+  HIPCHK(hipStreamSynchronize(b.stream));
+  return FX_OK;
+}
+
+int fx_surfint(fx_ctx *c, const double *Flux_master, const double *Flux_slave, double *Ut)
+{
+  if (!c || !Flux_master || !Flux_slave || !Ut) return set_err(FX_EINVAL, "null argument");
+  if (!c->mesh_ok) return set_err(FX_ESTATE, "mesh not uploaded");
+  Batch &b = c->b;
+  const size_t nf = (size_t)b.cfg.nVar * b.n2 * b.cfg.nSides;
+  double *fs = b.FluxS;
+  if (!fs) {
+    if (!c->api_fs) HIPCHK(hipMalloc((void **)&c->api_fs, nf * 8));
+    fs = c->api_fs;
+  }
+  // the device face-flux buffers are scratch of the right-hand side: they take the caller's fluxes
+  HIPCHK(hipMemcpyAsync(b.Flux, Flux_master, nf * 8, hipMemcpyHostToDevice, b.stream));
+  HIPCHK(hipMemcpyAsync(fs, Flux_slave, nf * 8, hipMemcpyHostToDevice, b.stream));
+  HIPCHK(hipMemcpyAsync(b.Ut, Ut, vol_doubles(b.cfg) * 8, hipMemcpyHostToDevice, b.stream));
+  FX_DISPATCH_N(b.cfg.N, launch_surfint_n, b, b.Flux, fs);
+  HIPCHK(hipGetLastError());
+  return fx_ut_download(c, Ut);
+}
+
+int fx_volint_nfvse(fx_ctx *c, double *Ut_host)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  Batch &b = c->b;
+  if (!b.cfg.nfvse) return set_err(FX_ESTATE, "VolInt_NFVSE needs nfvse=1");
+  const double *a = q_nfvse_alpha(b);
+  FX_DISPATCH_N(b.cfg.N, launch_nfvse_fv_n, b, a);
+  RKArgs rk{};
+  rk.alpha = a;
+  rk.Ffv = b.Ffv;
+  q_volume(b, VMODE_VOLINT, rk);
+  HIPCHK(hipGetLastError());
+  if (Ut_host) return fx_ut_download(c, Ut_host);
+  return FX_OK;
+}
+
 int fx_blending(fx_ctx *c, double *alpha)
 {
   int rc = ready(c);
@@ -346,7 +393,7 @@ int fx_calc_dt(fx_ctx *c, double CFLscale, double DFLscale, double *dt, int *err
   if (rc) return rc;
   const int N = c->b.cfg.N;
   const double cfl = cfl_number(N, CFLscale), dfl = dfl_number(N, DFLscale);
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, 0.0, 0.0, 1);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, 0.0, 0.0, 1, NO_END_TIME);
   q_dt(c->b, cfl, dfl, 0);
   HIPCHK(hipGetLastError());
   DtState h;
@@ -361,21 +408,15 @@ int fx_rk_step(fx_ctx *c, double t, double dt)
 {
   int rc = ready(c);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, dt, t, 0);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, dt, t, 0, NO_END_TIME);
   q_rk_step(c->b);
   HIPCHK(hipGetLastError());
   return FX_OK;
 }
 
-int fx_run(fx_ctx *c, int nSteps, double CFLscale, double DFLscale, double *t, double *dt, int *errType)
+// The step graph of the context (CALCTIMESTEP + the five stages), captured for cfl/dfl.
+static int step_graph(fx_ctx *c, double cfl, double dfl)
 {
-  int rc = ready(c);
-  if (rc) return rc;
-  if (nSteps < 0) return set_err(FX_EINVAL, "nSteps < 0");
-  const int N = c->b.cfg.N;
-  const double cfl = cfl_number(N, CFLscale), dfl = dfl_number(N, DFLscale);
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, 0.0, t ? *t : 0.0, 1);
-  HIPCHK(hipGetLastError());
   if (!c->graph || c->graph_cfl != cfl || c->graph_dfl != dfl) {
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     hipGraph_t g;
@@ -388,12 +429,60 @@ int fx_run(fx_ctx *c, int nSteps, double CFLscale, double DFLscale, double *t, d
     if (e != hipSuccess) { c->graph = nullptr; return set_err(FX_EHIP, "graph instantiate: %s", hipGetErrorString(e)); }
     c->graph_cfl = cfl; c->graph_dfl = dfl;
   }
+  return FX_OK;
+}
+
+int fx_run(fx_ctx *c, int nSteps, double CFLscale, double DFLscale, double *t, double *dt, int *errType)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (nSteps < 0) return set_err(FX_EINVAL, "nSteps < 0");
+  const int N = c->b.cfg.N;
+  const double cfl = cfl_number(N, CFLscale), dfl = dfl_number(N, DFLscale);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, 0.0, t ? *t : 0.0, 1, NO_END_TIME);
+  HIPCHK(hipGetLastError());
+  if ((rc = step_graph(c, cfl, dfl))) return rc;
   for (int s = 0; s < nSteps; s++) HIPCHK(hipGraphLaunch(c->graph, c->b.stream));
   DtState h;
   HIPCHK(hipMemcpyAsync(&h, c->b.st, sizeof(h), hipMemcpyDeviceToHost, c->b.stream));
   HIPCHK(hipStreamSynchronize(c->b.stream));
   if (t) *t = h.t;
   if (dt) *dt = h.dt;
+  if (errType) *errType = h.errType;
+  return FX_OK;
+}
+
+int fx_run_to(fx_ctx *c, int maxSteps, double CFLscale, double DFLscale, double tEnd, double *t, double *dt,
+              int *steps, int *errType)
+{
+  int rc = ready(c);
+  if (rc) return rc;
+  if (!t) return set_err(FX_EINVAL, "null t");
+  if (!(tEnd == tEnd)) return set_err(FX_EINVAL, "tEnd is NaN");
+  const int N = c->b.cfg.N;
+  const double cfl = cfl_number(N, CFLscale), dfl = dfl_number(N, DFLscale);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c->b.stream, c->b.st, 0.0, *t, 1, tEnd);
+  HIPCHK(hipGetLastError());
+  if ((rc = step_graph(c, cfl, dfl))) return rc;
+  DtState h{};
+  h.t = *t;
+  int done = 0;
+  if (!(*t < tEnd) || maxSteps == 0) done = 1;  // nothing to do (timedisc.f90:236)
+  // launch in chunks sized from the last dt, so that few steps run after tEnd (those have
+  // dt = 0 and change nothing); stop on the end time, a CALCTIMESTEP error or maxSteps
+  int chunk = 1;
+  while (!done) {
+    if (maxSteps > 0) chunk = std::min(chunk, maxSteps - h.step);
+    for (int s = 0; s < chunk; s++) HIPCHK(hipGraphLaunch(c->graph, c->b.stream));
+    HIPCHK(hipMemcpyAsync(&h, c->b.st, sizeof(h), hipMemcpyDeviceToHost, c->b.stream));
+    HIPCHK(hipStreamSynchronize(c->b.stream));
+    if (h.done || h.errType != 0 || (maxSteps > 0 && h.step >= maxSteps)) break;
+    const double left = (tEnd - h.t) / (h.dt > 0.0 ? h.dt : 1.0);
+    chunk = (int)std::max(1.0, std::min(100.0, std::floor(left)));
+  }
+  *t = h.t;
+  if (dt) *dt = h.dt;
+  if (steps) *steps = h.step;
   if (errType) *errType = h.errType;
   return FX_OK;
 }
@@ -613,7 +702,7 @@ int fx_analyze(fx_ctx *c, double t, const fx_analyze_config *cfg, fx_analysis *o
   RKArgs rk{};
   q_stage(b, VMODE_RHS, rk);
   if (b.cfg.parabolic) q_lifting(b, 0);
-  FX_DISPATCH_N(N, launch_analyze_n, c, a);
+  FX_DISPATCH_N(N, launch_analyze_n, b, a, c->xgp, c->ana_ops, c->ana_part, c->ana_out);
   HIPCHK(hipGetLastError());
   double h[AnaRec<9>::COUNT];
   HIPCHK(hipMemcpyAsync(h, c->ana_out, (size_t)L * 8, hipMemcpyDeviceToHost, b.stream));

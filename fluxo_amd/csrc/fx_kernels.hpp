@@ -47,7 +47,9 @@ template <> struct OpsOf<float> {
   __device__ static __forceinline__ float ds(const Ops &o, int i, int l) { return o.Ds32[i][l]; }
   __device__ static __forceinline__ float dhat(const Ops &o, int i, int l) { return o.Dhat32[i][l]; }
 };
-extern __constant__ Ops c_ops[MAXN + 1];
+// One copy per compilation unit (internal linkage): the library is built from several units
+// that each hold kernels, and every unit's copy is filled by ensure_ops (fx_launch.hpp).
+static __constant__ Ops c_ops[MAXN + 1];
 
 // local side ids (defines.h:68-73)
 enum { ZETA_MINUS = 1, ETA_MINUS = 2, XI_PLUS = 3, ETA_PLUS = 4, XI_MINUS = 5, ZETA_PLUS = 6 };
@@ -184,7 +186,24 @@ struct DtState {
   int glm_init;                // GLM-MHD: dtch1 computed (InitTimeStep_GLM done)
   double glm_ch;               // GLM-MHD cleaning speed (0.5 before the first CALCTIMESTEP)
   double glm_dtch1;            // GLM-MHD time step of a unit wave speed
+  double tEnd;                 // end time (huge: none); the step reaching it is shortened
+  int done;                    // tEnd reached: later steps have dt = 0 and do not advance
+  int pad;
 };
+
+// The reference's end-of-run time step (timedisc.f90:313-317): when the CFL time step dt
+// reaches tEnd within 1e-4 of dt, the step is shortened to end exactly at tEnd and it is the
+// last one.  Returns the step's dt and sets S.done; once done, dt = 0 (the stage updates
+// leave U unchanged) and time and step count stay where they are.
+__device__ __forceinline__ double end_time_step(DtState &S, double dt, int advance)
+{
+  if (S.done) return 0.0;
+  if (S.tEnd - S.t <= dt * (1.0 + 1.0e-4)) {
+    dt = S.tEnd - S.t;
+    if (advance) S.done = 1;
+  }
+  return dt;
+}
 
 // Element -> problem map of a batch of independent meshes sharing one launch: eprob[e] is
 // the problem (index into the per-problem time-step states) of element e, or -1 for an
@@ -391,7 +410,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     double tot[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) tot[v] = double(SA(el2, v, l));
-    if (MODE != VMODE_VOLINT && rk.alpha) {
+    if (rk.alpha) {
       // VolInt_NFVSE (nfvse.f90:550): Ut = (1 - alpha) Ut + alpha F_FV on the volume part
       const double a = rk.alpha[e2];
       if (a != 0.0) {
@@ -437,6 +456,61 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 
 #undef SL
 #undef SA
+
+// SurfInt alone (surfint.f90:48-264, Gauss-Lobatto branch :177-184, 253-259): Ut +=
+// Flux_master L_HatMinus0 at the master element's face nodes and Ut -= Flux_slave L_HatMinus0
+// at the slave element's, one thread per volume node.  A node on several faces (one xi, one
+// eta, one zeta face at most) adds them in increasing side id, the order of the reference's
+// loop over sides.
+template <int N, int NV>
+__global__ void __launch_bounds__(256)
+k_surfint(const double *__restrict__ Fm, const double *__restrict__ Fs, const int *__restrict__ e2s,
+          double *__restrict__ Ut, int nElems)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int NONE = 0x7fffffff;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)nElems * n3) return;
+  const int e = (int)(g / n3), node = (int)(g - (long long)e * n3);
+  const int i = node % n, j = (node / n) % n, k = node / n2;
+  // slot d: the node's face normal to direction d (side id, flux offset, master?)
+  int sid[3] = {NONE, NONE, NONE}, off[3] = {0, 0, 0}, mas[3] = {0, 0, 0};
+  const int locs[3] = {i == 0 ? XI_MINUS : (i == N ? XI_PLUS : 0), j == 0 ? ETA_MINUS : (j == N ? ETA_PLUS : 0),
+                       k == 0 ? ZETA_MINUS : (k == N ? ZETA_PLUS : 0)};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int loc = locs[d];
+    if (loc == 0) continue;
+    const int sd = e2s[(e * 6 + loc - 1) * 2] - 1, fl = e2s[(e * 6 + loc - 1) * 2 + 1];
+    int p, q;
+    vol_to_side<N>(loc, fl, i, j, k, p, q);
+    sid[d] = sd;
+    off[d] = ((sd * n + q) * n + p) * NV;
+    mas[d] = (fl == 0);
+  }
+  // sort the three slots by side id (compare-exchange network)
+  auto cx = [&](int a, int b) {
+    if (sid[b] < sid[a]) {
+      int t = sid[a]; sid[a] = sid[b]; sid[b] = t;
+      t = off[a]; off[a] = off[b]; off[b] = t;
+      t = mas[a]; mas[a] = mas[b]; mas[b] = t;
+    }
+  };
+  cx(0, 1); cx(1, 2); cx(0, 1);
+  const double L0 = c_ops[N].L0;
+  double *u = Ut + (size_t)g * NV;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    double r = u[v];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      if (sid[t] == NONE) continue;
+      const double f = (mas[t] ? Fm[off[t] + v] : Fs[off[t] + v]) * L0;
+      r = mas[t] ? r + f : r - f;
+    }
+    u[v] = r;
+  }
+}
 
 // Side-centric numerical flux: Flux(:,p,q,s) = F*(U_master, U_slave; n) * SurfElem, with
 // the BR1 viscous average for Navier-Stokes (FillFlux fillflux.f90:41-110, Riemann
@@ -843,6 +917,7 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   }
 }
 
+#ifdef FX_KERNEL_HOME  // non-template kernels: compiled in one unit only
 // Minimum of dtElem (NaN entries skipped) and maximum error code, one block, fixed order.
 __global__ void __launch_bounds__(1024)
 k_dt_reduce(const double *__restrict__ dtElem, const int *__restrict__ errElem, int nElems, DtState *st,
@@ -869,10 +944,11 @@ k_dt_reduce(const double *__restrict__ dtElem, const int *__restrict__ errElem, 
     __syncthreads();
   }
   if (tid == 0) {
-    st->dt = s_dt[0];
+    const bool was_done = st->done != 0;
+    st->dt = end_time_step(*st, s_dt[0], advance);
     st->errType = max(st->errType, s_er[0]);
     if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
-    if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+    if (advance && !was_done) { st->t = st->t + st->dt; st->step = st->step + 1; }
   }
 }
 
@@ -914,6 +990,7 @@ k_dt_reduce_seg(const double *__restrict__ dtElem, const int *__restrict__ errEl
     if (advance) { S.t = S.t + S.dt; S.step = S.step + 1; }
   }
 }
+#endif  // FX_KERNEL_HOME
 
 // CalcEntropy (analyze_equation.f90:549-599): per-block partial sums (fixed order),
 // reduced by a second pass in block order.
@@ -955,6 +1032,8 @@ k_entropy(const double *__restrict__ U, const double *__restrict__ Ut, const dou
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
 }
 
+// (templated on the degree of the caller so that each unit instantiates its own)
+template <int TAG>
 __global__ void k_sum_pairs(const double *__restrict__ part, int nPart, double *__restrict__ out)
 {
   __shared__ double s0[256], s1[256];

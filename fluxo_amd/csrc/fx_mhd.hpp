@@ -577,6 +577,7 @@ k_mhd_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const
   }
 }
 
+#ifdef FX_KERNEL_HOME  // non-template kernel: compiled in one unit only
 // minimum dt (and, on the first call, the unit-speed dt), GLM_ch = scale * dtch1 / dt
 __global__ void __launch_bounds__(1024)
 k_mhd_dt_reduce(const double *__restrict__ dtElem, const double *__restrict__ dtch1Elem,
@@ -604,13 +605,19 @@ k_mhd_dt_reduce(const double *__restrict__ dtElem, const double *__restrict__ dt
     __syncthreads();
   }
   if (tid == 0) {
-    st->dt = s_dt[0];
-    if (!st->glm_init) { st->glm_dtch1 = s_c[0]; st->glm_init = 1; }
-    st->glm_ch = glm_scale * (st->glm_dtch1 / st->dt);
+    const bool was_done = st->done != 0;
+    if (!was_done) {
+      // GLM_ch from the CFL time step itself (calctimestep.f90:175-177), before the end-time cut
+      if (!st->glm_init) { st->glm_dtch1 = s_c[0]; st->glm_init = 1; }
+      st->glm_ch = glm_scale * (st->glm_dtch1 / s_dt[0]);
+    }
+    st->dt = end_time_step(*st, s_dt[0], advance);
     st->errType = max(st->errType, s_er[0]);
     if (st->errType != 0 && st->errStep < 0) st->errStep = st->step;
-    if (advance) { st->t = st->t + st->dt; st->step = st->step + 1; }
+    if (advance && !was_done) { st->t = st->t + st->dt; st->step = st->step + 1; }
   }
 }
+
+#endif  // FX_KERNEL_HOME
 
 }  // namespace fx

@@ -97,6 +97,7 @@ k_nfvse_alpha(const double *__restrict__ U, const int *__restrict__ eprob, doubl
   }
 }
 
+#ifdef FX_KERNEL_HOME  // non-template kernel: compiled in one unit only
 // One sweep of PropagateBlendingCoeff (nfvse_mpi.f90:78-153) on conforming sides: every
 // side raises both its elements to factor * max(alpha_master, alpha_slave), from the values
 // before the sweep, so the order of the sides does not matter.
@@ -116,6 +117,7 @@ k_nfvse_prop(const double *__restrict__ ain, double *__restrict__ aout, const in
   }
   aout[e] = r;
 }
+#endif  // FX_KERNEL_HOME
 
 // Compute_FVFluxes_1st_Order + Compute_FVFluxes_1D + the F_FV sum of VolInt_NFVSE
 // (nfvse.f90:536-548, 566-670, 1389-1443) for elements with alpha > 0.

@@ -276,6 +276,7 @@ inline int upload_request(Pool &p, Request &q)
   DtState z{};
   z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
   z.glm_init = 0; z.glm_ch = 0.5; z.glm_dtch1 = 0.0;
+  z.tEnd = 1.0e300; z.done = 0; z.pad = 0;
   HIPCHK(hipMemcpyAsync(b.st + q.r, &z, sizeof(z), hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.rfirst + q.r, &q.e0, 4, hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.rcount + q.r, &E, 4, hipMemcpyHostToDevice, b.stream));

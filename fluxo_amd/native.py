@@ -96,7 +96,8 @@ EXPORTS = ["fx_ctx_create", "fx_ctx_destroy", "fx_last_error", "fx_mesh_upload",
            "fx_server_status", "fx_server_wait", "fx_server_get_stats", "fx_server_destroy",
            "fx_request_bytes", "fx_server_last_error", "fx_tune_volume", "fx_volume_tile",
            "fx_use_volume_tile", "fx_blending", "fx_glm", "fx_elem_xgp_upload", "fx_analyze",
-           "fx_rhs_noalloc", "fx_rhs_noalloc_bytes"]
+           "fx_rhs_noalloc", "fx_rhs_noalloc_bytes", "fx_fillflux", "fx_surfint", "fx_volint_nfvse",
+           "fx_run_to"]
 
 (FX_KERNEL_VOLUME_RK, FX_KERNEL_SURFACE, FX_KERNEL_LIFTING, FX_KERNEL_DT, FX_KERNEL_VOLUME_RHS,
  FX_KERNEL_NFVSE_ALPHA, FX_KERNEL_NFVSE_FV) = range(7)
@@ -121,6 +122,10 @@ def lib():
     L.fx_rhs.argtypes = [vp, C.c_double, vp]
     L.fx_lifting.argtypes = [vp, vp, vp, vp]
     L.fx_blending.argtypes = [vp, vp]
+    L.fx_fillflux.argtypes = [vp, vp, vp]
+    L.fx_surfint.argtypes = [vp, vp, vp, vp]
+    L.fx_volint_nfvse.argtypes = [vp, vp]
+    L.fx_run_to.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, ip, ip]
     L.fx_glm.argtypes = [vp, dp, dp]
     L.fx_calc_dt.argtypes = [vp, C.c_double, C.c_double, dp, ip]
     L.fx_rk_step.argtypes = [vp, C.c_double, C.c_double]
@@ -273,6 +278,38 @@ class Engine:
         a = np.empty(self.nElems)
         _chk(lib().fx_blending(self._ctx, _ptr(a)))
         return a
+
+    def fillflux(self):
+        """FillFlux on the device state: (Flux_master, Flux_slave), each (nSides, n, n, nVar)."""
+        n = self.N + 1
+        fm = np.empty((self.nSides, n, n, int(self.cfg.nVar)))
+        fs = np.empty_like(fm)
+        _chk(lib().fx_fillflux(self._ctx, _ptr(fm), _ptr(fs)))
+        return fm, fs
+
+    def surfint(self, flux_master, flux_slave, Ut):
+        """SurfInt: Ut + the surface integral of the given fluxes (a new array)."""
+        fm = np.ascontiguousarray(flux_master, dtype=np.float64)
+        fs = np.ascontiguousarray(flux_slave, dtype=np.float64)
+        out = np.array(Ut, dtype=np.float64, order="C", copy=True)
+        assert fm.shape == fs.shape == (self.nSides, self.N + 1, self.N + 1, int(self.cfg.nVar))
+        assert out.shape == self.shape()
+        _chk(lib().fx_surfint(self._ctx, _ptr(fm), _ptr(fs), _ptr(out)))
+        return out
+
+    def volint_nfvse(self):
+        """VolInt followed by VolInt_NFVSE: the blended advective volume integral (before -sJ)."""
+        Ut = np.empty(self.shape())
+        _chk(lib().fx_volint_nfvse(self._ctx, _ptr(Ut)))
+        return Ut
+
+    def run_to(self, t_end, cfl, dfl=0.0, t0=0.0, max_steps=-1):
+        """TimeDisc up to t_end (the last step shortened to end there); returns
+        (t, dt_last, steps, errType)."""
+        t, dt, st, err = C.c_double(t0), C.c_double(), C.c_int(), C.c_int()
+        _chk(lib().fx_run_to(self._ctx, max_steps, cfl, dfl, t_end, C.byref(t), C.byref(dt), C.byref(st),
+                             C.byref(err)))
+        return t.value, dt.value, st.value, err.value
 
     def calc_dt(self, cfl, dfl=0.0):
         dt, err = C.c_double(), C.c_int()

@@ -36,7 +36,7 @@ enum { FX_PREC_FP64 = 0, FX_PREC_FP32_SPLIT = 1 };
 
 /* Options that the reference reads from parameter.ini / CMake defines. */
 typedef struct {
-  int N;            /* polynomial degree, 'N' (interpolation.f90:125-126); 1..7 */
+  int N;            /* polynomial degree, 'N' (interpolation.f90:125-126); 2..5 */
   int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) or 9 (ideal GLM-MHD) */
   int nElems;       /* local elements */
   int nSides;       /* sides (all inner sides on periodic conforming meshes) */
@@ -136,6 +136,27 @@ int fx_volint(fx_ctx *ctx, double *Ut_host);
    (nVar,0:N,0:N,0:N,nElems). */
 int fx_lifting(fx_ctx *ctx, double *gradPx, double *gradPy, double *gradPz);
 
+/* FillFlux(Flux_master, Flux_slave, doMPISides=.FALSE.) (dg/fillflux.f90:41-110) on the device
+   state: the numerical flux of every side times SurfElem, Flux_master and Flux_slave
+   (nVar,0:N,0:N,nSides) (all sides are inner sides on a periodic conforming mesh, so
+   firstSlaveSide = 1 and LastSlaveSide = nSides).  Flux_slave = Flux_master, plus each side's
+   own Powell term for GLM-MHD (AddNonConsFlux).  With parabolic = 1 the lifting runs first and
+   the BR1 viscous flux is included, as in the reference's FillFlux after Lifting.  Either
+   output may be NULL. */
+int fx_fillflux(fx_ctx *ctx, double *Flux_master, double *Flux_slave);
+
+/* SurfInt(Flux_master, Flux_slave, Ut, doMPISides=.FALSE.) (dg/surfint.f90:48-264, Gauss-Lobatto
+   branch :177-184, 253-259): Ut(nVar,0:N,0:N,0:N,nElems) += Flux_master L_HatMinus0 at the
+   master element's face nodes and -= Flux_slave L_HatMinus0 at the slave element's, nodes on
+   several faces in the reference's side order.  Ut is read and written (host arrays). */
+int fx_surfint(fx_ctx *ctx, const double *Flux_master, const double *Flux_slave, double *Ut);
+
+/* VolInt_NFVSE(Ut) (shockcapturing/nfvse.f90:475-561) after VolInt on the device state: the
+   blending coefficient (CalcBlendingCoefficient + PropagateBlendingCoeff), the first-order
+   sub-cell finite-volume term F_FV, and Ut = (1 - alpha) VolInt_SplitForm + alpha F_FV per
+   element (the advective volume integral, before -sJ); needs nfvse = 1. */
+int fx_volint_nfvse(fx_ctx *ctx, double *Ut_host);
+
 /* CalcBlendingCoefficient + PropagateBlendingCoeff (shockcapturing/nfvse.f90:1537-1615,
    nfvse_mpi.f90:31-153) on the device state: alpha(nElems), as the RHS uses it. */
 int fx_blending(fx_ctx *ctx, double *alpha);
@@ -154,6 +175,15 @@ int fx_rk_step(fx_ctx *ctx, double t, double dt);
    CALCTIMESTEP error (0 if none). */
 int fx_run(fx_ctx *ctx, int nSteps, double CFLscale, double DFLscale, double *t, double *dt,
            int *errType);
+
+/* The TimeDisc loop (timedisc.f90:274-413) up to the end time tEnd, on the device: like fx_run,
+   with the reference's last step (timedisc.f90:313-317): when the CFL time step reaches tEnd
+   within 1e-4 of itself, the step is shortened to end exactly at tEnd and is the last one.
+   Runs at most maxSteps steps (maxSteps < 0: no limit, the reference's maxIter = -1) and stops
+   early on a CALCTIMESTEP error.  *t is the start time on entry and the final time on return
+   (tEnd when it was reached); *steps the steps taken, *dt the last step's dt. */
+int fx_run_to(fx_ctx *ctx, int maxSteps, double CFLscale, double DFLscale, double tEnd, double *t, double *dt,
+              int *steps, int *errType);
 
 /* CalcEntropy (navierstokes/analyze_equation.f90:549-599) on the device state and
    the last Ut: S = int -rho s/(kappa-1), dSdU_Ut = int dS/dU . Ut. */

@@ -108,6 +108,13 @@ void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, dou
 void ora_volint_splitform(const ora_problem *P, const double *U, double *Ut);
 void ora_surface_integral(const ora_problem *P, const double *U, const double *gx, const double *gy,
                           const double *gz, double *Ut);
+/* FillFlux (src/dg/fillflux.f90:41-110) of the state: Flux_master and Flux_slave
+   (5,0:N,0:N,nSides), the numerical flux (with the BR1/BR2 viscous mean when parabolic) times
+   SurfElem; Flux_slave = Flux_master */
+void ora_fill_flux(const ora_problem *P, const double *U, double *Fm, double *Fs);
+/* SurfInt (src/dg/surfint.f90:48-264, Gauss-Lobatto branch :177-184, 253-259): Ut +=
+   Flux_master L_HatMinus0 (master face nodes), -= Flux_slave L_HatMinus0 (slave), side by side */
+void ora_surf_int(const ora_problem *P, int nvar, const double *Fm, const double *Fs, double *Ut);
 /* BR1 lifting of nvar lifting variables w (volume nodes): physical gradients of w */
 void ora_lift(const ora_problem *P, int nvar, const double *w, double *gx, double *gy, double *gz);
 /* BR2 lifting of nvar lifting variables w: volume gradients and each side's master/slave
@@ -156,6 +163,9 @@ void ora_mhd_derigs(const ora_mhd *M, double glm_ch, const double *UL, const dou
 void ora_mhd_riemann(const ora_mhd *M, double glm_ch, const double *UL, const double *UR, const double *nv,
                      const double *t1, const double *t2, double *F);
 void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut);
+/* FillFlux of GLM-MHD (fillflux.f90:41-110 with NONCONS): Flux_master, Flux_slave (9,0:N,0:N,nSides),
+   which differ by each side's own Powell term */
+void ora_mhd_fill_flux(const ora_mhd *M, const double *U, double glm_ch, double *FluxM, double *FluxS);
 double ora_mhd_glm_dtch1(const ora_mhd *M, double CFLscale);
 /* CALCTIMESTEP; DFLscale sets the viscous/resistive limit of the resistive case */
 double ora_mhd_calc_timestep(const ora_mhd *M, const double *U, double CFLscale, double DFLscale,

@@ -433,6 +433,74 @@ static void add_noncons_surface(const double *UL, const double *UR, const double
 /* DGTimeDerivative_weakForm (dg.f90:282-452) for ideal GLM-MHD with NONCONS=1 on LGL
    periodic conforming meshes: VolInt_SplitForm with EvalAdvFluxAverage3D +
    AddNonConsFluxTilde3D, FillFlux (Riemann + AddNonConsFlux on both sides), SurfInt, -sJ. */
+/* FillFlux for GLM-MHD (src/dg/fillflux.f90:41-110 with NONCONS): the surface flux, in the
+   resistive case plus the mean of the two sides' diffusive fluxes (mhd/riemann.f90:116-138) from
+   the volume gradients (BR1) or the sides' face gradients fg (BR2), then each side's Powell term
+   (AddNonConsFlux), times SurfElem -> FluxM (Flux_master), FluxS (Flux_slave) */
+static void mhd_fill_flux(const ora_mhd *M, const double *U, double glm_ch, const double *gx, const double *gy,
+                          const double *gz, double *const fg[6], double *FluxM, double *FluxS)
+{
+  const ora_problem *P = M->P;
+  const int N = P->N, n = N + 1, n3 = n * n * n;
+  const int br2 = M->parabolic && M->lifting == 2;
+  for (int s = 0; s < P->nSides; s++) {
+    const int *t = P->SideToElem + 5 * s;
+    const int em = t[0] - 1, es = t[1] - 1, lm = t[2], ls = t[3], fl = t[4];
+    for (int q = 0; q < n; q++)
+      for (int p = 0; p < n; p++) {
+        int im[3], is[3];
+        ora_side_to_vol(N, 0, p, q, 0, lm, im);
+        ora_side_to_vol(N, 0, p, q, fl, ls, is);
+        const size_t om = MV * ((size_t)em * n3 + im[0] + n * (im[1] + n * im[2]));
+        const size_t os = MV * ((size_t)es * n3 + is[0] + n * (is[1] + n * is[2]));
+        const size_t f = (size_t)s * n * n + p + n * q;
+        const double *nv = P->NormVec + 3 * f, *t1 = P->TangVec1 + 3 * f, *t2 = P->TangVec2 + 3 * f;
+        double Fm[MV], Fs[MV];
+        ora_mhd_riemann(M, glm_ch, U + om, U + os, nv, t1, t2, Fm);
+        if (M->parabolic) {
+          double kL[MV], gL[MV], jL[MV], kR[MV], gR[MV], jR[MV];
+          if (br2) {
+            const size_t fo = MV * f;
+            mhd_diff_flux(M, U + om, fg[0] + fo, fg[2] + fo, fg[4] + fo, kL, gL, jL);
+            mhd_diff_flux(M, U + os, fg[1] + fo, fg[3] + fo, fg[5] + fo, kR, gR, jR);
+          } else {
+            mhd_diff_flux(M, U + om, gx + om, gy + om, gz + om, kL, gL, jL);
+            mhd_diff_flux(M, U + os, gx + os, gy + os, gz + os, kR, gR, jR);
+          }
+          for (int v = 1; v < MV; v++)
+            Fm[v] = Fm[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
+        }
+        memcpy(Fs, Fm, sizeof(Fs));
+        add_noncons_surface(U + om, U + os, nv, Fm);
+        add_noncons_surface(U + os, U + om, nv, Fs);
+        for (int v = 0; v < MV; v++) {
+          FluxM[MV * f + v] = Fm[v] * P->SurfElem[f];
+          FluxS[MV * f + v] = Fs[v] * P->SurfElem[f];
+        }
+      }
+  }
+}
+
+void ora_mhd_fill_flux(const ora_mhd *M, const double *U, double glm_ch, double *FluxM, double *FluxS)
+{
+  const ora_problem *P = M->P;
+  const int n = P->N + 1;
+  const size_t nt = (size_t)MV * n * n * n * P->nElems;
+  double *gx = NULL, *gy = NULL, *gz = NULL, *fg[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
+  if (M->parabolic) {
+    gx = malloc(sizeof(double) * nt); gy = malloc(sizeof(double) * nt); gz = malloc(sizeof(double) * nt);
+    if (M->lifting == 2) {
+      for (int d = 0; d < 6; d++) fg[d] = malloc(sizeof(double) * MV * n * n * P->nSides);
+      mhd_br2_gradients(M, U, gx, gy, gz, fg);
+    } else {
+      ora_mhd_lifting(M, U, gx, gy, gz);
+    }
+  }
+  mhd_fill_flux(M, U, glm_ch, gx, gy, gz, fg, FluxM, FluxS);
+  free(gx); free(gy); free(gz);
+  for (int d = 0; d < 6; d++) free(fg[d]);
+}
+
 void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
 {
   const ora_problem *P = M->P;
@@ -496,44 +564,11 @@ void ora_mhd_rhs(const ora_mhd *M, const double *U, double glm_ch, double *Ut)
     mhd_volint_visc(M, U, gx, gy, gz, Dh, Ut);
   }
   /* surface: FillFlux + SurfInt (LGL) */
-  const double L0 = Lm[0];
-  for (int s = 0; s < P->nSides; s++) {
-    const int *t = P->SideToElem + 5 * s;
-    const int em = t[0] - 1, es = t[1] - 1, lm = t[2], ls = t[3], fl = t[4];
-    for (int q = 0; q < n; q++)
-      for (int p = 0; p < n; p++) {
-        int im[3], is[3];
-        ora_side_to_vol(N, 0, p, q, 0, lm, im);
-        ora_side_to_vol(N, 0, p, q, fl, ls, is);
-        const size_t om = MV * ((size_t)em * n3 + im[0] + n * (im[1] + n * im[2]));
-        const size_t os = MV * ((size_t)es * n3 + is[0] + n * (is[1] + n * is[2]));
-        const size_t f = (size_t)s * n * n + p + n * q;
-        const double *nv = P->NormVec + 3 * f, *t1 = P->TangVec1 + 3 * f, *t2 = P->TangVec2 + 3 * f;
-        double Fm[MV], Fs[MV];
-        ora_mhd_riemann(M, glm_ch, U + om, U + os, nv, t1, t2, Fm);
-        if (M->parabolic) {  /* mean of the two sides' diffusive fluxes (mhd/riemann.f90:116-138)
-                                with the volume (BR1) or the sides' face gradients (BR2) */
-          double kL[MV], gL[MV], jL[MV], kR[MV], gR[MV], jR[MV];
-          if (br2) {
-            const size_t fo = MV * f;
-            mhd_diff_flux(M, U + om, fg[0] + fo, fg[2] + fo, fg[4] + fo, kL, gL, jL);
-            mhd_diff_flux(M, U + os, fg[1] + fo, fg[3] + fo, fg[5] + fo, kR, gR, jR);
-          } else {
-            mhd_diff_flux(M, U + om, gx + om, gy + om, gz + om, kL, gL, jL);
-            mhd_diff_flux(M, U + os, gx + os, gy + os, gz + os, kR, gR, jR);
-          }
-          for (int v = 1; v < MV; v++)
-            Fm[v] = Fm[v] + 0.5 * (nv[0] * (kL[v] + kR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (jL[v] + jR[v]));
-        }
-        memcpy(Fs, Fm, sizeof(Fs));
-        add_noncons_surface(U + om, U + os, nv, Fm);
-        add_noncons_surface(U + os, U + om, nv, Fs);
-        for (int v = 0; v < MV; v++) {
-          Ut[om + v] += (Fm[v] * P->SurfElem[f]) * L0;
-          Ut[os + v] -= (Fs[v] * P->SurfElem[f]) * L0;
-        }
-      }
-  }
+  const size_t nface = (size_t)MV * n * n * P->nSides;
+  double *FluxM = malloc(sizeof(double) * nface), *FluxS = malloc(sizeof(double) * nface);
+  mhd_fill_flux(M, U, glm_ch, gx, gy, gz, fg, FluxM, FluxS);
+  ora_surf_int(P, MV, FluxM, FluxS, Ut);
+  free(FluxM); free(FluxS);
   for (size_t p = 0; p < (size_t)n3 * E; p++) {
     const double sj = -P->sJ[p];
     for (int v = 0; v < MV; v++) Ut[MV * p + v] = Ut[MV * p + v] * sj;

@@ -1069,13 +1069,15 @@ void ora_lifting(const ora_problem *P, const double *U, double *gx, double *gy, 
 
 /* FillFlux + SurfInt (fillflux.f90:41-110, riemann.f90:104-169, surfint.f90) with the face
    gradients (primitive, (5,0:N,0:N,nSides) master/slave) given, or NULL for Euler */
-static void surface_integral_faces(const ora_problem *P, const double *U, double *const fg[6], double *Ut)
+/* FillFlux (src/dg/fillflux.f90:41-110): the numerical flux of every side (Riemann,
+   riemann.f90:104-169, with the BR1/BR2 viscous mean from the sides' face gradients fg when
+   parabolic) times SurfElem -> Fm (Flux_master; Flux_slave is the same for these equations) */
+static void fill_flux_faces(const ora_problem *P, const double *U, double *const fg[6], double *Fm)
 {
   ops_t o; make_ops(P, &o);
   int n = o.n;
   size_t nface = (size_t)NV * n * n * o.S;
   double *Um = malloc(sizeof(double) * nface), *Us = malloc(sizeof(double) * nface);
-  double *Fm = malloc(sizeof(double) * nface);
   prolong_to_face(P, &o, NV, U, Um, Us);
   for (int s = 0; s < o.S; s++) {
     double lam = 0.0;  /* LambdaMaxMax of the face (Riemann 16) */
@@ -1101,8 +1103,23 @@ static void surface_integral_faces(const ora_problem *P, const double *U, double
         for (int v = 0; v < NV; v++) F[v] = F[v] * se;
       }
   }
+  free(Um); free(Us);
+}
+
+/* FillFlux + SurfInt with the sides' face gradients fg */
+static void surface_integral_faces(const ora_problem *P, const double *U, double *const fg[6], double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  double *Fm = malloc(sizeof(double) * (size_t)NV * o.n * o.n * o.S);
+  fill_flux_faces(P, U, fg, Fm);
   surf_int(P, &o, NV, Fm, Fm, -1.0, Ut);
-  free(Um); free(Us); free(Fm);
+  free(Fm);
+}
+
+void ora_surf_int(const ora_problem *P, int nvar, const double *Fm, const double *Fs, double *Ut)
+{
+  ops_t o; make_ops(P, &o);
+  surf_int(P, &o, nvar, Fm, Fs, -1.0, Ut);
 }
 
 /* FillFlux + SurfInt with BR1 face gradients: the volume gradients prolonged to the faces
@@ -1227,6 +1244,30 @@ void ora_dg_time_derivative(const ora_problem *P, const double *U, double t, dou
     for (int v = 0; v < NV; v++) Ut[NV * p + v] = Ut[NV * p + v] * s;
   }
   if (P->parabolic) { free(gx); free(gy); free(gz); }
+}
+
+void ora_fill_flux(const ora_problem *P, const double *U, double *Fm, double *Fs)
+{
+  const size_t n3E = (size_t)(P->N + 1) * (P->N + 1) * (P->N + 1) * P->nElems;
+  const size_t nface = (size_t)NV * (P->N + 1) * (P->N + 1) * P->nSides;
+  double *fg[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
+  if (P->parabolic) {
+    double *gx = malloc(sizeof(double) * NV * n3E), *gy = malloc(sizeof(double) * NV * n3E);
+    double *gz = malloc(sizeof(double) * NV * n3E);
+    for (int d = 0; d < 6; d++) fg[d] = malloc(sizeof(double) * nface);
+    if (P->lifting == 2) {
+      br2_gradients(P, U, gx, gy, gz, fg);
+    } else {  /* BR1: the volume gradients prolonged to the faces */
+      ops_t o; make_ops(P, &o);
+      ora_lifting(P, U, gx, gy, gz);
+      const double *g[3] = {gx, gy, gz};
+      for (int d = 0; d < 3; d++) prolong_to_face(P, &o, NV, g[d], fg[2 * d], fg[2 * d + 1]);
+    }
+    free(gx); free(gy); free(gz);
+  }
+  fill_flux_faces(P, U, fg, Fm);
+  memcpy(Fs, Fm, sizeof(double) * nface);
+  for (int d = 0; d < 6; d++) free(fg[d]);
 }
 
 /* Carpenter RK4-5 (timedisc_vars.f90:204-233) */

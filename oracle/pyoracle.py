@@ -76,6 +76,9 @@ def lib():
         L.ora_persson_peraire.argtypes = [C.c_void_p, _dp, _dp]
         L.ora_blending_coefficient.argtypes = [C.c_void_p, _dp, _dp]
         L.ora_volint_nfvse.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ora_fill_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ora_surf_int.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.ora_mhd_fill_flux.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
         L.ora_analyze.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         _LIB = L
@@ -187,6 +190,30 @@ class Problem:
         g = [np.zeros(self.shape()) for _ in range(3)]
         lib().ora_lifting(self.ptr, np.ascontiguousarray(U), *g)
         return g
+
+    def face_shape(self, nvar=5):
+        n = self.N + 1
+        return (self.mesh["nSides"], n, n, nvar)
+
+    def fillflux(self, U):
+        """FillFlux: (Flux_master, Flux_slave), each (nSides, n, n, 5)."""
+        fm, fs = np.zeros(self.face_shape()), np.zeros(self.face_shape())
+        lib().ora_fill_flux(self.ptr, np.ascontiguousarray(U), fm, fs)
+        return fm, fs
+
+    def surfint(self, fm, fs, Ut, nvar=5):
+        """SurfInt: Ut + the surface integral of the fluxes (a new array)."""
+        out = np.array(Ut, dtype=np.float64, order="C", copy=True)
+        lib().ora_surf_int(self.ptr, nvar, np.ascontiguousarray(fm, dtype=np.float64),
+                           np.ascontiguousarray(fs, dtype=np.float64), out)
+        return out
+
+    def volint_nfvse(self, U):
+        """VolInt then VolInt_NFVSE with the blending coefficient of U."""
+        U = np.ascontiguousarray(U)
+        Ut = self.volint(U)
+        lib().ora_volint_nfvse(self.ptr, U, self.blending(U), Ut)
+        return Ut
 
     def lserk_step(self, U, Ut_tmp, t, dt):
         lib().ora_lserk_step(self.ptr, U, Ut_tmp, t, dt)
@@ -383,6 +410,18 @@ class MhdProblem:
         Ut = np.zeros(self.shape())
         lib().ora_mhd_rhs(self.ptr, np.ascontiguousarray(U), self.glm_ch if glm_ch is None else glm_ch, Ut)
         return Ut
+
+    def fillflux(self, U, glm_ch=None):
+        """FillFlux: (Flux_master, Flux_slave), each (nSides, n, n, 9)."""
+        n = self.N + 1
+        shp = (self.mesh["nSides"], n, n, 9)
+        fm, fs = np.zeros(shp), np.zeros(shp)
+        lib().ora_mhd_fill_flux(self.ptr, np.ascontiguousarray(U), self.glm_ch if glm_ch is None else glm_ch,
+                                fm, fs)
+        return fm, fs
+
+    def surfint(self, fm, fs, Ut):
+        return self.base.surfint(fm, fs, Ut, nvar=9)
 
     def lifting(self, U):
         """Primitive gradients (rho, v, p, B, psi) x, y, z of the resistive case."""

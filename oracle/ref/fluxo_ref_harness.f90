@@ -12,6 +12,10 @@
 !> the values it collects for AnalyzeToFile are written as 'name value' lines to analysis.txt
 !> in the working directory.
 !>
+!> After the analysis it writes the face fluxes Flux_master/Flux_slave of that right-hand side
+!> (FillFlux, src/dg/fillflux.f90:41-110) and, with shock capturing, VolInt_NFVSE's blended
+!> volume integral (src/shockcapturing/nfvse.f90:475-561) of the same state.
+!>
 !> usage: fluxo_ref_harness parameter.ini input.bin output.bin
 !==================================================================================================
 #include "defines.h"
@@ -28,7 +32,7 @@ program fluxo_ref_harness
   use MOD_Testcase,            only: DefineParametersTestcase, InitTestcase
   use MOD_GetBoundaryFlux,     only: InitBC
   use MOD_DG,                  only: InitDG, DGTimeDerivative
-  use MOD_DG_Vars,             only: U, Ut, nTotalU
+  use MOD_DG_Vars,             only: U, Ut, nTotalU, Flux_master, Flux_slave
   use MOD_VolInt,              only: VolInt_adv_SplitForm
   use MOD_Restart_Vars,        only: DoRestart, RestartInitIsDone
   use MOD_Mappings,            only: buildMappings
@@ -47,6 +51,7 @@ program fluxo_ref_harness
 #if SHOCK_NFVSE
   use MOD_ShockCapturing,      only: DefineParametersShockCapturing, InitShockCapturing
   use MOD_NFVSE_Vars,          only: alpha
+  use MOD_NFVSE,               only: CalcBlendingCoefficient, VolInt_NFVSE
 #endif
 #ifdef PP_GLM
   use MOD_Equation_Vars,       only: GLM_ch, GLM_dtch1
@@ -173,6 +178,17 @@ program fluxo_ref_harness
     write(u_an, '(A,1X,ES25.17)') trim(A2F_VarNames(iv)), A2F_Data(iv)
   end do
   close(u_an)
+  ! the face fluxes of FillFlux from that right-hand side (module arrays of MOD_DG_Vars; all sides
+  ! are inner sides, so Flux_slave spans 1:nSides)
+  write(u_out) Flux_master, Flux_slave
+#if SHOCK_NFVSE
+  ! VolInt followed by VolInt_NFVSE alone: blending coefficient, volume split form, blend
+  call CalcBlendingCoefficient(U)
+  Ut = 0.
+  call VolInt_adv_SplitForm(Ut)
+  call VolInt_NFVSE(Ut)
+  write(u_out) Ut
+#endif
   ! CFL/DFL scaling of FillCFL_DFL (timedisc.f90:697-741) with the RK4-5 tables
   call SetTimeDiscCoefs('carpenterrk4-5')
   CFLScale = cflIn*CFLScaleAlpha(MIN(15, PP_N))/(2.*PP_N + 1.)

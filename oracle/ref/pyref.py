@@ -52,11 +52,13 @@ def read_analysis(d):
 
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
         cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0, analyze=None):
-    """Metrics, VolInt, RHS (+ gradients), entropy, dt and an nsteps trajectory computed by
-    the reference's code on the given mesh (connectivity + NodeCoords) and state.
+    """Metrics, VolInt, RHS (+ gradients), the face fluxes of FillFlux, entropy, dt and an
+    nsteps trajectory computed by the reference's code on the given mesh (connectivity +
+    NodeCoords) and state.
     nfvse: None, or dict of NFVSE parameters (ComputeAlpha, ShockBlendCoef, alpha_max,
     alpha_min, SpacePropSweeps, SpacePropFactor) for the shock-capturing build, which also
-    returns the blending coefficient alpha of the RHS evaluation.  lifting 2: the BR2 build
+    returns the blending coefficient alpha of the RHS evaluation and VolInt_NFVSE's blended
+    volume integral of the state (volint_nfvse).  lifting 2: the BR2 build
     with penalty etaBR2 (gradients returned are the volume gradients).  out["analysis"]
     holds what the reference's Analyze collects for this state at time t0 (error norms
     against AnalyzeExactFunc, bulk integrals, entropy); `analyze` adds parameters to it."""
@@ -114,6 +116,10 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
             take(k, (E, n, n, n, 5))
     if nfvse is not None:
         take("alpha", (E,))
+    take("Flux_master", (S, n, n, 5))
+    take("Flux_slave", (S, n, n, 5))
+    if nfvse is not None:
+        take("volint_nfvse", (E, n, n, n, 5))
     take("dt0", (2,))
     out["errType0"] = int(out["dt0"][1])
     out["dt0"] = float(out["dt0"][0])
@@ -194,6 +200,8 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
     if par_on:
         for k in ("gradPx", "gradPy", "gradPz"):
             take(k, (E, n, n, n, 9))
+    take("Flux_master", (S, n, n, 9))
+    take("Flux_slave", (S, n, n, 9))
     take("dt0", (2,))
     out["errType0"] = int(out["dt0"][1])
     out["dt0"] = float(out["dt0"][0])

@@ -72,6 +72,12 @@ CASES = [
     for rs, N, vf in ((2, 3, 32), (22, 3, 32), (23, 4, 32), (24, 3, 32), (3, 3, 32), (3, 2, 2), (3, 4, 3),
                       (3, 3, 4))]
 
+# 100-step trajectories (the trajectory bar: 1e-10 relative L2 after 100 LSERK steps):
+# Navier-Stokes Taylor-Green BR1 on a warped 3^3 mesh with 1e-3 relative noise, CFL/DFL 0.9
+CASES += [
+    ("ns_n3_tgv_br1_warped_100steps", 3, 3, 3, True, (-PI,) * 3, (2 * PI,) * 3, 32, 1, True, 7, 1e-3, 100),
+]
+
 # NFVSE shock capturing (the reference built with SHOCK_NFVSE): a Gaussian pressure pulse
 # (fluxo_amd.workloads.sedov_blast with the width and background pressure given) on warped
 # meshes of [-1.5,1.5]^3, CFL 0.5.  fixed: None = Persson-Peraire indicator (ComputeAlpha=1),
@@ -144,29 +150,133 @@ MHD_BR2_CASES = [
 ]
 
 
-def make_mhd(case, riemann=1, lifting=1, etaBR2=2.0):
+def make_mhd(case, riemann=1, lifting=1, etaBR2=2.0, cfl=0.9, bench_ic=False):
     name, K, N, Ngeo, warp, nsteps = case[:6]
     res = dict(mu=case[6], eta=case[7], Pr=case[8]) if len(case) > 6 else None
     m = O.box_mesh(K, N, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=Ngeo, warp=warp)
-    P = O.MhdProblem(m)
-    U = P.exact_orszag_tang()
-    rng = np.random.default_rng(20261202)
-    U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
-    U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
-    U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
-    r = R.run_mhd(m, U, cfl=0.9, nsteps=nsteps, resistive=res, dfl=0.9, riemann=riemann, lifting=lifting,
+    if bench_ic:
+        from fluxo_amd.workloads import perturbed_orszag_tang
+        U = perturbed_orszag_tang(m["Elem_xGP"], 2026)
+    else:
+        P = O.MhdProblem(m)
+        U = P.exact_orszag_tang()
+        rng = np.random.default_rng(20261202)
+        U[..., 0] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 0].shape)
+        U[..., 5:8] *= 1.0 + 1e-3 * rng.uniform(-1, 1, U[..., 5:8].shape)
+        U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
+    r = R.run_mhd(m, U, cfl=cfl, nsteps=nsteps, resistive=res, dfl=cfl, riemann=riemann, lifting=lifting,
                   etaBR2=etaBR2)
     out = dict(K=K, N=N, Ngeo=Ngeo, warp=int(warp), x0=np.zeros(3), L=np.ones(3), kappa=5.0 / 3.0, mu_0=1.0,
                riemann=riemann, lifting=lifting, etaBR2=etaBR2,
-               glm_scale=0.5, cfl=0.9, nsteps=nsteps, U=U, Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"],
+               glm_scale=0.5, cfl=cfl, nsteps=nsteps, U=U, Ut=r["Ut"], volint=r["volint"], dt0=r["dt0"],
                errType0=r["errType0"], glm_ch0=r["glm_ch0"], glm_dtch1=r["glm_dtch1"], U_end=r["U_end"],
                t_end=r["t_end"], sJ=r["sJ"], Mf=r["Mf"], SurfElem=r["SurfElem"])
     if res is not None:
-        out.update(mu=res["mu"], eta=res["eta"], Pr=res["Pr"], dfl=0.9, gradPx=r["gradPx"], gradPy=r["gradPy"],
+        out.update(mu=res["mu"], eta=res["eta"], Pr=res["Pr"], dfl=cfl, gradPx=r["gradPx"], gradPy=r["gradPy"],
                    gradPz=r["gradPz"])
     os.makedirs(os.path.join(HERE, "mhd"), exist_ok=True)
     np.savez(os.path.join(HERE, "mhd", name + ".npz"), **out)
     print(f"{name}: dt0={r['dt0']:.6e} GLM_ch={r['glm_ch0']:.6f}")
+
+
+# 100-step resistive GLM-MHD trajectory: the GLM-MHD benchmark's own initial state
+# (fluxo_amd.workloads.perturbed_orszag_tang, seed 2026) and physics (eta 1e-3, mu 0, Pr 0.72,
+# CFL/DFL 0.5) on a warped 4^3 mesh of [0,1]^3, N=4.  On 2^3 and 3^3 meshes the reference's own
+# run ends in NaN within 100 steps (the Orszag-Tang modes are not resolved there).
+MHD_TRAJ_CASES = [
+    ("mhd_resistive_n4_eta1e-3_100steps", 4, 4, 3, True, 100, 0.0, 1e-3, 0.72),
+]
+
+
+# C4's own initial state (fluxo_amd.workloads.sedov_blast: sigma 0.06, p0 1e-5, E0 1 over rho 1
+# on [-1.5,1.5]^3) on a periodic Cartesian K^3 mesh, N=3, Ranocha + LLF, Persson-Peraire
+# blending with alpha_max 0.5, alpha_min 0.01, one propagation sweep (factor 0.5), CFL 0.5,
+# the C4 settings on a smaller mesh.  The state is regenerated by the tests from the formula,
+# so only the outputs are stored (compressed).
+SEDOV_CASES = [
+    # name, K, N, nsteps
+    ("euler_nfvse_c4_sedov_k12_100steps", 12, 3, 100),
+]
+
+
+def make_sedov(case):
+    from fluxo_amd.workloads import sedov_blast, SEDOV_SIGMA, SEDOV_E0
+    name, K, N, nsteps = case
+    x0, L = (-1.5,) * 3, (3.0,) * 3
+    m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=1, warp=False)
+    U = sedov_blast(m["Elem_xGP"])
+    par_nfvse = dict(ComputeAlpha=1, ShockBlendCoef=0.0, alpha_max=0.5, alpha_min=0.01, SpacePropSweeps=1,
+                     SpacePropFactor=0.5)
+    r = R.run(m, U, volflux=32, riemann=1, cfl=0.5, dfl=0.5, nsteps=nsteps, nfvse=par_nfvse, timeout=3600)
+    out = dict(K=K, N=N, Ngeo=1, warp=0, x0=np.array(x0), L=np.array(L), volflux=32, riemann=1, parabolic=0,
+               mu0=0.0, Pr=0.72, kappa=1.4, cfl=0.5, dfl=0.5, nsteps=nsteps, ic="sedov", sigma=SEDOV_SIGMA,
+               E0=SEDOV_E0, p0=1e-5, Ut=r["Ut"], dt0=r["dt0"], errType0=r["errType0"], U_end=r["U_end"],
+               t_end=r["t_end"], Entropy=r["Entropy"], dSdU_Ut=r["dSdU_Ut"], nfvse=1, alpha=r["alpha"],
+               alpha_max=0.5, alpha_min=0.01, sweeps=1, prop_factor=0.5, fixed_alpha=-1.0)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    Ue = r["U_end"]
+    p = 0.4 * (Ue[..., 4] - 0.5 * (Ue[..., 1:4] ** 2).sum(-1) / Ue[..., 0])
+    print(f"{name}: t_end={r['t_end']:.6e} dt0={r['dt0']:.6e} alpha>0 {np.sum(r['alpha'] > 0)} "
+          f"rho_min {Ue[..., 0].min():.3e} p_min {p.min():.3e}")
+
+
+# The reference's sub-interfaces on their own (tests/golden/iface): FillFlux's Flux_master /
+# Flux_slave of the right-hand side, VolInt_SplitForm, and VolInt_NFVSE with shock capturing,
+# with the state and the full RHS.  kind: euler | ns (lifting 1/2) | nfvse | mhd (resistive or ideal)
+IFACE_CASES = [
+    # name, kind, K, N, options
+    ("iface_euler_n3_ranocha_llf_warped", "euler", 3, 3, dict(vf=32, rs=1)),
+    ("iface_euler_n4_es44_warped", "euler", 2, 4, dict(vf=32, rs=44)),
+    ("iface_ns_n3_tgv_br1_warped", "ns", 2, 3, dict(lifting=1)),
+    ("iface_ns_n2_tgv_br2_warped", "ns", 3, 2, dict(lifting=2)),
+    ("iface_euler_nfvse_n3_pulse_warped", "nfvse", 4, 3, dict(sigma=0.2, p0=1e-2)),
+    ("iface_mhd_resistive_n3_warped", "mhd", 2, 3, dict(res=(0.01, 0.02, 0.72), rs=1)),
+    ("iface_mhd_n4_es_llf_warped", "mhd", 2, 4, dict(res=None, rs=10)),
+]
+
+
+def make_iface(case):
+    from fluxo_amd.workloads import sedov_blast
+    name, kind, K, N, o = case
+    rng = np.random.default_rng(20261205)
+    out = dict(kind=kind, K=K, N=N, Ngeo=3, warp=1)
+    if kind in ("euler", "ns"):
+        tgv = kind == "ns"
+        x0, L = ((-PI,) * 3, (2 * PI,) * 3) if tgv else ((0.0,) * 3, (2.0,) * 3)
+        m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=True)
+        U = O.exactfunc(7 if tgv else 2, m["Elem_xGP"])
+        U = U * (1.0 + 0.02 * rng.uniform(-1.0, 1.0, U.shape))
+        vf, rs, lift = o.get("vf", 32), o.get("rs", 1), o.get("lifting", 1)
+        mu0 = 1.0 / 1600 if tgv else 0.0
+        r = R.run(m, U, volflux=vf, riemann=rs, parabolic=tgv, mu0=mu0, Pr=0.72, lifting=lift)
+        out.update(volflux=vf, riemann=rs, parabolic=int(tgv), lifting=lift, etaBR2=2.0, mu0=mu0, Pr=0.72,
+                   kappa=1.4)
+    elif kind == "nfvse":
+        x0, L = (-1.5,) * 3, (3.0,) * 3
+        m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=True)
+        U = sedov_blast(m["Elem_xGP"], sigma=o["sigma"], p0=o["p0"])
+        par = dict(ComputeAlpha=1, ShockBlendCoef=0.0, alpha_max=0.5, alpha_min=0.01, SpacePropSweeps=1,
+                   SpacePropFactor=0.5)
+        r = R.run(m, U, volflux=32, riemann=1, nfvse=par)
+        out.update(volflux=32, riemann=1, parabolic=0, kappa=1.4, alpha=r["alpha"], volint_nfvse=r["volint_nfvse"])
+    else:
+        x0, L = (0.0,) * 3, (1.0,) * 3
+        m = O.box_mesh(K, N, x0=x0, length=L, Ngeo=3, warp=True)
+        from fluxo_amd.workloads import perturbed_orszag_tang
+        U = perturbed_orszag_tang(m["Elem_xGP"], 7)
+        U[..., 8] = 1e-2 * rng.uniform(-1, 1, U[..., 8].shape)
+        res = o["res"]
+        r = R.run_mhd(m, U, resistive=None if res is None else dict(mu=res[0], eta=res[1], Pr=res[2]),
+                      riemann=o["rs"])
+        out.update(riemann=o["rs"], parabolic=int(res is not None), kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5)
+        if res is not None:
+            out.update(mu=res[0], eta=res[1], Pr=res[2])
+    out.update(x0=np.array(x0, float), L=np.array(L, float), U=U, Ut=r["Ut"], volint=r["volint"],
+               Flux_master=r["Flux_master"], Flux_slave=r["Flux_slave"])
+    os.makedirs(os.path.join(HERE, "iface"), exist_ok=True)
+    np.savez(os.path.join(HERE, "iface", name + ".npz"), **out)
+    d = np.abs(r["Flux_master"] - r["Flux_slave"]).max()
+    print(f"{name}: |Flux_master - Flux_slave| max {d:.3e}")
 
 
 # The reference's Analyze of a state (CalcErrorNorms, CalcBulk, AnalyzeEquation and, for the
@@ -299,10 +409,19 @@ def make_nfvse(case, mu0=None, riemann=1):
 
 if __name__ == "__main__":
     sys.path.insert(0, ROOT)
-    if len(sys.argv) > 1:  # only the named cases of CASES and NFVSE_RIEMANN_CASES
+    if len(sys.argv) > 1:  # only the named cases
         for c in CASES:
             if c[0] in sys.argv[1:]:
                 make(c)
+        for c in MHD_TRAJ_CASES:
+            if c[0] in sys.argv[1:]:
+                make_mhd(c, cfl=0.5, bench_ic=True)
+        for c in SEDOV_CASES:
+            if c[0] in sys.argv[1:]:
+                make_sedov(c)
+        for c in IFACE_CASES:
+            if c[0] in sys.argv[1:] or "iface" in sys.argv[1:]:
+                make_iface(c)
         for c in NFVSE_RIEMANN_CASES:
             if c[0] in sys.argv[1:]:
                 make_nfvse(c[:8], riemann=c[8])
@@ -321,6 +440,12 @@ if __name__ == "__main__":
         make_mhd((name, K, N, Ngeo, warp, nsteps) + (tuple(res) if res else ()), riemann=rs)
     for c in MHD_BR2_CASES:
         make_mhd(c[:9], lifting=2, etaBR2=c[9])
+    for c in MHD_TRAJ_CASES:
+        make_mhd(c, cfl=0.5, bench_ic=True)
+    for c in SEDOV_CASES:
+        make_sedov(c)
+    for c in IFACE_CASES:
+        make_iface(c)
     for c in ANALYZE_CASES:
         make_analyze(c)
     for c in HOPR_CASES:

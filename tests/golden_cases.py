@@ -47,9 +47,14 @@ def mesh_args(g):
 
 
 def initial_state(g, exactfunc, mesh):
-    """The stored U, or for the large noise-free case the ExactFunc state it was made from."""
+    """The stored U, or for the large cases the state it was made from: the Sedov-type pulse
+    (fluxo_amd.workloads.sedov_blast with the stored sigma, E0, p0) or the noise-free
+    ExactFunc 2 density wave."""
     if "U" in g:
         return g["U"]
+    if str(g.get("ic", "")) == "sedov":
+        from fluxo_amd.workloads import sedov_blast
+        return sedov_blast(mesh["Elem_xGP"], sigma=float(g["sigma"]), E0=float(g["E0"]), p0=float(g["p0"]))
     return exactfunc(2, mesh["Elem_xGP"])
 
 

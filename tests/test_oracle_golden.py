@@ -31,7 +31,8 @@ def test_oracle_matches_reference(name):
     assert dS == pytest.approx(g["dSdU_Ut"], rel=1e-11, abs=1e-14)
     dt, err = P.calc_dt(U, g["cfl"], g["dfl"])
     assert err == g["errType0"] and dt == pytest.approx(g["dt0"], rel=1e-15)
-    if g["nsteps"] <= 20:
+    # trajectories where the oracle finishes in seconds (node-steps bounded)
+    if g["nsteps"] <= 20 or g["K"] ** 3 * (g["N"] + 1) ** 3 * g["nsteps"] <= 2e5:
         Ue, Utmp, t = U.copy(), np.zeros_like(U), 0.0
         for _ in range(g["nsteps"]):
             d, _ = P.calc_dt(Ue, g["cfl"], g["dfl"])

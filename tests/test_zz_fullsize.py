@@ -92,3 +92,43 @@ def test_shift_equivariance_full_size():
     err = rel_l2(Rs[nxt], R)
     print(f"full size shift equivariance: relL2 {err:.3e}")
     assert err <= 1e-12
+
+
+def test_c4_sedov_full_size_300_steps():
+    """The shock-capturing configuration at its own size and length: Euler + NFVSE (Persson-
+    Peraire on rho*p, alpha_max 0.5, alpha_min 0.01, one propagation sweep with factor 0.5),
+    Ranocha + LLF, N=3 on the periodic Cartesian 40^3 mesh of [-1.5,1.5]^3, 300 LSERK steps at
+    CFL 0.5 from the Sedov-type pulse (sigma 0.06, E0 1, rho 1, p0 1e-5).  Properties: no
+    CALCTIMESTEP error, density and pressure positive, total mass, momentum and energy
+    conserved to round-off (the blended scheme is conservative element by element), every
+    blending coefficient in [0, alpha_max], blending active in some elements and not in all.
+    The same state and settings on a 12^3 mesh are pinned against the reference's own 100-step
+    run (golden euler_nfvse_c4_sedov_k12_100steps)."""
+    from fluxo_amd.mesh import box_mesh
+    from fluxo_amd.native import Engine
+    from fluxo_amd.workloads import sedov_blast
+    m = box_mesh(40, 3, x0=(-1.5,) * 3, length=(3.0,) * 3)
+    U0 = sedov_blast(m["Elem_xGP"])
+    e = Engine(m, volume_flux=32, riemann=1, nfvse=True, alpha_max=0.5, alpha_min=0.01, sweeps=1, prop_factor=0.5)
+    e.upload(U0)
+    print(f"C4 full size: {m['nElems']} elements, N=3, 300 steps", flush=True)
+    t_end, dt_last, err = e.run(300, 0.5)
+    a = e.blending()
+    U = e.download()
+    e.close()
+    assert err == 0
+    # integral of U: LGL weights (N=3: 1/6, 5/6, 5/6, 1/6) times J = 1/sJ
+    w1 = np.array([1.0, 5.0, 5.0, 1.0]) / 6.0
+    wJ = (w1[None, :, None, None] * w1[None, None, :, None] * w1[None, None, None, :]) / m["sJ"]
+    tot0 = np.einsum("eijk,eijkv->v", wJ, U0)
+    tot1 = np.einsum("eijk,eijkv->v", wJ, U)
+    scale = np.array([tot0[0], tot0[0], tot0[0], tot0[0], tot0[4]])  # momentum against the mass
+    drift = np.abs(tot1 - tot0) / scale
+    p = 0.4 * (U[..., 4] - 0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0])
+    print(f"  t_end {t_end:.4e}, dt_last {dt_last:.3e}; mass {tot0[0]:.6f}, energy {tot0[4]:.6f}; "
+          f"relative drift {drift}; rho_min {U[..., 0].min():.3e}, p_min {p.min():.3e}; "
+          f"alpha > 0 in {(a > 0).sum()} of {a.size} elements, max {a.max():.3f}")
+    assert np.all(drift < 1e-11)
+    assert U[..., 0].min() > 0.0 and p.min() > 0.0
+    assert a.min() >= 0.0 and a.max() <= 0.5
+    assert 0 < (a > 0).sum() < a.size
