@@ -165,6 +165,31 @@ module fluxo_gpu_bind
       real(c_double), intent(out) :: dt
       integer(c_int), intent(out) :: errType
     end function fx_run
+    integer(c_int) function fx_run_to(ctx, maxSteps, cfl, dfl, tEnd, t, dt, steps, errType) bind(C, name="fx_run_to")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: maxSteps
+      real(c_double), value :: cfl, dfl, tEnd
+      real(c_double), intent(inout) :: t
+      real(c_double), intent(out) :: dt
+      integer(c_int), intent(out) :: steps, errType
+    end function fx_run_to
+    integer(c_int) function fx_fillflux(ctx, Flux_master, Flux_slave) bind(C, name="fx_fillflux")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: Flux_master(*), Flux_slave(*)
+    end function fx_fillflux
+    integer(c_int) function fx_surfint(ctx, Flux_master, Flux_slave, Ut) bind(C, name="fx_surfint")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: Flux_master(*), Flux_slave(*)
+      real(c_double), intent(inout) :: Ut(*)
+    end function fx_surfint
+    integer(c_int) function fx_volint_nfvse(ctx, Ut) bind(C, name="fx_volint_nfvse")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: Ut(*)
+    end function fx_volint_nfvse
     integer(c_int) function fx_entropy(ctx, S, dSdU_Ut) bind(C, name="fx_entropy")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

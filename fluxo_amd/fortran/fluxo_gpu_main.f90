@@ -2,9 +2,17 @@
 !> fluxo_gpu: parameter.ini-driven time loop on the MI355X engine (the reference's PROGRAM Fluxo,
 !> src/fluxo.f90, restricted to periodic box meshes, with TimeDisc timedisc.f90:144-415).
 !>
-!> Keys (reference meaning unless noted): N, Riemann, VolumeFlux, IniExactFunc (1, 2, 7),
-!> RefState, AdvVel, IniAmplitude, kappa, mu0, Pr, CFLscale, DFLscale, tEnd, maxIter,
-!> Analyze_dt.
+!> Keys (reference meaning unless noted): N, Riemann, VolumeFlux, IniExactFunc (1, 2, 7; GLM-MHD
+!> 1, 312), RefState, AdvVel, IniAmplitude, kappa, mu0, Pr, CFLscale, DFLscale, TEnd (required),
+!> maxIter (default -1: no limit), Analyze_dt.  The run stops at TEnd as the reference's TimeDisc
+!> does (timedisc.f90:313-317): the step that reaches TEnd is shortened to end there.
+!> EquationSystem (this driver's run-time form of the reference's compile-time FLUXO_EQNSYSNAME):
+!> navierstokes (default) or mhd; GLM-MHD keys of mhd/equation.f90:174-208: mu_0, GLM_scale, eta,
+!> mu (viscosity), Pr, kappa (default 5/3); GLM_scr other than 0 is rejected (not built).
+!> ShockCapturing (logical, the run-time form of FLUXO_SHOCKCAPTURE + FLUXO_SHOCKCAP_NFVSE) with
+!> the keys of InitNFVSE (nfvse.f90:63-146): ComputeAlpha, ShockBlendCoef, alpha_max, alpha_min,
+!> SpacePropFactor, SpacePropSweeps; SubFVMethod (1), ModalThreshold (1), TimeRelFactor (0) and
+!> ShockIndicatorQuantity (DensityTimesPressure) accept only the built values.
 !> Mesh keys (this driver's periodic box generator replaces MeshFile): BoxElems (3 ints),
 !> BoxCorner, BoxLength (3 reals each), Ngeo, meshdeform (logical), meshdeformAmp.
 !> Parabolic (logical) selects Navier-Stokes at run time, with BR1 lifting or, with Lifting = 2
@@ -13,9 +21,9 @@
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
 !> MeshFile (optional): a HOPR mesh file (conforming, periodic boundaries; libfluxo_io) instead
 !> of the box generator.  RestartFile (optional): initial state from a state file (Restart,
-!> restart.f90:147-232; same N) instead of IniExactFunc, its Time as the start time.
-!> ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
-!> <ProjectName>_State_<time>.h5.  StateFile (optional): the final U(5,0:N,0:N,0:N,nElems) as
+!> restart.f90:147-232; same N and number of variables) instead of IniExactFunc, its Time as the
+!> start time.  ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
+!> <ProjectName>_State_<time>.h5.  StateFile (optional): the final U(nVar,0:N,0:N,0:N,nElems) as
 !> raw FP64 in Fortran order.
 !==================================================================================================
 program fluxo_gpu
@@ -29,14 +37,15 @@ program fluxo_gpu
   implicit none
   character(len=512) :: fname
   character(len=256) :: sfile, meshFile, project
-  character(len=32) :: tstamp
-  character(len=40), target :: vnames(5)
-  type(c_ptr) :: vptr(5)
-  integer :: u_out, nodeGL, rN, rVar, rE, iv
+  character(len=32) :: tstamp, eqsys, str
+  character(len=40), target :: vnames(9)
+  type(c_ptr) :: vptr(9)
+  integer :: u_out, nodeGL, rN, rVar, rE, iv, nvar
   type(fx_config) :: c
-  integer :: KB(3), Ngeo, E, S, ierr, errType, iter, maxIter, nChunk, n
-  real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(5), L2c(5), Linf(5)
-  real(rk) :: xg(0:15), wg(0:15), t0, t1, rate, A(5), b(5), RKc(5)
+  integer :: KB(3), Ngeo, E, S, ierr, errType, iter, maxIter, n
+  real(rk) :: x0(3), L(3), amp, t, tEnd, Analyze_dt, tAnalyze, Ssum, dSdU, L2(9), L2c(9), Linf(9)
+  real(rk) :: xg(0:15), wg(0:15), t0, t1, rate, A(5), b(5), RKc(5), prim(8)
+  logical :: finished, lastStep
   integer(c_int), allocatable :: e2s(:, :, :), s2e(:, :)
   real(rk), allocatable :: NC(:, :, :, :, :), Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
   real(rk), allocatable :: sJm(:, :, :, :), nv(:, :, :, :), t1v(:, :, :, :), t2v(:, :, :, :), se(:, :, :)
@@ -52,26 +61,71 @@ program fluxo_gpu
   call read_ini(trim(fname))
 
   c%N = GETINT('N')
+  eqsys = GETSTR('EquationSystem', 'navierstokes')
+  select case (trim(eqsys))
+  case ('navierstokes')
+    c%nVar = 5
+  case ('mhd')
+    c%nVar = 9
+  case default
+    write(*, '(A)') ' ERROR: EquationSystem must be navierstokes or mhd, not '//trim(eqsys)
+    stop 2
+  end select
+  nvar = c%nVar
   c%riemann = GETINT('Riemann', '1')
-  c%volumeFlux = GETINT('VolumeFlux', '0')
+  c%volumeFlux = GETINT('VolumeFlux', merge('10', '0 ', nvar == 9))
   c%parabolic = merge(1, 0, GETLOGICAL('Parabolic', 'F'))
-  c%kappa = GETREAL('kappa', '1.4')
-  c%mu0 = GETREAL('mu0', '0.')
   c%Pr = GETREAL('Pr', '0.72')
   c%lifting = GETINT('Lifting', '1')
   c%etaBR2 = GETREAL('etaBR2', '2.')
+  if (nvar == 9) then  ! InitEquation of mhd/equation.f90:174-208
+    c%kappa = GETREAL('kappa', '1.6666666666666667')
+    c%mu_0 = GETREAL('mu_0', '1.')
+    c%GLM_scale = GETREAL('GLM_scale', '0.5')
+    c%eta = GETREAL('eta', '0.')
+    c%mu0 = GETREAL('mu', '0.')
+    if (GETREAL('GLM_scr', '0.') /= 0.0_rk) then
+      write(*, '(A)') ' ERROR: GLM_scr (GLM damping source) is not built; only GLM_scr = 0'
+      stop 2
+    end if
+  else
+    c%kappa = GETREAL('kappa', '1.4')
+    c%mu0 = GETREAL('mu0', '0.')
+  end if
+  if (GETLOGICAL('ShockCapturing', 'F')) then  ! InitNFVSE (nfvse.f90:63-146)
+    c%nfvse = 1
+    c%computeAlpha = GETINT('ComputeAlpha', '1')
+    c%shockBlendCoef = GETREAL('ShockBlendCoef', '0.0')
+    c%alpha_max = GETREAL('alpha_max', '0.5')
+    c%alpha_min = GETREAL('alpha_min', '0.01')
+    c%spacePropFactor = GETREAL('SpacePropFactor', '0.5')
+    c%spacePropSweeps = GETINT('SpacePropSweeps', '1')
+    if (GETINT('SubFVMethod', '1') /= 1 .or. GETINT('ModalThreshold', '1') /= 1 .or. &
+        GETREAL('TimeRelFactor', '0.0') /= 0.0_rk) then
+      write(*, '(A)') ' ERROR: shock capturing is built for SubFVMethod=1, ModalThreshold=1, TimeRelFactor=0'
+      stop 2
+    end if
+    str = GETSTR('ShockIndicatorQuantity', 'DensityTimesPressure')
+    if (trim(str) /= 'DensityTimesPressure') then
+      write(*, '(A)') ' ERROR: ShockIndicatorQuantity '//trim(str)//' not built (DensityTimesPressure)'
+      stop 2
+    end if
+  end if
+  Kappa = c%kappa
+  mu_0 = c%mu_0
+  PP_nVar = nvar
   IniExactFunc = GETINT('IniExactFunc')
   AdvVel = GETREALARRAY('AdvVel', 3, '1.,1.,1.')
   IniAmplitude = GETREAL('IniAmplitude', '0.1')
-  if (has_key('RefState')) then  ! primitive (rho, v, p) -> RefStateCons (equation.f90:236-250)
-    A = GETREALARRAY('RefState', 5)
-    RefStateCons(1) = A(1); RefStateCons(2:4) = A(1)*A(2:4)
-    RefStateCons(5) = A(5)/(c%kappa - 1.0_rk) + 0.5_rk*A(1)*sum(A(2:4)**2)
+  if (has_key('RefState')) then  ! primitive -> RefStateCons (equation.f90:236-250)
+    prim = 0.0_rk
+    prim(1:merge(8, 5, nvar == 9)) = GETREALARRAY('RefState', merge(8, 5, nvar == 9))
+    call SetRefState(prim(1:merge(8, 5, nvar == 9)))
   end if
   CFLScale = GETREAL('CFLscale')
   DFLScale = GETREAL('DFLscale', '0.9')
-  tEnd = GETREAL('tEnd', '1.0e30')
-  maxIter = GETINT('maxIter', '100')
+  tEnd = GETREAL('TEnd')
+  maxIter = GETINT('maxIter', '-1')
   Analyze_dt = GETREAL('Analyze_dt', '1.0e30')
   resident = GETLOGICAL('DeviceResident', 'T')
   n = c%N
@@ -109,7 +163,11 @@ program fluxo_gpu
   if (has_key('RestartFile')) then
     sfile = GETSTR('RestartFile')
     call io_chk(fxio_state_info(trim(sfile)//c_null_char, rN, rVar, rE, t), 'fxio_state_info')
-    call io_chk(fxio_state_read(trim(sfile)//c_null_char, n, 5, E, U), 'fxio_state_read')
+    if (rN /= n .or. rVar /= nvar .or. rE /= E) then
+      write(*, '(A)') ' ERROR: restart file N / nVar / nElems differ from the run (no interpolation)'
+      stop 2
+    end if
+    call io_chk(fxio_state_read(trim(sfile)//c_null_char, n, nvar, E, U), 'fxio_state_read')
     write(*, '(A,A,A,ES14.6)') ' restart from ', trim(sfile), ' at t=', t
   else
     call FillIni()
@@ -117,21 +175,20 @@ program fluxo_gpu
   call StateToDevice()
   write(*, '(A,I0,A,I0,A,I0,A,I0)') ' fluxo_gpu: nElems=', E, ' N=', n, ' VolumeFlux=', c%volumeFlux, &
       ' Riemann=', c%riemann
-  call CalcEntropy(Ssum, dSdU)
-  write(*, '(A,ES22.14,A,ES22.14)') ' t=0 Entropy=', Ssum, ' dSdU*Ut=', dSdU
+  if (nvar == 5) then
+    call CalcEntropy(Ssum, dSdU)
+    write(*, '(A,ES22.14,A,ES22.14)') ' t=0 Entropy=', Ssum, ' dSdU*Ut=', dSdU
+  end if
   tAnalyze = Analyze_dt
   iter = 0
   call system_clock(c0, cr)
   if (resident) then
-    do while (iter < maxIter .and. t < tEnd)
-      nChunk = min(maxIter - iter, 100)  ! tEnd is checked between chunks (the last chunk may pass it)
-      call TimeDiscRun(nChunk, t, errType)
-      iter = iter + nChunk
-      if (errType /= 0) then
-        write(*, '(A,I0)') ' ERROR: CALCTIMESTEP errType=', errType
-        stop 3
-      end if
-    end do
+    ! the TimeDisc loop on the device up to TEnd (or maxIter steps)
+    call TimeDiscRunTo(maxIter, tEnd, t, iter, errType)
+    if (errType /= 0) then
+      write(*, '(A,I0)') ' ERROR: CALCTIMESTEP errType=', errType
+      stop 3
+    end if
   else
     ! host RK loop calling the drop-in RHS once per stage (timedisc.f90:424-480)
     A = [0.0_rk, 567301805773.0_rk/1357537059087.0_rk, 2404267990393.0_rk/2016746695238.0_rk, &
@@ -142,13 +199,16 @@ program fluxo_gpu
     RKc = [0.0_rk, 1432997174477.0_rk/9575080441755.0_rk, 2526269341429.0_rk/6820363962896.0_rk, &
            2006345519317.0_rk/3224310063776.0_rk, 2802321613138.0_rk/2924317926251.0_rk]
     allocate(Uttmp, mold=U)
-    do while (iter < maxIter .and. t < tEnd)
+    finished = .not. (t < tEnd) .or. maxIter == 0   ! timedisc.f90:236
+    do while (.not. finished)
       call StateToDevice()
       dt = CALCTIMESTEP(errType)
       if (errType /= 0) then
         write(*, '(A,I0)') ' ERROR: CALCTIMESTEP errType=', errType
         stop 3
       end if
+      lastStep = tEnd - t <= dt*(1.0_rk + 1.0e-4_rk)   ! timedisc.f90:313-317
+      if (lastStep) dt = tEnd - t
       call DGTimeDerivative_weakForm(t)
       Uttmp = Ut
       U = U + Ut*(b(1)*dt)
@@ -160,6 +220,7 @@ program fluxo_gpu
       n = c%N
       t = t + dt
       iter = iter + 1
+      finished = lastStep .or. iter == maxIter
     end do
     call StateToDevice()
   end if
@@ -168,14 +229,16 @@ program fluxo_gpu
   t0 = 0.0_rk
   rate = real(E, rk)*real((c%N+1)**3, rk)*5.0_rk*real(iter, rk)/max(t1 - t0, 1.0e-30_rk)
   call StateToHost()
-  call CalcEntropy(Ssum, dSdU)
   write(*, '(A,I0,A,ES22.14,A,ES14.6)') ' iter=', iter, ' t=', t, ' dt=', dt
-  write(*, '(A,ES22.14,A,ES22.14)') ' Entropy=', Ssum, ' dSdU*Ut=', dSdU
-  if (IniExactFunc == 1 .or. IniExactFunc == 2) then
-    call CalcErrorNorms(t, L2, L2c, Linf)
-    write(*, '(A14,5ES21.12)') ' L_2        : ', L2
-    write(*, '(A14,5ES21.12)') ' L_2_colloc : ', L2c
-    write(*, '(A14,5ES21.12)') ' L_inf      : ', Linf
+  if (nvar == 5) then
+    call CalcEntropy(Ssum, dSdU)
+    write(*, '(A,ES22.14,A,ES22.14)') ' Entropy=', Ssum, ' dSdU*Ut=', dSdU
+  end if
+  if (IniExactFunc == 1 .or. (IniExactFunc == 2 .and. nvar == 5)) then
+    call CalcErrorNorms(t, L2(1:nvar), L2c(1:nvar), Linf(1:nvar))
+    write(*, '(A14,9ES21.12)') ' L_2        : ', L2(1:nvar)
+    write(*, '(A14,9ES21.12)') ' L_2_colloc : ', L2c(1:nvar)
+    write(*, '(A14,9ES21.12)') ' L_inf      : ', Linf(1:nvar)
   end if
   write(*, '(A,ES14.6,A,ES14.6)') ' DOF-updates/s=', rate, '  PID [s/(DOF*timestep)]=', 5.0_rk/rate
   if (has_key('ProjectName')) then
@@ -186,13 +249,19 @@ program fluxo_gpu
       if (tstamp(iv:iv) == ' ') tstamp(iv:iv) = '0'
     end do
     sfile = trim(project)//'_State_'//trim(tstamp)//'.h5'
-    vnames = [character(len=40) :: 'Density', 'MomentumX', 'MomentumY', 'MomentumZ', 'EnergyStagnationDensity']
-    do iv = 1, 5
+    if (nvar == 9) then  ! StrVarNames of mhd/equation_vars.f90:78-91
+      vnames = [character(len=40) :: 'Density', 'MomentumX', 'MomentumY', 'MomentumZ', 'TotalEnergy', &
+                'MagneticFieldX', 'MagneticFieldY', 'MagneticFieldZ', 'Psi']
+    else
+      vnames(1:5) = [character(len=40) :: 'Density', 'MomentumX', 'MomentumY', 'MomentumZ', &
+                     'EnergyStagnationDensity']
+    end if
+    do iv = 1, nvar
       vnames(iv) = trim(vnames(iv))//c_null_char
       vptr(iv) = c_loc(vnames(iv))
     end do
     call io_chk(fxio_state_write(trim(sfile)//c_null_char, trim(project)//c_null_char, &
-                                 trim(meshFile)//c_null_char, n, 5, E, t, vptr, U), 'fxio_state_write')
+                                 trim(meshFile)//c_null_char, n, nvar, E, t, vptr, U), 'fxio_state_write')
     write(*, '(A,A)') ' WRITE STATE TO HDF5 FILE: ', trim(sfile)
   end if
   if (has_key('StateFile')) then

@@ -28,7 +28,7 @@ def test_fortran_host_matches_oracle(tmp_path, resident, lifting):
     ini.write_text(
         f"N = {N}\nVolumeFlux = 32\nRiemann = 1\nParabolic = T\nLifting = {lifting}\netaBR2 = 2.5\n"
         "mu0 = 0.01\nPr = 0.72\nkappa = 1.4\nIniExactFunc = 7\nCFLscale = 0.9\nDFLscale = 0.9\n"
-        f"maxIter = {STEPS}\nDeviceResident = {'T' if resident else 'F'}\n"
+        f"maxIter = {STEPS}\nTEnd = 1.0e3\nDeviceResident = {'T' if resident else 'F'}\n"
         f"BoxElems = (/{K},{K},{K}/)\nBoxCorner = (/{-math.pi!r},{-math.pi!r},{-math.pi!r}/)\n"
         f"BoxLength = (/{2 * math.pi!r},{2 * math.pi!r},{2 * math.pi!r}/)\n"
         f"Ngeo = 3\nmeshdeform = T\nmeshdeformAmp = 0.1\nStateFile = {sfile}\n")
@@ -61,7 +61,7 @@ def test_fortran_host_error_norms(tmp_path):
     ini = tmp_path / "parameter.ini"
     ini.write_text(
         f"N = {Nd}\nVolumeFlux = 32\nRiemann = 1\nkappa = 1.4\nIniExactFunc = 2\nAdvVel = (/1.,1.,1./)\n"
-        f"IniAmplitude = 0.1\nCFLscale = 0.9\nmaxIter = {steps}\nBoxElems = (/{Kd},{Kd},{Kd}/)\n"
+        f"IniAmplitude = 0.1\nCFLscale = 0.9\nmaxIter = {steps}\nTEnd = 1.0e3\nBoxElems = (/{Kd},{Kd},{Kd}/)\n"
         "BoxCorner = (/0.,0.,0./)\nBoxLength = (/2.,2.,2./)\nNgeo = 3\nmeshdeform = T\nmeshdeformAmp = 0.1\n")
     print("fluxo_gpu density wave error norms", flush=True)
     p = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
@@ -113,8 +113,112 @@ def test_fortran_host_runs_the_reference_free_stream_input(tmp_path):
             linf = np.array([float(x) for x in line.split(":")[1].split()])
     print(f"  L_inf {linf}")
     assert linf is not None and np.all(linf <= 1e-11)
+    # the run ends at tEnd exactly as the reference's does (the last step shortened), and the
+    # state file carries that time in its name
     files = sorted(tmp_path.glob("NAVIERSTOKES_FREESTREAM_State_*.h5"))
-    assert len(files) == 1
+    assert [f.name for f in files] == ["NAVIERSTOKES_FREESTREAM_State_0000020.000000000.h5"]
     U, t = io.read_state(files[0])
-    assert t >= 20.0 and U.shape == (27, 4, 4, 4, 5)
+    assert t == pytest.approx(20.0, rel=1e-12) and U.shape == (27, 4, 4, 4, 5)
     assert np.abs(U - [1.0, 0.1, 0.2, -0.3, 50.07]).max() <= 1e-11
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("resident", [True, False])
+def test_fortran_host_mhd_against_reference(tmp_path, resident):
+    """Resistive GLM-MHD through the driver's parameter.ini (EquationSystem = mhd and the
+    reference's mu_0, GLM_scale, eta, mu, Pr keys): the GLM-MHD benchmark's perturbed
+    Orszag-Tang state (RestartFile written from fluxo_amd.workloads.perturbed_orszag_tang) on the
+    warped 4^3 mesh, N=4, 100 steps at CFL/DFL 0.5, against the reference's own 100-step run
+    (golden mhd_resistive_n4_eta1e-3_100steps): relative L2 <= 1e-10."""
+    from fluxo_amd import io
+    from fluxo_amd.workloads import perturbed_orszag_tang
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    with np.load(os.path.join(GOLD, "mhd", "mhd_resistive_n4_eta1e-3_100steps.npz"), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    K, Nm, steps = int(g["K"]), int(g["N"]), int(g["nsteps"])
+    m = O.box_mesh(K, Nm, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    U0 = perturbed_orszag_tang(m["Elem_xGP"], 2026)
+    assert np.array_equal(U0, g["U"])
+    io.write_state(tmp_path / "ot_start.h5", U0, 0.0)
+    sfile = tmp_path / "state.bin"
+    (tmp_path / "parameter.ini").write_text(
+        f"EquationSystem = mhd\nN = {Nm}\nVolumeFlux = 10\nRiemann = 1\nParabolic = T\nmu_0 = 1.\n"
+        f"GLM_scale = 0.5\neta = {float(g['eta'])!r}\nmu = {float(g['mu'])!r}\nPr = {float(g['Pr'])!r}\n"
+        "kappa = 1.6666666666666667\nIniExactFunc = 312\nRestartFile = ot_start.h5\n"
+        f"CFLscale = 0.5\nDFLscale = 0.5\nmaxIter = {steps}\nTEnd = 1.0e3\n"
+        f"DeviceResident = {'T' if resident else 'F'}\nBoxElems = (/{K},{K},{K}/)\nBoxCorner = (/0.,0.,0./)\n"
+        f"BoxLength = (/1.,1.,1./)\nNgeo = 3\nmeshdeform = T\nmeshdeformAmp = 0.1\nStateFile = {sfile}\n")
+    print(f"fluxo_gpu GLM-MHD resident={resident}", flush=True)
+    p = subprocess.run([EXE, "parameter.ini"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    n = Nm + 1
+    U = np.fromfile(sfile, np.float64).reshape(K ** 3, n, n, n, 9)
+    err = float(np.linalg.norm(U - g["U_end"]) / np.linalg.norm(g["U_end"]))
+    print(f"  {steps} steps relL2 vs the reference {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_fortran_host_mhd_orszag_tang_exactfunc(tmp_path):
+    """IniExactFunc 312 (the reference's 3D Orszag-Tang vortex, mhd/equation.f90:1257-1266) in
+    the driver's own ExactFunc: 3 steps of ideal GLM-MHD (Parabolic = F) against the oracle from
+    the oracle's ExactFunc 312 state."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    K, Nm, steps = 3, 3, 3
+    sfile = tmp_path / "state.bin"
+    (tmp_path / "parameter.ini").write_text(
+        f"EquationSystem = mhd\nN = {Nm}\nRiemann = 10\nIniExactFunc = 312\nCFLscale = 0.5\n"
+        f"maxIter = {steps}\nTEnd = 1.0e3\nBoxElems = (/{K},{K},{K}/)\nBoxCorner = (/0.,0.,0./)\n"
+        f"BoxLength = (/1.,1.,1./)\nNgeo = 3\nmeshdeform = T\nStateFile = {sfile}\n")
+    p = subprocess.run([EXE, "parameter.ini"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    m = O.box_mesh(K, Nm, x0=(0, 0, 0), length=(1, 1, 1), Ngeo=3, warp=True)
+    P = O.MhdProblem(m, riemann=10)
+    Ue = P.exact_orszag_tang()
+    Utmp = np.zeros_like(Ue)
+    for _ in range(steps):
+        d, _ = P.calc_dt(Ue, 0.5, 0.0)
+        P.lserk_step(Ue, Utmp, d)
+    n = Nm + 1
+    U = np.fromfile(sfile, np.float64).reshape(K ** 3, n, n, n, 9)
+    err = float(np.linalg.norm(U - Ue) / np.linalg.norm(Ue))
+    print(f"  IniExactFunc 312, {steps} steps relL2 vs oracle {err:.3e}")
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("resident", [True, False])
+def test_fortran_host_shock_capturing_against_reference(tmp_path, resident):
+    """Euler with NFVSE shock capturing through the driver's parameter.ini (ShockCapturing = T
+    and the reference's InitNFVSE keys): the shock-capturing benchmark's own Sedov-type state
+    (RestartFile written from fluxo_amd.workloads.sedov_blast) on the Cartesian 12^3 mesh of
+    [-1.5,1.5]^3, N=3, 100 steps at CFL 0.5, against the reference's own 100-step run (golden
+    euler_nfvse_c4_sedov_k12_100steps): relative L2 <= 1e-10."""
+    from fluxo_amd import io
+    from fluxo_amd.workloads import sedov_blast
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    with np.load(os.path.join(GOLD, "euler_nfvse_c4_sedov_k12_100steps.npz"), allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    K, Nn, steps = int(g["K"]), int(g["N"]), int(g["nsteps"])
+    m = O.box_mesh(K, Nn, x0=(-1.5,) * 3, length=(3.0,) * 3)
+    io.write_state(tmp_path / "sedov_start.h5", sedov_blast(m["Elem_xGP"]), 0.0)
+    sfile = tmp_path / "state.bin"
+    (tmp_path / "parameter.ini").write_text(
+        f"N = {Nn}\nVolumeFlux = 32\nRiemann = 1\nkappa = 1.4\nShockCapturing = T\nComputeAlpha = 1\n"
+        "alpha_max = 0.5\nalpha_min = 0.01\nSpacePropSweeps = 1\nSpacePropFactor = 0.5\nSubFVMethod = 1\n"
+        "ShockIndicatorQuantity = DensityTimesPressure\nIniExactFunc = 1\nRestartFile = sedov_start.h5\n"
+        f"CFLscale = 0.5\nmaxIter = {steps}\nTEnd = 1.0e3\nDeviceResident = {'T' if resident else 'F'}\n"
+        f"BoxElems = (/{K},{K},{K}/)\nBoxCorner = (/-1.5,-1.5,-1.5/)\nBoxLength = (/3.,3.,3./)\nNgeo = 1\n"
+        f"StateFile = {sfile}\n")
+    print(f"fluxo_gpu Sedov NFVSE resident={resident}", flush=True)
+    p = subprocess.run([EXE, "parameter.ini"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    n = Nn + 1
+    U = np.fromfile(sfile, np.float64).reshape(K ** 3, n, n, n, 5)
+    err = float(np.linalg.norm(U - g["U_end"]) / np.linalg.norm(g["U_end"]))
+    print(f"  {steps} steps relL2 vs the reference {err:.3e}")
+    assert err <= 1e-10
+    assert f"iter={steps}" in p.stdout.replace(" ", "")
