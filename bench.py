@@ -284,7 +284,8 @@ def serving_leg(n_req, nsteps, seed, load):
             "p50_ms": float(np.percentile(live["lat"], 50) * 1e3),
             "p99_ms": float(np.percentile(live["lat"], 99) * 1e3),
             "stream_DOFps": live["dofs"] / live["makespan"],
-            "graph_captures": live["stats"]["graph_captures"],
+            "graph_instantiations": live["stats"]["graph_captures"],
+            "graph_updates": live["stats"]["graph_updates"],
             "peak_request_bytes": live["stats"]["bytes_peak"],
         }
     svc = run_plain(reqs, FX_PREC_FP64)

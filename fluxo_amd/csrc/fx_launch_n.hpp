@@ -187,15 +187,17 @@ void launch_dt_n(const Batch &b, double cfl, double dfl)
 template <int N, int MODE, bool PARA>
 void launch_mhd_volume_p(const Batch &b, RKArgs rk)
 {
-  constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W;
+  constexpr int EPB = MhdVolGeom<N>::EPB, T = MhdVolGeom<N>::T;
   const int nb = (b.cfg.nElems + EPB - 1) / EPB;
   if (nb <= 0) return;
-  if (b.cfg.precision == FX_PREC_FP32_SPLIT)
-    hipLaunchKernelGGL((k_mhd_volume<N, MODE, float, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf32,
-                       b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
-                       b.mhd_args(), b.st);
-  else
-    hipLaunchKernelGGL((k_mhd_volume<N, MODE, double, PARA>), dim3(nb), dim3(3 * W), 0, b.stream, b.U, b.Mf, b.Mg,
+  if (b.cfg.precision == FX_PREC_FP32_SPLIT) {  // the line form (faster in FP32)
+    constexpr int EPBL = MhdLineGeom<N>::EPB, WL = MhdLineGeom<N>::W;
+    const int nbl = (b.cfg.nElems + EPBL - 1) / EPBL;
+    hipLaunchKernelGGL((k_mhd_volume_line<N, MODE, float, PARA>), dim3(nbl), dim3(3 * WL), 0, b.stream, b.U,
+                       b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.FluxS, b.Uv, b.Ut, b.Uttmp, rk,
+                       b.cfg.nElems, b.mhd_args(), b.st);
+  } else
+    hipLaunchKernelGGL((k_mhd_volume<N, MODE, double, PARA>), dim3(nb), dim3(T), 0, b.stream, b.U, b.Mf, b.Mg,
                        b.Mh, b.sJ, b.e2s, b.Flux, b.FluxS, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems, b.mhd_args(),
                        b.st);
 }

@@ -332,21 +332,217 @@ k_mhd_surface(const double *__restrict__ U, const double *__restrict__ FvN, cons
 }
 
 // Element-centric GLM-MHD volume kernel (same phases as k_volume).
+// Launch geometry of the GLM-MHD volume kernel: EPB elements per block, one thread per
+// (element, direction, node), T threads rounded up to whole wavefronts.
 template <int N> struct MhdGeom;
-template <> struct MhdGeom<2> { static constexpr int EPB = 7, W = 64; };
-template <> struct MhdGeom<3> { static constexpr int EPB = 4, W = 64; };
-template <> struct MhdGeom<4> { static constexpr int EPB = 2, W = 64; };
-template <> struct MhdGeom<5> { static constexpr int EPB = 1, W = 64; };
+template <> struct MhdGeom<2> { static constexpr int EPB = 3; };
+template <> struct MhdGeom<3> { static constexpr int EPB = 2; };
+template <> struct MhdGeom<4> { static constexpr int EPB = 1; };
+template <> struct MhdGeom<5> { static constexpr int EPB = 1; };
+template <int N> struct MhdVolGeom {
+  static constexpr int n3 = (N + 1) * (N + 1) * (N + 1), EPB = MhdGeom<N>::EPB;
+  static constexpr int T = ((EPB * 3 * n3 + 63) / 64) * 64;
+};
 
+// Element-centric GLM-MHD volume kernel (VolInt_SplitForm with EvalAdvFluxAverage3D and
+// AddNonConsFluxTilde3D, mhd/flux_average.f90:112-186, 268-366; VolInt_visc; SurfInt; -sJ; the
+// LSERK stage update).  One thread per (element, direction d, node k): it evaluates the
+// two-point Derigs flux of its node with each other node l of its line in direction d, the
+// lower line index always the left state (so the flux of a pair is the same value in both of
+// its nodes' threads), and accumulates sum_l Dvolsurf(k,l) F(k,l) in increasing l and the
+// Powell line sum; each direction's sums go to LDS and the node pass adds xi + eta + zeta in
+// that order.  Every pair's flux is thus evaluated twice, which keeps a thread's registers to
+// one node's sums (the line-per-thread form held n nodes' sums and ran one wave per SIMD).
+//   phase 1  one thread per (element, direction, node): the node fields (direction 0) and the
+//            node's metric of its direction into LDS
+//   phase 2  the flux differencing as above, sums into LDS over the fields (after a barrier)
+//   phase 3  face gather +FluxM L0 / -FluxS L0 into LDS, then one thread per node: direction
+//            sums, viscous volume term, surface terms, -sJ, Ut or the RK stage update
 template <int N, int MODE, typename VR, bool PARA>
-__global__ void __launch_bounds__(3 * MhdGeom<N>::W)
+__global__ void __launch_bounds__(MhdVolGeom<N>::T)
 k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
              const double *__restrict__ sJ, const int *__restrict__ e2s, const double *__restrict__ FluxM,
              const double *__restrict__ FluxS, const double *__restrict__ Uv, double *__restrict__ Ut,
              double *__restrict__ Uttmp, RKArgs rk, int nElems, MhdArgs a, const DtState *__restrict__ st)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-  constexpr int EPB = MhdGeom<N>::EPB, W = MhdGeom<N>::W, NP = Lds<N>::NP;
+  constexpr int EPB = MhdVolGeom<N>::EPB, T = MhdVolGeom<N>::T, NP = Lds<N>::NP;
+  // per element: the node fields and the three metric vectors (phases 1-2), later the three
+  // direction sums (MV values per node each)
+  constexpr int FSLOTS = MF_COUNT + 9, ASLOTS = 3 * MV, SLOTS = FSLOTS > ASLOTS ? FSLOTS : ASLOTS;
+  __shared__ VR s_buf[EPB][SLOTS * NP];
+#define MF_AT(el_, c_, l_) s_buf[el_][(c_) * NP + (l_)]
+#define MM_AT(el_, d_, c_, l_) s_buf[el_][(MF_COUNT + 3 * (d_) + (c_)) * NP + (l_)]
+#define MA_AT(el_, d_, v_, l_) s_buf[el_][(MV * (d_) + (v_)) * NP + (l_)]
+  const Ops &op = c_ops[N];
+  const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
+  const VR ch = VR(st->glm_ch), smu0 = VR(a.smu0), skm1 = VR(1.0 / (a.kappa - 1.0));
+
+  const int el = tid / (3 * n3), rr = tid - el * (3 * n3), dir = rr / n3, node = rr - dir * n3;
+  const int e = e0 + el;
+  const bool active = (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
+  const int i = node % n, j = (node / n) % n, k = node / n2;
+  const int ln = Lds<N>::li(node);
+  if (active) {
+    if (dir == 0) {
+      const double *u = U + ((size_t)e * n3 + node) * MV;
+      double uu[MV], f[MF_COUNT];
+#pragma unroll
+      for (int v = 0; v < MV; v++) uu[v] = u[v];
+      mhd_node_fields(uu, a, f);
+#pragma unroll
+      for (int c = 0; c < MF_COUNT; c++) MF_AT(el, c, ln) = VR(f[c]);
+    }
+    const VR *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
+    const VR *mp = M + ((size_t)e * n3 + node) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; c++) MM_AT(el, dir, c, ln) = mp[c];
+  }
+  __syncthreads();
+
+  VR acc[MV];
+#pragma unroll
+  for (int v = 0; v < MV; v++) acc[v] = VR(0);
+  if (active) {
+    const int pos = (dir == 0) ? i : ((dir == 1) ? j : k);  // the node's index along its line
+    const int stride = (dir == 0) ? 1 : ((dir == 1) ? (n + 1) : (n + 1) * n);  // LDS step along the line
+    const int l0 = ln - pos * stride;                        // LDS index of the line's node 0
+    VR fk[MF_COUNT], mk[3];
+#pragma unroll
+    for (int c = 0; c < MF_COUNT; c++) fk[c] = MF_AT(el, c, ln);
+#pragma unroll
+    for (int c = 0; c < 3; c++) mk[c] = MM_AT(el, dir, c, ln);
+    VR snc = VR(0);
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+      if (l == pos) continue;
+      const int ll = l0 + l * stride;
+      VR fl[MF_COUNT], ml[3], F[MV];
+#pragma unroll
+      for (int c = 0; c < MF_COUNT; c++) fl[c] = MF_AT(el, c, ll);
+#pragma unroll
+      for (int c = 0; c < 3; c++) ml[c] = MM_AT(el, dir, c, ll);
+      VR mb0, mb1, mb2;
+      if (pos < l) {
+        derigs_flux<VR>(fk, fl, mk, ml, ch, smu0, skm1, F);
+        mb0 = VR(0.5) * (mk[0] + ml[0]); mb1 = VR(0.5) * (mk[1] + ml[1]); mb2 = VR(0.5) * (mk[2] + ml[2]);
+      } else {
+        derigs_flux<VR>(fl, fk, ml, mk, ch, smu0, skm1, F);
+        mb0 = VR(0.5) * (ml[0] + mk[0]); mb1 = VR(0.5) * (ml[1] + mk[1]); mb2 = VR(0.5) * (ml[2] + mk[2]);
+      }
+      const VR dkl = OpsOf<VR>::ds(op, pos, l);
+#pragma unroll
+      for (int v = 0; v < MV; v++) acc[v] += dkl * F[v];
+      // Powell: B_l.{{m}} towards node k
+      snc += dkl * (mb0 * fl[MF_B1] + mb1 * fl[MF_B2] + mb2 * fl[MF_B3]);
+    }
+    const VR b1 = fk[MF_B1], b2 = fk[MF_B2], b3 = fk[MF_B3];
+    const VR bhat = mk[0] * b1 + mk[1] * b2 + mk[2] * b3;
+    const VR t = VR(0.5) * (bhat * VR(op.DsRow[pos]) + snc);
+    acc[1] += t * b1; acc[2] += t * b2; acc[3] += t * b3;
+    acc[4] += t * fk[MF_VB];
+    acc[5] += t * fk[MF_V1];
+    acc[6] += t * fk[MF_V2];
+    acc[7] += t * fk[MF_V3];
+  }
+  __syncthreads();  // the sums overwrite the fields: every thread must have read them
+  if (active) {
+#pragma unroll
+    for (int v = 0; v < MV; v++) MA_AT(el, dir, v, ln) = acc[v];
+  }
+
+  // the element's SurfInt terms +FluxM L0 (master) / -FluxS L0 (slave) into LDS in
+  // element-local face coordinates, one work item per face point
+  const double L0 = op.L0;
+  constexpr int NF = 6 * n2;
+  __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * MV : 1];
+  if (MODE != VMODE_VOLINT) {
+    for (int it = tid; it < EPB * NF; it += T) {
+      const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
+      const int ee = e0 + fe;
+      if (ee >= nElems || elem_problem(rk.eprob, ee) < 0) continue;
+      const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+      const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+      int p, q;
+      vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
+      const size_t f = ((size_t)(sd * n + q) * n + p) * MV;
+      const double *F = (fl == 0) ? FluxM + f : FluxS + f;
+      const double sg = (fl == 0) ? L0 : -L0;
+#pragma unroll
+      for (int v = 0; v < MV; v++) s_face[fe][r * MV + v] = F[v] * sg;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < EPB * n3; idx += T) {
+    const int el2 = idx / n3, node2 = idx - el2 * n3, e2 = e0 + el2;
+    if (e2 >= nElems) continue;
+    const int prob = elem_problem(rk.eprob, e2);
+    if (prob < 0) continue;
+    const int i2 = node2 % n, j2 = (node2 / n) % n, k2 = node2 / n2, l2 = Lds<N>::li(node2);
+    double tot[MV];
+#pragma unroll
+    for (int v = 0; v < MV; v++) {
+      VR sd = MA_AT(el2, 0, v, l2);
+      sd = sd + MA_AT(el2, 1, v, l2);
+      sd = sd + MA_AT(el2, 2, v, l2);
+      tot[v] = double(sd);
+    }
+    if (PARA && MODE != VMODE_VOLINT) {  // VolInt_visc (volint.f90:228-271), formed by the lifting kernel
+      const double *uv = Uv + ((size_t)e2 * n3 + node2) * 7;
+#pragma unroll
+      for (int v = 1; v < 8; v++) tot[v] = tot[v] + uv[v - 1];
+    }
+    if (MODE != VMODE_VOLINT) {
+      const NodeFaces nf = node_faces<N>(i2, j2, k2);
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const int r = nf.get(t);
+        if (r < 0) continue;
+#pragma unroll
+        for (int v = 0; v < MV; v++) tot[v] = tot[v] + s_face[el2][r * MV + v];
+      }
+      const double msj = -sJ[(size_t)e2 * n3 + node2];
+#pragma unroll
+      for (int v = 0; v < MV; v++) tot[v] = tot[v] * msj;
+    }
+    const size_t o = ((size_t)e2 * n3 + node2) * MV;
+    if (MODE == VMODE_RK) {
+      const double bdt = rk.b * rk.st[prob].dt;
+#pragma unroll
+      for (int v = 0; v < MV; v++) {
+        const double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
+        Uttmp[o + v] = ut;
+        U[o + v] = U[o + v] + ut * bdt;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < MV; v++) Ut[o + v] = tot[v];
+    }
+  }
+#undef MF_AT
+#undef MM_AT
+#undef MA_AT
+}
+
+// The line form of the GLM-MHD volume kernel, used for the split-precision path: one thread
+// per (direction, element, line) evaluates each pair's flux once and keeps the n nodes' sums
+// in registers (in FP32 these fit at two waves per SIMD, and evaluating every pair once is
+// faster there than the node form above).  Same phases and sums otherwise.
+template <int N> struct MhdLineGeom;
+template <> struct MhdLineGeom<2> { static constexpr int EPB = 7, W = 64; };
+template <> struct MhdLineGeom<3> { static constexpr int EPB = 4, W = 64; };
+template <> struct MhdLineGeom<4> { static constexpr int EPB = 2, W = 64; };
+template <> struct MhdLineGeom<5> { static constexpr int EPB = 1, W = 64; };
+
+template <int N, int MODE, typename VR, bool PARA>
+__global__ void __launch_bounds__(3 * MhdLineGeom<N>::W)
+k_mhd_volume_line(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
+             const double *__restrict__ sJ, const int *__restrict__ e2s, const double *__restrict__ FluxM,
+             const double *__restrict__ FluxS, const double *__restrict__ Uv, double *__restrict__ Ut,
+             double *__restrict__ Uttmp, RKArgs rk, int nElems, MhdArgs a, const DtState *__restrict__ st)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  constexpr int EPB = MhdLineGeom<N>::EPB, W = MhdLineGeom<N>::W, NP = Lds<N>::NP;
   static_assert(EPB * n2 <= W, "MHD volume launch geometry");
   __shared__ VR s_f[EPB][MF_COUNT * NP];
   __shared__ VR s_acc[EPB][MV * NP];

@@ -115,7 +115,7 @@ struct fx_server {
   bool stop = false;
   std::thread worker;
   fx::srv::Pool pools[2 * (fx::MAXN + 1)];
-  long long bytes_in_use = 0, bytes_peak = 0, steps_launched = 0, graph_captures = 0;
+  long long bytes_in_use = 0, bytes_peak = 0, steps_launched = 0, graph_captures = 0, graph_updates = 0;
   long long bytes_allocated = 0;   // sum of the pools' device arrays
   int nActive = 0, nDone = 0, nFailed = 0;
   char worker_err[512] = "";
@@ -333,12 +333,15 @@ inline int admit(fx_server *s, Request &q)
   return FX_OK;
 }
 
-// Step graph of a pool over its current extents: CALCTIMESTEP (per problem) + 5 stages.
+// Step graph of a pool over its current extents: CALCTIMESTEP (per problem) + 5 stages.  When
+// the extents or arrays change, the step is recorded again and the pool's executable graph is
+// updated in place (same kernels in the same order, only grid sizes and arguments differ);
+// instantiation, the costly part, happens only for a pool's first graph or if an update is
+// refused.
 inline int pool_graph(fx_server *s, Pool &p)
 {
   const int E = p.ea.high_water(), S = p.sa.high_water(), R = p.ra.high_water();
   if (p.graph && !p.ptrs_changed && E == p.gE && S == p.gS && R == p.gR) return FX_OK;
-  if (p.graph) { (void)hipGraphExecDestroy(p.graph); p.graph = nullptr; }
   Batch &b = p.b;
   b.cfg.nElems = E; b.cfg.nSides = S; b.nProb = R;
   hipGraph_t g;
@@ -346,12 +349,27 @@ inline int pool_graph(fx_server *s, Pool &p)
   q_dt(b, cfl_number(p.N, s->cfg.CFLscale), p.parabolic ? dfl_number(p.N, s->cfg.DFLscale) : 0.0, 1);
   q_rk_step(b);
   HIPCHK(hipStreamEndCapture(b.stream, &g));
-  hipError_t e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
+  hipError_t e = hipSuccess;
+  bool updated = false;
+  if (p.graph) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    e = hipGraphExecUpdate(p.graph, g, &bad, &res);
+    if (e == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      updated = true;
+    } else {
+      (void)hipGetLastError();  // a refused update leaves no error behind
+      (void)hipGraphExecDestroy(p.graph);
+      p.graph = nullptr;
+    }
+  }
+  if (!updated) e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) { p.graph = nullptr; return set_err(FX_EHIP, "graph instantiate: %s", hipGetErrorString(e)); }
   p.gE = E; p.gS = S; p.gR = R;
   p.ptrs_changed = false;
-  s->graph_captures++;
+  if (updated) s->graph_updates++;
+  else s->graph_captures++;
   return FX_OK;
 }
 
@@ -599,6 +617,7 @@ int fx_server_get_stats(fx_server *s, fx_server_stats *o)
   std::lock_guard<std::mutex> lk(s->mu);
   o->steps_launched = s->steps_launched;
   o->graph_captures = s->graph_captures;
+  o->graph_updates = s->graph_updates;
   o->bytes_in_use = s->bytes_in_use;
   o->bytes_peak = s->bytes_peak;
   o->bytes_allocated = s->bytes_allocated;

@@ -67,7 +67,7 @@ class fx_server_stats(C.Structure):
     _fields_ = [("steps_launched", C.c_longlong), ("graph_captures", C.c_longlong),
                 ("bytes_in_use", C.c_longlong), ("bytes_peak", C.c_longlong),
                 ("bytes_allocated", C.c_longlong), ("active", C.c_int), ("queued", C.c_int),
-                ("done", C.c_int), ("failed", C.c_int)]
+                ("done", C.c_int), ("failed", C.c_int), ("graph_updates", C.c_longlong)]
 
 
 class fx_analyze_config(C.Structure):

@@ -328,11 +328,12 @@ typedef struct {
 
 typedef struct {
   long long steps_launched;   /* pool step graphs launched */
-  long long graph_captures;   /* step graphs (re)captured after pool extent changes */
+  long long graph_captures;   /* step graphs instantiated (a pool's first, or an update refused) */
   long long bytes_in_use;     /* fx_request_bytes of the admitted requests */
   long long bytes_peak;
   long long bytes_allocated;  /* device memory held by the pools */
   int active, queued, done, failed;
+  long long graph_updates;    /* executable step graphs updated in place after pool extent changes */
 } fx_server_stats;
 
 int fx_server_create(const fx_server_config *cfg, fx_server **out);
