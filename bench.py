@@ -178,14 +178,16 @@ def run_server(reqs, precision, arrivals=None):
     from fluxo_amd.native import Server, FX_REQ_DONE
     srv = Server(precision=precision, mu0=1.0 / 1600, Pr=0.72, cfl=SERVE_CFL, dfl=SERVE_CFL,
                  hbm_budget=96 << 30, max_requests=len(reqs) + 1)
-    ids = []
+    ids, outs = [], []
     t0 = time.perf_counter()
     for i, r in enumerate(reqs):
         if arrivals is not None:
             wait = arrivals[i] - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
-        ids.append(srv.submit(r["mesh"], r["U0"], r["nsteps"], parabolic=r["parabolic"])[0])
+        rid, out = srv.submit(r["mesh"], r["U0"], r["nsteps"], parabolic=r["parabolic"])
+        ids.append(rid)
+        outs.append(out)
     srv.wait(-1, timeout=1200)
     st = [srv.status(i) for i in ids]
     stats = srv.stats()
@@ -202,7 +204,7 @@ def run_server(reqs, precision, arrivals=None):
     makespan = max(s["t_done"] for s in st) - start
     lat = np.array([s["t_done"] - s["t_submit"] for s in st])
     dofs = sum(r["nodes"] * 5 * r["nsteps"] for r in reqs)
-    return dict(makespan=makespan, lat=lat, dofs=dofs, stats=stats)
+    return dict(makespan=makespan, lat=lat, dofs=dofs, stats=stats, results=outs)
 
 
 def run_plain(reqs, precision):
@@ -270,8 +272,10 @@ def serving_leg(n_req, nsteps, seed, load):
            "latency": "submission to result in host memory", "volume_tiles": tiles}
     rng = np.random.default_rng(seed + 1)
     arrivals = None
+    results = {}
     for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
         off = run_server(reqs, prec)
+        results[name] = off.pop("results")
         if arrivals is None:
             rate = load * n_req / off["makespan"]
             arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
@@ -288,6 +292,11 @@ def serving_leg(n_req, nsteps, seed, load):
             "graph_updates": live["stats"]["graph_updates"],
             "peak_request_bytes": live["stats"]["bytes_peak"],
         }
+    # the split-precision results against the FP64 ones, request by request (final states)
+    l2 = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(results["fp32_split"], results["fp64"])])
+    linf = np.array([np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(results["fp32_split"], results["fp64"])])
+    out["fp32_split"].update(relL2_vs_fp64_max=float(l2.max()), relL2_vs_fp64_median=float(np.median(l2)),
+                             relLinf_vs_fp64_max=float(linf.max()))
     svc = run_plain(reqs, FX_PREC_FP64)
     lat = fifo_latency(arrivals, svc)
     out["plain_fp64"] = {
@@ -326,33 +335,44 @@ def sedov_leg(K=40, N=3, steps=300, cfl=0.5):
     from fluxo_amd.workloads import sedov_blast
     m = box_mesh(K, N, x0=(-1.5,) * 3, length=(3.0,) * 3)
     U0 = sedov_blast(m["Elem_xGP"])
+    from fluxo_amd.native import FX_PREC_FP64, FX_PREC_FP32_SPLIT
     kw = dict(volume_flux=32, riemann=1, nfvse=True, alpha_max=0.5, alpha_min=0.01, sweeps=1,
               prop_factor=0.5)
-    warm = Engine(m, **kw)
-    warm.upload(U0)
-    warm.run(3, cfl)
-    warm.close()
-    e = Engine(m, **kw)
-    e.upload(U0)
-    e.sync()
-    t0 = time.perf_counter()
-    t_end, dt_last, err = e.run(steps, cfl)
-    e.sync()
-    el = time.perf_counter() - t0
-    a = e.blending()
-    U = e.download()
-    ms = {"nfvse_alpha": e.profile_kernel(FX_KERNEL_NFVSE_ALPHA, 10),
-          "nfvse_fv": e.profile_kernel(FX_KERNEL_NFVSE_FV, 10),
-          "volume": e.profile_kernel(FX_KERNEL_VOLUME_RK, 10)}
-    e.close()
     nodes = m["nElems"] * (N + 1) ** 3
-    p = 0.4 * (U[..., 4] - 0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0])
-    return {"workload": f"Euler + NFVSE Sedov-type pulse, {K}^3 periodic Cartesian hexes of [-1.5,1.5]^3, "
-                        f"N={N}, {steps} steps CFL {cfl}",
-            "value": nodes * 5 * steps / el, "unit": "DOF-updates/s", "ms_per_step": el / steps * 1e3,
-            "errType": err, "t_end": t_end, "rho_min": float(U[..., 0].min()), "p_min": float(p.min()),
-            "elements_blended_at_end": int((a > 0).sum()), "elements": int(m["nElems"]),
-            "kernel_ms": {k: round(v, 4) for k, v in ms.items()}}
+    res = {}
+    for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
+        warm = Engine(m, precision=prec, **kw)
+        warm.upload(U0)
+        warm.run(3, cfl)
+        warm.close()
+        e = Engine(m, precision=prec, **kw)
+        e.upload(U0)
+        e.sync()
+        t0 = time.perf_counter()
+        t_end, dt_last, err = e.run(steps, cfl)
+        e.sync()
+        el = time.perf_counter() - t0
+        a = e.blending()
+        U = e.download()
+        ms = {"nfvse_alpha": e.profile_kernel(FX_KERNEL_NFVSE_ALPHA, 10),
+              "nfvse_fv": e.profile_kernel(FX_KERNEL_NFVSE_FV, 10),
+              "volume": e.profile_kernel(FX_KERNEL_VOLUME_RK, 10)}
+        e.close()
+        p = 0.4 * (U[..., 4] - 0.5 * (U[..., 1:4] ** 2).sum(-1) / U[..., 0])
+        res[name] = {"value": nodes * 5 * steps / el, "unit": "DOF-updates/s", "ms_per_step": el / steps * 1e3,
+                     "errType": err, "t_end": t_end, "rho_min": float(U[..., 0].min()), "p_min": float(p.min()),
+                     "elements_blended_at_end": int((a > 0).sum()),
+                     "kernel_ms": {k: round(v, 4) for k, v in ms.items()}, "U": U}
+    U64, U32 = res["fp64"].pop("U"), res["fp32_split"].pop("U")
+    d = U32 - U64
+    out = dict(res["fp64"])
+    out["workload"] = (f"Euler + NFVSE Sedov-type pulse, {K}^3 periodic Cartesian hexes of [-1.5,1.5]^3, N={N}, "
+                       f"{steps} steps CFL {cfl}")
+    out["elements"] = int(m["nElems"])
+    out["fp32_split"] = dict(res["fp32_split"], speedup_vs_fp64=res["fp32_split"]["value"] / res["fp64"]["value"],
+                             relL2_vs_fp64=float(np.linalg.norm(d) / np.linalg.norm(U64)),
+                             relLinf_vs_fp64=float(np.max(np.abs(d)) / np.max(np.abs(U64))))
+    return out
 
 
 def _sample_mesh_and_state(K):

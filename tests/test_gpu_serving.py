@@ -138,3 +138,27 @@ def test_submit_rejects_bad_requests():
     with pytest.raises(FxError):
         srv.submit(bad, r["U0"], 3)
     srv.close()
+
+
+def test_fp32split_serving_error_vs_fp64():
+    """The split-precision server against the FP64 server on the same mixed requests: every
+    request's final state within the stated error (relative L2 / Linf printed per request;
+    measured at most 2.2e-7 / 1.6e-6 over these requests, the bounds about 4x that)."""
+    from fluxo_amd.native import Server, FX_PREC_FP64, FX_PREC_FP32_SPLIT
+    reqs = mixed_requests()
+    res = {}
+    for prec in (FX_PREC_FP64, FX_PREC_FP32_SPLIT):
+        srv = Server(precision=prec, **NS)
+        outs = [srv.submit(r["mesh"], r["U0"], r["nsteps"], parabolic=r["parabolic"])[1] for r in reqs]
+        srv.wait(-1, timeout=120)
+        srv.close()
+        res[prec] = outs
+    worst = 0.0
+    for r, a, b in zip(reqs, res[FX_PREC_FP32_SPLIT], res[FX_PREC_FP64]):
+        l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        linf = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"N={r['N']} K={r['K']} {r['ic']} par={r['parabolic']} {r['nsteps']} steps: "
+              f"FP32-split vs FP64 relL2 {l2:.2e} relLinf {linf:.2e}")
+        worst = max(worst, l2)
+        assert l2 <= 1e-6 and linf <= 5e-6  # measured at most 2.2e-7 / 1.6e-6
+    print(f"largest relative L2 {worst:.2e}")
