@@ -543,6 +543,21 @@ def main():
     roof = dict(roofs[dom])
     if roof["traffic"] is not None:
         roof["traffic_source"] = "profiles/pmc_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
+    # the whole path against both peaks: algorithmic flops per DOF-update counted from the CPU
+    # restatement (profiles/flops.json) and the compulsory bytes per DOF-update of a single
+    # fused pass, 8 (4V + 10) + 8 (2 3V) for the BR1 gradients = 480 B for V = 5
+    try:
+        with open(os.path.join(ROOT, "profiles", "flops.json")) as f:
+            fl = json.load(f)["per_dof_update"]["navier_stokes_tgv_n3_br1_ranocha_llf"]
+    except (OSError, ValueError, KeyError):
+        fl = None
+    compulsory = 8 * (4 * 5 + 10) + 8 * 2 * 3 * 5
+    roof["path_hbm_frac"] = round(value * compulsory / (HBM_PEAK_GBS * 1e9), 4)
+    roof["path_compulsory_bytes_per_dof_update"] = compulsory
+    if fl is not None:
+        roof["path_flops_per_dof_update"] = round(fl, 1)
+        roof["path_TFLOPs"] = round(value * fl / 1e12, 3)
+        roof["path_flops_frac"] = round(value * fl / (FP64_PEAK_TFLOPS * 1e12), 4)
     t_end, dt_last = r64["t_end"], r64["dt_last"]
     sedov = None if args.no_sedov else sedov_leg()
     mhd = None if args.no_mhd else mhd_leg(steps=args.mhd_steps, warmup=min(5, args.mhd_steps))
