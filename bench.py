@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DOF-updates/s on 1 GPU (FP64 vs FP32-split) + p50/p99 serving latency"
 HBM_PEAK_GBS = 8000.0
+CPU_SHARE = 16
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -382,20 +383,46 @@ def _sample_mesh_and_state(K):
     return m, tgv_state(m["Elem_xGP"])
 
 
+def host_cpus():
+    """CPUs this process may use: the affinity mask, the cgroup CPU quota, the model name."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = aff if quota is None else max(1, min(aff, int(quota)))
+    # at most CPU_SHARE processes: a one-GPU job's share of a multi-GPU node's host CPUs
+    return dict(nproc=aff, cgroup_cpu_quota=quota, model=model, usable=min(usable, CPU_SHARE))
+
+
 def cpu_baseline(steps, procs):
     """CPU baseline on a bounded sample of the same workload (8^3 warped TGV elements, N=3,
     NS + BR1, LSERK steps with CALCTIMESTEP).  Preferred: the reference's own Fortran behind
-    the harness in oracle/_ref (kind "reference"), run as `procs` independent single-core
-    processes at once (the reference scales as independent MPI ranks); the per-process
-    setup (metrics, initial RHS) is timed separately with nsteps=0 and subtracted.  Without
-    that binary: the plain-C oracle on one core (kind "port")."""
+    the harness in oracle/_ref (kind "reference"; the -O3 x86-64-v3 build), run as `procs`
+    independent single-core processes at once (the reference scales as independent MPI ranks);
+    the per-process setup (metrics, initial RHS) is timed separately with nsteps=0 and
+    subtracted.  Without that binary: the plain-C oracle on one core (kind "port")."""
     K = 8
     m, U = _sample_mesh_and_state(K)
     dofs = K ** 3 * 64 * 5 * steps
+    cpus = host_cpus()
     sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
     try:
         import pyref as R
-        R.binary(True, 32)
+        R.binary(True, 32, fast=True)
         have_ref = True
     except Exception:
         have_ref = False
@@ -405,7 +432,7 @@ def cpu_baseline(steps, procs):
         def one(n):
             t0 = time.perf_counter()
             R.run(m, U, volflux=32, riemann=1, parabolic=True, mu0=1.0 / 1600, Pr=0.72, nsteps=n,
-                  timeout=900)
+                  timeout=900, fast=True)
             return time.perf_counter() - t0
 
         with ThreadPoolExecutor(procs) as ex:
@@ -414,9 +441,10 @@ def cpu_baseline(steps, procs):
             list(ex.map(one, [steps] * procs))
             el = time.perf_counter() - t0 - setup
         return {"value": dofs * procs / el, "unit": "DOF-updates/s", "cores": procs,
-                "kind": "reference",
-                "sample": f"reference Fortran (oracle/_ref harness, flang -O2), {procs} concurrent "
-                          f"single-core runs of NS TGV N=3 BR1 on 8^3 warped elements, {steps} RK "
+                "kind": "reference", "nproc": cpus["nproc"], "cpu_share": CPU_SHARE, "cgroup_cpu_quota": cpus["cgroup_cpu_quota"],
+                "cpu_model": cpus["model"],
+                "sample": f"reference Fortran (oracle/_ref harness, flang -O3 -march=x86-64-v3), {procs} "
+                          f"concurrent single-core runs of NS TGV N=3 BR1 on 8^3 warped elements, {steps} RK "
                           f"steps each ({el:.1f} s after {setup:.1f} s setup)"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -430,6 +458,7 @@ def cpu_baseline(steps, procs):
         t += dt
     el = time.perf_counter() - t0
     return {"value": dofs / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "nproc": cpus["nproc"], "cgroup_cpu_quota": cpus["cgroup_cpu_quota"], "cpu_model": cpus["model"],
             "sample": f"CPU oracle, NS TGV N=3 BR1 on 8^3 warped elements, {steps} RK steps "
                       f"({el:.1f} s)"}
 
@@ -441,7 +470,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--elems", type=int, default=32, help="elements per direction")
     ap.add_argument("--cpu-steps", type=int, default=200)
-    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="concurrent CPU baseline processes (0: every CPU this process may use)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=50)
@@ -563,7 +593,7 @@ def main():
     mhd = None if args.no_mhd else mhd_leg(steps=args.mhd_steps, warmup=min(5, args.mhd_steps))
     serving = None if args.no_serving else serving_leg(args.serve_requests, args.serve_steps, 2026,
                                                           args.serve_load)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_steps, args.cpu_procs or host_cpus()["usable"])
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,

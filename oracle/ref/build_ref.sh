@@ -19,13 +19,16 @@ OUT=$(cd "$HERE/.." && pwd)/_ref
 FC=${FC:-flang}
 [ -d "$REF" ] || { echo "reference sources not found at $REF; nothing built"; exit 0; }
 mkdir -p "$OUT"
-COMMON="-cpp -O2 -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -I$REF"
+COMMON_NOOPT="-cpp -fdefault-real-8 -fdefault-double-8 -DPP_N=N -DPP_NodeType=2 -DPP_DiscType=2 -DMPI=0 -DUSE_AMR=0 -DPOSITIVITYPRES=0 -I$REF"
+# golden-value builds: -O2; the CPU-baseline build (OPT set per variant below) optimises further
+OPT_DEFAULT="-O2"
 
 # VARIANTS="a b ..." builds only the named variants (all when unset)
 build_variant() {
   local name=$1; shift
   case " ${VARIANTS:-$name} " in *" $name "*) ;; *) return 0 ;; esac
   local defs="$*"
+  local COMMON="$COMMON_NOOPT ${OPT:-$OPT_DEFAULT}"
   local obj="$OUT/obj_$name"
   mkdir -p "$obj"
   # the lifting directory not selected by PP_Lifting is left out, as the reference's CMake does
@@ -61,7 +64,7 @@ build_variant() {
     [ $n -eq 0 ] && break
   done
   $FC $COMMON $defs -module-dir "$obj" -I "$obj" -c "$HERE/fluxo_ref_harness.f90" -o "$obj/harness.o"
-  $FC -o "$OUT/fluxo_ref_$name" "$obj/harness.o" $(ls "$obj"/*.o | grep -v harness.o)
+  $FC ${OPT:-$OPT_DEFAULT} -o "$OUT/fluxo_ref_$name" "$obj/harness.o" $(ls "$obj"/*.o | grep -v harness.o)
   echo "built $OUT/fluxo_ref_$name; reference files not compiled:"
   sed 's|^|    |' "$obj/todo.txt"
 }
@@ -96,4 +99,8 @@ TC=taylorgreenvortex build_variant mhd_br1_vf10_tgv $MHD $NOSC -DPP_VolFlux=10 -
 build_variant euler_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=0
 # Navier-Stokes (BR1) with NFVSE: the blended advective volume integral plus VolInt_visc (dg.f90:368-398)
 build_variant ns_br1_nfvse_vf32 $NS -DSHOCKCAPTURE=1 -DSHOCK_NFVSE=1 -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 \
+  -DPP_Lifting_Var=3 -DPP_VISC=0
+# the CPU baseline: Navier-Stokes BR1 with the Ranocha flux, -O3 with AVX2/FMA (x86-64-v3, which
+# the GPU hosts' CPUs support; -march=native would target this container's CPU instead)
+OPT="-O3 -march=x86-64-v3" build_variant ns_br1_vf32_fast $NS $NOSC -DPP_VolFlux=32 -DPARABOLIC=1 -DPP_Lifting=1 \
   -DPP_Lifting_Var=3 -DPP_VISC=0

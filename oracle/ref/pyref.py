@@ -15,12 +15,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(os.path.dirname(HERE), "_ref")
 
 
-def binary(parabolic, volflux, nfvse=False, mhd=False, lifting=1):
+def binary(parabolic, volflux, nfvse=False, mhd=False, lifting=1, fast=False):
+    """Path of the harness build for the configuration; fast: the -O3 x86-64-v3 build (made only
+    for Navier-Stokes BR1 with VolumeFlux 32, the CPU baseline's configuration)."""
     if mhd:
         name = f"fluxo_ref_mhd{f'_br{lifting}' if parabolic else ''}_vf{volflux}"
     else:
         ns = f"ns_br{lifting}"
         name = f"fluxo_ref_{ns if parabolic else 'euler'}{'_nfvse' if nfvse else ''}_vf{volflux}"
+    if fast:
+        name += "_fast"
     path = os.path.join(REF_DIR, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} not built (oracle/ref/build_ref.sh)")
@@ -51,7 +55,8 @@ def read_analysis(d):
 
 
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
-        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0, analyze=None):
+        cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0, analyze=None,
+        fast=False):
     """Metrics, VolInt, RHS (+ gradients), the face fluxes of FillFlux, entropy, dt and an
     nsteps trajectory computed by the reference's code on the given mesh (connectivity +
     NodeCoords) and state.
@@ -64,7 +69,7 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     against AnalyzeExactFunc, bulk integrals, entropy); `analyze` adds parameters to it."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
     n = N + 1
-    exe = binary(parabolic, volflux, nfvse is not None, lifting=lifting)
+    exe = binary(parabolic, volflux, nfvse is not None, lifting=lifting, fast=fast)
     with tempfile.TemporaryDirectory() as d:
         par = os.path.join(d, "parameter.ini")
         with open(par, "w") as f:
