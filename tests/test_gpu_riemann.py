@@ -136,3 +136,21 @@ def test_ismail_roe_volume_flux(N):
         err = rel_l2(e.rhs(), O.Problem(m, volflux=10, riemann=1).rhs(U))
         print(f"VolumeFlux 10 FP32-split N=3: RHS relL2 {err:.3e}")
         assert err <= 1e-4
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+@pytest.mark.parametrize("rs", [2, 3, 22, 23, 24])
+def test_hll_family_and_roe_supersonic(rs, sign):
+    """HLLC, Roe, HLL, HLLE and HLLEM where the flow crosses every face supersonically (density
+    wave advected with velocity 3 (1,1,1) or -3 (1,1,1), sound speed about 1.2): the upwind
+    branches (all waves to one side: the left state's flux for +, the right state's for -) are
+    taken at every face point; RHS against the oracle, whose upwind branches are pinned by the
+    physical flux in test_oracle_riemann."""
+    from fluxo_amd.native import Engine
+    m = O.box_mesh(3, 3, x0=(0, 0, 0), length=(2, 2, 2), Ngeo=3, warp=True)
+    U = O.exactfunc(2, m["Elem_xGP"], advvel=(3.0 * sign,) * 3)
+    e = Engine(m, volume_flux=32, riemann=rs)
+    e.upload(U)
+    err = rel_l2(e.rhs(), O.Problem(m, volflux=32, riemann=rs).rhs(U))
+    print(f"Riemann {rs} supersonic sign {sign:+.0f}: RHS relL2 {err:.3e}")
+    assert err <= 1e-12

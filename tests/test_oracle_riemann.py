@@ -169,3 +169,9 @@ def test_hll_and_roe_solvers(rs):
         vn = 5.0
         FL = [UL[0] * vn, *(UL[1:4] * vn + 1.0 * n), (UL[4] + 1.0) * vn]
         assert np.allclose(flux(rs, UL, UR, n, t1, t2), FL, rtol=1e-12, atol=1e-12)
+        # supersonic against n: every wave moves left, the flux is the right state's
+        UL = np.array([1.1, -5.2 * n[0], -5.2 * n[1], -5.2 * n[2], 1.1 / (KAPPA - 1) + 0.55 * 27.04])
+        UR = np.array([1.0, -5.0 * n[0], -5.0 * n[1], -5.0 * n[2], 1.0 / (KAPPA - 1) + 12.5])
+        vn = -5.0
+        FR = [UR[0] * vn, *(UR[1:4] * vn + 1.0 * n), (UR[4] + 1.0) * vn]
+        assert np.allclose(flux(rs, UL, UR, n, t1, t2), FR, rtol=1e-12, atol=1e-12)
