@@ -21,8 +21,9 @@
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
 !> MeshFile (optional): a HOPR mesh file (conforming, periodic boundaries; libfluxo_io) instead
 !> of the box generator.  RestartFile (optional): initial state from a state file (Restart,
-!> restart.f90:147-232; same N and number of variables) instead of IniExactFunc, its Time as the
-!> start time.  ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
+!> restart.f90:147-232; same number of variables and elements) instead of IniExactFunc, its Time
+!> as the start time; a file of another N is interpolated (lower N) or projected conservatively
+!> through J U (higher N) with the reference's modal Vandermonde matrices.  ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
 !> <ProjectName>_State_<time>.h5.  StateFile (optional): the final U(nVar,0:N,0:N,0:N,nElems) as
 !> raw FP64 in Fortran order.
 !==================================================================================================
@@ -49,7 +50,7 @@ program fluxo_gpu
   integer(c_int), allocatable :: e2s(:, :, :), s2e(:, :)
   real(rk), allocatable :: NC(:, :, :, :, :), Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
   real(rk), allocatable :: sJm(:, :, :, :), nv(:, :, :, :), t1v(:, :, :, :), t2v(:, :, :, :), se(:, :, :)
-  real(rk), allocatable :: xGP(:, :, :, :, :), Uttmp(:, :, :, :, :)
+  real(rk), allocatable :: xGP(:, :, :, :, :), Uttmp(:, :, :, :, :), DetJacRef(:, :, :, :, :)
   logical :: warp, resident
   integer(8) :: c0, c1, cr
 
@@ -152,7 +153,8 @@ program fluxo_gpu
   c%nElems = E; c%nSides = S
   allocate(Mf(3, 0:n, 0:n, 0:n, E), Mg(3, 0:n, 0:n, 0:n, E), Mh(3, 0:n, 0:n, 0:n, E), xGP(3, 0:n, 0:n, 0:n, E))
   allocate(sJm(0:n, 0:n, 0:n, E), nv(3, 0:n, 0:n, S), t1v(3, 0:n, 0:n, S), t2v(3, 0:n, 0:n, S), se(0:n, 0:n, S))
-  call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0)
+  allocate(DetJacRef(1, 0:3*Ngeo, 0:3*Ngeo, 0:3*Ngeo, E))
+  call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0, DetJacRef)
   if (ierr /= 0) then
     write(*, '(A)') ' ERROR: negative or badly scaled Jacobian in the generated mesh'
     stop 2
@@ -163,11 +165,15 @@ program fluxo_gpu
   if (has_key('RestartFile')) then
     sfile = GETSTR('RestartFile')
     call io_chk(fxio_state_info(trim(sfile)//c_null_char, rN, rVar, rE, t), 'fxio_state_info')
-    if (rN /= n .or. rVar /= nvar .or. rE /= E) then
-      write(*, '(A)') ' ERROR: restart file N / nVar / nElems differ from the run (no interpolation)'
+    if (rVar /= nvar .or. rE /= E) then
+      write(*, '(A)') ' ERROR: restart file nVar / nElems differ from the run'
       stop 2
     end if
-    call io_chk(fxio_state_read(trim(sfile)//c_null_char, n, nvar, E, U), 'fxio_state_read')
+    if (rN == n) then
+      call io_chk(fxio_state_read(trim(sfile)//c_null_char, n, nvar, E, U), 'fxio_state_read')
+    else
+      call restart_change_n(rN)
+    end if
     write(*, '(A,A,A,ES14.6)') ' restart from ', trim(sfile), ' at t=', t
   else
     call FillIni()
@@ -274,6 +280,43 @@ program fluxo_gpu
   call FinalizeDG_GPU()
 
 contains
+
+  !> Restart from a state of degree Nr (restart.f90:183-232, InterpolateSolution): the modal
+  !> Vandermonde matrix Nr -> N (interpolation for Nr < N; for Nr > N the Legendre modes above N
+  !> are dropped, applied to J U with J interpolated from the 3 Ngeo nodes to Nr, and the
+  !> result divided by J again, which keeps each element's integral of U).
+  subroutine restart_change_n(Nr)
+    integer, intent(in) :: Nr
+    real(rk), allocatable :: Ur(:, :, :, :, :), Vdm(:, :), Vdm3(:, :), JNR(:, :, :, :)
+    integer :: iel, i, j, k
+    allocate(Ur(nvar, 0:Nr, 0:Nr, 0:Nr, E), Vdm(0:n, 0:Nr), Vdm3(0:Nr, 0:3*Ngeo), JNR(1, 0:Nr, 0:Nr, 0:Nr))
+    call io_chk(fxio_state_read(trim(sfile)//c_null_char, Nr, nvar, E, Ur), 'fxio_state_read')
+    write(*, '(A,I0,A,I0)') ' interpolating the restart state from N=', Nr, ' to N=', n
+    call vandermonde(Nr, NODE_GL, n, NODE_GL, .true., Vdm)
+    call vandermonde(3*Ngeo, NODE_GL, Nr, NODE_GL, .true., Vdm3)
+    do iel = 1, E
+      if (Nr > n) then
+        call change_basis_3d(1, 3*Ngeo, Nr, Vdm3, DetJacRef(:, :, :, :, iel), JNR)
+        do k = 0, Nr
+          do j = 0, Nr
+            do i = 0, Nr
+              Ur(:, i, j, k, iel) = Ur(:, i, j, k, iel)*JNR(1, i, j, k)
+            end do
+          end do
+        end do
+      end if
+      call change_basis_3d(nvar, Nr, n, Vdm, Ur(:, :, :, :, iel), U(:, :, :, :, iel))
+      if (Nr > n) then
+        do k = 0, n
+          do j = 0, n
+            do i = 0, n
+              U(:, i, j, k, iel) = U(:, i, j, k, iel)*sJ(i, j, k, iel)
+            end do
+          end do
+        end do
+      end if
+    end do
+  end subroutine restart_change_n
 
   subroutine io_chk(rc, where)
     integer(c_int), intent(in) :: rc

@@ -153,9 +153,11 @@ contains
 
   !> CalcMetrics + CalcSurfMetrics; ierr=1 for non-positive or badly scaled Jacobians.
   subroutine calc_metrics(N, Ngeo, nElems, nSides, NodeCoords, e2s, Mf, Mg, Mh, sJ, NormVec, &
-                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr, nodesGL)
+                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr, nodesGL, DetJacRef)
     integer, intent(in) :: N, Ngeo, nElems, nSides
     logical, intent(in), optional :: nodesGL   ! NodeCoords on Gauss-Lobatto instead of equidistant nodes
+    ! the Jacobian on the 3 Ngeo Gauss-Lobatto nodes (DetJac_Ref of metrics.f90), for Restart
+    real(rk), intent(out), optional :: DetJacRef(1, 0:3*Ngeo, 0:3*Ngeo, 0:3*Ngeo, nElems)
     real(rk), intent(in) :: NodeCoords(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, nElems)
     integer, intent(in) :: e2s(2, 6, nElems)
     real(rk), intent(out), dimension(3, 0:N, 0:N, 0:N, nElems) :: Mf, Mg, Mh, Elem_xGP
@@ -221,6 +223,7 @@ contains
           end do
         end do
       end do
+      if (present(DetJacRef)) DetJacRef(:, :, :, :, e) = detRef
       call change_basis_3d(1, NgeoRef, N, V_ref_N, detRef, detN)
       sJ(:, :, :, e) = 1.0_rk/detN(1, :, :, :)
       jmin = minval(detN); jmax = maxval(detN)

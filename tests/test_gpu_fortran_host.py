@@ -222,3 +222,76 @@ def test_fortran_host_shock_capturing_against_reference(tmp_path, resident):
     print(f"  {steps} steps relL2 vs the reference {err:.3e}")
     assert err <= 1e-10
     assert f"iter={steps}" in p.stdout.replace(" ", "")
+
+
+def _restart_run(tmp_path, Ur, K, N, Ngeo, warp, x0=0.0, L=2.0):
+    """Run the driver for no steps from a restart file holding Ur (another N); returns the
+    state it starts from (StateFile)."""
+    from fluxo_amd import io
+    io.write_state(tmp_path / "start.h5", Ur, 0.0)
+    sfile = tmp_path / "state.bin"
+    (tmp_path / "parameter.ini").write_text(
+        f"N = {N}\nVolumeFlux = 32\nRiemann = 1\nIniExactFunc = 1\nRestartFile = start.h5\nCFLscale = 0.9\n"
+        f"maxIter = 0\nTEnd = 1.0\nBoxElems = (/{K},{K},{K}/)\nBoxCorner = (/{x0},{x0},{x0}/)\n"
+        f"BoxLength = (/{L},{L},{L}/)\nNgeo = {Ngeo}\nmeshdeform = {'T' if warp else 'F'}\nStateFile = {sfile}\n")
+    p = subprocess.run([EXE, "parameter.ini"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "interpolating the restart state" in p.stdout
+    n = N + 1
+    return np.fromfile(sfile, np.float64).reshape(K ** 3, n, n, n, 5)
+
+
+def _poly_state(x, deg):
+    """A state whose variables are polynomials of total degree <= deg per direction."""
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    U = np.empty(x.shape[:-1] + (5,))
+    U[..., 0] = 2.0 + 0.1 * X + 0.05 * X * Y ** (deg - 1) + 0.02 * Z ** deg
+    U[..., 1] = 0.3 - 0.1 * Y ** deg
+    U[..., 2] = 0.1 * X * Z
+    U[..., 3] = -0.2 + 0.03 * (X * Y * Z) ** 1
+    U[..., 4] = 6.0 + 0.1 * X ** deg * Y
+    return U
+
+
+@pytest.mark.parametrize("Nr,N", [(5, 3), (2, 4)])
+def test_fortran_host_restart_change_of_n_polynomial(tmp_path, Nr, N):
+    """Restart with a change of N (restart.f90:183-232) on a Cartesian mesh: a state polynomial
+    of degree <= min(Nr, N) per direction comes back exactly at the new nodes (interpolation for
+    Nr < N; for Nr > N the modal projection keeps every mode up to N), and for Nr > N the
+    element integrals of a non-polynomial state are kept (projection of J U with constant J)."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    K = 2
+    mr = O.box_mesh(K, Nr, x0=(0, 0, 0), length=(2, 2, 2))
+    m = O.box_mesh(K, N, x0=(0, 0, 0), length=(2, 2, 2))
+    deg = min(Nr, N)
+    U = _restart_run(tmp_path, _poly_state(mr["Elem_xGP"], deg), K, N, 1, False)
+    err = np.max(np.abs(U - _poly_state(m["Elem_xGP"], deg)))
+    print(f"restart N={Nr} -> N={N}: polynomial of degree {deg}, max error {err:.2e}")
+    assert err <= 1e-12
+    if Nr > N:
+        Ur = O.exactfunc(7, mr["Elem_xGP"])
+        U = _restart_run(tmp_path, Ur, K, N, 1, False)
+        wr, w = O.dg_operators(Nr)["w"], O.dg_operators(N)["w"]
+        Wr = wr[:, None, None] * wr[None, :, None] * wr[None, None, :]
+        W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+        ir = np.einsum("kji,ekjiv->ev", Wr, Ur / mr["sJ"][..., None])
+        iN = np.einsum("kji,ekjiv->ev", W, U / m["sJ"][..., None])
+        d = np.max(np.abs(iN - ir)) / np.max(np.abs(ir))
+        print(f"  element integrals of the Taylor-Green state kept to {d:.2e}")
+        assert d <= 1e-13
+
+
+def test_fortran_host_restart_change_of_n_free_stream_warped(tmp_path):
+    """On a curved mesh (Ngeo 3, warped) the conservative projection from N=5 to N=3 multiplies
+    by J interpolated to N=5 and divides by J at N=3; both come from the same 3 Ngeo Jacobian by
+    modal truncation, so a constant state stays constant."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    K, Nr, N = 2, 5, 3
+    c = np.array([1.2, 0.3, -0.2, 0.1, 2.5])
+    Ur = np.broadcast_to(c, (K ** 3, Nr + 1, Nr + 1, Nr + 1, 5)).copy()
+    U = _restart_run(tmp_path, Ur, K, N, 3, True)
+    err = np.max(np.abs(U - c))
+    print(f"warped free stream restart N={Nr} -> N={N}: max error {err:.2e}")
+    assert err <= 1e-13
