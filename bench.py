@@ -463,6 +463,21 @@ def cpu_baseline(steps, procs):
                       f"({el:.1f} s)"}
 
 
+def copy_calibration():
+    """Device-to-device copies of known size (256 MiB and 1 GiB, 3 times each) for the HBM
+    traffic counters: each copy reads and writes its size once."""
+    import torch
+    for mib in (256, 1024):
+        n = mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {mib} MiB x 3", flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -481,10 +496,15 @@ def main():
     ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--mhd-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only device-to-device copies of 256 MiB and 1 GiB (known bytes, for calibrating "
+                         "the FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU element partitioning is a design "
-              "only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; partitioning the elements over several GPUs "
+              "is not built. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from fluxo_amd.mesh import box_mesh
