@@ -207,7 +207,8 @@ void fx_ctx_destroy(fx_ctx *c)
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
-                  b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs};
+                  b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs,
+                  b.mort, b.mops, b.Umort};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -221,10 +222,39 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
       !m->Metrics_hTilde || !m->sJ || !m->NormVec || !m->SurfElem)
     return set_err(FX_EINVAL, "missing mesh array");
   Batch &b = c->b;
-  const int E = b.cfg.nElems, S = b.cfg.nSides;
-  int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S);
+  const int E = b.cfg.nElems, S = b.cfg.nSides, nm = m->nMortarSides;
+  std::vector<int> mort;
+  int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S, nm, m->MortarType, m->MortarInfo, &mort);
   if (rc) return rc;
+  if (nm > 0 && (b.mhd() || b.cfg.parabolic || b.cfg.nfvse))
+    return set_err(FX_EINVAL, "meshes with mortars: Euler without shock capturing only (nVar 5, parabolic 0, "
+                   "nfvse 0)");
+  if (nm > 0 && b.cfg.whichMortar != 0 && b.cfg.whichMortar != 1)
+    return set_err(FX_EINVAL, "whichMortar=%d (0 projection, 1 collocation)", b.cfg.whichMortar);
   const size_t n3E = (size_t)b.n3 * E, nf = (size_t)b.n2 * S;
+  if (nm > 0) {
+    HIPCHK(hipStreamSynchronize(b.stream));
+    if (b.nMortar != nm) {  // the mortar buffers are sized by the mesh
+      for (void *p : {(void *)b.mort, (void *)b.mops, (void *)b.Umort})
+        if (p) HIPCHK(hipFree(p));
+      b.mort = nullptr; b.mops = nullptr; b.Umort = nullptr; b.nMortar = 0;
+      HIPCHK(hipMalloc((void **)&b.mort, mort.size() * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&b.mops, 4 * (size_t)b.n2 * 8));
+      HIPCHK(hipMalloc((void **)&b.Umort, 5 * nf * 8));
+    }
+    double mops[4 * (MAXN + 1) * (MAXN + 1)];
+    make_mortar_ops(b.cfg.N, b.cfg.whichMortar, mops);
+    HIPCHK(hipMemcpy(b.mort, mort.data(), mort.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.mops, mops, 4 * (size_t)b.n2 * 8, hipMemcpyHostToDevice));
+    b.nMortar = nm;
+  } else {
+    b.nMortar = 0;
+  }
+  if (c->graph) {  // a step graph captured for another mesh's launches
+    HIPCHK(hipStreamSynchronize(b.stream));
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
   std::vector<double4> nvse(nf);
   for (size_t f = 0; f < nf; f++)
     nvse[f] = make_double4(m->NormVec[3 * f], m->NormVec[3 * f + 1], m->NormVec[3 * f + 2], m->SurfElem[f]);
@@ -864,6 +894,7 @@ int fx_rhs_noalloc(const fx_config *cfg, const fx_mesh *mesh, const double *U, d
   if (!mesh->ElemToSide || !mesh->SideToElem || !mesh->Metrics_fTilde || !mesh->Metrics_gTilde ||
       !mesh->Metrics_hTilde || !mesh->sJ || !mesh->NormVec || !mesh->SurfElem)
     return set_err(FX_EINVAL, "missing mesh array");
+  if (mesh->nMortarSides != 0) return set_err(FX_EINVAL, "fx_rhs_noalloc: conforming meshes only (no mortars)");
   const NoallocLayout L = noalloc_layout(*cfg);
   if (L.total > (size_t)FX_NOALLOC_SCRATCH_BYTES)
     return set_err(FX_EINVAL, "fx_rhs_noalloc: N=%d, %d elements, %d sides need %zu bytes of scratch, above "

@@ -521,10 +521,14 @@ template <int N> struct SurfGeom {
   static constexpr int n2 = (N + 1) * (N + 1), SPB = 256 / n2, T = SPB * n2;
 };
 
+// Um (meshes with mortars, else null): the master states of the small sides of mortars
+// (k_mortar_u), which have no master element; a big mortar side has no slave element and is
+// skipped here (k_flux_mortar fills its flux).
 template <int N, int RS, bool PARA>
 __global__ void __launch_bounds__(256)
 k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
-          const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa)
+          const double4 *__restrict__ nvse, double *__restrict__ Flux, int nSides, double kappa,
+          const double *__restrict__ Um)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   constexpr bool ES = (RS == 4 || RS == 5 || RS == 13 || RS == 16 || RS == 33 || RS == 44);
@@ -532,13 +536,15 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
   const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
   const bool valid = threadIdx.x < SurfGeom<N>::T && gid < nSides * n2;
   const int em = valid ? s2e[5 * s] - 1 : -1;
-  const bool used = valid && em >= 0;  // unused side slots of a batch have SideToElem zeroed
+  const int es = valid ? s2e[5 * s + 1] - 1 : -1;
+  // unused side slots of a batch have SideToElem zeroed
+  const bool used = valid && es >= 0 && (em >= 0 || (em == -2 && Um != nullptr));
   double UL[5], UR[5], nv[3] = {0.0, 0.0, 0.0}, se = 0.0;
   if (used) {
-    const int es = s2e[5 * s + 1] - 1;
     const int lm = s2e[5 * s + 2], ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
-    const int vm = side_to_vol<N>(lm, 0, p, q), vs = side_to_vol<N>(ls, fl, p, q);
-    const double *um = U + ((size_t)em * n3 + vm) * 5, *us = U + ((size_t)es * n3 + vs) * 5;
+    const int vs = side_to_vol<N>(ls, fl, p, q);
+    const double *um = em >= 0 ? U + ((size_t)em * n3 + side_to_vol<N>(lm, 0, p, q)) * 5 : Um + (size_t)gid * 5;
+    const double *us = U + ((size_t)es * n3 + vs) * 5;
 #pragma unroll
     for (int v = 0; v < 5; v++) { UL[v] = um[v]; UR[v] = us[v]; }
     const double4 g = nvse[gid];
@@ -567,6 +573,126 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
   double *o = Flux + (size_t)gid * 5;
 #pragma unroll
   for (int v = 0; v < 5; v++) o[v] = F[v] * se;
+}
+
+// ---------------------------------------------------------------------------------
+// Non-conforming interfaces (mortars) on one GPU.  mort holds per mortar [type, big element
+// (0-based), big element's local side, small side ids (1-based, 4 or 2), big side id]:
+// type 1 = 1->4, 2 = 1->2 in eta, 3 = 1->2 in xi (MortarType, MortarInfo of
+// prepare_mesh.f90:700-718).  mops holds MInt[k][l][p] = l_l(xi^k_p) and then
+// MProj[k][l][p] (the reference's MInt(l,p,k), MProj(l,p,k); InitMortarBase
+// mortar.f90:63-110), k = 0 for the half [-1,0], 1 for [0,1].
+constexpr int MORT_INTS = 8;
+
+// U_Mortar (fillmortar.f90:72-129) with InterpolateBigToSmall (mortar_vars.f90:203-266): the
+// master state of each small side, interpolated from the big element's face (its master side,
+// read from the volume as ProlongToFace would), in the reference's order of summation, into
+// Um at the small side's face points.  One thread per (mortar, small side, p, q).
+template <int N>
+__global__ void __launch_bounds__(256)
+k_mortar_u(const double *__restrict__ U, const int *__restrict__ mort, const double *__restrict__ mops,
+           double *__restrict__ Um, int nMortar)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = gid / (4 * n2), r = gid - m * (4 * n2), im = r / n2, pq = r - im * n2;
+  const int p = pq % n, q = pq / n;
+  if (m >= nMortar) return;
+  const int *mi = mort + MORT_INTS * m;
+  const int type = mi[0], lb = mi[2];
+  if (type != 1 && im >= 2) return;
+  const double *ub = U + (size_t)mi[1] * n3 * 5;
+  const double *MI = mops;
+  double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (type == 1) {  // eta first: tmp(l,q) = sum_m MInt(m,q,jNb) Big(l,m); then xi
+    const int iNb = im & 1, jNb = im >> 1;
+    for (int l = 0; l < n; l++) {
+      double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int mm = 0; mm < n; mm++) {
+        const double c = MI[(jNb * n + mm) * n + q];
+        const double *u = ub + side_to_vol<N>(lb, 0, l, mm) * 5;
+#pragma unroll
+        for (int v = 0; v < 5; v++) t[v] = t[v] + c * u[v];
+      }
+      const double c = MI[(iNb * n + l) * n + p];
+#pragma unroll
+      for (int v = 0; v < 5; v++) o[v] = o[v] + c * t[v];
+    }
+  } else if (type == 2) {  // halves in eta: sum_l MInt(l,q,jNb) Big(p,l)
+    for (int l = 0; l < n; l++) {
+      const double c = MI[(im * n + l) * n + q];
+      const double *u = ub + side_to_vol<N>(lb, 0, p, l) * 5;
+#pragma unroll
+      for (int v = 0; v < 5; v++) o[v] = o[v] + c * u[v];
+    }
+  } else {  // halves in xi: sum_l MInt(l,p,iNb) Big(l,q)
+    for (int l = 0; l < n; l++) {
+      const double c = MI[(im * n + l) * n + p];
+      const double *u = ub + side_to_vol<N>(lb, 0, l, q) * 5;
+#pragma unroll
+      for (int v = 0; v < 5; v++) o[v] = o[v] + c * u[v];
+    }
+  }
+  double *dst = Um + ((size_t)(mi[3 + im] - 1) * n2 + pq) * 5;
+#pragma unroll
+  for (int v = 0; v < 5; v++) dst[v] = o[v];
+}
+
+// Flux_Mortar (fillmortar.f90:239-368, inner mortars, weak form): the big side's flux as the
+// projection of its small sides' fluxes (each small side's master flux: the mortar is the
+// master of its small sides), in the reference's order of summation.  One thread per
+// (mortar, p, q) of the big side.
+template <int N>
+__global__ void __launch_bounds__(256)
+k_flux_mortar(double *__restrict__ Flux, const int *__restrict__ mort, const double *__restrict__ mops,
+              int nMortar)
+{
+  constexpr int n = N + 1, n2 = n * n;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = gid / n2, pq = gid - m * n2, p = pq % n, q = pq / n;
+  if (m >= nMortar) return;
+  const int *mi = mort + MORT_INTS * m;
+  const int type = mi[0];
+  const double *MP = mops + 2 * n2;
+  double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (type == 1) {  // xi first into Flux_tmp(p,l), then eta
+    for (int jNb = 0; jNb < 2; jNb++)
+      for (int l = 0; l < n; l++) {
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int iNb = 0; iNb < 2; iNb++) {
+          const double *fs = Flux + ((size_t)(mi[3 + iNb + 2 * jNb] - 1) * n2 + l * n) * 5;
+          for (int l2 = 0; l2 < n; l2++) {
+            const double c = MP[(iNb * n + l2) * n + p];
+#pragma unroll
+            for (int v = 0; v < 5; v++) t[v] = t[v] + c * fs[l2 * 5 + v];
+          }
+        }
+        const double c = MP[(jNb * n + l) * n + q];
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v] = o[v] + c * t[v];
+      }
+  } else if (type == 2) {  // halves in eta
+    for (int jNb = 0; jNb < 2; jNb++) {
+      const double *fs = Flux + (size_t)(mi[3 + jNb] - 1) * n2 * 5;
+      for (int l = 0; l < n; l++) {
+        const double c = MP[(jNb * n + l) * n + q];
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v] = o[v] + c * fs[(l * n + p) * 5 + v];
+      }
+    }
+  } else {  // halves in xi
+    for (int iNb = 0; iNb < 2; iNb++) {
+      const double *fs = Flux + (size_t)(mi[3 + iNb] - 1) * n2 * 5;
+      for (int l = 0; l < n; l++) {
+        const double c = MP[(iNb * n + l) * n + p];
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v] = o[v] + c * fs[(q * n + l) * 5 + v];
+      }
+    }
+  }
+  double *dst = Flux + ((size_t)(mi[7] - 1) * n2 + pq) * 5;
+#pragma unroll
+  for (int v = 0; v < 5; v++) dst[v] = o[v];
 }
 
 // BR1 lifting with entropy variables (Lifting br1.f90:154-364: Lifting_VolInt

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "../../include/fluxo_gpu.h"
 #include "fx_kernels.hpp"
@@ -193,6 +194,83 @@ inline void make_ops(int N, Ops &o)
   }
 }
 
+// Mortar operators of one degree on LGL nodes (InitMortarBase, mortar.f90:63-110, with
+// MortarBasis_BigToSmall and MortarBasis_SmallToBig_Projection / _Collocation,
+// mortar_vars.f90:70-196) as the kernels read them: MInt[k][l][p] = l_l(xi^k_p) (Lagrange
+// polynomial l of the big side at node p of the half k: 0 = [-1,0], 1 = [0,1]), then
+// MProj[k][l][p], the weight of node l of half k in node p of the big side: whichMortar 0 the
+// L2 projection through the orthonormal Legendre modes with N+1 Gauss points, 1 collocation
+// (w_l / w_p l_p(xi^k_l)).
+inline void make_mortar_ops(int N, int whichMortar, double *mops)
+{
+  const int n = N + 1;
+  double x[MAXN + 1], w[MAXN + 1], wb[MAXN + 1];
+  lgl_nodes(N, x, w);
+  for (int i = 0; i < n; i++) wb[i] = 1.0;
+  for (int i = 1; i <= N; i++)
+    for (int j = 0; j < i; j++) { wb[j] *= (x[j] - x[i]); wb[i] *= (x[i] - x[j]); }
+  for (int i = 0; i < n; i++) wb[i] = 1.0 / wb[i];
+  // barycentric Lagrange polynomials of the LGL nodes at y (a node itself: unit vector,
+  // LagrangeInterpolationPolys basis.f90:732-767)
+  auto lagrange = [&](double y, double *L) {
+    int hit = -1;
+    for (int k = 0; k < n; k++) {
+      const double a = y, b = x[k], tol = 2.220446049250313e-16;
+      const bool eq = (a == 0.0 || b == 0.0) ? std::fabs(a - b) <= 2.0 * tol
+                                             : (std::fabs(a - b) <= tol * std::fabs(a) && std::fabs(a - b) <= tol * std::fabs(b));
+      if (eq) hit = k;
+    }
+    if (hit >= 0) {
+      for (int k = 0; k < n; k++) L[k] = k == hit ? 1.0 : 0.0;
+      return;
+    }
+    double sum = 0.0;
+    for (int k = 0; k < n; k++) { L[k] = wb[k] / (y - x[k]); sum += L[k]; }
+    for (int k = 0; k < n; k++) L[k] = L[k] / sum;
+  };
+  double V[2][MAXN + 1][MAXN + 1];  // V[k][p][l] = l_l(xi^k_p)
+  for (int k = 0; k < 2; k++)
+    for (int p = 0; p < n; p++) {
+      const double y = k == 0 ? 0.5 * (x[p] - 1.0) : 0.5 * (x[p] + 1.0);
+      lagrange(y, V[k][p]);
+    }
+  double *MI = mops, *MP = mops + 2 * n * n;
+  for (int k = 0; k < 2; k++)
+    for (int l = 0; l < n; l++)
+      for (int p = 0; p < n; p++) MI[(k * n + l) * n + p] = V[k][p][l];
+  if (whichMortar == 1) {
+    for (int k = 0; k < 2; k++)
+      for (int l = 0; l < n; l++)
+        for (int p = 0; p < n; p++) MP[(k * n + l) * n + p] = w[l] / w[p] * V[k][l][p];
+    return;
+  }
+  double xg[MAXN + 1], wg[MAXN + 1];
+  gauss_nodes(N, xg, wg);
+  double VGP[MAXN + 1][MAXN + 1];  // wg_g l_j(xg_g)
+  for (int g = 0; g < n; g++) {
+    double L[MAXN + 1];
+    lagrange(xg[g], L);
+    for (int j = 0; j < n; j++) VGP[g][j] = wg[g] * L[j];
+  }
+  for (int k = 0; k < 2; k++) {
+    // T(jm, l) = sum_g phi_jm(xg_g on half k) VGP(g, l);  M_k0(p, l) = sum_jm phi_jm(x_p) T(jm, l)
+    double T[MAXN + 1][MAXN + 1];
+    for (int jm = 0; jm < n; jm++)
+      for (int l = 0; l < n; l++) {
+        double sum = 0.0;
+        for (int g = 0; g < n; g++)
+          sum += legendre_normalized(jm, k == 0 ? 0.5 * (xg[g] - 1.0) : 0.5 * (xg[g] + 1.0)) * VGP[g][l];
+        T[jm][l] = sum;
+      }
+    for (int p = 0; p < n; p++)
+      for (int l = 0; l < n; l++) {
+        double sum = 0.0;
+        for (int jm = 0; jm < n; jm++) sum += legendre_normalized(jm, x[p]) * T[jm][l];
+        MP[(k * n + l) * n + p] = sum;
+      }
+  }
+}
+
 // Every compilation unit with kernels keeps its own c_ops and registers, at load time, a
 // function that fills its copy for one degree on the current device.
 using OpsUploader = int (*)(const Ops &o, int N);
@@ -300,6 +378,12 @@ struct Batch {
   // GLM-MHD (cfg.nVar == 9): slave-side face fluxes (Flux holds the master side) and the
   // per-element dt of a unit wave speed
   double *FluxS = nullptr, *dtch1Elem = nullptr;
+  // non-conforming interfaces: nMortar big mortar sides (sides 1..nMortar), their description
+  // (MORT_INTS ints each), the interpolation / projection operators and the master states of
+  // the small sides (face-point layout of Flux)
+  int nMortar = 0;
+  int *mort = nullptr;
+  double *mops = nullptr, *Umort = nullptr;
 
   bool mhd() const { return cfg.nVar == 9; }
   MhdArgs mhd_args() const
@@ -326,6 +410,9 @@ struct Batch {
 // (c_ops); ensure_ops fills every unit's copy.
 template <int N> void launch_volume_n(const Batch &b, int mode, RKArgs rk);
 template <int N> void launch_surface_n(const Batch &b);
+// U_Mortar into b.Umort, Flux_Mortar into the big sides of b.Flux (meshes with mortars)
+template <int N> void launch_mortar_u_n(const Batch &b);
+template <int N> void launch_flux_mortar_n(const Batch &b);
 template <int N> void launch_lifting_n(const Batch &b, int out);
 template <int N> void launch_dt_n(const Batch &b, double cfl, double dfl);
 template <int N> void launch_mhd_volume_n(const Batch &b, int mode, RKArgs rk);
@@ -356,7 +443,11 @@ inline void q_volume(const Batch &b, int mode, RKArgs rk)
 inline void q_surface(const Batch &b)
 {
   if (b.mhd()) { FX_DISPATCH_N(b.cfg.N, launch_mhd_surface_n, b); }
-  else { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
+  else {
+    if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_mortar_u_n, b); }
+    FX_DISPATCH_N(b.cfg.N, launch_surface_n, b);
+    if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_flux_mortar_n, b); }
+  }
 }
 inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out); }
 
@@ -441,19 +532,54 @@ inline void q_rk_step(const Batch &b)
 
 #endif  // FX_KERNEL_HOME
 
-// Connectivity check of one conforming periodic mesh (every side an inner side): every
-// index the kernels derive from ElemToSide/SideToElem must stay inside the mesh.
-inline int check_connectivity(const int *ElemToSide, const int *SideToElem, int E, int S)
+// Connectivity check of one periodic mesh: every index the kernels derive from
+// ElemToSide/SideToElem (and, with mortars, from MortarType/MortarInfo) must stay inside the
+// mesh.  Without mortars every side is a conforming inner side.  With nMortar > 0 the sides
+// 1..nMortar are big mortar sides (master element only, MortarType(2,s) = s) whose 4 (type 1)
+// or 2 (types 2, 3) small sides each have a slave element only and belong to exactly one
+// mortar, with flip 0 towards it; mort receives the kernels' description of each mortar.
+inline int check_connectivity(const int *ElemToSide, const int *SideToElem, int E, int S, int nMortar = 0,
+                              const int *MortarType = nullptr, const int *MortarInfo = nullptr,
+                              std::vector<int> *mort = nullptr)
 {
-  for (int s = 0; s < S; s++) {
-    const int *t = SideToElem + 5 * s;
-    if (t[0] < 1 || t[0] > E || t[1] < 1 || t[1] > E || t[2] < 1 || t[2] > 6 || t[3] < 1 || t[3] > 6 ||
-        t[4] < 1 || t[4] > 4)
-      return set_err(FX_EINVAL, "SideToElem(:,%d) invalid (only inner conforming sides are supported)", s + 1);
+  if (nMortar < 0 || nMortar >= S || (nMortar > 0 && (!MortarType || !MortarInfo)))
+    return set_err(FX_EINVAL, "nMortarSides=%d invalid or mortar arrays missing", nMortar);
+  std::vector<int> owner(S, -1);  // mortar of each small side
+  if (mort) mort->assign((size_t)MORT_INTS * nMortar, 0);
+  for (int m = 0; m < nMortar; m++) {
+    const int type = MortarType[2 * m], *mi = MortarInfo + 10 * m, *t = SideToElem + 5 * m;
+    if (type < 1 || type > 3 || MortarType[2 * m + 1] != m + 1 || mi[0] != m + 1 || mi[1] != 0)
+      return set_err(FX_EINVAL, "side %d: not a big mortar side in MortarType / MortarInfo", m + 1);
+    if (t[0] < 1 || t[0] > E || t[2] < 1 || t[2] > 6 || t[1] != -1)
+      return set_err(FX_EINVAL, "SideToElem(:,%d) invalid for a big mortar side", m + 1);
     const int *am = ElemToSide + 2 * (6 * (t[0] - 1) + t[2] - 1);
+    if (am[0] != m + 1 || am[1] != 0) return set_err(FX_EINVAL, "ElemToSide inconsistent at mortar side %d", m + 1);
+    const int ns = type == 1 ? 4 : 2;
+    int *d = mort ? mort->data() + (size_t)MORT_INTS * m : nullptr;
+    if (d) { d[0] = type; d[1] = t[0] - 1; d[2] = t[2]; d[7] = m + 1; }
+    for (int im = 1; im <= 4; im++) {
+      const int sid = mi[2 * im], fl = mi[2 * im + 1];
+      if (im > ns) continue;
+      if (sid <= nMortar || sid > S || fl != 0 || MortarType[2 * (sid - 1)] != 0 || owner[sid - 1] >= 0)
+        return set_err(FX_EINVAL, "mortar side %d: small side %d invalid (id %d, flip %d)", m + 1, im, sid, fl);
+      owner[sid - 1] = m;
+      if (d) d[2 + im] = sid;
+    }
+  }
+  for (int s = nMortar; s < S; s++) {
+    const int *t = SideToElem + 5 * s;
+    const bool small = owner[s] >= 0;
+    if (t[1] < 1 || t[1] > E || t[3] < 1 || t[3] > 6 || t[4] < 1 || t[4] > 4 ||
+        (small ? t[0] != -1 : (t[0] < 1 || t[0] > E || t[2] < 1 || t[2] > 6)))
+      return set_err(FX_EINVAL, "SideToElem(:,%d) invalid (inner sides and the small sides of mortars only)",
+                     s + 1);
     const int *as = ElemToSide + 2 * (6 * (t[1] - 1) + t[3] - 1);
-    if (am[0] != s + 1 || am[1] != 0 || as[0] != s + 1 || as[1] != t[4])
-      return set_err(FX_EINVAL, "ElemToSide inconsistent with SideToElem at side %d", s + 1);
+    if (as[0] != s + 1 || as[1] != t[4]) return set_err(FX_EINVAL, "ElemToSide inconsistent with SideToElem at side %d", s + 1);
+    if (!small) {
+      const int *am = ElemToSide + 2 * (6 * (t[0] - 1) + t[2] - 1);
+      if (am[0] != s + 1 || am[1] != 0)
+        return set_err(FX_EINVAL, "ElemToSide inconsistent with SideToElem at side %d", s + 1);
+    }
   }
   for (int e = 0; e < E; e++)
     for (int l = 0; l < 6; l++) {

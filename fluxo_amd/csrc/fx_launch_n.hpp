@@ -80,10 +80,28 @@ void launch_surface_t(const Batch &b)
   if (nb <= 0) return;
   if (b.cfg.parabolic)
     hipLaunchKernelGGL((k_surface<N, RS, true>), dim3(nb), dim3(T), 0, b.stream, b.U, b.FvN, b.s2e,
-                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
+                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa, b.Umort);
   else
     hipLaunchKernelGGL((k_surface<N, RS, false>), dim3(nb), dim3(T), 0, b.stream, b.U, b.FvN, b.s2e,
-                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa);
+                       b.nvse, b.Flux, b.cfg.nSides, b.cfg.kappa, b.Umort);
+}
+
+template <int N>
+void launch_mortar_u_n(const Batch &b)
+{
+  const long long tot = (long long)b.nMortar * 4 * b.n2;
+  const int nb = (int)((tot + 255) / 256);
+  if (nb > 0)
+    hipLaunchKernelGGL((k_mortar_u<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
+}
+
+template <int N>
+void launch_flux_mortar_n(const Batch &b)
+{
+  const long long tot = (long long)b.nMortar * b.n2;
+  const int nb = (int)((tot + 255) / 256);
+  if (nb > 0)
+    hipLaunchKernelGGL((k_flux_mortar<N>), dim3(nb), dim3(256), 0, b.stream, b.Flux, b.mort, b.mops, b.nMortar);
 }
 
 template <int N>

@@ -3,7 +3,8 @@
 // parallel (fluxo_amd/build.py builds this file once per pair and links the objects with
 // fx_engine.hip into libfluxo_gpu.so).
 //   group 0  split-form volume kernel (every VolumeFlux, mode, precision and tile)
-//   group 1  surface kernel (every Riemann solver), SurfInt alone, NFVSE shock capturing
+//   group 1  surface kernel (every Riemann solver), SurfInt alone, NFVSE shock capturing,
+//            the mortar interpolation and projection
 //   group 2  lifting, CALCTIMESTEP, CalcEntropy, analysis
 //   group 3  GLM-MHD volume, surface and CALCTIMESTEP kernels
 #include <hip/hip_runtime.h>
@@ -26,6 +27,8 @@ template void launch_surface_n<FX_TU_N>(const Batch &);
 template void launch_nfvse_alpha_n<FX_TU_N>(const Batch &);
 template void launch_nfvse_fv_n<FX_TU_N>(const Batch &, const double *);
 template void launch_surfint_n<FX_TU_N>(const Batch &, const double *, const double *);
+template void launch_mortar_u_n<FX_TU_N>(const Batch &);
+template void launch_flux_mortar_n<FX_TU_N>(const Batch &);
 #elif FX_TU_GROUP == 2
 template void launch_lifting_n<FX_TU_N>(const Batch &, int);
 template void launch_dt_n<FX_TU_N>(const Batch &, double, double);

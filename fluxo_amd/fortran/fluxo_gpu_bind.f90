@@ -12,7 +12,7 @@ module fluxo_gpu_bind
   type, bind(C) :: fx_config
     integer(c_int) :: N = 3, nVar = 5, nElems = 0, nSides = 0
     integer(c_int) :: volumeFlux = 0, riemann = 1, parabolic = 0, lifting = 1
-    integer(c_int) :: precision = 0, reserved = 0
+    integer(c_int) :: precision = 0, whichMortar = 0
     real(c_double) :: kappa = 1.4d0, mu0 = 0.0d0, Pr = 0.72d0, etaBR2 = 2.0d0
     integer(c_int) :: nfvse = 0, computeAlpha = 1, spacePropSweeps = 1, reserved3 = 0
     real(c_double) :: alpha_max = 0.5d0, alpha_min = 0.01d0, spacePropFactor = 0.5d0, shockBlendCoef = 0.0d0
@@ -24,6 +24,8 @@ module fluxo_gpu_bind
     type(c_ptr) :: Metrics_fTilde = c_null_ptr, Metrics_gTilde = c_null_ptr, Metrics_hTilde = c_null_ptr
     type(c_ptr) :: sJ = c_null_ptr, NormVec = c_null_ptr, TangVec1 = c_null_ptr, TangVec2 = c_null_ptr
     type(c_ptr) :: SurfElem = c_null_ptr
+    type(c_ptr) :: MortarType = c_null_ptr, MortarInfo = c_null_ptr
+    integer(c_int) :: nMortarSides = 0, reserved_m = 0
   end type fx_mesh
 
   type, bind(C) :: fx_analyze_config
@@ -52,6 +54,13 @@ module fluxo_gpu_bind
       integer(c_int), intent(out) :: e2s(*), s2e(*)
       real(c_double), intent(out) :: NodeCoords(*)
     end function fxio_mesh_read
+    integer(c_int) function fxio_mesh_mortars(file, nMortarSides, MortarType, MortarInfo) &
+        bind(C, name="fxio_mesh_mortars")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), intent(in) :: file(*)
+      integer(c_int), intent(out) :: nMortarSides
+      type(c_ptr), value :: MortarType, MortarInfo
+    end function fxio_mesh_mortars
     integer(c_int) function fxio_state_write(file, project, meshFile, N, nVar, nElems, time, varNames, U) &
         bind(C, name="fxio_state_write")
       import :: c_int, c_char, c_double, c_ptr

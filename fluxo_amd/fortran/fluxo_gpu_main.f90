@@ -19,8 +19,9 @@
 !> (the reference's compile-time PP_Lifting), BR2 with penalty etaBR2 (br2.f90:116, default 2).
 !> DeviceResident (logical, default T): T keeps U on the device and runs whole steps there;
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
-!> MeshFile (optional): a HOPR mesh file (conforming, periodic boundaries; libfluxo_io) instead
-!> of the box generator.  RestartFile (optional): initial state from a state file (Restart,
+!> MeshFile (optional): a HOPR mesh file (periodic boundaries; libfluxo_io) instead of the box
+!> generator; non-conforming meshes (mortars; Euler only) with whichMortar (0 projection, 1
+!> collocation).  RestartFile (optional): initial state from a state file (Restart,
 !> restart.f90:147-232; same number of variables and elements) instead of IniExactFunc, its Time
 !> as the start time; a file of another N is interpolated (lower N) or projected conservatively
 !> through J U (higher N) with the reference's modal Vandermonde matrices.  ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to
@@ -32,7 +33,8 @@ program fluxo_gpu
   use fxh_basis
   use fxh_mesh
   use fxh_readini
-  use fluxo_gpu_bind, only: fx_config, fxio_mesh_info, fxio_mesh_read, fxio_state_write, fxio_state_info, &
+  use fluxo_gpu_bind, only: fx_config, fxio_mesh_info, fxio_mesh_read, fxio_mesh_mortars, fxio_state_write, &
+                            fxio_state_info, &
                             fxio_state_read, fxio_error_text
   use fxh_dg
   implicit none
@@ -48,6 +50,8 @@ program fluxo_gpu
   real(rk) :: xg(0:15), wg(0:15), t0, t1, rate, A(5), b(5), RKc(5), prim(8)
   logical :: finished, lastStep
   integer(c_int), allocatable :: e2s(:, :, :), s2e(:, :)
+  integer(c_int), allocatable, target :: MType(:, :), MInfo(:, :, :)
+  integer(c_int) :: nMortar = 0
   real(rk), allocatable :: NC(:, :, :, :, :), Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
   real(rk), allocatable :: sJm(:, :, :, :), nv(:, :, :, :), t1v(:, :, :, :), t2v(:, :, :, :), se(:, :, :)
   real(rk), allocatable :: xGP(:, :, :, :, :), Uttmp(:, :, :, :, :), DetJacRef(:, :, :, :, :)
@@ -137,6 +141,13 @@ program fluxo_gpu
     call io_chk(fxio_mesh_info(trim(meshFile)//c_null_char, E, S, Ngeo, nodeGL), 'fxio_mesh_info')
     allocate(e2s(2, 6, E), s2e(5, S), NC(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, E))
     call io_chk(fxio_mesh_read(trim(meshFile)//c_null_char, e2s, s2e, NC), 'fxio_mesh_read')
+    ! non-conforming interfaces (mortars) and 'whichMortar' (mortar.f90:88-104)
+    call io_chk(fxio_mesh_mortars(trim(meshFile)//c_null_char, nMortar, c_null_ptr, c_null_ptr), &
+                'fxio_mesh_mortars')
+    allocate(MType(2, S), MInfo(2, 0:4, max(1, nMortar)))
+    call io_chk(fxio_mesh_mortars(trim(meshFile)//c_null_char, nMortar, c_loc(MType), c_loc(MInfo)), &
+                'fxio_mesh_mortars')
+    c%whichMortar = GETINT('whichMortar', '0')
   else
     meshFile = 'box'
     KB = GETINTARRAY('BoxElems', 3)
@@ -154,12 +165,21 @@ program fluxo_gpu
   allocate(Mf(3, 0:n, 0:n, 0:n, E), Mg(3, 0:n, 0:n, 0:n, E), Mh(3, 0:n, 0:n, 0:n, E), xGP(3, 0:n, 0:n, 0:n, E))
   allocate(sJm(0:n, 0:n, 0:n, E), nv(3, 0:n, 0:n, S), t1v(3, 0:n, 0:n, S), t2v(3, 0:n, 0:n, S), se(0:n, 0:n, S))
   allocate(DetJacRef(1, 0:3*Ngeo, 0:3*Ngeo, 0:3*Ngeo, E))
-  call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0, DetJacRef)
+  if (nMortar > 0) then
+    call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0, DetJacRef, &
+                      MType, MInfo)
+  else
+    call calc_metrics(n, Ngeo, E, S, NC, e2s, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, ierr, nodeGL /= 0, DetJacRef)
+  end if
   if (ierr /= 0) then
     write(*, '(A)') ' ERROR: negative or badly scaled Jacobian in the generated mesh'
     stop 2
   end if
-  call InitDG_GPU(c, e2s, s2e, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP)
+  if (nMortar > 0) then
+    call InitDG_GPU(c, e2s, s2e, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP, nMortar, MType, MInfo)
+  else
+    call InitDG_GPU(c, e2s, s2e, Mf, Mg, Mh, sJm, nv, t1v, t2v, se, xGP)
+  end if
   call lgl_nodes(n, xg(0:n), wg(0:n))
   t = 0.0_rk
   if (has_key('RestartFile')) then

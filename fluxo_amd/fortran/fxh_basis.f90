@@ -6,7 +6,10 @@
 !> LegendreGaussNodesAndWeights :430-497, LegGaussLobNodesAndWeights :544-608,
 !> BarycentricWeights :615-634, PolynomialDerivativeMatrix :641-663,
 !> LagrangeInterpolationPolys :732-767, BuildLegendreVdm :196-252),
-!> interpolation.f90 (GetVandermonde :271-339) and changeBasis.f90 (ChangeBasis3D :225-275).
+!> interpolation.f90 (GetVandermonde :271-339), changeBasis.f90 (ChangeBasis3D :225-275) and the
+!> mortar operators of src/mortar/mortar_vars.f90 (MortarBasis_BigToSmall :70-96,
+!> MortarBasis_SmallToBig_Collocation :107-146, MortarBasis_SmallToBig_Projection :153-196,
+!> InterpolateBigToSmall :203-266).
 !==================================================================================================
 module fxh_basis
   use, intrinsic :: iso_c_binding, only: c_double
@@ -15,7 +18,7 @@ module fxh_basis
   integer, parameter, public :: rk = c_double
   integer, parameter, public :: NODE_EQUI = 0, NODE_GL = 1, NODE_GAUSS = 2
   public :: legendre, gauss_nodes, lgl_nodes, nodes_of, barycentric, derivative_matrix
-  public :: lagrange_polys, legendre_vdm, vandermonde, change_basis_3d
+  public :: lagrange_polys, legendre_vdm, vandermonde, change_basis_3d, mortar_ops, interp_big_to_small
 
 contains
 
@@ -323,5 +326,117 @@ contains
       end do
     end do
   end subroutine change_basis_3d
+
+  !> Operators of a non-conforming interface between a big side and its two halves in one
+  !> direction (interval 1 = [-1,0], 2 = [0,1]), on Legendre-Gauss-Lobatto nodes, in the
+  !> reference's transposed layout (InitMortarBase, mortar.f90:63-110):
+  !>   MInt(l,p,k)  = l_l(xi^k_p): big-side Lagrange polynomial l at node p of interval k;
+  !>   MProj(l,p,k): weight of the interval-k node l in big-side node p of the flux projection,
+  !>   whichMortar 0 = L2 projection through the Legendre modes with Gauss quadrature,
+  !>   1 = collocation, M(i,j) = w_j / w_i l_i(xi^k_j).
+  subroutine mortar_ops(N, whichMortar, MInt, MProj)
+    integer, intent(in) :: N, whichMortar
+    real(rk), intent(out) :: MInt(0:N, 0:N, 2), MProj(0:N, 0:N, 2)
+    real(rk) :: x(0:N), w(0:N), wb(0:N), xs(0:N), V(0:N, 0:N, 2)
+    real(rk) :: xg(0:N), wg(0:N), VGP(0:N, 0:N), VL(0:N, 0:N), Vphi(0:N, 0:N), dummy
+    integer :: k, i, j
+    call lgl_nodes(N, x, w)
+    call barycentric(N, x, wb)
+    do k = 1, 2
+      if (k == 1) then
+        xs = 0.5_rk*(x - 1.0_rk)
+      else
+        xs = 0.5_rk*(x + 1.0_rk)
+      end if
+      call nodal_vdm(N, N, wb, x, xs, V(:, :, k))   ! V(p,l) = l_l(xs_p)
+      MInt(:, :, k) = transpose(V(:, :, k))
+    end do
+    if (whichMortar == 1) then
+      do k = 1, 2
+        do i = 0, N
+          do j = 0, N
+            MProj(j, i, k) = w(j)/w(i)*V(j, i, k)
+          end do
+        end do
+      end do
+    else
+      call gauss_nodes(N, xg, wg)
+      call vandermonde(N, NODE_GL, N, NODE_GAUSS, .false., VGP)   ! VGP(g,j) = l_j(xg_g)
+      do i = 0, N
+        VGP(i, :) = wg(i)*VGP(i, :)
+      end do
+      do i = 0, N
+        do j = 0, N
+          call legendre(j, x(i), VL(i, j), dummy)
+        end do
+      end do
+      do k = 1, 2
+        do i = 0, N
+          do j = 0, N
+            if (k == 1) then
+              call legendre(j, 0.5_rk*(xg(i) - 1.0_rk), Vphi(i, j), dummy)
+            else
+              call legendre(j, 0.5_rk*(xg(i) + 1.0_rk), Vphi(i, j), dummy)
+            end if
+          end do
+        end do
+        MProj(:, :, k) = transpose(matmul(VL, matmul(transpose(Vphi), VGP)))
+      end do
+    end if
+  end subroutine mortar_ops
+
+  !> Big side data -> the 4 (type 1) or 2 (type 2: halves in eta, 3: halves in xi) small sides,
+  !> in the reference's order of summation (InterpolateBigToSmall, mortar_vars.f90:203-266).
+  subroutine interp_big_to_small(nd, N, mtype, MInt, Big, Small)
+    integer, intent(in) :: nd, N, mtype
+    real(rk), intent(in) :: MInt(0:N, 0:N, 2), Big(nd, 0:N, 0:N)
+    real(rk), intent(out) :: Small(nd, 0:N, 0:N, 4)
+    real(rk) :: tmp(nd, 0:N, 0:N)
+    integer :: p, q, l, iNb, jNb, im
+    Small = 0.0_rk
+    select case (mtype)
+    case (1)
+      do jNb = 1, 2
+        tmp = 0.0_rk
+        do q = 0, N
+          do p = 0, N
+            do l = 0, N
+              tmp(:, p, q) = tmp(:, p, q) + MInt(l, q, jNb)*Big(:, p, l)
+            end do
+          end do
+        end do
+        do iNb = 1, 2
+          im = iNb + 2*(jNb - 1)
+          do q = 0, N
+            do p = 0, N
+              do l = 0, N
+                Small(:, p, q, im) = Small(:, p, q, im) + MInt(l, p, iNb)*tmp(:, l, q)
+              end do
+            end do
+          end do
+        end do
+      end do
+    case (2)
+      do jNb = 1, 2
+        do q = 0, N
+          do p = 0, N
+            do l = 0, N
+              Small(:, p, q, jNb) = Small(:, p, q, jNb) + MInt(l, q, jNb)*Big(:, p, l)
+            end do
+          end do
+        end do
+      end do
+    case (3)
+      do iNb = 1, 2
+        do q = 0, N
+          do p = 0, N
+            do l = 0, N
+              Small(:, p, q, iNb) = Small(:, p, q, iNb) + MInt(l, p, iNb)*Big(:, l, q)
+            end do
+          end do
+        end do
+      end do
+    end select
+  end subroutine interp_big_to_small
 
 end module fxh_basis

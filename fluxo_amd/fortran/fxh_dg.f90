@@ -54,9 +54,13 @@ contains
 
   !> InitDG + mesh upload: cfgIn carries the parameter.ini options, the mesh arrays are in
   !> the reference layout.
-  subroutine InitDG_GPU(cfgIn, e2s, s2e, Mf, Mg, Mh, sJin, NormVec, TangVec1, TangVec2, SurfElem, xGP)
+  subroutine InitDG_GPU(cfgIn, e2s, s2e, Mf, Mg, Mh, sJin, NormVec, TangVec1, TangVec2, SurfElem, xGP, &
+                        nMortarSides, MortarType, MortarInfo)
     type(fx_config), intent(in) :: cfgIn
     integer(c_int), intent(in), target :: e2s(:, :, :), s2e(:, :)
+    ! non-conforming interfaces (fxio_mesh_mortars): the first nMortarSides sides are big mortar sides
+    integer, intent(in), optional :: nMortarSides
+    integer(c_int), intent(in), target, optional :: MortarType(:, :), MortarInfo(:, :, :)
     real(rk), intent(in), target :: Mf(:, :, :, :, :), Mg(:, :, :, :, :), Mh(:, :, :, :, :)
     real(rk), intent(in), target :: sJin(:, :, :, :), NormVec(:, :, :, :), TangVec1(:, :, :, :)
     real(rk), intent(in), target :: TangVec2(:, :, :, :), SurfElem(:, :, :), xGP(:, :, :, :, :)
@@ -69,6 +73,12 @@ contains
     m%Metrics_fTilde = c_loc(Mf); m%Metrics_gTilde = c_loc(Mg); m%Metrics_hTilde = c_loc(Mh)
     m%sJ = c_loc(sJin); m%NormVec = c_loc(NormVec); m%TangVec1 = c_loc(TangVec1)
     m%TangVec2 = c_loc(TangVec2); m%SurfElem = c_loc(SurfElem)
+    if (present(nMortarSides)) then
+      if (nMortarSides > 0) then
+        m%nMortarSides = nMortarSides
+        m%MortarType = c_loc(MortarType); m%MortarInfo = c_loc(MortarInfo)
+      end if
+    end if
     call chk(fx_mesh_upload(ctx, m), 'fx_mesh_upload')
     allocate(U(PP_nVar, 0:PP_N, 0:PP_N, 0:PP_N, nElems), Ut(PP_nVar, 0:PP_N, 0:PP_N, 0:PP_N, nElems))
     allocate(Elem_xGP(3, 0:PP_N, 0:PP_N, 0:PP_N, nElems), sJ(0:PP_N, 0:PP_N, 0:PP_N, nElems))

@@ -11,14 +11,15 @@
 !>
 !> The metrics restate CalcMetrics (src/mesh/metrics.f90:78-323: curl-form metric terms,
 !> Jacobian on NgeoRef=3*Ngeo projected modally to N) and CalcSurfMetrics/SurfMetricsFromJa
-!> (metrics.f90:334-483) for Legendre-Gauss-Lobatto volume nodes.
+!> (metrics.f90:334-483) for Legendre-Gauss-Lobatto volume nodes, with the small sides of
+!> non-conforming interfaces (Mortar_CalcSurfMetrics, src/mortar/mortar_metrics.f90:60-119).
 !==================================================================================================
 module fxh_mesh
   use, intrinsic :: iso_c_binding
   use fxh_basis
   implicit none
   private
-  public :: box_connectivity, box_nodecoords, calc_metrics, fxh_box_mesh, fxh_metrics
+  public :: box_connectivity, box_nodecoords, calc_metrics, fxh_box_mesh, fxh_metrics, fxh_metrics_nc
 
   integer, parameter :: ZETA_MINUS = 1, ETA_MINUS = 2, XI_PLUS = 3, ETA_PLUS = 4, XI_MINUS = 5, ZETA_PLUS = 6
   integer, parameter :: plus_side(3) = [XI_PLUS, ETA_PLUS, ZETA_PLUS]
@@ -153,11 +154,16 @@ contains
 
   !> CalcMetrics + CalcSurfMetrics; ierr=1 for non-positive or badly scaled Jacobians.
   subroutine calc_metrics(N, Ngeo, nElems, nSides, NodeCoords, e2s, Mf, Mg, Mh, sJ, NormVec, &
-                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr, nodesGL, DetJacRef)
+                          TangVec1, TangVec2, SurfElem, Elem_xGP, ierr, nodesGL, DetJacRef, MortarType, &
+                          MortarInfo)
     integer, intent(in) :: N, Ngeo, nElems, nSides
     logical, intent(in), optional :: nodesGL   ! NodeCoords on Gauss-Lobatto instead of equidistant nodes
     ! the Jacobian on the 3 Ngeo Gauss-Lobatto nodes (DetJac_Ref of metrics.f90), for Restart
     real(rk), intent(out), optional :: DetJacRef(1, 0:3*Ngeo, 0:3*Ngeo, 0:3*Ngeo, nElems)
+    ! non-conforming interfaces (fxio_mesh_mortars): the small sides of a big mortar side get the
+    ! big side's metric terms interpolated to them and scaled by the area ratio
+    ! (Mortar_CalcSurfMetrics, mortar_metrics.f90:60-119)
+    integer, intent(in), optional :: MortarType(2, nSides), MortarInfo(2, 0:4, *)
     real(rk), intent(in) :: NodeCoords(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, nElems)
     integer, intent(in) :: e2s(2, 6, nElems)
     real(rk), intent(out), dimension(3, 0:N, 0:N, 0:N, nElems) :: Mf, Mg, Mh, Elem_xGP
@@ -173,9 +179,12 @@ contains
     real(rk) :: dXgeo(3, 3, 0:Ngeo, 0:Ngeo, 0:Ngeo), dXN(3, 3, 0:N, 0:N, 0:N)
     real(rk) :: R(3, 3, 0:N, 0:N, 0:N), Ja(3, 3, 0:N, 0:N, 0:N), detN(1, 0:N, 0:N, 0:N)
     real(rk) :: sl_geo(3, 0:Ngeo, 0:Ngeo, 0:Ngeo), sl_N(3, 0:N, 0:N, 0:N)
-    real(rk) :: jmin, jmax, ns, se, jn(3), jt(3), nv(3), t1(3), t2(3), dot, tn
+    real(rk) :: jmin, jmax, ns
     real(rk), allocatable :: sl_ref(:, :, :, :)
+    real(rk) :: Jaf(3, 3, 0:N, 0:N), Jas(3, 3, 0:N, 0:N, 4), MInt(0:N, 0:N, 2), MProj(0:N, 0:N, 2)
+    integer :: im, nm, s2
     ierr = 0
+    if (present(MortarType)) call mortar_ops(N, 0, MInt, MProj)
     NgeoRef = 3*Ngeo
     allocate(V_gl_ref(0:NgeoRef, 0:Ngeo), V_ref_N(0:N, 0:NgeoRef))
     allocate(dXref(3, 3, 0:NgeoRef, 0:NgeoRef, 0:NgeoRef), detRef(1, 0:NgeoRef, 0:NgeoRef, 0:NgeoRef))
@@ -296,24 +305,44 @@ contains
             case (ZETA_MINUS); ijk = [pq(1), pq(2), 0]
             case default;      ijk = [pq(1), pq(2), N]
             end select
-            jn = Ja(nd, :, ijk(1), ijk(2), ijk(3))
-            jt = Ja(td, :, ijk(1), ijk(2), ijk(3))
-            se = sqrt(jn(1)*jn(1) + jn(2)*jn(2) + jn(3)*jn(3))
-            nv = ns*jn/se
-            dot = jt(1)*nv(1) + jt(2)*nv(2) + jt(3)*nv(3)
-            t1 = jt - dot*nv
-            tn = sqrt(t1(1)*t1(1) + t1(2)*t1(2) + t1(3)*t1(3))
-            t1 = t1/tn
-            t2 = [nv(2)*t1(3) - nv(3)*t1(2), nv(3)*t1(1) - nv(1)*t1(3), nv(1)*t1(2) - nv(2)*t1(1)]
-            SurfElem(p, q, SideID) = se
-            NormVec(:, p, q, SideID) = nv
-            TangVec1(:, p, q, SideID) = t1
-            TangVec2(:, p, q, SideID) = t2
+            Jaf(:, :, p, q) = Ja(:, :, ijk(1), ijk(2), ijk(3))
+            call surf_from_ja(Jaf(nd, :, p, q), Jaf(td, :, p, q), ns, NormVec(:, p, q, SideID), &
+                              TangVec1(:, p, q, SideID), TangVec2(:, p, q, SideID), SurfElem(p, q, SideID))
+          end do
+        end do
+        if (.not. present(MortarType)) cycle
+        if (MortarType(1, SideID) <= 0) cycle
+        call interp_big_to_small(9, N, MortarType(1, SideID), MInt, Jaf, Jas)
+        nm = merge(4, 2, MortarType(1, SideID) == 1)
+        do im = 1, nm
+          s2 = MortarInfo(1, im, MortarType(2, SideID))
+          if (MortarInfo(2, im, MortarType(2, SideID)) > 0) cycle
+          Jas(:, :, :, :, im) = merge(0.25_rk, 0.5_rk, nm == 4)*Jas(:, :, :, :, im)
+          do q = 0, N
+            do p = 0, N
+              call surf_from_ja(Jas(nd, :, p, q, im), Jas(td, :, p, q, im), ns, NormVec(:, p, q, s2), &
+                                TangVec1(:, p, q, s2), TangVec2(:, p, q, s2), SurfElem(p, q, s2))
+            end do
           end do
         end do
       end do
     end do
   end subroutine calc_metrics
+
+  !> Normal, tangents and surface element of one face point from the face's metric terms in the
+  !> normal (jn) and first tangential (jt) direction (SurfMetricsFromJa, metrics.f90:460-483).
+  subroutine surf_from_ja(jn, jt, ns, nv, t1, t2, se)
+    real(rk), intent(in) :: jn(3), jt(3), ns
+    real(rk), intent(out) :: nv(3), t1(3), t2(3), se
+    real(rk) :: dot, tn
+    se = sqrt(jn(1)*jn(1) + jn(2)*jn(2) + jn(3)*jn(3))
+    nv = ns*jn/se
+    dot = jt(1)*nv(1) + jt(2)*nv(2) + jt(3)*nv(3)
+    t1 = jt - dot*nv
+    tn = sqrt(t1(1)*t1(1) + t1(2)*t1(2) + t1(3)*t1(3))
+    t1 = t1/tn
+    t2 = [nv(2)*t1(3) - nv(3)*t1(2), nv(3)*t1(1) - nv(1)*t1(3), nv(1)*t1(2) - nv(2)*t1(1)]
+  end subroutine surf_from_ja
 
   !> C entry for callers that build a periodic box mesh through the host (Python bench and
   !> tests): connectivity, node coordinates and all metric arrays in the reference layout.
@@ -363,5 +392,28 @@ contains
                       TangVec2, SurfElem, Elem_xGP, ie, nodesGL /= 0)
     ierr = ie
   end subroutine fxh_metrics
+
+  !> fxh_metrics for a mesh with non-conforming interfaces: nSides sides of which the first
+  !> nMortarSides are big mortar sides, MortarType(2,nSides) / MortarInfo(2,0:4,nMortarSides)
+  !> as fxio_mesh_mortars gives them.
+  subroutine fxh_metrics_nc(N, Ngeo, nElems, nSides, nMortarSides, nodesGL, e2s, MortarType, MortarInfo, &
+                            NodeCoords, Mf, Mg, Mh, sJ, NormVec, TangVec1, TangVec2, SurfElem, Elem_xGP, ierr) &
+                            bind(C, name="fxh_metrics_nc")
+    integer(c_int), value :: N, Ngeo, nElems, nSides, nMortarSides, nodesGL
+    integer(c_int), intent(in) :: e2s(2, 6, nElems), MortarType(2, nSides), MortarInfo(2, 0:4, max(1, nMortarSides))
+    real(c_double), intent(in) :: NodeCoords(3, 0:Ngeo, 0:Ngeo, 0:Ngeo, nElems)
+    real(c_double), intent(out), dimension(3, 0:N, 0:N, 0:N, nElems) :: Mf, Mg, Mh, Elem_xGP
+    real(c_double), intent(out) :: sJ(0:N, 0:N, 0:N, nElems)
+    real(c_double), intent(out), dimension(3, 0:N, 0:N, nSides) :: NormVec, TangVec1, TangVec2
+    real(c_double), intent(out) :: SurfElem(0:N, 0:N, nSides)
+    integer(c_int), intent(out) :: ierr
+    integer :: ie
+    if (N < 1 .or. Ngeo < 1 .or. nElems < 1 .or. nSides < 1 .or. nMortarSides < 0) then
+      ierr = -1; return
+    end if
+    call calc_metrics(N, Ngeo, nElems, nSides, NodeCoords, e2s, Mf, Mg, Mh, sJ, NormVec, TangVec1, &
+                      TangVec2, SurfElem, Elem_xGP, ie, nodesGL /= 0, MortarType=MortarType, MortarInfo=MortarInfo)
+    ierr = ie
+  end subroutine fxh_metrics_nc
 
 end module fxh_mesh

@@ -30,6 +30,7 @@ def lib():
         L.fxio_last_error.restype = C.c_char_p
         L.fxio_mesh_info.argtypes = [C.c_char_p, ip, ip, ip, ip]
         L.fxio_mesh_read.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fxio_mesh_mortars.argtypes = [C.c_char_p, ip, C.c_void_p, C.c_void_p]
         L.fxio_state_write.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.POINTER(C.c_char_p), C.c_void_p]
         L.fxio_state_info.argtypes = [C.c_char_p, ip, ip, ip, dp]
@@ -51,14 +52,22 @@ def mesh_info(path):
 
 
 def read_mesh(path):
-    """Connectivity and node coordinates of a conforming, periodic HOPR mesh."""
+    """Connectivity and node coordinates of a periodic HOPR mesh, conforming or with mortars:
+    MortarType (nSides, 2) and MortarInfo (nMortarSides, 5, 2) as fxio_mesh_mortars gives
+    them (big mortar sides are sides 1..nMortarSides)."""
     E, S, Ngeo, gl = mesh_info(path)
     ng = Ngeo + 1
     e2s = np.zeros((E, 6, 2), np.int32)
     s2e = np.zeros((S, 5), np.int32)
     X = np.zeros((E, ng, ng, ng, 3))
     _chk(lib().fxio_mesh_read(os.fsencode(path), e2s.ctypes.data, s2e.ctypes.data, X.ctypes.data))
-    return dict(nElems=E, nSides=S, Ngeo=Ngeo, nodes_gl=gl, ElemToSide=e2s, SideToElem=s2e, NodeCoords=X)
+    nm = C.c_int()
+    _chk(lib().fxio_mesh_mortars(os.fsencode(path), C.byref(nm), None, None))
+    mt = np.zeros((S, 2), np.int32)
+    mi = np.zeros((nm.value, 5, 2), np.int32)
+    _chk(lib().fxio_mesh_mortars(os.fsencode(path), C.byref(nm), mt.ctypes.data, mi.ctypes.data))
+    return dict(nElems=E, nSides=S, Ngeo=Ngeo, nodes_gl=gl, ElemToSide=e2s, SideToElem=s2e, NodeCoords=X,
+                nMortarSides=nm.value, MortarType=mt, MortarInfo=mi)
 
 
 def write_state(path, U, time, mesh_file="", project="FLUXO_AMD"):

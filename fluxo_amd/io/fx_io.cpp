@@ -156,6 +156,175 @@ int open_hopr(const char *file, Hopr &m, bool withData)
   return FXIO_OK;
 }
 
+// Connectivity of a HOPR mesh as the reference builds it: ReadMesh (mesh_readin.f90:299-440)
+// reads each element's six sides, a big mortar side (negative SIDE_nbElemID: MortarType
+// 1 = 1->4, 2 = 1->2 in eta, 3 = 1->2 in xi) followed by its 4 or 2 small virtual sides;
+// setLocalSideIDs (prepare_mesh.f90:133-227) numbers the big mortar sides first and then
+// the inner sides, in element / local side / mortar order; fillMeshInfo
+// (prepare_mesh.f90:662-726) fills ElemToSide, SideToElem, MortarType and MortarInfo.
+struct HSide {
+  int elem = -1, loc = 0;      // owning element (0-based), local side 1..6 (0: virtual)
+  int ind = 0, flip = 0;       // global side index, flip
+  int nbElem = 0, bc = 0;      // SIDE_nbElemID, SIDE_BCID
+  int mortarType = 0;          // >0 big mortar side, -10 small side of an element at a mortar
+  int nMortars = 0, first = -1;  // small virtual sides: objects first .. first+nMortars-1
+  int conn = -1, sideID = -1;
+};
+
+struct Conn {
+  int E = 0, nSides = 0, nMortar = 0;
+  std::vector<int> e2s, s2e, mortarType, mortarInfo;
+};
+
+int build_conn(const Hopr &m, Conn &c)
+{
+  const int E = m.nElems;
+  const long long nSideRows = (long long)m.sideInfo.size() / 5;
+  std::vector<HSide> sd;
+  std::vector<int> loc(6 * E);  // object of each element's local side
+  sd.reserve(6 * E);
+  for (int e = 0; e < E; e++) {
+    const int *ei = &m.elemInfo[6 * (size_t)e];
+    const int type = ei[ELEM_TYPE];
+    if (type != 108 && type != 118 && type != 208)
+      return fail(FXIO_EUNSUPPORTED, "element %d has type %d; only hexahedra (108, 118, 208)", e + 1, type);
+    long long row = ei[ELEM_FIRSTSIDE];
+    auto next = [&](const int *&si) -> bool {
+      if (row < 0 || row >= nSideRows || row >= ei[ELEM_LASTSIDE]) return false;
+      si = &m.sideInfo[5 * (size_t)row++];
+      return true;
+    };
+    for (int l = 0; l < 6; l++) {
+      const int *si;
+      if (!next(si)) return fail(FXIO_EFORMAT, "element %d: side rows end before local side %d", e + 1, l + 1);
+      HSide a;
+      a.elem = e; a.loc = l + 1; a.nbElem = si[SIDE_NBELEM]; a.bc = si[SIDE_BCID];
+      if (si[SIDE_NBELEM] < 0) {
+        a.mortarType = -si[SIDE_NBELEM];
+        if (a.mortarType > 3) return fail(FXIO_EFORMAT, "element %d side %d: mortar type %d", e + 1, l + 1, a.mortarType);
+        a.nMortars = a.mortarType == 1 ? 4 : 2;
+      }
+      if (si[SIDE_TYPE] < 0) a.mortarType = -10;
+      if (a.mortarType <= 0) {
+        a.ind = std::abs(si[SIDE_ID]);
+        a.flip = si[SIDE_ID] > 0 ? 0 : si[SIDE_FLIP] % 10;
+        if (a.flip < 0 || a.flip > 4) return fail(FXIO_EFORMAT, "element %d side %d: flip %d", e + 1, l + 1, a.flip);
+      }
+      loc[6 * e + l] = (int)sd.size();
+      sd.push_back(a);
+      if (a.nMortars > 0) {
+        sd[loc[6 * e + l]].first = (int)sd.size();
+        for (int im = 0; im < a.nMortars; im++) {
+          if (!next(si)) return fail(FXIO_EFORMAT, "element %d side %d: small mortar side %d missing", e + 1, l + 1, im + 1);
+          if (si[SIDE_ID] < 0)
+            return fail(FXIO_EFORMAT, "element %d side %d: small mortar side %d is not oriented (flip 0)", e + 1, l + 1,
+                        im + 1);
+          HSide v;
+          v.elem = e; v.ind = si[SIDE_ID]; v.nbElem = si[SIDE_NBELEM]; v.bc = si[SIDE_BCID];
+          sd.push_back(v);
+        }
+      }
+    }
+    if (row != ei[ELEM_LASTSIDE])
+      return fail(FXIO_EFORMAT, "element %d: %lld side rows, the six sides and their mortars use %lld", e + 1,
+                  (long long)ei[ELEM_LASTSIDE] - ei[ELEM_FIRSTSIDE], row - ei[ELEM_FIRSTSIDE]);
+  }
+  // sides of element e, local side l, mortar im (0: the side itself), in the reference's order
+  auto each = [&](auto &&f) {
+    for (int e = 0; e < E; e++)
+      for (int l = 0; l < 6; l++) {
+        const int k = loc[6 * e + l];
+        f(k);
+        for (int im = 0; im < sd[k].nMortars; im++) f(sd[k].first + im);
+      }
+  };
+  int err = FXIO_OK;
+  // connections (mesh_readin.f90:368-440): the neighbour element's side or small virtual side
+  // with the same index; periodic boundaries connect like inner sides
+  each([&](int k) {
+    HSide &a = sd[k];
+    if (err) return;
+    if (a.bc != 0) {
+      if (a.bc < 1 || (size_t)a.bc > m.nBCs) { err = fail(FXIO_EFORMAT, "element %d: BC index %d", a.elem + 1, a.bc); return; }
+      const int bcType = m.bcType[4 * (size_t)(a.bc - 1)];
+      if (bcType != 1 && bcType != 100) {
+        err = fail(FXIO_EUNSUPPORTED, "element %d is on boundary %d of type %d: only periodic boundaries are "
+                   "supported", a.elem + 1, a.bc, bcType);
+        return;
+      }
+    }
+    if (a.mortarType > 0 || a.conn >= 0) return;
+    if (a.nbElem < 1 || a.nbElem > E) {
+      err = fail(FXIO_EUNSUPPORTED, "element %d side %d has no neighbour element: boundaries other than "
+                 "periodic are not supported", a.elem + 1, a.loc);
+      return;
+    }
+    const int nb = a.nbElem - 1;
+    for (int l = 0; l < 6 && a.conn < 0; l++) {
+      const int kb = loc[6 * nb + l];
+      for (int im = -1; im < sd[kb].nMortars; im++) {
+        const int kk = im < 0 ? kb : sd[kb].first + im;
+        if (kk != k && sd[kk].mortarType <= 0 && sd[kk].ind == a.ind) { a.conn = kk; sd[kk].conn = k; break; }
+      }
+    }
+    if (a.conn < 0) err = fail(FXIO_EFORMAT, "element %d side %d: neighbour side not found", a.elem + 1, a.loc);
+  });
+  if (err) return err;
+  // side ids (setLocalSideIDs): big mortar sides 1..nMortar, then the inner sides
+  int nMortar = 0;
+  for (const HSide &a : sd) nMortar += a.mortarType > 0;
+  int iMortar = 0, iInner = nMortar;
+  each([&](int k) {
+    HSide &a = sd[k];
+    if (a.sideID >= 0) return;
+    if (a.conn >= 0) a.sideID = sd[a.conn].sideID = ++iInner;
+    else a.sideID = ++iMortar;  // every unconnected side is a big mortar side here
+  });
+  const int nS = iInner;
+  c.E = E; c.nSides = nS; c.nMortar = nMortar;
+  c.e2s.assign(12 * (size_t)E, 0);
+  c.s2e.assign(5 * (size_t)nS, -1);
+  c.mortarType.assign(2 * (size_t)nS, -1);
+  c.mortarInfo.assign(10 * (size_t)nMortar, -1);
+  // fillMeshInfo
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 6; l++) {
+      const HSide &a = sd[loc[6 * e + l]];
+      c.e2s[12 * (size_t)e + 2 * l] = a.sideID;
+      c.e2s[12 * (size_t)e + 2 * l + 1] = a.flip;
+      int *t = &c.s2e[5 * (size_t)(a.sideID - 1)];
+      if (a.flip == 0) {
+        if (t[0] != -1) return fail(FXIO_EFORMAT, "side %d has two master elements", a.sideID);
+        t[0] = e + 1; t[2] = l + 1;
+      } else {
+        if (t[1] != -1) return fail(FXIO_EFORMAT, "side %d has two slave elements", a.sideID);
+        t[1] = e + 1; t[3] = l + 1; t[4] = a.flip;
+      }
+      if (a.nMortars > 0) {
+        const int ms = a.sideID;  // 1..nMortar: its own index in MortarInfo
+        c.mortarType[2 * (size_t)(ms - 1)] = a.mortarType;
+        c.mortarType[2 * (size_t)(ms - 1) + 1] = ms;
+        int *mi = &c.mortarInfo[10 * (size_t)(ms - 1)];
+        mi[0] = ms; mi[1] = 0;
+        for (int im = 0; im < a.nMortars; im++) {
+          const HSide &v = sd[a.first + im];
+          c.mortarType[2 * (size_t)(v.sideID - 1)] = 0;
+          mi[2 * (im + 1)] = v.sideID;
+          mi[2 * (im + 1) + 1] = v.flip;
+        }
+      }
+    }
+  // every inner side has its slave element; a conforming side its master, a small mortar side
+  // none (its master data come from the big side)
+  for (int s = nMortar; s < nS; s++) {
+    const int *t = &c.s2e[5 * (size_t)s];
+    if (t[1] < 1) return fail(FXIO_EFORMAT, "side %d lacks a slave element (orientation flags)", s + 1);
+    if (t[0] < 1 && c.mortarType[2 * (size_t)s] != 0)
+      return fail(FXIO_EFORMAT, "side %d lacks a master element (orientation flags)", s + 1);
+  }
+  return FXIO_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -165,10 +334,12 @@ const char *fxio_last_error(void) { return g_err; }
 int fxio_mesh_info(const char *file, int *nElems, int *nSides, int *Ngeo, int *nodeTypeGL)
 {
   Hopr m;
-  int rc = open_hopr(file, m, false);
+  Conn c;
+  int rc = open_hopr(file, m, true);
+  if (!rc) rc = build_conn(m, c);
   if (rc) return rc;
   if (nElems) *nElems = m.nElems;
-  if (nSides) *nSides = 3 * m.nElems;
+  if (nSides) *nSides = c.nSides;
   if (Ngeo) *Ngeo = m.Ngeo;
   if (nodeTypeGL) *nodeTypeGL = m.gl;
   return FXIO_OK;
@@ -178,83 +349,38 @@ int fxio_mesh_read(const char *file, int *ElemToSide, int *SideToElem, double *N
 {
   if (!ElemToSide || !SideToElem || !NodeCoords) return fail(FXIO_EINVAL, "null output array");
   Hopr m;
+  Conn c;
   int rc = open_hopr(file, m, true);
+  if (!rc) rc = build_conn(m, c);
   if (rc) return rc;
   const int E = m.nElems, ng = m.Ngeo + 1, ng3 = ng * ng * ng;
-  const long long nSideRows = (long long)m.sideInfo.size() / 5, nNodeRows = (long long)m.nodes.size() / 3;
-  // element sides (ReadMesh, mesh_readin.f90:299-364): global side index, flip, neighbour
-  std::vector<int> ind(6 * E), flip(6 * E), nbElem(6 * E), sideID(6 * E, -1), conn(6 * E, -1);
+  const long long nNodeRows = (long long)m.nodes.size() / 3;
   for (int e = 0; e < E; e++) {
     const int *ei = &m.elemInfo[6 * (size_t)e];
-    const int type = ei[ELEM_TYPE];
-    if (type != 108 && type != 118 && type != 208)
-      return fail(FXIO_EUNSUPPORTED, "element %d has type %d; only hexahedra (108, 118, 208)", e + 1, type);
-    if (ei[ELEM_LASTSIDE] - ei[ELEM_FIRSTSIDE] != 6)
-      return fail(FXIO_EUNSUPPORTED, "element %d has %d side entries: mortar sides are not supported", e + 1,
-                  ei[ELEM_LASTSIDE] - ei[ELEM_FIRSTSIDE]);
     if (ei[ELEM_LASTNODE] - ei[ELEM_FIRSTNODE] != ng3 || ei[ELEM_FIRSTNODE] < 0 || ei[ELEM_LASTNODE] > nNodeRows)
       return fail(FXIO_EFORMAT, "element %d: node range [%d,%d) is not (Ngeo+1)^3 = %d nodes", e + 1,
                   ei[ELEM_FIRSTNODE], ei[ELEM_LASTNODE], ng3);
-    for (int l = 0; l < 6; l++) {
-      const long long row = (long long)ei[ELEM_FIRSTSIDE] + l;
-      if (row < 0 || row >= nSideRows) return fail(FXIO_EFORMAT, "element %d: side row %lld out of range", e + 1, row);
-      const int *si = &m.sideInfo[5 * (size_t)row];
-      const int k = 6 * e + l;
-      if (si[SIDE_NBELEM] < 0 || si[SIDE_TYPE] < 0)
-        return fail(FXIO_EUNSUPPORTED, "element %d side %d belongs to a mortar: not supported", e + 1, l + 1);
-      ind[k] = std::abs(si[SIDE_ID]);
-      flip[k] = si[SIDE_ID] > 0 ? 0 : si[SIDE_FLIP] % 10;
-      if (flip[k] < 0 || flip[k] > 4) return fail(FXIO_EFORMAT, "element %d side %d: flip %d", e + 1, l + 1, flip[k]);
-      const int bc = si[SIDE_BCID];
-      if (bc != 0) {
-        if (bc < 1 || (size_t)bc > m.nBCs) return fail(FXIO_EFORMAT, "element %d side %d: BC index %d", e + 1, l + 1, bc);
-        const int bcType = m.bcType[4 * (size_t)(bc - 1)];
-        if (bcType != 1 && bcType != 100)
-          return fail(FXIO_EUNSUPPORTED, "element %d side %d is on boundary %d of type %d: only periodic "
-                      "boundaries are supported", e + 1, l + 1, bc, bcType);
-      }
-      nbElem[k] = si[SIDE_NBELEM];
-      if (nbElem[k] < 1 || nbElem[k] > E)
-        return fail(FXIO_EUNSUPPORTED, "element %d side %d has no neighbour element: boundaries other than "
-                    "periodic are not supported", e + 1, l + 1);
-    }
   }
-  // side connection (mesh_readin.f90:368-440): the neighbour's side with the same index
-  for (int k = 0; k < 6 * E; k++) {
-    if (conn[k] >= 0) continue;
-    const int nb = nbElem[k] - 1;
-    for (int l = 0; l < 6; l++) {
-      const int kk = 6 * nb + l;
-      if (kk != k && ind[kk] == ind[k] && conn[kk] < 0) { conn[k] = kk; conn[kk] = k; break; }
-    }
-    if (conn[k] < 0) return fail(FXIO_EFORMAT, "element %d side %d: neighbour side not found", k / 6 + 1, k % 6 + 1);
-  }
-  // SideIDs of inner sides in element / local-side order (setLocalSideIDs, prepare_mesh.f90:157-200)
-  int nS = 0;
-  for (int k = 0; k < 6 * E; k++)
-    if (sideID[k] < 0) { sideID[k] = sideID[conn[k]] = ++nS; }
-  if (nS != 3 * E) return fail(FXIO_EFORMAT, "%d sides for %d elements (expected 3 per element)", nS, E);
-  // ElemToSide / SideToElem (fillMeshInfo, prepare_mesh.f90:662-700)
-  for (int s = 0; s < 5 * nS; s++) SideToElem[s] = -1;
-  for (int k = 0; k < 6 * E; k++) {
-    ElemToSide[2 * k] = sideID[k];
-    ElemToSide[2 * k + 1] = flip[k];
-    int *t = SideToElem + 5 * (size_t)(sideID[k] - 1);
-    if (flip[k] == 0) {
-      if (t[0] != -1) return fail(FXIO_EFORMAT, "side %d has two master elements", sideID[k]);
-      t[0] = k / 6 + 1; t[2] = k % 6 + 1;
-    } else {
-      if (t[1] != -1) return fail(FXIO_EFORMAT, "side %d has two slave elements", sideID[k]);
-      t[1] = k / 6 + 1; t[3] = k % 6 + 1; t[4] = flip[k];
-    }
-  }
-  for (int s = 0; s < nS; s++)
-    if (SideToElem[5 * s] < 1 || SideToElem[5 * s + 1] < 1)
-      return fail(FXIO_EFORMAT, "side %d lacks a master or a slave element (orientation flags)", s + 1);
+  std::memcpy(ElemToSide, c.e2s.data(), sizeof(int) * c.e2s.size());
+  std::memcpy(SideToElem, c.s2e.data(), sizeof(int) * c.s2e.size());
   // NodeCoords(3,0:Ngeo,0:Ngeo,0:Ngeo,nElems) (mesh_readin.f90:461-464)
   for (int e = 0; e < E; e++)
     std::memcpy(NodeCoords + (size_t)3 * ng3 * e, &m.nodes[3 * (size_t)m.elemInfo[6 * (size_t)e + ELEM_FIRSTNODE]],
                 sizeof(double) * 3 * ng3);
+  return FXIO_OK;
+}
+
+int fxio_mesh_mortars(const char *file, int *nMortarSides, int *MortarType, int *MortarInfo)
+{
+  if (!nMortarSides) return fail(FXIO_EINVAL, "null nMortarSides");
+  Hopr m;
+  Conn c;
+  int rc = open_hopr(file, m, true);
+  if (!rc) rc = build_conn(m, c);
+  if (rc) return rc;
+  *nMortarSides = c.nMortar;
+  if (MortarType) std::memcpy(MortarType, c.mortarType.data(), sizeof(int) * c.mortarType.size());
+  if (MortarInfo) std::memcpy(MortarInfo, c.mortarInfo.data(), sizeof(int) * c.mortarInfo.size());
   return FXIO_OK;
 }
 

@@ -29,6 +29,8 @@ def _lib():
         L.fxh_box_mesh.restype = None
         L.fxh_metrics.argtypes = [ci, ci, ci, ci, ip, dp] + [dp] * 9 + [C.POINTER(ci)]
         L.fxh_metrics.restype = None
+        L.fxh_metrics_nc.argtypes = [ci, ci, ci, ci, ci, ci, ip, ip, ip, dp] + [dp] * 9 + [C.POINTER(ci)]
+        L.fxh_metrics_nc.restype = None
         _LIB = L
     return _LIB
 
@@ -58,8 +60,11 @@ def box_mesh(K, N, x0=(0.0, 0.0, 0.0), length=(2.0, 2.0, 2.0), Ngeo=1, warp=Fals
 
 
 def hopr_mesh(path, N):
-    """A conforming, periodic HOPR mesh file (the reference's MeshFile) for polynomial degree
-    N: the reader's connectivity and node coordinates, the host's CalcMetrics/CalcSurfMetrics."""
+    """A periodic HOPR mesh file (the reference's MeshFile), conforming or with mortars, for
+    polynomial degree N: the reader's connectivity and node coordinates, the host's
+    CalcMetrics/CalcSurfMetrics (with Mortar_CalcSurfMetrics on the small sides of a mortar).
+    With mortars the dict also holds nMortarSides, MortarType (nSides, 2) and MortarInfo
+    (nMortarSides, 5, 2)."""
     from fluxo_amd import io
     r = io.read_mesh(path)
     E, S, Ngeo, n = r["nElems"], r["nSides"], r["Ngeo"], N + 1
@@ -72,9 +77,17 @@ def hopr_mesh(path, N):
         m[k] = np.zeros((S, n, n, 3))
     m["SurfElem"] = np.zeros((S, n, n))
     ierr = C.c_int(0)
-    _lib().fxh_metrics(N, Ngeo, E, int(r["nodes_gl"]), m["ElemToSide"], np.ascontiguousarray(m["NodeCoords"]),
-                       m["Mf"], m["Mg"], m["Mh"], m["sJ"], m["NormVec"], m["TangVec1"], m["TangVec2"],
-                       m["SurfElem"], m["Elem_xGP"], C.byref(ierr))
+    nm = r["nMortarSides"]
+    if nm == 0:
+        _lib().fxh_metrics(N, Ngeo, E, int(r["nodes_gl"]), m["ElemToSide"], np.ascontiguousarray(m["NodeCoords"]),
+                           m["Mf"], m["Mg"], m["Mh"], m["sJ"], m["NormVec"], m["TangVec1"], m["TangVec2"],
+                           m["SurfElem"], m["Elem_xGP"], C.byref(ierr))
+    else:
+        m.update(nMortarSides=nm, MortarType=r["MortarType"], MortarInfo=r["MortarInfo"])
+        _lib().fxh_metrics_nc(N, Ngeo, E, S, nm, int(r["nodes_gl"]), m["ElemToSide"], r["MortarType"],
+                              np.ascontiguousarray(r["MortarInfo"]), np.ascontiguousarray(m["NodeCoords"]),
+                              m["Mf"], m["Mg"], m["Mh"], m["sJ"], m["NormVec"], m["TangVec1"], m["TangVec2"],
+                              m["SurfElem"], m["Elem_xGP"], C.byref(ierr))
     if ierr.value != 0:
         raise ValueError(f"{path}: metrics failed (ierr={ierr.value}): non-positive or badly scaled Jacobian")
     return m

@@ -29,7 +29,7 @@ class FxError(RuntimeError):
 class fx_config(C.Structure):
     _fields_ = [("N", C.c_int), ("nVar", C.c_int), ("nElems", C.c_int), ("nSides", C.c_int),
                 ("volumeFlux", C.c_int), ("riemann", C.c_int), ("parabolic", C.c_int),
-                ("lifting", C.c_int), ("precision", C.c_int), ("reserved", C.c_int),
+                ("lifting", C.c_int), ("precision", C.c_int), ("whichMortar", C.c_int),
                 ("kappa", C.c_double), ("mu0", C.c_double), ("Pr", C.c_double),
                 ("etaBR2", C.c_double),
                 ("nfvse", C.c_int), ("computeAlpha", C.c_int), ("spacePropSweeps", C.c_int),
@@ -41,7 +41,8 @@ class fx_config(C.Structure):
 class fx_mesh(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("ElemToSide", "SideToElem", "Metrics_fTilde",
                                           "Metrics_gTilde", "Metrics_hTilde", "sJ", "NormVec",
-                                          "TangVec1", "TangVec2", "SurfElem")]
+                                          "TangVec1", "TangVec2", "SurfElem", "MortarType", "MortarInfo")] + [
+        ("nMortarSides", C.c_int), ("reserved_m", C.c_int)]
 
 
 class fx_server_config(C.Structure):
@@ -169,11 +170,11 @@ def _ptr(a):
 
 def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, kappa=1.4,
                 mu0=0.0, Pr=0.72, precision=FX_PREC_FP64, nfvse=False, alpha_max=0.5, alpha_min=0.01,
-                sweeps=1, prop_factor=0.5, fixed_alpha=None, lifting=1, etaBR2=2.0):
+                sweeps=1, prop_factor=0.5, fixed_alpha=None, lifting=1, etaBR2=2.0, which_mortar=0):
     """Engine options; the NFVSE ones follow the reference's parameter names (alpha_max,
     alpha_min, SpacePropSweeps, SpacePropFactor; fixed_alpha = ComputeAlpha 3 with that
     ShockBlendCoef, None = Persson-Peraire indicator); lifting 1 BR1, 2 BR2 (penalty
-    etaBR2)."""
+    etaBR2); which_mortar = whichMortar on meshes with mortars (0 projection, 1 collocation)."""
     c = fx_config()
     c.mu_0, c.GLM_scale = 1.0, 0.5
     c.nfvse, c.spacePropSweeps = int(bool(nfvse)), sweeps
@@ -184,6 +185,7 @@ def make_config(N, nElems, nSides, volume_flux=32, riemann=1, parabolic=False, k
     c.volumeFlux, c.riemann, c.parabolic, c.lifting = volume_flux, riemann, int(bool(parabolic)), lifting
     c.etaBR2 = etaBR2
     c.precision = precision
+    c.whichMortar = which_mortar
     c.kappa, c.mu0, c.Pr = kappa, mu0, Pr
     return c
 
@@ -199,7 +201,8 @@ _MESH_KEYS = (("ElemToSide", "ElemToSide"), ("SideToElem", "SideToElem"),
 
 
 def _mesh_struct(mesh):
-    """fx_mesh pointing at contiguous copies of the mesh arrays (returned to keep alive)."""
+    """fx_mesh pointing at contiguous copies of the mesh arrays (returned to keep alive), with
+    the mortar arrays of a mesh that has them (nMortarSides, MortarType, MortarInfo)."""
     m = fx_mesh()
     keep = {}
     for field, key in _MESH_KEYS:
@@ -207,6 +210,12 @@ def _mesh_struct(mesh):
         a = np.ascontiguousarray(mesh[key], dtype=dt)
         keep[key] = a
         setattr(m, field, _ptr(a))
+    nm = int(mesh.get("nMortarSides", 0))
+    if nm:
+        for key in ("MortarType", "MortarInfo"):
+            keep[key] = np.ascontiguousarray(mesh[key], dtype=np.int32)
+            setattr(m, key, _ptr(keep[key]))
+        m.nMortarSides = nm
     return m, keep
 
 

@@ -39,7 +39,7 @@ typedef struct {
   int N;            /* polynomial degree, 'N' (interpolation.f90:125-126); 2..5 */
   int nVar;         /* PP_nVar: 5 (Euler / Navier-Stokes) or 9 (ideal GLM-MHD) */
   int nElems;       /* local elements */
-  int nSides;       /* sides (all inner sides on periodic conforming meshes) */
+  int nSides;       /* sides: the inner sides, after the big mortar sides of a mesh with mortars */
   int volumeFlux;   /* 'VolumeFlux' (navierstokes/equation.f90:414-460): 0 standard DG, 1
                        standard DG with averaged metrics, 2 / 8 Kennedy-Gruber, 3 Ducros,
                        4 Morinishi, 5 Chandrashekar, 6 EC with arithmetic means, 7 gg,
@@ -61,7 +61,8 @@ typedef struct {
   int lifting;      /* PP_Lifting with entropy-variable lifting: 1 = BR1, 2 = BR2
                        (src/dg/br1, src/dg/br2; Navier-Stokes and resistive GLM-MHD) */
   int precision;    /* FX_PREC_FP64 or FX_PREC_FP32_SPLIT */
-  int reserved;
+  int whichMortar;  /* 'whichMortar' (mortar.f90:88-104), meshes with mortars: 0 L2 projection of
+                       the small sides' fluxes, 1 collocation */
   double kappa;     /* 'kappa' (equation.f90:218) */
   double mu0;       /* constant dynamic viscosity: 'mu0' (navierstokes/equation.f90:229-230),
                        'mu' for MHD (mhd/equation.f90:190) */
@@ -107,6 +108,15 @@ typedef struct {
   const double *TangVec1;         /* (3,0:N,0:N,nSides) (accepted for interface parity) */
   const double *TangVec2;         /* (3,0:N,0:N,nSides) (accepted for interface parity) */
   const double *SurfElem;         /* (0:N,0:N,nSides) */
+  /* Non-conforming interfaces (fillMeshInfo prepare_mesh.f90:700-718; fxio_mesh_mortars):
+     the sides 1..nMortarSides are big mortar sides, each with 4 (MortarType 1) or 2 (2: halves
+     in eta, 3: halves in xi) small sides, whose master data are interpolated from the big
+     side (U_Mortar) and whose fluxes are projected back onto it (Flux_Mortar).  NULL / 0 on
+     a conforming mesh.  Euler only (nVar 5, parabolic 0, nfvse 0). */
+  const int *MortarType;          /* (2,nSides) */
+  const int *MortarInfo;          /* (2,0:4,nMortarSides) */
+  int nMortarSides;
+  int reserved_m;
 } fx_mesh;
 
 typedef struct fx_ctx fx_ctx;

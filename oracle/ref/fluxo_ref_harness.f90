@@ -16,6 +16,10 @@
 !> (FillFlux, src/dg/fillflux.f90:41-110) and, with shock capturing, VolInt_NFVSE's blended
 !> volume integral (src/shockcapturing/nfvse.f90:475-561) of the same state.
 !>
+!> A mesh with non-conforming interfaces brings MortarType / MortarInfo in the input (the reference's
+!> fillMeshInfo arrays, prepare_mesh.f90:700-718); InitMortarBase, CalcMetrics' mortar branch and
+!> U_Mortar / Flux_Mortar in DGTimeDerivative are then the reference's.
+!>
 !> usage: fluxo_ref_harness parameter.ini input.bin output.bin
 !==================================================================================================
 #include "defines.h"
@@ -92,10 +96,12 @@ program fluxo_ref_harness
   if (PP_N .ne. hdr(1)) stop 'N of parameter file and input differ'
   call InitMortarBase()
 
-  ! ---- the part of InitMesh that follows readMesh: periodic conforming mesh, inner sides only
+  ! ---- the part of InitMesh that follows readMesh: periodic mesh; hdr(7) big mortar sides come
+  ! first (sides 1..hdr(7)), then the inner sides (setLocalSideIDs, prepare_mesh.f90:186-189)
   nGlobalElems = nElems; offsetElem = 0
-  nBCSides = 0; nMortarInnerSides = 0; nInnerSides = nSides; nMPISides_MINE = 0; nMPISides_YOUR = 0
-  nMPISides = 0; nMortarMPISides = 0; nMortarSides = 0; nBCs = 0; nUserBCs = 0; nAnalyzeSides = 0
+  nMortarInnerSides = hdr(7); nMortarSides = hdr(7)
+  nBCSides = 0; nInnerSides = nSides - nMortarInnerSides; nMPISides_MINE = 0; nMPISides_YOUR = 0
+  nMPISides = 0; nMortarMPISides = 0; nBCs = 0; nUserBCs = 0; nAnalyzeSides = 0
   firstBCSide = 1
   firstMortarInnerSide = firstBCSide + nBCSides
   firstInnerSide = firstMortarInnerSide + nMortarInnerSides
@@ -120,6 +126,10 @@ program fluxo_ref_harness
   read(u_in) NodeCoords
   read(u_in) ElemToSide
   read(u_in) SideToElem
+  if (nMortarSides > 0) then
+    read(u_in) MortarType
+    read(u_in) MortarInfo
+  end if
   call buildMappings(PP_N, V2S=V2S, V2S2=V2S2, S2V=S2V, S2V2=S2V2, S2V3=S2V3, CS2V2=CS2V2, FS2M=FS2M)
   allocate(Vdm_GLN_N(0:PP_N, 0:PP_N), Vdm_N_GLN(0:PP_N, 0:PP_N))
   call GetVandermonde(PP_N, NodeTypeGL, PP_N, NodeType, Vdm_GLN_N, Vdm_N_GLN, modal=.false.)
@@ -178,8 +188,8 @@ program fluxo_ref_harness
     write(u_an, '(A,1X,ES25.17)') trim(A2F_VarNames(iv)), A2F_Data(iv)
   end do
   close(u_an)
-  ! the face fluxes of FillFlux from that right-hand side (module arrays of MOD_DG_Vars; all sides
-  ! are inner sides, so Flux_slave spans 1:nSides)
+  ! the face fluxes of FillFlux and Flux_Mortar from that right-hand side (module arrays of
+  ! MOD_DG_Vars; Flux_slave spans the inner sides firstSlaveSide:lastSlaveSide)
   write(u_out) Flux_master, Flux_slave
 #if SHOCK_NFVSE
   ! VolInt followed by VolInt_NFVSE alone: blending coefficient, volume split form, blend

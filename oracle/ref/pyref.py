@@ -54,9 +54,16 @@ def read_analysis(d):
     return out
 
 
+def _mortar_input(mesh, f):
+    """MortarType / MortarInfo of a mesh with non-conforming interfaces after the connectivity."""
+    if mesh.get("nMortarSides", 0):
+        np.ascontiguousarray(mesh["MortarType"], np.int32).tofile(f)
+        np.ascontiguousarray(mesh["MortarInfo"], np.int32).tofile(f)
+
+
 def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=0.72,
         cfl=0.9, dfl=0.9, nsteps=0, t0=0.0, timeout=600, nfvse=None, lifting=1, etaBR2=2.0, analyze=None,
-        fast=False):
+        fast=False, whichMortar=0):
     """Metrics, VolInt, RHS (+ gradients), the face fluxes of FillFlux, entropy, dt and an
     nsteps trajectory computed by the reference's code on the given mesh (connectivity +
     NodeCoords) and state.
@@ -66,8 +73,11 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     volume integral of the state (volint_nfvse).  lifting 2: the BR2 build
     with penalty etaBR2 (gradients returned are the volume gradients).  out["analysis"]
     holds what the reference's Analyze collects for this state at time t0 (error norms
-    against AnalyzeExactFunc, bulk integrals, entropy); `analyze` adds parameters to it."""
+    against AnalyzeExactFunc, bulk integrals, entropy); `analyze` adds parameters to it.
+    A mesh with mortars (nMortarSides > 0) runs with the reference's whichMortar (0 L2
+    projection, 1 collocation); its Flux_slave covers the inner sides only."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
+    nm = int(mesh.get("nMortarSides", 0))
     n = N + 1
     exe = binary(parabolic, volflux, nfvse is not None, lifting=lifting, fast=fast)
     with tempfile.TemporaryDirectory() as d:
@@ -77,7 +87,7 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
             ini = analyze.pop("IniExactFunc", 1)
             f.write(f"N = {N}\nRiemann = {riemann}\nVolumeFlux = {volflux}\nIniExactFunc = {ini}\n"
                     f"IniRefState = 1\nRefState = (/1.,0.,0.,0.,1./)\nkappa = {kappa!r}\n"
-                    f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\n")
+                    f"mu0 = {mu0!r}\nPr = {Pr!r}\nCalcEntropy = T\nwhichMortar = {int(whichMortar)}\n")
             f.write(analyze_lines(analyze))
             if lifting == 2:
                 f.write(f"etaBR2 = {etaBR2!r}\n")
@@ -88,11 +98,12 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
                     f.write(f"{k} = {v!r}\n")
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
-            np.array([N, Ngeo, E, S, nsteps, int(parabolic), 0, 0], np.int32).tofile(f)
+            np.array([N, Ngeo, E, S, nsteps, int(parabolic), nm, 0], np.int32).tofile(f)
             np.array([cfl, dfl, t0, 0.0], np.float64).tofile(f)
             np.ascontiguousarray(mesh["NodeCoords"], np.float64).tofile(f)
             np.ascontiguousarray(mesh["ElemToSide"], np.int32).tofile(f)
             np.ascontiguousarray(mesh["SideToElem"], np.int32).tofile(f)
+            _mortar_input(mesh, f)
             np.ascontiguousarray(U, np.float64).tofile(f)
         p = subprocess.run([exe, par, fin, fout], cwd=d, capture_output=True, text=True, errors="replace",
                            timeout=timeout)
@@ -122,7 +133,7 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
     if nfvse is not None:
         take("alpha", (E,))
     take("Flux_master", (S, n, n, 5))
-    take("Flux_slave", (S, n, n, 5))
+    take("Flux_slave", (S - nm, n, n, 5))
     if nfvse is not None:
         take("volint_nfvse", (E, n, n, n, 5))
     take("dt0", (2,))

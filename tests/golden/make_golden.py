@@ -365,6 +365,40 @@ def make_hopr(case):
     print(f"{name}: dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} dSdU_Ut={rr['dSdU_Ut']}")
 
 
+# The reference's non-conforming HOPR meshes (tests/golden/io): 2:1 (mortar type 3 / 2) and 4:1
+# (type 1) Cartesian boxes and the curved Ngeo=2 box with all three types.  Connectivity and
+# MortarType / MortarInfo from fluxo_amd.io's reader; metrics (incl. Mortar_CalcSurfMetrics),
+# U_Mortar, Flux_Mortar and everything else from the reference's code through the harness.
+# name, mesh file, N, (volflux, riemann), whichMortar, steps
+MORTAR_CASES = [
+    ("mortar_2to1_n3_ranocha_llf_proj", "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5", 3, (32, 1), 0, 100),
+    ("mortar_4to1_n3_ranocha_llf_coll", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 1, 100),
+    ("mortar_deformed_n4_ranocha_llf_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 1), 0,
+     100),
+    ("mortar_deformed_n4_ec_coll", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 32), 1, 20),
+    ("mortar_deformed_n3_dg_hllc_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 3, (1, 2), 0, 20),
+]
+
+
+def make_mortar(case):
+    from fluxo_amd.mesh import hopr_mesh
+    name, mesh_file, N, (vf, rs), wm, nsteps = case
+    m = hopr_mesh(os.path.join(HERE, "io", mesh_file), N)
+    U = O.exactfunc(2, m["Elem_xGP"])
+    U = U * (1.0 + 1e-3 * np.random.default_rng(20261205).uniform(-1.0, 1.0, U.shape))
+    rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=False, cfl=0.9, dfl=0.9, nsteps=nsteps, whichMortar=wm)
+    out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, whichMortar=wm, kappa=1.4, cfl=0.9,
+               nsteps=nsteps, U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
+               Flux_slave=rr["Flux_slave"], dt0=rr["dt0"], errType0=rr["errType0"], U_end=rr["U_end"],
+               t_end=rr["t_end"], sJ=rr["sJ"], Mf=rr["Mf"], Mg=rr["Mg"], Mh=rr["Mh"], NormVec=rr["NormVec"],
+               TangVec1=rr["TangVec1"], TangVec2=rr["TangVec2"], SurfElem=rr["SurfElem"], Entropy=rr["Entropy"],
+               dSdU_Ut=rr["dSdU_Ut"])
+    os.makedirs(os.path.join(HERE, "mortar"), exist_ok=True)
+    np.savez(os.path.join(HERE, "mortar", name + ".npz"), **out)
+    print(f"{name}: {m['nMortarSides']} mortars, dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} "
+          f"|Ut|max={np.abs(rr['Ut']).max():.3e}")
+
+
 # Navier-Stokes (BR1, mu0) with NFVSE: name, K, N, sigma, p0, alpha_max, fixed alpha, steps, mu0
 # NFVSE with other point-wise surface fluxes at the sub-cell interfaces (Riemann at run time)
 NFVSE_RIEMANN_CASES = [
@@ -425,6 +459,9 @@ if __name__ == "__main__":
         for c in NFVSE_RIEMANN_CASES:
             if c[0] in sys.argv[1:]:
                 make_nfvse(c[:8], riemann=c[8])
+        for c in MORTAR_CASES:
+            if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
+                make_mortar(c)
         sys.exit(0)
     for c in CASES:
         make(c)
@@ -450,3 +487,5 @@ if __name__ == "__main__":
         make_analyze(c)
     for c in HOPR_CASES:
         make_hopr(c)
+    for c in MORTAR_CASES:
+        make_mortar(c)

@@ -41,7 +41,7 @@ def test_io_library_exports_every_declared_symbol():
     from fluxo_amd import io
     lib = ctypes.CDLL(io.IO_LIB)
     names = io_declared_symbols()
-    assert len(names) == 6
+    assert len(names) == 7
     for name in names:
         assert hasattr(lib, name), name
 

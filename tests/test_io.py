@@ -131,8 +131,6 @@ def test_reference_state_file_and_roundtrip(tmp_path):
 
 
 def test_reader_rejects_what_the_engine_cannot_run():
-    with pytest.raises(io().IOError_, match="mortar"):
-        io().read_mesh(os.path.join(IO_DIR, "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5"))
     with pytest.raises(io().IOError_, match="periodic"):
         io().read_mesh(os.path.join(IO_DIR, "ConformBoxUnstruct2_Ngeo2_mesh.h5"))
     with pytest.raises(io().IOError_, match="cannot open"):
