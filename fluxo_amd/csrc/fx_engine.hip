@@ -208,7 +208,7 @@ void fx_ctx_destroy(fx_ctx *c)
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
                   b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs,
-                  b.mort, b.mops, b.Umort};
+                  b.mort, b.mops, b.Umort, b.LiftS, b.LiftM, b.GradM};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -226,21 +226,28 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   std::vector<int> mort;
   int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S, nm, m->MortarType, m->MortarInfo, &mort);
   if (rc) return rc;
-  if (nm > 0 && (b.mhd() || b.cfg.parabolic || b.cfg.nfvse))
-    return set_err(FX_EINVAL, "meshes with mortars: Euler without shock capturing only (nVar 5, parabolic 0, "
-                   "nfvse 0)");
+  if (nm > 0 && (b.mhd() || (b.cfg.parabolic && b.cfg.lifting != 1) || b.cfg.nfvse))
+    return set_err(FX_EINVAL, "meshes with mortars: Euler or Navier-Stokes with BR1 lifting, without shock "
+                   "capturing (nVar 5, lifting 1, nfvse 0)");
   if (nm > 0 && b.cfg.whichMortar != 0 && b.cfg.whichMortar != 1)
     return set_err(FX_EINVAL, "whichMortar=%d (0 projection, 1 collocation)", b.cfg.whichMortar);
   const size_t n3E = (size_t)b.n3 * E, nf = (size_t)b.n2 * S;
   if (nm > 0) {
     HIPCHK(hipStreamSynchronize(b.stream));
     if (b.nMortar != nm) {  // the mortar buffers are sized by the mesh
-      for (void *p : {(void *)b.mort, (void *)b.mops, (void *)b.Umort})
+      for (void *p : {(void *)b.mort, (void *)b.mops, (void *)b.Umort, (void *)b.LiftS, (void *)b.LiftM,
+                      (void *)b.GradM})
         if (p) HIPCHK(hipFree(p));
       b.mort = nullptr; b.mops = nullptr; b.Umort = nullptr; b.nMortar = 0;
+      b.LiftS = nullptr; b.LiftM = nullptr; b.GradM = nullptr;
       HIPCHK(hipMalloc((void **)&b.mort, mort.size() * sizeof(int)));
       HIPCHK(hipMalloc((void **)&b.mops, 4 * (size_t)b.n2 * 8));
       HIPCHK(hipMalloc((void **)&b.Umort, 5 * nf * 8));
+      if (b.cfg.parabolic) {
+        HIPCHK(hipMalloc((void **)&b.LiftS, 15 * nf * 8));
+        HIPCHK(hipMalloc((void **)&b.LiftM, 15 * (size_t)nm * b.n2 * 8));
+        HIPCHK(hipMalloc((void **)&b.GradM, 15 * (size_t)nm * b.n2 * 8));
+      }
     }
     double mops[4 * (MAXN + 1) * (MAXN + 1)];
     make_mortar_ops(b.cfg.N, b.cfg.whichMortar, mops);

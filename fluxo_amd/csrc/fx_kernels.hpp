@@ -584,6 +584,110 @@ k_surface(const double *__restrict__ U, const double *__restrict__ FvN, const in
 // mortar.f90:63-110), k = 0 for the half [-1,0], 1 for [0,1].
 constexpr int MORT_INTS = 8;
 
+// Flux_Mortar (fillmortar.f90:239-368; Flux_Cons_Mortar :377-393 for the lifting fluxes):
+// the big side's values as the projection of its small sides' values, in the reference's
+// order of summation, NC components per face point.  src holds the small sides' values (the
+// master values of the small sides: the mortar is their master side), dst receives the big
+// side's, both in the face-point layout of Flux; src and dst may be one array (the sides
+// differ).  One thread per (mortar, p, q) of the big side.
+template <int N, int NC>
+__global__ void __launch_bounds__(256)
+k_flux_mortar(const double *src, double *dst, const int *__restrict__ mort, const double *__restrict__ mops,
+              int nMortar)
+{
+  constexpr int n = N + 1, n2 = n * n;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = gid / n2, pq = gid - m * n2, p = pq % n, q = pq / n;
+  if (m >= nMortar) return;
+  const int *mi = mort + MORT_INTS * m;
+  const int type = mi[0];
+  const double *MP = mops + 2 * n2;
+  double o[NC];
+#pragma unroll
+  for (int v = 0; v < NC; v++) o[v] = 0.0;
+  if (type == 1) {  // xi first into Flux_tmp(p,l), then eta
+    for (int jNb = 0; jNb < 2; jNb++)
+      for (int l = 0; l < n; l++) {
+        double t[NC];
+#pragma unroll
+        for (int v = 0; v < NC; v++) t[v] = 0.0;
+        for (int iNb = 0; iNb < 2; iNb++) {
+          const double *fs = src + ((size_t)(mi[3 + iNb + 2 * jNb] - 1) * n2 + l * n) * NC;
+          for (int l2 = 0; l2 < n; l2++) {
+            const double c = MP[(iNb * n + l2) * n + p];
+#pragma unroll
+            for (int v = 0; v < NC; v++) t[v] = t[v] + c * fs[l2 * NC + v];
+          }
+        }
+        const double c = MP[(jNb * n + l) * n + q];
+#pragma unroll
+        for (int v = 0; v < NC; v++) o[v] = o[v] + c * t[v];
+      }
+  } else if (type == 2) {  // halves in eta
+    for (int jNb = 0; jNb < 2; jNb++) {
+      const double *fs = src + (size_t)(mi[3 + jNb] - 1) * n2 * NC;
+      for (int l = 0; l < n; l++) {
+        const double c = MP[(jNb * n + l) * n + q];
+#pragma unroll
+        for (int v = 0; v < NC; v++) o[v] = o[v] + c * fs[(l * n + p) * NC + v];
+      }
+    }
+  } else {  // halves in xi
+    for (int iNb = 0; iNb < 2; iNb++) {
+      const double *fs = src + (size_t)(mi[3 + iNb] - 1) * n2 * NC;
+      for (int l = 0; l < n; l++) {
+        const double c = MP[(iNb * n + l) * n + p];
+#pragma unroll
+        for (int v = 0; v < NC; v++) o[v] = o[v] + c * fs[(q * n + l) * NC + v];
+      }
+    }
+  }
+  double *d = dst + ((size_t)(mi[7] - 1) * n2 + pq) * NC;
+#pragma unroll
+  for (int v = 0; v < NC; v++) d[v] = o[v];
+}
+
+// Big side face data -> one small side's face point (InterpolateBigToSmall, as k_mortar_u),
+// NC components; big(a, b) returns the pointer to the big side's values at (a, b).
+template <int N, int NC, class BIG>
+__device__ __forceinline__ void mortar_interp(int type, int im, int p, int q, const double *MI, BIG big, double *o)
+{
+  constexpr int n = N + 1;
+#pragma unroll
+  for (int v = 0; v < NC; v++) o[v] = 0.0;
+  if (type == 1) {
+    const int iNb = im & 1, jNb = im >> 1;
+    for (int l = 0; l < n; l++) {
+      double t[NC];
+#pragma unroll
+      for (int v = 0; v < NC; v++) t[v] = 0.0;
+      for (int mm = 0; mm < n; mm++) {
+        const double c = MI[(jNb * n + mm) * n + q];
+        const double *u = big(l, mm);
+#pragma unroll
+        for (int v = 0; v < NC; v++) t[v] = t[v] + c * u[v];
+      }
+      const double c = MI[(iNb * n + l) * n + p];
+#pragma unroll
+      for (int v = 0; v < NC; v++) o[v] = o[v] + c * t[v];
+    }
+  } else if (type == 2) {
+    for (int l = 0; l < n; l++) {
+      const double c = MI[(im * n + l) * n + q];
+      const double *u = big(p, l);
+#pragma unroll
+      for (int v = 0; v < NC; v++) o[v] = o[v] + c * u[v];
+    }
+  } else {
+    for (int l = 0; l < n; l++) {
+      const double c = MI[(im * n + l) * n + p];
+      const double *u = big(l, q);
+#pragma unroll
+      for (int v = 0; v < NC; v++) o[v] = o[v] + c * u[v];
+    }
+  }
+}
+
 // U_Mortar (fillmortar.f90:72-129) with InterpolateBigToSmall (mortar_vars.f90:203-266): the
 // master state of each small side, interpolated from the big element's face (its master side,
 // read from the volume as ProlongToFace would), in the reference's order of summation, into
@@ -596,101 +700,15 @@ k_mortar_u(const double *__restrict__ U, const int *__restrict__ mort, const dou
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int m = gid / (4 * n2), r = gid - m * (4 * n2), im = r / n2, pq = r - im * n2;
-  const int p = pq % n, q = pq / n;
   if (m >= nMortar) return;
   const int *mi = mort + MORT_INTS * m;
   const int type = mi[0], lb = mi[2];
   if (type != 1 && im >= 2) return;
   const double *ub = U + (size_t)mi[1] * n3 * 5;
-  const double *MI = mops;
-  double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (type == 1) {  // eta first: tmp(l,q) = sum_m MInt(m,q,jNb) Big(l,m); then xi
-    const int iNb = im & 1, jNb = im >> 1;
-    for (int l = 0; l < n; l++) {
-      double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int mm = 0; mm < n; mm++) {
-        const double c = MI[(jNb * n + mm) * n + q];
-        const double *u = ub + side_to_vol<N>(lb, 0, l, mm) * 5;
-#pragma unroll
-        for (int v = 0; v < 5; v++) t[v] = t[v] + c * u[v];
-      }
-      const double c = MI[(iNb * n + l) * n + p];
-#pragma unroll
-      for (int v = 0; v < 5; v++) o[v] = o[v] + c * t[v];
-    }
-  } else if (type == 2) {  // halves in eta: sum_l MInt(l,q,jNb) Big(p,l)
-    for (int l = 0; l < n; l++) {
-      const double c = MI[(im * n + l) * n + q];
-      const double *u = ub + side_to_vol<N>(lb, 0, p, l) * 5;
-#pragma unroll
-      for (int v = 0; v < 5; v++) o[v] = o[v] + c * u[v];
-    }
-  } else {  // halves in xi: sum_l MInt(l,p,iNb) Big(l,q)
-    for (int l = 0; l < n; l++) {
-      const double c = MI[(im * n + l) * n + p];
-      const double *u = ub + side_to_vol<N>(lb, 0, l, q) * 5;
-#pragma unroll
-      for (int v = 0; v < 5; v++) o[v] = o[v] + c * u[v];
-    }
-  }
+  double o[5];
+  mortar_interp<N, 5>(type, im, pq % n, pq / n, mops,
+                      [&](int a, int b) { return ub + side_to_vol<N>(lb, 0, a, b) * 5; }, o);
   double *dst = Um + ((size_t)(mi[3 + im] - 1) * n2 + pq) * 5;
-#pragma unroll
-  for (int v = 0; v < 5; v++) dst[v] = o[v];
-}
-
-// Flux_Mortar (fillmortar.f90:239-368, inner mortars, weak form): the big side's flux as the
-// projection of its small sides' fluxes (each small side's master flux: the mortar is the
-// master of its small sides), in the reference's order of summation.  One thread per
-// (mortar, p, q) of the big side.
-template <int N>
-__global__ void __launch_bounds__(256)
-k_flux_mortar(double *__restrict__ Flux, const int *__restrict__ mort, const double *__restrict__ mops,
-              int nMortar)
-{
-  constexpr int n = N + 1, n2 = n * n;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = gid / n2, pq = gid - m * n2, p = pq % n, q = pq / n;
-  if (m >= nMortar) return;
-  const int *mi = mort + MORT_INTS * m;
-  const int type = mi[0];
-  const double *MP = mops + 2 * n2;
-  double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (type == 1) {  // xi first into Flux_tmp(p,l), then eta
-    for (int jNb = 0; jNb < 2; jNb++)
-      for (int l = 0; l < n; l++) {
-        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int iNb = 0; iNb < 2; iNb++) {
-          const double *fs = Flux + ((size_t)(mi[3 + iNb + 2 * jNb] - 1) * n2 + l * n) * 5;
-          for (int l2 = 0; l2 < n; l2++) {
-            const double c = MP[(iNb * n + l2) * n + p];
-#pragma unroll
-            for (int v = 0; v < 5; v++) t[v] = t[v] + c * fs[l2 * 5 + v];
-          }
-        }
-        const double c = MP[(jNb * n + l) * n + q];
-#pragma unroll
-        for (int v = 0; v < 5; v++) o[v] = o[v] + c * t[v];
-      }
-  } else if (type == 2) {  // halves in eta
-    for (int jNb = 0; jNb < 2; jNb++) {
-      const double *fs = Flux + (size_t)(mi[3 + jNb] - 1) * n2 * 5;
-      for (int l = 0; l < n; l++) {
-        const double c = MP[(jNb * n + l) * n + q];
-#pragma unroll
-        for (int v = 0; v < 5; v++) o[v] = o[v] + c * fs[(l * n + p) * 5 + v];
-      }
-    }
-  } else {  // halves in xi
-    for (int iNb = 0; iNb < 2; iNb++) {
-      const double *fs = Flux + (size_t)(mi[3 + iNb] - 1) * n2 * 5;
-      for (int l = 0; l < n; l++) {
-        const double c = MP[(iNb * n + l) * n + p];
-#pragma unroll
-        for (int v = 0; v < 5; v++) o[v] = o[v] + c * fs[(q * n + l) * 5 + v];
-      }
-    }
-  }
-  double *dst = Flux + ((size_t)(mi[7] - 1) * n2 + pq) * 5;
 #pragma unroll
   for (int v = 0; v < 5; v++) dst[v] = o[v];
 }
@@ -746,7 +764,8 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
-          int nElems, EQ eq, double etaBR2)
+          int nElems, EQ eq, double etaBR2, const double *__restrict__ Um, const double *__restrict__ LiftM,
+          double *__restrict__ GradM, int nMortar)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
   constexpr int EPB = LiftGeom<N>::EPB, NF = 6 * n2;  // face points per element
@@ -787,20 +806,26 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
     int p, q;
     vol_to_side<N>(loc, fl, fi, fj, fk, p, q);
-    int enb, vnb;
-    if (fl == 0) {
-      enb = s2e[5 * sd + 1] - 1;
-      vnb = side_to_vol<N>(s2e[5 * sd + 3], s2e[5 * sd + 4], p, q);
-    } else {
-      enb = s2e[5 * sd] - 1;
-      vnb = side_to_vol<N>(s2e[5 * sd + 2], 0, p, q);
+    const int fpt = (sd * n + q) * n + p;
+    // a big mortar side has no neighbour element: its lifting flux comes projected (LiftM); the
+    // small side of a mortar has the big side's interpolated state (Um) as its master
+    if (fl == 0 && sd < nMortar) {
+      s_fpt[fe][r] = 2 * fpt;
+      continue;
     }
-    const double *unbp = U + ((size_t)enb * n3 + vnb) * NV;
+    const int enb = (fl == 0) ? s2e[5 * sd + 1] - 1 : s2e[5 * sd] - 1;
+    const double *unbp;
+    if (enb >= 0) {
+      const int vnb = (fl == 0) ? side_to_vol<N>(s2e[5 * sd + 3], s2e[5 * sd + 4], p, q)
+                                : side_to_vol<N>(s2e[5 * sd + 2], 0, p, q);
+      unbp = U + ((size_t)enb * n3 + vnb) * NV;
+    } else {
+      unbp = Um + (size_t)fpt * NV;
+    }
     double unb[NV], wnb[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) unb[v] = unbp[v];
     eq.entropy(unb, wnb);
-    const int fpt = (sd * n + q) * n + p;
     const double4 g = nvse[fpt];
     double *d = &s_buf[fe][r * (NV + 3)];
 #pragma unroll
@@ -855,7 +880,17 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       if (r < 0) continue;
       const double *d = &s_buf[el][r * (NV + 3)];
       const double nx = d[NV], ny = d[NV + 1], nz = d[NV + 2];
-      if (LIFT == 1) {
+      const int mcode = s_fpt[el][r];
+      if (LIFT == 1 && (mcode >> 1) < nMortar * n2) {
+        // big mortar side: Flux_Cons_Mortar of the small sides' FluxX, FluxY, FluxZ
+        const double *lm = LiftM + (size_t)(mcode >> 1) * 3 * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          gx[v] = gx[v] + lm[v] * L0;
+          gy[v] = gy[v] + lm[NV + v] * L0;
+          gz[v] = gz[v] + lm[2 * NV + v] * L0;
+        }
+      } else if (LIFT == 1) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           const double f = d[v];
@@ -899,6 +934,19 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     eq.grad_prim(u, gx);
     eq.grad_prim(u, gy);
     eq.grad_prim(u, gz);
+    if (nMortar > 0) {  // the big side's prolonged gradients, for U_Mortar of gradPx/y/z
+      const NodeFaces nfm = node_faces<N>(i, j, k);
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const int r = nfm.get(t);
+        if (r < 0) continue;
+        const int code = s_fpt[el][r];
+        if ((code >> 1) >= nMortar * n2) continue;
+        double *gm = GradM + (size_t)(code >> 1) * 3 * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) { gm[v] = gx[v]; gm[NV + v] = gy[v]; gm[2 * NV + v] = gz[v]; }
+      }
+    }
     if (OUT == 0) {
 #pragma unroll
       for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
@@ -963,6 +1011,79 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 #pragma unroll
     for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
   }
+}
+
+// BR1 lifting on the small sides of mortars (Lifting_FillFlux br1_fillflux.f90:49-91 with
+// EvalLiftingSurfFlux flux.f90:677-716): F = 1/2 (w_slave - w_master) SurfElem with the master
+// state interpolated from the big side (Um), times each normal component into LiftS (3 NV
+// per face point: FluxX, FluxY, FluxZ), for Flux_Cons_Mortar onto the big side.  One thread
+// per (mortar, small side, p, q).
+template <int N, class EQ>
+__global__ void __launch_bounds__(256)
+k_lift_mortar_small(const double *__restrict__ U, const double *__restrict__ Um, const int *__restrict__ s2e,
+                    const double4 *__restrict__ nvse, const int *__restrict__ mort, double *__restrict__ LiftS,
+                    int nMortar, EQ eq)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = gid / (4 * n2), r = gid - m * (4 * n2), im = r / n2, pq = r - im * n2;
+  if (m >= nMortar) return;
+  const int *mi = mort + MORT_INTS * m;
+  if (mi[0] != 1 && im >= 2) return;
+  const int s = mi[3 + im] - 1;
+  const size_t fpt = (size_t)s * n2 + pq;
+  const int es = s2e[5 * s + 1] - 1, ls = s2e[5 * s + 3], fl = s2e[5 * s + 4];
+  const double *us = U + ((size_t)es * n3 + side_to_vol<N>(ls, fl, pq % n, pq / n)) * NV;
+  const double *um = Um + fpt * NV;
+  double u[NV], ws[NV], wm[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) u[v] = us[v];
+  eq.entropy(u, ws);
+#pragma unroll
+  for (int v = 0; v < NV; v++) u[v] = um[v];
+  eq.entropy(u, wm);
+  const double4 g = nvse[fpt];
+  double *o = LiftS + fpt * 3 * NV;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double f = 0.5 * (ws[v] - wm[v]) * g.w;
+    o[v] = f * g.x;
+    o[NV + v] = f * g.y;
+    o[2 * NV + v] = f * g.z;
+  }
+}
+
+// The master side's viscous flux on the small sides of mortars: U_Mortar of the big side's
+// primitive gradients (br1.f90:245-322) and of its state (Um), EvalDiffFlux3D and the
+// small side's normal into the master slot of FvN (as the lifting kernel writes it for an
+// element's own faces), whose mean with the small element's is the BR1 viscous surface flux
+// (riemann.f90:155-168).  One thread per (mortar, small side, p, q).
+template <int N, class EQ>
+__global__ void __launch_bounds__(256)
+k_mortar_fv(const double *__restrict__ Um, const double *__restrict__ GradM, const double4 *__restrict__ nvse,
+            const int *__restrict__ mort, const double *__restrict__ mops, double *__restrict__ FvN, int nMortar,
+            EQ eq)
+{
+  constexpr int n = N + 1, n2 = n * n, NV = EQ::NV, NVV = EQ::NVV;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = gid / (4 * n2), r = gid - m * (4 * n2), im = r / n2, pq = r - im * n2;
+  if (m >= nMortar) return;
+  const int *mi = mort + MORT_INTS * m;
+  if (mi[0] != 1 && im >= 2) return;
+  const size_t fpt = (size_t)(mi[3 + im] - 1) * n2 + pq;
+  const double *gb = GradM + (size_t)(mi[7] - 1) * n2 * 3 * NV;
+  double gr[3 * NV];
+  mortar_interp<N, 3 * NV>(mi[0], im, pq % n, pq / n, mops,
+                           [&](int a, int b) { return gb + (size_t)(b * n + a) * 3 * NV; }, gr);
+  double u[NV], fv[12], o[NVV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) u[v] = Um[fpt * NV + v];
+  eq.fluxes(u, gr, gr + NV, gr + 2 * NV, fv);
+  const double4 g = nvse[fpt];
+  EQ::vis_dir(fv, g.x, g.y, g.z, o);
+  double *dst = FvN + (2 * fpt) * NVV;
+#pragma unroll
+  for (int v = 0; v < NVV; v++) dst[v] = o[v];
 }
 
 // ---------------------------------------------------------------------------------

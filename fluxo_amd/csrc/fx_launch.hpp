@@ -384,6 +384,9 @@ struct Batch {
   int nMortar = 0;
   int *mort = nullptr;
   double *mops = nullptr, *Umort = nullptr;
+  // BR1 lifting with mortars: the small sides' lifting fluxes (3 nVar per face point), their
+  // projection onto the big sides, and the big sides' primitive gradients (3 nVar each)
+  double *LiftS = nullptr, *LiftM = nullptr, *GradM = nullptr;
 
   bool mhd() const { return cfg.nVar == 9; }
   MhdArgs mhd_args() const

@@ -101,7 +101,8 @@ void launch_flux_mortar_n(const Batch &b)
   const long long tot = (long long)b.nMortar * b.n2;
   const int nb = (int)((tot + 255) / 256);
   if (nb > 0)
-    hipLaunchKernelGGL((k_flux_mortar<N>), dim3(nb), dim3(256), 0, b.stream, b.Flux, b.mort, b.mops, b.nMortar);
+    hipLaunchKernelGGL((k_flux_mortar<N, 5>), dim3(nb), dim3(256), 0, b.stream, b.Flux, b.Flux, b.mort, b.mops,
+                       b.nMortar);
 }
 
 template <int N>
@@ -156,10 +157,30 @@ void launch_lifting_l(const Batch &b, int out, const EQ &eq)
   if (out == 0)
     hipLaunchKernelGGL((k_lifting<N, 0, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
                        b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq,
-                       b.cfg.etaBR2);
+                       b.cfg.etaBR2, b.Umort, b.LiftM, b.GradM, b.nMortar);
   else
     hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2);
+                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
+                       b.Umort, b.LiftM, b.GradM, b.nMortar);
+}
+
+// BR1 lifting on a mesh with mortars: U_Mortar of the state, the small sides' lifting fluxes
+// and their Flux_Cons_Mortar onto the big sides before the lifting kernel; after it (viscous
+// form) U_Mortar of the big sides' gradients and the master viscous flux of the small sides.
+template <int N, class EQ>
+void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
+{
+  const int nbs = (int)(((long long)b.nMortar * 4 * b.n2 + 255) / 256);
+  const int nbb = (int)(((long long)b.nMortar * b.n2 + 255) / 256);
+  hipLaunchKernelGGL((k_mortar_u<N>), dim3(nbs), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
+  hipLaunchKernelGGL((k_lift_mortar_small<N, EQ>), dim3(nbs), dim3(256), 0, b.stream, b.U, b.Umort, b.s2e, b.nvse,
+                     b.mort, b.LiftS, b.nMortar, eq);
+  hipLaunchKernelGGL((k_flux_mortar<N, 3 * EQ::NV>), dim3(nbb), dim3(256), 0, b.stream, b.LiftS, b.LiftM, b.mort,
+                     b.mops, b.nMortar);
+  launch_lifting_l<N, EQ, 1>(b, out, eq);
+  if (out == 1)
+    hipLaunchKernelGGL((k_mortar_fv<N, EQ>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse, b.mort,
+                       b.mops, b.FvN, b.nMortar, eq);
 }
 
 template <int N, class EQ>
@@ -185,7 +206,8 @@ void launch_lifting_n(const Batch &b, int out)
     eq.kappa = b.cfg.kappa;
     eq.mu0 = b.cfg.mu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    launch_lifting_e<N>(b, out, eq);
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);  // BR1 only (checked at mesh upload)
+    else launch_lifting_e<N>(b, out, eq);
   }
 }
 

@@ -20,8 +20,8 @@
 !> DeviceResident (logical, default T): T keeps U on the device and runs whole steps there;
 !> F calls DGTimeDerivative_weakForm once per stage from the host RK loop (drop-in form).
 !> MeshFile (optional): a HOPR mesh file (periodic boundaries; libfluxo_io) instead of the box
-!> generator; non-conforming meshes (mortars; Euler only) with whichMortar (0 projection, 1
-!> collocation).  RestartFile (optional): initial state from a state file (Restart,
+!> generator; non-conforming meshes (mortars; Euler, Navier-Stokes with BR1) with whichMortar (0
+!> projection, 1 collocation).  RestartFile (optional): initial state from a state file (Restart,
 !> restart.f90:147-232; same number of variables and elements) instead of IniExactFunc, its Time
 !> as the start time; a file of another N is interpolated (lower N) or projected conservatively
 !> through J U (higher N) with the reference's modal Vandermonde matrices.  ProjectName (optional): WriteState (hdf5_output.f90:74-165) of the final state to

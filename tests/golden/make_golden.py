@@ -378,21 +378,32 @@ MORTAR_CASES = [
     ("mortar_deformed_n4_ec_coll", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 32), 1, 20),
     ("mortar_deformed_n3_dg_hllc_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 3, (1, 2), 0, 20),
 ]
+# Navier-Stokes with BR1 lifting (mu0 1e-3): the same, + mu0
+MORTAR_NS_CASES = [
+    ("mortar_ns_deformed_n4_br1_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 1), 0, 100,
+     1e-3),
+    ("mortar_ns_2to1_n3_br1_coll", "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5", 3, (32, 1), 1, 20, 1e-3),
+    ("mortar_ns_4to1_n3_br1_proj", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 0, 20, 1e-3),
+]
 
 
-def make_mortar(case):
+def make_mortar(case, mu0=None):
     from fluxo_amd.mesh import hopr_mesh
     name, mesh_file, N, (vf, rs), wm, nsteps = case
+    par = mu0 is not None
     m = hopr_mesh(os.path.join(HERE, "io", mesh_file), N)
     U = O.exactfunc(2, m["Elem_xGP"])
     U = U * (1.0 + 1e-3 * np.random.default_rng(20261205).uniform(-1.0, 1.0, U.shape))
-    rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=False, cfl=0.9, dfl=0.9, nsteps=nsteps, whichMortar=wm)
-    out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, whichMortar=wm, kappa=1.4, cfl=0.9,
-               nsteps=nsteps, U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
+    rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=par, mu0=mu0 if par else 0.0, Pr=0.72, cfl=0.9, dfl=0.9,
+               nsteps=nsteps, whichMortar=wm)
+    out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, whichMortar=wm, kappa=1.4, cfl=0.9, dfl=0.9,
+               parabolic=int(par), mu0=mu0 if par else 0.0, Pr=0.72, nsteps=nsteps, U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
                Flux_slave=rr["Flux_slave"], dt0=rr["dt0"], errType0=rr["errType0"], U_end=rr["U_end"],
                t_end=rr["t_end"], sJ=rr["sJ"], Mf=rr["Mf"], Mg=rr["Mg"], Mh=rr["Mh"], NormVec=rr["NormVec"],
                TangVec1=rr["TangVec1"], TangVec2=rr["TangVec2"], SurfElem=rr["SurfElem"], Entropy=rr["Entropy"],
                dSdU_Ut=rr["dSdU_Ut"])
+    if par:
+        out.update(gradPx=rr["gradPx"], gradPy=rr["gradPy"], gradPz=rr["gradPz"])
     os.makedirs(os.path.join(HERE, "mortar"), exist_ok=True)
     np.savez(os.path.join(HERE, "mortar", name + ".npz"), **out)
     print(f"{name}: {m['nMortarSides']} mortars, dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} "
@@ -462,6 +473,9 @@ if __name__ == "__main__":
         for c in MORTAR_CASES:
             if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
                 make_mortar(c)
+        for c in MORTAR_NS_CASES:
+            if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
+                make_mortar(c[:6], mu0=c[6])
         sys.exit(0)
     for c in CASES:
         make(c)
@@ -489,3 +503,5 @@ if __name__ == "__main__":
         make_hopr(c)
     for c in MORTAR_CASES:
         make_mortar(c)
+    for c in MORTAR_NS_CASES:
+        make_mortar(c[:6], mu0=c[6])

@@ -122,12 +122,14 @@ def test_fortran_host_runs_the_reference_free_stream_input(tmp_path):
     assert np.abs(U - [1.0, 0.1, 0.2, -0.3, 50.07]).max() <= 1e-11
 
 
+@pytest.mark.parametrize("parabolic", [False, True])
 @pytest.mark.parametrize("which_mortar", [0, 1])
-def test_fortran_host_free_stream_on_a_mortar_mesh(tmp_path, which_mortar):
+def test_fortran_host_free_stream_on_a_mortar_mesh(tmp_path, which_mortar, parabolic):
     """The reference's nonconforming free-stream input (tests/runs/navst/freestream/
     nonconforming_projmortar and _collmortar: DeformedBoxMortarPeriodic Ngeo 2, N=4, Riemann 1,
-    VolumeFlux 1, whichMortar 0 / 1, RefState (1, 0.1, 0.2, -0.3, 20), CFL 0.9) for Euler (the
-    mortar path is built without viscosity), to tEnd 1: L_inf of CalcErrorNorms below 1e-11."""
+    VolumeFlux 1, whichMortar 0 / 1, RefState (1, 0.1, 0.2, -0.3, 20), mu0 1e-6, CFL 0.9,
+    DFL 0.5; and the same for Euler) to tEnd 1 instead of 5: L_inf of CalcErrorNorms below the
+    reference test's 1e-11."""
     if not os.path.exists(EXE):
         pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "io")
@@ -136,8 +138,9 @@ def test_fortran_host_free_stream_on_a_mortar_mesh(tmp_path, which_mortar):
         f"N = 4\nRiemann = 1\nVolumeFlux = 1\nwhichMortar = {which_mortar}\n"
         f"MeshFile = {gold}/DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5\n"
         "IniExactFunc = 1\nIniRefState = 1\nRefState = (/1.0, 0.1, 0.2, -0.3, 20. /)\n"
-        "kappa = 1.4\ntEnd = 1.0\nCFLscale = 0.9\n")
-    print(f"fluxo_gpu mortar free stream whichMortar={which_mortar}", flush=True)
+        f"Parabolic = {'T' if parabolic else 'F'}\nmu0 = 1.0E-06\n"
+        "kappa = 1.4\ntEnd = 1.0\nCFLscale = 0.9\nDFLscale = 0.5\n")
+    print(f"fluxo_gpu mortar free stream whichMortar={which_mortar} parabolic={parabolic}", flush=True)
     p = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
     linf = None

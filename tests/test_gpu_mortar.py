@@ -1,6 +1,7 @@
 """The engine on the reference's non-conforming HOPR meshes (mortars; read by fluxo_amd.io,
 metrics from the Fortran host) against the reference's code on the same meshes and states
-(tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt and the LSERK trajectory of
+(tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt, for Navier-Stokes the BR1
+lifting with Flux_Cons_Mortar and U_Mortar of the gradients, and the LSERK trajectory of
 the reference, run by its harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
 RHS and face flux, <= 1e-10 after the stored RK steps (100 on three cases).  Properties the
 reference's own free-stream mortar tests check (tests/runs/navst/freestream/
@@ -34,7 +35,9 @@ def engine(g, **kw):
     from fluxo_amd.mesh import hopr_mesh
     from fluxo_amd.native import Engine
     m = hopr_mesh(os.path.join(IO_DIR, str(g["mesh_file"])), int(g["N"]))
-    e = Engine(m, volume_flux=int(g["volflux"]), riemann=int(g["riemann"]), which_mortar=int(g["whichMortar"]), **kw)
+    par = bool(int(g.get("parabolic", 0)))
+    e = Engine(m, volume_flux=int(g["volflux"]), riemann=int(g["riemann"]), which_mortar=int(g["whichMortar"]),
+               parabolic=par, mu0=float(g.get("mu0", 0.0)), Pr=float(g.get("Pr", 0.72)), **kw)
     return e, m
 
 
@@ -44,6 +47,10 @@ def test_rhs_and_face_fluxes_match_reference(name):
     print(f"case {name}", flush=True)
     e, m = engine(g)
     e.upload(g["U"])
+    if "gradPx" in g:  # BR1 lifting: Flux_Cons_Mortar of the lifting fluxes on the big sides
+        gerr = max(rel_l2(a, g[k]) for a, k in zip(e.lifting(), ("gradPx", "gradPy", "gradPz")))
+        print(f"{name}: gradients relL2 vs reference {gerr:.3e}")
+        assert gerr <= 1e-12
     err = rel_l2(e.rhs(), g["Ut"])
     fm, fs = e.fillflux()
     nm = m["nMortarSides"]
@@ -52,7 +59,9 @@ def test_rhs_and_face_fluxes_match_reference(name):
     big = rel_l2(fm[:nm], g["Flux_master"][:nm])
     print(f"{name}: RHS {err:.3e}  Flux_master {ferr:.3e} (big sides {big:.3e})  Flux_slave {serr:.3e}")
     assert err <= 1e-12 and ferr <= 1e-12 and serr <= 1e-12 and big <= 1e-12
-    # SurfInt alone on the reference's fluxes and volume integral gives the reference's RHS
+    if "gradPx" in g:
+        return
+    # Euler: SurfInt alone on the reference's fluxes and volume integral gives its RHS
     fsp = np.concatenate([np.zeros_like(fm[:nm]), g["Flux_slave"]])
     ut = e.surfint(g["Flux_master"], fsp, g["volint"]) * (-m["sJ"][..., None])
     assert rel_l2(ut, g["Ut"]) <= 1e-12
@@ -64,7 +73,7 @@ def test_trajectory_matches_reference(name):
     print(f"case {name}", flush=True)
     e, _ = engine(g)
     e.upload(g["U"])
-    t, _, errt = e.run(int(g["nsteps"]), float(g["cfl"]))
+    t, _, errt = e.run(int(g["nsteps"]), float(g["cfl"]), float(g.get("dfl", 0.0)))
     traj = rel_l2(e.download(), g["U_end"])
     print(f"{name}: {int(g['nsteps'])} steps relL2 vs reference {traj:.3e}")
     assert errt == 0 and traj <= 1e-10
@@ -72,24 +81,27 @@ def test_trajectory_matches_reference(name):
 
 
 @pytest.mark.parametrize("which_mortar", [0, 1])
-@pytest.mark.parametrize("volume_flux,riemann", [(32, 1), (1, 1), (10, 32)])
-def test_free_stream_on_curved_mortars(which_mortar, volume_flux, riemann):
-    """The reference's nonconforming free-stream setting: DeformedBoxMortarPeriodic (Ngeo 2,
-    mortar types 1, 2, 3), N=4, RefState (1, 0.1, 0.2, -0.3, 20), projection and collocation
-    mortars; the RHS vanishes and 100 steps keep the state to L_inf 1e-11 (the reference
-    test's bound, for its VolumeFlux 1 / Riemann 1), to 1e-12 of the state's magnitude for
-    the entropy-conservative flux pairs."""
+@pytest.mark.parametrize("volume_flux,riemann,mu0", [(32, 1, None), (1, 1, None), (10, 32, None), (1, 1, 1e-6),
+                                                    (32, 1, 1e-3)])
+def test_free_stream_on_curved_mortars(which_mortar, volume_flux, riemann, mu0):
+    """The reference's nonconforming free-stream tests (navst/freestream/nonconforming_projmortar
+    and _collmortar): DeformedBoxMortarPeriodic (Ngeo 2, mortar types 1, 2, 3), N=4, RefState
+    (1, 0.1, 0.2, -0.3, 20), projection and collocation mortars, VolumeFlux 1, Riemann 1,
+    mu0 1e-6 (BR1), CFL 0.9, DFL 0.5; also Euler and other flux pairs.  The RHS vanishes and
+    100 steps keep the state to L_inf 1e-11 (the reference test's bound) for VolumeFlux 1 /
+    Riemann 1, to 1e-12 of the state's magnitude for the other pairs."""
     from fluxo_amd.mesh import hopr_mesh
     from fluxo_amd.native import Engine
-    print(f"free stream whichMortar={which_mortar} VF={volume_flux} RS={riemann}", flush=True)
+    print(f"free stream whichMortar={which_mortar} VF={volume_flux} RS={riemann} mu0={mu0}", flush=True)
     m = hopr_mesh(os.path.join(IO_DIR, DEFORMED), 4)
     cons = np.array([1.0, 0.1, 0.2, -0.3, 20.0 / 0.4 + 0.5 * (0.01 + 0.04 + 0.09)])  # kappa 1.4
     U = np.broadcast_to(cons, (m["nElems"], 5, 5, 5, 5)).copy()
-    e = Engine(m, volume_flux=volume_flux, riemann=riemann, which_mortar=which_mortar)
+    e = Engine(m, volume_flux=volume_flux, riemann=riemann, which_mortar=which_mortar, parabolic=mu0 is not None,
+               mu0=mu0 or 0.0)
     e.upload(U)
     ut = e.rhs()
     assert np.abs(ut).max() <= 1e-10, np.abs(ut).max()
-    _, _, errt = e.run(100, 0.9)
+    _, _, errt = e.run(100, 0.9, 0.5)
     assert errt == 0
     err = np.abs(e.download() - U).max()
     print(f"  RHS max {np.abs(ut).max():.2e}, L_inf after 100 steps {err:.2e}")
@@ -145,7 +157,7 @@ def test_mortars_reject_what_is_not_built():
     from fluxo_amd.native import Engine, FxError
     from fluxo_amd.mesh import hopr_mesh
     m = hopr_mesh(os.path.join(IO_DIR, DEFORMED), 3)
-    for kw in (dict(parabolic=True, mu0=1e-3), dict(nfvse=True)):
+    for kw in (dict(parabolic=True, mu0=1e-3, lifting=2), dict(nfvse=True)):
         with pytest.raises(FxError, match="mortars"):
             Engine(m, **kw)
     with pytest.raises(FxError, match="whichMortar"):
