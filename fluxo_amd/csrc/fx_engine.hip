@@ -226,9 +226,8 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   std::vector<int> mort;
   int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S, nm, m->MortarType, m->MortarInfo, &mort);
   if (rc) return rc;
-  if (nm > 0 && (b.mhd() || (b.cfg.parabolic && b.cfg.lifting != 1) || b.cfg.nfvse))
-    return set_err(FX_EINVAL, "meshes with mortars: Euler or Navier-Stokes with BR1 lifting, without shock "
-                   "capturing (nVar 5, lifting 1, nfvse 0)");
+  if (nm > 0 && ((b.cfg.parabolic && b.cfg.lifting != 1) || b.cfg.nfvse))
+    return set_err(FX_EINVAL, "meshes with mortars: BR1 lifting only and no shock capturing (lifting 1, nfvse 0)");
   if (nm > 0 && b.cfg.whichMortar != 0 && b.cfg.whichMortar != 1)
     return set_err(FX_EINVAL, "whichMortar=%d (0 projection, 1 collocation)", b.cfg.whichMortar);
   const size_t n3E = (size_t)b.n3 * E, nf = (size_t)b.n2 * S;
@@ -242,11 +241,12 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
       b.LiftS = nullptr; b.LiftM = nullptr; b.GradM = nullptr;
       HIPCHK(hipMalloc((void **)&b.mort, mort.size() * sizeof(int)));
       HIPCHK(hipMalloc((void **)&b.mops, 4 * (size_t)b.n2 * 8));
-      HIPCHK(hipMalloc((void **)&b.Umort, 5 * nf * 8));
+      const size_t nv = (size_t)b.cfg.nVar;
+      HIPCHK(hipMalloc((void **)&b.Umort, nv * nf * 8));
       if (b.cfg.parabolic) {
-        HIPCHK(hipMalloc((void **)&b.LiftS, 15 * nf * 8));
-        HIPCHK(hipMalloc((void **)&b.LiftM, 15 * (size_t)nm * b.n2 * 8));
-        HIPCHK(hipMalloc((void **)&b.GradM, 15 * (size_t)nm * b.n2 * 8));
+        HIPCHK(hipMalloc((void **)&b.LiftS, 3 * nv * nf * 8));
+        HIPCHK(hipMalloc((void **)&b.LiftM, 3 * nv * nm * b.n2 * 8));
+        HIPCHK(hipMalloc((void **)&b.GradM, 3 * nv * nm * b.n2 * 8));
       }
     }
     double mops[4 * (MAXN + 1) * (MAXN + 1)];

@@ -692,7 +692,7 @@ __device__ __forceinline__ void mortar_interp(int type, int im, int p, int q, co
 // master state of each small side, interpolated from the big element's face (its master side,
 // read from the volume as ProlongToFace would), in the reference's order of summation, into
 // Um at the small side's face points.  One thread per (mortar, small side, p, q).
-template <int N>
+template <int N, int NV>
 __global__ void __launch_bounds__(256)
 k_mortar_u(const double *__restrict__ U, const int *__restrict__ mort, const double *__restrict__ mops,
            double *__restrict__ Um, int nMortar)
@@ -704,13 +704,13 @@ k_mortar_u(const double *__restrict__ U, const int *__restrict__ mort, const dou
   const int *mi = mort + MORT_INTS * m;
   const int type = mi[0], lb = mi[2];
   if (type != 1 && im >= 2) return;
-  const double *ub = U + (size_t)mi[1] * n3 * 5;
-  double o[5];
-  mortar_interp<N, 5>(type, im, pq % n, pq / n, mops,
-                      [&](int a, int b) { return ub + side_to_vol<N>(lb, 0, a, b) * 5; }, o);
-  double *dst = Um + ((size_t)(mi[3 + im] - 1) * n2 + pq) * 5;
+  const double *ub = U + (size_t)mi[1] * n3 * NV;
+  double o[NV];
+  mortar_interp<N, NV>(type, im, pq % n, pq / n, mops,
+                       [&](int a, int b) { return ub + side_to_vol<N>(lb, 0, a, b) * NV; }, o);
+  double *dst = Um + ((size_t)(mi[3 + im] - 1) * n2 + pq) * NV;
 #pragma unroll
-  for (int v = 0; v < 5; v++) dst[v] = o[v];
+  for (int v = 0; v < NV; v++) dst[v] = o[v];
 }
 
 // BR1 lifting with entropy variables (Lifting br1.f90:154-364: Lifting_VolInt

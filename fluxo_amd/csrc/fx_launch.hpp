@@ -445,12 +445,10 @@ inline void q_volume(const Batch &b, int mode, RKArgs rk)
 }
 inline void q_surface(const Batch &b)
 {
+  if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_mortar_u_n, b); }
   if (b.mhd()) { FX_DISPATCH_N(b.cfg.N, launch_mhd_surface_n, b); }
-  else {
-    if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_mortar_u_n, b); }
-    FX_DISPATCH_N(b.cfg.N, launch_surface_n, b);
-    if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_flux_mortar_n, b); }
-  }
+  else { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
+  if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_flux_mortar_n, b); }
 }
 inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out); }
 

@@ -91,8 +91,11 @@ void launch_mortar_u_n(const Batch &b)
 {
   const long long tot = (long long)b.nMortar * 4 * b.n2;
   const int nb = (int)((tot + 255) / 256);
-  if (nb > 0)
-    hipLaunchKernelGGL((k_mortar_u<N>), dim3(nb), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
+  if (nb <= 0) return;
+  if (b.mhd())
+    hipLaunchKernelGGL((k_mortar_u<N, 9>), dim3(nb), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
+  else
+    hipLaunchKernelGGL((k_mortar_u<N, 5>), dim3(nb), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
 }
 
 template <int N>
@@ -100,7 +103,11 @@ void launch_flux_mortar_n(const Batch &b)
 {
   const long long tot = (long long)b.nMortar * b.n2;
   const int nb = (int)((tot + 255) / 256);
-  if (nb > 0)
+  if (nb <= 0) return;
+  if (b.mhd())  // the small sides' master fluxes (with their Powell terms) onto the big side
+    hipLaunchKernelGGL((k_flux_mortar<N, 9>), dim3(nb), dim3(256), 0, b.stream, b.Flux, b.Flux, b.mort, b.mops,
+                       b.nMortar);
+  else
     hipLaunchKernelGGL((k_flux_mortar<N, 5>), dim3(nb), dim3(256), 0, b.stream, b.Flux, b.Flux, b.mort, b.mops,
                        b.nMortar);
 }
@@ -172,7 +179,8 @@ void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
 {
   const int nbs = (int)(((long long)b.nMortar * 4 * b.n2 + 255) / 256);
   const int nbb = (int)(((long long)b.nMortar * b.n2 + 255) / 256);
-  hipLaunchKernelGGL((k_mortar_u<N>), dim3(nbs), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort, b.nMortar);
+  hipLaunchKernelGGL((k_mortar_u<N, EQ::NV>), dim3(nbs), dim3(256), 0, b.stream, b.U, b.mort, b.mops, b.Umort,
+                     b.nMortar);
   hipLaunchKernelGGL((k_lift_mortar_small<N, EQ>), dim3(nbs), dim3(256), 0, b.stream, b.U, b.Umort, b.s2e, b.nvse,
                      b.mort, b.LiftS, b.nMortar, eq);
   hipLaunchKernelGGL((k_flux_mortar<N, 3 * EQ::NV>), dim3(nbb), dim3(256), 0, b.stream, b.LiftS, b.LiftM, b.mort,
@@ -200,7 +208,8 @@ void launch_lifting_n(const Batch &b, int out)
     eq.mu = b.cfg.mu0;
     eq.etasmu0 = b.cfg.eta * eq.smu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    launch_lifting_e<N>(b, out, eq);
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);  // BR1 only (checked at mesh upload)
+    else launch_lifting_e<N>(b, out, eq);
   } else {
     NsLift eq;
     eq.kappa = b.cfg.kappa;
@@ -265,10 +274,10 @@ void launch_mhd_surface_r(const Batch &b)
   if (nb <= 0) return;
   if (b.cfg.parabolic)
     hipLaunchKernelGGL((k_mhd_surface<N, RS, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
-                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
+                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st, b.Umort);
   else
     hipLaunchKernelGGL((k_mhd_surface<N, RS, false>), dim3(nb), dim3(256), 0, b.stream, b.U, b.FvN, b.s2e, b.nvse,
-                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st);
+                       b.Flux, b.FluxS, b.cfg.nSides, b.mhd_args(), b.st, b.Umort);
 }
 
 template <int N>

@@ -286,20 +286,24 @@ __device__ __forceinline__ void mhd_riemann_ec(const double *UL, const double *U
 // (RiemannSolverByRusanov mhd/riemann.f90:273-308), entropy-stable LLF 10, Derigs EC 11 --
 // in the resistive case plus the BR1 average of the diffusive fluxes (Riemann
 // mhd/riemann.f90:83-142), then each side's Powell term, times SurfElem.
+// Um (meshes with mortars, else null): the master states of the small sides of mortars; a big
+// mortar side has no slave element and is skipped (its fluxes come from Flux_Mortar).
 template <int N, int RS, bool PARA>
 __global__ void __launch_bounds__(256)
 k_mhd_surface(const double *__restrict__ U, const double *__restrict__ FvN, const int *__restrict__ s2e,
               const double4 *__restrict__ nvse, double *__restrict__ FluxM, double *__restrict__ FluxS, int nSides,
-              MhdArgs a, const DtState *__restrict__ st)
+              MhdArgs a, const DtState *__restrict__ st, const double *__restrict__ Um)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nSides * n2) return;
   const int s = gid / n2, pq = gid - s * n2, p = pq % n, q = pq / n;
   const int em = s2e[5 * s] - 1, es = s2e[5 * s + 1] - 1;
-  if (em < 0) return;
-  const int vm = side_to_vol<N>(s2e[5 * s + 2], 0, p, q), vs = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
-  const double *um = U + ((size_t)em * n3 + vm) * MV, *us = U + ((size_t)es * n3 + vs) * MV;
+  if (es < 0 || !(em >= 0 || (em == -2 && Um != nullptr))) return;
+  const int vs = side_to_vol<N>(s2e[5 * s + 3], s2e[5 * s + 4], p, q);
+  const double *um = em >= 0 ? U + ((size_t)em * n3 + side_to_vol<N>(s2e[5 * s + 2], 0, p, q)) * MV
+                             : Um + (size_t)gid * MV;
+  const double *us = U + ((size_t)es * n3 + vs) * MV;
   double UL[MV], UR[MV];
 #pragma unroll
   for (int v = 0; v < MV; v++) { UL[v] = um[v]; UR[v] = us[v]; }

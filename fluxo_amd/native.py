@@ -470,7 +470,7 @@ class MhdEngine(Engine):
     (lifting of the entropy variables: 1 BR1, 2 BR2 with etaBR2)."""
 
     def __init__(self, mesh, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, precision=FX_PREC_FP64, resistive=None,
-                 riemann=1, lifting=1, etaBR2=2.0):
+                 riemann=1, lifting=1, etaBR2=2.0, which_mortar=0):
         L = lib()
         self.N = int(mesh["N"])
         self.nElems = int(mesh["nElems"])
@@ -481,6 +481,7 @@ class MhdEngine(Engine):
         self.cfg.nVar, self.cfg.mu_0, self.cfg.GLM_scale = 9, mu_0, glm_scale
         self.cfg.eta = float(res.get("eta", 0.0))
         self.cfg.lifting, self.cfg.etaBR2 = lifting, etaBR2
+        self.cfg.whichMortar = which_mortar
         self._ctx = C.c_void_p()
         _chk(L.fx_ctx_create(C.byref(self.cfg), C.byref(self._ctx)))
         m, _keep = _mesh_struct(mesh)

@@ -112,9 +112,9 @@ typedef struct {
      the sides 1..nMortarSides are big mortar sides, each with 4 (MortarType 1) or 2 (2: halves
      in eta, 3: halves in xi) small sides, whose master data are interpolated from the big
      side (U_Mortar) and whose fluxes are projected back onto it (Flux_Mortar).  NULL / 0 on
-     a conforming mesh.  Euler, or Navier-Stokes with BR1 lifting (nVar 5, lifting 1; br1.f90
-     Flux_Cons_Mortar of the lifting fluxes and U_Mortar of the gradients), no shock
-     capturing. */
+     a conforming mesh.  Euler, GLM-MHD (the small sides' master fluxes with their Powell
+     terms are projected), and with parabolic = 1 BR1 lifting only (br1.f90: Flux_Cons_Mortar
+     of the lifting fluxes, U_Mortar of the gradients); no shock capturing. */
   const int *MortarType;          /* (2,nSides) */
   const int *MortarInfo;          /* (2,0:4,nMortarSides) */
   int nMortarSides;

@@ -151,7 +151,7 @@ def run(mesh, U, volflux=32, riemann=1, parabolic=False, kappa=1.4, mu0=0.0, Pr=
 
 
 def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0, t0=0.0, timeout=600,
-            resistive=None, dfl=0.9, riemann=1, analyze=None, tgv=False, lifting=1, etaBR2=2.0):
+            resistive=None, dfl=0.9, riemann=1, analyze=None, tgv=False, lifting=1, etaBR2=2.0, whichMortar=0):
     """GLM-MHD (Derigs volume flux, surface flux `riemann` -- 0 central, 1 Rusanov, 10 Derigs +
     entropy-stable LLF, 11 Derigs -- Powell non-conservative terms) with
     the reference's code: metrics, VolInt, RHS (GLM_ch = 0.5, its value before the first
@@ -160,8 +160,9 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
     resistivity and heat conduction, which also returns the primitive gradients.
     U: (nElems, n, n, n, 9).  out["analysis"] as in run(); tgv=True runs the build with the
     Taylor-Green vortex testcase (resistive only), whose AnalyzeTestcase adds energies,
-    enstrophies and dissipation rates."""
+    enstrophies and dissipation rates.  Meshes with mortars as in run()."""
     N, Ngeo, E, S = mesh["N"], mesh["Ngeo"], mesh["nElems"], mesh["nSides"]
+    nm = int(mesh.get("nMortarSides", 0))
     n = N + 1
     par_on = resistive is not None
     exe = binary(par_on, 10, mhd=True, lifting=lifting)
@@ -174,7 +175,7 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
             ini = analyze.pop("IniExactFunc", 1)
             f.write(f"N = {N}\nRiemann = {int(riemann)}\nVolumeFlux = 10\nIniExactFunc = {ini}\nIniRefState = 1\n"
                     f"RefState = (/1.,0.,0.,0.,1.,0.,0.,0./)\nkappa = {kappa!r}\nmu_0 = {mu_0!r}\n"
-                    f"GLM_scale = {glm_scale!r}\n")
+                    f"GLM_scale = {glm_scale!r}\nwhichMortar = {int(whichMortar)}\n")
             f.write(analyze_lines(analyze))
             if tgv:
                 f.write("doTCanalyze = T\n")
@@ -185,11 +186,12 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
                         f"Pr = {float(resistive.get('Pr', 0.72))!r}\n")
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
-            np.array([N, Ngeo, E, S, nsteps, int(par_on), 0, 0], np.int32).tofile(f)
+            np.array([N, Ngeo, E, S, nsteps, int(par_on), nm, 0], np.int32).tofile(f)
             np.array([cfl, dfl if par_on else 0.0, t0, 0.0], np.float64).tofile(f)
             np.ascontiguousarray(mesh["NodeCoords"], np.float64).tofile(f)
             np.ascontiguousarray(mesh["ElemToSide"], np.int32).tofile(f)
             np.ascontiguousarray(mesh["SideToElem"], np.int32).tofile(f)
+            _mortar_input(mesh, f)
             np.ascontiguousarray(U, np.float64).tofile(f)
         p = subprocess.run([exe, par, fin, fout], cwd=d, capture_output=True, text=True, errors="replace",
                            timeout=timeout)
@@ -217,7 +219,7 @@ def run_mhd(mesh, U, kappa=5.0 / 3.0, mu_0=1.0, glm_scale=0.5, cfl=0.9, nsteps=0
         for k in ("gradPx", "gradPy", "gradPz"):
             take(k, (E, n, n, n, 9))
     take("Flux_master", (S, n, n, 9))
-    take("Flux_slave", (S, n, n, 9))
+    take("Flux_slave", (S - nm, n, n, 9))
     take("dt0", (2,))
     out["errType0"] = int(out["dt0"][1])
     out["dt0"] = float(out["dt0"][0])

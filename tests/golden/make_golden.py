@@ -410,6 +410,52 @@ def make_mortar(case, mu0=None):
           f"|Ut|max={np.abs(rr['Ut']).max():.3e}")
 
 
+# GLM-MHD on the mortar meshes (Derigs volume flux, Powell terms): name, mesh file, N, Riemann,
+# whichMortar, steps, resistive (None or dict of mu, eta, Pr), CFL
+MORTAR_MHD_CASES = [
+    ("mortar_mhd_deformed_n4_rusanov_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, 1, 0, 100, None,
+     0.5),
+    # (with collocation mortars the reference's own run of this state diverges to NaN within
+    # 20 steps with Riemann 10; projection mortars keep it stable)
+    ("mortar_mhd_4to1_n3_es_proj", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, 10, 0, 20, None, 0.5),
+    ("mortar_mhd_4to1_n3_rusanov_coll", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, 1, 1, 20, None, 0.5),
+    ("mortar_mhd_deformed_n4_resistive_br1_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, 1, 0, 100,
+     dict(mu=1e-3, eta=1e-3, Pr=0.72), 0.5),
+]
+
+
+def make_mortar_mhd(case):
+    from fluxo_amd.mesh import hopr_mesh
+    name, mesh_file, N, rs, wm, nsteps, res, cfl = case
+    m = hopr_mesh(os.path.join(HERE, "io", mesh_file), N)
+    x = m["Elem_xGP"]
+    # a smooth state with all 9 variables varying (Alfven-like), periodic on the unit box
+    s2 = 2.0 * np.pi
+    rho = 1.0 + 0.1 * np.sin(s2 * x[..., 0]) * np.cos(s2 * x[..., 1])
+    v = np.stack([0.3 + 0.1 * np.sin(s2 * x[..., 1]), -0.2 + 0.1 * np.cos(s2 * x[..., 2]),
+                  0.1 + 0.1 * np.sin(s2 * x[..., 0])], -1)
+    B = np.stack([0.5 + 0.05 * np.cos(s2 * x[..., 2]), -0.03 + 0.05 * np.sin(s2 * x[..., 0]),
+                  0.02 + 0.05 * np.cos(s2 * x[..., 1])], -1)
+    p = 1.0 + 0.1 * np.cos(s2 * (x[..., 0] + x[..., 1]))
+    psi = 0.01 * np.sin(s2 * x[..., 2])
+    kappa = 5.0 / 3.0
+    U = np.concatenate([rho[..., None], rho[..., None] * v,
+                        (p / (kappa - 1.0) + 0.5 * rho * np.sum(v * v, -1) + 0.5 * np.sum(B * B, -1)
+                         + 0.5 * psi * psi)[..., None], B, psi[..., None]], -1)
+    rr = R.run_mhd(m, U, kappa=kappa, mu_0=1.0, glm_scale=0.5, cfl=cfl, nsteps=nsteps, resistive=res, dfl=0.5,
+                   riemann=rs, whichMortar=wm)
+    out = dict(mesh_file=mesh_file, N=N, riemann=rs, whichMortar=wm, kappa=kappa, mu_0=1.0, glm_scale=0.5, cfl=cfl,
+               dfl=0.5, nsteps=nsteps, U=U, Ut=rr["Ut"], Flux_master=rr["Flux_master"], Flux_slave=rr["Flux_slave"],
+               dt0=rr["dt0"], glm_ch0=rr["glm_ch0"], glm_dtch1=rr["glm_dtch1"], U_end=rr["U_end"],
+               t_end=rr["t_end"], resistive=int(res is not None))
+    if res is not None:
+        out.update(mu=res["mu"], eta=res["eta"], Pr=res["Pr"], gradPx=rr["gradPx"], gradPy=rr["gradPy"],
+                   gradPz=rr["gradPz"])
+    np.savez(os.path.join(HERE, "mortar", name + ".npz"), **out)
+    print(f"{name}: {m['nMortarSides']} mortars, dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} "
+          f"|Ut|max={np.abs(rr['Ut']).max():.3e}")
+
+
 # Navier-Stokes (BR1, mu0) with NFVSE: name, K, N, sigma, p0, alpha_max, fixed alpha, steps, mu0
 # NFVSE with other point-wise surface fluxes at the sub-cell interfaces (Riemann at run time)
 NFVSE_RIEMANN_CASES = [
@@ -476,6 +522,9 @@ if __name__ == "__main__":
         for c in MORTAR_NS_CASES:
             if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
                 make_mortar(c[:6], mu0=c[6])
+        for c in MORTAR_MHD_CASES:
+            if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
+                make_mortar_mhd(c)
         sys.exit(0)
     for c in CASES:
         make(c)
@@ -505,3 +554,5 @@ if __name__ == "__main__":
         make_mortar(c)
     for c in MORTAR_NS_CASES:
         make_mortar(c[:6], mu0=c[6])
+    for c in MORTAR_MHD_CASES:
+        make_mortar_mhd(c)

@@ -151,6 +151,34 @@ def test_fortran_host_free_stream_on_a_mortar_mesh(tmp_path, which_mortar, parab
     assert linf is not None and np.all(linf <= 1e-11)
 
 
+@pytest.mark.parametrize("which_mortar", [0, 1])
+def test_fortran_host_mhd_free_stream_on_a_mortar_mesh(tmp_path, which_mortar):
+    """The reference's GLM-MHD nonconforming free-stream input (tests/runs/mhd/freestream/
+    nonconforming_projmortar and _collmortar: DeformedBoxMortarPeriodic Ngeo 2, N=4, Riemann 1,
+    RefState (1, 0.1, 0.2, -0.3, 20, 0.5, -0.03, 0.02), eta 1e-5, mu 1e-6, CFL 0.9, DFL 0.5)
+    with the Derigs volume flux (VolumeFlux 10; the reference's input uses 1, which is not
+    built for GLM-MHD here), to tEnd 0.5 instead of 3: L_inf below 1e-11."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "io")
+    ini = tmp_path / "parameter.ini"
+    ini.write_text(
+        f"EquationSystem = mhd\nN = 4\nRiemann = 1\nVolumeFlux = 10\nwhichMortar = {which_mortar}\n"
+        f"MeshFile = {gold}/DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5\n"
+        "IniExactFunc = 1\nIniRefState = 1\nRefState = (/1.0, 0.1, 0.2, -0.3, 20., 0.5, -0.03, 0.02 /)\n"
+        "Parabolic = T\neta = 1.0E-05\nmu = 1.0E-06\n"
+        "tEnd = 0.5\nCFLscale = 0.9\nDFLscale = 0.5\n")
+    print(f"fluxo_gpu MHD mortar free stream whichMortar={which_mortar}", flush=True)
+    p = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert p.returncode == 0 and "FLUXO_GPU FINISHED" in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+    linf = None
+    for line in p.stdout.splitlines():
+        if line.strip().startswith("L_inf "):
+            linf = np.array([float(x) for x in line.split(":")[1].split()])
+    print(f"  L_inf {linf}")
+    assert linf is not None and np.all(linf <= 1e-11)
+
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

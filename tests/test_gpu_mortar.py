@@ -1,8 +1,9 @@
 """The engine on the reference's non-conforming HOPR meshes (mortars; read by fluxo_amd.io,
 metrics from the Fortran host) against the reference's code on the same meshes and states
-(tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt, for Navier-Stokes the BR1
-lifting with Flux_Cons_Mortar and U_Mortar of the gradients, and the LSERK trajectory of
-the reference, run by its harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
+(tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt, for Navier-Stokes and
+resistive GLM-MHD the BR1 lifting with Flux_Cons_Mortar and U_Mortar of the gradients, for
+GLM-MHD the Powell terms and GLM_ch, and the LSERK trajectory of the reference, run by its
+harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
 RHS and face flux, <= 1e-10 after the stored RK steps (100 on three cases).  Properties the
 reference's own free-stream mortar tests check (tests/runs/navst/freestream/
 nonconforming_projmortar, nonconforming_collmortar: L_inf <= 1e-11) and conservation across
@@ -33,8 +34,13 @@ def load(name):
 
 def engine(g, **kw):
     from fluxo_amd.mesh import hopr_mesh
-    from fluxo_amd.native import Engine
+    from fluxo_amd.native import Engine, MhdEngine
     m = hopr_mesh(os.path.join(IO_DIR, str(g["mesh_file"])), int(g["N"]))
+    if "glm_scale" in g:  # GLM-MHD
+        res = dict(mu=float(g["mu"]), eta=float(g["eta"]), Pr=float(g["Pr"])) if int(g["resistive"]) else None
+        e = MhdEngine(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
+                      riemann=int(g["riemann"]), resistive=res, which_mortar=int(g["whichMortar"]), **kw)
+        return e, m
     par = bool(int(g.get("parabolic", 0)))
     e = Engine(m, volume_flux=int(g["volflux"]), riemann=int(g["riemann"]), which_mortar=int(g["whichMortar"]),
                parabolic=par, mu0=float(g.get("mu0", 0.0)), Pr=float(g.get("Pr", 0.72)), **kw)
@@ -59,7 +65,7 @@ def test_rhs_and_face_fluxes_match_reference(name):
     big = rel_l2(fm[:nm], g["Flux_master"][:nm])
     print(f"{name}: RHS {err:.3e}  Flux_master {ferr:.3e} (big sides {big:.3e})  Flux_slave {serr:.3e}")
     assert err <= 1e-12 and ferr <= 1e-12 and serr <= 1e-12 and big <= 1e-12
-    if "gradPx" in g:
+    if "gradPx" in g or "glm_scale" in g:
         return
     # Euler: SurfInt alone on the reference's fluxes and volume integral gives its RHS
     fsp = np.concatenate([np.zeros_like(fm[:nm]), g["Flux_slave"]])
@@ -73,7 +79,8 @@ def test_trajectory_matches_reference(name):
     print(f"case {name}", flush=True)
     e, _ = engine(g)
     e.upload(g["U"])
-    t, _, errt = e.run(int(g["nsteps"]), float(g["cfl"]), float(g.get("dfl", 0.0)))
+    dfl = float(g.get("dfl", 0.0)) if int(g.get("parabolic", g.get("resistive", 0))) else 0.0
+    t, _, errt = e.run(int(g["nsteps"]), float(g["cfl"]), dfl)
     traj = rel_l2(e.download(), g["U_end"])
     print(f"{name}: {int(g['nsteps'])} steps relL2 vs reference {traj:.3e}")
     assert errt == 0 and traj <= 1e-10
@@ -160,5 +167,8 @@ def test_mortars_reject_what_is_not_built():
     for kw in (dict(parabolic=True, mu0=1e-3, lifting=2), dict(nfvse=True)):
         with pytest.raises(FxError, match="mortars"):
             Engine(m, **kw)
+    from fluxo_amd.native import MhdEngine
+    with pytest.raises(FxError, match="mortars"):
+        MhdEngine(m, resistive=dict(mu=1e-3, eta=1e-3, Pr=0.72), lifting=2)
     with pytest.raises(FxError, match="whichMortar"):
         Engine(m, which_mortar=2)
