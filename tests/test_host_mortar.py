@@ -113,6 +113,8 @@ def test_host_mortar_metrics_match_reference(name):
     from fluxo_amd.mesh import hopr_mesh
     with np.load(os.path.join(MORTAR_DIR, name + ".npz"), allow_pickle=False) as z:
         g = {k: z[k] for k in z.files}
+    if "Mf" not in g:
+        pytest.skip("case stores no metrics (the GLM-MHD cases run on meshes the others cover)")
     m = hopr_mesh(os.path.join(IO_DIR, str(g["mesh_file"])), int(g["N"]))
     for k in ("Mf", "Mg", "Mh", "sJ", "NormVec", "TangVec1", "TangVec2", "SurfElem"):
         assert np.abs(m[k] - g[k]).max() <= 1e-14 * np.abs(g[k]).max(), k
