@@ -226,8 +226,7 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   std::vector<int> mort;
   int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S, nm, m->MortarType, m->MortarInfo, &mort);
   if (rc) return rc;
-  if (nm > 0 && ((b.cfg.parabolic && b.cfg.lifting != 1) || b.cfg.nfvse))
-    return set_err(FX_EINVAL, "meshes with mortars: BR1 lifting only and no shock capturing (lifting 1, nfvse 0)");
+  if (nm > 0 && b.cfg.nfvse) return set_err(FX_EINVAL, "meshes with mortars: no shock capturing (nfvse 0)");
   if (nm > 0 && b.cfg.whichMortar != 0 && b.cfg.whichMortar != 1)
     return set_err(FX_EINVAL, "whichMortar=%d (0 projection, 1 collocation)", b.cfg.whichMortar);
   const size_t n3E = (size_t)b.n3 * E, nf = (size_t)b.n2 * S;

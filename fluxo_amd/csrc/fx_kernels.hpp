@@ -902,11 +902,18 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
         // F_loc = sJ L_HatMinus0 Flux: into the volume gradient, and etaBR2 F_loc on the
         // volume-only gradient is this face's gradient
         const double sl = sj * L0;
+        const bool mbig = (mcode >> 1) < nMortar * n2;  // big mortar side: the projected fluxes
         double fx[NV], fy[NV], fz[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          const double f = d[v];
-          const double lx = sl * (f * nx), ly = sl * (f * ny), lz = sl * (f * nz);
+          double lx, ly, lz;
+          if (mbig) {
+            const double *lm = LiftM + (size_t)(mcode >> 1) * 3 * NV;
+            lx = sl * lm[v]; ly = sl * lm[NV + v]; lz = sl * lm[2 * NV + v];
+          } else {
+            const double f = d[v];
+            lx = sl * (f * nx); ly = sl * (f * ny); lz = sl * (f * nz);
+          }
           gx[v] = gx[v] + lx;
           gy[v] = gy[v] + ly;
           gz[v] = gz[v] + lz;
@@ -914,7 +921,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
           fy[v] = geta[v] + etaBR2 * ly;
           fz[v] = gzeta[v] + etaBR2 * lz;
         }
-        if (OUT == 1) {
+        if (mbig) {
+          // the big side's face gradients of the entropy variables, for U_Mortar (converted to
+          // primitive gradients with the small sides' interpolated states, br2.f90:271-290)
+          double *gm = GradM + (size_t)(mcode >> 1) * 3 * NV;
+#pragma unroll
+          for (int v = 0; v < NV; v++) { gm[v] = fx[v]; gm[NV + v] = fy[v]; gm[2 * NV + v] = fz[v]; }
+        } else if (OUT == 1) {
           double fv[12], o[NVV];
           eq.grad_prim(u, fx);
           eq.grad_prim(u, fy);
@@ -934,7 +947,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     eq.grad_prim(u, gx);
     eq.grad_prim(u, gy);
     eq.grad_prim(u, gz);
-    if (nMortar > 0) {  // the big side's prolonged gradients, for U_Mortar of gradPx/y/z
+    if (LIFT == 1 && nMortar > 0) {  // the big side's prolonged gradients, for U_Mortar of gradPx/y/z
       const NodeFaces nfm = node_faces<N>(i, j, k);
 #pragma unroll
       for (int t = 0; t < 3; t++) {
@@ -1054,11 +1067,12 @@ k_lift_mortar_small(const double *__restrict__ U, const double *__restrict__ Um,
 }
 
 // The master side's viscous flux on the small sides of mortars: U_Mortar of the big side's
-// primitive gradients (br1.f90:245-322) and of its state (Um), EvalDiffFlux3D and the
-// small side's normal into the master slot of FvN (as the lifting kernel writes it for an
-// element's own faces), whose mean with the small element's is the BR1 viscous surface flux
-// (riemann.f90:155-168).  One thread per (mortar, small side, p, q).
-template <int N, class EQ>
+// gradients (BR1: primitive gradients, br1.f90:245-322; BR2 (CONV): the face gradients of the
+// entropy variables, converted here with the interpolated state, br2.f90:271-290) and of its
+// state (Um), EvalDiffFlux3D and the small side's normal into the master slot of FvN (as the
+// lifting kernel writes it for an element's own faces), whose mean with the small element's
+// is the viscous surface flux (riemann.f90:155-168).  One thread per (mortar, small side, p, q).
+template <int N, class EQ, bool CONV>
 __global__ void __launch_bounds__(256)
 k_mortar_fv(const double *__restrict__ Um, const double *__restrict__ GradM, const double4 *__restrict__ nvse,
             const int *__restrict__ mort, const double *__restrict__ mops, double *__restrict__ FvN, int nMortar,
@@ -1078,6 +1092,11 @@ k_mortar_fv(const double *__restrict__ Um, const double *__restrict__ GradM, con
   double u[NV], fv[12], o[NVV];
 #pragma unroll
   for (int v = 0; v < NV; v++) u[v] = Um[fpt * NV + v];
+  if (CONV) {
+    eq.grad_prim(u, gr);
+    eq.grad_prim(u, gr + NV);
+    eq.grad_prim(u, gr + 2 * NV);
+  }
   eq.fluxes(u, gr, gr + NV, gr + 2 * NV, fv);
   const double4 g = nvse[fpt];
   EQ::vis_dir(fv, g.x, g.y, g.z, o);

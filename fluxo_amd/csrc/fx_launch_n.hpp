@@ -171,9 +171,10 @@ void launch_lifting_l(const Batch &b, int out, const EQ &eq)
                        b.Umort, b.LiftM, b.GradM, b.nMortar);
 }
 
-// BR1 lifting on a mesh with mortars: U_Mortar of the state, the small sides' lifting fluxes
-// and their Flux_Cons_Mortar onto the big sides before the lifting kernel; after it (viscous
-// form) U_Mortar of the big sides' gradients and the master viscous flux of the small sides.
+// Lifting (BR1 or BR2) on a mesh with mortars: U_Mortar of the state, the small sides' lifting
+// fluxes and their Flux_Cons_Mortar onto the big sides before the lifting kernel; after it
+// (viscous form) U_Mortar of the big sides' gradients and the master viscous flux of the small
+// sides.
 template <int N, class EQ>
 void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
 {
@@ -185,10 +186,17 @@ void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
                      b.mort, b.LiftS, b.nMortar, eq);
   hipLaunchKernelGGL((k_flux_mortar<N, 3 * EQ::NV>), dim3(nbb), dim3(256), 0, b.stream, b.LiftS, b.LiftM, b.mort,
                      b.mops, b.nMortar);
-  launch_lifting_l<N, EQ, 1>(b, out, eq);
-  if (out == 1)
-    hipLaunchKernelGGL((k_mortar_fv<N, EQ>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse, b.mort,
-                       b.mops, b.FvN, b.nMortar, eq);
+  if (b.cfg.lifting == 2) {
+    launch_lifting_l<N, EQ, 2>(b, out, eq);
+    if (out == 1)
+      hipLaunchKernelGGL((k_mortar_fv<N, EQ, true>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse,
+                         b.mort, b.mops, b.FvN, b.nMortar, eq);
+  } else {
+    launch_lifting_l<N, EQ, 1>(b, out, eq);
+    if (out == 1)
+      hipLaunchKernelGGL((k_mortar_fv<N, EQ, false>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse,
+                         b.mort, b.mops, b.FvN, b.nMortar, eq);
+  }
 }
 
 template <int N, class EQ>
@@ -208,14 +216,14 @@ void launch_lifting_n(const Batch &b, int out)
     eq.mu = b.cfg.mu0;
     eq.etasmu0 = b.cfg.eta * eq.smu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);  // BR1 only (checked at mesh upload)
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);
     else launch_lifting_e<N>(b, out, eq);
   } else {
     NsLift eq;
     eq.kappa = b.cfg.kappa;
     eq.mu0 = b.cfg.mu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);  // BR1 only (checked at mesh upload)
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);
     else launch_lifting_e<N>(b, out, eq);
   }
 }

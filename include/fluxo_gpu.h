@@ -112,9 +112,10 @@ typedef struct {
      the sides 1..nMortarSides are big mortar sides, each with 4 (MortarType 1) or 2 (2: halves
      in eta, 3: halves in xi) small sides, whose master data are interpolated from the big
      side (U_Mortar) and whose fluxes are projected back onto it (Flux_Mortar).  NULL / 0 on
-     a conforming mesh.  Euler, GLM-MHD (the small sides' master fluxes with their Powell
-     terms are projected), and with parabolic = 1 BR1 lifting only (br1.f90: Flux_Cons_Mortar
-     of the lifting fluxes, U_Mortar of the gradients); no shock capturing. */
+     a conforming mesh.  Euler, Navier-Stokes and GLM-MHD (the small sides' master fluxes
+     with their Powell terms are projected), BR1 or BR2 lifting (Flux_Cons_Mortar of the
+     lifting fluxes, U_Mortar of the gradients: br1.f90:217-322, br2.f90:220-290); no shock
+     capturing. */
   const int *MortarType;          /* (2,nSides) */
   const int *MortarInfo;          /* (2,0:4,nMortarSides) */
   int nMortarSides;

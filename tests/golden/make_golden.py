@@ -385,9 +385,15 @@ MORTAR_NS_CASES = [
     ("mortar_ns_2to1_n3_br1_coll", "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5", 3, (32, 1), 1, 20, 1e-3),
     ("mortar_ns_4to1_n3_br1_proj", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 0, 20, 1e-3),
 ]
+# the same with BR2 lifting (etaBR2 2): name, mesh file, N, (volflux, riemann), whichMortar, steps, mu0
+MORTAR_NS_BR2_CASES = [
+    ("mortar_ns_deformed_n4_br2_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 1), 0, 100,
+     1e-3),
+    ("mortar_ns_4to1_n3_br2_coll", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 1, 20, 1e-3),
+]
 
 
-def make_mortar(case, mu0=None):
+def make_mortar(case, mu0=None, lifting=1):
     from fluxo_amd.mesh import hopr_mesh
     name, mesh_file, N, (vf, rs), wm, nsteps = case
     par = mu0 is not None
@@ -395,9 +401,10 @@ def make_mortar(case, mu0=None):
     U = O.exactfunc(2, m["Elem_xGP"])
     U = U * (1.0 + 1e-3 * np.random.default_rng(20261205).uniform(-1.0, 1.0, U.shape))
     rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=par, mu0=mu0 if par else 0.0, Pr=0.72, cfl=0.9, dfl=0.9,
-               nsteps=nsteps, whichMortar=wm)
+               nsteps=nsteps, whichMortar=wm, lifting=lifting, etaBR2=2.0)
     out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, whichMortar=wm, kappa=1.4, cfl=0.9, dfl=0.9,
-               parabolic=int(par), mu0=mu0 if par else 0.0, Pr=0.72, nsteps=nsteps, U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
+               parabolic=int(par), mu0=mu0 if par else 0.0, Pr=0.72, lifting=lifting, etaBR2=2.0, nsteps=nsteps,
+               U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
                Flux_slave=rr["Flux_slave"], dt0=rr["dt0"], errType0=rr["errType0"], U_end=rr["U_end"],
                t_end=rr["t_end"], sJ=rr["sJ"], Mf=rr["Mf"], Mg=rr["Mg"], Mh=rr["Mh"], NormVec=rr["NormVec"],
                TangVec1=rr["TangVec1"], TangVec2=rr["TangVec2"], SurfElem=rr["SurfElem"], Entropy=rr["Entropy"],
@@ -421,6 +428,8 @@ MORTAR_MHD_CASES = [
     ("mortar_mhd_4to1_n3_rusanov_coll", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, 1, 1, 20, None, 0.5),
     ("mortar_mhd_deformed_n4_resistive_br1_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, 1, 0, 100,
      dict(mu=1e-3, eta=1e-3, Pr=0.72), 0.5),
+    ("mortar_mhd_deformed_n4_resistive_br2_coll", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, 1, 1, 20,
+     dict(mu=1e-3, eta=1e-3, Pr=0.72, lifting=2), 0.5),
 ]
 
 
@@ -442,15 +451,16 @@ def make_mortar_mhd(case):
     U = np.concatenate([rho[..., None], rho[..., None] * v,
                         (p / (kappa - 1.0) + 0.5 * rho * np.sum(v * v, -1) + 0.5 * np.sum(B * B, -1)
                          + 0.5 * psi * psi)[..., None], B, psi[..., None]], -1)
+    lifting = int(res.get("lifting", 1)) if res else 1
     rr = R.run_mhd(m, U, kappa=kappa, mu_0=1.0, glm_scale=0.5, cfl=cfl, nsteps=nsteps, resistive=res, dfl=0.5,
-                   riemann=rs, whichMortar=wm)
+                   riemann=rs, whichMortar=wm, lifting=lifting, etaBR2=2.0)
     out = dict(mesh_file=mesh_file, N=N, riemann=rs, whichMortar=wm, kappa=kappa, mu_0=1.0, glm_scale=0.5, cfl=cfl,
                dfl=0.5, nsteps=nsteps, U=U, Ut=rr["Ut"], Flux_master=rr["Flux_master"], Flux_slave=rr["Flux_slave"],
                dt0=rr["dt0"], glm_ch0=rr["glm_ch0"], glm_dtch1=rr["glm_dtch1"], U_end=rr["U_end"],
                t_end=rr["t_end"], resistive=int(res is not None))
     if res is not None:
         out.update(mu=res["mu"], eta=res["eta"], Pr=res["Pr"], gradPx=rr["gradPx"], gradPy=rr["gradPy"],
-                   gradPz=rr["gradPz"])
+                   gradPz=rr["gradPz"], lifting=lifting, etaBR2=2.0)
     np.savez(os.path.join(HERE, "mortar", name + ".npz"), **out)
     print(f"{name}: {m['nMortarSides']} mortars, dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} "
           f"|Ut|max={np.abs(rr['Ut']).max():.3e}")
@@ -525,6 +535,9 @@ if __name__ == "__main__":
         for c in MORTAR_MHD_CASES:
             if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
                 make_mortar_mhd(c)
+        for c in MORTAR_NS_BR2_CASES:
+            if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
+                make_mortar(c[:6], mu0=c[6], lifting=2)
         sys.exit(0)
     for c in CASES:
         make(c)
@@ -556,3 +569,5 @@ if __name__ == "__main__":
         make_mortar(c[:6], mu0=c[6])
     for c in MORTAR_MHD_CASES:
         make_mortar_mhd(c)
+    for c in MORTAR_NS_BR2_CASES:
+        make_mortar(c[:6], mu0=c[6], lifting=2)

@@ -1,7 +1,7 @@
 """The engine on the reference's non-conforming HOPR meshes (mortars; read by fluxo_amd.io,
 metrics from the Fortran host) against the reference's code on the same meshes and states
 (tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt, for Navier-Stokes and
-resistive GLM-MHD the BR1 lifting with Flux_Cons_Mortar and U_Mortar of the gradients, for
+resistive GLM-MHD the BR1 and BR2 lifting with Flux_Cons_Mortar and U_Mortar of the gradients, for
 GLM-MHD the Powell terms and GLM_ch, and the LSERK trajectory of the reference, run by its
 harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
 RHS and face flux, <= 1e-10 after the stored RK steps (100 on three cases).  Properties the
@@ -39,11 +39,13 @@ def engine(g, **kw):
     if "glm_scale" in g:  # GLM-MHD
         res = dict(mu=float(g["mu"]), eta=float(g["eta"]), Pr=float(g["Pr"])) if int(g["resistive"]) else None
         e = MhdEngine(m, kappa=float(g["kappa"]), mu_0=float(g["mu_0"]), glm_scale=float(g["glm_scale"]),
-                      riemann=int(g["riemann"]), resistive=res, which_mortar=int(g["whichMortar"]), **kw)
+                      riemann=int(g["riemann"]), resistive=res, which_mortar=int(g["whichMortar"]),
+                      lifting=int(g.get("lifting", 1)), etaBR2=float(g.get("etaBR2", 2.0)), **kw)
         return e, m
     par = bool(int(g.get("parabolic", 0)))
     e = Engine(m, volume_flux=int(g["volflux"]), riemann=int(g["riemann"]), which_mortar=int(g["whichMortar"]),
-               parabolic=par, mu0=float(g.get("mu0", 0.0)), Pr=float(g.get("Pr", 0.72)), **kw)
+               parabolic=par, mu0=float(g.get("mu0", 0.0)), Pr=float(g.get("Pr", 0.72)),
+               lifting=int(g.get("lifting", 1)), etaBR2=float(g.get("etaBR2", 2.0)), **kw)
     return e, m
 
 
@@ -164,11 +166,8 @@ def test_mortars_reject_what_is_not_built():
     from fluxo_amd.native import Engine, FxError
     from fluxo_amd.mesh import hopr_mesh
     m = hopr_mesh(os.path.join(IO_DIR, DEFORMED), 3)
-    for kw in (dict(parabolic=True, mu0=1e-3, lifting=2), dict(nfvse=True)):
+    for kw in (dict(nfvse=True), dict(nfvse=True, parabolic=True, mu0=1e-3)):
         with pytest.raises(FxError, match="mortars"):
             Engine(m, **kw)
-    from fluxo_amd.native import MhdEngine
-    with pytest.raises(FxError, match="mortars"):
-        MhdEngine(m, resistive=dict(mu=1e-3, eta=1e-3, Pr=0.72), lifting=2)
     with pytest.raises(FxError, match="whichMortar"):
         Engine(m, which_mortar=2)
