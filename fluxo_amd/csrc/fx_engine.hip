@@ -208,7 +208,7 @@ void fx_ctx_destroy(fx_ctx *c)
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
                   c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
                   b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs,
-                  b.mort, b.mops, b.Umort, b.LiftS, b.LiftM, b.GradM};
+                  b.mort, b.mside, b.mops, b.Umort, b.LiftS, b.LiftM, b.GradM};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b.stream) (void)hipStreamDestroy(b.stream);
@@ -226,19 +226,19 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   std::vector<int> mort;
   int rc = check_connectivity(m->ElemToSide, m->SideToElem, E, S, nm, m->MortarType, m->MortarInfo, &mort);
   if (rc) return rc;
-  if (nm > 0 && b.cfg.nfvse) return set_err(FX_EINVAL, "meshes with mortars: no shock capturing (nfvse 0)");
   if (nm > 0 && b.cfg.whichMortar != 0 && b.cfg.whichMortar != 1)
     return set_err(FX_EINVAL, "whichMortar=%d (0 projection, 1 collocation)", b.cfg.whichMortar);
   const size_t n3E = (size_t)b.n3 * E, nf = (size_t)b.n2 * S;
   if (nm > 0) {
     HIPCHK(hipStreamSynchronize(b.stream));
     if (b.nMortar != nm) {  // the mortar buffers are sized by the mesh
-      for (void *p : {(void *)b.mort, (void *)b.mops, (void *)b.Umort, (void *)b.LiftS, (void *)b.LiftM,
-                      (void *)b.GradM})
+      for (void *p : {(void *)b.mort, (void *)b.mside, (void *)b.mops, (void *)b.Umort, (void *)b.LiftS,
+                      (void *)b.LiftM, (void *)b.GradM})
         if (p) HIPCHK(hipFree(p));
-      b.mort = nullptr; b.mops = nullptr; b.Umort = nullptr; b.nMortar = 0;
+      b.mort = nullptr; b.mside = nullptr; b.mops = nullptr; b.Umort = nullptr; b.nMortar = 0;
       b.LiftS = nullptr; b.LiftM = nullptr; b.GradM = nullptr;
       HIPCHK(hipMalloc((void **)&b.mort, mort.size() * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&b.mside, (size_t)S * sizeof(int)));
       HIPCHK(hipMalloc((void **)&b.mops, 4 * (size_t)b.n2 * 8));
       const size_t nv = (size_t)b.cfg.nVar;
       HIPCHK(hipMalloc((void **)&b.Umort, nv * nf * 8));
@@ -250,7 +250,11 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
     }
     double mops[4 * (MAXN + 1) * (MAXN + 1)];
     make_mortar_ops(b.cfg.N, b.cfg.whichMortar, mops);
+    std::vector<int> mside(S, -1);
+    for (int m = 0; m < nm; m++)
+      for (int im = 0; im < (mort[MORT_INTS * m] == 1 ? 4 : 2); im++) mside[mort[MORT_INTS * m + 3 + im] - 1] = m;
     HIPCHK(hipMemcpy(b.mort, mort.data(), mort.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.mside, mside.data(), (size_t)S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.mops, mops, 4 * (size_t)b.n2 * 8, hipMemcpyHostToDevice));
     b.nMortar = nm;
   } else {

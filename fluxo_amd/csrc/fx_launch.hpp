@@ -382,7 +382,7 @@ struct Batch {
   // (MORT_INTS ints each), the interpolation / projection operators and the master states of
   // the small sides (face-point layout of Flux)
   int nMortar = 0;
-  int *mort = nullptr;
+  int *mort = nullptr, *mside = nullptr;  // mside: mortar of each small side, -1 elsewhere
   double *mops = nullptr, *Umort = nullptr;
   // BR1 lifting with mortars: the small sides' lifting fluxes (3 nVar per face point), their
   // projection onto the big sides, and the big sides' primitive gradients (3 nVar each)
@@ -495,7 +495,7 @@ inline const double *q_nfvse_alpha(const Batch &b)
   const int nb = (b.cfg.nElems + 255) / 256;
   for (int sw = 0; sw < b.cfg.spacePropSweeps && nb > 0; sw++) {
     hipLaunchKernelGGL(k_nfvse_prop, dim3(nb), dim3(256), 0, b.stream, src, dst, b.e2s, b.s2e, b.eprob,
-                       b.cfg.nElems, b.cfg.spacePropFactor);
+                       b.cfg.nElems, b.cfg.spacePropFactor, b.mort, b.mside, b.nMortar);
     double *t = src; src = dst; dst = t;
   }
   return src;

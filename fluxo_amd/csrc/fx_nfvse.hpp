@@ -98,12 +98,16 @@ k_nfvse_alpha(const double *__restrict__ U, const int *__restrict__ eprob, doubl
 }
 
 #ifdef FX_KERNEL_HOME  // non-template kernel: compiled in one unit only
-// One sweep of PropagateBlendingCoeff (nfvse_mpi.f90:78-153) on conforming sides: every
-// side raises both its elements to factor * max(alpha_master, alpha_slave), from the values
-// before the sweep, so the order of the sides does not matter.
+// One sweep of PropagateBlendingCoeff (nfvse_mpi.f90:78-153): every side raises both its
+// elements to factor * max(alpha_master, alpha_slave), from the values before the sweep, so
+// the order of the sides does not matter.  Across a mortar (Alpha_Mortar nfvse_mpi.f90:
+// 192-240 gives each small side the big element's alpha as its master value) the big element
+// takes the maximum over its small sides and each small element the big element's value;
+// mside maps a small side to its mortar (-1 elsewhere), mort describes the mortars.
 __global__ void __launch_bounds__(256)
 k_nfvse_prop(const double *__restrict__ ain, double *__restrict__ aout, const int *__restrict__ e2s,
-             const int *__restrict__ s2e, const int *__restrict__ eprob, int nElems, double factor)
+             const int *__restrict__ s2e, const int *__restrict__ eprob, int nElems, double factor,
+             const int *__restrict__ mort, const int *__restrict__ mside, int nMortar)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nElems || elem_problem(eprob, e) < 0) return;
@@ -111,7 +115,17 @@ k_nfvse_prop(const double *__restrict__ ain, double *__restrict__ aout, const in
   double r = ae;
   for (int l = 0; l < 6; l++) {
     const int s = e2s[(e * 6 + l) * 2] - 1, fl = e2s[(e * 6 + l) * 2 + 1];
-    const int nb = (fl == 0 ? s2e[5 * s + 1] : s2e[5 * s]) - 1;
+    if (fl == 0 && s < nMortar) {  // big mortar side: its small sides' elements
+      const int *mi = mort + MORT_INTS * s;
+      const int ns = mi[0] == 1 ? 4 : 2;
+      for (int im = 0; im < ns; im++) {
+        const double m = factor * fmax(ae, ain[s2e[5 * (mi[3 + im] - 1) + 1] - 1]);
+        if (r < m) r = m;
+      }
+      continue;
+    }
+    int nb = (fl == 0 ? s2e[5 * s + 1] : s2e[5 * s]) - 1;
+    if (nb < 0) nb = mort[MORT_INTS * mside[s] + 1];  // small side: the big element
     const double m = factor * fmax(ae, ain[nb]);
     if (r < m) r = m;
   }

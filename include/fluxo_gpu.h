@@ -114,8 +114,9 @@ typedef struct {
      side (U_Mortar) and whose fluxes are projected back onto it (Flux_Mortar).  NULL / 0 on
      a conforming mesh.  Euler, Navier-Stokes and GLM-MHD (the small sides' master fluxes
      with their Powell terms are projected), BR1 or BR2 lifting (Flux_Cons_Mortar of the
-     lifting fluxes, U_Mortar of the gradients: br1.f90:217-322, br2.f90:220-290); no shock
-     capturing. */
+     lifting fluxes, U_Mortar of the gradients: br1.f90:217-322, br2.f90:220-290), NFVSE
+     shock capturing (the blending coefficient propagates across mortars as Alpha_Mortar /
+     PropagateBlendingCoeff do, nfvse_mpi.f90:78-240). */
   const int *MortarType;          /* (2,nSides) */
   const int *MortarInfo;          /* (2,0:4,nMortarSides) */
   int nMortarSides;

@@ -385,6 +385,16 @@ MORTAR_NS_CASES = [
     ("mortar_ns_2to1_n3_br1_coll", "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5", 3, (32, 1), 1, 20, 1e-3),
     ("mortar_ns_4to1_n3_br1_proj", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 0, 20, 1e-3),
 ]
+# NFVSE shock capturing (Persson-Peraire indicator, two propagation sweeps) on the mortar
+# meshes, Euler and Navier-Stokes (BR1, mu0): name, mesh file, N, (volflux, riemann), whichMortar,
+# steps, mu0 (None: Euler)
+MORTAR_NFVSE_CASES = [
+    ("mortar_nfvse_deformed_n4_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 1), 0, 20, None),
+    ("mortar_nfvse_2to1_n3_coll", "CartBoxMortarPeriodic_Ngeo1_2to1_mesh.h5", 3, (32, 1), 1, 20, None),
+    ("mortar_nfvse_ns_4to1_n3_proj", "CartBoxMortarPeriodic_Ngeo1_4to1_mesh.h5", 3, (32, 1), 0, 20, 1e-3),
+]
+MORTAR_NFVSE = dict(ComputeAlpha=1, ShockBlendCoef=0.0, alpha_max=0.5, alpha_min=0.01, SpacePropSweeps=2,
+                    SpacePropFactor=0.5)
 # the same with BR2 lifting (etaBR2 2): name, mesh file, N, (volflux, riemann), whichMortar, steps, mu0
 MORTAR_NS_BR2_CASES = [
     ("mortar_ns_deformed_n4_br2_proj", "DeformedBoxMortarPeriodic__Ngeo_2_Level_01_mesh.h5", 4, (32, 1), 0, 100,
@@ -393,15 +403,19 @@ MORTAR_NS_BR2_CASES = [
 ]
 
 
-def make_mortar(case, mu0=None, lifting=1):
+def make_mortar(case, mu0=None, lifting=1, nfvse=None):
     from fluxo_amd.mesh import hopr_mesh
     name, mesh_file, N, (vf, rs), wm, nsteps = case
     par = mu0 is not None
     m = hopr_mesh(os.path.join(HERE, "io", mesh_file), N)
     U = O.exactfunc(2, m["Elem_xGP"])
     U = U * (1.0 + 1e-3 * np.random.default_rng(20261205).uniform(-1.0, 1.0, U.shape))
+    if nfvse is not None:  # a pressure pulse that the indicator sees in some elements
+        x = m["Elem_xGP"]
+        r2 = np.sum((x - 0.3) ** 2, -1)
+        U[..., 4] = U[..., 4] + 2.5 * np.exp(-r2 / 0.02)
     rr = R.run(m, U, volflux=vf, riemann=rs, parabolic=par, mu0=mu0 if par else 0.0, Pr=0.72, cfl=0.9, dfl=0.9,
-               nsteps=nsteps, whichMortar=wm, lifting=lifting, etaBR2=2.0)
+               nsteps=nsteps, whichMortar=wm, lifting=lifting, etaBR2=2.0, nfvse=nfvse)
     out = dict(mesh_file=mesh_file, N=N, volflux=vf, riemann=rs, whichMortar=wm, kappa=1.4, cfl=0.9, dfl=0.9,
                parabolic=int(par), mu0=mu0 if par else 0.0, Pr=0.72, lifting=lifting, etaBR2=2.0, nsteps=nsteps,
                U=U, Ut=rr["Ut"], volint=rr["volint"], Flux_master=rr["Flux_master"],
@@ -411,6 +425,13 @@ def make_mortar(case, mu0=None, lifting=1):
                dSdU_Ut=rr["dSdU_Ut"])
     if par:
         out.update(gradPx=rr["gradPx"], gradPy=rr["gradPy"], gradPz=rr["gradPz"])
+    if nfvse is not None:
+        out.update(nfvse=1, alpha=rr["alpha"], volint_nfvse=rr["volint_nfvse"], alpha_max=nfvse["alpha_max"],
+                   alpha_min=nfvse["alpha_min"], sweeps=nfvse["SpacePropSweeps"],
+                   prop_factor=nfvse["SpacePropFactor"])
+        a = rr["alpha"]
+        print(f"  alpha: zero {np.sum(a == 0)} max {np.sum(a == nfvse['alpha_max'])} between "
+              f"{np.sum((a > 0) & (a < nfvse['alpha_max']))}")
     os.makedirs(os.path.join(HERE, "mortar"), exist_ok=True)
     np.savez(os.path.join(HERE, "mortar", name + ".npz"), **out)
     print(f"{name}: {m['nMortarSides']} mortars, dt0={rr['dt0']:.6e} t_end={rr['t_end']:.6e} "
@@ -538,6 +559,9 @@ if __name__ == "__main__":
         for c in MORTAR_NS_BR2_CASES:
             if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
                 make_mortar(c[:6], mu0=c[6], lifting=2)
+        for c in MORTAR_NFVSE_CASES:
+            if c[0] in sys.argv[1:] or "mortar" in sys.argv[1:]:
+                make_mortar(c[:6], mu0=c[6], nfvse=MORTAR_NFVSE)
         sys.exit(0)
     for c in CASES:
         make(c)
@@ -571,3 +595,5 @@ if __name__ == "__main__":
         make_mortar_mhd(c)
     for c in MORTAR_NS_BR2_CASES:
         make_mortar(c[:6], mu0=c[6], lifting=2)
+    for c in MORTAR_NFVSE_CASES:
+        make_mortar(c[:6], mu0=c[6], nfvse=MORTAR_NFVSE)

@@ -2,8 +2,8 @@
 metrics from the Fortran host) against the reference's code on the same meshes and states
 (tests/golden/mortar: U_Mortar, FillFlux, Flux_Mortar, SurfInt, for Navier-Stokes and
 resistive GLM-MHD the BR1 and BR2 lifting with Flux_Cons_Mortar and U_Mortar of the gradients, for
-GLM-MHD the Powell terms and GLM_ch, and the LSERK trajectory of the reference, run by its
-harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
+GLM-MHD the Powell terms and GLM_ch, for NFVSE the blending coefficient's propagation across
+mortars, and the LSERK trajectory of the reference, run by its harness), through the C ABI.  FP64 bar: relative L2 <= 1e-12 per
 RHS and face flux, <= 1e-10 after the stored RK steps (100 on three cases).  Properties the
 reference's own free-stream mortar tests check (tests/runs/navst/freestream/
 nonconforming_projmortar, nonconforming_collmortar: L_inf <= 1e-11) and conservation across
@@ -43,6 +43,9 @@ def engine(g, **kw):
                       lifting=int(g.get("lifting", 1)), etaBR2=float(g.get("etaBR2", 2.0)), **kw)
         return e, m
     par = bool(int(g.get("parabolic", 0)))
+    if "nfvse" in g:
+        kw.update(nfvse=True, alpha_max=float(g["alpha_max"]), alpha_min=float(g["alpha_min"]),
+                  sweeps=int(g["sweeps"]), prop_factor=float(g["prop_factor"]))
     e = Engine(m, volume_flux=int(g["volflux"]), riemann=int(g["riemann"]), which_mortar=int(g["whichMortar"]),
                parabolic=par, mu0=float(g.get("mu0", 0.0)), Pr=float(g.get("Pr", 0.72)),
                lifting=int(g.get("lifting", 1)), etaBR2=float(g.get("etaBR2", 2.0)), **kw)
@@ -67,7 +70,12 @@ def test_rhs_and_face_fluxes_match_reference(name):
     big = rel_l2(fm[:nm], g["Flux_master"][:nm])
     print(f"{name}: RHS {err:.3e}  Flux_master {ferr:.3e} (big sides {big:.3e})  Flux_slave {serr:.3e}")
     assert err <= 1e-12 and ferr <= 1e-12 and serr <= 1e-12 and big <= 1e-12
-    if "gradPx" in g or "glm_scale" in g:
+    if "alpha" in g:  # NFVSE: blending coefficient propagated across the mortars, VolInt_NFVSE
+        aerr = float(np.abs(e.blending() - g["alpha"]).max())
+        verr = rel_l2(e.volint_nfvse(), g["volint_nfvse"])
+        print(f"{name}: alpha max diff {aerr:.2e}, VolInt_NFVSE relL2 {verr:.3e}")
+        assert aerr <= 1e-13 and verr <= 1e-12
+    if "gradPx" in g or "glm_scale" in g or "alpha" in g:
         return
     # Euler: SurfInt alone on the reference's fluxes and volume integral gives its RHS
     fsp = np.concatenate([np.zeros_like(fm[:nm]), g["Flux_slave"]])
@@ -162,12 +170,11 @@ def test_fp32_split_on_mortars():
     assert err <= 3e-5 and li <= 3e-4
 
 
-def test_mortars_reject_what_is_not_built():
+def test_mortar_options_are_checked():
     from fluxo_amd.native import Engine, FxError
     from fluxo_amd.mesh import hopr_mesh
     m = hopr_mesh(os.path.join(IO_DIR, DEFORMED), 3)
-    for kw in (dict(nfvse=True), dict(nfvse=True, parabolic=True, mu0=1e-3)):
-        with pytest.raises(FxError, match="mortars"):
-            Engine(m, **kw)
+    with pytest.raises(FxError, match="whichMortar"):
+        Engine(m, which_mortar=-1)
     with pytest.raises(FxError, match="whichMortar"):
         Engine(m, which_mortar=2)
