@@ -19,9 +19,10 @@ difference of its final state from the FP64 run are reported under "fp32_split".
 Serving: a stream of independent requests (fluxo_amd.workloads, 512 by default, 50 steps
 each) goes through the batched server, first all at once (offline throughput), then as a
 Poisson stream at 0.7x that request rate (p50/p99 latency from submission to result on the
-host), in FP64 and FP32-split; every request also runs alone through its own context, back
-to back (the plain per-request path), for the batched/plain throughput ratio and the FIFO
-latency a one-at-a-time server would give at the same arrivals.
+host), in FP64 and FP32-split, with the admission cap SERVE_ADMIT_KNODES (and, for FP64,
+without it); every request also runs alone through its own context, back to back (the plain
+per-request path), for the batched/plain throughput ratio and the FIFO latency a
+one-at-a-time server would give at the same arrivals.
 
 Prints one JSON line.  Only one GPU is used: --gpus > 1 exits with a message.
 """
@@ -173,12 +174,18 @@ def serving_stream(n_req, nsteps, seed):
     return [W.make_request(seed, s, cache, nsteps) for s in range(n_req)]
 
 
-def run_server(reqs, precision, arrivals=None):
+# admission cap of the serving legs: the admitted requests hold at most this many thousand
+# volume nodes, strict FIFO at the cap (fx_server_config.admitKNodes); one N=5 24^3 request is
+# 2,986 thousand nodes
+SERVE_ADMIT_KNODES = 4096
+
+
+def run_server(reqs, precision, arrivals=None, admit_knodes=SERVE_ADMIT_KNODES):
     """Submit every request (at the given arrival offsets in seconds, else all at once) to one
     server; returns makespan, per-request latencies (s) and the DOF-updates done."""
     from fluxo_amd.native import Server, FX_REQ_DONE
     srv = Server(precision=precision, mu0=1.0 / 1600, Pr=0.72, cfl=SERVE_CFL, dfl=SERVE_CFL,
-                 hbm_budget=96 << 30, max_requests=len(reqs) + 1)
+                 hbm_budget=96 << 30, max_requests=len(reqs) + 1, admit_knodes=admit_knodes)
     ids, outs = [], []
     t0 = time.perf_counter()
     for i, r in enumerate(reqs):
@@ -274,15 +281,20 @@ def serving_leg(n_req, nsteps, seed, load):
     rng = np.random.default_rng(seed + 1)
     arrivals = None
     results = {}
-    for name, prec in (("fp64", FX_PREC_FP64), ("fp32_split", FX_PREC_FP32_SPLIT)):
-        off = run_server(reqs, prec)
-        results[name] = off.pop("results")
+    for name, prec, cap in (("fp64", FX_PREC_FP64, SERVE_ADMIT_KNODES), ("fp32_split", FX_PREC_FP32_SPLIT,
+                                                                          SERVE_ADMIT_KNODES),
+                            ("fp64_no_admission_cap", FX_PREC_FP64, 0)):
+        off = run_server(reqs, prec, admit_knodes=cap)
+        res = off.pop("results")
+        if name not in results:
+            results[name] = res
         if arrivals is None:
             rate = load * n_req / off["makespan"]
             arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
             arrivals -= arrivals[0]
-        live = run_server(reqs, prec, arrivals)
+        live = run_server(reqs, prec, arrivals, admit_knodes=cap)
         out[name] = {
+            "admit_knodes": cap,
             "offline_DOFps": off["dofs"] / off["makespan"],
             "offline_makespan_s": off["makespan"],
             "poisson_rate_per_s": rate,

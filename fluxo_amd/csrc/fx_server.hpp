@@ -116,6 +116,7 @@ struct fx_server {
   std::thread worker;
   fx::srv::Pool pools[2 * (fx::MAXN + 1)];
   long long bytes_in_use = 0, bytes_peak = 0, steps_launched = 0, graph_captures = 0, graph_updates = 0;
+  long long nodes_active = 0;      // volume nodes of the admitted requests (admitKNodes cap)
   long long bytes_allocated = 0;   // sum of the pools' device arrays
   int nActive = 0, nDone = 0, nFailed = 0;
   char worker_err[512] = "";
@@ -374,12 +375,19 @@ inline int pool_graph(fx_server *s, Pool &p)
 }
 
 // completion of an admitted request (called with the server lock held)
+inline long long request_nodes(const Request &q)
+{
+  const long long n = q.req.N + 1;
+  return n * n * n * q.req.nElems;
+}
+
 inline void finish(fx_server *s, Request &q, int state)
 {
   q.state = state;
   q.t_done = seconds_since(s);
   if (state == FX_REQ_DONE) s->nDone++; else s->nFailed++;
   s->bytes_in_use -= q.bytes;
+  s->nodes_active -= request_nodes(q);
   s->nActive--;
 }
 
@@ -450,11 +458,16 @@ inline void worker_loop(fx_server *s)
     if (rc != FX_OK) { fail_running(s, lk); continue; }
     if (progressed) s->cv_done.notify_all();
     // admission in FIFO order; a request that does not fit the budget, or whose pool is
-    // between launch and completion of a step, waits; later ones may pass it.  Submissions
-    // may arrive while the lock is released.
+    // between launch and completion of a step, waits; later ones may pass it.  With an
+    // active-work cap (admitKNodes), a request beyond the cap waits and no later one passes
+    // it (strict FIFO at the cap, so a large request is not starved by small ones; it then
+    // runs with a bounded number of others).  Submissions may arrive while the lock is
+    // released.
     const std::vector<int> waiting(s->queue.begin(), s->queue.end());
+    const long long cap = 1000LL * s->cfg.admitKNodes;
     for (int qid : waiting) {
       Request &q = s->reqs[qid];
+      if (cap > 0 && s->nodes_active > 0 && s->nodes_active + request_nodes(q) > cap) break;
       if (s->bytes_in_use + q.bytes > s->cfg.hbmBudget) continue;
       if (s->pools[2 * q.req.N + (q.req.parabolic ? 1 : 0)].inflight) continue;
       lk.unlock();
@@ -473,6 +486,7 @@ inline void worker_loop(fx_server *s)
       q.state = FX_REQ_RUNNING;
       q.t_admit = seconds_since(s);
       s->nActive++;
+      s->nodes_active += request_nodes(q);
       s->bytes_in_use += q.bytes;
       s->bytes_peak = std::max(s->bytes_peak, s->bytes_in_use);
     }
@@ -537,6 +551,7 @@ int fx_server_create(const fx_server_config *cfg, fx_server **out)
   if (cfg->lifting != 1) return set_err(FX_EINVAL, "only BR1 lifting (lifting=1)");
   if (cfg->hbmBudget <= 0) return set_err(FX_EINVAL, "hbmBudget must be > 0");
   if (cfg->maxRequests < 1) return set_err(FX_EINVAL, "maxRequests must be >= 1");
+  if (cfg->admitKNodes < 0) return set_err(FX_EINVAL, "admitKNodes must be >= 0");
   if (!(cfg->CFLscale > 0.0)) return set_err(FX_EINVAL, "CFLscale must be > 0");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(FX_ENODEV, "no HIP device");

@@ -49,7 +49,7 @@ class fx_server_config(C.Structure):
     _fields_ = [("volumeFlux", C.c_int), ("riemann", C.c_int), ("precision", C.c_int),
                 ("lifting", C.c_int), ("kappa", C.c_double), ("mu0", C.c_double),
                 ("Pr", C.c_double), ("CFLscale", C.c_double), ("DFLscale", C.c_double),
-                ("hbmBudget", C.c_longlong), ("maxRequests", C.c_int), ("reserved", C.c_int)]
+                ("hbmBudget", C.c_longlong), ("maxRequests", C.c_int), ("admitKNodes", C.c_int)]
 
 
 class fx_request(C.Structure):
@@ -387,11 +387,13 @@ class Engine:
 
 
 def server_config(volume_flux=32, riemann=1, precision=FX_PREC_FP64, kappa=1.4, mu0=0.0, Pr=0.72,
-                  cfl=0.9, dfl=0.9, hbm_budget=64 << 30, max_requests=4096):
+                  cfl=0.9, dfl=0.9, hbm_budget=64 << 30, max_requests=4096, admit_knodes=0):
+    """admit_knodes > 0 caps the admitted work at that many thousand volume nodes, with strict
+    FIFO order at the cap (fx_server_config.admitKNodes)."""
     c = fx_server_config()
     c.volumeFlux, c.riemann, c.precision, c.lifting = volume_flux, riemann, precision, 1
     c.kappa, c.mu0, c.Pr, c.CFLscale, c.DFLscale = kappa, mu0, Pr, cfl, dfl
-    c.hbmBudget, c.maxRequests = hbm_budget, max_requests
+    c.hbmBudget, c.maxRequests, c.admitKNodes = hbm_budget, max_requests, admit_knodes
     return c
 
 

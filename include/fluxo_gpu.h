@@ -315,7 +315,9 @@ typedef struct {
   double CFLscale, DFLscale;
   long long hbmBudget; /* device bytes the admitted requests may hold (fx_request_bytes) */
   int maxRequests;     /* request ids available over the server's life */
-  int reserved;
+  int admitKNodes;     /* 0: admit as the HBM budget allows; > 0: the admitted requests hold at
+                          most this many thousand volume nodes (a request alone is always
+                          admitted), and a request beyond the cap is not passed by later ones */
 } fx_server_config;
 
 typedef struct {

@@ -105,6 +105,29 @@ def test_hbm_budget_serialises_admission():
     srv.close()
 
 
+def test_admission_cap_keeps_fifo_order_and_results():
+    """admitKNodes: with a cap below two requests' nodes, a request beyond the cap is admitted
+    only after the running ones completed, and a later (smaller) request does not pass it;
+    results are bitwise those of the runs alone."""
+    from fluxo_amd.native import Server, FX_REQ_DONE
+    cache = W.MeshCache()
+    spec = [(3, 4, "wave", 6), (3, 4, "wave", 5), (2, 2, "wave", 3)]  # 4096, 4096 and 216 nodes
+    reqs = []
+    for N, K, ic, nsteps in spec:
+        m = cache.get(K, N, ic)
+        reqs.append(dict(N=N, K=K, ic=ic, parabolic=False, mesh=m, U0=W.density_wave(m["Elem_xGP"]), nsteps=nsteps))
+    srv = Server(admit_knodes=5, **NS)  # 5,000 nodes: the first request alone fits, not two
+    ids = [srv.submit(r["mesh"], r["U0"], r["nsteps"], parabolic=r["parabolic"]) for r in reqs]
+    srv.wait(-1, timeout=120)
+    st = [srv.status(i) for i, _ in ids]
+    assert all(x["state"] == FX_REQ_DONE for x in st)
+    assert st[1]["t_admit"] >= st[0]["t_done"]   # beyond the cap: after the first completed
+    assert st[2]["t_admit"] >= st[1]["t_admit"]  # the small one did not pass it
+    for (rid, out), r in zip(ids, reqs):
+        assert np.array_equal(out, alone(r)[0])
+    srv.close()
+
+
 def test_failed_request_does_not_disturb_others():
     """A request whose state has a non-positive pressure is flagged by CALCTIMESTEP and
     fails; the request batched with it completes unchanged."""
