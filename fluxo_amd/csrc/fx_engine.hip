@@ -455,14 +455,21 @@ int fx_rk_step(fx_ctx *c, double t, double dt)
 }
 
 // The step graph of the context (CALCTIMESTEP + the five stages), captured for cfl/dfl.
+// Navier-Stokes: the first stage's lifting kernel forms the per-element CALCTIMESTEP from the
+// U and metrics it reads anyway, and the step needs no CALCTIMESTEP pass of its own.
 static int step_graph(fx_ctx *c, double cfl, double dfl)
 {
   if (!c->graph || c->graph_cfl != cfl || c->graph_dfl != dfl) {
     if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->b.stream, hipStreamCaptureModeThreadLocal));
-    q_dt(c->b, cfl, dfl, 1);
-    q_rk_step(c->b);
+    if (dt_fusable(c->b)) {
+      const DtFuse f = dt_fuse(c->b, cfl, dfl);
+      q_rk_step(c->b, &f);
+    } else {
+      q_dt(c->b, cfl, dfl, 1);
+      q_rk_step(c->b);
+    }
     HIPCHK(hipStreamEndCapture(c->b.stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

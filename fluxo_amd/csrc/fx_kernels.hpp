@@ -722,6 +722,57 @@ k_mortar_u(const double *__restrict__ U, const int *__restrict__ mort, const dou
 // surface kernels need of them: the weak-form viscous volume term Uv and the face-normal
 // fluxes FvN.  EQ supplies the equation's entropy variables, gradient conversion and
 // diffusive fluxes (NsLift; MhdLift for resistive GLM-MHD).
+// CALCTIMESTEP of one node (navierstokes/calctimestep.f90:99-136): the convective
+// eigenvalue bounds l[0..2] of the xi/eta/zeta directions and, with parabolic terms, the
+// viscous ones l[3..5], each at least 1e-12 (a NaN bound becomes 1e-12); err 1 for a NaN
+// density, 2 for a non-positive pressure.  Shared by the CALCTIMESTEP kernel and the
+// first-stage lifting kernel, so that both form the same bits.
+__device__ __forceinline__ void node_dt_bounds(const double *u, const double *mf, const double *mg,
+                                               const double *mh, double sj, double kappa, double mu0,
+                                               double KsPr_max, int parabolic, double *l, int &err)
+{
+  if (isnan(u[0])) err = 1;
+  const double srho = 1.0 / u[0];
+  const double v0 = u[1] * srho, v1 = u[2] * srho, v2 = u[3] * srho;
+  const double p = (kappa - 1.0) * (u[4] - 0.5 * u[0] * (v0 * v0 + v1 * v1 + v2 * v2));
+  if (p <= 0.0) err = 2;
+  const double cs = sqrt(kappa * p * srho);
+  l[0] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
+  l[1] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
+  l[2] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
+  l[3] = l[4] = l[5] = 1.0e-12;
+  if (parabolic) {
+    const double muX = srho * KsPr_max * mu0;
+    l[3] = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
+    l[4] = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
+    l[5] = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
+  }
+#pragma unroll
+  for (int d = 0; d < 6; d++) l[d] = fmax(l[d], 1.0e-12);
+}
+
+// The element's time step from the maxima of its node bounds (calctimestep.f90:137-156);
+// err rises to 3 / 4 for a NaN convective / viscous time step.
+__device__ __forceinline__ double elem_dt(const double *lmax, double cfl, double dfl, int parabolic, int &err)
+{
+  double dte = cfl * 2.0 / (lmax[0] + lmax[1] + lmax[2]);
+  if (isnan(dte)) err = max(err, 3);
+  if (parabolic) {
+    const double dve = dfl * 4.0 / (lmax[3] + lmax[4] + lmax[5]);
+    if (isnan(dve)) err = max(err, 4);
+    dte = fmin(dte, dve);
+  }
+  return dte;
+}
+
+// CALCTIMESTEP formed inside the first stage's lifting kernel: per-element time steps and
+// error codes into dtElem / errElem, as the CALCTIMESTEP kernel writes them.
+struct DtFuse {
+  double *dtElem;
+  int *errElem;
+  double cfl, dfl, mu0, KsPr_max;
+};
+
 template <int N> struct LiftGeom { static constexpr int EPB = (256 / ((N + 1) * (N + 1) * (N + 1))) > 0 ? (256 / ((N + 1) * (N + 1) * (N + 1))) : 1; };
 
 // Navier-Stokes lifting physics: entropy variables (ConsToEntropy), the primitive-gradient
@@ -758,14 +809,18 @@ struct NsLift {
 // applied inside the volume integral and to every surface term, and the gradient each side
 // sees at its face is the node's volume-only gradient plus etaBR2 times that face's own
 // surface term (instead of the full gradient of BR1); the volume gradient is the same sum.
-template <int N, int OUT, class EQ, int LIFT = 1>
+// DT (Navier-Stokes, first stage of a step): the kernel also forms the step's CALCTIMESTEP
+// from the U and metrics it reads (dtf), a segmented maximum over each wavefront's lanes of
+// one element, then one LDS maximum per segment; the bounds are positive and not NaN, so
+// their bit patterns order as the values and the maxima are exact in any order.
+template <int N, int OUT, class EQ, int LIFT = 1, bool DT = false>
 __global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
           const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
           int nElems, EQ eq, double etaBR2, const double *__restrict__ Um, const double *__restrict__ LiftM,
-          double *__restrict__ GradM, int nMortar)
+          double *__restrict__ GradM, int nMortar, DtFuse dtf)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
   constexpr int EPB = LiftGeom<N>::EPB, NF = 6 * n2;  // face points per element
@@ -784,6 +839,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   const bool active = (el < EPB) && (e < nElems) && elem_problem(eprob, e) >= 0;
   const int i = node % n, j = (node / n) % n, k = node / n2;
   const size_t c = (size_t)e * n3 + node;
+  static_assert(!DT || (NV == 5 && OUT == 1), "CALCTIMESTEP in the Navier-Stokes stage lifting only");
+  __shared__ unsigned long long s_dtl[DT ? EPB : 1][6];  // bit patterns of the element maxima
+  __shared__ int s_dterr[DT ? EPB : 1];
+  if (DT && tid < EPB * 7) {
+    if (tid < EPB * 6) s_dtl[tid / 6][tid % 6] = 0ull;
+    else s_dterr[tid - EPB * 6] = 0;
+  }
   double u[NV], w[NV];
   if (active) {
     const double *up = U + c * NV;
@@ -965,8 +1027,46 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       for (int v = 0; v < NV; v++) { out0[NV * c + v] = gx[v]; out1[NV * c + v] = gy[v]; out2[NV * c + v] = gz[v]; }
     }
   }
+  if constexpr (DT) {  // every lane of the (full, 256-thread) block takes part
+    double l[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int err = 0;
+    const int key = active ? el : -1;
+    if (active)
+      node_dt_bounds(u, Mf + 3 * c, Mg + 3 * c, Mh + 3 * c, sJ[c], eq.kappa, dtf.mu0, dtf.KsPr_max, 1, l, err);
+    const int lane = tid & 63;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {  // suffix maximum within the lanes of one element
+      const int ko = __shfl_down(key, s);
+      const int eo = __shfl_down(err, s);
+      double lo[6];
+#pragma unroll
+      for (int d = 0; d < 6; d++) lo[d] = __shfl_down(l[d], s);
+      if (lane + s < 64 && ko == key) {
+#pragma unroll
+        for (int d = 0; d < 6; d++) l[d] = fmax(l[d], lo[d]);
+        err = max(err, eo);
+      }
+    }
+    const int kprev = __shfl_up(key, 1);
+    if (key >= 0 && (lane == 0 || kprev != key)) {
+#pragma unroll
+      for (int d = 0; d < 6; d++) atomicMax(&s_dtl[el][d], (unsigned long long)__double_as_longlong(l[d]));
+      atomicMax(&s_dterr[el], err);
+    }
+  }
   if (OUT == 0) return;
   __syncthreads();  // the face data is no longer read; its memory takes the fluxes
+  if constexpr (DT) {
+    const int ee = blockIdx.x * EPB + tid;
+    if (tid < EPB && ee < nElems && elem_problem(eprob, ee) >= 0) {
+      double lmax[6];
+#pragma unroll
+      for (int d = 0; d < 6; d++) lmax[d] = __longlong_as_double((long long)s_dtl[tid][d]);
+      int er = s_dterr[tid];
+      dtf.dtElem[ee] = elem_dt(lmax, dtf.cfl, dtf.dfl, 1, er);
+      dtf.errElem[ee] = er;
+    }
+  }
   if (active) {
     // the 12 Cartesian viscous flux values of this node into its own slots
     double fv[12];
@@ -1136,26 +1236,7 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   int err = 0;
   if (node < n3 && used) {
     const size_t c = (size_t)e * n3 + node;
-    const double *u = U + 5 * c;
-    if (isnan(u[0])) err = 1;
-    const double srho = 1.0 / u[0];
-    const double v0 = u[1] * srho, v1 = u[2] * srho, v2 = u[3] * srho;
-    const double p = (kappa - 1.0) * (u[4] - 0.5 * u[0] * (v0 * v0 + v1 * v1 + v2 * v2));
-    if (p <= 0.0) err = 2;
-    const double cs = sqrt(kappa * p * srho), sj = sJ[c];
-    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-    l[0] = sj * (fabs(mf[0] * v0 + mf[1] * v1 + mf[2] * v2) + cs * sqrt(mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]));
-    l[1] = sj * (fabs(mg[0] * v0 + mg[1] * v1 + mg[2] * v2) + cs * sqrt(mg[0] * mg[0] + mg[1] * mg[1] + mg[2] * mg[2]));
-    l[2] = sj * (fabs(mh[0] * v0 + mh[1] * v1 + mh[2] * v2) + cs * sqrt(mh[0] * mh[0] + mh[1] * mh[1] + mh[2] * mh[2]));
-    if (parabolic) {
-      const double muX = srho * KsPr_max * mu0;
-      l[3] = muX * ((mf[0] * sj) * (mf[0] * sj) + (mf[1] * sj) * (mf[1] * sj) + (mf[2] * sj) * (mf[2] * sj));
-      l[4] = muX * ((mg[0] * sj) * (mg[0] * sj) + (mg[1] * sj) * (mg[1] * sj) + (mg[2] * sj) * (mg[2] * sj));
-      l[5] = muX * ((mh[0] * sj) * (mh[0] * sj) + (mh[1] * sj) * (mh[1] * sj) + (mh[2] * sj) * (mh[2] * sj));
-    }
-    // a NaN eigenvalue leaves the running maximum unchanged (fmax)
-#pragma unroll
-    for (int d = 0; d < 6; d++) l[d] = fmax(l[d], 1.0e-12);
+    node_dt_bounds(U + 5 * c, Mf + 3 * c, Mg + 3 * c, Mh + 3 * c, sJ[c], kappa, mu0, KsPr_max, parabolic, l, err);
   }
 #pragma unroll
   for (int d = 0; d < 6; d++) s_l[d][tid] = l[d];
@@ -1171,14 +1252,8 @@ k_calc_dt(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   }
   if (node == 0 && used) {
     int er = s_err[tid];
-    double dte = cfl * 2.0 / (s_l[0][tid] + s_l[1][tid] + s_l[2][tid]);
-    if (isnan(dte)) er = max(er, 3);
-    if (parabolic) {
-      const double dve = dfl * 4.0 / (s_l[3][tid] + s_l[4][tid] + s_l[5][tid]);
-      if (isnan(dve)) er = max(er, 4);
-      dte = fmin(dte, dve);
-    }
-    dtElem[e] = dte;
+    const double lmax[6] = {s_l[0][tid], s_l[1][tid], s_l[2][tid], s_l[3][tid], s_l[4][tid], s_l[5][tid]};
+    dtElem[e] = elem_dt(lmax, cfl, dfl, parabolic, er);
     errElem[e] = er;
   }
 }

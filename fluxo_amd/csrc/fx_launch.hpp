@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fluxo_gpu.h"
@@ -349,6 +350,13 @@ inline int tile_epb(int N, int idx)
   }
 }
 
+// max(4/3, kappa/Pr): the viscous eigenvalue factor of CALCTIMESTEP (calctimestep.f90:85)
+inline double ks_pr_max(const fx_config &c)
+{
+  const double KsPr = c.kappa / c.Pr;
+  return (4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr;
+}
+
 // ---- batch ----------------------------------------------------------------------------
 // Device arrays of one launch domain.  cfg.nElems / cfg.nSides are the launch extents;
 // with eprob == nullptr all elements belong to problem 0 (st[0]), otherwise eprob maps
@@ -416,7 +424,8 @@ template <int N> void launch_surface_n(const Batch &b);
 // U_Mortar into b.Umort, Flux_Mortar into the big sides of b.Flux (meshes with mortars)
 template <int N> void launch_mortar_u_n(const Batch &b);
 template <int N> void launch_flux_mortar_n(const Batch &b);
-template <int N> void launch_lifting_n(const Batch &b, int out);
+// dtf (Navier-Stokes, viscous form): the kernel also forms CALCTIMESTEP's dtElem / errElem
+template <int N> void launch_lifting_n(const Batch &b, int out, const DtFuse *dtf = nullptr);
 template <int N> void launch_dt_n(const Batch &b, double cfl, double dfl);
 template <int N> void launch_mhd_volume_n(const Batch &b, int mode, RKArgs rk);
 template <int N> void launch_mhd_surface_n(const Batch &b);
@@ -450,7 +459,24 @@ inline void q_surface(const Batch &b)
   else { FX_DISPATCH_N(b.cfg.N, launch_surface_n, b); }
   if (b.nMortar > 0) { FX_DISPATCH_N(b.cfg.N, launch_flux_mortar_n, b); }
 }
-inline void q_lifting(const Batch &b, int out) { FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out); }
+inline void q_lifting(const Batch &b, int out, const DtFuse *dtf = nullptr)
+{
+  FX_DISPATCH_N(b.cfg.N, launch_lifting_n, b, out, dtf);
+}
+
+// The minimum of the per-element time steps into the problem states (advance=1 also moves t
+// and the step counter)
+inline void q_dt_reduce(const Batch &b, int advance)
+{
+  if (b.eprob) {
+    if (b.nProb > 0)
+      hipLaunchKernelGGL(k_dt_reduce_seg, dim3(b.nProb), dim3(256), 0, b.stream, b.dtElem, b.errElem,
+                         b.rfirst, b.rcount, b.st, advance);
+  } else {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.errElem, b.cfg.nElems,
+                       b.st, advance);
+  }
+}
 
 // CALCTIMESTEP into the problem states (advance=1 also moves t and the step counter)
 inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
@@ -462,14 +488,7 @@ inline void q_dt(const Batch &b, double cfl, double dfl, int advance)
     return;
   }
   FX_DISPATCH_N(b.cfg.N, launch_dt_n, b, cfl, dfl);
-  if (b.eprob) {
-    if (b.nProb > 0)
-      hipLaunchKernelGGL(k_dt_reduce_seg, dim3(b.nProb), dim3(256), 0, b.stream, b.dtElem, b.errElem,
-                         b.rfirst, b.rcount, b.st, advance);
-  } else {
-    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.errElem, b.cfg.nElems,
-                       b.st, advance);
-  }
+  q_dt_reduce(b, advance);
 }
 
 #endif  // FX_KERNEL_HOME
@@ -501,10 +520,26 @@ inline const double *q_nfvse_alpha(const Batch &b)
   return src;
 }
 
-// one RHS evaluation, followed by the RK stage update in VMODE_RK
-inline void q_stage(const Batch &b, int mode, RKArgs rk)
+// CALCTIMESTEP inside the first stage of a step: a single Navier-Stokes problem, whose
+// first-stage lifting kernel reads the step's U and metrics at every node anyway
+inline bool dt_fusable(const Batch &b) { return !b.mhd() && !b.eprob && b.cfg.parabolic; }
+inline DtFuse dt_fuse(const Batch &b, double cfl, double dfl)
 {
-  if (b.cfg.parabolic) q_lifting(b, 1);
+  DtFuse f;
+  f.dtElem = b.dtElem; f.errElem = b.errElem;
+  f.cfl = cfl; f.dfl = dfl; f.mu0 = b.cfg.mu0; f.KsPr_max = ks_pr_max(b.cfg);
+  return f;
+}
+
+// one RHS evaluation, followed by the RK stage update in VMODE_RK; dtf (non-null): the
+// lifting kernel also forms the step's CALCTIMESTEP, whose minimum (advancing t and the
+// step counter) follows it
+inline void q_stage(const Batch &b, int mode, RKArgs rk, const DtFuse *dtf = nullptr)
+{
+  if (b.cfg.parabolic) {
+    q_lifting(b, 1, dtf);
+    if (dtf) q_dt_reduce(b, 1);
+  }
   if (b.cfg.nfvse && mode != VMODE_VOLINT) {
     const double *a = q_nfvse_alpha(b);
     FX_DISPATCH_N(b.cfg.N, launch_nfvse_fv_n, b, a);
@@ -515,8 +550,9 @@ inline void q_stage(const Batch &b, int mode, RKArgs rk)
   q_volume(b, mode, rk);
 }
 
-// the five stages of one TimeStepByLSERKW2 step with dt from the problem states
-inline void q_rk_step(const Batch &b)
+// the five stages of one TimeStepByLSERKW2 step with dt from the problem states, or, with
+// dtf (dt_fusable batches), with the step's CALCTIMESTEP formed in its first stage
+inline void q_rk_step(const Batch &b, const DtFuse *dtf = nullptr)
 {
   for (int s = 0; s < 5; s++) {
     RKArgs rk;
@@ -527,7 +563,7 @@ inline void q_rk_step(const Batch &b)
     rk.first = (s == 0);
     rk.alpha = nullptr;
     rk.Ffv = nullptr;
-    q_stage(b, VMODE_RK, rk);
+    q_stage(b, VMODE_RK, rk, s == 0 ? dtf : nullptr);
   }
 }
 

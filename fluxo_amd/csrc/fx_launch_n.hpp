@@ -155,20 +155,29 @@ void launch_surface_n(const Batch &b)
 }
 
 template <int N, class EQ, int LIFT>
-void launch_lifting_l(const Batch &b, int out, const EQ &eq)
+void launch_lifting_l(const Batch &b, int out, const EQ &eq, const DtFuse *dtf)
 {
   const int epb = LiftGeom<N>::EPB;
   const int nb = (b.cfg.nElems + epb - 1) / epb;
   if (nb <= 0) return;
   const size_t nv = (size_t)EQ::NV * b.n3 * b.cfg.nElems;
+  const DtFuse none{};
   if (out == 0)
     hipLaunchKernelGGL((k_lifting<N, 0, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
                        b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq,
-                       b.cfg.etaBR2, b.Umort, b.LiftM, b.GradM, b.nMortar);
-  else
+                       b.cfg.etaBR2, b.Umort, b.LiftM, b.GradM, b.nMortar, none);
+  else if constexpr (std::is_same<EQ, NsLift>::value) {
+    if (dtf) {
+      hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh,
+                         b.sJ, b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
+                         b.Umort, b.LiftM, b.GradM, b.nMortar, *dtf);
+      return;
+    }
+  }
+  if (out != 0)
     hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
                        b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
-                       b.Umort, b.LiftM, b.GradM, b.nMortar);
+                       b.Umort, b.LiftM, b.GradM, b.nMortar, none);
 }
 
 // Lifting (BR1 or BR2) on a mesh with mortars: U_Mortar of the state, the small sides' lifting
@@ -176,7 +185,7 @@ void launch_lifting_l(const Batch &b, int out, const EQ &eq)
 // (viscous form) U_Mortar of the big sides' gradients and the master viscous flux of the small
 // sides.
 template <int N, class EQ>
-void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
+void launch_lifting_mortar(const Batch &b, int out, const EQ &eq, const DtFuse *dtf)
 {
   const int nbs = (int)(((long long)b.nMortar * 4 * b.n2 + 255) / 256);
   const int nbb = (int)(((long long)b.nMortar * b.n2 + 255) / 256);
@@ -187,12 +196,12 @@ void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
   hipLaunchKernelGGL((k_flux_mortar<N, 3 * EQ::NV>), dim3(nbb), dim3(256), 0, b.stream, b.LiftS, b.LiftM, b.mort,
                      b.mops, b.nMortar);
   if (b.cfg.lifting == 2) {
-    launch_lifting_l<N, EQ, 2>(b, out, eq);
+    launch_lifting_l<N, EQ, 2>(b, out, eq, dtf);
     if (out == 1)
       hipLaunchKernelGGL((k_mortar_fv<N, EQ, true>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse,
                          b.mort, b.mops, b.FvN, b.nMortar, eq);
   } else {
-    launch_lifting_l<N, EQ, 1>(b, out, eq);
+    launch_lifting_l<N, EQ, 1>(b, out, eq, dtf);
     if (out == 1)
       hipLaunchKernelGGL((k_mortar_fv<N, EQ, false>), dim3(nbs), dim3(256), 0, b.stream, b.Umort, b.GradM, b.nvse,
                          b.mort, b.mops, b.FvN, b.nMortar, eq);
@@ -200,14 +209,14 @@ void launch_lifting_mortar(const Batch &b, int out, const EQ &eq)
 }
 
 template <int N, class EQ>
-void launch_lifting_e(const Batch &b, int out, const EQ &eq)
+void launch_lifting_e(const Batch &b, int out, const EQ &eq, const DtFuse *dtf)
 {
-  if (b.cfg.lifting == 2) launch_lifting_l<N, EQ, 2>(b, out, eq);
-  else launch_lifting_l<N, EQ, 1>(b, out, eq);
+  if (b.cfg.lifting == 2) launch_lifting_l<N, EQ, 2>(b, out, eq, dtf);
+  else launch_lifting_l<N, EQ, 1>(b, out, eq, dtf);
 }
 
 template <int N>
-void launch_lifting_n(const Batch &b, int out)
+void launch_lifting_n(const Batch &b, int out, const DtFuse *dtf)
 {
   if (b.mhd()) {
     MhdLift eq;
@@ -216,15 +225,15 @@ void launch_lifting_n(const Batch &b, int out)
     eq.mu = b.cfg.mu0;
     eq.etasmu0 = b.cfg.eta * eq.smu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);
-    else launch_lifting_e<N>(b, out, eq);
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq, dtf);
+    else launch_lifting_e<N>(b, out, eq, dtf);
   } else {
     NsLift eq;
     eq.kappa = b.cfg.kappa;
     eq.mu0 = b.cfg.mu0;
     eq.KappasPr = b.cfg.kappa / b.cfg.Pr;
-    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq);
-    else launch_lifting_e<N>(b, out, eq);
+    if (b.nMortar > 0) launch_lifting_mortar<N>(b, out, eq, dtf);
+    else launch_lifting_e<N>(b, out, eq, dtf);
   }
 }
 
@@ -234,8 +243,7 @@ void launch_dt_n(const Batch &b, double cfl, double dfl)
   const int epb = DtGeom<N>::EPB;
   const int nb = (b.cfg.nElems + epb - 1) / epb;
   if (nb <= 0) return;
-  const double KsPr = b.cfg.kappa / b.cfg.Pr;
-  const double KsPr_max = (4.0 / 3.0 > KsPr) ? 4.0 / 3.0 : KsPr;
+  const double KsPr_max = ks_pr_max(b.cfg);
   hipLaunchKernelGGL((k_calc_dt<N>), dim3(nb), dim3(DtGeom<N>::T), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
                      b.dtElem, b.errElem, b.eprob, b.cfg.nElems, b.cfg.kappa, cfl, dfl, b.cfg.mu0, KsPr_max,
                      b.cfg.parabolic);

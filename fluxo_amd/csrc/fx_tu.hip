@@ -30,7 +30,7 @@ template void launch_surfint_n<FX_TU_N>(const Batch &, const double *, const dou
 template void launch_mortar_u_n<FX_TU_N>(const Batch &);
 template void launch_flux_mortar_n<FX_TU_N>(const Batch &);
 #elif FX_TU_GROUP == 2
-template void launch_lifting_n<FX_TU_N>(const Batch &, int);
+template void launch_lifting_n<FX_TU_N>(const Batch &, int, const DtFuse *);
 template void launch_dt_n<FX_TU_N>(const Batch &, double, double);
 template void launch_entropy_n<FX_TU_N>(const Batch &, double *, double *);
 template void launch_analyze_n<FX_TU_N>(const Batch &, const AnaArgs &, const double *, const double *, double *,
