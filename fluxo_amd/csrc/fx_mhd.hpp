@@ -350,18 +350,19 @@ template <int N> struct MhdVolGeom {
 
 // Element-centric GLM-MHD volume kernel (VolInt_SplitForm with EvalAdvFluxAverage3D and
 // AddNonConsFluxTilde3D, mhd/flux_average.f90:112-186, 268-366; VolInt_visc; SurfInt; -sJ; the
-// LSERK stage update).  One thread per (element, direction d, node k): it evaluates the
-// two-point Derigs flux of its node with each other node l of its line in direction d, the
-// lower line index always the left state (so the flux of a pair is the same value in both of
-// its nodes' threads), and accumulates sum_l Dvolsurf(k,l) F(k,l) in increasing l and the
-// Powell line sum; each direction's sums go to LDS and the node pass adds xi + eta + zeta in
-// that order.  Every pair's flux is thus evaluated twice, which keeps a thread's registers to
-// one node's sums (the line-per-thread form held n nodes' sums and ran one wave per SIMD).
+// LSERK stage update).  One thread per (element, direction d, node k at position pos of its
+// line in direction d).  Each pair of a line is evaluated once, by one of its two nodes: in
+// slot s = 1..n/2 node pos takes the pair (pos, pos + s mod n) (for even n the last slot only
+// for pos < n/2), so every node evaluates at most n/2 two-point Derigs fluxes, the lower line
+// index always the left state.  The thread adds Dvolsurf(pos,l) F and its Powell term to its
+// own sums and leaves Dvolsurf(l,pos) F and the partner's Powell term in an LDS exchange
+// buffer, from which the partner adds them after a barrier (a thread keeps one node's sums).
 //   phase 1  one thread per (element, direction, node): the node fields (direction 0) and the
 //            node's metric of its direction into LDS
 //   phase 2  the flux differencing as above, sums into LDS over the fields (after a barrier)
-//   phase 3  face gather +FluxM L0 / -FluxS L0 into LDS, then one thread per node: direction
-//            sums, viscous volume term, surface terms, -sJ, Ut or the RK stage update
+//   phase 3  face gather +FluxM L0 / -FluxS L0 into LDS (the memory of the exchange buffer),
+//            then one thread per node: direction sums, viscous volume term, surface terms,
+//            -sJ, Ut or the RK stage update
 template <int N, int MODE, typename VR, bool PARA>
 __global__ void __launch_bounds__(MhdVolGeom<N>::T)
 k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg, const VR *__restrict__ Mh,
@@ -404,42 +405,61 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
   }
   __syncthreads();
 
+  // exchange buffer of phase 2 (MV flux values and the Powell term per thread), later the
+  // face terms of phase 3
+  constexpr int NF = 6 * n2, NX = (MV + 1) * T, NFACE = MODE != VMODE_VOLINT ? EPB * NF * MV : 1;
+  __shared__ double s_xf[NX > NFACE ? NX : NFACE];
   VR acc[MV];
 #pragma unroll
   for (int v = 0; v < MV; v++) acc[v] = VR(0);
+  VR snc = VR(0);
+  const int pos = (dir == 0) ? i : ((dir == 1) ? j : k);  // the node's index along its line
+  const int stride = (dir == 0) ? 1 : ((dir == 1) ? (n + 1) : (n + 1) * n);  // LDS step along the line
+  const int l0 = ln - pos * stride;                        // LDS index of the line's node 0
+  const int tstep = (dir == 0) ? 1 : ((dir == 1) ? n : n2);  // thread step along the line
+  VR fk[MF_COUNT], mk[3];
   if (active) {
-    const int pos = (dir == 0) ? i : ((dir == 1) ? j : k);  // the node's index along its line
-    const int stride = (dir == 0) ? 1 : ((dir == 1) ? (n + 1) : (n + 1) * n);  // LDS step along the line
-    const int l0 = ln - pos * stride;                        // LDS index of the line's node 0
-    VR fk[MF_COUNT], mk[3];
 #pragma unroll
     for (int c = 0; c < MF_COUNT; c++) fk[c] = MF_AT(el, c, ln);
 #pragma unroll
     for (int c = 0; c < 3; c++) mk[c] = MM_AT(el, dir, c, ln);
-    VR snc = VR(0);
-#pragma unroll
-    for (int l = 0; l < n; l++) {
-      if (l == pos) continue;
+  }
+  constexpr int SLOT_UNROLL = n / 2 <= 2 ? n / 2 : 1;  // N=5 unrolled spills
+#pragma unroll SLOT_UNROLL
+  for (int sl = 1; sl <= n / 2; sl++) {
+    const bool half = (n % 2 == 0) && (sl == n / 2);  // the pairs (p, p + n/2): once each
+    if (active && (!half || pos < n / 2)) {
+      const int l = pos + sl < n ? pos + sl : pos + sl - n;
       const int ll = l0 + l * stride;
       VR fl[MF_COUNT], ml[3], F[MV];
 #pragma unroll
       for (int c = 0; c < MF_COUNT; c++) fl[c] = MF_AT(el, c, ll);
 #pragma unroll
       for (int c = 0; c < 3; c++) ml[c] = MM_AT(el, dir, c, ll);
-      VR mb0, mb1, mb2;
-      if (pos < l) {
-        derigs_flux<VR>(fk, fl, mk, ml, ch, smu0, skm1, F);
-        mb0 = VR(0.5) * (mk[0] + ml[0]); mb1 = VR(0.5) * (mk[1] + ml[1]); mb2 = VR(0.5) * (mk[2] + ml[2]);
-      } else {
-        derigs_flux<VR>(fl, fk, ml, mk, ch, smu0, skm1, F);
-        mb0 = VR(0.5) * (ml[0] + mk[0]); mb1 = VR(0.5) * (ml[1] + mk[1]); mb2 = VR(0.5) * (ml[2] + mk[2]);
-      }
-      const VR dkl = OpsOf<VR>::ds(op, pos, l);
+      if (pos < l) derigs_flux<VR>(fk, fl, mk, ml, ch, smu0, skm1, F);
+      else derigs_flux<VR>(fl, fk, ml, mk, ch, smu0, skm1, F);
+      const VR mb0 = VR(0.5) * (mk[0] + ml[0]), mb1 = VR(0.5) * (mk[1] + ml[1]), mb2 = VR(0.5) * (mk[2] + ml[2]);
+      const VR dkl = OpsOf<VR>::ds(op, pos, l), dlk = OpsOf<VR>::ds(op, l, pos);
 #pragma unroll
-      for (int v = 0; v < MV; v++) acc[v] += dkl * F[v];
-      // Powell: B_l.{{m}} towards node k
+      for (int v = 0; v < MV; v++) {
+        acc[v] += dkl * F[v];
+        s_xf[v * T + tid] = dlk * F[v];
+      }
+      // Powell: B_l.{{m}} towards node pos, B_pos.{{m}} towards node l
       snc += dkl * (mb0 * fl[MF_B1] + mb1 * fl[MF_B2] + mb2 * fl[MF_B3]);
+      s_xf[MV * T + tid] = dlk * (mb0 * fk[MF_B1] + mb1 * fk[MF_B2] + mb2 * fk[MF_B3]);
     }
+    __syncthreads();
+    if (active && (!half || pos >= n / 2)) {  // the pair (m, pos) taken by node m in this slot
+      const int m = pos - sl >= 0 ? pos - sl : pos - sl + n;
+      const int ts = tid + (m - pos) * tstep;
+#pragma unroll
+      for (int v = 0; v < MV; v++) acc[v] += s_xf[v * T + ts];
+      snc += s_xf[MV * T + ts];
+    }
+    __syncthreads();  // the next slot overwrites the exchange buffer
+  }
+  if (active) {
     const VR b1 = fk[MF_B1], b2 = fk[MF_B2], b3 = fk[MF_B3];
     const VR bhat = mk[0] * b1 + mk[1] * b2 + mk[2] * b3;
     const VR t = VR(0.5) * (bhat * VR(op.DsRow[pos]) + snc);
@@ -456,10 +476,10 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
   }
 
   // the element's SurfInt terms +FluxM L0 (master) / -FluxS L0 (slave) into LDS in
-  // element-local face coordinates, one work item per face point
+  // element-local face coordinates, one work item per face point (the exchange buffer is no
+  // longer read: the last slot ended with a barrier)
   const double L0 = op.L0;
-  constexpr int NF = 6 * n2;
-  __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * MV : 1];
+  auto s_face = reinterpret_cast<double (*)[NF * MV]>(s_xf);
   if (MODE != VMODE_VOLINT) {
     for (int it = tid; it < EPB * NF; it += T) {
       const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
