@@ -824,13 +824,15 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NV = EQ::NV, NVV = EQ::NVV;
   constexpr int EPB = LiftGeom<N>::EPB, NF = 6 * n2;  // face points per element
-  // per element: the face data of phase F ((NV + 3) per face point: the BR1 lifting flux
-  // 1/2 (w_slave - w_master) SurfElem and the normal), later (same memory, after a barrier)
-  // the node's 12 Cartesian viscous flux values, then its transformed fluxes (3 NVV)
+  // one LDS buffer per element: the entropy variables (WSZ) and after them the face data of
+  // phase F ((NV + 3) per face point: the BR1 lifting flux 1/2 (w_slave - w_master) SurfElem
+  // and the normal) until the gradients are formed; then (after a barrier) from its start the
+  // node's 12 Cartesian viscous flux values, then its transformed fluxes (3 NVV)
   constexpr int FTW = 3 * NVV > 12 ? 3 * NVV : 12;
-  constexpr int SLOT = (NV + 3) * NF > FTW * n3 ? (NV + 3) * NF : FTW * n3;
-  __shared__ double s_w[EPB][NV][n3];
-  __shared__ double s_buf[EPB][SLOT];
+  constexpr int WSZ = NV * n3, FSZ = (NV + 3) * NF;
+  constexpr int USZ = WSZ + FSZ > FTW * n3 ? WSZ + FSZ : FTW * n3;
+  __shared__ double s_u[EPB][USZ];
+#define LW(el_, v_, node_) s_u[el_][(v_) * n3 + (node_)]
   __shared__ int s_fpt[EPB][NF];  // 2 * face point ((s n + q) n + p) + (0 master, 1 slave)
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
@@ -853,7 +855,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     for (int v = 0; v < NV; v++) u[v] = up[v];
     eq.entropy(u, w);
 #pragma unroll
-    for (int v = 0; v < NV; v++) s_w[el][v][node] = w[v];
+    for (int v = 0; v < NV; v++) LW(el, v, node) = w[v];
   }
   __syncthreads();
   // phase F: one work item per face point of the block's elements (the element-local
@@ -889,10 +891,10 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     for (int v = 0; v < NV; v++) unb[v] = unbp[v];
     eq.entropy(unb, wnb);
     const double4 g = nvse[fpt];
-    double *d = &s_buf[fe][r * (NV + 3)];
+    double *d = &s_u[fe][WSZ + r * (NV + 3)];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      const double wo = s_w[fe][v][fnode];
+      const double wo = LW(fe, v, fnode);
       const double ws = (fl == 0) ? wnb[v] : wo;
       const double wm = (fl == 0) ? wo : wnb[v];
       d[v] = 0.5 * (ws - wm) * g.w;
@@ -906,18 +908,18 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     double gxi[NV], geta[NV], gzeta[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      gxi[v] = op.D[i][0] * s_w[el][v][0 + n * (j + n * k)];
-      geta[v] = op.D[j][0] * s_w[el][v][i + n * (0 + n * k)];
-      gzeta[v] = op.D[k][0] * s_w[el][v][i + n * (j + n * 0)];
+      gxi[v] = op.D[i][0] * LW(el, v, 0 + n * (j + n * k));
+      geta[v] = op.D[j][0] * LW(el, v, i + n * (0 + n * k));
+      gzeta[v] = op.D[k][0] * LW(el, v, i + n * (j + n * 0));
     }
     // not unrolled: all n NV LDS operands in flight at once would cost an occupancy level
 #pragma unroll 1
     for (int l = 1; l < n; l++)
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        gxi[v] = gxi[v] + op.D[i][l] * s_w[el][v][l + n * (j + n * k)];
-        geta[v] = geta[v] + op.D[j][l] * s_w[el][v][i + n * (l + n * k)];
-        gzeta[v] = gzeta[v] + op.D[k][l] * s_w[el][v][i + n * (j + n * l)];
+        gxi[v] = gxi[v] + op.D[i][l] * LW(el, v, l + n * (j + n * k));
+        geta[v] = geta[v] + op.D[j][l] * LW(el, v, i + n * (l + n * k));
+        gzeta[v] = gzeta[v] + op.D[k][l] * LW(el, v, i + n * (j + n * l));
       }
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
     const double sj = sJ[c];
@@ -940,7 +942,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     for (int t = 0; t < 3; t++) {
       const int r = nf.get(t);
       if (r < 0) continue;
-      const double *d = &s_buf[el][r * (NV + 3)];
+      const double *d = &s_u[el][WSZ + r * (NV + 3)];
       const double nx = d[NV], ny = d[NV + 1], nz = d[NV + 2];
       const int mcode = s_fpt[el][r];
       if (LIFT == 1 && (mcode >> 1) < nMortar * n2) {
@@ -1055,7 +1057,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     }
   }
   if (OUT == 0) return;
-  __syncthreads();  // the face data is no longer read; its memory takes the fluxes
+  __syncthreads();  // the entropy variables and face data are no longer read; the fluxes take their memory
   if constexpr (DT) {
     const int ee = blockIdx.x * EPB + tid;
     if (tid < EPB && ee < nElems && elem_problem(eprob, ee) >= 0) {
@@ -1072,13 +1074,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     double fv[12];
     eq.fluxes(u, gx, gy, gz, fv);
 #pragma unroll
-    for (int m = 0; m < 12; m++) s_buf[el][m * n3 + node] = fv[m];
+    for (int m = 0; m < 12; m++) s_u[el][m * n3 + node] = fv[m];
   }
   __syncthreads();
   if (active) {
     double fv[12], o[NVV];
 #pragma unroll
-    for (int m = 0; m < 12; m++) fv[m] = s_buf[el][m * n3 + node];
+    for (int m = 0; m < 12; m++) fv[m] = s_u[el][m * n3 + node];
     // face-normal viscous flux of this element at each of its face points (BR1: from the
     // node's full gradient; BR2 wrote them from the face gradients above)
     if (LIFT == 1) {
@@ -1099,13 +1101,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
     EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_buf[el][v * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][v * n3 + node] = o[v];
     EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_buf[el][(NVV + v) * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][(NVV + v) * n3 + node] = o[v];
     EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_buf[el][(2 * NVV + v) * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][(2 * NVV + v) * n3 + node] = o[v];
   }
   __syncthreads();
   if (active) {
@@ -1117,14 +1119,16 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
 #pragma unroll
       for (int v = 0; v < NVV; v++)
-        uv[v] = uv[v] + di * s_buf[el][v * n3 + l + n * (j + n * k)] +
-                dj * s_buf[el][(NVV + v) * n3 + i + n * (l + n * k)] +
-                dk * s_buf[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
+        uv[v] = uv[v] + di * s_u[el][v * n3 + l + n * (j + n * k)] +
+                dj * s_u[el][(NVV + v) * n3 + i + n * (l + n * k)] +
+                dk * s_u[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
     }
 #pragma unroll
     for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
   }
 }
+#undef LW
+
 
 // BR1 lifting on the small sides of mortars (Lifting_FillFlux br1_fillflux.f90:49-91 with
 // EvalLiftingSurfFlux flux.f90:677-716): F = 1/2 (w_slave - w_master) SurfElem with the master
