@@ -765,11 +765,12 @@ __device__ __forceinline__ double elem_dt(const double *lmax, double cfl, double
   return dte;
 }
 
-// CALCTIMESTEP formed inside the first stage's lifting kernel: per-element time steps and
-// error codes into dtElem / errElem, as the CALCTIMESTEP kernel writes them.
+// CALCTIMESTEP formed inside the first stage's lifting kernel: the minimum time step of each
+// block's elements (a NaN element time step skipped, as by the CALCTIMESTEP minimum) and their
+// maximum error code into dtBlock / errBlock, one entry per block of the launch.
 struct DtFuse {
-  double *dtElem;
-  int *errElem;
+  double *dtBlock;
+  int *errBlock;
   double cfl, dfl, mu0, KsPr_max;
 };
 
@@ -1058,15 +1059,23 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   }
   if (OUT == 0) return;
   __syncthreads();  // the entropy variables and face data are no longer read; the fluxes take their memory
-  if constexpr (DT) {
-    const int ee = blockIdx.x * EPB + tid;
-    if (tid < EPB && ee < nElems && elem_problem(eprob, ee) >= 0) {
-      double lmax[6];
+  if constexpr (DT) {  // the block's minimum time step (NaN skipped) and maximum error code
+    if (tid == 0) {
+      double m = 1.79e308;
+      int er = 0;
+      for (int q = 0; q < EPB; q++) {
+        const int ee = blockIdx.x * EPB + q;
+        if (ee >= nElems || elem_problem(eprob, ee) < 0) continue;
+        double lmax[6];
 #pragma unroll
-      for (int d = 0; d < 6; d++) lmax[d] = __longlong_as_double((long long)s_dtl[tid][d]);
-      int er = s_dterr[tid];
-      dtf.dtElem[ee] = elem_dt(lmax, dtf.cfl, dtf.dfl, 1, er);
-      dtf.errElem[ee] = er;
+        for (int d = 0; d < 6; d++) lmax[d] = __longlong_as_double((long long)s_dtl[q][d]);
+        int eq_er = s_dterr[q];
+        const double dte = elem_dt(lmax, dtf.cfl, dtf.dfl, 1, eq_er);
+        if (dte < m) m = dte;
+        er = max(er, eq_er);
+      }
+      dtf.dtBlock[blockIdx.x] = m;
+      dtf.errBlock[blockIdx.x] = er;
     }
   }
   if (active) {

@@ -350,6 +350,13 @@ inline int tile_epb(int N, int idx)
   }
 }
 
+// blocks of a lifting launch (LiftGeom<N>::EPB elements each)
+inline int lift_blocks(int N, int nElems)
+{
+  const int n3 = (N + 1) * (N + 1) * (N + 1), epb = 256 / n3 > 0 ? 256 / n3 : 1;
+  return (nElems + epb - 1) / epb;
+}
+
 // max(4/3, kappa/Pr): the viscous eigenvalue factor of CALCTIMESTEP (calctimestep.f90:85)
 inline double ks_pr_max(const fx_config &c)
 {
@@ -465,16 +472,16 @@ inline void q_lifting(const Batch &b, int out, const DtFuse *dtf = nullptr)
 }
 
 // The minimum of the per-element time steps into the problem states (advance=1 also moves t
-// and the step counter)
-inline void q_dt_reduce(const Batch &b, int advance)
+// and the step counter); count > 0: of the first count entries (per-block minima) instead
+inline void q_dt_reduce(const Batch &b, int advance, int count = 0)
 {
   if (b.eprob) {
     if (b.nProb > 0)
       hipLaunchKernelGGL(k_dt_reduce_seg, dim3(b.nProb), dim3(256), 0, b.stream, b.dtElem, b.errElem,
                          b.rfirst, b.rcount, b.st, advance);
   } else {
-    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.errElem, b.cfg.nElems,
-                       b.st, advance);
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, b.stream, b.dtElem, b.errElem,
+                       count > 0 ? count : b.cfg.nElems, b.st, advance);
   }
 }
 
@@ -526,7 +533,7 @@ inline bool dt_fusable(const Batch &b) { return !b.mhd() && !b.eprob && b.cfg.pa
 inline DtFuse dt_fuse(const Batch &b, double cfl, double dfl)
 {
   DtFuse f;
-  f.dtElem = b.dtElem; f.errElem = b.errElem;
+  f.dtBlock = b.dtElem; f.errBlock = b.errElem;  // one entry per lifting block (fewer than elements)
   f.cfl = cfl; f.dfl = dfl; f.mu0 = b.cfg.mu0; f.KsPr_max = ks_pr_max(b.cfg);
   return f;
 }
@@ -538,7 +545,7 @@ inline void q_stage(const Batch &b, int mode, RKArgs rk, const DtFuse *dtf = nul
 {
   if (b.cfg.parabolic) {
     q_lifting(b, 1, dtf);
-    if (dtf) q_dt_reduce(b, 1);
+    if (dtf) q_dt_reduce(b, 1, lift_blocks(b.cfg.N, b.cfg.nElems));
   }
   if (b.cfg.nfvse && mode != VMODE_VOLINT) {
     const double *a = q_nfvse_alpha(b);
