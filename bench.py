@@ -78,12 +78,12 @@ def volume_kernel_bytes(N, nElems, parabolic, metric_bytes=8):
 def lifting_kernel_bytes(N, nElems):
     """Algorithmic HBM bytes of one launch of the Navier-Stokes lifting kernel (viscous
     form): U, metrics and sJ of every node; at each of the element's 6 faces the
-    neighbour's face-node state (5) and the face normal with SurfElem (4); written: the
-    4-component viscous volume term per node and the 4-component normal viscous flux at
-    each of the 6 faces."""
+    neighbour's face-node state (5) and the face normal with SurfElem (4), located through the
+    face-point table (two 4-byte integers per face point); written: the 4-component viscous
+    volume term per node and the 4-component normal viscous flux at each of the 6 faces."""
     n = N + 1
     n3, n2 = n ** 3, n ** 2
-    return 8 * (5 * n3 + 9 * n3 + n3 + 6 * n2 * 5 + 6 * n2 * 4 + 4 * n3 + 6 * n2 * 4) * nElems
+    return (8 * (5 * n3 + 9 * n3 + n3 + 6 * n2 * 5 + 6 * n2 * 4 + 4 * n3 + 6 * n2 * 4) + 8 * 6 * n2) * nElems
 
 
 def surface_kernel_bytes(N, nSides, parabolic):

@@ -75,6 +75,9 @@ int check_config(const fx_config *c)
   if (!c) return set_err(FX_EINVAL, "null config");
   if (c->N < 2 || c->N > 5) return set_err(FX_EINVAL, "N=%d not supported (2..5)", c->N);
   if (c->nVar != 5 && c->nVar != 9) return set_err(FX_EINVAL, "nVar=%d not supported (5, 9 GLM-MHD)", c->nVar);
+  if (!face_table_fits(c->N, c->nElems, c->nSides))
+    return set_err(FX_EINVAL, "%d elements / %d sides at N=%d: volume nodes or side points beyond 32-bit indices",
+                   c->nElems, c->nSides, c->N);
   if (c->nVar == 9) {
     if (c->volumeFlux != 10)
       return set_err(FX_EINVAL, "GLM-MHD: VolumeFlux=10 (Derigs) only");
@@ -116,6 +119,7 @@ long long fx_ctx_bytes(const fx_config *cfg)
   b += 5LL * n * n * S * 8;          // Flux
   b += 4LL * n * n * S * 8;          // nvse
   b += (12LL + 1) * E * 4 + 5LL * S * 4;
+  b += 6LL * n * n * E * 8;          // face-point table
   if (cfg->parabolic) {  // Uv, FvN, gradients
     const long long nvv = cfg->nVar == 9 ? 7 : 4;
     b += nvv * n3 * E * 8 + 2 * nvv * n * n * S * 8 + 3LL * cfg->nVar * n3 * E * 8;
@@ -172,6 +176,7 @@ int fx_ctx_create(const fx_config *cfg, fx_ctx **out)
   ALLOC(b.nvse, nf * sizeof(double4));
   ALLOC(b.e2s, (size_t)12 * cfg->nElems * 4);
   ALLOC(b.s2e, (size_t)5 * cfg->nSides * 4);
+  ALLOC(b.ftab, (size_t)6 * b.n2 * cfg->nElems * sizeof(int2));
   if (cfg->parabolic) {
     const size_t nvv = cfg->nVar == 9 ? 7 : 4;  // nonzero viscous components
     ALLOC(b.Uv, nvv * n3E * 8);
@@ -206,7 +211,7 @@ void fx_ctx_destroy(fx_ctx *c)
   if (b.stream) (void)hipStreamSynchronize(b.stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void *ptrs[] = {b.U, b.Ut, b.Uttmp, b.Mf, b.Mg, b.Mh, b.Mf32, b.Mg32, b.Mh32, b.sJ, b.Flux, b.Uv, b.FvN, b.grad,
-                  c->part, c->red, b.nvse, b.e2s, b.s2e, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
+                  c->part, c->red, b.nvse, b.e2s, b.s2e, b.ftab, b.st, b.dtElem, b.errElem, b.alpha, b.alpha2,
                   b.Ffv, b.FluxS, b.dtch1Elem, c->xgp, c->ana_part, c->ana_ops, c->ana_out, c->api_fs,
                   b.mort, b.mside, b.mops, b.Umort, b.LiftS, b.LiftM, b.GradM};
   for (void *p : ptrs)
@@ -285,6 +290,8 @@ int fx_mesh_upload(fx_ctx *c, const fx_mesh *m)
   HIPCHK(hipMemcpyAsync(b.nvse, nvse.data(), nf * sizeof(double4), hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.e2s, m->ElemToSide, (size_t)12 * E * 4, hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.s2e, m->SideToElem, (size_t)5 * S * 4, hipMemcpyHostToDevice, b.stream));
+  FX_DISPATCH_N(b.cfg.N, launch_face_table_n, b, 0, E);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b.stream));
   c->mesh_ok = true;
   return FX_OK;
@@ -828,7 +835,7 @@ int scratch_ptr(unsigned char **out)
 
 // Scratch layout of one call, each part 256-byte aligned; returns the bytes needed.
 struct NoallocLayout {
-  size_t nvse = 0, flux = 0, uv = 0, fvn = 0, alpha = 0, alpha2 = 0, ffv = 0, flag = 0, total = 0;
+  size_t nvse = 0, ftab = 0, flux = 0, uv = 0, fvn = 0, alpha = 0, alpha2 = 0, ffv = 0, flag = 0, total = 0;
 };
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 NoallocLayout noalloc_layout(const fx_config &c)
@@ -840,6 +847,7 @@ NoallocLayout noalloc_layout(const fx_config &c)
   take(L.nvse, nf * sizeof(double4));
   take(L.flux, 5 * nf * 8);
   if (c.parabolic) {
+    take(L.ftab, (size_t)6 * n * n * c.nElems * sizeof(int2));
     take(L.uv, 4 * n3E * 8);
     take(L.fvn, 8 * nf * 8);
   }
@@ -948,6 +956,7 @@ int fx_rhs_noalloc(const fx_config *cfg, const fx_mesh *mesh, const double *U, d
   b.nvse = reinterpret_cast<double4 *>(sp + L.nvse);
   b.Flux = reinterpret_cast<double *>(sp + L.flux);
   if (cfg->parabolic) {
+    b.ftab = reinterpret_cast<int2 *>(sp + L.ftab);
     b.Uv = reinterpret_cast<double *>(sp + L.uv);
     b.FvN = reinterpret_cast<double *>(sp + L.fvn);
   }
@@ -959,6 +968,7 @@ int fx_rhs_noalloc(const fx_config *cfg, const fx_mesh *mesh, const double *U, d
   const long long nf = (long long)b.n2 * cfg->nSides;
   hipLaunchKernelGGL(k_noalloc_pack_nvse, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, mesh->NormVec,
                      mesh->SurfElem, b.nvse, nf);
+  if (cfg->parabolic) FX_DISPATCH_N(b.cfg.N, launch_face_table_n, b, 0, cfg->nElems);
   RKArgs rk{};
   q_stage(b, VMODE_RHS, rk);
   HIPCHK(hipGetLastError());

@@ -146,6 +146,44 @@ __device__ __forceinline__ NodeFaces node_faces(int i, int j, int k)
   return f;
 }
 
+// Face-point table of the lifting kernel, one int2 per element face point
+// r = (locSide - 1) n^2 + a + n b (the element-local face coordinates of face_node), built once
+// per mesh from ElemToSide / SideToElem so that the kernel reads the neighbour's state and the
+// side point with one coalesced load instead of the connectivity chain and its index arithmetic:
+//   .x  the neighbour's volume node at that point, enb n^3 + vnb (SideToVol of the neighbour's
+//       side and flip); -1 on a big mortar side (no neighbour element: its lifting flux is
+//       projected from the small sides); -2 on the small side of a mortar (its master state is
+//       the big side's interpolated state at the side point)
+//   .y  2 fpt + (0 master, 1 slave), fpt = (s n + q) n + p the side point in the master frame
+// (VolToSide mappings.f90:412-426, SideToVol :459-472).
+template <int N>
+__global__ void __launch_bounds__(256)
+k_face_table(const int *__restrict__ e2s, const int *__restrict__ s2e, int e0, int nE, int nMortar,
+             int2 *__restrict__ ftab)
+{
+  constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, NF = 6 * n2;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)nE * NF) return;
+  const int e = e0 + (int)(g / NF), r = (int)(g % NF), face = r / n2, ab = r - face * n2;
+  const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+  const int sd = e2s[(e * 6 + face) * 2] - 1, fl = e2s[(e * 6 + face) * 2 + 1];
+  int p, q;
+  vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
+  const int fpt = (sd * n + q) * n + p;
+  int nb = -1;
+  if (!(fl == 0 && sd < nMortar)) {
+    const int enb = (fl == 0) ? s2e[5 * sd + 1] - 1 : s2e[5 * sd] - 1;
+    if (enb >= 0) {
+      const int vnb = (fl == 0) ? side_to_vol<N>(s2e[5 * sd + 3], s2e[5 * sd + 4], p, q)
+                                : side_to_vol<N>(s2e[5 * sd + 2], 0, p, q);
+      nb = enb * n3 + vnb;
+    } else {
+      nb = -2;
+    }
+  }
+  ftab[(size_t)e * NF + r] = make_int2(nb, 2 * fpt + (fl == 0 ? 0 : 1));
+}
+
 // -----------------------------------------------------------------------------------
 // Launch geometry of the volume kernel: EPB elements per block and W = EPB*(N+1)^2
 // threads per direction rounded up to a multiple of 32 (3*W threads per block).  Tiles[0]
@@ -817,8 +855,8 @@ struct NsLift {
 template <int N, int OUT, class EQ, int LIFT = 1, bool DT = false>
 __global__ void __launch_bounds__(256, FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
-          const double *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
-          const int *__restrict__ s2e, const double4 *__restrict__ nvse, double *__restrict__ out0,
+          const double *__restrict__ Mh, const double *__restrict__ sJ, const int2 *__restrict__ ftab,
+          const double4 *__restrict__ nvse, double *__restrict__ out0,
           double *__restrict__ out1, double *__restrict__ out2, const int *__restrict__ eprob,
           int nElems, EQ eq, double etaBR2, const double *__restrict__ Um, const double *__restrict__ LiftM,
           double *__restrict__ GradM, int nMortar, DtFuse dtf)
@@ -863,45 +901,32 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   // coordinates keep a wavefront on few sides): neighbour state, its entropy variables and
   // the strong lifting flux, Lifting_FillFlux (br1_fillflux.f90:49-91)
   for (int it = tid; it < EPB * NF; it += 256) {
-    const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
+    const int fe = it / NF, r = it - fe * NF;
     const int ee = blockIdx.x * EPB + fe;
     if (ee >= nElems || elem_problem(eprob, ee) < 0) continue;
-    const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
-    const int fi = fnode % n, fj = (fnode / n) % n, fk = fnode / n2;
-    const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
-    int p, q;
-    vol_to_side<N>(loc, fl, fi, fj, fk, p, q);
-    const int fpt = (sd * n + q) * n + p;
+    const int2 t = ftab[(size_t)blockIdx.x * EPB * NF + it];
+    s_fpt[fe][r] = t.y;
     // a big mortar side has no neighbour element: its lifting flux comes projected (LiftM); the
     // small side of a mortar has the big side's interpolated state (Um) as its master
-    if (fl == 0 && sd < nMortar) {
-      s_fpt[fe][r] = 2 * fpt;
-      continue;
-    }
-    const int enb = (fl == 0) ? s2e[5 * sd + 1] - 1 : s2e[5 * sd] - 1;
-    const double *unbp;
-    if (enb >= 0) {
-      const int vnb = (fl == 0) ? side_to_vol<N>(s2e[5 * sd + 3], s2e[5 * sd + 4], p, q)
-                                : side_to_vol<N>(s2e[5 * sd + 2], 0, p, q);
-      unbp = U + ((size_t)enb * n3 + vnb) * NV;
-    } else {
-      unbp = Um + (size_t)fpt * NV;
-    }
+    if (t.x == -1) continue;
+    const int fpt = t.y >> 1;
+    const bool master = (t.y & 1) == 0;
+    const double *unbp = (t.x >= 0) ? U + (size_t)t.x * NV : Um + (size_t)fpt * NV;
     double unb[NV], wnb[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) unb[v] = unbp[v];
-    eq.entropy(unb, wnb);
     const double4 g = nvse[fpt];
+    eq.entropy(unb, wnb);
+    const int face = r / n2, ab = r - face * n2, fnode = face_node<N>(face + 1, ab % n, ab / n);
     double *d = &s_u[fe][WSZ + r * (NV + 3)];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const double wo = LW(fe, v, fnode);
-      const double ws = (fl == 0) ? wnb[v] : wo;
-      const double wm = (fl == 0) ? wo : wnb[v];
+      const double ws = master ? wnb[v] : wo;
+      const double wm = master ? wo : wnb[v];
       d[v] = 0.5 * (ws - wm) * g.w;
     }
     d[NV] = g.x; d[NV + 1] = g.y; d[NV + 2] = g.z;
-    s_fpt[fe][r] = 2 * fpt + (fl == 0 ? 0 : 1);
   }
   __syncthreads();
   double gx[NV], gy[NV], gz[NV];

@@ -412,6 +412,7 @@ struct Batch {
     return a;
   }
   double rkA[5], rkb[5], rkc[5];
+  int2 *ftab = nullptr;  // face-point table of the lifting kernel (k_face_table)
 
   void set_degree(int N)
   {
@@ -421,6 +422,13 @@ struct Batch {
   }
 };
 
+// The face-point table holds volume nodes and 2 x side points as 32-bit integers.
+inline bool face_table_fits(int N, long long nElems, long long nSides)
+{
+  const long long n = N + 1;
+  return nElems * n * n * n < 2147483647LL && 2 * nSides * n * n + 1 < 2147483647LL;
+}
+
 // ---- per-degree launchers ---------------------------------------------------------------
 // Defined in fx_launch_n.hpp and instantiated for one degree per compilation unit
 // (fx_tu.hip, built once per N and kernel group), so the kernels of different degrees
@@ -428,6 +436,9 @@ struct Batch {
 // (c_ops); ensure_ops fills every unit's copy.
 template <int N> void launch_volume_n(const Batch &b, int mode, RKArgs rk);
 template <int N> void launch_surface_n(const Batch &b);
+// the face-point table of elements e0 .. e0 + nE - 1 from b.e2s / b.s2e (after every upload of
+// their connectivity)
+template <int N> void launch_face_table_n(const Batch &b, int e0, int nE);
 // U_Mortar into b.Umort, Flux_Mortar into the big sides of b.Flux (meshes with mortars)
 template <int N> void launch_mortar_u_n(const Batch &b);
 template <int N> void launch_flux_mortar_n(const Batch &b);

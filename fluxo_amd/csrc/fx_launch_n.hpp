@@ -154,6 +154,15 @@ void launch_surface_n(const Batch &b)
   }
 }
 
+template <int N>
+void launch_face_table_n(const Batch &b, int e0, int nE)
+{
+  const long long cnt = (long long)nE * 6 * b.n2;
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(k_face_table<N>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, b.stream, b.e2s, b.s2e, e0, nE,
+                     b.nMortar, b.ftab);
+}
+
 template <int N, class EQ, int LIFT>
 void launch_lifting_l(const Batch &b, int out, const EQ &eq, const DtFuse *dtf)
 {
@@ -164,19 +173,19 @@ void launch_lifting_l(const Batch &b, int out, const EQ &eq, const DtFuse *dtf)
   const DtFuse none{};
   if (out == 0)
     hipLaunchKernelGGL((k_lifting<N, 0, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq,
+                       b.ftab, b.nvse, b.grad, b.grad + nv, b.grad + 2 * nv, b.eprob, b.cfg.nElems, eq,
                        b.cfg.etaBR2, b.Umort, b.LiftM, b.GradM, b.nMortar, none);
   else if constexpr (std::is_same<EQ, NsLift>::value) {
     if (dtf) {
       hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT, true>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh,
-                         b.sJ, b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
+                         b.sJ, b.ftab, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
                          b.Umort, b.LiftM, b.GradM, b.nMortar, *dtf);
       return;
     }
   }
   if (out != 0)
     hipLaunchKernelGGL((k_lifting<N, 1, EQ, LIFT>), dim3(nb), dim3(256), 0, b.stream, b.U, b.Mf, b.Mg, b.Mh, b.sJ,
-                       b.e2s, b.s2e, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
+                       b.ftab, b.nvse, b.Uv, b.FvN, nullptr, b.eprob, b.cfg.nElems, eq, b.cfg.etaBR2,
                        b.Umort, b.LiftM, b.GradM, b.nMortar, none);
 }
 

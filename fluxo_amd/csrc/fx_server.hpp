@@ -143,6 +143,8 @@ inline int pool_realloc(Pool &p, const fx_server_config &c, int Ecap, int Scap, 
   Batch &b = p.b;
   const size_t n3 = b.n3, n2 = b.n2;
   const size_t oE = p.ea.cap, oS = p.sa.cap, oR = p.ra.cap;
+  if (!face_table_fits(p.N, Ecap, Scap))
+    return set_err(FX_ENOMEM, "pool of N=%d: %d elements / %d sides beyond 32-bit face-table indices", p.N, Ecap, Scap);
   long long bytes = 0;
   auto grow = [&](auto *&ptr, size_t per, size_t oldn, size_t newn, int fill) -> int {
     using T = std::remove_reference_t<decltype(*ptr)>;
@@ -174,6 +176,7 @@ inline int pool_realloc(Pool &p, const fx_server_config &c, int Ecap, int Scap, 
   }
   if (p.parabolic) GROW(b.Uv, 4 * n3, oE, Ecap, -1);
   GROW(b.e2s, 12, oE, Ecap, 0);
+  GROW(b.ftab, 6 * n2, oE, Ecap, 0);
   GROW(b.eprob, 1, oE, Ecap, 0xFF);  // -1: unused
   GROW(b.dtElem, 1, oE, Ecap, -1);
   GROW(b.errElem, 1, oE, Ecap, 0);
@@ -206,7 +209,7 @@ inline void pool_free(Pool &p)
   if (b.stream) (void)hipStreamSynchronize(b.stream);
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
   p.graph = nullptr;
-  void *ptrs[] = {b.U, b.Uttmp, b.Mf, b.Mg, b.Mh, b.sJ, b.Mf32, b.Mg32, b.Mh32, b.Uv, b.FvN, b.e2s, b.eprob,
+  void *ptrs[] = {b.U, b.Uttmp, b.Mf, b.Mg, b.Mh, b.sJ, b.Mf32, b.Mg32, b.Mh32, b.Uv, b.FvN, b.e2s, b.ftab, b.eprob,
                   b.dtElem, b.errElem, b.Flux, b.nvse, b.s2e, b.st, b.rfirst, b.rcount};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -274,6 +277,8 @@ inline int upload_request(Pool &p, Request &q)
   HIPCHK(hipMemcpyAsync(b.e2s + 12 * eo, e2s.data(), 12 * (size_t)E * 4, hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.s2e + 5 * so, s2e.data(), 5 * (size_t)S * 4, hipMemcpyHostToDevice, b.stream));
   HIPCHK(hipMemcpyAsync(b.eprob + eo, ep.data(), (size_t)E * 4, hipMemcpyHostToDevice, b.stream));
+  FX_DISPATCH_N(p.N, launch_face_table_n, b, q.e0, E);
+  HIPCHK(hipGetLastError());
   DtState z{};
   z.errType = 0; z.errStep = -1; z.dt = 0.0; z.t = 0.0; z.step = 0;
   z.glm_init = 0; z.glm_ch = 0.5; z.glm_dtch1 = 0.0;

@@ -4,7 +4,7 @@
 // fx_engine.hip into libfluxo_gpu.so).
 //   group 0  split-form volume kernel (every VolumeFlux, mode, precision and tile)
 //   group 1  surface kernel (every Riemann solver), SurfInt alone, NFVSE shock capturing,
-//            the mortar interpolation and projection
+//            the mortar interpolation and projection, the face-point table
 //   group 2  lifting, CALCTIMESTEP, CalcEntropy, analysis
 //   group 3  GLM-MHD volume, surface and CALCTIMESTEP kernels
 #include <hip/hip_runtime.h>
@@ -24,6 +24,7 @@ namespace fx {
 template void launch_volume_n<FX_TU_N>(const Batch &, int, RKArgs);
 #elif FX_TU_GROUP == 1
 template void launch_surface_n<FX_TU_N>(const Batch &);
+template void launch_face_table_n<FX_TU_N>(const Batch &, int, int);
 template void launch_nfvse_alpha_n<FX_TU_N>(const Batch &);
 template void launch_nfvse_fv_n<FX_TU_N>(const Batch &, const double *);
 template void launch_surfint_n<FX_TU_N>(const Batch &, const double *, const double *);
