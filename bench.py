@@ -592,7 +592,7 @@ def main():
         pass
     tile = r64["tiles"]["chosen_elems_per_workgroup"]
     kern = {"volume": ("k_volume<3, 32, true, 1, double, %d>" % tile, volume_kernel_bytes(N, mesh["nElems"], True)),
-            "lifting": ("k_lifting<3, 1, fx::NsLift, 1>", lifting_kernel_bytes(N, mesh["nElems"])),
+            "lifting": ("k_lifting<3, 1, fx::NsLift, 1, false>", lifting_kernel_bytes(N, mesh["nElems"])),
             "surface": ("k_surface<3, 1, true>", surface_kernel_bytes(N, mesh["nSides"], True))}
     roofs = {}
     for k, (kname, nbytes) in kern.items():

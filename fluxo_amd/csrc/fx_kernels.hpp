@@ -186,7 +186,7 @@ k_face_table(const int *__restrict__ e2s, const int *__restrict__ s2e, int e0, i
 
 // -----------------------------------------------------------------------------------
 // Launch geometry of the volume kernel: EPB elements per block and W = EPB*(N+1)^2
-// threads per direction rounded up to a multiple of 32 (3*W threads per block).  Tiles[0]
+// threads per direction rounded up to a multiple of 32 (3*W threads in the line phase).  Tiles[0]
 // is the default EPB of each N; the others are the candidates the volume-kernel tuner
 // times (fx_tune_volume) for the LSERK form with the Ranocha flux.
 template <int N> struct VolTiles;
@@ -194,9 +194,14 @@ template <> struct VolTiles<2> { static constexpr int n = 3, T[3] = {7, 14, 3}; 
 template <> struct VolTiles<3> { static constexpr int n = 3, T[3] = {4, 2, 8}; };
 template <> struct VolTiles<4> { static constexpr int n = 3, T[3] = {5, 2, 3}; };
 template <> struct VolTiles<5> { static constexpr int n = 3, T[3] = {3, 1, 2}; };
+// TB threads per block: the 3 W of the line phase, plus one wavefront that sits out the line
+// phase where it gives every node of the block its own thread in the node passes (N=3 with
+// 4 elements: 256 nodes on 256 threads instead of two rounds on 192).
 template <int N, int EPB_> struct VolGeom {
   static constexpr int EPB = EPB_;
   static constexpr int W = ((EPB * (N + 1) * (N + 1) + 31) / 32) * 32;
+  static constexpr int NODES = EPB * (N + 1) * (N + 1) * (N + 1);
+  static constexpr int TB = 3 * W + ((3 * W < NODES && 3 * W + 64 >= NODES) ? 64 : 0);
 };
 
 // LDS node index with the xi-rows padded to n+1 so xi-line owners (stride n) do not
@@ -284,14 +289,14 @@ struct RKArgs {
 #define FX_VOL_WAVES32 1
 #endif
 template <int N, int VF, bool PARA, int MODE, typename VR, int EPB_>
-__global__ void __launch_bounds__((3 * VolGeom<N, EPB_>::W), (sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES))
+__global__ void __launch_bounds__((VolGeom<N, EPB_>::TB), (sizeof(VR) == 4 ? FX_VOL_WAVES32 : FX_VOL_WAVES))
 k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict__ Mg,
          const VR *__restrict__ Mh, const double *__restrict__ sJ, const int *__restrict__ e2s,
          const double *__restrict__ Flux, const double *__restrict__ Uv, double *__restrict__ Ut,
          double *__restrict__ Uttmp, RKArgs rk, int nElems, double kappa)
 {
   constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-  constexpr int EPB = EPB_, W = VolGeom<N, EPB_>::W, NP = Lds<N>::NP;
+  constexpr int EPB = EPB_, W = VolGeom<N, EPB_>::W, TB = VolGeom<N, EPB_>::TB, NP = Lds<N>::NP;
   constexpr int NFV = n_fields<VF>();
   constexpr int SLOTS = NFV > 5 ? NFV : 5;
   static_assert(EPB * n2 <= W, "volume launch geometry");
@@ -321,7 +326,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   const VR skappaM1 = VR(1.0 / kappaM1);
 
   // phase 1: per-node fields into LDS
-  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+  for (int idx = tid; idx < EPB * n3; idx += TB) {
     const int el = idx / n3, node = idx - el * n3;
     const int e = e0 + el;
     if (e < nElems && elem_problem(rk.eprob, e) >= 0) {
@@ -337,11 +342,11 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   }
   __syncthreads();
 
-  // phase 2: one thread per (direction, element, line)
+  // phase 2: one thread per (direction, element, line); an extra wavefront (dir 3) sits it out
   const int dir = tid / W, lt = tid - dir * W;
   const int el = lt / n2, line = lt - el * n2;
   const int e = e0 + el;
-  const bool active = (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
+  const bool active = (dir < 3) && (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
   const int a = line % n, b = line / n;
   int nodes[n], lds[n];
 #pragma unroll
@@ -421,7 +426,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   constexpr int NF = 6 * n2;
   __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * 5 : 1];
   if (MODE != VMODE_VOLINT) {
-    for (int it = tid; it < EPB * NF; it += 3 * W) {
+    for (int it = tid; it < EPB * NF; it += TB) {
       const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
       const int ee = e0 + fe;
       if (ee >= nElems || elem_problem(rk.eprob, ee) < 0) continue;
@@ -437,7 +442,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     __syncthreads();
   }
   // phase 3c: one thread per node
-  for (int idx = tid; idx < EPB * n3; idx += 3 * W) {
+  for (int idx = tid; idx < EPB * n3; idx += TB) {
     const int el2 = idx / n3, node = idx - el2 * n3;
     const int e2 = e0 + el2;
     if (e2 >= nElems) continue;

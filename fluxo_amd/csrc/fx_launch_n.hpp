@@ -14,13 +14,13 @@ void launch_volume_e(const Batch &b, RKArgs rk)
   if (nb <= 0) return;
   if constexpr (has_fp32(VF)) {
     if (b.cfg.precision == FX_PREC_FP32_SPLIT) {
-      hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+      hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, float, EPB>), dim3(nb), dim3(G::TB), 0, b.stream, b.U,
                          b.Mf32, b.Mg32, b.Mh32, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                          b.cfg.kappa);
       return;
     }
   }
-  hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(3 * G::W), 0, b.stream, b.U,
+  hipLaunchKernelGGL((k_volume<N, VF, PARA, MODE, double, EPB>), dim3(nb), dim3(G::TB), 0, b.stream, b.U,
                        b.Mf, b.Mg, b.Mh, b.sJ, b.e2s, b.Flux, b.Uv, b.Ut, b.Uttmp, rk, b.cfg.nElems,
                        b.cfg.kappa);
 }
