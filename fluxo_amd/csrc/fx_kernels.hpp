@@ -858,7 +858,9 @@ struct NsLift {
 // one element, then one LDS maximum per segment; the bounds are positive and not NaN, so
 // their bit patterns order as the values and the maxima are exact in any order.
 template <int N, int OUT, class EQ, int LIFT = 1, bool DT = false>
-__global__ void __launch_bounds__(256, FX_LIFT_WAVES)
+// (Navier-Stokes BR1 at N = 3 asks for 4 waves per SIMD: the matrix-core operands then stay in
+// 96-109 VGPRs instead of taking accumulation registers that cost an occupancy level.)
+__global__ void __launch_bounds__(256, (N == 3 && EQ::NV == 5 && LIFT == 1) ? 4 : FX_LIFT_WAVES)
 k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const double *__restrict__ Mg,
           const double *__restrict__ Mh, const double *__restrict__ sJ, const int2 *__restrict__ ftab,
           const double4 *__restrict__ nvse, double *__restrict__ out0,
@@ -937,21 +939,49 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   double gx[NV], gy[NV], gz[NV];
   if (active) {
     double gxi[NV], geta[NV], gzeta[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      gxi[v] = op.D[i][0] * LW(el, v, 0 + n * (j + n * k));
-      geta[v] = op.D[j][0] * LW(el, v, i + n * (0 + n * k));
-      gzeta[v] = op.D[k][0] * LW(el, v, i + n * (j + n * 0));
-    }
-    // not unrolled: all n NV LDS operands in flight at once would cost an occupancy level
-#pragma unroll 1
-    for (int l = 1; l < n; l++)
+    if constexpr (N == 3 && LIFT == 1) {
+      // The strong-form D contractions on the matrix core (BR1 at N = 3: 5 % faster than the
+      // FMA loop below on the Navier-Stokes benchmark; with BR2 the matrix-core registers would
+      // cost an occupancy level): a wavefront is one element, lane = node i + 4 j + 16 k.  v_mfma_f64_4x4x4_4b_f64 forms 4 blocks b of D(m,n) = sum_q A(m,q)
+      // B(q,n) with A(m,q) in lane m + 4 b + 16 q, B(q,n) in lane n + 4 b + 16 q and D(m,n) in
+      // lane n + 4 b + 16 m.  Every contraction takes the operator entry D(r,q) in lane
+      // r + 4 b + 16 q (dop):
+      //   zeta: A = D(k, q) [m=k, b=j], B = W(i, j, q) = the lane's own entropy variables, D in
+      //         the node's lane;
+      //   xi:   A = W(q, j, k) [m=k, b=j], read from LDS at node q + 4 j + 16 k, B = D(i, q);
+      //   eta:  A = D(j, q) [m=j, b=k], B = W(i, q, k) from LDS; D(j, i; b=k) lands in lane
+      //         i + 4 k + 16 j and is fetched from there (ds_bpermute).
+      const int L = node;
+      const double dop = op.D[L & 3][L >> 4];
+      const int nx = (L >> 4) + (L & 12) + 16 * (L & 3);      // lane z+4j+16q <- node q+4j+16z
+      const int ny = (L & 3) + 4 * (L >> 4) + 16 * ((L >> 2) & 3);  // lane i+4k+16q <- node i+4q+16k
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        gxi[v] = gxi[v] + op.D[i][l] * LW(el, v, l + n * (j + n * k));
-        geta[v] = geta[v] + op.D[j][l] * LW(el, v, i + n * (l + n * k));
-        gzeta[v] = gzeta[v] + op.D[k][l] * LW(el, v, i + n * (j + n * l));
+        gzeta[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(dop, w[v], 0.0, 0, 0, 0);
+        gxi[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(LW(el, v, nx), dop, 0.0, 0, 0, 0);
+        const double te = __builtin_amdgcn_mfma_f64_4x4x4f64(dop, LW(el, v, ny), 0.0, 0, 0, 0);
+        const long long tb = __double_as_longlong(te);
+        const int lo = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb & 0xffffffffLL));
+        const int hi = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb >> 32));
+        geta[v] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
       }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        gxi[v] = op.D[i][0] * LW(el, v, 0 + n * (j + n * k));
+        geta[v] = op.D[j][0] * LW(el, v, i + n * (0 + n * k));
+        gzeta[v] = op.D[k][0] * LW(el, v, i + n * (j + n * 0));
+      }
+      // not unrolled: all n NV LDS operands in flight at once would cost an occupancy level
+#pragma unroll 1
+      for (int l = 1; l < n; l++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          gxi[v] = gxi[v] + op.D[i][l] * LW(el, v, l + n * (j + n * k));
+          geta[v] = geta[v] + op.D[j][l] * LW(el, v, i + n * (l + n * k));
+          gzeta[v] = gzeta[v] + op.D[k][l] * LW(el, v, i + n * (j + n * l));
+        }
+    }
     const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
     const double sj = sJ[c];
 #pragma unroll
@@ -1151,16 +1181,36 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   __syncthreads();
   if (active) {
     double uv[NVV];
+    if constexpr (N == 3 && LIFT == 1) {
+      // the D_hat contractions on the matrix core, with the lane layout of the D contractions
+      // above: xi and zeta accumulate in the node's lane (zeta reads the node's own h~), eta
+      // lands in lane i + 4 k + 16 j and is fetched from there
+      const int L = node;
+      const double dh = op.Dhat[L & 3][L >> 4];
+      const int nx = (L >> 4) + (L & 12) + 16 * (L & 3);
+      const int ny = (L & 3) + 4 * (L >> 4) + 16 * ((L >> 2) & 3);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) uv[v] = 0.0;
+      for (int v = 0; v < NVV; v++) {
+        double acc = __builtin_amdgcn_mfma_f64_4x4x4f64(s_u[el][v * n3 + nx], dh, 0.0, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(2 * NVV + v) * n3 + L], acc, 0, 0, 0);
+        const double te = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(NVV + v) * n3 + ny], 0.0, 0, 0, 0);
+        const long long tb = __double_as_longlong(te);
+        const int lo = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb & 0xffffffffLL));
+        const int hi = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb >> 32));
+        uv[v] = acc + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NVV; v++) uv[v] = 0.0;
 #pragma unroll 1
-    for (int l = 0; l < n; l++) {
-      const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
+      for (int l = 0; l < n; l++) {
+        const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
 #pragma unroll
-      for (int v = 0; v < NVV; v++)
-        uv[v] = uv[v] + di * s_u[el][v * n3 + l + n * (j + n * k)] +
-                dj * s_u[el][(NVV + v) * n3 + i + n * (l + n * k)] +
-                dk * s_u[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
+        for (int v = 0; v < NVV; v++)
+          uv[v] = uv[v] + di * s_u[el][v * n3 + l + n * (j + n * k)] +
+                  dj * s_u[el][(NVV + v) * n3 + i + n * (l + n * k)] +
+                  dk * s_u[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
+      }
     }
 #pragma unroll
     for (int v = 0; v < NVV; v++) out0[NVV * c + v] = uv[v];
