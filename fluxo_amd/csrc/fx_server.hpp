@@ -130,7 +130,7 @@ inline double seconds_since(const fx_server *s) { return std::chrono::duration<d
 inline long long request_bytes(const fx_server_config &c, int N, int par, long long E, long long S)
 {
   const long long n = N + 1, n3 = n * n * n, n2 = n * n;
-  long long per_e = 2 * 5 * n3 * 8 + 9 * n3 * 8 + n3 * 8 + 12 * 4 + 4 + 8 + 4;
+  long long per_e = 2 * 5 * n3 * 8 + 9 * n3 * 8 + n3 * 8 + 12 * 4 + 4 + 8 + 4 + 6 * n2 * 8;  // ... face-point table
   if (c.precision == FX_PREC_FP32_SPLIT) per_e += 9 * n3 * 4;
   if (par) per_e += 4 * n3 * 8;
   const long long per_s = 5 * n2 * 8 + n2 * (long long)sizeof(double4) + 5 * 4 + (par ? 8 * n2 * 8 : 0);
