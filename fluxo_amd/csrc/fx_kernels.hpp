@@ -325,24 +325,9 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   const double kappaM1 = kappa - 1.0;
   const VR skappaM1 = VR(1.0 / kappaM1);
 
-  // phase 1: per-node fields into LDS
-  for (int idx = tid; idx < EPB * n3; idx += TB) {
-    const int el = idx / n3, node = idx - el * n3;
-    const int e = e0 + el;
-    if (e < nElems && elem_problem(rk.eprob, e) >= 0) {
-      const double *u = U + ((size_t)e * n3 + node) * 5;
-      double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
-      double f[NF_COUNT];
-      node_fields<VF>(uu, kappaM1, f);
-      const int l = Lds<N>::li(node);
-#pragma unroll
-      for (int c = 0; c < NF_COUNT; c++)
-        if (uses_field<VF>(c)) SL(el, field_slot<VF>(c), l) = VR(f[c]);
-    }
-  }
-  __syncthreads();
-
-  // phase 2: one thread per (direction, element, line); an extra wavefront (dir 3) sits it out
+  // phase 2's thread: one per (direction, element, line); an extra wavefront (dir 3) sits it
+  // out.  Its line's metric vectors are loaded before phase 1, so that they are in flight
+  // together with phase 1's state loads instead of after the barrier.
   const int dir = tid / W, lt = tid - dir * W;
   const int el = lt / n2, line = lt - el * n2;
   const int e = e0 + el;
@@ -355,6 +340,34 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     nodes[l] = i + n * (j + n * k);
     lds[l] = Lds<N>::li(i, j, k);
   }
+  VR m[n][3];
+  if (active) {
+    const VR *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
+#pragma unroll
+    for (int l = 0; l < n; l++) {
+      const VR *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
+      m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
+    }
+  }
+
+  // phase 1: per-node fields into LDS
+  for (int idx = tid; idx < EPB * n3; idx += TB) {
+    const int el1 = idx / n3, node = idx - el1 * n3;
+    const int e1 = e0 + el1;
+    if (e1 < nElems && elem_problem(rk.eprob, e1) >= 0) {
+      const double *u = U + ((size_t)e1 * n3 + node) * 5;
+      double uu[5] = {u[0], u[1], u[2], u[3], u[4]};
+      double f[NF_COUNT];
+      node_fields<VF>(uu, kappaM1, f);
+      const int l = Lds<N>::li(node);
+#pragma unroll
+      for (int c = 0; c < NF_COUNT; c++)
+        if (uses_field<VF>(c)) SL(el1, field_slot<VF>(c), l) = VR(f[c]);
+    }
+  }
+  __syncthreads();
+
+  // phase 2: the line's two-point fluxes
 #if !FX_VOL_ND_LDS
   VR nd[n][NF_COUNT];
 #pragma unroll
@@ -371,13 +384,6 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 #pragma unroll
     for (int v = 0; v < 5; v++) acc[i][v] = VR(0);
   if (active) {
-    const VR *M = (dir == 0) ? Mf : ((dir == 1) ? Mg : Mh);
-    VR m[n][3];
-#pragma unroll
-    for (int l = 0; l < n; l++) {
-      const VR *mp = M + ((size_t)e * n3 + nodes[l]) * 3;
-      m[l][0] = mp[0]; m[l][1] = mp[1]; m[l][2] = mp[2];
-    }
 #pragma unroll
     for (int i = 0; i < n; i++)
 #pragma unroll
