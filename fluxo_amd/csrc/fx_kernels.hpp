@@ -303,6 +303,9 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 #ifndef FX_VOL_EPAD
 #define FX_VOL_EPAD 0
 #endif
+#ifndef FX_VOL_PRE
+#define FX_VOL_PRE 1
+#endif
   // per-element LDS blocks padded by FX_VOL_EPAD doubles to spread elements over banks
   __shared__ VR s_lds_raw[EPB][SLOTS * NP + FX_VOL_EPAD];  // node fields
   auto s_lds = reinterpret_cast<VR (*)[SLOTS * NP + FX_VOL_EPAD]>(s_lds_raw);
@@ -408,6 +411,29 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
         }
       }
   }
+  // where every node of the block has its own thread, the node pass's inputs from memory are
+  // loaded here, before the direction-sum barriers, so that their latency overlaps them
+  constexpr bool PRE = (TB >= EPB * n3) && MODE != VMODE_VOLINT && FX_VOL_PRE;
+  double pre_uv[4] = {0.0, 0.0, 0.0, 0.0}, pre_tt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double pre_u[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, pre_sj = 0.0;
+  if (PRE && tid < EPB * n3) {
+    const int el2 = tid / n3, e2 = e0 + el2;
+    if (e2 < nElems && elem_problem(rk.eprob, e2) >= 0) {
+      const size_t c = (size_t)e2 * n3 + (tid - el2 * n3);
+      if (PARA) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) pre_uv[q] = Uv[c * 4 + q];
+      }
+      pre_sj = sJ[c];
+      if (MODE == VMODE_RK) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          pre_u[v] = U[c * 5 + v];
+          if (!rk.first) pre_tt[v] = Uttmp[c * 5 + v];
+        }
+      }
+    }
+  }
   // phase 3a: xi + eta + zeta partial sums, in this order, into LDS
 #if FX_VOL_ND_LDS && FX_VOL_ACC_ALIAS
   __syncthreads();  // the sums overwrite the node fields: every line must have read them
@@ -471,7 +497,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     if (PARA && MODE != VMODE_VOLINT) {  // VolInt_visc, formed by the lifting kernel
       const double *uv = Uv + ((size_t)e2 * n3 + node) * 4;
 #pragma unroll
-      for (int c = 0; c < 4; c++) tot[c + 1] = tot[c + 1] + uv[c];
+      for (int c = 0; c < 4; c++) tot[c + 1] = tot[c + 1] + (PRE ? pre_uv[c] : uv[c]);
     }
     if (MODE != VMODE_VOLINT) {
       // SurfInt: the faces touched by node (i,j,k), in local side order
@@ -483,7 +509,7 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
 #pragma unroll
         for (int v = 0; v < 5; v++) tot[v] = tot[v] + s_face[el2][r * 5 + v];
       }
-      const double msj = -sJ[(size_t)e2 * n3 + node];
+      const double msj = -(PRE ? pre_sj : sJ[(size_t)e2 * n3 + node]);
 #pragma unroll
       for (int v = 0; v < 5; v++) tot[v] = tot[v] * msj;
     }
@@ -492,9 +518,9 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
       const double bdt = rk.b * rk.st[prob].dt;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double ut = rk.first ? tot[v] : Uttmp[o + v] * (-rk.A) + tot[v];
+        const double ut = rk.first ? tot[v] : (PRE ? pre_tt[v] : Uttmp[o + v]) * (-rk.A) + tot[v];
         Uttmp[o + v] = ut;
-        U[o + v] = U[o + v] + ut * bdt;
+        U[o + v] = (PRE ? pre_u[v] : U[o + v]) + ut * bdt;
       }
     } else {
 #pragma unroll
