@@ -926,6 +926,43 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     if (tid < EPB * 6) s_dtl[tid / 6][tid % 6] = 0ull;
     else s_dterr[tid - EPB * 6] = 0;
   }
+  // the node's metric vectors and Jacobian, loaded ahead of the face phase so that their latency
+  // overlaps it instead of following its barrier (Navier-Stokes BR1 at N = 3, whose 4 waves per
+  // SIMD leave room for them; elsewhere they would cost an occupancy level)
+#ifndef FX_LIFT_PREM
+#define FX_LIFT_PREM 1
+#endif
+  constexpr bool PREM = FX_LIFT_PREM && N == 3 && NV == 5 && LIFT == 1;
+  double mreg[9], sjreg = 0.0;
+  if (PREM && active) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      mreg[q] = Mf[3 * c + q];
+      mreg[3 + q] = Mg[3 * c + q];
+      mreg[6 + q] = Mh[3 * c + q];
+    }
+    sjreg = sJ[c];
+  }
+  // the thread's first face point (phase F): table entry, neighbour state and normal, loaded
+  // before the element's own state so that both are in flight together (where the metric
+  // prefetch above is made too)
+  constexpr bool PREF = PREM;
+  int2 ft0 = make_int2(-1, -1);
+  double un0[NV];
+  double4 g0 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (PREF && tid < EPB * NF) {
+    const int ee = blockIdx.x * EPB + tid / NF;
+    if (ee < nElems && elem_problem(eprob, ee) >= 0) {
+      ft0 = ftab[(size_t)blockIdx.x * EPB * NF + tid];
+      if (ft0.x != -1) {
+        const int fpt = ft0.y >> 1;
+        const double *unbp = (ft0.x >= 0) ? U + (size_t)ft0.x * NV : Um + (size_t)fpt * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) un0[v] = unbp[v];
+        g0 = nvse[fpt];
+      }
+    }
+  }
   double u[NV], w[NV];
   if (active) {
     const double *up = U + c * NV;
@@ -943,18 +980,26 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     const int fe = it / NF, r = it - fe * NF;
     const int ee = blockIdx.x * EPB + fe;
     if (ee >= nElems || elem_problem(eprob, ee) < 0) continue;
-    const int2 t = ftab[(size_t)blockIdx.x * EPB * NF + it];
+    const bool pre = PREF && it == tid;
+    const int2 t = pre ? ft0 : ftab[(size_t)blockIdx.x * EPB * NF + it];
     s_fpt[fe][r] = t.y;
     // a big mortar side has no neighbour element: its lifting flux comes projected (LiftM); the
     // small side of a mortar has the big side's interpolated state (Um) as its master
     if (t.x == -1) continue;
     const int fpt = t.y >> 1;
     const bool master = (t.y & 1) == 0;
-    const double *unbp = (t.x >= 0) ? U + (size_t)t.x * NV : Um + (size_t)fpt * NV;
     double unb[NV], wnb[NV];
+    double4 g;
+    if (pre) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) unb[v] = unbp[v];
-    const double4 g = nvse[fpt];
+      for (int v = 0; v < NV; v++) unb[v] = un0[v];
+      g = g0;
+    } else {
+      const double *unbp = (t.x >= 0) ? U + (size_t)t.x * NV : Um + (size_t)fpt * NV;
+#pragma unroll
+      for (int v = 0; v < NV; v++) unb[v] = unbp[v];
+      g = nvse[fpt];
+    }
     eq.entropy(unb, wnb);
     const int face = r / n2, ab = r - face * n2, fnode = face_node<N>(face + 1, ab % n, ab / n);
     double *d = &s_u[fe][WSZ + r * (NV + 3)];
@@ -1014,8 +1059,8 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
           gzeta[v] = gzeta[v] + op.D[k][l] * LW(el, v, i + n * (j + n * l));
         }
     }
-    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
-    const double sj = sJ[c];
+    const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
+    const double sj = PREM ? sjreg : sJ[c];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       gx[v] = mf[0] * gxi[v] + mg[0] * geta[v] + mh[0] * gzeta[v];
@@ -1127,7 +1172,8 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     int err = 0;
     const int key = active ? el : -1;
     if (active)
-      node_dt_bounds(u, Mf + 3 * c, Mg + 3 * c, Mh + 3 * c, sJ[c], eq.kappa, dtf.mu0, dtf.KsPr_max, 1, l, err);
+      node_dt_bounds(u, PREM ? mreg : Mf + 3 * c, PREM ? mreg + 3 : Mg + 3 * c, PREM ? mreg + 6 : Mh + 3 * c,
+                     PREM ? sjreg : sJ[c], eq.kappa, dtf.mu0, dtf.KsPr_max, 1, l, err);
     const int lane = tid & 63;
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) {  // suffix maximum within the lanes of one element
@@ -1199,7 +1245,7 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       }
     }
     // transformed fluxes f~, g~, h~ (EvalDiffFluxTilde3D) over the node's own slots
-    const double *mf = Mf + 3 * c, *mg = Mg + 3 * c, *mh = Mh + 3 * c;
+    const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
     EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
 #pragma unroll
     for (int v = 0; v < NVV; v++) s_u[el][v * n3 + node] = o[v];
