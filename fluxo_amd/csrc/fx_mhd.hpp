@@ -388,6 +388,30 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
   const bool active = (el < EPB) && (e < nElems) && elem_problem(rk.eprob, e) >= 0;
   const int i = node % n, j = (node / n) % n, k = node / n2;
   const int ln = Lds<N>::li(node);
+  // the face gather's fluxes (phase 3), loaded before phase 1 and kept in registers through the
+  // pair slots, whose exchange buffer the face terms later reuse
+#ifndef FX_MHDVOL_GPRE
+#define FX_MHDVOL_GPRE 1
+#endif
+  constexpr int NFP = 6 * n2;
+  constexpr bool GPRE = FX_MHDVOL_GPRE && MODE != VMODE_VOLINT && (EPB * NFP <= T);
+  double gpre[MV];
+  double gsg = 0.0;
+  if (GPRE && tid < EPB * NFP) {
+    const int fe = tid / NFP, r = tid - fe * NFP, face = r / n2, ab = r - face * n2;
+    const int ee = e0 + fe;
+    if (ee < nElems && elem_problem(rk.eprob, ee) >= 0) {
+      const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
+      const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+      int p, q;
+      vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
+      const size_t f = ((size_t)(sd * n + q) * n + p) * MV;
+      const double *F = (fl == 0) ? FluxM + f : FluxS + f;
+      gsg = (fl == 0) ? op.L0 : -op.L0;
+#pragma unroll
+      for (int v = 0; v < MV; v++) gpre[v] = F[v];
+    }
+  }
   if (active) {
     if (dir == 0) {
       const double *u = U + ((size_t)e * n3 + node) * MV;
@@ -480,7 +504,13 @@ k_mhd_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__rest
   // longer read: the last slot ended with a barrier)
   const double L0 = op.L0;
   auto s_face = reinterpret_cast<double (*)[NF * MV]>(s_xf);
-  if (MODE != VMODE_VOLINT) {
+  if (GPRE) {
+    if (tid < EPB * NF && gsg != 0.0) {
+      const int fe = tid / NF, r = tid - fe * NF;
+#pragma unroll
+      for (int v = 0; v < MV; v++) s_face[fe][r * MV + v] = gpre[v] * gsg;
+    }
+  } else if (MODE != VMODE_VOLINT) {
     for (int it = tid; it < EPB * NF; it += T) {
       const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
       const int ee = e0 + fe;
