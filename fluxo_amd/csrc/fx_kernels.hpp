@@ -910,7 +910,15 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   constexpr int WSZ = NV * n3, FSZ = (NV + 3) * NF;
   constexpr int USZ = WSZ + FSZ > FTW * n3 ? WSZ + FSZ : FTW * n3;
   __shared__ double s_u[EPB][USZ];
-#define LW(el_, v_, node_) s_u[el_][(v_) * n3 + (node_)]
+  // node slot of a per-node LDS array: at N = 3 bits 1 and 3 are flipped where bit 5 is set,
+  // which leaves the node's own, the xi- and the eta-transposed reads of the matrix-core
+  // contractions free of bank conflicts (each 32-lane half of a wavefront hits 32 distinct
+  // 8-byte slots modulo 32)
+#ifndef FX_LIFT_SWZ
+#define FX_LIFT_SWZ 1
+#endif
+  auto SW = [](int x) { return (N == 3 && FX_LIFT_SWZ) ? (x ^ (((x >> 5) & 1) * 10)) : x; };
+#define LW(el_, v_, node_) s_u[el_][(v_) * n3 + SW(node_)]
   __shared__ int s_fpt[EPB][NF];  // 2 * face point ((s n + q) n + p) + (0 master, 1 slave)
   const Ops &op = c_ops[N];
   const int tid = threadIdx.x;
@@ -1221,13 +1229,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     double fv[12];
     eq.fluxes(u, gx, gy, gz, fv);
 #pragma unroll
-    for (int m = 0; m < 12; m++) s_u[el][m * n3 + node] = fv[m];
+    for (int m = 0; m < 12; m++) s_u[el][m * n3 + SW(node)] = fv[m];
   }
   __syncthreads();
   if (active) {
     double fv[12], o[NVV];
 #pragma unroll
-    for (int m = 0; m < 12; m++) fv[m] = s_u[el][m * n3 + node];
+    for (int m = 0; m < 12; m++) fv[m] = s_u[el][m * n3 + SW(node)];
     // face-normal viscous flux of this element at each of its face points (BR1: from the
     // node's full gradient; BR2 wrote them from the face gradients above)
     if (LIFT == 1) {
@@ -1248,13 +1256,13 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
     EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][v * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][v * n3 + SW(node)] = o[v];
     EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][(NVV + v) * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][(NVV + v) * n3 + SW(node)] = o[v];
     EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][(2 * NVV + v) * n3 + node] = o[v];
+    for (int v = 0; v < NVV; v++) s_u[el][(2 * NVV + v) * n3 + SW(node)] = o[v];
   }
   __syncthreads();
   if (active) {
@@ -1269,9 +1277,9 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       const int ny = (L & 3) + 4 * (L >> 4) + 16 * ((L >> 2) & 3);
 #pragma unroll
       for (int v = 0; v < NVV; v++) {
-        double acc = __builtin_amdgcn_mfma_f64_4x4x4f64(s_u[el][v * n3 + nx], dh, 0.0, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(2 * NVV + v) * n3 + L], acc, 0, 0, 0);
-        const double te = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(NVV + v) * n3 + ny], 0.0, 0, 0, 0);
+        double acc = __builtin_amdgcn_mfma_f64_4x4x4f64(s_u[el][v * n3 + SW(nx)], dh, 0.0, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(2 * NVV + v) * n3 + SW(L)], acc, 0, 0, 0);
+        const double te = __builtin_amdgcn_mfma_f64_4x4x4f64(dh, s_u[el][(NVV + v) * n3 + SW(ny)], 0.0, 0, 0, 0);
         const long long tb = __double_as_longlong(te);
         const int lo = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb & 0xffffffffLL));
         const int hi = __builtin_amdgcn_ds_bpermute(4 * ny, (int)(tb >> 32));
@@ -1285,9 +1293,9 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
         const double di = op.Dhat[i][l], dj = op.Dhat[j][l], dk = op.Dhat[k][l];
 #pragma unroll
         for (int v = 0; v < NVV; v++)
-          uv[v] = uv[v] + di * s_u[el][v * n3 + l + n * (j + n * k)] +
-                  dj * s_u[el][(NVV + v) * n3 + i + n * (l + n * k)] +
-                  dk * s_u[el][(2 * NVV + v) * n3 + i + n * (j + n * l)];
+          uv[v] = uv[v] + di * s_u[el][v * n3 + SW(l + n * (j + n * k))] +
+                  dj * s_u[el][(NVV + v) * n3 + SW(i + n * (l + n * k))] +
+                  dk * s_u[el][(2 * NVV + v) * n3 + SW(i + n * (j + n * l))];
       }
     }
 #pragma unroll
