@@ -955,6 +955,14 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   // before the element's own state so that both are in flight together (where the metric
   // prefetch above is made too)
   constexpr bool PREF = PREM;
+  // EARLYF: the node's viscous fluxes, its face-normal fluxes (FvN, with the normals that phase F
+  // left in LDS) and its transformed fluxes are formed before the LDS is handed over, so that
+  // only the 3 NVV transformed values cross the barrier (one barrier and the normals' global
+  // loads fewer than storing the 12 Cartesian values first)
+#ifndef FX_LIFT_EARLYF
+#define FX_LIFT_EARLYF 1
+#endif
+  constexpr bool EARLYF = FX_LIFT_EARLYF && PREM && OUT == 1;
   int2 ft0 = make_int2(-1, -1);
   double un0[NV];
   double4 g0 = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -993,7 +1001,14 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
     s_fpt[fe][r] = t.y;
     // a big mortar side has no neighbour element: its lifting flux comes projected (LiftM); the
     // small side of a mortar has the big side's interpolated state (Um) as its master
-    if (t.x == -1) continue;
+    if (t.x == -1) {
+      if (EARLYF) {  // the normal alone, for the face-normal viscous flux
+        const double4 gm = nvse[t.y >> 1];
+        double *d = &s_u[fe][WSZ + r * (NV + 3)];
+        d[NV] = gm.x; d[NV + 1] = gm.y; d[NV + 2] = gm.z;
+      }
+      continue;
+    }
     const int fpt = t.y >> 1;
     const bool master = (t.y & 1) == 0;
     double unb[NV], wnb[NV];
@@ -1203,6 +1218,25 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       atomicMax(&s_dterr[el], err);
     }
   }
+  double ftr[EARLYF ? 3 * NVV : 1];
+  if (EARLYF && active) {
+    double fv[12], o[NVV];
+    eq.fluxes(u, gx, gy, gz, fv);
+    const NodeFaces nf = node_faces<N>(i, j, k);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int r = nf.get(t);
+      if (r < 0) continue;
+      const double *d = &s_u[el][WSZ + r * (NV + 3)];
+      EQ::vis_dir(fv, d[NV], d[NV + 1], d[NV + 2], o);
+      double *dst = out1 + (size_t)s_fpt[el][r] * NVV;
+#pragma unroll
+      for (int v = 0; v < NVV; v++) dst[v] = o[v];
+    }
+    EQ::vis_dir(fv, mreg[0], mreg[1], mreg[2], ftr);
+    EQ::vis_dir(fv, mreg[3], mreg[4], mreg[5], ftr + NVV);
+    EQ::vis_dir(fv, mreg[6], mreg[7], mreg[8], ftr + 2 * NVV);
+  }
   if (OUT == 0) return;
   __syncthreads();  // the entropy variables and face data are no longer read; the fluxes take their memory
   if constexpr (DT) {  // the block's minimum time step (NaN skipped) and maximum error code
@@ -1224,45 +1258,52 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
       dtf.errBlock[blockIdx.x] = er;
     }
   }
-  if (active) {
-    // the 12 Cartesian viscous flux values of this node into its own slots
-    double fv[12];
-    eq.fluxes(u, gx, gy, gz, fv);
+  if (EARLYF) {
+    if (active) {
 #pragma unroll
-    for (int m = 0; m < 12; m++) s_u[el][m * n3 + SW(node)] = fv[m];
-  }
-  __syncthreads();
-  if (active) {
-    double fv[12], o[NVV];
-#pragma unroll
-    for (int m = 0; m < 12; m++) fv[m] = s_u[el][m * n3 + SW(node)];
-    // face-normal viscous flux of this element at each of its face points (BR1: from the
-    // node's full gradient; BR2 wrote them from the face gradients above)
-    if (LIFT == 1) {
-      const NodeFaces nf = node_faces<N>(i, j, k);
-#pragma unroll
-      for (int t = 0; t < 3; t++) {
-        const int r = nf.get(t);
-        if (r < 0) continue;
-        const int code = s_fpt[el][r];
-        const double4 g = nvse[code >> 1];
-        EQ::vis_dir(fv, g.x, g.y, g.z, o);
-        double *dst = out1 + (size_t)code * NVV;
-#pragma unroll
-        for (int v = 0; v < NVV; v++) dst[v] = o[v];
-      }
+      for (int m = 0; m < 3 * NVV; m++) s_u[el][m * n3 + SW(node)] = ftr[m];
     }
-    // transformed fluxes f~, g~, h~ (EvalDiffFluxTilde3D) over the node's own slots
-    const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
-    EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
+  } else {
+    if (active) {
+      // the 12 Cartesian viscous flux values of this node into its own slots
+      double fv[12];
+      eq.fluxes(u, gx, gy, gz, fv);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][v * n3 + SW(node)] = o[v];
-    EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
+      for (int m = 0; m < 12; m++) s_u[el][m * n3 + SW(node)] = fv[m];
+    }
+    __syncthreads();
+    if (active) {
+      double fv[12], o[NVV];
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][(NVV + v) * n3 + SW(node)] = o[v];
-    EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
+      for (int m = 0; m < 12; m++) fv[m] = s_u[el][m * n3 + SW(node)];
+      // face-normal viscous flux of this element at each of its face points (BR1: from the
+      // node's full gradient; BR2 wrote them from the face gradients above)
+      if (LIFT == 1) {
+        const NodeFaces nf = node_faces<N>(i, j, k);
 #pragma unroll
-    for (int v = 0; v < NVV; v++) s_u[el][(2 * NVV + v) * n3 + SW(node)] = o[v];
+        for (int t = 0; t < 3; t++) {
+          const int r = nf.get(t);
+          if (r < 0) continue;
+          const int code = s_fpt[el][r];
+          const double4 g = nvse[code >> 1];
+          EQ::vis_dir(fv, g.x, g.y, g.z, o);
+          double *dst = out1 + (size_t)code * NVV;
+#pragma unroll
+          for (int v = 0; v < NVV; v++) dst[v] = o[v];
+        }
+      }
+      // transformed fluxes f~, g~, h~ (EvalDiffFluxTilde3D) over the node's own slots
+      const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
+      EQ::vis_dir(fv, mf[0], mf[1], mf[2], o);
+#pragma unroll
+      for (int v = 0; v < NVV; v++) s_u[el][v * n3 + SW(node)] = o[v];
+      EQ::vis_dir(fv, mg[0], mg[1], mg[2], o);
+#pragma unroll
+      for (int v = 0; v < NVV; v++) s_u[el][(NVV + v) * n3 + SW(node)] = o[v];
+      EQ::vis_dir(fv, mh[0], mh[1], mh[2], o);
+#pragma unroll
+      for (int v = 0; v < NVV; v++) s_u[el][(2 * NVV + v) * n3 + SW(node)] = o[v];
+    }
   }
   __syncthreads();
   if (active) {
