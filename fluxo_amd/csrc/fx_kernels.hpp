@@ -955,14 +955,14 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
   // before the element's own state so that both are in flight together (where the metric
   // prefetch above is made too)
   constexpr bool PREF = PREM;
-  // EARLYF: the node's viscous fluxes, its face-normal fluxes (FvN, with the normals that phase F
-  // left in LDS) and its transformed fluxes are formed before the LDS is handed over, so that
-  // only the 3 NVV transformed values cross the barrier (one barrier and the normals' global
+  // EARLYF (BR1): the node's viscous fluxes, its face-normal fluxes (FvN, with the normals that
+  // phase F left in LDS) and its transformed fluxes are formed before the LDS is handed over, so
+  // that only the 3 NVV transformed values cross the barrier (one barrier and the normals' global
   // loads fewer than storing the 12 Cartesian values first)
 #ifndef FX_LIFT_EARLYF
 #define FX_LIFT_EARLYF 1
 #endif
-  constexpr bool EARLYF = FX_LIFT_EARLYF && PREM && OUT == 1;
+  constexpr bool EARLYF = FX_LIFT_EARLYF && LIFT == 1 && OUT == 1;
   int2 ft0 = make_int2(-1, -1);
   double un0[NV];
   double4 g0 = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1233,9 +1233,10 @@ k_lifting(const double *__restrict__ U, const double *__restrict__ Mf, const dou
 #pragma unroll
       for (int v = 0; v < NVV; v++) dst[v] = o[v];
     }
-    EQ::vis_dir(fv, mreg[0], mreg[1], mreg[2], ftr);
-    EQ::vis_dir(fv, mreg[3], mreg[4], mreg[5], ftr + NVV);
-    EQ::vis_dir(fv, mreg[6], mreg[7], mreg[8], ftr + 2 * NVV);
+    const double *mf = PREM ? mreg : Mf + 3 * c, *mg = PREM ? mreg + 3 : Mg + 3 * c, *mh = PREM ? mreg + 6 : Mh + 3 * c;
+    EQ::vis_dir(fv, mf[0], mf[1], mf[2], ftr);
+    EQ::vis_dir(fv, mg[0], mg[1], mg[2], ftr + NVV);
+    EQ::vis_dir(fv, mh[0], mh[1], mh[2], ftr + 2 * NVV);
   }
   if (OUT == 0) return;
   __syncthreads();  // the entropy variables and face data are no longer read; the fluxes take their memory
