@@ -353,6 +353,24 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
     }
   }
 
+  // the face gather's (side, flip) pairs of this thread's face points (phase 3b), loaded here so
+  // that only the face flux load follows the direction-sum barriers
+#ifndef FX_VOL_E2SPRE
+#define FX_VOL_E2SPRE 1
+#endif
+  constexpr int NFG = 6 * n2, NGI = (EPB * NFG + TB - 1) / TB;
+  constexpr bool GPRE = FX_VOL_E2SPRE && MODE != VMODE_VOLINT;
+  constexpr int NGP = GPRE ? 1 : 0;  // prefetched items (the first; more cost an occupancy level)
+  int2 gsd[1];
+  if (GPRE) {
+#pragma unroll
+    for (int g = 0; g < NGP; g++) {
+      const int it = tid + g * TB, fe = it / NFG, face = (it - fe * NFG) / n2;
+      const int q = ((e0 + fe) * 6 + face) * 2;
+      gsd[g] = (it < EPB * NFG && e0 + fe < nElems) ? make_int2(e2s[q], e2s[q + 1]) : make_int2(0, 0);
+    }
+  }
+
   // phase 1: per-node fields into LDS
   for (int idx = tid; idx < EPB * n3; idx += TB) {
     const int el1 = idx / n3, node = idx - el1 * n3;
@@ -458,12 +476,16 @@ k_volume(double *__restrict__ U, const VR *__restrict__ Mf, const VR *__restrict
   constexpr int NF = 6 * n2;
   __shared__ double s_face[MODE != VMODE_VOLINT ? EPB : 1][MODE != VMODE_VOLINT ? NF * 5 : 1];
   if (MODE != VMODE_VOLINT) {
-    for (int it = tid; it < EPB * NF; it += TB) {
+#pragma unroll
+    for (int g = 0; g < NGI; g++) {
+      const int it = tid + g * TB;
+      if (it >= EPB * NF) break;
       const int fe = it / NF, r = it - fe * NF, face = r / n2, ab = r - face * n2;
       const int ee = e0 + fe;
       if (ee >= nElems || elem_problem(rk.eprob, ee) < 0) continue;
       const int loc = face + 1, fnode = face_node<N>(loc, ab % n, ab / n);
-      const int sd = e2s[(ee * 6 + face) * 2] - 1, fl = e2s[(ee * 6 + face) * 2 + 1];
+      const int2 sf = (g < NGP) ? gsd[0] : make_int2(e2s[(ee * 6 + face) * 2], e2s[(ee * 6 + face) * 2 + 1]);
+      const int sd = sf.x - 1, fl = sf.y;
       int p, q;
       vol_to_side<N>(loc, fl, fnode % n, (fnode / n) % n, fnode / n2, p, q);
       const double *F = Flux + ((size_t)(sd * n + q) * n + p) * 5;
